@@ -1,0 +1,134 @@
+/*
+ * vacd.h -- C ABI of the MI355X visibility-edge concavity engine.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (VACD_OK == 0); vacd_last_error() gives the message of the last failure
+ * on that context (or of the last context-less failure when ctx is NULL).
+ *
+ * Host-buffer entry points (no suffix) copy inputs to the device, run, copy
+ * results back and return when done.  "_d" entry points take device
+ * pointers, enqueue on the context's stream and return without waiting.
+ *
+ * One context per host thread; calls on one context are serialised by the
+ * caller.  Device memory (BVHs aside) is pooled in the context and reused
+ * across calls.
+ */
+#ifndef VACD_H
+#define VACD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VACD_OK 0
+#define VACD_ERR_INVALID 1  /* bad argument (null pointer, negative size, ...) */
+#define VACD_ERR_HIP 2      /* HIP runtime error (launch, copy, allocation)    */
+#define VACD_ERR_CAPACITY 3 /* output buffer smaller than the result            */
+#define VACD_ERR_LIMIT 4    /* input beyond a documented size limit              */
+
+typedef struct vacd_ctx vacd_ctx;
+typedef struct vacd_bvh vacd_bvh;
+
+/* Statistics of the last visibility call (vacd_visibility_stats). */
+typedef struct vacd_vis_stats {
+  int64_t segments;   /* pairs enumerated, n(n-1)/2                       */
+  int64_t masked;     /* skipped by the new-vertex mask (reference state 2) */
+  int64_t blocked;    /* touched the self mesh                            */
+  int64_t edges;      /* visibility edges                                 */
+  int64_t exact_calls;/* triangle tests decided by the exact fallback     */
+} vacd_vis_stats;
+
+/* ---- context -------------------------------------------------------------
+ * Replaces createContext / optixDeviceContextDestroy
+ * (lib/include/optixUtils.hpp:80-108, called at lib/src/process.cpp:246,331)
+ * and the CUCHK/OCHK error macros (lib/include/optixUtils.hpp:56-71). */
+int vacd_ctx_create(int device, vacd_ctx **out);
+int vacd_ctx_destroy(vacd_ctx *ctx);
+const char *vacd_last_error(const vacd_ctx *ctx);
+/* The hipStream_t the context enqueues on (returned as void*). */
+int vacd_ctx_stream(vacd_ctx *ctx, void **stream);
+/* Enqueue on a caller-owned hipStream_t instead (NULL restores the own one). */
+int vacd_ctx_set_stream(vacd_ctx *ctx, void *stream);
+int vacd_sync(vacd_ctx *ctx);
+
+/* ---- BVH -------------------------------------------------------------------
+ * Replaces the OptiX GAS build of run_optix (lib/src/intersections.cpp:131-180)
+ * with an on-device LBVH (30-bit Morton codes, LDS radix sort, Karras
+ * hierarchy, bottom-up AABB refit).  vertices: float32 [n_vertices*3];
+ * tris: uint32 [n_tris*3] vertex indices.  vacd_bvh_build copies from host
+ * memory; vacd_bvh_build_d reads device memory.  Both wait for the build.
+ * Limits: 1 <= n_tris < 2^30. */
+int vacd_bvh_build(vacd_ctx *ctx, const float *vertices, int64_t n_vertices,
+                   const uint32_t *tris, int64_t n_tris, vacd_bvh **out);
+int vacd_bvh_build_d(vacd_ctx *ctx, const float *d_vertices, int64_t n_vertices,
+                     const uint32_t *d_tris, int64_t n_tris, vacd_bvh **out);
+int vacd_bvh_destroy(vacd_bvh *bvh);
+/* n_tris, tree depth (root = 1), scene bounds lo[3], hi[3]. */
+int vacd_bvh_info(const vacd_bvh *bvh, int64_t *n_tris, int32_t *depth,
+                  float *bounds6);
+/* Copies the node array (64 B per internal node, n_tris-1 nodes) and the
+ * leaf-ordered triangle array (48 B per triangle) to host memory. */
+int vacd_bvh_export(const vacd_bvh *bvh, void *nodes, void *tris);
+
+/* ---- visibility --------------------------------------------------------------
+ * Replaces run_optix x2 + the combine loop of compute_intersection_matrix
+ * (lib/src/intersections.cpp:75-325, 327-385; device programs
+ * lib/ptx/ray_segments.cu:49-125).  Tests every pair (i<j) of the n points
+ * pts[n*3]: pairs with new_mask[i]==0 && new_mask[j]==0 are skipped when
+ * new_mask is non-NULL; a pair is a visibility edge iff its segment does not
+ * touch the self mesh and (cage != NULL) touches the cage mesh, or
+ * (cage == NULL) its midpoint lies outside the self mesh.  Edges are written
+ * to pairs[2*e], pairs[2*e+1] = (i, j) in increasing k = i(2n-i-1)/2 + j-i-1
+ * order, at most cap pairs; *count receives the total number of edges.
+ * Returns VACD_ERR_CAPACITY (after setting *count) if count > cap.
+ * Limits: 2 <= n <= 2^26 (k must fit 52 bits); n = 0 or 1 gives no edges. */
+int vacd_visibility(vacd_ctx *ctx, const float *pts, int64_t n,
+                    const uint8_t *new_mask, const vacd_bvh *self,
+                    const vacd_bvh *cage, uint32_t *pairs, int64_t cap,
+                    int64_t *count);
+/* Device-pointer form: d_count is a device int64; no capacity error is
+ * raised (the caller reads *d_count and compares with cap). */
+int vacd_visibility_d(vacd_ctx *ctx, const float *d_pts, int64_t n,
+                      const uint8_t *d_new_mask, const vacd_bvh *self,
+                      const vacd_bvh *cage, uint32_t *d_pairs, int64_t cap,
+                      int64_t *d_count);
+/* Statistics of the last visibility call on ctx (waits for it). */
+int vacd_visibility_stats(vacd_ctx *ctx, vacd_vis_stats *out);
+
+/* ---- plane scoring ------------------------------------------------------------
+ * Replaces classify_and_rate_planes + plane_point_side_and_rate_kernel
+ * (lib/cuda/plane_intersections.cu:7-133).  planes: float32 [P*4] (a,b,c,d);
+ * pts: float32 [n*3]; edges: uint32 [E*2].  side(v) = sign(fmaf(c,z,
+ * fmaf(b,y,a*x)) + d) with dead zone 1e-6; an edge whose endpoints have
+ * nonzero opposite sides adds its length sqrtf(fmaf(dz,dz,fmaf(dy,dy,dx*dx)))
+ * rounded once to Q32.32.  Outputs (each nullable): score_q32 int64 [P]
+ * (exact, order-independent), score_f float32 [P] (= score_q32 * 2^-32),
+ * argmax (first maximum of score_q32; -1 when P == 0).
+ * Edges with an endpoint index >= n are ignored (as the reference does).
+ * Limits: P < 2^31, E < 2^40. */
+int vacd_score_planes(vacd_ctx *ctx, const float *planes, int64_t P,
+                      const float *pts, int64_t n, const uint32_t *edges,
+                      int64_t E, int64_t *score_q32, float *score_f,
+                      int32_t *argmax);
+int vacd_score_planes_d(vacd_ctx *ctx, const float *d_planes, int64_t P,
+                        const float *d_pts, int64_t n, const uint32_t *d_edges,
+                        int64_t E, int64_t *d_score_q32, float *d_score_f,
+                        int32_t *d_argmax);
+
+/* Reference-signature shim: same arguments and behaviour as
+ * classify_and_rate_planes (lib/cuda/plane_intersections.cu:71-78,
+ * declared at lib/src/process.cpp:74-79): host pointers, synchronous,
+ * scores[] overwritten.  Uses a per-thread default context on device 0;
+ * failures are reported on stderr, as the reference does. */
+void classify_and_rate_planes(const float *planes, const float *points,
+                              const unsigned int *edges, float *scores,
+                              int numPlanes, int numPoints, int numEdges);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VACD_H */

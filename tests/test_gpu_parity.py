@@ -1,0 +1,241 @@
+"""HIP engine vs the FP64 oracle, through the C ABI (libvacd.so).
+
+Visibility and Q32.32 scores are integer/index results: the bar is bit
+equality with the oracle.  The float32 score view must match the FP64 form
+to 1e-5 relative where the FP64 score is not tiny.
+"""
+import numpy as np
+import pytest
+
+import meshes
+import oracle
+from visacd_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+# ----------------------------------------------------------------- LBVH
+@pytest.mark.parametrize("name", ["torus", "ico4", "single", "two", "lshape"])
+def test_lbvh_invariants(ctx, name):
+    if name == "torus":
+        V, T = wl.torus()
+    elif name == "ico4":
+        V, T = wl.perturbed_icosphere(4, 0.3, 7)
+    elif name == "single":
+        V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+        T = np.array([[0, 1, 2]], np.uint32)
+    elif name == "two":
+        V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 1]], np.float32)
+        T = np.array([[0, 1, 2], [1, 3, 2]], np.uint32)
+    else:
+        V, T = meshes.l_shape()
+    _print("lbvh", name, T.shape)
+    b = ctx.bvh(V, T)
+    info = b.info()
+    nodes, tris = b.export()
+    nt = T.shape[0]
+    assert info["n_tris"] == nt
+    # every triangle exactly once, coordinates preserved
+    orig = tris[:, 3].view(np.uint32)
+    assert sorted(orig.tolist()) == list(range(nt))
+    np.testing.assert_array_equal(tris[:, 0:3], V[T[orig, 0]])
+    np.testing.assert_array_equal(tris[:, 4:7], V[T[orig, 1]])
+    np.testing.assert_array_equal(tris[:, 8:11], V[T[orig, 2]])
+    if nt == 1:
+        return
+    child = nodes[:, 12:14].view(np.int32)
+    seen_leaf = np.zeros(nt, bool)
+    seen_int = np.zeros(nt - 1, bool)
+    depth = 0
+
+    def box_of_leaf(j):
+        t = tris[j]
+        pts = np.stack([t[0:3], t[4:7], t[8:11]])
+        return pts.min(0), pts.max(0)
+
+    stack = [(0, 1)]
+    seen_int[0] = True
+    while stack:
+        x, d = stack.pop()
+        depth = max(depth, d + 1)
+        for s in range(2):
+            c = int(child[x, s])
+            lo = nodes[x, 6 * s: 6 * s + 3]
+            hi = nodes[x, 6 * s + 3: 6 * s + 6]
+            if c < 0:
+                j = ~c
+                assert not seen_leaf[j]
+                seen_leaf[j] = True
+                blo, bhi = box_of_leaf(j)
+                assert (lo < blo).all() and (hi > bhi).all()
+            else:
+                assert not seen_int[c]
+                seen_int[c] = True
+                # child box encloses both of the child's child boxes
+                assert (lo <= nodes[c, 0:3]).all() and (lo <= nodes[c, 6:9]).all()
+                assert (hi >= nodes[c, 3:6]).all() and (hi >= nodes[c, 9:12]).all()
+                stack.append((c, d + 1))
+    assert seen_leaf.all() and seen_int.all()
+    assert info["depth"] == depth
+
+
+# ----------------------------------------------------------------- visibility
+def _gpu_vs_oracle(ctx, V, T, pts=None, mask=None, cage=None, brute=False):
+    pts = V if pts is None else pts
+    b = ctx.bvh(V, T)
+    cb = ctx.bvh(*cage) if cage is not None else None
+    g = ctx.visibility(pts, b, cage=cb, new_mask=mask)
+    stats = ctx.visibility_stats()
+    o, ost = oracle.visibility(pts, V, T, mask=mask, cage=cage, brute=brute)
+    return g, o, stats, ost
+
+
+@pytest.mark.parametrize("name", ["lshape", "ushape", "box", "ico2"])
+def test_visibility_closed_forms(ctx, name):
+    _print("closed form", name)
+    if name == "lshape":
+        V, T = meshes.l_shape()
+        expect = 16
+    elif name == "ushape":
+        V, T = meshes.u_channel()
+        expect = 12
+    elif name == "box":
+        V, T = meshes.box((0, 0, 0), (2, 2, 2))
+        expect = 0
+    else:
+        v, T = wl.icosphere(2)
+        V = wl.normalize_like_reference(v).astype(np.float32)
+        T = T.astype(np.uint32)
+        expect = 0
+    g, o, st, ost = _gpu_vs_oracle(ctx, V, T, brute=True)
+    assert len(o) == expect
+    np.testing.assert_array_equal(g, o)
+    assert st["edges"] == expect
+    assert st["blocked"] == ost["blocked"]
+
+
+def test_visibility_torus_full(ctx):
+    W = wl.torus_c1()
+    _print("torus", W.n_segments)
+    g, o, st, ost = _gpu_vs_oracle(ctx, W.vertices, W.triangles, W.points)
+    _print("torus edges", len(g), len(o), st)
+    assert len(o) > 0
+    np.testing.assert_array_equal(g, o)
+    assert st["blocked"] == ost["blocked"]
+    assert st["segments"] == W.n_segments
+
+
+def test_visibility_torus_masked(ctx):
+    W = wl.torus_c1()
+    rng = np.random.default_rng(5)
+    mask = (rng.uniform(size=W.n_points) < 0.1).astype(np.uint8)
+    _print("torus masked", int(mask.sum()))
+    g, o, st, ost = _gpu_vs_oracle(ctx, W.vertices, W.triangles, W.points, mask=mask)
+    np.testing.assert_array_equal(g, o)
+    assert st["masked"] == ost["masked"]
+
+
+def test_visibility_cage_mode(ctx):
+    V, T = wl.perturbed_icosphere(3, 0.3, 11)
+    cage = (meshes.cage_of(V, 1.06), T)
+    _print("cage mode", T.shape)
+    g, o, st, ost = _gpu_vs_oracle(ctx, V, T, cage=cage)
+    assert len(o) > 0
+    np.testing.assert_array_equal(g, o)
+    # cage edges are a subset of parity edges (definition)
+    gp, op_, _, _ = _gpu_vs_oracle(ctx, V, T)
+    np.testing.assert_array_equal(gp, op_)
+    kp = set(map(tuple, op_.tolist()))
+    assert all(tuple(e) in kp for e in o.tolist())
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_visibility_degenerate_polycube(ctx, seed):
+    rng = np.random.default_rng(seed)
+    cells = {(x, y, z) for x in range(4) for y in range(4) for z in range(2)
+             if rng.uniform() < 0.7}
+    cells |= {(0, 0, 0)}
+    # keep a face-connected, edge-manifold set: drop cells touching others only by an edge
+    V, T = meshes.polycube(sorted(cells))
+    _print("polycube", seed, V.shape, T.shape)
+    g, o, st, ost = _gpu_vs_oracle(ctx, V, T)
+    np.testing.assert_array_equal(g, o)
+    _print("exact fallbacks", st["exact_calls"])
+
+
+def test_visibility_icosphere_c2_sampled(ctx):
+    """Full-size C2 run; sampled pairs checked against the oracle."""
+    W = wl.icosphere_c2()
+    _print("c2", W.n_segments)
+    b = ctx.bvh(W.vertices, W.triangles)
+    g = ctx.visibility(W.points, b)
+    st = ctx.visibility_stats()
+    _print("c2 stats", st)
+    n = W.n_points
+    k = g[:, 0].astype(np.int64) * (2 * n - g[:, 0].astype(np.int64) - 1) // 2 + \
+        g[:, 1].astype(np.int64) - g[:, 0].astype(np.int64) - 1
+    assert (g[:, 0] < g[:, 1]).all()
+    assert (np.diff(k) > 0).all()  # strictly increasing k order
+    assert st["edges"] == len(g)
+    rng = np.random.default_rng(3)
+    ks = np.unique(np.concatenate([rng.integers(0, W.n_segments, 20000),
+                                   k[rng.integers(0, len(k), 5000)]]))
+    det = oracle.visibility_sample(W.points, W.vertices, W.triangles, ks)
+    is_edge = np.isin(ks, k)
+    np.testing.assert_array_equal(is_edge, det == 1)
+
+
+# ----------------------------------------------------------------- scoring
+def test_scoring_torus_exact(ctx):
+    W = wl.torus_c1()
+    edges, _ = oracle.visibility(W.points, W.vertices, W.triangles)
+    _print("scoring torus", len(edges), W.planes.shape)
+    q, f, am = ctx.score_planes(W.planes, W.points, edges)
+    oq = oracle.score_q32(W.planes, W.points, edges)
+    np.testing.assert_array_equal(q, oq)
+    assert am == oracle.argmax_i64(oq)
+    o64 = oracle.score_fp64(W.planes, W.points, edges)
+    big = o64 > 1e-3 * o64.max()
+    np.testing.assert_allclose(f[big], o64[big], rtol=1e-5)
+    np.testing.assert_array_equal(f, (oq.astype(np.float64) * 2.0 ** -32).astype(np.float32))
+
+
+def test_scoring_ties_and_bounds(ctx):
+    pts = np.array([[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, 0]], np.float32)
+    edges = np.array([[0, 1], [2, 3], [0, 4], [1, 7]], np.uint32)  # last is out of range
+    planes = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0], [1, 0, 0, -2]], np.float32)
+    q, f, am = ctx.score_planes(planes, pts, edges)
+    # x=0 cuts edge 0 (len 2); y=0 cuts edge 1 (len 2): tie -> first
+    assert q.tolist() == [2 << 32, 2 << 32, 0, 0]
+    assert am == 0
+    np.testing.assert_array_equal(q, oracle.score_q32(planes, pts, edges))
+
+
+def test_scoring_c2_sized(ctx):
+    W = wl.icosphere_c2()
+    b = ctx.bvh(W.vertices, W.triangles)
+    edges = ctx.visibility(W.points, b)
+    _print("scoring c2", len(edges))
+    sub = W.planes[:64]
+    q, f, am = ctx.score_planes(sub, W.points, edges)
+    oq = oracle.score_q32(sub, W.points, edges)
+    np.testing.assert_array_equal(q, oq)
+    # permutation of edges changes nothing (integer sums)
+    perm = np.random.default_rng(0).permutation(len(edges))
+    q2, _, am2 = ctx.score_planes(sub, W.points, edges[perm])
+    np.testing.assert_array_equal(q2, q)
+    assert am2 == am
+
+
+def test_reference_shim(ctx):
+    W = wl.torus_c1()
+    edges, _ = oracle.visibility(W.points, W.vertices, W.triangles)
+    from visacd_amd import native
+
+    s = native.classify_and_rate_planes(W.planes[:32], W.points, edges)
+    oq = oracle.score_q32(W.planes[:32], W.points, edges)
+    np.testing.assert_array_equal(s, (oq.astype(np.float64) * 2.0 ** -32).astype(np.float32))

@@ -1,0 +1,134 @@
+// Internal declarations shared by the HIP translation units of libvacd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/vacd.h"
+
+namespace vacd {
+
+// ---------------------------------------------------------------- layouts
+// BVH node (64 B): boxes of both children plus their ids.  A child id >= 0
+// is an internal node, a negative id c is leaf ~c (a leaf-ordered triangle).
+//   f0 = lo0.x lo0.y lo0.z hi0.x   f1 = hi0.y hi0.z lo1.x lo1.y
+//   f2 = lo1.z hi1.x hi1.y hi1.z   f3 = c0 c1 (as int bits) pad pad
+struct alignas(16) Node {
+  float4 f0, f1, f2, f3;
+};
+static_assert(sizeof(Node) == 64, "node is 64 bytes");
+
+// Leaf-ordered triangle (48 B): a.xyz + original id bits, b.xyz, c.xyz.
+struct alignas(16) Tri {
+  float4 a, b, c;
+};
+static_assert(sizeof(Tri) == 48, "triangle is 48 bytes");
+
+// ---------------------------------------------------------------- errors
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define VACD_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw ::vacd::Error{VACD_ERR_HIP, std::string(#call) + ": " +           \
+                                            hipGetErrorString(e_)};           \
+  } while (0)
+
+#define VACD_REQUIRE(cond, code, msg)                                         \
+  do {                                                                        \
+    if (!(cond)) throw ::vacd::Error{(code), (msg)};                          \
+  } while (0)
+
+// Grow-only device buffer owned by a context or a BVH.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    if (ptr) VACD_HIP(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    size_t want = n < 256 ? 256 : n;
+    VACD_HIP(hipMalloc(&ptr, want));
+    bytes = want;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(ptr);
+  }
+};
+
+}  // namespace vacd
+
+// Pool slots used by the entry points.
+enum VacdSlot {
+  SLOT_PTS = 0,
+  SLOT_MASK,
+  SLOT_PAIRS,
+  SLOT_COUNT,
+  SLOT_BITS,
+  SLOT_DEFER,
+  SLOT_SCAN,
+  SLOT_STATS,
+  SLOT_PLANES,
+  SLOT_EDGES,
+  SLOT_EREC,
+  SLOT_SCORES,
+  SLOT_SCORES_F,
+  SLOT_ARGMAX,
+  SLOT_VERTS,
+  SLOT_TRIS,
+  SLOT_SORT,
+  SLOT_MISC,
+  SLOT_COUNT_MAX
+};
+
+struct vacd_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  vacd::DevBuf buf[SLOT_COUNT_MAX];
+  int64_t last_segments = 0;
+};
+
+struct vacd_bvh {
+  int device = 0;
+  int64_t n_tris = 0;
+  int32_t depth = 0;
+  float bounds[6] = {0, 0, 0, 0, 0, 0};
+  vacd::DevBuf nodes;  // Node[max(1, n_tris-1)]
+  vacd::DevBuf tris;   // Tri[n_tris]
+};
+
+namespace vacd {
+
+// LBVH build from device vertices/triangles (lbvh.hip).
+void build_lbvh(vacd_ctx *ctx, const float *d_vertices, int64_t n_vertices,
+                const uint32_t *d_tris, int64_t n_tris, vacd_bvh *out);
+
+// Visibility (visibility.hip): enqueue everything; writes pairs and count.
+void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n,
+                       const uint8_t *d_mask, const vacd_bvh *self,
+                       const vacd_bvh *cage, uint32_t *d_pairs, int64_t cap,
+                       int64_t *d_count);
+
+// Scoring (scoring.hip).
+void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P,
+                    const float *d_pts, int64_t n, const uint32_t *d_edges,
+                    int64_t E, int64_t *d_q32, float *d_f, int32_t *d_argmax);
+
+}  // namespace vacd

@@ -1,0 +1,426 @@
+// On-device LBVH build for gfx950.
+//
+// Replaces the OptiX GAS build of run_optix (lib/src/intersections.cpp:131-180).
+// Steps, all on the context stream:
+//   1. k_bounds   triangle centroid bounds and vertex bounds (block reduce +
+//                 ordered-integer atomics).
+//   2. k_morton   30-bit Morton code of each centroid quantised to 10 bits per
+//                 axis over the centroid bounds.
+//   3. radix sort four stable LSD passes of 8 bits: per 256-key tile a digit
+//                 histogram, one exclusive scan of the digit-major histogram
+//                 table, then a scatter that ranks keys with wave ballots
+//                 (match on the 8 digit bits) and per-wave digit counts in LDS.
+//   4. k_karras   Karras (HPG 2012) hierarchy: each internal node finds its key
+//                 range and split, with the key index breaking ties between
+//                 equal codes.
+//   5. k_leaves   leaf-ordered 48-byte triangles.
+//   6. k_refit    bottom-up child boxes: every leaf walks to the root; at each
+//                 node the second arrival (agent-scope acq_rel counter) merges
+//                 the two child boxes and continues.
+//   7. k_depth    tree depth (longest leaf-to-root chain) for stack sizing.
+// Child boxes are widened by 2^-17 of the scene extent so that the float32
+// slab tests of the traversal stay conservative.
+#include "internal.hpp"
+
+#include <algorithm>
+
+namespace vacd {
+namespace {
+
+constexpr int kTile = 256;  // keys per sort tile (one per thread)
+
+__device__ __forceinline__ unsigned f2ord(float f) {
+  unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(unsigned u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// bounds layout (unsigned ordered floats): [0..2] centroid min, [3..5]
+// centroid max, [6..8] vertex min, [9..11] vertex max.
+__global__ void k_bounds_init(unsigned *b) {
+  int t = threadIdx.x;
+  if (t < 12) b[t] = ((t / 3) % 2 == 0) ? 0xffffffffu : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_bounds(const float *__restrict__ V,
+                                                const uint32_t *__restrict__ T, int64_t nT,
+                                                unsigned *__restrict__ bounds) {
+  float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float vmin[3] = {INFINITY, INFINITY, INFINITY}, vmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nT;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t i0 = T[3 * t], i1 = T[3 * t + 1], i2 = T[3 * t + 2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float a = V[3 * (size_t)i0 + k], b = V[3 * (size_t)i1 + k], c = V[3 * (size_t)i2 + k];
+      const float cc = (a + b + c) * (1.0f / 3.0f);
+      cmin[k] = fminf(cmin[k], cc);
+      cmax[k] = fmaxf(cmax[k], cc);
+      vmin[k] = fminf(vmin[k], fminf(a, fminf(b, c)));
+      vmax[k] = fmaxf(vmax[k], fmaxf(a, fmaxf(b, c)));
+    }
+  }
+  __shared__ float red[12][256];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    red[k][threadIdx.x] = cmin[k];
+    red[3 + k][threadIdx.x] = cmax[k];
+    red[6 + k][threadIdx.x] = vmin[k];
+    red[9 + k][threadIdx.x] = vmax[k];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+#pragma unroll
+      for (int r = 0; r < 12; ++r) {
+        const float a = red[r][threadIdx.x], b = red[r][threadIdx.x + s];
+        red[r][threadIdx.x] = ((r / 3) % 2 == 0) ? fminf(a, b) : fmaxf(a, b);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 12) {
+    const int r = threadIdx.x;
+    const unsigned v = f2ord(red[r][0]);
+    if ((r / 3) % 2 == 0)
+      atomicMin(&bounds[r], v);
+    else
+      atomicMax(&bounds[r], v);
+  }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float *__restrict__ V,
+                                                const uint32_t *__restrict__ T, int64_t nT,
+                                                const unsigned *__restrict__ bounds,
+                                                uint32_t *__restrict__ keys,
+                                                uint32_t *__restrict__ vals) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nT) return;
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float lo = ord2f(bounds[k]), hi = ord2f(bounds[3 + k]);
+    const float a = V[3 * (size_t)T[3 * t] + k], b = V[3 * (size_t)T[3 * t + 1] + k],
+                c = V[3 * (size_t)T[3 * t + 2] + k];
+    const float cc = (a + b + c) * (1.0f / 3.0f);
+    const float ext = hi - lo;
+    float u = ext > 0.0f ? (cc - lo) / ext : 0.5f;
+    u = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+    q[k] = (uint32_t)u;
+  }
+  keys[t] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  vals[t] = (uint32_t)t;
+}
+
+// --- radix sort ---------------------------------------------------------
+// hist layout: digit-major, hist[d * ntiles + tile].
+__global__ __launch_bounds__(kTile) void k_hist(const uint32_t *__restrict__ keys, int64_t n,
+                                                int shift, uint32_t *__restrict__ hist,
+                                                int ntiles) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t i = blockIdx.x * (int64_t)kTile + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 0xffu], 1u);
+  __syncthreads();
+  hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of m entries with one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_u32(uint32_t *__restrict__ data, int64_t m) {
+  __shared__ uint32_t part[1024];
+  const int64_t per = (m + 1023) / 1024;
+  const int64_t beg = threadIdx.x * per;
+  const int64_t end = beg + per < m ? beg + per : m;
+  uint32_t s = 0;
+  for (int64_t i = beg; i < end; ++i) s += data[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+  for (int64_t i = beg; i < end; ++i) {
+    const uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kTile) void k_scatter(const uint32_t *__restrict__ kin,
+                                                   const uint32_t *__restrict__ vin, int64_t n,
+                                                   int shift, const uint32_t *__restrict__ hist,
+                                                   int ntiles, uint32_t *__restrict__ kout,
+                                                   uint32_t *__restrict__ vout) {
+  constexpr int kWaves = kTile / 64;
+  __shared__ uint32_t wcnt[kWaves][256];
+  for (int t = threadIdx.x; t < kWaves * 256; t += kTile) (&wcnt[0][0])[t] = 0;
+  __syncthreads();
+  const int64_t i = blockIdx.x * (int64_t)kTile + threadIdx.x;
+  const bool valid = i < n;
+  const uint32_t key = valid ? kin[i] : 0u;
+  const uint32_t digit = (key >> shift) & 0xffu;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // lanes of this wave holding the same digit
+  uint64_t same = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint64_t m = __ballot(valid && ((digit >> b) & 1u));
+    same &= ((digit >> b) & 1u) ? m : ~m;
+  }
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint32_t rank = (uint32_t)__popcll(same & below);
+  if (valid && rank == 0) wcnt[wave][digit] = (uint32_t)__popcll(same);
+  __syncthreads();
+  if (valid) {
+    uint32_t off = hist[(size_t)digit * ntiles + blockIdx.x] + rank;
+    for (int w = 0; w < wave; ++w) off += wcnt[w][digit];
+    kout[off] = key;
+    vout[off] = vin[i];
+  }
+}
+
+// --- hierarchy ------------------------------------------------------------
+__device__ __forceinline__ int delta(const uint32_t *__restrict__ k, int64_t n, int64_t i,
+                                     int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = k[i], b = k[j];
+  if (a != b) return __clz(a ^ b);
+  return 32 + __clz((uint32_t)(i ^ j));
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ keys, int64_t n,
+                                                Node *__restrict__ nodes,
+                                                int32_t *__restrict__ parent_int,
+                                                int32_t *__restrict__ parent_leaf) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int64_t lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int64_t l = 0;
+  for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int64_t j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int64_t s = 0;
+  int64_t div = 2;
+  for (;;) {
+    const int64_t t = (l + div - 1) / div;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+    div <<= 1;
+  }
+  const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+  const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+  int32_t c0, c1;
+  if (lo == g) {
+    c0 = ~(int32_t)g;
+    parent_leaf[g] = (int32_t)i;
+  } else {
+    c0 = (int32_t)g;
+    parent_int[g] = (int32_t)i;
+  }
+  if (hi == g + 1) {
+    c1 = ~(int32_t)(g + 1);
+    parent_leaf[g + 1] = (int32_t)i;
+  } else {
+    c1 = (int32_t)(g + 1);
+    parent_int[g + 1] = (int32_t)i;
+  }
+  nodes[i].f3 = make_float4(__int_as_float(c0), __int_as_float(c1), 0.0f, 0.0f);
+  if (i == 0) parent_int[0] = -1;
+}
+
+__global__ __launch_bounds__(256) void k_leaves(const float *__restrict__ V,
+                                                const uint32_t *__restrict__ T,
+                                                const uint32_t *__restrict__ order, int64_t n,
+                                                Tri *__restrict__ tris) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t t = order[j];
+  const float *a = V + 3 * (size_t)T[3 * (size_t)t];
+  const float *b = V + 3 * (size_t)T[3 * (size_t)t + 1];
+  const float *c = V + 3 * (size_t)T[3 * (size_t)t + 2];
+  Tri r;
+  r.a = make_float4(a[0], a[1], a[2], __uint_as_float(t));
+  r.b = make_float4(b[0], b[1], b[2], 0.0f);
+  r.c = make_float4(c[0], c[1], c[2], 0.0f);
+  tris[j] = r;
+}
+
+__device__ __forceinline__ void put_box(Node *nd, int slot, const float lo[3], const float hi[3]) {
+  float *f = reinterpret_cast<float *>(nd);
+  const int o = slot * 6;
+  // plain stores; made visible by the release half of the counter update
+  f[o + 0] = lo[0];
+  f[o + 1] = lo[1];
+  f[o + 2] = lo[2];
+  f[o + 3] = hi[0];
+  f[o + 4] = hi[1];
+  f[o + 5] = hi[2];
+}
+
+__global__ __launch_bounds__(256) void k_refit(const Tri *__restrict__ tris, int64_t n,
+                                               Node *nodes, const int32_t *__restrict__ parent_int,
+                                               const int32_t *__restrict__ parent_leaf,
+                                               uint32_t *flags, const unsigned *__restrict__ bounds,
+                                               float *root_box) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float ext = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    ext = fmaxf(ext, fabsf(ord2f(bounds[6 + k])) + fabsf(ord2f(bounds[9 + k])));
+  const float eps = ext * 7.62939453125e-06f + 1e-30f;  // 2^-17 of the extent
+  const Tri t = tris[j];
+  float lo[3] = {fminf(t.a.x, fminf(t.b.x, t.c.x)) - eps, fminf(t.a.y, fminf(t.b.y, t.c.y)) - eps,
+                 fminf(t.a.z, fminf(t.b.z, t.c.z)) - eps};
+  float hi[3] = {fmaxf(t.a.x, fmaxf(t.b.x, t.c.x)) + eps, fmaxf(t.a.y, fmaxf(t.b.y, t.c.y)) + eps,
+                 fmaxf(t.a.z, fmaxf(t.b.z, t.c.z)) + eps};
+  if (n == 1) {
+    Node *nd = &nodes[0];
+    put_box(nd, 0, lo, hi);
+    const float elo[3] = {INFINITY, INFINITY, INFINITY}, ehi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    put_box(nd, 1, elo, ehi);
+    nd->f3 = make_float4(__int_as_float(~0), __int_as_float(~0), 0.0f, 0.0f);
+    for (int k = 0; k < 3; ++k) root_box[k] = lo[k], root_box[3 + k] = hi[k];
+    return;
+  }
+  int32_t code = ~(int32_t)j;
+  int32_t x = parent_leaf[j];
+  // a valid tree is at most 30 + 30 + 1 levels deep; the cap only guards the walk
+  for (int hop = 0; hop < 128; ++hop) {
+    Node *nd = &nodes[x];
+    const int32_t c0 = __float_as_int(nd->f3.x);
+    put_box(nd, c0 == code ? 0 : 1, lo, hi);
+    const uint32_t old =
+        __hip_atomic_fetch_add(&flags[x], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    // the acquire half of the counter update makes the sibling's stores visible
+    const float *f = reinterpret_cast<const float *>(nd);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fminf(f[k], f[6 + k]);
+      hi[k] = fmaxf(f[3 + k], f[9 + k]);
+    }
+    if (x == 0) {
+      for (int k = 0; k < 3; ++k) root_box[k] = lo[k], root_box[3 + k] = hi[k];
+      return;
+    }
+    code = x;
+    x = parent_int[x];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_depth(int64_t n, const int32_t *__restrict__ parent_int,
+                                               const int32_t *__restrict__ parent_leaf,
+                                               int32_t *depth) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int32_t d = 1;
+  if (n > 1) {
+    int32_t x = parent_leaf[j];
+    while (x >= 0 && d < 1000) {
+      ++d;
+      x = parent_int[x];
+    }
+  } else {
+    d = 2;
+  }
+  atomicMax(depth, d);
+}
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace
+
+void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T, int64_t nT,
+                vacd_bvh *out) {
+  (void)nV;
+  VACD_REQUIRE(nT >= 1 && nT < (int64_t(1) << 30), VACD_ERR_LIMIT,
+               "BVH needs 1 <= n_tris < 2^30");
+  hipStream_t st = ctx->stream;
+  const int64_t ntiles = (nT + kTile - 1) / kTile;
+  const int64_t n_int = nT > 1 ? nT - 1 : 1;
+  // scratch: bounds(12 u32) | depth | keys x2 | vals x2 | hist | parents | flags | root box
+  const size_t hist_n = (size_t)256 * ntiles;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  };
+  const size_t o_bounds = take(16 * 4), o_depth = take(4), o_k0 = take(4 * nT),
+               o_k1 = take(4 * nT), o_v0 = take(4 * nT), o_v1 = take(4 * nT),
+               o_hist = take(4 * hist_n), o_pint = take(4 * n_int), o_pleaf = take(4 * nT),
+               o_flags = take(4 * n_int), o_root = take(6 * 4);
+  vacd::DevBuf &scratch = ctx->buf[SLOT_SORT];
+  scratch.ensure(off);
+  char *base = scratch.as<char>();
+  unsigned *bounds = reinterpret_cast<unsigned *>(base + o_bounds);
+  int32_t *depth = reinterpret_cast<int32_t *>(base + o_depth);
+  uint32_t *k0 = reinterpret_cast<uint32_t *>(base + o_k0);
+  uint32_t *k1 = reinterpret_cast<uint32_t *>(base + o_k1);
+  uint32_t *v0 = reinterpret_cast<uint32_t *>(base + o_v0);
+  uint32_t *v1 = reinterpret_cast<uint32_t *>(base + o_v1);
+  uint32_t *hist = reinterpret_cast<uint32_t *>(base + o_hist);
+  int32_t *pint = reinterpret_cast<int32_t *>(base + o_pint);
+  int32_t *pleaf = reinterpret_cast<int32_t *>(base + o_pleaf);
+  uint32_t *flags = reinterpret_cast<uint32_t *>(base + o_flags);
+  float *root = reinterpret_cast<float *>(base + o_root);
+
+  out->n_tris = nT;
+  out->nodes.ensure(sizeof(Node) * (size_t)n_int);
+  out->tris.ensure(sizeof(Tri) * (size_t)nT);
+  Node *nodes = out->nodes.as<Node>();
+  Tri *tris = out->tris.as<Tri>();
+
+  k_bounds_init<<<1, 64, 0, st>>>(bounds);
+  const unsigned gb = (unsigned)std::min<int64_t>(grid_for(nT, 256), 1024);
+  k_bounds<<<gb, 256, 0, st>>>(d_V, d_T, nT, bounds);
+  k_morton<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, bounds, k0, v0);
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 8 * pass;
+    k_hist<<<(unsigned)ntiles, kTile, 0, st>>>(k0, nT, shift, hist, (int)ntiles);
+    k_scan_u32<<<1, 1024, 0, st>>>(hist, (int64_t)hist_n);
+    k_scatter<<<(unsigned)ntiles, kTile, 0, st>>>(k0, v0, nT, shift, hist, (int)ntiles, k1, v1);
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  VACD_HIP(hipGetLastError());
+  VACD_HIP(hipMemsetAsync(flags, 0, 4 * n_int, st));
+  VACD_HIP(hipMemsetAsync(depth, 0, 4, st));
+  if (nT > 1) {
+    VACD_HIP(hipMemsetAsync(nodes, 0, sizeof(Node) * (size_t)n_int, st));
+    k_karras<<<grid_for(nT - 1, 256), 256, 0, st>>>(k0, nT, nodes, pint, pleaf);
+  }
+  k_leaves<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, v0, nT, tris);
+  k_refit<<<grid_for(nT, 256), 256, 0, st>>>(tris, nT, nodes, pint, pleaf, flags, bounds, root);
+  k_depth<<<grid_for(nT, 256), 256, 0, st>>>(nT, pint, pleaf, depth);
+  VACD_HIP(hipGetLastError());
+  int32_t h_depth = 0;
+  float h_root[6];
+  VACD_HIP(hipMemcpyAsync(&h_depth, depth, 4, hipMemcpyDeviceToHost, st));
+  VACD_HIP(hipMemcpyAsync(h_root, root, 24, hipMemcpyDeviceToHost, st));
+  VACD_HIP(hipStreamSynchronize(st));
+  out->depth = h_depth;
+  for (int k = 0; k < 6; ++k) out->bounds[k] = h_root[k];
+}
+
+}  // namespace vacd

@@ -1,0 +1,339 @@
+// C ABI of libvacd (declared in include/vacd.h).
+//
+// Error convention: every entry point catches vacd::Error, stores its message
+// on the context (or in a thread-local slot when there is none) and returns
+// its code; the pybind layer turns a non-zero code into RuntimeError, where
+// the reference threw std::runtime_error from CUCHK/OCHK
+// (lib/include/optixUtils.hpp:56-71).
+#include "internal.hpp"
+
+#include <cstring>
+#include <new>
+
+using vacd::Error;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(vacd_ctx *ctx, const Error &e) {
+  if (ctx) ctx->last_error = e.msg;
+  g_last_error = e.msg;
+  return e.code;
+}
+
+template <class F>
+int guarded(vacd_ctx *ctx, F &&f) {
+  try {
+    f();
+    return VACD_OK;
+  } catch (const Error &e) {
+    return fail(ctx, e);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, Error{VACD_ERR_HIP, "host allocation failed"});
+  } catch (...) {
+    return fail(ctx, Error{VACD_ERR_HIP, "unknown failure"});
+  }
+}
+
+void set_device(const vacd_ctx *ctx) { VACD_HIP(hipSetDevice(ctx->device)); }
+
+template <class T>
+T *stage(vacd_ctx *ctx, int slot, const T *host, size_t count) {
+  vacd::DevBuf &b = ctx->buf[slot];
+  b.ensure(sizeof(T) * (count ? count : 1));
+  if (count)
+    VACD_HIP(hipMemcpyAsync(b.ptr, host, sizeof(T) * count, hipMemcpyHostToDevice, ctx->stream));
+  return b.as<T>();
+}
+
+bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int vacd_ctx_create(int device, vacd_ctx **out) {
+  if (!out) return fail(nullptr, Error{VACD_ERR_INVALID, "out is NULL"});
+  *out = nullptr;
+  vacd_ctx *ctx = nullptr;
+  int rc = guarded(nullptr, [&] {
+    int ndev = 0;
+    VACD_HIP(hipGetDeviceCount(&ndev));
+    VACD_REQUIRE(device >= 0 && device < ndev, VACD_ERR_INVALID, "no such HIP device");
+    ctx = new vacd_ctx();
+    ctx->device = device;
+    VACD_HIP(hipSetDevice(device));
+    VACD_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+    ctx->stream = ctx->own_stream;
+  });
+  if (rc != VACD_OK) {
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return VACD_OK;
+}
+
+int vacd_ctx_destroy(vacd_ctx *ctx) {
+  if (!ctx) return VACD_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &b : ctx->buf) b.release();
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+  return VACD_OK;
+}
+
+const char *vacd_last_error(const vacd_ctx *ctx) {
+  return ctx ? ctx->last_error.c_str() : g_last_error.c_str();
+}
+
+int vacd_ctx_stream(vacd_ctx *ctx, void **stream) {
+  if (!ctx || !stream) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  *stream = (void *)ctx->stream;
+  return VACD_OK;
+}
+
+int vacd_ctx_set_stream(vacd_ctx *ctx, void *stream) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+  return VACD_OK;
+}
+
+int vacd_sync(vacd_ctx *ctx) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+// ------------------------------------------------------------------ BVH
+static int bvh_build_impl(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+                          bool device_ptrs, vacd_bvh **out) {
+  if (!ctx || !out) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  *out = nullptr;
+  vacd_bvh *b = nullptr;
+  int rc = guarded(ctx, [&] {
+    VACD_REQUIRE(V && T, VACD_ERR_INVALID, "NULL mesh arrays");
+    VACD_REQUIRE(nV >= 1 && nT >= 1, VACD_ERR_INVALID, "empty mesh");
+    VACD_REQUIRE(nT < (int64_t(1) << 30) && nV < (int64_t(1) << 32), VACD_ERR_LIMIT,
+                 "mesh too large");
+    set_device(ctx);
+    const float *dV = V;
+    const uint32_t *dT = T;
+    if (!device_ptrs) {
+      // indices are validated on the host copy
+      for (int64_t t = 0; t < 3 * nT; ++t)
+        VACD_REQUIRE((int64_t)T[t] < nV, VACD_ERR_INVALID, "triangle index out of range");
+      dV = stage(ctx, SLOT_VERTS, V, (size_t)(3 * nV));
+      dT = stage(ctx, SLOT_TRIS, T, (size_t)(3 * nT));
+    }
+    b = new vacd_bvh();
+    b->device = ctx->device;
+    vacd::build_lbvh(ctx, dV, nV, dT, nT, b);
+  });
+  if (rc != VACD_OK) {
+    if (b) {
+      b->nodes.release();
+      b->tris.release();
+      delete b;
+    }
+    return rc;
+  }
+  *out = b;
+  return VACD_OK;
+}
+
+int vacd_bvh_build(vacd_ctx *ctx, const float *vertices, int64_t n_vertices, const uint32_t *tris,
+                   int64_t n_tris, vacd_bvh **out) {
+  return bvh_build_impl(ctx, vertices, n_vertices, tris, n_tris, false, out);
+}
+
+int vacd_bvh_build_d(vacd_ctx *ctx, const float *d_vertices, int64_t n_vertices,
+                     const uint32_t *d_tris, int64_t n_tris, vacd_bvh **out) {
+  return bvh_build_impl(ctx, d_vertices, n_vertices, d_tris, n_tris, true, out);
+}
+
+int vacd_bvh_destroy(vacd_bvh *bvh) {
+  if (!bvh) return VACD_OK;
+  (void)hipSetDevice(bvh->device);
+  bvh->nodes.release();
+  bvh->tris.release();
+  delete bvh;
+  return VACD_OK;
+}
+
+int vacd_bvh_info(const vacd_bvh *bvh, int64_t *n_tris, int32_t *depth, float *bounds6) {
+  if (!bvh) return fail(nullptr, Error{VACD_ERR_INVALID, "NULL BVH"});
+  if (n_tris) *n_tris = bvh->n_tris;
+  if (depth) *depth = bvh->depth;
+  if (bounds6)
+    for (int k = 0; k < 6; ++k) bounds6[k] = bvh->bounds[k];
+  return VACD_OK;
+}
+
+int vacd_bvh_export(const vacd_bvh *bvh, void *nodes, void *tris) {
+  if (!bvh) return fail(nullptr, Error{VACD_ERR_INVALID, "NULL BVH"});
+  return guarded(nullptr, [&] {
+    VACD_HIP(hipSetDevice(bvh->device));
+    const int64_t n_int = bvh->n_tris > 1 ? bvh->n_tris - 1 : 1;
+    if (nodes)
+      VACD_HIP(hipMemcpy(nodes, bvh->nodes.ptr, sizeof(vacd::Node) * n_int, hipMemcpyDeviceToHost));
+    if (tris)
+      VACD_HIP(hipMemcpy(tris, bvh->tris.ptr, sizeof(vacd::Tri) * bvh->n_tris,
+                         hipMemcpyDeviceToHost));
+  });
+}
+
+// ------------------------------------------------------------ visibility
+int vacd_visibility_d(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8_t *d_new_mask,
+                      const vacd_bvh *self, const vacd_bvh *cage, uint32_t *d_pairs, int64_t cap,
+                      int64_t *d_count) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(self && d_count, VACD_ERR_INVALID, "NULL argument");
+    VACD_REQUIRE(n >= 0 && cap >= 0, VACD_ERR_INVALID, "negative size");
+    VACD_REQUIRE(n < 2 || d_pts, VACD_ERR_INVALID, "NULL points");
+    VACD_REQUIRE(cap == 0 || d_pairs, VACD_ERR_INVALID, "NULL pair buffer");
+    VACD_REQUIRE(self->device == ctx->device && (!cage || cage->device == ctx->device),
+                 VACD_ERR_INVALID, "BVH built on another device");
+    set_device(ctx);
+    uint32_t *pairs = d_pairs;
+    if (cap > 0 && !aligned(d_pairs, 8)) {
+      vacd::DevBuf &pb = ctx->buf[SLOT_PAIRS];
+      pb.ensure(8 * (size_t)cap);
+      pairs = pb.as<uint32_t>();
+    }
+    vacd::launch_visibility(ctx, d_pts, n, d_new_mask, self, cage, pairs, cap, d_count);
+    if (pairs != d_pairs)
+      VACD_HIP(hipMemcpyAsync(d_pairs, pairs, 8 * (size_t)cap, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+  });
+}
+
+int vacd_visibility(vacd_ctx *ctx, const float *pts, int64_t n, const uint8_t *new_mask,
+                    const vacd_bvh *self, const vacd_bvh *cage, uint32_t *pairs, int64_t cap,
+                    int64_t *count) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  int64_t total = 0;
+  int rc = guarded(ctx, [&] {
+    VACD_REQUIRE(self && count, VACD_ERR_INVALID, "NULL argument");
+    VACD_REQUIRE(n >= 0 && cap >= 0, VACD_ERR_INVALID, "negative size");
+    VACD_REQUIRE(n < 2 || pts, VACD_ERR_INVALID, "NULL points");
+    VACD_REQUIRE(cap == 0 || pairs, VACD_ERR_INVALID, "NULL pair buffer");
+    VACD_REQUIRE(n <= (int64_t(1) << 26), VACD_ERR_LIMIT, "visibility needs n <= 2^26");
+    set_device(ctx);
+    const float *dp = stage(ctx, SLOT_PTS, pts, (size_t)(3 * n));
+    const uint8_t *dm = new_mask ? stage(ctx, SLOT_MASK, new_mask, (size_t)n) : nullptr;
+    vacd::DevBuf &pb = ctx->buf[SLOT_PAIRS];
+    pb.ensure(8 * (size_t)(cap > 0 ? cap : 1));
+    vacd::DevBuf &cb = ctx->buf[SLOT_COUNT];
+    cb.ensure(sizeof(int64_t));
+    vacd::launch_visibility(ctx, dp, n, dm, self, cage, pb.as<uint32_t>(), cap, cb.as<int64_t>());
+    VACD_HIP(hipMemcpyAsync(&total, cb.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long bad = 0;
+    if (n >= 2)
+      VACD_HIP(hipMemcpyAsync(&bad, ctx->buf[SLOT_STATS].as<unsigned long long>() + 4,
+                              sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+    VACD_REQUIRE(bad == 0, VACD_ERR_HIP, "visibility: BVH walk exceeded its node budget");
+    const int64_t ncopy = total < cap ? total : cap;
+    if (ncopy > 0)
+      VACD_HIP(hipMemcpy(pairs, pb.ptr, 8 * (size_t)ncopy, hipMemcpyDeviceToHost));
+  });
+  if (rc != VACD_OK) return rc;
+  *count = total;
+  if (total > cap)
+    return fail(ctx, Error{VACD_ERR_CAPACITY, "visibility: more edges than the pair buffer holds"});
+  return VACD_OK;
+}
+
+int vacd_visibility_stats(vacd_ctx *ctx, vacd_vis_stats *out) {
+  if (!ctx || !out) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    unsigned long long s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->buf[SLOT_STATS].ptr)
+      VACD_HIP(hipMemcpy(s, ctx->buf[SLOT_STATS].ptr, sizeof(s), hipMemcpyDeviceToHost));
+    out->segments = ctx->last_segments;
+    out->masked = (int64_t)s[0];
+    out->blocked = (int64_t)s[1];
+    out->exact_calls = (int64_t)s[2];
+    out->edges = (int64_t)s[3];
+  });
+}
+
+// ------------------------------------------------------------ scoring
+int vacd_score_planes_d(vacd_ctx *ctx, const float *d_planes, int64_t P, const float *d_pts,
+                        int64_t n, const uint32_t *d_edges, int64_t E, int64_t *d_score_q32,
+                        float *d_score_f, int32_t *d_argmax) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(P >= 0 && n >= 0 && E >= 0, VACD_ERR_INVALID, "negative size");
+    VACD_REQUIRE(P == 0 || d_planes, VACD_ERR_INVALID, "NULL planes");
+    VACD_REQUIRE(E == 0 || (d_edges && d_pts), VACD_ERR_INVALID, "NULL edges or points");
+    set_device(ctx);
+    const float *pl = d_planes;
+    if (P > 0 && !aligned(d_planes, 16)) {
+      vacd::DevBuf &b = ctx->buf[SLOT_PLANES];
+      b.ensure(16 * (size_t)P);
+      VACD_HIP(hipMemcpyAsync(b.ptr, d_planes, 16 * (size_t)P, hipMemcpyDeviceToDevice, ctx->stream));
+      pl = b.as<float>();
+    }
+    const uint32_t *ed = d_edges;
+    if (E > 0 && !aligned(d_edges, 8)) {
+      vacd::DevBuf &b = ctx->buf[SLOT_EDGES];
+      b.ensure(8 * (size_t)E);
+      VACD_HIP(hipMemcpyAsync(b.ptr, d_edges, 8 * (size_t)E, hipMemcpyDeviceToDevice, ctx->stream));
+      ed = b.as<uint32_t>();
+    }
+    vacd::launch_scoring(ctx, pl, P, d_pts, n, ed, E, d_score_q32, d_score_f, d_argmax);
+  });
+}
+
+int vacd_score_planes(vacd_ctx *ctx, const float *planes, int64_t P, const float *pts, int64_t n,
+                      const uint32_t *edges, int64_t E, int64_t *score_q32, float *score_f,
+                      int32_t *argmax) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(P >= 0 && n >= 0 && E >= 0, VACD_ERR_INVALID, "negative size");
+    VACD_REQUIRE(P == 0 || planes, VACD_ERR_INVALID, "NULL planes");
+    VACD_REQUIRE(E == 0 || (edges && pts), VACD_ERR_INVALID, "NULL edges or points");
+    set_device(ctx);
+    const float *dpl = stage(ctx, SLOT_PLANES, planes, (size_t)(4 * P));
+    const float *dpt = stage(ctx, SLOT_PTS, pts, (size_t)(3 * n));
+    const uint32_t *ded = stage(ctx, SLOT_EDGES, edges, (size_t)(2 * E));
+    vacd::DevBuf &q = ctx->buf[SLOT_SCORES_F];  // q32 then f32 view
+    q.ensure(12 * (size_t)(P > 0 ? P : 1));
+    int64_t *dq = q.as<int64_t>();
+    float *df = reinterpret_cast<float *>(q.as<char>() + 8 * (size_t)(P > 0 ? P : 1));
+    vacd::DevBuf &am = ctx->buf[SLOT_ARGMAX];
+    am.ensure(sizeof(int32_t));
+    vacd::launch_scoring(ctx, dpl, P, dpt, n, ded, E, dq, df, am.as<int32_t>());
+    if (score_q32 && P)
+      VACD_HIP(hipMemcpyAsync(score_q32, dq, 8 * (size_t)P, hipMemcpyDeviceToHost, ctx->stream));
+    if (score_f && P)
+      VACD_HIP(hipMemcpyAsync(score_f, df, 4 * (size_t)P, hipMemcpyDeviceToHost, ctx->stream));
+    int32_t h_am = -1;
+    if (P) VACD_HIP(hipMemcpyAsync(&h_am, am.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+    if (argmax) *argmax = h_am;
+  });
+}
+
+void classify_and_rate_planes(const float *planes, const float *points, const unsigned int *edges,
+                              float *scores, int numPlanes, int numPoints, int numEdges) {
+  thread_local vacd_ctx *tctx = nullptr;
+  if (!tctx && vacd_ctx_create(0, &tctx) != VACD_OK) {
+    std::fprintf(stderr, "classify_and_rate_planes: %s\n", vacd_last_error(nullptr));
+    return;
+  }
+  int rc = vacd_score_planes(tctx, planes, numPlanes, points, numPoints, edges, numEdges, nullptr,
+                             scores, nullptr);
+  if (rc != VACD_OK) std::fprintf(stderr, "classify_and_rate_planes: %s\n", vacd_last_error(tctx));
+}
+
+}  // extern "C"

@@ -1,0 +1,630 @@
+// Segment-visibility kernels for gfx950.
+//
+// Replaces the OptiX raygen/any-hit programs (lib/ptx/ray_segments.cu:49-125),
+// the dense uint32 result matrix and the host combine loop of
+// compute_intersection_matrix (lib/src/intersections.cpp:327-385).
+//
+// k_visibility: one pair per lane, 64 consecutive pair ids k per wave
+//   (k = i(2n-i-1)/2 + j-i-1, so a wave's ballot word holds 64 results in k
+//   order).  Each lane walks the self BVH for any contact (short stack of
+//   node ids in LDS, lane-interleaved so the 64 lanes hit 64 distinct
+//   banks), then either walks the cage BVH for any contact (cage mode) or
+//   counts crossings of the +x ray from the segment midpoint (parity mode).
+//   Triangle tests use float64 filters; a test the filter cannot decide is
+//   not guessed: the lane finishes its walk and, unless a certain contact
+//   blocked it, marks the pair deferred.
+// k_vis_exact: every deferred pair is redone with exact predicates.
+// k_word_scan / k_scan_blocks / k_emit: popcount prefix over the ballot
+//   words and emission of (i, j) in k order.
+#include "internal.hpp"
+#include "predicates.hpp"
+
+#include <algorithm>
+
+namespace vacd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kScanTile = 1024;  // ballot words per scan block (256 threads x 4)
+
+__device__ __forceinline__ int64_t row_start(int64_t i, int64_t n) {
+  return i * (2 * n - i - 1) / 2;
+}
+
+// k -> (i, j), i < j: closed form in float64 (lib/ptx/ray_segments.cu:39-46),
+// then exact integer correction.
+__device__ __forceinline__ void pair_of(int64_t k, int64_t n, int64_t &i, int64_t &j) {
+  const double nd = (double)n, kd = (double)k;
+  const double t = sqrt(fmax(0.0, -8.0 * kd + 4.0 * nd * (nd - 1.0) - 7.0));
+  int64_t ii = (int64_t)(nd - 2.0 - floor(t * 0.5 - 0.5));
+  ii = ii < 0 ? 0 : (ii > n - 2 ? n - 2 : ii);
+  while (ii > 0 && row_start(ii, n) > k) --ii;
+  while (ii < n - 2 && row_start(ii + 1, n) <= k) ++ii;
+  i = ii;
+  j = k - row_start(ii, n) + ii + 1;
+}
+
+struct Seg {
+  float p[3], q[3];
+  float inv[3], pinv[3];  // slab test: t = lo*inv - p*inv
+  float m[3];             // midpoint, float32 (box tests only)
+};
+
+__device__ __forceinline__ void make_seg(Seg &s) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float d = s.q[k] - s.p[k];
+    s.inv[k] = fabsf(d) > 1e-30f ? 1.0f / d : 1e30f;
+    s.pinv[k] = s.p[k] * s.inv[k];
+    s.m[k] = 0.5f * (s.p[k] + s.q[k]);
+  }
+}
+
+__device__ __forceinline__ bool seg_box(const Seg &s, float lx, float ly, float lz, float hx,
+                                        float hy, float hz) {
+  const float ax = fmaf(lx, s.inv[0], -s.pinv[0]), bx = fmaf(hx, s.inv[0], -s.pinv[0]);
+  const float ay = fmaf(ly, s.inv[1], -s.pinv[1]), by = fmaf(hy, s.inv[1], -s.pinv[1]);
+  const float az = fmaf(lz, s.inv[2], -s.pinv[2]), bz = fmaf(hz, s.inv[2], -s.pinv[2]);
+  const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+  const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), 1.0f));
+  return t0 <= t1;
+}
+
+__device__ __forceinline__ bool ray_box(const Seg &s, float lx, float ly, float lz, float hx,
+                                        float hy, float hz) {
+  (void)lx;
+  return s.m[1] >= ly && s.m[1] <= hy && s.m[2] >= lz && s.m[2] <= hz && hx >= s.m[0];
+}
+
+// Contact of the segment with a triangle: 0 none, 1 contact, 2 undecided.
+__device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t) {
+  const float a[3] = {t.a.x, t.a.y, t.a.z}, b[3] = {t.b.x, t.b.y, t.b.z},
+              c[3] = {t.c.x, t.c.y, t.c.z};
+  const bool ip = same3(s.p, a) || same3(s.p, b) || same3(s.p, c);
+  const bool iq = same3(s.q, a) || same3(s.q, b) || same3(s.q, c);
+  if (ip && iq) return 1;  // the segment is an edge of the triangle
+  if (ip || iq) {
+    double bound;
+    const double det = o3_fast(a, b, c, ip ? s.q : s.p, bound);
+    return (det > bound || -det > bound) ? 0 : 2;
+  }
+  // plane sides of p and q: det[a-x, b-a, c-a] = orient3d(a, b, c, x)
+  const double e1x = (double)b[0] - a[0], e1y = (double)b[1] - a[1], e1z = (double)b[2] - a[2];
+  const double e2x = (double)c[0] - a[0], e2y = (double)c[1] - a[1], e2z = (double)c[2] - a[2];
+  const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+  const double mx = fabs(e1y * e2z) + fabs(e1z * e2y), my = fabs(e1z * e2x) + fabs(e1x * e2z),
+               mz = fabs(e1x * e2y) + fabs(e1y * e2x);
+  const double apx = (double)a[0] - s.p[0], apy = (double)a[1] - s.p[1], apz = (double)a[2] - s.p[2];
+  const double aqx = (double)a[0] - s.q[0], aqy = (double)a[1] - s.q[1], aqz = (double)a[2] - s.q[2];
+  const double dp = apx * nx + apy * ny + apz * nz;
+  const double dq = aqx * nx + aqy * ny + aqz * nz;
+  const double bp = kFilter * (fabs(apx) * mx + fabs(apy) * my + fabs(apz) * mz);
+  const double bq = kFilter * (fabs(aqx) * mx + fabs(aqy) * my + fabs(aqz) * mz);
+  const bool cp = dp > bp || -dp > bp, cq = dq > bq || -dq > bq;
+  if (cp && cq && ((dp > 0.0) == (dq > 0.0))) return 0;  // strictly one side
+  if (!(cp && cq)) return 2;
+  // opposite sides: does the line pq pass through the triangle?
+  // u = D.(BxC), v = D.(CxA), w = D.(AxB) with A = a-p, B = b-p, C = c-p, D = q-p
+  const double Ax = apx, Ay = apy, Az = apz;
+  const double Bx = (double)b[0] - s.p[0], By = (double)b[1] - s.p[1], Bz = (double)b[2] - s.p[2];
+  const double Cx = (double)c[0] - s.p[0], Cy = (double)c[1] - s.p[1], Cz = (double)c[2] - s.p[2];
+  const double Dx = (double)s.q[0] - s.p[0], Dy = (double)s.q[1] - s.p[1],
+               Dz = (double)s.q[2] - s.p[2];
+  const double u = Dx * (By * Cz - Bz * Cy) + Dy * (Bz * Cx - Bx * Cz) + Dz * (Bx * Cy - By * Cx);
+  const double v = Dx * (Cy * Az - Cz * Ay) + Dy * (Cz * Ax - Cx * Az) + Dz * (Cx * Ay - Cy * Ax);
+  const double w = Dx * (Ay * Bz - Az * By) + Dy * (Az * Bx - Ax * Bz) + Dz * (Ax * By - Ay * Bx);
+  const double adx = fabs(Dx), ady = fabs(Dy), adz = fabs(Dz);
+  const double bu = kFilter * (adx * (fabs(By * Cz) + fabs(Bz * Cy)) +
+                               ady * (fabs(Bz * Cx) + fabs(Bx * Cz)) +
+                               adz * (fabs(Bx * Cy) + fabs(By * Cx)));
+  const double bv = kFilter * (adx * (fabs(Cy * Az) + fabs(Cz * Ay)) +
+                               ady * (fabs(Cz * Ax) + fabs(Cx * Az)) +
+                               adz * (fabs(Cx * Ay) + fabs(Cy * Ax)));
+  const double bw = kFilter * (adx * (fabs(Ay * Bz) + fabs(Az * By)) +
+                               ady * (fabs(Az * Bx) + fabs(Ax * Bz)) +
+                               adz * (fabs(Ax * By) + fabs(Ay * Bx)));
+  const bool pos = u > bu || v > bv || w > bw;
+  const bool neg = -u > bu || -v > bv || -w > bw;
+  if (pos && neg) return 0;  // the line misses the triangle
+  const bool all = (u > bu || -u > bu) && (v > bv || -v > bv) && (w > bw || -w > bw);
+  return all ? 1 : 2;
+}
+
+// +x ray from m = (p+q)/2 against a triangle: 0 no crossing, 1 crossing,
+// 2 undecided.
+__device__ __forceinline__ int cross_fast(const Seg &s, const Tri &t) {
+  const double my = 0.5 * ((double)s.p[1] + (double)s.q[1]);
+  const double mz = 0.5 * ((double)s.p[2] + (double)s.q[2]);
+  const double mag = fmax(fabs(my), fabs(mz));
+  const double ay = (double)t.a.y - my, az = (double)t.a.z - mz;
+  const double by = (double)t.b.y - my, bz = (double)t.b.z - mz;
+  const double cy = (double)t.c.y - my, cz = (double)t.c.z - mz;
+  // orient2d(a, b, m), orient2d(b, c, m), orient2d(c, a, m) in (y, z)
+  const double o1 = ay * bz - az * by, o2 = by * cz - bz * cy, o3 = cy * az - cz * ay;
+  const double b1 = kFilter * (fabs(ay * bz) + fabs(az * by) +
+                               mag * (fabs((double)t.a.z - t.b.z) + fabs((double)t.b.y - t.a.y)));
+  const double b2 = kFilter * (fabs(by * cz) + fabs(bz * cy) +
+                               mag * (fabs((double)t.b.z - t.c.z) + fabs((double)t.c.y - t.b.y)));
+  const double b3 = kFilter * (fabs(cy * az) + fabs(cz * ay) +
+                               mag * (fabs((double)t.c.z - t.a.z) + fabs((double)t.a.y - t.c.y)));
+  const bool pos = o1 > b1 || o2 > b2 || o3 > b3;
+  const bool neg = -o1 > b1 || -o2 > b2 || -o3 > b3;
+  if (pos && neg) return 0;
+  const bool all = (o1 > b1 || -o1 > b1) && (o2 > b2 || -o2 > b2) && (o3 > b3 || -o3 > b3);
+  if (!all) return 2;
+  // the ray line passes inside; crossing at t > 0 iff orient3d(a,b,c,m) has
+  // the sign of the projected orientation.
+  const double mx = 0.5 * ((double)s.p[0] + (double)s.q[0]);
+  const double ax = (double)t.a.x - mx, bx = (double)t.b.x - mx, cx = (double)t.c.x - mx;
+  const double det = ax * (by * cz - bz * cy) + bx * (cy * az - cz * ay) + cx * (ay * bz - az * by);
+  const double mag3 = fmax(mag, fabs(mx));
+  const double bd = kFilter * (fabs(ax) * (fabs(by * cz) + fabs(bz * cy)) +
+                               fabs(bx) * (fabs(cy * az) + fabs(cz * ay)) +
+                               fabs(cx) * (fabs(ay * bz) + fabs(az * by)) +
+                               mag3 * ((fabs(by) + fabs(cy)) * (fabs(bz) + fabs(cz)) + (fabs(ay) + fabs(cy)) * (fabs(az) + fabs(cz)) + (fabs(ay) + fabs(by)) * (fabs(az) + fabs(bz))));
+  if (!(det > bd || -det > bd)) return 2;
+  return ((det > 0.0) == pos) ? 1 : 0;
+}
+
+__device__ __forceinline__ void load_node(const Node *__restrict__ nodes, int idx, float bx[12],
+                                          int &c0, int &c1) {
+  const Node nd = nodes[idx];
+  bx[0] = nd.f0.x, bx[1] = nd.f0.y, bx[2] = nd.f0.z, bx[3] = nd.f0.w;
+  bx[4] = nd.f1.x, bx[5] = nd.f1.y, bx[6] = nd.f1.z, bx[7] = nd.f1.w;
+  bx[8] = nd.f2.x, bx[9] = nd.f2.y, bx[10] = nd.f2.z, bx[11] = nd.f2.w;
+  c0 = __float_as_int(nd.f3.x);
+  c1 = __float_as_int(nd.f3.y);
+}
+
+// Any contact along the segment: 1 contact, 0 none, 2 none certain but some
+// test undecided.  stk: this lane's LDS stack, stride 64 entries.
+template <int STACK>
+__device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
+                                        const Seg &s, int *stk, int cap) {
+  int node = 0, sp = 0;
+  bool unsure = false;
+  for (int it = 0;; ++it) {
+    if (it > cap) return 3;  // visited more nodes than the tree has: malformed tree
+    float bx[12];
+    int c0, c1;
+    load_node(nodes, node, bx, c0, c1);
+    bool h0 = seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+    bool h1 = seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+    if (h0 && c0 < 0) {
+      const int r = contact_fast(s, tris[~c0]);
+      if (r == 1) return 1;
+      unsure |= r == 2;
+      h0 = false;
+    }
+    if (h1 && c1 < 0) {
+      const int r = contact_fast(s, tris[~c1]);
+      if (r == 1) return 1;
+      unsure |= r == 2;
+      h1 = false;
+    }
+    if (h0 && h1) {
+      stk[(sp++) * 64] = c1;
+      node = c0;
+    } else if (h0) {
+      node = c0;
+    } else if (h1) {
+      node = c1;
+    } else {
+      if (sp == 0) break;
+      node = stk[(--sp) * 64];
+    }
+  }
+  return unsure ? 2 : 0;
+}
+
+// Crossing parity of the midpoint ray: 0 even, 1 odd, 2 undecided.
+template <int STACK>
+__device__ __forceinline__ int walk_parity(const Node *__restrict__ nodes,
+                                           const Tri *__restrict__ tris, const Seg &s, int *stk,
+                                           int cap) {
+  int node = 0, sp = 0, cnt = 0;
+  bool unsure = false;
+  for (int it = 0;; ++it) {
+    if (it > cap) return 3;
+    float bx[12];
+    int c0, c1;
+    load_node(nodes, node, bx, c0, c1);
+    bool h0 = ray_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+    bool h1 = ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+    if (h0 && c0 < 0) {
+      const int r = cross_fast(s, tris[~c0]);
+      cnt += r == 1;
+      unsure |= r == 2;
+      h0 = false;
+    }
+    if (h1 && c1 < 0) {
+      const int r = cross_fast(s, tris[~c1]);
+      cnt += r == 1;
+      unsure |= r == 2;
+      h1 = false;
+    }
+    if (h0 && h1) {
+      stk[(sp++) * 64] = c1;
+      node = c0;
+    } else if (h0) {
+      node = c0;
+    } else if (h1) {
+      node = c1;
+    } else {
+      if (sp == 0) break;
+      node = stk[(--sp) * 64];
+    }
+  }
+  if (unsure) return 2;
+  return cnt & 1;
+}
+
+struct VisArgs {
+  const float *pts;
+  const uint8_t *mask;
+  int64_t n, S, nwords;
+  const Node *snodes;
+  const Tri *stris;
+  const Node *cnodes;
+  const Tri *ctris;
+  uint64_t *bits;
+  uint64_t *defer;
+  unsigned long long *stats;  // [0] masked, [1] blocked, [2] deferred, [3] edges, [4] malformed
+  int scap, ccap;             // node-visit caps (2 * n_tris + 64)
+};
+
+template <int STACK, bool CAGE>
+__global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
+  __shared__ int stk_all[kWaves][STACK][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int *stk = &stk_all[wave][0][lane];
+  unsigned n_masked = 0, n_blocked = 0, n_defer = 0, n_bad = 0;
+  const int64_t total_waves = (int64_t)gridDim.x * kWaves;
+  for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < a.nwords; w += total_waves) {
+    const int64_t k = w * 64 + lane;
+    bool edge = false, defer = false;
+    if (k < a.S) {
+      int64_t i, j;
+      pair_of(k, a.n, i, j);
+      if (a.mask && !a.mask[i] && !a.mask[j]) {
+        ++n_masked;
+      } else {
+        Seg s;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.p[c] = a.pts[3 * i + c], s.q[c] = a.pts[3 * j + c];
+        if (!same3(s.p, s.q)) {
+          make_seg(s);
+          const int hs = walk_any<STACK>(a.snodes, a.stris, s, stk, a.scap);
+          if (hs == 1) {
+            ++n_blocked;
+          } else if (hs == 2) {
+            defer = true;
+          } else if (hs == 3) {
+            ++n_bad;
+          } else if (CAGE) {
+            const int hc = walk_any<STACK>(a.cnodes, a.ctris, s, stk, a.ccap);
+            edge = hc == 1;
+            defer = hc == 2;
+            n_bad += hc == 3;
+          } else {
+            const int par = walk_parity<STACK>(a.snodes, a.stris, s, stk, a.scap);
+            edge = par == 0;
+            defer = par == 2;
+            n_bad += par == 3;
+          }
+        }
+      }
+    }
+    const uint64_t eb = __ballot(edge), db = __ballot(defer);
+    n_defer += defer;
+    if (lane == 0) {
+      a.bits[w] = eb;
+      a.defer[w] = db;
+    }
+  }
+  // per-wave totals, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    n_masked += __shfl_down(n_masked, off);
+    n_blocked += __shfl_down(n_blocked, off);
+    n_defer += __shfl_down(n_defer, off);
+    n_bad += __shfl_down(n_bad, off);
+  }
+  if (lane == 0) {
+    if (n_masked) atomicAdd(&a.stats[0], (unsigned long long)n_masked);
+    if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
+    if (n_defer) atomicAdd(&a.stats[2], (unsigned long long)n_defer);
+    if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
+  }
+}
+
+// ------------------------------------------------------------ exact redo
+__device__ bool exact_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
+                          const Seg &s, int cap) {
+  int stk[64];
+  int node = 0, sp = 0;
+  for (int it = 0; it <= cap; ++it) {
+    float bx[12];
+    int c0, c1;
+    load_node(nodes, node, bx, c0, c1);
+    int ch[2] = {c0, c1};
+    bool h[2] = {seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]),
+                 seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11])};
+    for (int t = 0; t < 2; ++t) {
+      if (h[t] && ch[t] < 0) {
+        const Tri tr = tris[~ch[t]];
+        int r = contact_fast(s, tr);
+        if (r == 2) {
+          const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
+                      c3[3] = {tr.c.x, tr.c.y, tr.c.z};
+          r = contact_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
+        }
+        if (r == 1) return true;
+        h[t] = false;
+      }
+    }
+    if (h[0] && h[1]) {
+      stk[sp++] = ch[1];
+      node = ch[0];
+    } else if (h[0]) {
+      node = ch[0];
+    } else if (h[1]) {
+      node = ch[1];
+    } else {
+      if (sp == 0) break;
+      node = stk[--sp];
+    }
+  }
+  return false;
+}
+
+__device__ int exact_parity(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
+                            const Seg &s, int cap) {
+  int stk[64];
+  int node = 0, sp = 0, cnt = 0;
+  for (int it = 0; it <= cap; ++it) {
+    float bx[12];
+    int c0, c1;
+    load_node(nodes, node, bx, c0, c1);
+    int ch[2] = {c0, c1};
+    bool h[2] = {ray_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]),
+                 ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11])};
+    for (int t = 0; t < 2; ++t) {
+      if (h[t] && ch[t] < 0) {
+        const Tri tr = tris[~ch[t]];
+        int r = cross_fast(s, tr);
+        if (r == 2) {
+          const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
+                      c3[3] = {tr.c.x, tr.c.y, tr.c.z};
+          r = ray_cross_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
+        }
+        cnt += r;
+        h[t] = false;
+      }
+    }
+    if (h[0] && h[1]) {
+      stk[sp++] = ch[1];
+      node = ch[0];
+    } else if (h[0]) {
+      node = ch[0];
+    } else if (h[1]) {
+      node = ch[1];
+    } else {
+      if (sp == 0) break;
+      node = stk[--sp];
+    }
+  }
+  return cnt & 1;
+}
+
+template <bool CAGE>
+__global__ __launch_bounds__(64) void k_vis_exact(VisArgs a) {
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.nwords;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t dm = a.defer[w];
+    if (!dm) continue;
+    uint64_t eb = a.bits[w];
+    unsigned blocked = 0;
+    while (dm) {
+      const int b = __ffsll((unsigned long long)dm) - 1;
+      dm &= dm - 1;
+      int64_t i, j;
+      pair_of(w * 64 + b, a.n, i, j);
+      Seg s;
+      for (int c = 0; c < 3; ++c) s.p[c] = a.pts[3 * i + c], s.q[c] = a.pts[3 * j + c];
+      make_seg(s);
+      bool edge;
+      if (exact_any(a.snodes, a.stris, s, a.scap)) {
+        edge = false;
+        ++blocked;
+      } else if (CAGE) {
+        edge = exact_any(a.cnodes, a.ctris, s, a.ccap);
+      } else {
+        edge = exact_parity(a.snodes, a.stris, s, a.scap) == 0;
+      }
+      if (edge) eb |= 1ull << b;
+    }
+    a.bits[w] = eb;
+    if (blocked) atomicAdd(&a.stats[1], (unsigned long long)blocked);
+  }
+}
+
+// ------------------------------------------------------------ compaction
+__device__ __forceinline__ unsigned long long block_excl_scan(unsigned long long v,
+                                                              unsigned long long *sh,
+                                                              unsigned long long &total) {
+  // 256 threads
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    unsigned long long x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    sh[threadIdx.x] += x;
+    __syncthreads();
+  }
+  total = sh[255];
+  const unsigned long long incl = sh[threadIdx.x];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_word_scan(const uint64_t *__restrict__ bits,
+                                                   int64_t nwords,
+                                                   unsigned long long *__restrict__ block_tot) {
+  __shared__ unsigned long long sh[256];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
+  unsigned long long c = 0;
+  for (int t = 0; t < 4; ++t)
+    if (base + t < nwords) c += __popcll(bits[base + t]);
+  unsigned long long total;
+  block_excl_scan(c, sh, total);
+  if (threadIdx.x == 0) block_tot[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(unsigned long long *__restrict__ v,
+                                                      int64_t m, int64_t *__restrict__ count,
+                                                      unsigned long long *stats_edges) {
+  __shared__ unsigned long long part[1024];
+  const int64_t per = (m + 1023) / 1024;
+  const int64_t beg = threadIdx.x * per;
+  const int64_t end = beg + per < m ? beg + per : m;
+  unsigned long long s = 0;
+  for (int64_t i = beg; i < end; ++i) s += v[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    unsigned long long x = threadIdx.x >= off ? part[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    part[threadIdx.x] += x;
+    __syncthreads();
+  }
+  unsigned long long run = threadIdx.x ? part[threadIdx.x - 1] : 0ull;
+  for (int64_t i = beg; i < end; ++i) {
+    const unsigned long long x = v[i];
+    v[i] = run;
+    run += x;
+  }
+  if (threadIdx.x == 1023) {
+    *count = (int64_t)part[1023];
+    *stats_edges = part[1023];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_emit(const uint64_t *__restrict__ bits, int64_t nwords,
+                                              int64_t n, const unsigned long long *__restrict__ block_off,
+                                              uint32_t *__restrict__ pairs, int64_t cap) {
+  __shared__ unsigned long long sh[256];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
+  uint64_t w4[4];
+  unsigned long long c = 0;
+  for (int t = 0; t < 4; ++t) {
+    w4[t] = base + t < nwords ? bits[base + t] : 0ull;
+    c += __popcll(w4[t]);
+  }
+  unsigned long long total;
+  unsigned long long out = block_excl_scan(c, sh, total) + block_off[blockIdx.x];
+  if (c == 0) return;
+  for (int t = 0; t < 4; ++t) {
+    uint64_t m = w4[t];
+    if (!m) continue;
+    int64_t k0 = (base + t) * 64;
+    int64_t i, j;
+    pair_of(k0, n, i, j);
+    int64_t kc = k0;
+    while (m) {
+      const int b = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      int64_t delta = k0 + b - kc;
+      while (delta > 0) {
+        const int64_t room = n - 1 - j;  // entries left in row i after j
+        if (delta <= room) {
+          j += delta;
+          delta = 0;
+        } else {
+          delta -= room + 1;
+          ++i;
+          j = i + 1;
+        }
+      }
+      kc = k0 + b;
+      if ((int64_t)out < cap) {
+        uint2 pr = make_uint2((uint32_t)i, (uint32_t)j);
+        *reinterpret_cast<uint2 *>(pairs + 2 * out) = pr;
+      }
+      ++out;
+    }
+  }
+}
+
+}  // namespace
+
+void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8_t *d_mask,
+                       const vacd_bvh *self, const vacd_bvh *cage, uint32_t *d_pairs, int64_t cap,
+                       int64_t *d_count) {
+  hipStream_t st = ctx->stream;
+  VACD_REQUIRE(n >= 0 && n <= (int64_t(1) << 26), VACD_ERR_LIMIT, "visibility needs n <= 2^26");
+  VACD_REQUIRE(self != nullptr, VACD_ERR_INVALID, "self BVH is required");
+  const int64_t S = n >= 2 ? n * (n - 1) / 2 : 0;
+  const int64_t nwords = (S + 63) / 64;
+  ctx->last_segments = S;
+  vacd::DevBuf &stats = ctx->buf[SLOT_STATS];
+  stats.ensure(8 * sizeof(unsigned long long));
+  VACD_HIP(hipMemsetAsync(stats.ptr, 0, 8 * sizeof(unsigned long long), st));
+  if (S == 0) {
+    VACD_HIP(hipMemsetAsync(d_count, 0, sizeof(int64_t), st));
+    return;
+  }
+  vacd::DevBuf &bits = ctx->buf[SLOT_BITS];
+  vacd::DevBuf &defer = ctx->buf[SLOT_DEFER];
+  vacd::DevBuf &scan = ctx->buf[SLOT_SCAN];
+  bits.ensure(8 * (size_t)nwords);
+  defer.ensure(8 * (size_t)nwords);
+  const int64_t nscan = (nwords + kScanTile - 1) / kScanTile;
+  scan.ensure(8 * (size_t)nscan);
+
+  VisArgs a;
+  a.pts = d_pts;
+  a.mask = d_mask;
+  a.n = n;
+  a.S = S;
+  a.nwords = nwords;
+  a.snodes = self->nodes.as<Node>();
+  a.stris = self->tris.as<Tri>();
+  a.cnodes = cage ? cage->nodes.as<Node>() : nullptr;
+  a.ctris = cage ? cage->tris.as<Tri>() : nullptr;
+  a.bits = bits.as<uint64_t>();
+  a.defer = defer.as<uint64_t>();
+  a.stats = stats.as<unsigned long long>();
+  a.scap = (int)std::min<int64_t>(2 * self->n_tris + 64, INT32_MAX);
+  a.ccap = cage ? (int)std::min<int64_t>(2 * cage->n_tris + 64, INT32_MAX) : 0;
+
+  const int depth = std::max(self->depth, cage ? cage->depth : 0);
+  VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
+  const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
+  const unsigned grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
+  if (depth <= 32) {
+    if (cage)
+      k_visibility<32, true><<<grid, kBlock, 0, st>>>(a);
+    else
+      k_visibility<32, false><<<grid, kBlock, 0, st>>>(a);
+  } else {
+    if (cage)
+      k_visibility<64, true><<<grid, kBlock, 0, st>>>(a);
+    else
+      k_visibility<64, false><<<grid, kBlock, 0, st>>>(a);
+  }
+  VACD_HIP(hipGetLastError());
+  const unsigned egrid = (unsigned)std::min<int64_t>((nwords + 63) / 64, 2048);
+  if (cage)
+    k_vis_exact<true><<<egrid, 64, 0, st>>>(a);
+  else
+    k_vis_exact<false><<<egrid, 64, 0, st>>>(a);
+  VACD_HIP(hipGetLastError());
+  unsigned long long *blk = scan.as<unsigned long long>();
+  k_word_scan<<<(unsigned)nscan, 256, 0, st>>>(a.bits, nwords, blk);
+  k_scan_blocks<<<1, 1024, 0, st>>>(blk, nscan, d_count, a.stats + 3);
+  k_emit<<<(unsigned)nscan, 256, 0, st>>>(a.bits, nwords, n, blk, d_pairs, cap);
+  VACD_HIP(hipGetLastError());
+}
+
+}  // namespace vacd

@@ -1,0 +1,242 @@
+"""ctypes binding of libvacd (the C ABI in ``include/vacd.h``).
+
+This module is the Python-side caller of the HIP engine: it loads the
+in-tree ``libvacd.so`` and fails loudly when it is missing -- there is no
+CPU fallback.  Host (numpy) and device (torch tensor) entry points mirror
+the C ABI one to one.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvacd.so")
+
+VACD_OK = 0
+VACD_ERR_INVALID = 1
+VACD_ERR_HIP = 2
+VACD_ERR_CAPACITY = 3
+VACD_ERR_LIMIT = 4
+
+# Every symbol include/vacd.h declares (checked by tests/test_abi.py).
+EXPORTED = [
+    "vacd_ctx_create", "vacd_ctx_destroy", "vacd_last_error", "vacd_ctx_stream",
+    "vacd_ctx_set_stream", "vacd_sync", "vacd_bvh_build", "vacd_bvh_build_d",
+    "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_export", "vacd_visibility",
+    "vacd_visibility_d", "vacd_visibility_stats", "vacd_score_planes",
+    "vacd_score_planes_d", "classify_and_rate_planes",
+]
+
+
+class VacdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vacd error {code}: {msg}")
+        self.code = code
+
+
+class VisStats(ctypes.Structure):
+    _fields_ = [("segments", ctypes.c_int64), ("masked", ctypes.c_int64),
+                ("blocked", ctypes.c_int64), ("edges", ctypes.c_int64),
+                ("exact_calls", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libvacd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m visacd_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+        L.vacd_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.vacd_ctx_destroy.argtypes = [vp]
+        L.vacd_last_error.argtypes = [vp]
+        L.vacd_last_error.restype = ctypes.c_char_p
+        L.vacd_ctx_stream.argtypes = [vp, ctypes.POINTER(vp)]
+        L.vacd_ctx_set_stream.argtypes = [vp, vp]
+        L.vacd_sync.argtypes = [vp]
+        L.vacd_bvh_build.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(vp)]
+        L.vacd_bvh_build_d.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(vp)]
+        L.vacd_bvh_destroy.argtypes = [vp]
+        L.vacd_bvh_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i32), vp]
+        L.vacd_bvh_export.argtypes = [vp, vp, vp]
+        L.vacd_visibility.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, ctypes.POINTER(i64)]
+        L.vacd_visibility_d.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, vp]
+        L.vacd_visibility_stats.argtypes = [vp, ctypes.POINTER(VisStats)]
+        L.vacd_score_planes.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp,
+                                        ctypes.POINTER(i32)]
+        L.vacd_score_planes_d.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp, vp]
+        L.classify_and_rate_planes.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int]
+        L.classify_and_rate_planes.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+class Context:
+    """One vacd context (device + stream + pooled buffers)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        self._check(lib().vacd_ctx_create(device, ctypes.byref(self._h)), None)
+        self.device = device
+
+    def _check(self, rc, h="self"):
+        if rc != VACD_OK:
+            handle = self._h if h == "self" else None
+            msg = lib().vacd_last_error(handle).decode(errors="replace")
+            raise VacdError(rc, msg)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().vacd_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        self._check(lib().vacd_ctx_stream(self._h, ctypes.byref(s)))
+        return s.value or 0
+
+    def set_stream(self, stream_ptr):
+        self._check(lib().vacd_ctx_set_stream(self._h, ctypes.c_void_p(stream_ptr or 0)))
+
+    def sync(self):
+        self._check(lib().vacd_sync(self._h))
+
+    # -- BVH -----------------------------------------------------------------
+    def bvh(self, vertices, tris) -> "Bvh":
+        """BVH from numpy (host) or torch (device) arrays."""
+        h = ctypes.c_void_p()
+        if hasattr(vertices, "data_ptr"):
+            rc = lib().vacd_bvh_build_d(self._h, _p(vertices), vertices.shape[0], _p(tris),
+                                        tris.shape[0], ctypes.byref(h))
+        else:
+            V = np.ascontiguousarray(vertices, dtype=np.float32)
+            T = np.ascontiguousarray(tris, dtype=np.uint32)
+            rc = lib().vacd_bvh_build(self._h, _p(V), V.shape[0], _p(T), T.shape[0],
+                                      ctypes.byref(h))
+        self._check(rc)
+        return Bvh(h)
+
+    # -- visibility ----------------------------------------------------------
+    def visibility(self, points, self_bvh: "Bvh", cage: "Bvh" = None, new_mask=None,
+                   cap: int | None = None):
+        """Visibility edges (E, 2) uint32 in k order, from host points."""
+        pts = np.ascontiguousarray(points, dtype=np.float32)
+        n = pts.shape[0]
+        m = None if new_mask is None else np.ascontiguousarray(new_mask, dtype=np.uint8)
+        if cap is None:
+            cap = max(1, min(n * (n - 1) // 2, 1 << 22))
+        while True:
+            out = np.empty((cap, 2), dtype=np.uint32)
+            cnt = ctypes.c_int64()
+            rc = lib().vacd_visibility(self._h, _p(pts), n, _p(m), self_bvh.h,
+                                       cage.h if cage is not None else None, _p(out), cap,
+                                       ctypes.byref(cnt))
+            if rc == VACD_ERR_CAPACITY:
+                cap = int(cnt.value)
+                continue
+            self._check(rc)
+            return out[: cnt.value].copy()
+
+    def visibility_d(self, d_pts, n, self_bvh, cage=None, d_mask=None, d_pairs=None, cap=0,
+                     d_count=None):
+        self._check(lib().vacd_visibility_d(self._h, _p(d_pts), n, _p(d_mask), self_bvh.h,
+                                            cage.h if cage is not None else None,
+                                            _p(d_pairs), cap, _p(d_count)))
+
+    def visibility_stats(self) -> dict:
+        s = VisStats()
+        self._check(lib().vacd_visibility_stats(self._h, ctypes.byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in VisStats._fields_}
+
+    # -- scoring ---------------------------------------------------------------
+    def score_planes(self, planes, points, edges):
+        """(score_q32 int64[P], score_f float32[P], argmax) from host arrays."""
+        pl = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        e = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 2)
+        P = pl.shape[0]
+        q = np.zeros(P, dtype=np.int64)
+        f = np.zeros(P, dtype=np.float32)
+        am = ctypes.c_int32(-1)
+        self._check(lib().vacd_score_planes(self._h, _p(pl), P, _p(pts), pts.shape[0], _p(e),
+                                            e.shape[0], _p(q), _p(f), ctypes.byref(am)))
+        return q, f, int(am.value)
+
+    def score_planes_d(self, d_planes, P, d_pts, n, d_edges, E, d_q32=None, d_f=None,
+                       d_argmax=None):
+        self._check(lib().vacd_score_planes_d(self._h, _p(d_planes), P, _p(d_pts), n,
+                                              _p(d_edges), E, _p(d_q32), _p(d_f),
+                                              _p(d_argmax)))
+
+
+class Bvh:
+    def __init__(self, h):
+        self.h = h
+
+    def info(self):
+        n = ctypes.c_int64()
+        d = ctypes.c_int32()
+        b = np.zeros(6, dtype=np.float32)
+        rc = lib().vacd_bvh_info(self.h, ctypes.byref(n), ctypes.byref(d), _p(b))
+        if rc != VACD_OK:
+            raise VacdError(rc, lib().vacd_last_error(None).decode())
+        return {"n_tris": n.value, "depth": d.value, "bounds": b}
+
+    def export(self):
+        """(nodes (N-1, 16) float32 view, tris (N, 12) float32 view)."""
+        nt = self.info()["n_tris"]
+        nodes = np.zeros((max(nt - 1, 1), 16), dtype=np.float32)
+        tris = np.zeros((nt, 12), dtype=np.float32)
+        rc = lib().vacd_bvh_export(self.h, _p(nodes), _p(tris))
+        if rc != VACD_OK:
+            raise VacdError(rc, lib().vacd_last_error(None).decode())
+        return nodes, tris
+
+    def close(self):
+        if self.h:
+            lib().vacd_bvh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def classify_and_rate_planes(planes, points, edges):
+    """Reference-signature shim: float32 scores, synchronous."""
+    pl = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    e = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 2)
+    out = np.zeros(pl.shape[0], dtype=np.float32)
+    lib().classify_and_rate_planes(_p(pl), _p(pts), _p(e), _p(out), pl.shape[0], pts.shape[0],
+                                   e.shape[0])
+    return out
