@@ -38,8 +38,26 @@ typedef struct vacd_vis_stats {
   int64_t masked;     /* skipped by the new-vertex mask (reference state 2) */
   int64_t blocked;    /* touched the self mesh                            */
   int64_t edges;      /* visibility edges                                 */
-  int64_t exact_calls;/* triangle tests decided by the exact fallback     */
+  int64_t exact_calls;/* pairs redone by the exact-predicate fallback     */
+  int64_t node_visits;/* BVH nodes loaded (VACD_OPT_COUNT_WORK only)     */
+  int64_t tri_tests;  /* triangle tests (VACD_OPT_COUNT_WORK only)       */
 } vacd_vis_stats;
+
+/* Context options (vacd_ctx_set_option). */
+#define VACD_OPT_TIMING 1     /* record HIP events around the main kernels  */
+#define VACD_OPT_COUNT_WORK 2 /* visibility counts node visits / tri tests  */
+
+/* Accumulated device time of the main kernels since the last reset
+ * (VACD_OPT_TIMING): the visibility traversal kernel, the plane-scoring
+ * kernel and a whole LBVH build. */
+typedef struct vacd_timing {
+  double vis_ms;
+  int64_t vis_launches;
+  double score_ms;
+  int64_t score_launches;
+  double bvh_ms;
+  int64_t bvh_launches;
+} vacd_timing;
 
 /* ---- context -------------------------------------------------------------
  * Replaces createContext / optixDeviceContextDestroy
@@ -53,6 +71,9 @@ int vacd_ctx_stream(vacd_ctx *ctx, void **stream);
 /* Enqueue on a caller-owned hipStream_t instead (NULL restores the own one). */
 int vacd_ctx_set_stream(vacd_ctx *ctx, void *stream);
 int vacd_sync(vacd_ctx *ctx);
+int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value);
+/* Waits for the context stream, fills *out, and clears the record if reset. */
+int vacd_ctx_timing(vacd_ctx *ctx, vacd_timing *out, int reset);
 
 /* ---- BVH -------------------------------------------------------------------
  * Replaces the OptiX GAS build of run_optix (lib/src/intersections.cpp:131-180)

@@ -638,9 +638,14 @@ void orc_visibility_sample(const float* pts, int64_t n, const uint8_t* mask, con
 // score = sum of float64 edge lengths over strictly crossing edges.
 void orc_score_fp64(const float* planes, int64_t P, const float* pts, int64_t n,
                     const uint32_t* edges, int64_t E, double* scores) {
-  (void)n;
+  // edges with an endpoint index >= n contribute nothing
+  // (lib/cuda/plane_intersections.cu:34-36)
   std::vector<double> len(E);
   for (int64_t e = 0; e < E; ++e) {
+    if ((int64_t)edges[2 * e] >= n || (int64_t)edges[2 * e + 1] >= n) {
+      len[e] = -1.0;
+      continue;
+    }
     P3 u = ld(pts + 3 * (size_t)edges[2 * e]), v = ld(pts + 3 * (size_t)edges[2 * e + 1]);
     double dx = v.x - u.x, dy = v.y - u.y, dz = v.z - u.z;
     len[e] = std::sqrt(dx * dx + dy * dy + dz * dz);
@@ -650,6 +655,7 @@ void orc_score_fp64(const float* planes, int64_t P, const float* pts, int64_t n,
     double a = planes[4 * p], b = planes[4 * p + 1], c = planes[4 * p + 2], d = planes[4 * p + 3];
     long double acc = 0.0L;
     for (int64_t e = 0; e < E; ++e) {
+      if (len[e] < 0.0) continue;
       P3 u = ld(pts + 3 * (size_t)edges[2 * e]), v = ld(pts + 3 * (size_t)edges[2 * e + 1]);
       double vu = a * u.x + b * u.y + c * u.z + d;
       double vv = a * v.x + b * v.y + c * v.z + d;
@@ -667,9 +673,14 @@ void orc_score_fp64(const float* planes, int64_t P, const float* pts, int64_t n,
 //   q    = round-to-nearest-even(len * 2^32) as int64; score = sum of q.
 void orc_score_q32(const float* planes, int64_t P, const float* pts, int64_t n,
                    const uint32_t* edges, int64_t E, int64_t* scores) {
-  (void)n;
   std::vector<int64_t> lq(E);
+  std::vector<char> ok(E);
   for (int64_t e = 0; e < E; ++e) {
+    ok[e] = (int64_t)edges[2 * e] < n && (int64_t)edges[2 * e + 1] < n;
+    if (!ok[e]) {
+      lq[e] = 0;
+      continue;
+    }
     const float* u = pts + 3 * (size_t)edges[2 * e];
     const float* v = pts + 3 * (size_t)edges[2 * e + 1];
     float dx = v[0] - u[0], dy = v[1] - u[1], dz = v[2] - u[2];
@@ -681,6 +692,7 @@ void orc_score_q32(const float* planes, int64_t P, const float* pts, int64_t n,
     float a = planes[4 * p], b = planes[4 * p + 1], c = planes[4 * p + 2], d = planes[4 * p + 3];
     int64_t acc = 0;
     for (int64_t e = 0; e < E; ++e) {
+      if (!ok[e]) continue;
       const float* u = pts + 3 * (size_t)edges[2 * e];
       const float* v = pts + 3 * (size_t)edges[2 * e + 1];
       float vu = std::fmaf(c, u[2], std::fmaf(b, u[1], a * u[0])) + d;

@@ -103,7 +103,28 @@ struct vacd_ctx {
   std::string last_error;
   vacd::DevBuf buf[SLOT_COUNT_MAX];
   int64_t last_segments = 0;
+  bool timing = false;
+  bool count_work = false;
+  // recorded (start, stop) event pairs per timed kernel kind
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
 };
+
+namespace vacd {
+enum TimedKind { TIMED_VIS = 0, TIMED_SCORE = 1, TIMED_BVH = 2 };
+// Records a start event when timing is on; returns the index to close.
+inline void timed_begin(vacd_ctx *ctx, int kind) {
+  if (!ctx->timing) return;
+  hipEvent_t a, b;
+  VACD_HIP(hipEventCreate(&a));
+  VACD_HIP(hipEventCreate(&b));
+  VACD_HIP(hipEventRecord(a, ctx->stream));
+  ctx->ev[kind].push_back({a, b});
+}
+inline void timed_end(vacd_ctx *ctx, int kind) {
+  if (!ctx->timing || ctx->ev[kind].empty()) return;
+  VACD_HIP(hipEventRecord(ctx->ev[kind].back().second, ctx->stream));
+}
+}  // namespace vacd
 
 struct vacd_bvh {
   int device = 0;

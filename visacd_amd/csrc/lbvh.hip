@@ -391,6 +391,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   Node *nodes = out->nodes.as<Node>();
   Tri *tris = out->tris.as<Tri>();
 
+  timed_begin(ctx, TIMED_BVH);
   k_bounds_init<<<1, 64, 0, st>>>(bounds);
   const unsigned gb = (unsigned)std::min<int64_t>(grid_for(nT, 256), 1024);
   k_bounds<<<gb, 256, 0, st>>>(d_V, d_T, nT, bounds);
@@ -414,6 +415,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   k_refit<<<grid_for(nT, 256), 256, 0, st>>>(tris, nT, nodes, pint, pleaf, flags, bounds, root);
   k_depth<<<grid_for(nT, 256), 256, 0, st>>>(nT, pint, pleaf, depth);
   VACD_HIP(hipGetLastError());
+  timed_end(ctx, TIMED_BVH);
   int32_t h_depth = 0;
   float h_root[6];
   VACD_HIP(hipMemcpyAsync(&h_depth, depth, 4, hipMemcpyDeviceToHost, st));

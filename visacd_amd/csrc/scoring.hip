@@ -159,8 +159,10 @@ void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P, const float
     nchunks = (E + chunk - 1) / chunk;
     VACD_REQUIRE(nchunks < 65536, VACD_ERR_LIMIT, "too many edge chunks");
     dim3 grid((unsigned)ptiles, (unsigned)nchunks);
+    timed_begin(ctx, TIMED_SCORE);
     k_score<<<grid, kBlock, 0, st>>>(reinterpret_cast<const float4 *>(d_planes), P, rec, E, chunk,
                                      acc);
+    timed_end(ctx, TIMED_SCORE);
   }
   k_finalize<<<1, 1024, 0, st>>>(acc, P, d_q32, d_f, d_argmax);
   VACD_HIP(hipGetLastError());

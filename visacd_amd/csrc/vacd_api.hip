@@ -80,6 +80,11 @@ int vacd_ctx_destroy(vacd_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto &b : ctx->buf) b.release();
+  for (auto &v : ctx->ev)
+    for (auto &pr : v) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return VACD_OK;
@@ -106,6 +111,48 @@ int vacd_sync(vacd_ctx *ctx) {
   return guarded(ctx, [&] {
     set_device(ctx);
     VACD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  if (option == VACD_OPT_TIMING)
+    ctx->timing = value != 0;
+  else if (option == VACD_OPT_COUNT_WORK)
+    ctx->count_work = value != 0;
+  else
+    return fail(ctx, Error{VACD_ERR_INVALID, "unknown option"});
+  return VACD_OK;
+}
+
+int vacd_ctx_timing(vacd_ctx *ctx, vacd_timing *out, int reset) {
+  if (!ctx || !out) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+    double ms[3] = {0, 0, 0};
+    int64_t cnt[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+      for (auto &pr : ctx->ev[k]) {
+        float t = 0.0f;
+        VACD_HIP(hipEventElapsedTime(&t, pr.first, pr.second));
+        ms[k] += t;
+        ++cnt[k];
+      }
+      if (reset) {
+        for (auto &pr : ctx->ev[k]) {
+          (void)hipEventDestroy(pr.first);
+          (void)hipEventDestroy(pr.second);
+        }
+        ctx->ev[k].clear();
+      }
+    }
+    out->vis_ms = ms[0];
+    out->vis_launches = cnt[0];
+    out->score_ms = ms[1];
+    out->score_launches = cnt[1];
+    out->bvh_ms = ms[2];
+    out->bvh_launches = cnt[2];
   });
 }
 
@@ -263,6 +310,8 @@ int vacd_visibility_stats(vacd_ctx *ctx, vacd_vis_stats *out) {
     out->blocked = (int64_t)s[1];
     out->exact_calls = (int64_t)s[2];
     out->edges = (int64_t)s[3];
+    out->node_visits = (int64_t)s[5];
+    out->tri_tests = (int64_t)s[6];
   });
 }
 

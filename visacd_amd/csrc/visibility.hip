@@ -179,9 +179,10 @@ __device__ __forceinline__ void load_node(const Node *__restrict__ nodes, int id
 
 // Any contact along the segment: 1 contact, 0 none, 2 none certain but some
 // test undecided.  stk: this lane's LDS stack, stride 64 entries.
-template <int STACK>
+template <int STACK, bool COUNT>
 __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
-                                        const Seg &s, int *stk, int cap) {
+                                        const Seg &s, int *stk, int cap, unsigned &nv,
+                                        unsigned &nt) {
   int node = 0, sp = 0;
   bool unsure = false;
   for (int it = 0;; ++it) {
@@ -189,8 +190,10 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
     float bx[12];
     int c0, c1;
     load_node(nodes, node, bx, c0, c1);
+    if (COUNT) ++nv;
     bool h0 = seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
     bool h1 = seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+    if (COUNT) nt += (h0 && c0 < 0) + (h1 && c1 < 0);
     if (h0 && c0 < 0) {
       const int r = contact_fast(s, tris[~c0]);
       if (r == 1) return 1;
@@ -219,10 +222,10 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
 }
 
 // Crossing parity of the midpoint ray: 0 even, 1 odd, 2 undecided.
-template <int STACK>
+template <int STACK, bool COUNT>
 __device__ __forceinline__ int walk_parity(const Node *__restrict__ nodes,
                                            const Tri *__restrict__ tris, const Seg &s, int *stk,
-                                           int cap) {
+                                           int cap, unsigned &nv, unsigned &nt) {
   int node = 0, sp = 0, cnt = 0;
   bool unsure = false;
   for (int it = 0;; ++it) {
@@ -230,8 +233,10 @@ __device__ __forceinline__ int walk_parity(const Node *__restrict__ nodes,
     float bx[12];
     int c0, c1;
     load_node(nodes, node, bx, c0, c1);
+    if (COUNT) ++nv;
     bool h0 = ray_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
     bool h1 = ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+    if (COUNT) nt += (h0 && c0 < 0) + (h1 && c1 < 0);
     if (h0 && c0 < 0) {
       const int r = cross_fast(s, tris[~c0]);
       cnt += r == 1;
@@ -270,16 +275,17 @@ struct VisArgs {
   const Tri *ctris;
   uint64_t *bits;
   uint64_t *defer;
-  unsigned long long *stats;  // [0] masked, [1] blocked, [2] deferred, [3] edges, [4] malformed
+  unsigned long long *stats;  // [0] masked, [1] blocked, [2] deferred, [3] edges, [4] malformed,
+                              // [5] node visits, [6] triangle tests
   int scap, ccap;             // node-visit caps (2 * n_tris + 64)
 };
 
-template <int STACK, bool CAGE>
+template <int STACK, bool CAGE, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   __shared__ int stk_all[kWaves][STACK][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int *stk = &stk_all[wave][0][lane];
-  unsigned n_masked = 0, n_blocked = 0, n_defer = 0, n_bad = 0;
+  unsigned n_masked = 0, n_blocked = 0, n_defer = 0, n_bad = 0, nv = 0, nt = 0;
   const int64_t total_waves = (int64_t)gridDim.x * kWaves;
   for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < a.nwords; w += total_waves) {
     const int64_t k = w * 64 + lane;
@@ -295,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
         for (int c = 0; c < 3; ++c) s.p[c] = a.pts[3 * i + c], s.q[c] = a.pts[3 * j + c];
         if (!same3(s.p, s.q)) {
           make_seg(s);
-          const int hs = walk_any<STACK>(a.snodes, a.stris, s, stk, a.scap);
+          const int hs = walk_any<STACK, COUNT>(a.snodes, a.stris, s, stk, a.scap, nv, nt);
           if (hs == 1) {
             ++n_blocked;
           } else if (hs == 2) {
@@ -303,12 +309,12 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           } else if (hs == 3) {
             ++n_bad;
           } else if (CAGE) {
-            const int hc = walk_any<STACK>(a.cnodes, a.ctris, s, stk, a.ccap);
+            const int hc = walk_any<STACK, COUNT>(a.cnodes, a.ctris, s, stk, a.ccap, nv, nt);
             edge = hc == 1;
             defer = hc == 2;
             n_bad += hc == 3;
           } else {
-            const int par = walk_parity<STACK>(a.snodes, a.stris, s, stk, a.scap);
+            const int par = walk_parity<STACK, COUNT>(a.snodes, a.stris, s, stk, a.scap, nv, nt);
             edge = par == 0;
             defer = par == 2;
             n_bad += par == 3;
@@ -335,6 +341,17 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
     if (n_defer) atomicAdd(&a.stats[2], (unsigned long long)n_defer);
     if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
+  }
+  if (COUNT) {
+    unsigned long long v = nv, t = nt;
+    for (int off = 32; off > 0; off >>= 1) {
+      v += __shfl_down(v, off);
+      t += __shfl_down(t, off);
+    }
+    if (lane == 0) {
+      atomicAdd(&a.stats[5], v);
+      atomicAdd(&a.stats[6], t);
+    }
   }
 }
 
@@ -602,18 +619,20 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
   VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
   const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
   const unsigned grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
-  if (depth <= 32) {
-    if (cage)
-      k_visibility<32, true><<<grid, kBlock, 0, st>>>(a);
-    else
-      k_visibility<32, false><<<grid, kBlock, 0, st>>>(a);
-  } else {
-    if (cage)
-      k_visibility<64, true><<<grid, kBlock, 0, st>>>(a);
-    else
-      k_visibility<64, false><<<grid, kBlock, 0, st>>>(a);
+  timed_begin(ctx, TIMED_VIS);
+  const int variant = (depth <= 32 ? 0 : 4) + (cage ? 2 : 0) + (ctx->count_work ? 1 : 0);
+  switch (variant) {
+    case 0: k_visibility<32, false, false><<<grid, kBlock, 0, st>>>(a); break;
+    case 1: k_visibility<32, false, true><<<grid, kBlock, 0, st>>>(a); break;
+    case 2: k_visibility<32, true, false><<<grid, kBlock, 0, st>>>(a); break;
+    case 3: k_visibility<32, true, true><<<grid, kBlock, 0, st>>>(a); break;
+    case 4: k_visibility<64, false, false><<<grid, kBlock, 0, st>>>(a); break;
+    case 5: k_visibility<64, false, true><<<grid, kBlock, 0, st>>>(a); break;
+    case 6: k_visibility<64, true, false><<<grid, kBlock, 0, st>>>(a); break;
+    default: k_visibility<64, true, true><<<grid, kBlock, 0, st>>>(a); break;
   }
   VACD_HIP(hipGetLastError());
+  timed_end(ctx, TIMED_VIS);
   const unsigned egrid = (unsigned)std::min<int64_t>((nwords + 63) / 64, 2048);
   if (cage)
     k_vis_exact<true><<<egrid, 64, 0, st>>>(a);

@@ -24,7 +24,7 @@ VACD_ERR_LIMIT = 4
 # Every symbol include/vacd.h declares (checked by tests/test_abi.py).
 EXPORTED = [
     "vacd_ctx_create", "vacd_ctx_destroy", "vacd_last_error", "vacd_ctx_stream",
-    "vacd_ctx_set_stream", "vacd_sync", "vacd_bvh_build", "vacd_bvh_build_d",
+    "vacd_ctx_set_stream", "vacd_sync", "vacd_ctx_set_option", "vacd_ctx_timing", "vacd_bvh_build", "vacd_bvh_build_d",
     "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_export", "vacd_visibility",
     "vacd_visibility_d", "vacd_visibility_stats", "vacd_score_planes",
     "vacd_score_planes_d", "classify_and_rate_planes",
@@ -40,7 +40,18 @@ class VacdError(RuntimeError):
 class VisStats(ctypes.Structure):
     _fields_ = [("segments", ctypes.c_int64), ("masked", ctypes.c_int64),
                 ("blocked", ctypes.c_int64), ("edges", ctypes.c_int64),
-                ("exact_calls", ctypes.c_int64)]
+                ("exact_calls", ctypes.c_int64), ("node_visits", ctypes.c_int64),
+                ("tri_tests", ctypes.c_int64)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("vis_ms", ctypes.c_double), ("vis_launches", ctypes.c_int64),
+                ("score_ms", ctypes.c_double), ("score_launches", ctypes.c_int64),
+                ("bvh_ms", ctypes.c_double), ("bvh_launches", ctypes.c_int64)]
+
+
+VACD_OPT_TIMING = 1
+VACD_OPT_COUNT_WORK = 2
 
 
 _lib = None
@@ -62,6 +73,8 @@ def lib():
         L.vacd_ctx_stream.argtypes = [vp, ctypes.POINTER(vp)]
         L.vacd_ctx_set_stream.argtypes = [vp, vp]
         L.vacd_sync.argtypes = [vp]
+        L.vacd_ctx_set_option.argtypes = [vp, ctypes.c_int, i64]
+        L.vacd_ctx_timing.argtypes = [vp, ctypes.POINTER(Timing), ctypes.c_int]
         L.vacd_bvh_build.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(vp)]
         L.vacd_bvh_build_d.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(vp)]
         L.vacd_bvh_destroy.argtypes = [vp]
@@ -127,6 +140,14 @@ class Context:
 
     def sync(self):
         self._check(lib().vacd_sync(self._h))
+
+    def set_option(self, option: int, value: int):
+        self._check(lib().vacd_ctx_set_option(self._h, option, value))
+
+    def timing(self, reset: bool = True) -> dict:
+        t = Timing()
+        self._check(lib().vacd_ctx_timing(self._h, ctypes.byref(t), 1 if reset else 0))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
 
     # -- BVH -----------------------------------------------------------------
     def bvh(self, vertices, tris) -> "Bvh":
