@@ -52,12 +52,14 @@ def cpu_baseline(W, seconds):
     import oracle
 
     rng = np.random.default_rng(123)
-    m = 2000
-    ks = rng.integers(0, W.n_segments, m)
-    t0 = time.perf_counter()
-    oracle.visibility_sample(W.points, W.vertices, W.triangles, ks)
-    dt = time.perf_counter() - t0
-    m = int(min(W.n_segments, max(m, m * seconds / max(dt, 1e-6))))
+    # rate estimate from two calls (the first also pays the CPU BVH build)
+    rate = 0.0
+    for m in (2000, 100000):
+        ks = rng.integers(0, W.n_segments, m)
+        t0 = time.perf_counter()
+        oracle.visibility_sample(W.points, W.vertices, W.triangles, ks)
+        rate = m / max(time.perf_counter() - t0, 1e-6)
+    m = int(min(W.n_segments, max(2000, rate * seconds)))
     ks = rng.integers(0, W.n_segments, m)
     t0 = time.perf_counter()
     oracle.visibility_sample(W.points, W.vertices, W.triangles, ks)

@@ -153,6 +153,29 @@ def test_visibility_cage_mode(ctx):
     assert all(tuple(e) in kp for e in o.tolist())
 
 
+def test_visibility_open_mesh(ctx):
+    """An open mesh takes the midpoint-ray path (fan parity needs a closed mesh)."""
+    V, T = wl.perturbed_icosphere(3, 0.3, 13)
+    keep = np.ones(T.shape[0], bool)
+    keep[::7] = False  # punch holes
+    T = np.ascontiguousarray(T[keep])
+    _print("open mesh", T.shape)
+    g, o, st, ost = _gpu_vs_oracle(ctx, V, T)
+    np.testing.assert_array_equal(g, o)
+
+
+def test_visibility_points_off_surface(ctx):
+    """Points that are not mesh vertices (inside and outside) use the midpoint ray."""
+    V, T = wl.perturbed_icosphere(3, 0.3, 17)
+    rng = np.random.default_rng(2)
+    pts = rng.uniform(-1.2, 1.2, size=(300, 3)).astype(np.float32)
+    pts = np.concatenate([pts, V[::5]], 0)
+    _print("off-surface points", pts.shape)
+    g, o, st, ost = _gpu_vs_oracle(ctx, V, T, pts=pts)
+    assert len(o) > 0
+    np.testing.assert_array_equal(g, o)
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_visibility_degenerate_polycube(ctx, seed):
     rng = np.random.default_rng(seed)

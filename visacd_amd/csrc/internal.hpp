@@ -93,6 +93,7 @@ enum VacdSlot {
   SLOT_TRIS,
   SLOT_SORT,
   SLOT_MISC,
+  SLOT_FAR,
   SLOT_COUNT_MAX
 };
 
@@ -130,6 +131,10 @@ struct vacd_bvh {
   int device = 0;
   int64_t n_tris = 0;
   int32_t depth = 0;
+  // every undirected edge (by vertex index) has an even number of incident
+  // triangles: crossing parity is then the same from any point of an open
+  // segment that touches no triangle
+  bool closed = false;
   float bounds[6] = {0, 0, 0, 0, 0, 0};
   vacd::DevBuf nodes;  // Node[max(1, n_tris-1)]
   vacd::DevBuf tris;   // Tri[n_tris]

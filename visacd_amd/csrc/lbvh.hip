@@ -346,6 +346,41 @@ __global__ __launch_bounds__(256) void k_depth(int64_t n, const int32_t *__restr
   atomicMax(depth, d);
 }
 
+// Closedness (mod 2): every undirected edge key lo<<32|hi toggles a parity bit
+// in an open-addressing table (linear probing, load factor <= 3/8).
+__global__ __launch_bounds__(256) void k_edge_parity(const uint32_t *__restrict__ T, int64_t nT,
+                                                     unsigned long long *keys, uint32_t *par,
+                                                     uint64_t mask) {
+  const int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (h >= 3 * nT) return;
+  const int64_t t = h / 3;
+  const int e = (int)(h % 3);
+  const uint32_t u = T[3 * t + e], v = T[3 * t + (e + 1) % 3];
+  if (u == v) return;
+  const unsigned long long key =
+      ((unsigned long long)(u < v ? u : v) << 32) | (unsigned long long)(u < v ? v : u);
+  uint64_t s = (key * 0x9E3779B97F4A7C15ull) >> 17;
+  for (uint64_t probe = 0; probe <= mask; ++probe) {
+    s &= mask;
+    const unsigned long long prev = atomicCAS(&keys[s], ~0ull, key);
+    if (prev == ~0ull || prev == key) {
+      atomicXor(&par[s], 1u);
+      return;
+    }
+    ++s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_any_odd(const uint32_t *__restrict__ par, uint64_t n,
+                                                 int32_t *odd) {
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n;
+       s += (uint64_t)gridDim.x * blockDim.x)
+    if (par[s] & 1u) {
+      *odd = 1;
+      return;
+    }
+}
+
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 }  // namespace
@@ -369,7 +404,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   const size_t o_bounds = take(16 * 4), o_depth = take(4), o_k0 = take(4 * nT),
                o_k1 = take(4 * nT), o_v0 = take(4 * nT), o_v1 = take(4 * nT),
                o_hist = take(4 * hist_n), o_pint = take(4 * n_int), o_pleaf = take(4 * nT),
-               o_flags = take(4 * n_int), o_root = take(6 * 4);
+               o_flags = take(4 * n_int), o_root = take(6 * 4), o_odd = take(4);
   vacd::DevBuf &scratch = ctx->buf[SLOT_SORT];
   scratch.ensure(off);
   char *base = scratch.as<char>();
@@ -384,6 +419,13 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   int32_t *pleaf = reinterpret_cast<int32_t *>(base + o_pleaf);
   uint32_t *flags = reinterpret_cast<uint32_t *>(base + o_flags);
   float *root = reinterpret_cast<float *>(base + o_root);
+  int32_t *odd = reinterpret_cast<int32_t *>(base + o_odd);
+  uint64_t hsize = 1;
+  while (hsize < (uint64_t)(8 * nT)) hsize <<= 1;
+  vacd::DevBuf &table = ctx->buf[SLOT_MISC];
+  table.ensure(12 * (size_t)hsize);
+  unsigned long long *hkeys = table.as<unsigned long long>();
+  uint32_t *hpar = reinterpret_cast<uint32_t *>(table.as<char>() + 8 * (size_t)hsize);
 
   out->n_tris = nT;
   out->nodes.ensure(sizeof(Node) * (size_t)n_int);
@@ -416,12 +458,21 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   k_depth<<<grid_for(nT, 256), 256, 0, st>>>(nT, pint, pleaf, depth);
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_BVH);
-  int32_t h_depth = 0;
+  VACD_HIP(hipMemsetAsync(hkeys, 0xff, 8 * (size_t)hsize, st));
+  VACD_HIP(hipMemsetAsync(hpar, 0, 4 * (size_t)hsize, st));
+  VACD_HIP(hipMemsetAsync(odd, 0, 4, st));
+  k_edge_parity<<<grid_for(3 * nT, 256), 256, 0, st>>>(d_T, nT, hkeys, hpar, hsize - 1);
+  k_any_odd<<<(unsigned)std::min<uint64_t>((hsize + 255) / 256, 2048), 256, 0, st>>>(hpar, hsize,
+                                                                                     odd);
+  VACD_HIP(hipGetLastError());
+  int32_t h_depth = 0, h_odd = 1;
   float h_root[6];
+  VACD_HIP(hipMemcpyAsync(&h_odd, odd, 4, hipMemcpyDeviceToHost, st));
   VACD_HIP(hipMemcpyAsync(&h_depth, depth, 4, hipMemcpyDeviceToHost, st));
   VACD_HIP(hipMemcpyAsync(h_root, root, 24, hipMemcpyDeviceToHost, st));
   VACD_HIP(hipStreamSynchronize(st));
   out->depth = h_depth;
+  out->closed = h_odd == 0;
   for (int k = 0; k < 6; ++k) out->bounds[k] = h_root[k];
 }
 

@@ -9,11 +9,18 @@
 //   order).  Each lane walks the self BVH for any contact (short stack of
 //   node ids in LDS, lane-interleaved so the 64 lanes hit 64 distinct
 //   banks), then either walks the cage BVH for any contact (cage mode) or
-//   counts crossings of the +x ray from the segment midpoint (parity mode).
+//   decides inside/outside (parity mode).  A segment that touches nothing
+//   lies wholly inside or outside, so the crossing parity of the +x ray from
+//   its midpoint equals that from p + e(q-p), e -> 0+.  On a closed mesh
+//   with p one of its vertices that parity is (crossings of p's own fan,
+//   tested during the self walk) xor (crossings of the +x ray from p with
+//   all other triangles, k_far_parity, once per point).  Otherwise the lane
+//   walks the midpoint ray itself.
 //   Triangle tests use float64 filters; a test the filter cannot decide is
 //   not guessed: the lane finishes its walk and, unless a certain contact
 //   blocked it, marks the pair deferred.
-// k_vis_exact: every deferred pair is redone with exact predicates.
+// k_vis_exact: every deferred pair is redone with exact predicates (midpoint
+//   parity by definition).
 // k_word_scan / k_scan_blocks / k_emit: popcount prefix over the ballot
 //   words and emission of (i, j) in k order.
 #include "internal.hpp"
@@ -77,17 +84,46 @@ __device__ __forceinline__ bool ray_box(const Seg &s, float lx, float ly, float 
   return s.m[1] >= ly && s.m[1] <= hy && s.m[2] >= lz && s.m[2] <= hz && hx >= s.m[0];
 }
 
+// orient2d in (y, z), float64, with its error bound.
+__device__ __forceinline__ double o2yz(const float *a, const float *b, const float *c,
+                                       double &bound) {
+  return o2_fast(a, b, c, 0, bound);
+}
+
 // Contact of the segment with a triangle: 0 none, 1 contact, 2 undecided.
-__device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t) {
+// For a triangle incident to p (and not to q) it also toggles xpar when the
+// +x ray from p + e(q-p), e -> 0+, crosses the triangle, and counts it in
+// ninc.  Over the full fan of p on a closed mesh, xpar is then the inside
+// parity of the segment's interior (the same as that of its midpoint when
+// the segment touches nothing).
+__device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t, int &xpar, int &ninc) {
   const float a[3] = {t.a.x, t.a.y, t.a.z}, b[3] = {t.b.x, t.b.y, t.b.z},
               c[3] = {t.c.x, t.c.y, t.c.z};
-  const bool ip = same3(s.p, a) || same3(s.p, b) || same3(s.p, c);
+  const bool pa = same3(s.p, a), pb = same3(s.p, b), pc = same3(s.p, c);
+  const bool ip = pa || pb || pc;
   const bool iq = same3(s.q, a) || same3(s.q, b) || same3(s.q, c);
   if (ip && iq) return 1;  // the segment is an edge of the triangle
-  if (ip || iq) {
+  if (iq) {
     double bound;
-    const double det = o3_fast(a, b, c, ip ? s.q : s.p, bound);
+    const double det = o3_fast(a, b, c, s.p, bound);
     return (det > bound || -det > bound) ? 0 : 2;
+  }
+  if (ip) {
+    double bound;
+    const double det = o3_fast(a, b, c, s.q, bound);  // orient3d(p, b', c', q)
+    if (!(det > bound || -det > bound)) return 2;
+    ++ninc;
+    // the triangle rotated to start at p: (p, v1, v2)
+    const float *v1 = pa ? b : (pb ? c : a);
+    const float *v2 = pa ? c : (pb ? a : b);
+    double b0, b1, b2;
+    const double o0 = o2yz(s.p, v1, v2, b0);
+    const double o1 = o2yz(s.p, v1, s.q, b1);
+    const double o2 = o2yz(v2, s.p, s.q, b2);
+    if (!(o0 > b0 || -o0 > b0) || !(o1 > b1 || -o1 > b1) || !(o2 > b2 || -o2 > b2)) return 2;
+    const bool s0 = o0 > 0.0;
+    if ((o1 > 0.0) == s0 && (o2 > 0.0) == s0 && (det > 0.0) == s0) xpar ^= 1;
+    return 0;
   }
   // plane sides of p and q: det[a-x, b-a, c-a] = orient3d(a, b, c, x)
   const double e1x = (double)b[0] - a[0], e1y = (double)b[1] - a[1], e1z = (double)b[2] - a[2];
@@ -182,7 +218,7 @@ __device__ __forceinline__ void load_node(const Node *__restrict__ nodes, int id
 template <int STACK, bool COUNT>
 __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
                                         const Seg &s, int *stk, int cap, unsigned &nv,
-                                        unsigned &nt) {
+                                        unsigned &nt, int &xpar, int &ninc) {
   int node = 0, sp = 0;
   bool unsure = false;
   for (int it = 0;; ++it) {
@@ -195,13 +231,13 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
     bool h1 = seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
     if (COUNT) nt += (h0 && c0 < 0) + (h1 && c1 < 0);
     if (h0 && c0 < 0) {
-      const int r = contact_fast(s, tris[~c0]);
+      const int r = contact_fast(s, tris[~c0], xpar, ninc);
       if (r == 1) return 1;
       unsure |= r == 2;
       h0 = false;
     }
     if (h1 && c1 < 0) {
-      const int r = contact_fast(s, tris[~c1]);
+      const int r = contact_fast(s, tris[~c1], xpar, ninc);
       if (r == 1) return 1;
       unsure |= r == 2;
       h1 = false;
@@ -268,6 +304,8 @@ __device__ __forceinline__ int walk_parity(const Node *__restrict__ nodes,
 struct VisArgs {
   const float *pts;
   const uint8_t *mask;
+  const uint8_t *far;  // per point: crossing parity of its +x ray with non-incident
+                       // triangles (0/1, 2 = undecided); NULL unless the mesh is closed
   int64_t n, S, nwords;
   const Node *snodes;
   const Tri *stris;
@@ -301,7 +339,9 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
         for (int c = 0; c < 3; ++c) s.p[c] = a.pts[3 * i + c], s.q[c] = a.pts[3 * j + c];
         if (!same3(s.p, s.q)) {
           make_seg(s);
-          const int hs = walk_any<STACK, COUNT>(a.snodes, a.stris, s, stk, a.scap, nv, nt);
+          int xpar = 0, ninc = 0;
+          const int hs =
+              walk_any<STACK, COUNT>(a.snodes, a.stris, s, stk, a.scap, nv, nt, xpar, ninc);
           if (hs == 1) {
             ++n_blocked;
           } else if (hs == 2) {
@@ -309,10 +349,19 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           } else if (hs == 3) {
             ++n_bad;
           } else if (CAGE) {
-            const int hc = walk_any<STACK, COUNT>(a.cnodes, a.ctris, s, stk, a.ccap, nv, nt);
+            int cx = 0, cn = 0;
+            const int hc =
+                walk_any<STACK, COUNT>(a.cnodes, a.ctris, s, stk, a.ccap, nv, nt, cx, cn);
             edge = hc == 1;
             defer = hc == 2;
             n_bad += hc == 3;
+          } else if (a.far && ninc > 0) {
+            // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
+            const uint8_t fp = a.far[i];
+            if (fp > 1)
+              defer = true;
+            else
+              edge = ((xpar ^ fp) & 1) == 0;
           } else {
             const int par = walk_parity<STACK, COUNT>(a.snodes, a.stris, s, stk, a.scap, nv, nt);
             edge = par == 0;
@@ -370,7 +419,8 @@ __device__ bool exact_any(const Node *__restrict__ nodes, const Tri *__restrict_
     for (int t = 0; t < 2; ++t) {
       if (h[t] && ch[t] < 0) {
         const Tri tr = tris[~ch[t]];
-        int r = contact_fast(s, tr);
+        int xp = 0, ni = 0;
+        int r = contact_fast(s, tr, xp, ni);
         if (r == 2) {
           const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
                       c3[3] = {tr.c.x, tr.c.y, tr.c.z};
@@ -434,36 +484,87 @@ __device__ int exact_parity(const Node *__restrict__ nodes, const Tri *__restric
   return cnt & 1;
 }
 
+// One wave per ballot word, one lane per deferred pair of that word.
 template <bool CAGE>
 __global__ __launch_bounds__(64) void k_vis_exact(VisArgs a) {
-  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.nwords;
-       w += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t dm = a.defer[w];
+  const int lane = threadIdx.x;
+  for (int64_t w = blockIdx.x; w < a.nwords; w += gridDim.x) {
+    const uint64_t dm = a.defer[w];
     if (!dm) continue;
-    uint64_t eb = a.bits[w];
-    unsigned blocked = 0;
-    while (dm) {
-      const int b = __ffsll((unsigned long long)dm) - 1;
-      dm &= dm - 1;
+    bool edge = false, blocked = false;
+    if ((dm >> lane) & 1ull) {
       int64_t i, j;
-      pair_of(w * 64 + b, a.n, i, j);
+      pair_of(w * 64 + lane, a.n, i, j);
       Seg s;
       for (int c = 0; c < 3; ++c) s.p[c] = a.pts[3 * i + c], s.q[c] = a.pts[3 * j + c];
       make_seg(s);
-      bool edge;
       if (exact_any(a.snodes, a.stris, s, a.scap)) {
-        edge = false;
-        ++blocked;
+        blocked = true;
       } else if (CAGE) {
         edge = exact_any(a.cnodes, a.ctris, s, a.ccap);
       } else {
         edge = exact_parity(a.snodes, a.stris, s, a.scap) == 0;
       }
-      if (edge) eb |= 1ull << b;
     }
-    a.bits[w] = eb;
-    if (blocked) atomicAdd(&a.stats[1], (unsigned long long)blocked);
+    const uint64_t eb = __ballot(edge);
+    const uint64_t bb = __ballot(blocked);
+    if (lane == 0) {
+      a.bits[w] |= eb;
+      if (bb) atomicAdd(&a.stats[1], (unsigned long long)__popcll(bb));
+    }
   }
+}
+
+// Crossing parity of the +x ray from each point with the triangles that do
+// not have the point as a vertex: 0/1, or 2 when a test is undecided.
+__global__ __launch_bounds__(64) void k_far_parity(const float *__restrict__ pts, int64_t n,
+                                                   const Node *__restrict__ nodes,
+                                                   const Tri *__restrict__ tris, int cap,
+                                                   uint8_t *__restrict__ far) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Seg s;
+  for (int c = 0; c < 3; ++c) s.p[c] = s.q[c] = pts[3 * i + c];
+  make_seg(s);
+  int stk[64];
+  int node = 0, sp = 0, cnt = 0;
+  bool unsure = false;
+  for (int it = 0; it <= cap; ++it) {
+    float bx[12];
+    int c0, c1;
+    load_node(nodes, node, bx, c0, c1);
+    int ch[2] = {c0, c1};
+    bool h[2] = {ray_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]),
+                 ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11])};
+    for (int t = 0; t < 2; ++t) {
+      if (h[t] && ch[t] < 0) {
+        const Tri tr = tris[~ch[t]];
+        const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
+                    c3[3] = {tr.c.x, tr.c.y, tr.c.z};
+        if (!(same3(s.p, a3) || same3(s.p, b3) || same3(s.p, c3))) {
+          const int r = cross_fast(s, tr);
+          cnt += r == 1;
+          unsure |= r == 2;
+        }
+        h[t] = false;
+      }
+    }
+    if (h[0] && h[1]) {
+      stk[sp++] = ch[1];
+      node = ch[0];
+    } else if (h[0]) {
+      node = ch[0];
+    } else if (h[1]) {
+      node = ch[1];
+    } else {
+      if (sp == 0) {
+        far[i] = unsure ? 2 : (uint8_t)(cnt & 1);
+        return;
+      }
+      node = stk[--sp];
+    }
+  }
+  far[i] = 2;  // node budget exceeded
 }
 
 // ------------------------------------------------------------ compaction
@@ -614,6 +715,15 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
   a.stats = stats.as<unsigned long long>();
   a.scap = (int)std::min<int64_t>(2 * self->n_tris + 64, INT32_MAX);
   a.ccap = cage ? (int)std::min<int64_t>(2 * cage->n_tris + 64, INT32_MAX) : 0;
+  a.far = nullptr;
+  if (!cage && self->closed) {
+    vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
+    farb.ensure((size_t)n);
+    k_far_parity<<<(unsigned)((n + 63) / 64), 64, 0, st>>>(d_pts, n, a.snodes, a.stris, a.scap,
+                                                          farb.as<uint8_t>());
+    VACD_HIP(hipGetLastError());
+    a.far = farb.as<uint8_t>();
+  }
 
   const int depth = std::max(self->depth, cage ? cage->depth : 0);
   VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
@@ -633,7 +743,7 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
   }
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_VIS);
-  const unsigned egrid = (unsigned)std::min<int64_t>((nwords + 63) / 64, 2048);
+  const unsigned egrid = (unsigned)std::min<int64_t>(nwords, 4096);
   if (cage)
     k_vis_exact<true><<<egrid, 64, 0, st>>>(a);
   else
