@@ -1,0 +1,89 @@
+"""visacd.process end to end on the GPU (drop-in surface of the reference)."""
+import numpy as np
+import pytest
+
+import meshes
+from visacd_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(vis, V, T):
+    m = vis.Mesh()
+    m.vertices = vis.VecArray3d([list(map(float, v)) for v in V])
+    m.triangles = vis.make_vecarray3i(np.asarray(T, np.int32))
+    return m
+
+
+def _volume(m):
+    V = np.array([list(v) for v in m.vertices])
+    T = np.array([list(t) for t in m.triangles])
+    return np.einsum("ij,ij->i", V[T[:, 0]], np.cross(V[T[:, 1]], V[T[:, 2]])).sum() / 6
+
+
+def _run(vis, V, T, conc, n, **cfg):
+    for k, v in cfg.items():
+        setattr(vis.config, k, v)
+    vis.set_seed(42)
+    return vis.process(_mesh(vis, V, T), conc, n)
+
+
+def test_process_parts_partition_the_volume():
+    import visacd as vis
+
+    vis.engine.verbose = True
+    V, T = wl.perturbed_icosphere(3, 0.4, 1)
+    print("process ico3", flush=True)
+    r = _run(vis, V, T, 0.02, 6, return_parts=True)
+    assert 2 <= r.num_parts == len(r.parts)
+    vol = _volume(_mesh(vis, V, T))
+    total = sum(_volume(p) for p in r.parts)
+    assert abs(total - vol) < 1e-6 * abs(vol) + 1e-9
+    assert np.isfinite(r.concavity) and r.concavity >= 0
+    vis.config.return_parts = False
+
+
+def test_process_hulls_deterministic_and_concavity_drops():
+    import visacd as vis
+
+    V, T = meshes.l_shape()
+    print("process L", flush=True)
+    r1 = _run(vis, V, T, 0.01, 4)
+    r2 = _run(vis, V, T, 0.01, 4)
+    assert r1.num_parts == r2.num_parts >= 2
+    assert r1.concavity == r2.concavity
+    for a, b in zip(r1.parts, r2.parts):
+        assert [list(v) for v in a.vertices] == [list(v) for v in b.vertices]
+    r0 = _run(vis, V, T, 0.01, 1)  # no split allowed
+    assert r0.num_parts == 1
+    assert r1.concavity < r0.concavity
+
+
+def test_process_edge_mode_and_sphere_planes():
+    import visacd as vis
+
+    V, T = wl.torus(nu=24, nv=12, seed=3)
+    print("process torus edge mode", flush=True)
+    vis.engine.planes = "sphere"
+    vis.engine.num_planes = 256
+    try:
+        r = _run(vis, V, T, 0.04, 4, score_mode="edge")
+        assert r.num_parts >= 2
+    finally:
+        vis.engine.planes = "bisector"
+        vis.engine.num_planes = 2500
+        vis.config.score_mode = "concavity"
+
+
+def test_process_with_point_limit():
+    import visacd as vis
+
+    V, T = wl.perturbed_icosphere(4, 0.3, 2)
+    print("process ico4 with max_points", flush=True)
+    vis.engine.max_points = 512
+    try:
+        r = _run(vis, V, T, 0.04, 4, return_parts=True)
+        assert r.num_parts >= 2
+    finally:
+        vis.engine.max_points = 0
+        vis.config.return_parts = False

@@ -1,0 +1,22 @@
+"""Drop-in module with the reference's Python API (``import visacd``).
+
+Same names and call signatures as the reference's pybind module
+(lib/bindings.cpp:14-70): ``process(mesh, concavity, num_parts)``,
+``config``, ``Config``, ``set_seed``, ``Mesh``, ``MeshList``, ``VecArray3d``,
+``VecArray3i``, ``make_vecarray3i`` and ``ProcessResult``.  ``engine`` holds
+the options of this engine that the reference does not have.
+"""
+from visacd_amd._visacd import (  # noqa: F401
+    Config,
+    EngineOptions,
+    Mesh,
+    MeshList,
+    ProcessResult,
+    VecArray3d,
+    VecArray3i,
+    config,
+    engine,
+    make_vecarray3i,
+    process,
+    set_seed,
+)

@@ -1,0 +1,87 @@
+// pybind11 module with the reference's Python surface (lib/bindings.cpp:14-70):
+// VecArray3d, VecArray3i, Mesh, MeshList, Config, make_vecarray3i, set_seed,
+// config (a reference to the process-global Config), ProcessResult, process.
+// Extra names of this build: `engine` (EngineOptions) and a few host
+// geometry services used by its tests.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+#include <pybind11/stl_bind.h>
+
+#include "mesh.hpp"
+
+namespace py = pybind11;
+
+PYBIND11_MAKE_OPAQUE(std::vector<std::array<double, 3>>);
+PYBIND11_MAKE_OPAQUE(std::vector<std::array<int, 3>>);
+
+PYBIND11_MODULE(_visacd, m) {
+  py::bind_vector<std::vector<std::array<double, 3>>>(m, "VecArray3d");
+  py::bind_vector<std::vector<std::array<int, 3>>>(m, "VecArray3i");
+
+  py::class_<vhost::Mesh>(m, "Mesh")
+      .def(py::init<>())
+      .def_readwrite("vertices", &vhost::Mesh::vertices)
+      .def_readwrite("triangles", &vhost::Mesh::triangles);
+
+  py::bind_vector<vhost::MeshList>(m, "MeshList");
+
+  py::class_<vhost::Config>(m, "Config")
+      .def(py::init<>())
+      .def_readwrite("return_parts", &vhost::Config::return_parts)
+      .def_readwrite("score_mode", &vhost::Config::score_mode)
+      .def_readwrite("flat_surface_min_area", &vhost::Config::flat_surface_min_area)
+      .def_readwrite("use_flat_surfaces", &vhost::Config::use_flat_surfaces)
+      .def_readwrite("flat_surface_k", &vhost::Config::flat_surface_k)
+      .def_readwrite("use_merging", &vhost::Config::use_merging);
+
+  m.def("make_vecarray3i", [](py::array_t<int, py::array::c_style | py::array::forcecast> a) {
+    if (a.ndim() != 2 || a.shape(1) != 3) throw std::invalid_argument("expected an (X, 3) array");
+    std::vector<std::array<int, 3>> out((size_t)a.shape(0));
+    auto r = a.unchecked<2>();
+    for (py::ssize_t i = 0; i < a.shape(0); ++i) out[i] = {r(i, 0), r(i, 1), r(i, 2)};
+    return out;
+  });
+
+  m.def("set_seed", &vhost::set_seed, py::arg("seed"));
+  m.attr("config") = py::cast(&vhost::config, py::return_value_policy::reference);
+
+  py::class_<vhost::ProcessResult>(m, "ProcessResult")
+      .def_readwrite("parts", &vhost::ProcessResult::parts)
+      .def_readwrite("concavity", &vhost::ProcessResult::concavity)
+      .def_readwrite("num_parts", &vhost::ProcessResult::num_parts);
+
+  m.def("process", &vhost::process, py::arg("mesh"), py::arg("concavity"), py::arg("num_parts"));
+
+  // ---- this build's own knobs and host services
+  py::class_<vhost::EngineOptions>(m, "EngineOptions")
+      .def(py::init<>())
+      .def_readwrite("device", &vhost::EngineOptions::device)
+      .def_readwrite("max_points", &vhost::EngineOptions::max_points)
+      .def_readwrite("num_planes", &vhost::EngineOptions::num_planes)
+      .def_readwrite("planes", &vhost::EngineOptions::planes)
+      .def_readwrite("hausdorff_resolution", &vhost::EngineOptions::hausdorff_resolution)
+      .def_readwrite("sample_seed", &vhost::EngineOptions::sample_seed)
+      .def_readwrite("verbose", &vhost::EngineOptions::verbose);
+  m.attr("engine") = py::cast(&vhost::engine, py::return_value_policy::reference);
+
+  m.def("convex_hull", [](const vhost::Mesh &mesh) { return vhost::convex_hull(mesh.vertices); });
+  m.def("mesh_volume", [](const vhost::Mesh &mesh) { return vhost::mesh_volume(mesh); });
+  m.def("surface_area", [](const vhost::Mesh &mesh) { return vhost::surface_area(mesh); });
+  m.def("separate_disjoint", [](const vhost::Mesh &mesh) { return vhost::separate_disjoint(mesh); });
+  m.def("clip", [](const vhost::Mesh &mesh, std::array<double, 4> p) {
+    std::vector<int> mp, mn;
+    vhost::MeshList parts = vhost::clip(mesh, vhost::Plane(p[0], p[1], p[2], p[3]), mp, mn);
+    std::vector<std::vector<bool>> flags;
+    for (auto &q : parts) flags.push_back(q.is_new);
+    return py::make_tuple(parts, mp, mn, flags);
+  });
+  m.def("concavity", [](const vhost::Mesh &part, const vhost::Mesh &hull, int res, uint32_t seed) {
+    return vhost::concavity(part, hull, res, seed);
+  });
+  m.def("flat_surface_planes", [](const vhost::Mesh &mesh, double min_area) {
+    std::vector<std::array<double, 4>> out;
+    for (auto &p : vhost::flat_surface_planes(mesh, min_area)) out.push_back({p.a, p.b, p.c, p.d});
+    return out;
+  });
+}

@@ -1,0 +1,319 @@
+// Plane clip of a closed triangle mesh with capped cross-sections.
+//
+// Semantics follow the reference clip (lib/src/clip.cpp:190-741): vertices
+// with side +1 / -1 (dead zone 1e-6) go to the positive / negative part in
+// increasing id order with 1-based maps (0 = absent); vertices on the plane
+// and edge/plane intersection points become cap vertices appended after
+// them and flagged is_new (lib/src/clip.cpp:664-673).  The cap polygons are
+// triangulated here by ear clipping with hole bridging (D. Eberly,
+// "Triangulation by Ear Clipping", Geometric Tools, 2002) instead of the
+// reference's constrained Delaunay triangulation.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <unordered_map>
+
+#include "mesh.hpp"
+
+namespace vhost {
+namespace {
+
+struct P2 {
+  double x, y;
+  int id;  // cap vertex index
+};
+
+inline double orient(const P2 &a, const P2 &b, const P2 &c) {
+  return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+}
+
+double signed_area(const std::vector<P2> &poly) {
+  double s = 0;
+  for (size_t i = 0, n = poly.size(); i < n; ++i) {
+    const P2 &a = poly[i], &b = poly[(i + 1) % n];
+    s += a.x * b.y - b.x * a.y;
+  }
+  return 0.5 * s;
+}
+
+bool point_in_poly(const std::vector<P2> &poly, double x, double y) {
+  bool in = false;
+  for (size_t i = 0, n = poly.size(), j = n - 1; i < n; j = i++) {
+    const P2 &a = poly[i], &b = poly[j];
+    if ((a.y > y) != (b.y > y)) {
+      const double xi = a.x + (y - a.y) * (b.x - a.x) / (b.y - a.y);
+      if (x < xi) in = !in;
+    }
+  }
+  return in;
+}
+
+bool in_tri(const P2 &a, const P2 &b, const P2 &c, const P2 &p) {
+  return orient(a, b, p) >= 0 && orient(b, c, p) >= 0 && orient(c, a, p) >= 0;
+}
+
+// Splices a CW hole into a CCW outer polygon through a mutually visible pair.
+void bridge_hole(std::vector<P2> &outer, const std::vector<P2> &hole) {
+  int mi = 0;
+  for (int i = 1; i < (int)hole.size(); ++i)
+    if (hole[i].x > hole[mi].x) mi = i;
+  const P2 M = hole[mi];
+  // nearest outer edge hit by the ray M + t(1, 0)
+  int best_e = -1;
+  double best_x = 1e300;
+  const int n = (int)outer.size();
+  for (int i = 0; i < n; ++i) {
+    const P2 &a = outer[i], &b = outer[(i + 1) % n];
+    if ((a.y > M.y) == (b.y > M.y) && !(a.y == M.y || b.y == M.y)) continue;
+    if (a.y == b.y) continue;
+    const double xi = a.x + (M.y - a.y) * (b.x - a.x) / (b.y - a.y);
+    if (xi >= M.x && xi < best_x) best_x = xi, best_e = i;
+  }
+  int pi;
+  if (best_e < 0) {
+    // fall back to the nearest outer vertex
+    pi = 0;
+    double bd = 1e300;
+    for (int i = 0; i < n; ++i) {
+      const double d = std::hypot(outer[i].x - M.x, outer[i].y - M.y);
+      if (d < bd) bd = d, pi = i;
+    }
+  } else {
+    const P2 &a = outer[best_e], &b = outer[(best_e + 1) % n];
+    pi = a.x > b.x ? best_e : (best_e + 1) % n;
+    const P2 I{best_x, M.y, -1};
+    // a reflex vertex inside (M, I, P) is a better candidate: smallest angle
+    const P2 P = outer[pi];
+    double best_ang = 1e300;
+    for (int i = 0; i < n; ++i) {
+      const P2 &prev = outer[(i + n - 1) % n], &cur = outer[i], &next = outer[(i + 1) % n];
+      if (orient(prev, cur, next) >= 0) continue;  // convex
+      const bool inside = orient(M, I, P) > 0 ? in_tri(M, I, P, cur) : in_tri(M, P, I, cur);
+      if (!inside || (cur.x == P.x && cur.y == P.y)) continue;
+      const double ang = std::atan2(std::fabs(cur.y - M.y), cur.x - M.x);
+      if (ang < best_ang) best_ang = ang, pi = i;
+    }
+  }
+  std::vector<P2> merged;
+  merged.reserve(outer.size() + hole.size() + 2);
+  for (int i = 0; i <= pi; ++i) merged.push_back(outer[i]);
+  for (int k = 0; k <= (int)hole.size(); ++k) merged.push_back(hole[(mi + k) % hole.size()]);
+  for (int i = pi; i < n; ++i) merged.push_back(outer[i]);
+  outer.swap(merged);
+}
+
+// Ear clipping of a simple CCW polygon (bridges allowed); appends triangles
+// of cap ids.  Returns false if it gets stuck.
+bool ear_clip(std::vector<P2> poly, std::vector<std::array<int, 3>> &tris) {
+  int n = (int)poly.size();
+  if (n < 3) return n == 0;
+  std::vector<int> prv(n), nxt(n);
+  for (int i = 0; i < n; ++i) prv[i] = (i + n - 1) % n, nxt[i] = (i + 1) % n;
+  int remaining = n, cur = 0, guard = 0;
+  double scale = 0;
+  for (auto &p : poly) scale = std::max({scale, std::fabs(p.x), std::fabs(p.y)});
+  const double eps = 1e-14 * std::max(1.0, scale * scale);
+  while (remaining > 3) {
+    if (guard++ > 4 * remaining + 16) {
+      // no strict ear: clip the most convex vertex (degenerate input)
+      int best = cur;
+      double bo = -1e300;
+      int v = cur;
+      for (int k = 0; k < remaining; ++k, v = nxt[v]) {
+        const double o = orient(poly[prv[v]], poly[v], poly[nxt[v]]);
+        if (o > bo) bo = o, best = v;
+      }
+      if (bo < -eps) return false;
+      cur = best;
+    } else {
+      const P2 &a = poly[prv[cur]], &b = poly[cur], &c = poly[nxt[cur]];
+      bool ear = orient(a, b, c) > eps;
+      if (ear) {
+        for (int v = nxt[nxt[cur]]; v != prv[cur]; v = nxt[v]) {
+          const P2 &p = poly[v];
+          if (p.id == a.id || p.id == b.id || p.id == c.id) continue;
+          if (in_tri(a, b, c, p)) {
+            ear = false;
+            break;
+          }
+        }
+      }
+      if (!ear) {
+        cur = nxt[cur];
+        continue;
+      }
+    }
+    tris.push_back({poly[prv[cur]].id, poly[cur].id, poly[nxt[cur]].id});
+    const int p = prv[cur], q = nxt[cur];
+    nxt[p] = q;
+    prv[q] = p;
+    --remaining;
+    cur = p;
+    guard = 0;
+  }
+  const int a = prv[cur], c = nxt[cur];
+  tris.push_back({poly[a].id, poly[cur].id, poly[c].id});
+  return true;
+}
+
+// Cap triangles (cap ids) closing loops of directed plane edges.  Edges are
+// given as the part's own boundary direction; the cap runs them backwards.
+bool build_cap(const std::vector<std::pair<int, int>> &edges, const std::vector<Vec3> &cap_pts,
+               const Vec3 &cap_normal, std::vector<std::array<int, 3>> &tris) {
+  if (edges.empty()) return true;
+  std::unordered_multimap<int, int> nxt;  // reversed: y -> x
+  for (auto &e : edges) nxt.emplace(e.second, e.first);
+  // 2D basis with e1 x e2 = cap_normal
+  Vec3 e1 = std::fabs(cap_normal[0]) < 0.9 ? cross(cap_normal, Vec3{1, 0, 0}) : cross(cap_normal, Vec3{0, 1, 0});
+  e1 = e1 * (1.0 / norm(e1));
+  const Vec3 e2 = cross(cap_normal, e1);
+  std::vector<std::vector<P2>> loops;
+  while (!nxt.empty()) {
+    auto it = nxt.begin();
+    const int start = it->first;
+    std::vector<P2> loop;
+    int v = start;
+    for (size_t guard = 0; guard <= edges.size(); ++guard) {
+      auto jt = nxt.find(v);
+      if (jt == nxt.end()) return false;  // open chain
+      const Vec3 &p = cap_pts[v];
+      loop.push_back(P2{dot(p, e1), dot(p, e2), v});
+      const int w = jt->second;
+      nxt.erase(jt);
+      v = w;
+      if (v == start) break;
+    }
+    if (v != start) return false;
+    if (loop.size() >= 3) loops.push_back(std::move(loop));
+  }
+  std::vector<int> outer, holes;
+  for (int i = 0; i < (int)loops.size(); ++i)
+    (signed_area(loops[i]) > 0 ? outer : holes).push_back(i);
+  if (outer.empty()) return false;
+  std::vector<std::vector<int>> holes_of(loops.size());
+  for (int h : holes) {
+    int owner = -1;
+    double owner_area = 1e300;
+    for (int o : outer) {
+      const double a = signed_area(loops[o]);
+      if (a < owner_area && point_in_poly(loops[o], loops[h][0].x, loops[h][0].y))
+        owner = o, owner_area = a;
+    }
+    if (owner < 0) return false;
+    holes_of[owner].push_back(h);
+  }
+  for (int o : outer) {
+    std::vector<P2> poly = loops[o];
+    auto &hs = holes_of[o];
+    std::sort(hs.begin(), hs.end(), [&](int a, int b) {
+      double ma = -1e300, mb = -1e300;
+      for (auto &p : loops[a]) ma = std::max(ma, p.x);
+      for (auto &p : loops[b]) mb = std::max(mb, p.x);
+      return ma > mb;
+    });
+    for (int h : hs) bridge_hole(poly, loops[h]);
+    if (!ear_clip(poly, tris)) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::vector<int> &map_neg) {
+  const int N = (int)m.vertices.size();
+  std::vector<int> side(N);
+  for (int i = 0; i < N; ++i) side[i] = pl.side(m.vertices[i]);
+  map_pos.assign(N, 0);
+  map_neg.assign(N, 0);
+  // cap points: on-plane vertices and edge crossings, keyed
+  std::map<std::pair<int, int>, int> cap_key;  // (a, b) a<=b; (v, v) for a vertex
+  std::vector<Vec3> cap_pts;
+  auto cap_vertex = [&](int v) {
+    auto it = cap_key.find({v, v});
+    if (it != cap_key.end()) return it->second;
+    cap_key[{v, v}] = (int)cap_pts.size();
+    cap_pts.push_back(m.vertices[v]);
+    return (int)cap_pts.size() - 1;
+  };
+  auto cap_cross = [&](int a, int b) {
+    const int lo = std::min(a, b), hi = std::max(a, b);
+    auto it = cap_key.find({lo, hi});
+    if (it != cap_key.end()) return it->second;
+    const Vec3 &p = m.vertices[lo], &q = m.vertices[hi];
+    const double fp = pl.eval(p), fq = pl.eval(q);
+    const double t = fp / (fp - fq);
+    cap_key[{lo, hi}] = (int)cap_pts.size();
+    cap_pts.push_back(p + (q - p) * t);
+    return (int)cap_pts.size() - 1;
+  };
+  // pieces: polygon corners encoded as vertex id >= 0 or ~cap id
+  std::vector<std::vector<int>> pieces[2];
+  std::vector<std::pair<int, int>> plane_edges[2];
+  for (const auto &t : m.triangles) {
+    const int s[3] = {side[t[0]], side[t[1]], side[t[2]]};
+    if (s[0] == 0 && s[1] == 0 && s[2] == 0) continue;  // on the plane: covered by the caps
+    for (int w = 0; w < 2; ++w) {
+      const int want = w == 0 ? 1 : -1;
+      if (s[0] != want && s[1] != want && s[2] != want) continue;
+      std::vector<int> poly;
+      for (int e = 0; e < 3; ++e) {
+        const int a = t[e], b = t[(e + 1) % 3];
+        const int sa = s[e], sb = s[(e + 1) % 3];
+        if (sa == want) poly.push_back(a);
+        else if (sa == 0) poly.push_back(~cap_vertex(a));
+        if (sa != 0 && sb != 0 && sa != sb) poly.push_back(~cap_cross(a, b));
+      }
+      const int k = (int)poly.size();
+      for (int e = 0; e < k; ++e) {
+        const int a = poly[e], b = poly[(e + 1) % k];
+        if (a < 0 && b < 0 && a != b) plane_edges[w].push_back({~a, ~b});
+      }
+      pieces[w].push_back(std::move(poly));
+    }
+  }
+  MeshList out;
+  int npos = 0, nneg = 0;
+  for (int i = 0; i < N; ++i) npos += side[i] > 0, nneg += side[i] < 0;
+  if (npos == 0 || nneg == 0) {
+    out.push_back(m);
+    return out;
+  }
+  const Vec3 n = {pl.a, pl.b, pl.c};
+  for (int w = 0; w < 2; ++w) {
+    Mesh part;
+    std::vector<int> &map = w == 0 ? map_pos : map_neg;
+    const int want = w == 0 ? 1 : -1;
+    for (int i = 0; i < N; ++i)
+      if (side[i] == want) {
+        map[i] = (int)part.vertices.size() + 1;
+        part.vertices.push_back(m.vertices[i]);
+        part.is_new.push_back(false);
+      }
+    std::vector<int> cap_local(cap_pts.size(), -1);
+    auto local = [&](int code) {
+      if (code >= 0) return map[code] - 1;
+      const int c = ~code;
+      if (cap_local[c] < 0) {
+        cap_local[c] = (int)part.vertices.size();
+        part.vertices.push_back(cap_pts[c]);
+        part.is_new.push_back(true);
+      }
+      return cap_local[c];
+    };
+    for (auto &poly : pieces[w])
+      for (size_t k = 1; k + 1 < poly.size(); ++k)
+        part.triangles.push_back({local(poly[0]), local(poly[k]), local(poly[k + 1])});
+    std::vector<std::array<int, 3>> cap;
+    const Vec3 cap_normal = w == 0 ? n * -1.0 : n;
+    if (!build_cap(plane_edges[w], cap_pts, cap_normal * (1.0 / norm(cap_normal)), cap)) {
+      out.clear();
+      out.push_back(m);
+      return out;
+    }
+    for (auto &t : cap) part.triangles.push_back({local(~t[0]), local(~t[1]), local(~t[2])});
+    out.push_back(std::move(part));
+  }
+  return out;
+}
+
+}  // namespace vhost

@@ -1,0 +1,114 @@
+// Host-side mesh types and geometry services of the decomposition driver.
+//
+// Mirrors the reference's data model (lib/include/core.hpp:18-65,
+// lib/include/config.hpp:6-30, lib/include/process.hpp:9-15) so the pybind
+// surface is signature-identical; the algorithms are this build's own.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <random>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace vhost {
+
+using Vec3 = std::array<double, 3>;
+
+inline Vec3 operator+(const Vec3 &a, const Vec3 &b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+inline Vec3 operator-(const Vec3 &a, const Vec3 &b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline Vec3 operator*(const Vec3 &a, double s) { return {a[0] * s, a[1] * s, a[2] * s}; }
+inline double dot(const Vec3 &a, const Vec3 &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline Vec3 cross(const Vec3 &a, const Vec3 &b) {
+  return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+inline double norm(const Vec3 &a) { return std::sqrt(dot(a, a)); }
+
+// Plane a*x + b*y + c*z + d = 0 (lib/include/core.hpp:18-35).
+struct Plane {
+  double a = 0, b = 0, c = 0, d = 0;
+  Plane() = default;
+  Plane(double a_, double b_, double c_, double d_) : a(a_), b(b_), c(c_), d(d_) {}
+  double eval(const Vec3 &p) const { return p[0] * a + p[1] * b + p[2] * c + d; }
+  // +1 / -1 / 0 with dead zone eps (lib/include/core.hpp:178-185)
+  int side(const Vec3 &p, double eps = 1e-6) const {
+    const double r = eval(p);
+    return r > eps ? 1 : (r < -eps ? -1 : 0);
+  }
+};
+
+struct Mesh {
+  std::vector<Vec3> vertices;
+  std::vector<std::array<int, 3>> triangles;
+  // per vertex: created by the last cut (cap vertex); pairs of two old
+  // vertices are not re-tested (lib/src/clip.cpp:664-673)
+  std::vector<bool> is_new;
+  // visibility edges as pairs of vertex ids of this mesh, k order first
+  std::vector<std::pair<unsigned, unsigned>> intersecting_edges;
+  // vertex ids whose pairs are tested; empty = all vertices
+  std::vector<int> points;
+
+  Mesh copy() const {  // geometry only (lib/src/core.cpp:239-246)
+    Mesh m;
+    m.vertices = vertices;
+    m.triangles = triangles;
+    return m;
+  }
+};
+using MeshList = std::vector<Mesh>;
+
+// Process-global knobs, same fields and defaults as the reference.
+struct Config {
+  bool return_parts = false;
+  std::string score_mode = "concavity";  // "edge" or "concavity"
+  double flat_surface_min_area = 0.1;
+  bool use_flat_surfaces = true;
+  double flat_surface_k = 2.0;
+  bool use_merging = false;
+};
+inline Config config;
+
+// Engine knobs that the reference does not have (kept off its Config so the
+// Config surface stays identical).
+struct EngineOptions {
+  int device = 0;
+  int max_points = 0;            // visibility points per part (seeded subsample); 0 = all
+  int num_planes = 2500;         // candidate planes per cut
+  std::string planes = "bisector";  // "bisector" (reference) or "sphere"
+  int hausdorff_resolution = 3000;
+  uint32_t sample_seed = 42;     // seeds subsampling and surface sampling
+  bool verbose = true;
+};
+inline EngineOptions engine;
+
+struct ProcessResult {
+  MeshList parts;
+  double concavity = 0.0;
+  int num_parts = 0;
+};
+
+// global generator of the candidate-plane draws (lib/include/core.hpp:13)
+std::mt19937 &random_engine();
+void set_seed(unsigned int seed);
+
+// geometry services
+std::vector<double> normalize(Mesh &m);                        // returns the original bbox
+void unnormalize(Mesh &m, const std::vector<double> &bbox);
+double mesh_volume(const Mesh &m);
+double surface_area(const Mesh &m);
+double tri_area(const Vec3 &a, const Vec3 &b, const Vec3 &c);
+Mesh convex_hull(const std::vector<Vec3> &pts);
+MeshList separate_disjoint(const Mesh &m, bool drop_flat = true);
+// Splits m by the plane; returns 2 parts (positive side first) or fewer on
+// failure.  map_pos/map_neg receive 1-based vertex maps (0 = not in part).
+MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos,
+              std::vector<int> &map_neg);
+double concavity(const Mesh &part, const Mesh &hull, int resolution, uint32_t seed);
+double hausdorff_boundary(const Mesh &a, const Mesh &b, int resolution, uint32_t seed);
+std::vector<Plane> flat_surface_planes(const Mesh &m, double min_area);
+
+ProcessResult process(Mesh mesh, double concavity, int num_parts);
+
+}  // namespace vhost

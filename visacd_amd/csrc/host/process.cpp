@@ -1,0 +1,352 @@
+// Decomposition driver: the caller of the visibility and scoring engine.
+//
+// Restates the control flow of lib/src/process.cpp:28-347 over the C ABI of
+// include/vacd.h:
+//   * candidate planes: perpendicular bisectors of visibility edges drawn
+//     with the global std::mt19937 and std::uniform_int_distribution<int>,
+//     5-degree / 1e-3 duplicate rejection (process.cpp:28-72), then the
+//     flat-surface planes (process.cpp:90-94);
+//   * scores from vacd_score_planes (exact Q32.32), flat planes times
+//     config.flat_surface_k, first maximum wins (process.cpp:101-134);
+//   * clip, edge inheritance through the 1-based maps (process.cpp:136-159),
+//     connected components, parts under 10 vertices dropped, visibility of
+//     the new parts for pairs touching a cap vertex (process.cpp:164-178);
+//   * part selection by concavity or by total edge length, stop below the
+//     threshold, hulls, final concavity, return hulls or parts
+//     (process.cpp:183-347).
+// Differences, by design: no OpenVDB remesh and no cage (visibility runs in
+// midpoint-parity mode on the part itself, so the input must be a closed
+// mesh); concavity sampling uses a generator seeded per evaluation, so each
+// part's concavity is computed once and cached; engine.max_points bounds the
+// points whose pairs are tested by a seeded subsample.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <iostream>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+
+#include "mesh.hpp"
+#include "vacd.h"
+
+namespace vhost {
+namespace {
+
+void log_line(const std::string &msg) {
+  if (!engine.verbose) return;
+  std::cout << "[visacd] " << msg << "\n";
+  std::cout.flush();
+}
+
+struct Engine {
+  vacd_ctx *ctx = nullptr;
+  explicit Engine(int device) {
+    if (vacd_ctx_create(device, &ctx) != VACD_OK)
+      throw std::runtime_error(std::string("visacd: ") + vacd_last_error(nullptr));
+  }
+  ~Engine() { vacd_ctx_destroy(ctx); }
+  void check(int rc) const {
+    if (rc != VACD_OK) throw std::runtime_error(std::string("visacd: ") + vacd_last_error(ctx));
+  }
+};
+
+std::vector<float> to_f32(const std::vector<Vec3> &v) {
+  std::vector<float> f(3 * v.size());
+  for (size_t i = 0; i < v.size(); ++i)
+    for (int k = 0; k < 3; ++k) f[3 * i + k] = (float)v[i][k];
+  return f;
+}
+
+std::vector<int> seeded_subsample(const std::vector<int> &from, size_t k, uint32_t seed) {
+  if (from.size() <= k) return from;
+  std::vector<int> v = from;
+  std::mt19937 rng(seed);
+  for (size_t i = 0; i < k; ++i) {
+    std::uniform_int_distribution<size_t> d(i, v.size() - 1);
+    std::swap(v[i], v[d(rng)]);
+  }
+  v.resize(k);
+  std::sort(v.begin(), v.end());
+  return v;
+}
+
+// Points of a part whose pairs are tested.  Without a limit: every vertex.
+void choose_points(Mesh &part, bool inherited, uint32_t seed) {
+  const int V = (int)part.vertices.size();
+  const int limit = engine.max_points;
+  std::vector<int> all(V);
+  for (int i = 0; i < V; ++i) all[i] = i;
+  if (limit <= 0 || V <= limit) {
+    part.points = all;
+    return;
+  }
+  if (!inherited) {
+    part.points = seeded_subsample(all, (size_t)limit, seed);
+    return;
+  }
+  std::vector<int> fresh;
+  for (int i = 0; i < V; ++i)
+    if (i < (int)part.is_new.size() && part.is_new[i]) fresh.push_back(i);
+  const int keep = (int)part.points.size();
+  const int budget = std::max(limit - keep, limit / 4);
+  std::vector<int> add = seeded_subsample(fresh, (size_t)std::max(budget, 0), seed);
+  std::vector<int> pts = part.points;
+  pts.insert(pts.end(), add.begin(), add.end());
+  std::sort(pts.begin(), pts.end());
+  pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+  part.points = pts;
+}
+
+// Visibility of a part in parity mode; with_mask restricts to pairs touching
+// a new vertex (lib/ptx/ray_segments.cu:61-75).  Edges are appended in k order
+// as pairs of part vertex ids.
+void add_visibility(const Engine &E, Mesh &part, bool with_mask) {
+  if (part.points.size() < 2) return;
+  std::vector<float> V = to_f32(part.vertices);
+  std::vector<uint32_t> T(3 * part.triangles.size());
+  for (size_t t = 0; t < part.triangles.size(); ++t)
+    for (int k = 0; k < 3; ++k) T[3 * t + k] = (uint32_t)part.triangles[t][k];
+  vacd_bvh *bvh = nullptr;
+  E.check(vacd_bvh_build(E.ctx, V.data(), (int64_t)part.vertices.size(), T.data(),
+                         (int64_t)part.triangles.size(), &bvh));
+  const int64_t n = (int64_t)part.points.size();
+  std::vector<float> P(3 * n);
+  std::vector<uint8_t> mask(n, 0);
+  bool any_new = false;
+  for (int64_t i = 0; i < n; ++i) {
+    const int v = part.points[i];
+    for (int k = 0; k < 3; ++k) P[3 * i + k] = V[3 * (size_t)v + k];
+    mask[i] = (v < (int)part.is_new.size() && part.is_new[v]) ? 1 : 0;
+    any_new |= mask[i] != 0;
+  }
+  const bool use_mask = with_mask;
+  if (use_mask && !any_new) {
+    vacd_bvh_destroy(bvh);
+    return;
+  }
+  int64_t cap = std::min<int64_t>(n * (n - 1) / 2, int64_t(1) << 24);
+  std::vector<uint32_t> pairs;
+  int64_t count = 0;
+  for (;;) {
+    pairs.resize(2 * (size_t)std::max<int64_t>(cap, 1));
+    const int rc = vacd_visibility(E.ctx, P.data(), n, use_mask ? mask.data() : nullptr, bvh,
+                                   nullptr, pairs.data(), cap, &count);
+    if (rc == VACD_ERR_CAPACITY) {
+      cap = count;
+      continue;
+    }
+    if (rc != VACD_OK) {
+      vacd_bvh_destroy(bvh);
+      E.check(rc);
+    }
+    break;
+  }
+  vacd_bvh_destroy(bvh);
+  part.intersecting_edges.reserve(part.intersecting_edges.size() + (size_t)count);
+  for (int64_t e = 0; e < count; ++e)
+    part.intersecting_edges.push_back(
+        {(unsigned)part.points[pairs[2 * e]], (unsigned)part.points[pairs[2 * e + 1]]});
+}
+
+std::vector<Plane> bisector_planes(const Mesh &m, int num_planes) {
+  std::vector<Plane> planes;
+  const auto &E = m.intersecting_edges;
+  if (E.empty()) return planes;
+  std::uniform_int_distribution<> dis(0, (int)E.size() - 1);
+  const double cos_tol = std::cos(5.0 * M_PI / 180.0), dist_tol = 1e-3;
+  for (int i = 0; i < num_planes * 5 && (int)planes.size() < num_planes; ++i) {
+    const auto &e = E[dis(random_engine())];
+    const Vec3 &p1 = m.vertices[e.first], &p2 = m.vertices[e.second];
+    Vec3 n = p2 - p1;
+    const double l = norm(n);
+    if (l < 1e-6) continue;
+    n = n * (1.0 / l);
+    const Vec3 mid = (p1 + p2) * 0.5;
+    const double d = -dot(n, mid);
+    bool similar = false;
+    for (const Plane &q : planes)
+      if (std::fabs(n[0] * q.a + n[1] * q.b + n[2] * q.c) > cos_tol && std::fabs(d - q.d) < dist_tol) {
+        similar = true;
+        break;
+      }
+    if (!similar) planes.emplace_back(n[0], n[1], n[2], d);
+  }
+  return planes;
+}
+
+// Normals uniform on S^2 and offsets uniform over the part's extent along
+// each normal (the benchmark workloads' candidate sets).
+std::vector<Plane> sphere_planes(const Mesh &m, int num_planes) {
+  std::vector<Plane> planes;
+  std::normal_distribution<double> g(0.0, 1.0);
+  std::uniform_real_distribution<double> u(0.0, 1.0);
+  for (int i = 0; i < num_planes; ++i) {
+    Vec3 n{g(random_engine()), g(random_engine()), g(random_engine())};
+    const double l = norm(n);
+    if (l == 0) continue;
+    n = n * (1.0 / l);
+    double lo = 1e300, hi = -1e300;
+    for (int v : m.points) lo = std::min(lo, dot(n, m.vertices[v])), hi = std::max(hi, dot(n, m.vertices[v]));
+    planes.emplace_back(n[0], n[1], n[2], -(lo + u(random_engine()) * (hi - lo)));
+  }
+  return planes;
+}
+
+double edge_score(const Mesh &m) {
+  double s = 0;
+  for (auto &e : m.intersecting_edges) s += norm(m.vertices[e.second] - m.vertices[e.first]);
+  return s;
+}
+
+struct Driver {
+  Engine E;
+  std::vector<Plane> flat;
+  uint32_t next_seed;
+  explicit Driver(int device) : E(device), next_seed(engine.sample_seed) {}
+
+  // returns true when nothing more can be split (no planes or no edges)
+  bool decompose_iteration(Mesh &mesh, MeshList &parts) {
+    std::vector<Plane> planes = engine.planes == "sphere" ? sphere_planes(mesh, engine.num_planes)
+                                                          : bisector_planes(mesh, engine.num_planes);
+    const size_t ss_offset = planes.size();
+    planes.insert(planes.end(), flat.begin(), flat.end());
+    if (planes.empty() || mesh.intersecting_edges.empty()) {
+      parts.push_back(std::move(mesh));
+      return true;
+    }
+    std::vector<float> hp(4 * planes.size());
+    for (size_t i = 0; i < planes.size(); ++i) {
+      hp[4 * i] = (float)planes[i].a, hp[4 * i + 1] = (float)planes[i].b;
+      hp[4 * i + 2] = (float)planes[i].c, hp[4 * i + 3] = (float)planes[i].d;
+    }
+    const std::vector<float> pts = to_f32(mesh.vertices);
+    std::vector<uint32_t> edges(2 * mesh.intersecting_edges.size());
+    for (size_t e = 0; e < mesh.intersecting_edges.size(); ++e)
+      edges[2 * e] = mesh.intersecting_edges[e].first, edges[2 * e + 1] = mesh.intersecting_edges[e].second;
+    std::vector<int64_t> q(planes.size());
+    int32_t am = -1;
+    E.check(vacd_score_planes(E.ctx, hp.data(), (int64_t)planes.size(), pts.data(),
+                              (int64_t)mesh.vertices.size(), edges.data(),
+                              (int64_t)mesh.intersecting_edges.size(), q.data(), nullptr, &am));
+    size_t best = 0;
+    double best_s = -1.0;
+    for (size_t i = 0; i < planes.size(); ++i) {
+      double s = (double)q[i] * 0x1p-32;
+      if (i >= ss_offset) s *= config.flat_surface_k;
+      if (s > best_s) best_s = s, best = i;
+    }
+    std::vector<int> mp, mn;
+    MeshList halves = clip(mesh, planes[best], mp, mn);
+    if (halves.size() < 2) {  // clipping failed
+      parts.push_back(std::move(mesh));
+      return false;
+    }
+    for (auto &e : mesh.intersecting_edges) {
+      if (mp[e.first] && mp[e.second])
+        halves[0].intersecting_edges.push_back({(unsigned)mp[e.first] - 1, (unsigned)mp[e.second] - 1});
+      if (mn[e.first] && mn[e.second])
+        halves[1].intersecting_edges.push_back({(unsigned)mn[e.first] - 1, (unsigned)mn[e.second] - 1});
+    }
+    for (int v : mesh.points) {
+      if (mp[v]) halves[0].points.push_back(mp[v] - 1);
+      if (mn[v]) halves[1].points.push_back(mn[v] - 1);
+    }
+    for (auto &h : halves) {
+      for (Mesh &part : separate_disjoint(h)) {
+        if (part.vertices.size() < 10) continue;
+        choose_points(part, true, next_seed++);
+        add_visibility(E, part, true);
+        parts.push_back(std::move(part));
+      }
+    }
+    return false;
+  }
+};
+
+}  // namespace
+
+ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
+  Driver D(engine.device);
+  const std::vector<double> bbox = normalize(mesh);
+  log_line("Preprocessing mesh (" + std::to_string(mesh.vertices.size()) + " verts)...");
+  mesh.is_new.clear();
+  mesh.intersecting_edges.clear();
+  mesh.points.clear();
+  if (config.use_flat_surfaces) {
+    D.flat = flat_surface_planes(mesh, config.flat_surface_min_area);
+    log_line("Detected " + std::to_string(D.flat.size()) + " flat surface(s).");
+  }
+  MeshList parts = separate_disjoint(mesh);
+  for (auto &part : parts) {
+    choose_points(part, false, D.next_seed++);
+    add_visibility(D.E, part, false);
+  }
+  std::ostringstream hdr;
+  hdr << "Starting decomposition (max parts=" << num_parts << ", concavity threshold="
+      << std::to_string(concavity_threshold) << ", mode=" << config.score_mode << ").";
+  log_line(hdr.str());
+  // cached concavity per part (NaN: not computed yet)
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  std::vector<double> conc(parts.size(), nan);
+  for (int step = 0; step < num_parts - 1; ++step) {
+    int idx = -1;
+    if (config.score_mode == "edge") {
+      double best = -1.0;
+      for (int i = 0; i < (int)parts.size(); ++i) {
+        const double s = edge_score(parts[i]);
+        if (s > best) best = s, idx = i;
+      }
+    } else if (config.score_mode == "concavity") {
+      double best = -1.0;
+      for (int i = 0; i < (int)parts.size(); ++i) {
+        if (std::isnan(conc[i])) {
+          const Mesh hull = convex_hull(parts[i].vertices);
+          conc[i] = concavity(parts[i], hull, engine.hausdorff_resolution, engine.sample_seed);
+        }
+        if (conc[i] > best) best = conc[i], idx = i;
+      }
+      if (best < concavity_threshold) {
+        log_line("Step " + std::to_string(step + 1) + ": concavity " +
+                 std::to_string(best).substr(0, 6) + " < threshold, stopping.");
+        break;
+      }
+    }
+    if (idx < 0) break;
+    log_line("Step " + std::to_string(step + 1) + "/" + std::to_string(num_parts - 1) +
+             ": splitting part " + std::to_string(idx) + " (" + std::to_string(parts.size()) +
+             " parts total).");
+    Mesh part = std::move(parts[idx]);
+    parts.erase(parts.begin() + idx);
+    conc.erase(conc.begin() + idx);
+    const size_t before = parts.size();
+    const bool stop = D.decompose_iteration(part, parts);
+    conc.resize(parts.size(), nan);
+    for (size_t i = before; i < parts.size(); ++i) conc[i] = nan;
+    if (stop) {
+      log_line("No more intersecting edges, stopping early.");
+      break;
+    }
+  }
+  log_line("Computing convex hulls for " + std::to_string(parts.size()) + " parts...");
+  MeshList hulls;
+  for (auto &p : parts) hulls.push_back(convex_hull(p.vertices));
+  double final_h = 0.0;
+  for (size_t i = 0; i < parts.size(); ++i)
+    final_h = std::max(final_h, hausdorff_boundary(parts[i], hulls[i], 10000, engine.sample_seed));
+  if (config.use_merging) log_line("Hull merging is not part of this engine; parts are kept.");
+  for (auto &h : hulls) unnormalize(h, bbox);
+  for (auto &p : parts) unnormalize(p, bbox);
+  ProcessResult r;
+  r.parts = config.return_parts ? parts : hulls;
+  r.concavity = final_h;
+  r.num_parts = (int)r.parts.size();
+  std::ostringstream s;
+  s.setf(std::ios::fixed);
+  s.precision(4);
+  s << "Done. parts=" << r.num_parts << "  concavity=" << final_h;
+  log_line(s.str());
+  return r;
+}
+
+}  // namespace vhost
