@@ -70,6 +70,31 @@ def cpu_baseline(W, seconds):
                       f"(exact predicates, CPU BVH), {dt:.1f} s"}
 
 
+def acd_wall(W):
+    """Full plain decomposition of the workload through visacd.process."""
+    import numpy as np
+
+    import visacd as vis
+
+    m = vis.Mesh()
+    m.vertices = vis.VecArray3d(W.vertices.astype(np.float64).tolist())
+    m.triangles = vis.make_vecarray3i(W.triangles.astype(np.int32))
+    vis.engine.verbose = False
+    vis.engine.max_points = W.n_points
+    vis.engine.planes = "sphere"
+    vis.engine.num_planes = int(W.planes.shape[0])
+    vis.config.return_parts = False
+    vis.set_seed(int(W.meta.get("seed", 42)))
+    conc = float(W.meta.get("concavity", 0.04))
+    nparts = int(W.meta.get("num_parts", 32))
+    t0 = time.perf_counter()
+    r = vis.process(m, conc, nparts)
+    dt = time.perf_counter() - t0
+    return {"wall_s": dt, "parts": int(r.num_parts), "concavity": float(r.concavity),
+            "concavity_threshold": conc, "max_parts": nparts, "method": "plain",
+            "points_per_part": W.n_points, "planes_per_cut": int(W.planes.shape[0])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +104,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--score-steps", type=int, default=10)
+    ap.add_argument("--no-acd", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -203,11 +229,14 @@ def main():
                      "kernel": "k_visibility"},
         "scoring": score,
     }
+    bvh.close()
+    ctx.close()
+    if not args.no_acd:
+        line["acd"] = acd_wall(W)
+        log("acd", line["acd"])
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
     print(json.dumps(line), flush=True)
-    bvh.close()
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -54,26 +54,26 @@ double surface_area(const Mesh &m) {
 }
 
 // ------------------------------------------------------------ convex hull
-// Quickhull (Barber, Dobkin, Huhdanpaa 1996): start from a tetrahedron of
-// extreme points, repeatedly take the farthest outside point of a face,
-// delete the faces it sees, and cone the horizon to it.
+// Quickhull (Barber, Dobkin, Huhdanpaa 1996) with face adjacency: start from
+// a tetrahedron of extreme points; repeatedly take the farthest outside
+// point of a face, collect the faces it sees by a flood over neighbours,
+// cone the horizon to it and hand the orphaned points to the new faces.
 namespace {
 struct HFace {
   int v[3];
+  int nb[3];  // face across edge (v[e], v[e+1])
   Vec3 n;
   double off;
   std::vector<int> outside;
   bool alive = true;
+  int mark = -1;
 };
 
-HFace make_face(const std::vector<Vec3> &P, int a, int b, int c) {
-  HFace f;
-  f.v[0] = a, f.v[1] = b, f.v[2] = c;
-  Vec3 n = cross(P[b] - P[a], P[c] - P[a]);
+void set_plane(HFace &f, const std::vector<Vec3> &P) {
+  Vec3 n = cross(P[f.v[1]] - P[f.v[0]], P[f.v[2]] - P[f.v[0]]);
   const double l = norm(n);
   f.n = l > 0 ? n * (1.0 / l) : Vec3{0, 0, 0};
-  f.off = dot(f.n, P[a]);
-  return f;
+  f.off = dot(f.n, P[f.v[0]]);
 }
 }  // namespace
 
@@ -84,7 +84,6 @@ Mesh convex_hull(const std::vector<Vec3> &P) {
   double scale = 0.0;
   for (const Vec3 &p : P) scale = std::max({scale, std::fabs(p[0]), std::fabs(p[1]), std::fabs(p[2])});
   const double eps = 1e-12 * std::max(1.0, scale);
-  // initial tetrahedron
   int i0 = 0, i1 = 0;
   for (int i = 1; i < n; ++i) {
     if (P[i][0] < P[i0][0]) i0 = i;
@@ -107,12 +106,21 @@ Mesh convex_hull(const std::vector<Vec3> &P) {
     if (d > best) best = d, i3 = i;
   }
   if (i3 < 0 || best <= eps * norm(nrm)) return out;
-  std::vector<HFace> F;
   if (dot(P[i3] - P[i0], nrm) > 0) std::swap(i1, i2);  // i3 below face (i0, i1, i2)
-  F.push_back(make_face(P, i0, i1, i2));
-  F.push_back(make_face(P, i0, i3, i1));
-  F.push_back(make_face(P, i1, i3, i2));
-  F.push_back(make_face(P, i2, i3, i0));
+  std::vector<HFace> F(4);
+  const int tv[4][3] = {{i0, i1, i2}, {i0, i3, i1}, {i1, i3, i2}, {i2, i3, i0}};
+  for (int f = 0; f < 4; ++f) {
+    for (int e = 0; e < 3; ++e) F[f].v[e] = tv[f][e];
+    set_plane(F[f], P);
+  }
+  // tetrahedron adjacency by matching reversed edges
+  for (int f = 0; f < 4; ++f)
+    for (int e = 0; e < 3; ++e) {
+      const int a = F[f].v[e], b = F[f].v[(e + 1) % 3];
+      for (int g = 0; g < 4; ++g)
+        for (int h = 0; h < 3; ++h)
+          if (F[g].v[h] == b && F[g].v[(h + 1) % 3] == a) F[f].nb[e] = g;
+    }
   for (int i = 0; i < n; ++i) {
     if (i == i0 || i == i1 || i == i2 || i == i3) continue;
     for (auto &f : F)
@@ -121,49 +129,84 @@ Mesh convex_hull(const std::vector<Vec3> &P) {
         break;
       }
   }
-  std::vector<int> stack = {0, 1, 2, 3};
-  while (!stack.empty()) {
-    const int fi = stack.back();
-    stack.pop_back();
+  std::vector<int> work = {0, 1, 2, 3};
+  std::vector<int> visible, queue;
+  struct Hz {
+    int a, b, other, other_e;
+  };
+  std::vector<Hz> horizon;
+  std::unordered_map<int, int> start_at, end_at;
+  int iter = 0;
+  while (!work.empty()) {
+    const int fi = work.back();
+    work.pop_back();
     if (!F[fi].alive || F[fi].outside.empty()) continue;
-    // farthest point of the face
+    ++iter;
     int apex = -1;
     double far = -1.0;
     for (int i : F[fi].outside) {
       const double d = dot(F[fi].n, P[i]) - F[fi].off;
       if (d > far) far = d, apex = i;
     }
-    // faces visible from the apex (flood over edge adjacency via a scan)
-    std::vector<int> visible;
-    for (int g = 0; g < (int)F.size(); ++g)
-      if (F[g].alive && dot(F[g].n, P[apex]) - F[g].off > eps) visible.push_back(g);
-    std::unordered_set<uint64_t> dedges;
-    auto key = [](int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; };
-    for (int g : visible)
-      for (int e = 0; e < 3; ++e) dedges.insert(key(F[g].v[e], F[g].v[(e + 1) % 3]));
-    std::vector<int> orphans;
-    std::vector<std::pair<int, int>> horizon;
-    for (int g : visible) {
+    const Vec3 &A = P[apex];
+    visible.clear();
+    horizon.clear();
+    queue.assign(1, fi);
+    F[fi].mark = iter;
+    visible.push_back(fi);
+    for (size_t q = 0; q < queue.size(); ++q) {
+      const int f = queue[q];
       for (int e = 0; e < 3; ++e) {
-        const int a = F[g].v[e], b = F[g].v[(e + 1) % 3];
-        if (!dedges.count(key(b, a))) horizon.push_back({a, b});
+        const int g = F[f].nb[e];
+        if (F[g].mark == iter) continue;
+        if (dot(F[g].n, A) - F[g].off > eps) {
+          F[g].mark = iter;
+          visible.push_back(g);
+          queue.push_back(g);
+        }
       }
-      F[g].alive = false;
-      for (int i : F[g].outside)
-        if (i != apex) orphans.push_back(i);
-      F[g].outside.clear();
     }
-    const int first_new = (int)F.size();
-    for (auto &h : horizon) F.push_back(make_face(P, h.first, h.second, apex));
+    for (int f : visible)
+      for (int e = 0; e < 3; ++e) {
+        const int g = F[f].nb[e];
+        if (F[g].mark == iter) continue;
+        int ge = 0;
+        while (F[g].nb[ge] != f) ++ge;
+        horizon.push_back({F[f].v[e], F[f].v[(e + 1) % 3], g, ge});
+      }
+    std::vector<int> orphans;
+    for (int f : visible) {
+      F[f].alive = false;
+      for (int i : F[f].outside)
+        if (i != apex) orphans.push_back(i);
+      std::vector<int>().swap(F[f].outside);
+    }
+    start_at.clear();
+    end_at.clear();
+    const int first = (int)F.size();
+    for (const Hz &h : horizon) {
+      HFace nf;
+      nf.v[0] = h.a, nf.v[1] = h.b, nf.v[2] = apex;
+      nf.nb[0] = h.other;
+      set_plane(nf, P);
+      const int id = (int)F.size();
+      F[h.other].nb[h.other_e] = id;
+      start_at[h.a] = id;
+      end_at[h.b] = id;
+      F.push_back(std::move(nf));
+    }
+    for (int id = first; id < (int)F.size(); ++id) {
+      F[id].nb[1] = start_at[F[id].v[1]];  // edge (b, apex): face whose horizon edge starts at b
+      F[id].nb[2] = end_at[F[id].v[0]];    // edge (apex, a): face whose horizon edge ends at a
+    }
     for (int i : orphans)
-      for (int g = first_new; g < (int)F.size(); ++g)
+      for (int g = first; g < (int)F.size(); ++g)
         if (dot(F[g].n, P[i]) - F[g].off > eps) {
           F[g].outside.push_back(i);
           break;
         }
-    for (int g = first_new; g < (int)F.size(); ++g) stack.push_back(g);
-    // compact dead faces now and then
-    if (F.size() > 4 * 1024 && F.size() > 8 * (size_t)n) break;
+    for (int g = first; g < (int)F.size(); ++g)
+      if (!F[g].outside.empty()) work.push_back(g);
   }
   std::unordered_map<int, int> remap;
   for (const auto &f : F) {
