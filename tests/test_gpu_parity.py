@@ -153,6 +153,32 @@ def test_visibility_cage_mode(ctx):
     assert all(tuple(e) in kp for e in o.tolist())
 
 
+@pytest.mark.parametrize("name", ["torus", "poly", "open", "single"])
+def test_visibility_binary_and_wide_walks_agree(ctx, name):
+    """Both node layouts give the oracle's edges (binary walk forced off and on)."""
+    from visacd_amd import native
+
+    if name == "torus":
+        V, T = wl.torus(nu=24, nv=12, seed=4)
+    elif name == "poly":
+        V, T = meshes.polycube([(x, y, 0) for x in range(3) for y in range(3) if (x, y) != (1, 1)])
+    elif name == "open":
+        V, T = wl.perturbed_icosphere(2, 0.3, 6)
+        T = np.ascontiguousarray(T[1:])
+    else:
+        V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0.2, 0.2, 1], [0.3, 0.1, -1]], np.float32)
+        T = np.array([[0, 1, 2]], np.uint32)
+    _print("walks", name, T.shape)
+    o, _ = oracle.visibility(V, V, T)
+    for wide in (0, 1):
+        ctx.set_option(native.VACD_OPT_WIDE_BVH, wide)
+        try:
+            g = ctx.visibility(V, ctx.bvh(V, T))
+        finally:
+            ctx.set_option(native.VACD_OPT_WIDE_BVH, 1)
+        np.testing.assert_array_equal(g, o)
+
+
 def test_visibility_open_mesh(ctx):
     """An open mesh takes the midpoint-ray path (fan parity needs a closed mesh)."""
     V, T = wl.perturbed_icosphere(3, 0.3, 13)

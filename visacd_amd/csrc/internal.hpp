@@ -22,6 +22,22 @@ struct alignas(16) Node {
 };
 static_assert(sizeof(Node) == 64, "node is 64 bytes");
 
+// 4-wide node (128 B), structure of arrays over its (up to) four children:
+//   f[0..2] = lo.x, lo.y, lo.z of children 0..3
+//   f[3..5] = hi.x, hi.y, hi.z of children 0..3
+//   f[6]    = child ids (int bits; same encoding as Node), f[7] unused
+// Node4 x holds the grandchildren of binary node x (a leaf child stays a
+// child); empty slots carry an inverted box.
+struct alignas(16) Node4 {
+  float4 f[8];
+};
+static_assert(sizeof(Node4) == 128, "node4 is 128 bytes");
+
+// Box coordinate of an empty child slot: the degenerate box at (1e30, 1e30,
+// 1e30) lies outside every scene, so both the slab test and the +x ray test
+// reject it (an inverted box would not be rejected by the slab test).
+constexpr float kEmpty = 1e30f;
+
 // Leaf-ordered triangle (48 B): a.xyz + original id bits, b.xyz, c.xyz.
 struct alignas(16) Tri {
   float4 a, b, c;
@@ -106,6 +122,7 @@ struct vacd_ctx {
   int64_t last_segments = 0;
   bool timing = false;
   bool count_work = false;
+  bool wide_bvh = true;
   // recorded (start, stop) event pairs per timed kernel kind
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
 };
@@ -136,8 +153,9 @@ struct vacd_bvh {
   // segment that touches no triangle
   bool closed = false;
   float bounds[6] = {0, 0, 0, 0, 0, 0};
-  vacd::DevBuf nodes;  // Node[max(1, n_tris-1)]
-  vacd::DevBuf tris;   // Tri[n_tris]
+  vacd::DevBuf nodes;   // Node[max(1, n_tris-1)]
+  vacd::DevBuf nodes4;  // Node4[max(1, n_tris-1)]
+  vacd::DevBuf tris;    // Tri[n_tris]
 };
 
 namespace vacd {

@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void k_refit(const Tri *__restrict__ tris, int
   if (n == 1) {
     Node *nd = &nodes[0];
     put_box(nd, 0, lo, hi);
-    const float elo[3] = {INFINITY, INFINITY, INFINITY}, ehi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const float elo[3] = {kEmpty, kEmpty, kEmpty}, ehi[3] = {kEmpty, kEmpty, kEmpty};
     put_box(nd, 1, elo, ehi);
     nd->f3 = make_float4(__int_as_float(~0), __int_as_float(~0), 0.0f, 0.0f);
     for (int k = 0; k < 3; ++k) root_box[k] = lo[k], root_box[3 + k] = hi[k];
@@ -381,6 +381,47 @@ __global__ __launch_bounds__(256) void k_any_odd(const uint32_t *__restrict__ pa
     }
 }
 
+// Node4[x] = the (up to four) grandchildren of binary node x.
+__global__ __launch_bounds__(256) void k_collapse4(const Node *__restrict__ nodes, int64_t n_int,
+                                                   Node4 *__restrict__ out) {
+  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (x >= n_int) return;
+  float lo[3][4], hi[3][4];
+  int id[4];
+  int k = 0;
+  auto take = [&](const Node &nd, int slot) {
+    const float *f = reinterpret_cast<const float *>(&nd);
+    for (int a = 0; a < 3; ++a) lo[a][k] = f[6 * slot + a], hi[a][k] = f[6 * slot + 3 + a];
+    id[k] = __float_as_int(slot == 0 ? nd.f3.x : nd.f3.y);
+    ++k;
+  };
+  const Node nd = nodes[x];
+  for (int s = 0; s < 2; ++s) {
+    const int c = __float_as_int(s == 0 ? nd.f3.x : nd.f3.y);
+    const float *f = reinterpret_cast<const float *>(&nd);
+    if (c < 0 || f[6 * s] == kEmpty) {  // leaf or empty slot stays a child
+      take(nd, s);
+    } else {
+      const Node cn = nodes[c];
+      take(cn, 0);
+      take(cn, 1);
+    }
+  }
+  for (; k < 4; ++k) {
+    for (int a = 0; a < 3; ++a) lo[a][k] = kEmpty, hi[a][k] = kEmpty;
+    id[k] = ~0;
+  }
+  Node4 r;
+  for (int a = 0; a < 3; ++a) {
+    r.f[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+    r.f[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+  }
+  r.f[6] = make_float4(__int_as_float(id[0]), __int_as_float(id[1]), __int_as_float(id[2]),
+                       __int_as_float(id[3]));
+  r.f[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+  out[x] = r;
+}
+
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 }  // namespace
@@ -456,6 +497,8 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   k_leaves<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, v0, nT, tris);
   k_refit<<<grid_for(nT, 256), 256, 0, st>>>(tris, nT, nodes, pint, pleaf, flags, bounds, root);
   k_depth<<<grid_for(nT, 256), 256, 0, st>>>(nT, pint, pleaf, depth);
+  out->nodes4.ensure(sizeof(Node4) * (size_t)n_int);
+  k_collapse4<<<grid_for(n_int, 256), 256, 0, st>>>(nodes, n_int, out->nodes4.as<Node4>());
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_BVH);
   VACD_HIP(hipMemsetAsync(hkeys, 0xff, 8 * (size_t)hsize, st));

@@ -120,6 +120,8 @@ int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value) {
     ctx->timing = value != 0;
   else if (option == VACD_OPT_COUNT_WORK)
     ctx->count_work = value != 0;
+  else if (option == VACD_OPT_WIDE_BVH)
+    ctx->wide_bvh = value != 0;
   else
     return fail(ctx, Error{VACD_ERR_INVALID, "unknown option"});
   return VACD_OK;
@@ -184,6 +186,7 @@ static int bvh_build_impl(vacd_ctx *ctx, const float *V, int64_t nV, const uint3
   if (rc != VACD_OK) {
     if (b) {
       b->nodes.release();
+      b->nodes4.release();
       b->tris.release();
       delete b;
     }
@@ -207,6 +210,7 @@ int vacd_bvh_destroy(vacd_bvh *bvh) {
   if (!bvh) return VACD_OK;
   (void)hipSetDevice(bvh->device);
   bvh->nodes.release();
+  bvh->nodes4.release();
   bvh->tris.release();
   delete bvh;
   return VACD_OK;

@@ -301,6 +301,90 @@ __device__ __forceinline__ int walk_parity(const Node *__restrict__ nodes,
   return cnt & 1;
 }
 
+__device__ __forceinline__ float lane_of(const float4 &v, int c) {
+  return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+}
+
+// 4-wide forms of walk_any / walk_parity over Node4 (same results; about half
+// as many dependent node loads).  Hit internal children are taken in slot
+// order: the first is visited next, the others are pushed.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ int walk_any4(const Node4 *__restrict__ nodes,
+                                         const Tri *__restrict__ tris, const Seg &s, int *stk,
+                                         int cap, unsigned &nv, unsigned &nt, int &xpar, int &ninc) {
+  int node = 0, sp = 0;
+  bool unsure = false;
+  for (int it = 0;; ++it) {
+    if (it > cap) return 3;
+    const Node4 nd = nodes[node];
+    if (COUNT) ++nv;
+    int next = -1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (!seg_box(s, lane_of(nd.f[0], c), lane_of(nd.f[1], c), lane_of(nd.f[2], c),
+                   lane_of(nd.f[3], c), lane_of(nd.f[4], c), lane_of(nd.f[5], c)))
+        continue;
+      const int id = __float_as_int(lane_of(nd.f[6], c));
+      if (id < 0) {
+        if (COUNT) ++nt;
+        const int r = contact_fast(s, tris[~id], xpar, ninc);
+        if (r == 1) return 1;
+        unsure |= r == 2;
+      } else if (next < 0) {
+        next = id;
+      } else {
+        stk[(sp++) * 64] = id;
+      }
+    }
+    if (next >= 0) {
+      node = next;
+    } else {
+      if (sp == 0) break;
+      node = stk[(--sp) * 64];
+    }
+  }
+  return unsure ? 2 : 0;
+}
+
+template <int STACK, bool COUNT>
+__device__ __forceinline__ int walk_parity4(const Node4 *__restrict__ nodes,
+                                            const Tri *__restrict__ tris, const Seg &s, int *stk,
+                                            int cap, unsigned &nv, unsigned &nt) {
+  int node = 0, sp = 0, cnt = 0;
+  bool unsure = false;
+  for (int it = 0;; ++it) {
+    if (it > cap) return 3;
+    const Node4 nd = nodes[node];
+    if (COUNT) ++nv;
+    int next = -1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (!ray_box(s, lane_of(nd.f[0], c), lane_of(nd.f[1], c), lane_of(nd.f[2], c),
+                   lane_of(nd.f[3], c), lane_of(nd.f[4], c), lane_of(nd.f[5], c)))
+        continue;
+      const int id = __float_as_int(lane_of(nd.f[6], c));
+      if (id < 0) {
+        if (COUNT) ++nt;
+        const int r = cross_fast(s, tris[~id]);
+        cnt += r == 1;
+        unsure |= r == 2;
+      } else if (next < 0) {
+        next = id;
+      } else {
+        stk[(sp++) * 64] = id;
+      }
+    }
+    if (next >= 0) {
+      node = next;
+    } else {
+      if (sp == 0) break;
+      node = stk[(--sp) * 64];
+    }
+  }
+  if (unsure) return 2;
+  return cnt & 1;
+}
+
 struct VisArgs {
   const float *pts;
   const uint8_t *mask;
@@ -308,8 +392,10 @@ struct VisArgs {
                        // triangles (0/1, 2 = undecided); NULL unless the mesh is closed
   int64_t n, S, nwords;
   const Node *snodes;
+  const Node4 *snodes4;
   const Tri *stris;
   const Node *cnodes;
+  const Node4 *cnodes4;
   const Tri *ctris;
   uint64_t *bits;
   uint64_t *defer;
@@ -318,7 +404,23 @@ struct VisArgs {
   int scap, ccap;             // node-visit caps (2 * n_tris + 64)
 };
 
-template <int STACK, bool CAGE, bool COUNT>
+template <int STACK, bool WIDE, bool COUNT>
+__device__ __forceinline__ int any_hit(const Node *n2, const Node4 *n4, const Tri *tris,
+                                       const Seg &s, int *stk, int cap, unsigned &nv,
+                                       unsigned &nt, int &xpar, int &ninc) {
+  if (WIDE) return walk_any4<STACK, COUNT>(n4, tris, s, stk, cap, nv, nt, xpar, ninc);
+  return walk_any<STACK, COUNT>(n2, tris, s, stk, cap, nv, nt, xpar, ninc);
+}
+
+template <int STACK, bool WIDE, bool COUNT>
+__device__ __forceinline__ int ray_parity(const Node *n2, const Node4 *n4, const Tri *tris,
+                                          const Seg &s, int *stk, int cap, unsigned &nv,
+                                          unsigned &nt) {
+  if (WIDE) return walk_parity4<STACK, COUNT>(n4, tris, s, stk, cap, nv, nt);
+  return walk_parity<STACK, COUNT>(n2, tris, s, stk, cap, nv, nt);
+}
+
+template <int STACK, bool CAGE, bool COUNT, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   __shared__ int stk_all[kWaves][STACK][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -340,8 +442,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
         if (!same3(s.p, s.q)) {
           make_seg(s);
           int xpar = 0, ninc = 0;
-          const int hs =
-              walk_any<STACK, COUNT>(a.snodes, a.stris, s, stk, a.scap, nv, nt, xpar, ninc);
+          const int hs = any_hit<STACK, WIDE, COUNT>(a.snodes, a.snodes4, a.stris, s, stk, a.scap,
+                                                     nv, nt, xpar, ninc);
           if (hs == 1) {
             ++n_blocked;
           } else if (hs == 2) {
@@ -350,8 +452,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             ++n_bad;
           } else if (CAGE) {
             int cx = 0, cn = 0;
-            const int hc =
-                walk_any<STACK, COUNT>(a.cnodes, a.ctris, s, stk, a.ccap, nv, nt, cx, cn);
+            const int hc = any_hit<STACK, WIDE, COUNT>(a.cnodes, a.cnodes4, a.ctris, s, stk,
+                                                       a.ccap, nv, nt, cx, cn);
             edge = hc == 1;
             defer = hc == 2;
             n_bad += hc == 3;
@@ -363,7 +465,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             else
               edge = ((xpar ^ fp) & 1) == 0;
           } else {
-            const int par = walk_parity<STACK, COUNT>(a.snodes, a.stris, s, stk, a.scap, nv, nt);
+            const int par =
+                ray_parity<STACK, WIDE, COUNT>(a.snodes, a.snodes4, a.stris, s, stk, a.scap, nv, nt);
             edge = par == 0;
             defer = par == 2;
             n_bad += par == 3;
@@ -727,19 +830,37 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
 
   const int depth = std::max(self->depth, cage ? cage->depth : 0);
   VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
+  // stack entries a walk can hold: binary, one pending sibling per level;
+  // 4-wide, up to three per visited level (every other binary level)
+  const int need2 = depth - 1, need4 = 3 * (depth / 2);
+  const bool wide = ctx->wide_bvh && need4 <= 64;
+  const int need = wide ? need4 : need2;
+  a.snodes4 = self->nodes4.as<Node4>();
+  a.cnodes4 = cage ? cage->nodes4.as<Node4>() : nullptr;
   const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
   const unsigned grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
   timed_begin(ctx, TIMED_VIS);
-  const int variant = (depth <= 32 ? 0 : 4) + (cage ? 2 : 0) + (ctx->count_work ? 1 : 0);
+  const int variant = (need <= 32 ? 0 : 8) + (cage ? 4 : 0) + (ctx->count_work ? 2 : 0) + (wide ? 1 : 0);
   switch (variant) {
-    case 0: k_visibility<32, false, false><<<grid, kBlock, 0, st>>>(a); break;
-    case 1: k_visibility<32, false, true><<<grid, kBlock, 0, st>>>(a); break;
-    case 2: k_visibility<32, true, false><<<grid, kBlock, 0, st>>>(a); break;
-    case 3: k_visibility<32, true, true><<<grid, kBlock, 0, st>>>(a); break;
-    case 4: k_visibility<64, false, false><<<grid, kBlock, 0, st>>>(a); break;
-    case 5: k_visibility<64, false, true><<<grid, kBlock, 0, st>>>(a); break;
-    case 6: k_visibility<64, true, false><<<grid, kBlock, 0, st>>>(a); break;
-    default: k_visibility<64, true, true><<<grid, kBlock, 0, st>>>(a); break;
+#define VACD_VIS_CASE(V, S, C, N, W) \
+  case V: k_visibility<S, C, N, W><<<grid, kBlock, 0, st>>>(a); break;
+    VACD_VIS_CASE(0, 32, false, false, false)
+    VACD_VIS_CASE(1, 32, false, false, true)
+    VACD_VIS_CASE(2, 32, false, true, false)
+    VACD_VIS_CASE(3, 32, false, true, true)
+    VACD_VIS_CASE(4, 32, true, false, false)
+    VACD_VIS_CASE(5, 32, true, false, true)
+    VACD_VIS_CASE(6, 32, true, true, false)
+    VACD_VIS_CASE(7, 32, true, true, true)
+    VACD_VIS_CASE(8, 64, false, false, false)
+    VACD_VIS_CASE(9, 64, false, false, true)
+    VACD_VIS_CASE(10, 64, false, true, false)
+    VACD_VIS_CASE(11, 64, false, true, true)
+    VACD_VIS_CASE(12, 64, true, false, false)
+    VACD_VIS_CASE(13, 64, true, false, true)
+    VACD_VIS_CASE(14, 64, true, true, false)
+    VACD_VIS_CASE(15, 64, true, true, true)
+#undef VACD_VIS_CASE
   }
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_VIS);
