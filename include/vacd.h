@@ -120,6 +120,27 @@ int vacd_visibility_d(vacd_ctx *ctx, const float *d_pts, int64_t n,
 /* Statistics of the last visibility call on ctx (waits for it). */
 int vacd_visibility_stats(vacd_ctx *ctx, vacd_vis_stats *out);
 
+/* One part of a batched visibility call: device points, optional device
+ * new-vertex mask, its own BVH and optional cage BVH. */
+typedef struct vacd_part {
+  const float *pts;
+  int64_t n;
+  const uint8_t *new_mask;
+  const vacd_bvh *self;
+  const vacd_bvh *cage;
+} vacd_part;
+
+/* Batched frontier: the visibility of every part in one launch sequence
+ * (the reference visits parts one at a time, lib/src/process.cpp:166-178).
+ * parts is a host array of nparts descriptors whose pointers are device
+ * pointers.  Edges are written part after part, each part's in its own k
+ * order with indices local to the part; d_part_off (device int64[nparts+1])
+ * receives each part's first edge index and the total; d_count the total.
+ * Pairs beyond cap are dropped (compare *d_count with cap). */
+int vacd_visibility_batch_d(vacd_ctx *ctx, const vacd_part *parts, int32_t nparts,
+                            uint32_t *d_pairs, int64_t cap, int64_t *d_count,
+                            int64_t *d_part_off);
+
 /* ---- plane scoring ------------------------------------------------------------
  * Replaces classify_and_rate_planes + plane_point_side_and_rate_kernel
  * (lib/cuda/plane_intersections.cu:7-133).  planes: float32 [P*4] (a,b,c,d);

@@ -1,0 +1,51 @@
+"""Batched frontier: many parts in one visibility launch equal per-part results."""
+import numpy as np
+import pytest
+
+import meshes
+import oracle
+from visacd_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_matches_oracle_per_part(ctx):
+    import torch
+
+    dev = torch.device("cuda:0")
+    cases = []
+    for s in range(5):
+        V, T = wl.perturbed_icosphere(2 + s % 2, 0.2 + 0.05 * s, 100 + s)
+        cases.append((V, T, None))
+    Vl, Tl = meshes.l_shape()
+    cases.append((Vl, Tl, None))
+    # a part with a mask, one with a single point (no pairs), an open mesh
+    Vm, Tm = wl.perturbed_icosphere(2, 0.3, 7)
+    rng = np.random.default_rng(0)
+    cases.append((Vm, Tm, (rng.uniform(size=Vm.shape[0]) < 0.2).astype(np.uint8)))
+    cases.append((Vm[:1].copy(), Tm, None))
+    Vo, To = wl.perturbed_icosphere(2, 0.3, 8)
+    cases.append((Vo, np.ascontiguousarray(To[2:]), None))
+    print("batch", len(cases), "parts", flush=True)
+    keep = []
+    parts = []
+    for V, T, m in cases:
+        b = ctx.bvh(V if V.shape[0] > 1 else Vm, T)
+        d_pts = torch.from_numpy(np.ascontiguousarray(V)).to(dev)
+        d_m = torch.from_numpy(m).to(dev) if m is not None else None
+        keep.append((b, d_pts, d_m))
+        parts.append((d_pts, V.shape[0], d_m, b, None))
+    cap = sum(V.shape[0] * (V.shape[0] - 1) // 2 for V, _, _ in cases)
+    d_pairs = torch.zeros((cap, 2), dtype=torch.int32, device=dev)
+    d_count = torch.zeros(1, dtype=torch.int64, device=dev)
+    d_off = torch.zeros(len(parts) + 1, dtype=torch.int64, device=dev)
+    ctx.visibility_batch_d(parts, d_pairs, cap, d_count, d_off)
+    torch.cuda.synchronize()
+    pairs = d_pairs.cpu().numpy().astype(np.uint32)
+    off = d_off.cpu().numpy()
+    assert off[-1] == int(d_count.item())
+    assert (np.diff(off) >= 0).all()
+    for t, (V, T, m) in enumerate(cases):
+        mesh_V = V if V.shape[0] > 1 else Vm
+        o, _ = oracle.visibility(V, mesh_V, T, mask=m)
+        np.testing.assert_array_equal(pairs[off[t]:off[t + 1]], o, err_msg=f"part {t}")

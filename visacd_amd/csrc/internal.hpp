@@ -110,6 +110,7 @@ enum VacdSlot {
   SLOT_SORT,
   SLOT_MISC,
   SLOT_FAR,
+  SLOT_PARTS,
   SLOT_COUNT_MAX
 };
 
@@ -169,6 +170,9 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n,
                        const uint8_t *d_mask, const vacd_bvh *self,
                        const vacd_bvh *cage, uint32_t *d_pairs, int64_t cap,
                        int64_t *d_count);
+
+void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, uint32_t *d_pairs,
+                             int64_t cap, int64_t *d_count, int64_t *d_part_off);
 
 // Scoring (scoring.hip).
 void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P,

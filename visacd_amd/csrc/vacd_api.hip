@@ -264,6 +264,21 @@ int vacd_visibility_d(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8_
   });
 }
 
+int vacd_visibility_batch_d(vacd_ctx *ctx, const vacd_part *parts, int32_t nparts,
+                            uint32_t *d_pairs, int64_t cap, int64_t *d_count,
+                            int64_t *d_part_off) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(parts && nparts > 0 && d_count && d_part_off, VACD_ERR_INVALID,
+                 "NULL argument");
+    VACD_REQUIRE(cap >= 0 && (cap == 0 || d_pairs), VACD_ERR_INVALID, "bad pair buffer");
+    VACD_REQUIRE(cap == 0 || aligned(d_pairs, 8), VACD_ERR_INVALID,
+                 "pair buffer must be 8-byte aligned");
+    set_device(ctx);
+    vacd::launch_visibility_batch(ctx, parts, nparts, d_pairs, cap, d_count, d_part_off);
+  });
+}
+
 int vacd_visibility(vacd_ctx *ctx, const float *pts, int64_t n, const uint8_t *new_mask,
                     const vacd_bvh *self, const vacd_bvh *cage, uint32_t *pairs, int64_t cap,
                     int64_t *count) {

@@ -385,24 +385,47 @@ __device__ __forceinline__ int walk_parity4(const Node4 *__restrict__ nodes,
   return cnt & 1;
 }
 
-struct VisArgs {
+// What a pair needs to know about its part (one mesh and its point set).
+struct Part {
   const float *pts;
   const uint8_t *mask;
   const uint8_t *far;  // per point: crossing parity of its +x ray with non-incident
                        // triangles (0/1, 2 = undecided); NULL unless the mesh is closed
-  int64_t n, S, nwords;
   const Node *snodes;
   const Node4 *snodes4;
   const Tri *stris;
-  const Node *cnodes;
+  const Node *cnodes;  // cage (NULL: parity mode)
   const Node4 *cnodes4;
   const Tri *ctris;
+  int64_t n, S;
+  int64_t word_begin;   // first ballot word of the part in the batch
+  int64_t point_begin;  // first point of the part in the batch
+  int scap, ccap;       // node-visit caps (2 * n_tris + 64)
+};
+
+struct VisArgs {
+  Part one;            // the part of a single-part call
+  const Part *parts;   // batch: parts sorted by word_begin
+  int nparts;
+  int64_t nwords, npoints;
   uint64_t *bits;
   uint64_t *defer;
   unsigned long long *stats;  // [0] masked, [1] blocked, [2] deferred, [3] edges, [4] malformed,
                               // [5] node visits, [6] triangle tests
-  int scap, ccap;             // node-visit caps (2 * n_tris + 64)
 };
+
+// Part owning global ballot word w (binary search over word_begin).
+template <bool BATCH>
+__device__ __forceinline__ Part part_of_word(const VisArgs &a, int64_t w) {
+  if (!BATCH) return a.one;
+  int lo = 0, hi = a.nparts - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.parts[mid].word_begin <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  return a.parts[lo];
+}
 
 template <int STACK, bool WIDE, bool COUNT>
 __device__ __forceinline__ int any_hit(const Node *n2, const Node4 *n4, const Tri *tris,
@@ -420,7 +443,7 @@ __device__ __forceinline__ int ray_parity(const Node *n2, const Node4 *n4, const
   return walk_parity<STACK, COUNT>(n2, tris, s, stk, cap, nv, nt);
 }
 
-template <int STACK, bool CAGE, bool COUNT, bool WIDE>
+template <int STACK, bool CAGE, bool COUNT, bool WIDE, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   __shared__ int stk_all[kWaves][STACK][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -428,21 +451,22 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   unsigned n_masked = 0, n_blocked = 0, n_defer = 0, n_bad = 0, nv = 0, nt = 0;
   const int64_t total_waves = (int64_t)gridDim.x * kWaves;
   for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < a.nwords; w += total_waves) {
-    const int64_t k = w * 64 + lane;
+    const Part P = part_of_word<BATCH>(a, w);  // wave-uniform
+    const int64_t k = (w - P.word_begin) * 64 + lane;
     bool edge = false, defer = false;
-    if (k < a.S) {
+    if (k < P.S) {
       int64_t i, j;
-      pair_of(k, a.n, i, j);
-      if (a.mask && !a.mask[i] && !a.mask[j]) {
+      pair_of(k, P.n, i, j);
+      if (P.mask && !P.mask[i] && !P.mask[j]) {
         ++n_masked;
       } else {
         Seg s;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s.p[c] = a.pts[3 * i + c], s.q[c] = a.pts[3 * j + c];
+        for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
         if (!same3(s.p, s.q)) {
           make_seg(s);
           int xpar = 0, ninc = 0;
-          const int hs = any_hit<STACK, WIDE, COUNT>(a.snodes, a.snodes4, a.stris, s, stk, a.scap,
+          const int hs = any_hit<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap,
                                                      nv, nt, xpar, ninc);
           if (hs == 1) {
             ++n_blocked;
@@ -450,23 +474,23 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             defer = true;
           } else if (hs == 3) {
             ++n_bad;
-          } else if (CAGE) {
+          } else if (CAGE && P.ctris) {
             int cx = 0, cn = 0;
-            const int hc = any_hit<STACK, WIDE, COUNT>(a.cnodes, a.cnodes4, a.ctris, s, stk,
-                                                       a.ccap, nv, nt, cx, cn);
+            const int hc = any_hit<STACK, WIDE, COUNT>(P.cnodes, P.cnodes4, P.ctris, s, stk,
+                                                       P.ccap, nv, nt, cx, cn);
             edge = hc == 1;
             defer = hc == 2;
             n_bad += hc == 3;
-          } else if (a.far && ninc > 0) {
+          } else if (P.far && ninc > 0) {
             // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
-            const uint8_t fp = a.far[i];
+            const uint8_t fp = P.far[i];
             if (fp > 1)
               defer = true;
             else
               edge = ((xpar ^ fp) & 1) == 0;
           } else {
             const int par =
-                ray_parity<STACK, WIDE, COUNT>(a.snodes, a.snodes4, a.stris, s, stk, a.scap, nv, nt);
+                ray_parity<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap, nv, nt);
             edge = par == 0;
             defer = par == 2;
             n_bad += par == 3;
@@ -588,25 +612,26 @@ __device__ int exact_parity(const Node *__restrict__ nodes, const Tri *__restric
 }
 
 // One wave per ballot word, one lane per deferred pair of that word.
-template <bool CAGE>
+template <bool BATCH>
 __global__ __launch_bounds__(64) void k_vis_exact(VisArgs a) {
   const int lane = threadIdx.x;
   for (int64_t w = blockIdx.x; w < a.nwords; w += gridDim.x) {
     const uint64_t dm = a.defer[w];
     if (!dm) continue;
+    const Part P = part_of_word<BATCH>(a, w);
     bool edge = false, blocked = false;
     if ((dm >> lane) & 1ull) {
       int64_t i, j;
-      pair_of(w * 64 + lane, a.n, i, j);
+      pair_of((w - P.word_begin) * 64 + lane, P.n, i, j);
       Seg s;
-      for (int c = 0; c < 3; ++c) s.p[c] = a.pts[3 * i + c], s.q[c] = a.pts[3 * j + c];
+      for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
       make_seg(s);
-      if (exact_any(a.snodes, a.stris, s, a.scap)) {
+      if (exact_any(P.snodes, P.stris, s, P.scap)) {
         blocked = true;
-      } else if (CAGE) {
-        edge = exact_any(a.cnodes, a.ctris, s, a.ccap);
+      } else if (P.ctris) {
+        edge = exact_any(P.cnodes, P.ctris, s, P.ccap);
       } else {
-        edge = exact_parity(a.snodes, a.stris, s, a.scap) == 0;
+        edge = exact_parity(P.snodes, P.stris, s, P.scap) == 0;
       }
     }
     const uint64_t eb = __ballot(edge);
@@ -620,14 +645,30 @@ __global__ __launch_bounds__(64) void k_vis_exact(VisArgs a) {
 
 // Crossing parity of the +x ray from each point with the triangles that do
 // not have the point as a vertex: 0/1, or 2 when a test is undecided.
-__global__ __launch_bounds__(64) void k_far_parity(const float *__restrict__ pts, int64_t n,
-                                                   const Node *__restrict__ nodes,
-                                                   const Tri *__restrict__ tris, int cap,
-                                                   uint8_t *__restrict__ far) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// In a batch, thread g handles global point g of the part whose point range
+// holds it; far is indexed by the global point.
+template <bool BATCH>
+__global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restrict__ far) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= a.npoints) return;
+  Part P = a.one;
+  if (BATCH) {
+    int lo = 0, hi = a.nparts - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.parts[mid].point_begin <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    P = a.parts[lo];
+  }
+  if (!P.far) return;  // open mesh or cage mode: not needed
+  const int64_t i = g - P.point_begin;
+  const Node *__restrict__ nodes = P.snodes;
+  const Tri *__restrict__ tris = P.stris;
+  const int cap = P.scap;
+  far += P.point_begin;
   Seg s;
-  for (int c = 0; c < 3; ++c) s.p[c] = s.q[c] = pts[3 * i + c];
+  for (int c = 0; c < 3; ++c) s.p[c] = s.q[c] = P.pts[3 * i + c];
   make_seg(s);
   int stk[64];
   int node = 0, sp = 0, cnt = 0;
@@ -731,10 +772,28 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(unsigned long long *__rest
   }
 }
 
-__global__ __launch_bounds__(256) void k_emit(const uint64_t *__restrict__ bits, int64_t nwords,
-                                              int64_t n, const unsigned long long *__restrict__ block_off,
-                                              uint32_t *__restrict__ pairs, int64_t cap) {
+template <bool BATCH>
+__device__ __forceinline__ int part_index(const VisArgs &a, int64_t w) {
+  if (!BATCH) return 0;
+  int lo = 0, hi = a.nparts - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.parts[mid].word_begin <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Pairs of every set bit in k order.  In a batch the pairs are local to
+// their part and part_off[p] receives the index of part p's first edge.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_emit(VisArgs a,
+                                              const unsigned long long *__restrict__ block_off,
+                                              uint32_t *__restrict__ pairs, int64_t cap,
+                                              int64_t *__restrict__ part_off) {
   __shared__ unsigned long long sh[256];
+  const uint64_t *__restrict__ bits = a.bits;
+  const int64_t nwords = a.nwords;
   const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
   uint64_t w4[4];
   unsigned long long c = 0;
@@ -744,11 +803,15 @@ __global__ __launch_bounds__(256) void k_emit(const uint64_t *__restrict__ bits,
   }
   unsigned long long total;
   unsigned long long out = block_excl_scan(c, sh, total) + block_off[blockIdx.x];
-  if (c == 0) return;
   for (int t = 0; t < 4; ++t) {
+    if (base + t >= nwords) break;
+    const int pi = part_index<BATCH>(a, base + t);
+    const Part &P = BATCH ? a.parts[pi] : a.one;
+    if (BATCH && P.word_begin == base + t) part_off[pi] = (int64_t)out;
     uint64_t m = w4[t];
     if (!m) continue;
-    int64_t k0 = (base + t) * 64;
+    const int64_t n = P.n;
+    int64_t k0 = (base + t - P.word_begin) * 64;
     int64_t i, j;
     pair_of(k0, n, i, j);
     int64_t kc = k0;
@@ -779,22 +842,68 @@ __global__ __launch_bounds__(256) void k_emit(const uint64_t *__restrict__ bits,
 
 }  // namespace
 
-void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8_t *d_mask,
-                       const vacd_bvh *self, const vacd_bvh *cage, uint32_t *d_pairs, int64_t cap,
-                       int64_t *d_count) {
+namespace {
+
+__global__ void k_fix_part_offsets(VisArgs a, const int64_t *__restrict__ total,
+                                   int64_t *__restrict__ part_off) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  part_off[a.nparts] = *total;
+  for (int p = a.nparts - 1; p >= 0; --p)
+    if (a.parts[p].S == 0) part_off[p] = part_off[p + 1];
+}
+
+template <int S, bool C, bool N, bool W, bool B>
+void launch_main(unsigned grid, hipStream_t st, const VisArgs &a) {
+  k_visibility<S, C, N, W, B><<<grid, kBlock, 0, st>>>(a);
+}
+template <int S, bool C, bool N, bool W>
+void pick_b(bool b, unsigned grid, hipStream_t st, const VisArgs &a) {
+  if (b) launch_main<S, C, N, W, true>(grid, st, a);
+  else launch_main<S, C, N, W, false>(grid, st, a);
+}
+template <int S, bool C, bool N>
+void pick_w(bool w, bool b, unsigned grid, hipStream_t st, const VisArgs &a) {
+  if (w) pick_b<S, C, N, true>(b, grid, st, a);
+  else pick_b<S, C, N, false>(b, grid, st, a);
+}
+template <int S, bool C>
+void pick_n(bool n, bool w, bool b, unsigned grid, hipStream_t st, const VisArgs &a) {
+  if (n) pick_w<S, C, true>(w, b, grid, st, a);
+  else pick_w<S, C, false>(w, b, grid, st, a);
+}
+template <int S>
+void pick_c(bool c, bool n, bool w, bool b, unsigned grid, hipStream_t st, const VisArgs &a) {
+  if (c) pick_n<S, true>(n, w, b, grid, st, a);
+  else pick_n<S, false>(n, w, b, grid, st, a);
+}
+
+Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_bvh *self,
+               const vacd_bvh *cage) {
+  Part P;
+  P.pts = d_pts;
+  P.mask = d_mask;
+  P.far = nullptr;
+  P.snodes = self->nodes.as<Node>();
+  P.snodes4 = self->nodes4.as<Node4>();
+  P.stris = self->tris.as<Tri>();
+  P.cnodes = cage ? cage->nodes.as<Node>() : nullptr;
+  P.cnodes4 = cage ? cage->nodes4.as<Node4>() : nullptr;
+  P.ctris = cage ? cage->tris.as<Tri>() : nullptr;
+  P.n = n;
+  P.S = n >= 2 ? n * (n - 1) / 2 : 0;
+  P.word_begin = 0;
+  P.point_begin = 0;
+  P.scap = (int)std::min<int64_t>(2 * self->n_tris + 64, INT32_MAX);
+  P.ccap = cage ? (int)std::min<int64_t>(2 * cage->n_tris + 64, INT32_MAX) : 0;
+  return P;
+}
+
+// Enqueues far parity, the main kernel, the exact redo and the compaction for
+// a prepared VisArgs (a.nwords > 0).
+void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int depth,
+         uint32_t *d_pairs, int64_t cap, int64_t *d_count, int64_t *d_part_off) {
   hipStream_t st = ctx->stream;
-  VACD_REQUIRE(n >= 0 && n <= (int64_t(1) << 26), VACD_ERR_LIMIT, "visibility needs n <= 2^26");
-  VACD_REQUIRE(self != nullptr, VACD_ERR_INVALID, "self BVH is required");
-  const int64_t S = n >= 2 ? n * (n - 1) / 2 : 0;
-  const int64_t nwords = (S + 63) / 64;
-  ctx->last_segments = S;
-  vacd::DevBuf &stats = ctx->buf[SLOT_STATS];
-  stats.ensure(8 * sizeof(unsigned long long));
-  VACD_HIP(hipMemsetAsync(stats.ptr, 0, 8 * sizeof(unsigned long long), st));
-  if (S == 0) {
-    VACD_HIP(hipMemsetAsync(d_count, 0, sizeof(int64_t), st));
-    return;
-  }
+  const int64_t nwords = a.nwords;
   vacd::DevBuf &bits = ctx->buf[SLOT_BITS];
   vacd::DevBuf &defer = ctx->buf[SLOT_DEFER];
   vacd::DevBuf &scan = ctx->buf[SLOT_SCAN];
@@ -802,70 +911,34 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
   defer.ensure(8 * (size_t)nwords);
   const int64_t nscan = (nwords + kScanTile - 1) / kScanTile;
   scan.ensure(8 * (size_t)nscan);
-
-  VisArgs a;
-  a.pts = d_pts;
-  a.mask = d_mask;
-  a.n = n;
-  a.S = S;
-  a.nwords = nwords;
-  a.snodes = self->nodes.as<Node>();
-  a.stris = self->tris.as<Tri>();
-  a.cnodes = cage ? cage->nodes.as<Node>() : nullptr;
-  a.ctris = cage ? cage->tris.as<Tri>() : nullptr;
   a.bits = bits.as<uint64_t>();
   a.defer = defer.as<uint64_t>();
-  a.stats = stats.as<unsigned long long>();
-  a.scap = (int)std::min<int64_t>(2 * self->n_tris + 64, INT32_MAX);
-  a.ccap = cage ? (int)std::min<int64_t>(2 * cage->n_tris + 64, INT32_MAX) : 0;
-  a.far = nullptr;
-  if (!cage && self->closed) {
+  if (any_far) {
+    const unsigned fg = (unsigned)((a.npoints + 63) / 64);
     vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
-    farb.ensure((size_t)n);
-    k_far_parity<<<(unsigned)((n + 63) / 64), 64, 0, st>>>(d_pts, n, a.snodes, a.stris, a.scap,
-                                                          farb.as<uint8_t>());
+    if (batch)
+      k_far_parity<true><<<fg, 64, 0, st>>>(a, farb.as<uint8_t>());
+    else
+      k_far_parity<false><<<fg, 64, 0, st>>>(a, farb.as<uint8_t>());
     VACD_HIP(hipGetLastError());
-    a.far = farb.as<uint8_t>();
   }
-
-  const int depth = std::max(self->depth, cage ? cage->depth : 0);
   VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
   // stack entries a walk can hold: binary, one pending sibling per level;
   // 4-wide, up to three per visited level (every other binary level)
   const int need2 = depth - 1, need4 = 3 * (depth / 2);
   const bool wide = ctx->wide_bvh && need4 <= 64;
   const int need = wide ? need4 : need2;
-  a.snodes4 = self->nodes4.as<Node4>();
-  a.cnodes4 = cage ? cage->nodes4.as<Node4>() : nullptr;
   const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
   const unsigned grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
   timed_begin(ctx, TIMED_VIS);
-  const int variant = (need <= 32 ? 0 : 8) + (cage ? 4 : 0) + (ctx->count_work ? 2 : 0) + (wide ? 1 : 0);
-  switch (variant) {
-#define VACD_VIS_CASE(V, S, C, N, W) \
-  case V: k_visibility<S, C, N, W><<<grid, kBlock, 0, st>>>(a); break;
-    VACD_VIS_CASE(0, 32, false, false, false)
-    VACD_VIS_CASE(1, 32, false, false, true)
-    VACD_VIS_CASE(2, 32, false, true, false)
-    VACD_VIS_CASE(3, 32, false, true, true)
-    VACD_VIS_CASE(4, 32, true, false, false)
-    VACD_VIS_CASE(5, 32, true, false, true)
-    VACD_VIS_CASE(6, 32, true, true, false)
-    VACD_VIS_CASE(7, 32, true, true, true)
-    VACD_VIS_CASE(8, 64, false, false, false)
-    VACD_VIS_CASE(9, 64, false, false, true)
-    VACD_VIS_CASE(10, 64, false, true, false)
-    VACD_VIS_CASE(11, 64, false, true, true)
-    VACD_VIS_CASE(12, 64, true, false, false)
-    VACD_VIS_CASE(13, 64, true, false, true)
-    VACD_VIS_CASE(14, 64, true, true, false)
-    VACD_VIS_CASE(15, 64, true, true, true)
-#undef VACD_VIS_CASE
-  }
+  if (need <= 32)
+    pick_c<32>(any_cage, ctx->count_work, wide, batch, grid, st, a);
+  else
+    pick_c<64>(any_cage, ctx->count_work, wide, batch, grid, st, a);
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_VIS);
   const unsigned egrid = (unsigned)std::min<int64_t>(nwords, 4096);
-  if (cage)
+  if (batch)
     k_vis_exact<true><<<egrid, 64, 0, st>>>(a);
   else
     k_vis_exact<false><<<egrid, 64, 0, st>>>(a);
@@ -873,8 +946,102 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
   unsigned long long *blk = scan.as<unsigned long long>();
   k_word_scan<<<(unsigned)nscan, 256, 0, st>>>(a.bits, nwords, blk);
   k_scan_blocks<<<1, 1024, 0, st>>>(blk, nscan, d_count, a.stats + 3);
-  k_emit<<<(unsigned)nscan, 256, 0, st>>>(a.bits, nwords, n, blk, d_pairs, cap);
+  if (batch) {
+    k_emit<true><<<(unsigned)nscan, 256, 0, st>>>(a, blk, d_pairs, cap, d_part_off);
+    k_fix_part_offsets<<<1, 64, 0, st>>>(a, d_count, d_part_off);
+  } else {
+    k_emit<false><<<(unsigned)nscan, 256, 0, st>>>(a, blk, d_pairs, cap, nullptr);
+  }
   VACD_HIP(hipGetLastError());
+}
+
+void reset_stats(vacd_ctx *ctx) {
+  vacd::DevBuf &stats = ctx->buf[SLOT_STATS];
+  stats.ensure(8 * sizeof(unsigned long long));
+  VACD_HIP(hipMemsetAsync(stats.ptr, 0, 8 * sizeof(unsigned long long), ctx->stream));
+}
+
+}  // namespace
+
+void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8_t *d_mask,
+                       const vacd_bvh *self, const vacd_bvh *cage, uint32_t *d_pairs, int64_t cap,
+                       int64_t *d_count) {
+  VACD_REQUIRE(n >= 0 && n <= (int64_t(1) << 26), VACD_ERR_LIMIT, "visibility needs n <= 2^26");
+  VACD_REQUIRE(self != nullptr, VACD_ERR_INVALID, "self BVH is required");
+  reset_stats(ctx);
+  VisArgs a;
+  a.one = make_part(d_pts, n, d_mask, self, cage);
+  a.parts = nullptr;
+  a.nparts = 1;
+  a.nwords = (a.one.S + 63) / 64;
+  a.npoints = n;
+  a.stats = ctx->buf[SLOT_STATS].as<unsigned long long>();
+  ctx->last_segments = a.one.S;
+  if (a.nwords == 0) {
+    VACD_HIP(hipMemsetAsync(d_count, 0, sizeof(int64_t), ctx->stream));
+    return;
+  }
+  const bool any_far = !cage && self->closed;
+  if (any_far) {
+    vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
+    farb.ensure((size_t)n);
+    a.one.far = farb.as<uint8_t>();
+  }
+  const int depth = std::max(self->depth, cage ? cage->depth : 0);
+  run(ctx, a, false, cage != nullptr, any_far, depth, d_pairs, cap, d_count, nullptr);
+}
+
+void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, uint32_t *d_pairs,
+                             int64_t cap, int64_t *d_count, int64_t *d_part_off) {
+  VACD_REQUIRE(nparts >= 1, VACD_ERR_INVALID, "empty batch");
+  reset_stats(ctx);
+  std::vector<Part> hp((size_t)nparts);
+  int64_t words = 0, points = 0, segs = 0;
+  int depth = 0;
+  bool any_cage = false, any_far = false;
+  for (int p = 0; p < nparts; ++p) {
+    const vacd_part &q = parts[p];
+    VACD_REQUIRE(q.self != nullptr, VACD_ERR_INVALID, "part without a self BVH");
+    VACD_REQUIRE(q.n >= 0 && q.n <= (int64_t(1) << 26), VACD_ERR_LIMIT,
+                 "visibility needs n <= 2^26");
+    VACD_REQUIRE(q.n < 2 || q.pts, VACD_ERR_INVALID, "NULL points");
+    VACD_REQUIRE(q.self->device == ctx->device && (!q.cage || q.cage->device == ctx->device),
+                 VACD_ERR_INVALID, "BVH built on another device");
+    Part P = make_part(q.pts, q.n, q.new_mask, q.self, q.cage);
+    P.word_begin = words;
+    P.point_begin = points;
+    words += (P.S + 63) / 64;
+    points += q.n;
+    segs += P.S;
+    depth = std::max({depth, q.self->depth, q.cage ? q.cage->depth : 0});
+    any_cage |= q.cage != nullptr;
+    any_far |= !q.cage && q.self->closed;
+    hp[p] = P;
+  }
+  ctx->last_segments = segs;
+  if (any_far) {
+    vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
+    farb.ensure((size_t)std::max<int64_t>(points, 1));
+    for (int p = 0; p < nparts; ++p)
+      if (!hp[p].ctris && parts[p].self->closed) hp[p].far = farb.as<uint8_t>() + hp[p].point_begin;
+  }
+  vacd::DevBuf &pb = ctx->buf[SLOT_PARTS];
+  pb.ensure(sizeof(Part) * (size_t)nparts);
+  VACD_HIP(hipMemcpyAsync(pb.ptr, hp.data(), sizeof(Part) * (size_t)nparts, hipMemcpyHostToDevice,
+                          ctx->stream));
+  VisArgs a;
+  a.one = hp[0];
+  a.parts = pb.as<Part>();
+  a.nparts = nparts;
+  a.nwords = words;
+  a.npoints = points;
+  a.stats = ctx->buf[SLOT_STATS].as<unsigned long long>();
+  if (words == 0) {
+    VACD_HIP(hipMemsetAsync(d_count, 0, sizeof(int64_t), ctx->stream));
+    VACD_HIP(hipMemsetAsync(d_part_off, 0, sizeof(int64_t) * (size_t)(nparts + 1), ctx->stream));
+    return;
+  }
+  run(ctx, a, true, any_cage, any_far, depth, d_pairs, cap, d_count, d_part_off);
 }
 
 }  // namespace vacd

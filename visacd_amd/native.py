@@ -26,7 +26,7 @@ EXPORTED = [
     "vacd_ctx_create", "vacd_ctx_destroy", "vacd_last_error", "vacd_ctx_stream",
     "vacd_ctx_set_stream", "vacd_sync", "vacd_ctx_set_option", "vacd_ctx_timing", "vacd_bvh_build", "vacd_bvh_build_d",
     "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_export", "vacd_visibility",
-    "vacd_visibility_d", "vacd_visibility_stats", "vacd_score_planes",
+    "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_batch_d", "vacd_score_planes",
     "vacd_score_planes_d", "classify_and_rate_planes",
 ]
 
@@ -42,6 +42,11 @@ class VisStats(ctypes.Structure):
                 ("blocked", ctypes.c_int64), ("edges", ctypes.c_int64),
                 ("exact_calls", ctypes.c_int64), ("node_visits", ctypes.c_int64),
                 ("tri_tests", ctypes.c_int64)]
+
+
+class Part(ctypes.Structure):
+    _fields_ = [("pts", ctypes.c_void_p), ("n", ctypes.c_int64), ("new_mask", ctypes.c_void_p),
+                ("self", ctypes.c_void_p), ("cage", ctypes.c_void_p)]
 
 
 class Timing(ctypes.Structure):
@@ -84,6 +89,8 @@ def lib():
         L.vacd_visibility.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, ctypes.POINTER(i64)]
         L.vacd_visibility_d.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, vp]
         L.vacd_visibility_stats.argtypes = [vp, ctypes.POINTER(VisStats)]
+        L.vacd_visibility_batch_d.argtypes = [vp, ctypes.POINTER(Part), ctypes.c_int32, vp, i64,
+                                              vp, vp]
         L.vacd_score_planes.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp,
                                         ctypes.POINTER(i32)]
         L.vacd_score_planes_d.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp, vp]
@@ -191,6 +198,14 @@ class Context:
         self._check(lib().vacd_visibility_d(self._h, _p(d_pts), n, _p(d_mask), self_bvh.h,
                                             cage.h if cage is not None else None,
                                             _p(d_pairs), cap, _p(d_count)))
+
+    def visibility_batch_d(self, parts, d_pairs, cap, d_count, d_part_off):
+        """parts: list of (d_pts, n, d_mask or None, Bvh, cage Bvh or None)."""
+        arr = (Part * len(parts))()
+        for t, (p, n, m, b, c) in enumerate(parts):
+            arr[t] = Part(_p(p), n, _p(m), b.h, c.h if c is not None else None)
+        self._check(lib().vacd_visibility_batch_d(self._h, arr, len(parts), _p(d_pairs), cap,
+                                                  _p(d_count), _p(d_part_off)))
 
     def visibility_stats(self) -> dict:
         s = VisStats()
