@@ -264,6 +264,18 @@ def test_scoring_ties_and_bounds(ctx):
     np.testing.assert_array_equal(q, oracle.score_q32(planes, pts, edges))
 
 
+def test_scoring_runs_of_first_vertex(ctx):
+    """Runs of edges sharing the first vertex, broken by out-of-range edges."""
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(-1, 1, size=(50, 3)).astype(np.float32)
+    e = [[0, 99], [0, 1], [0, 2], [3, 4], [3, 4], [99, 5], [99, 6], [7, 8], [7, 99], [7, 9]]
+    e += [[i, j] for i in range(10, 20) for j in range(i + 1, 50, 3)]
+    edges = np.array(e, np.uint32)
+    planes = wl.sphere_planes(pts, 300, 3)
+    q, f, am = ctx.score_planes(planes, pts, edges)
+    np.testing.assert_array_equal(q, oracle.score_q32(planes, pts, edges))
+
+
 def test_scoring_c2_sized(ctx):
     W = wl.icosphere_c2()
     b = ctx.bvh(W.vertices, W.triangles)

@@ -31,8 +31,11 @@ constexpr float kEps = 1e-6f;
 
 struct alignas(16) EdgeRec {
   float4 a;  // x0 y0 z0 x1
-  float4 b;  // y1 z1 lenq(lo) lenq(hi)
+  float4 b;  // y1 z1 lenq(lo) lenq(hi) | kSameFirst
 };
+// Set in lenq(hi) when the edge's first endpoint is the previous edge's (edges
+// in k order come in runs of one first vertex); lengths stay below 2^36.
+constexpr unsigned kSameFirst = 0x40000000u;
 
 __global__ __launch_bounds__(256) void k_edge_prep(const float *__restrict__ pts, int64_t n,
                                                    const uint32_t *__restrict__ edges, int64_t E,
@@ -53,9 +56,13 @@ __global__ __launch_bounds__(256) void k_edge_prep(const float *__restrict__ pts
     const float dx = x1 - x0, dy = y1 - y0, dz = z1 - z0;
     const float len = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
     const long long q = __double2ll_rn((double)len * 4294967296.0);
+    // the previous record must hold real coordinates for its first endpoint
+    const uint2 prev = e > 0 ? reinterpret_cast<const uint2 *>(edges)[e - 1] : make_uint2(~0u, ~0u);
+    const bool same_first =
+        e > 0 && prev.x == ij.x && (int64_t)prev.x < n && (int64_t)prev.y < n;
     r.a = make_float4(x0, y0, z0, x1);
     r.b = make_float4(y1, z1, __int_as_float((int)(q & 0xffffffffll)),
-                      __int_as_float((int)(q >> 32)));
+                      __uint_as_float((unsigned)(q >> 32) | (same_first ? kSameFirst : 0u)));
   }
   rec[e] = r;
 }
@@ -75,6 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_score(const float4 *__restrict__ pla
   }
   const int64_t e0 = (int64_t)blockIdx.y * chunk;
   const int64_t e1 = e0 + chunk < E ? e0 + chunk : E;
+  float vu[kPPT];
+  bool have_u = false;
   for (int64_t base = e0; base < e1; base += kEdgeTile) {
     const int cnt = (int)(e1 - base < kEdgeTile ? e1 - base : kEdgeTile);
     __syncthreads();
@@ -82,13 +91,20 @@ __global__ __launch_bounds__(kBlock) void k_score(const float4 *__restrict__ pla
     __syncthreads();
     for (int t = 0; t < cnt; ++t) {
       const float4 ra = tile[t].a, rb = tile[t].b;
+      const unsigned hi = __float_as_uint(rb.w);
       const unsigned long long lq =
-          ((unsigned long long)(unsigned)__float_as_int(rb.w) << 32) | (unsigned)__float_as_int(rb.z);
+          ((unsigned long long)(hi & ~kSameFirst) << 32) | (unsigned)__float_as_int(rb.z);
+      // wave-uniform: every lane reads the same edge
+      if (!have_u || !(hi & kSameFirst)) {
+#pragma unroll
+        for (int r = 0; r < kPPT; ++r)
+          vu[r] = fmaf(pc[r], ra.z, fmaf(pb[r], ra.y, pa[r] * ra.x)) + pd[r];
+        have_u = true;
+      }
 #pragma unroll
       for (int r = 0; r < kPPT; ++r) {
-        const float vu = fmaf(pc[r], ra.z, fmaf(pb[r], ra.y, pa[r] * ra.x)) + pd[r];
         const float vv = fmaf(pc[r], rb.y, fmaf(pb[r], rb.x, pa[r] * ra.w)) + pd[r];
-        const bool cross = fminf(vu, vv) < -kEps && fmaxf(vu, vv) > kEps;
+        const bool cross = fminf(vu[r], vv) < -kEps && fmaxf(vu[r], vv) > kEps;
         sum[r] += cross ? lq : 0ull;
       }
     }
