@@ -161,6 +161,19 @@ int vacd_score_planes_d(vacd_ctx *ctx, const float *d_planes, int64_t P,
                         int64_t E, int64_t *d_score_q32, float *d_score_f,
                         int32_t *d_argmax);
 
+/* ---- surface distance ---------------------------------------------------------
+ * Largest distance from m points to a triangle mesh (its BVH), the one-sided
+ * term of the sampled Hausdorff distance in the concavity metric
+ * (lib/src/cost.cpp:69-105, lib/include/hausdorff.hpp:214-298; the reference
+ * approximates each point's distance from its 10 nearest samples, this
+ * measures it against every triangle).  vacd_max_distance copies points
+ * from the host and waits; the _d form writes a device float.
+ * Limit: the BVH depth must allow a 64-entry walk stack (3 * depth/2 <= 64). */
+int vacd_max_distance(vacd_ctx *ctx, const float *pts, int64_t m,
+                      const vacd_bvh *mesh, float *out);
+int vacd_max_distance_d(vacd_ctx *ctx, const float *d_pts, int64_t m,
+                        const vacd_bvh *mesh, float *d_out);
+
 /* Reference-signature shim: same arguments and behaviour as
  * classify_and_rate_planes (lib/cuda/plane_intersections.cu:71-78,
  * declared at lib/src/process.cpp:74-79): host pointers, synchronous,

@@ -165,6 +165,10 @@ double one_side(const Mesh &ma, const Samples &sa, const KdTree &ta, const Sampl
 
 }  // namespace
 
+std::vector<Vec3> surface_samples(const Mesh &m, int resolution, std::mt19937 &rng) {
+  return sample_surface(m, resolution, rng).p;
+}
+
 double hausdorff_boundary(const Mesh &a, const Mesh &b, int resolution, uint32_t seed) {
   std::mt19937 rng(seed);
   const Samples sa = sample_surface(a, resolution, rng);

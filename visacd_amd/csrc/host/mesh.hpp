@@ -79,6 +79,7 @@ struct EngineOptions {
   std::string planes = "bisector";  // "bisector" (reference) or "sphere"
   int hausdorff_resolution = 3000;
   uint32_t sample_seed = 42;     // seeds subsampling and surface sampling
+  bool gpu_distance = true;      // Hausdorff term on the GPU (exact point-to-mesh)
   bool verbose = true;
 };
 inline EngineOptions engine;
@@ -106,6 +107,9 @@ MeshList separate_disjoint(const Mesh &m, bool drop_flat = true);
 MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos,
               std::vector<int> &map_neg);
 double concavity(const Mesh &part, const Mesh &hull, int resolution, uint32_t seed);
+// max(1000, resolution * area) area-weighted surface points plus one point in
+// every triangle left unsampled (lib/src/core.cpp:144-232)
+std::vector<Vec3> surface_samples(const Mesh &m, int resolution, std::mt19937 &rng);
 double hausdorff_boundary(const Mesh &a, const Mesh &b, int resolution, uint32_t seed);
 std::vector<Plane> flat_surface_planes(const Mesh &m, double min_area);
 

@@ -149,6 +149,51 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask) {
         {(unsigned)part.points[pairs[2 * e]], (unsigned)part.points[pairs[2 * e + 1]]});
 }
 
+vacd_bvh *build_bvh(const Engine &E, const Mesh &m) {
+  const std::vector<float> V = to_f32(m.vertices);
+  std::vector<uint32_t> T(3 * m.triangles.size());
+  for (size_t t = 0; t < m.triangles.size(); ++t)
+    for (int k = 0; k < 3; ++k) T[3 * t + k] = (uint32_t)m.triangles[t][k];
+  vacd_bvh *b = nullptr;
+  E.check(vacd_bvh_build(E.ctx, V.data(), (int64_t)m.vertices.size(), T.data(),
+                         (int64_t)m.triangles.size(), &b));
+  return b;
+}
+
+// Symmetric sampled surface distance with exact point-to-mesh distances
+// measured on the GPU (same samples as the host form).
+double gpu_hausdorff(const Engine &E, const Mesh &a, const Mesh &b, int resolution, uint32_t seed) {
+  std::mt19937 rng(seed);
+  const std::vector<Vec3> sa = surface_samples(a, resolution, rng);
+  const std::vector<Vec3> sb = surface_samples(b, resolution, rng);
+  if (sa.empty() || sb.empty()) return 1e300;
+  vacd_bvh *ba = build_bvh(E, a), *bb = nullptr;
+  float da = 0.f, db = 0.f;
+  int rc = VACD_OK;
+  try {
+    bb = build_bvh(E, b);
+    const std::vector<float> fa = to_f32(sa), fb = to_f32(sb);
+    rc = vacd_max_distance(E.ctx, fa.data(), (int64_t)sa.size(), bb, &da);
+    if (rc == VACD_OK) rc = vacd_max_distance(E.ctx, fb.data(), (int64_t)sb.size(), ba, &db);
+  } catch (...) {
+    vacd_bvh_destroy(ba);
+    vacd_bvh_destroy(bb);
+    throw;
+  }
+  vacd_bvh_destroy(ba);
+  vacd_bvh_destroy(bb);
+  E.check(rc);
+  return std::max((double)da, (double)db);
+}
+
+double part_concavity(const Engine &E, const Mesh &part) {
+  const Mesh hull = convex_hull(part.vertices);
+  if (!engine.gpu_distance || hull.triangles.empty())
+    return concavity(part, hull, engine.hausdorff_resolution, engine.sample_seed);
+  const double rv = std::cbrt(3.0 * std::fabs(mesh_volume(part) - mesh_volume(hull)) / (4.0 * M_PI));
+  return std::max(0.3 * rv, gpu_hausdorff(E, part, hull, engine.hausdorff_resolution, engine.sample_seed));
+}
+
 std::vector<Plane> bisector_planes(const Mesh &m, int num_planes) {
   std::vector<Plane> planes;
   const auto &E = m.intersecting_edges;
@@ -300,10 +345,7 @@ ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
     } else if (config.score_mode == "concavity") {
       double best = -1.0;
       for (int i = 0; i < (int)parts.size(); ++i) {
-        if (std::isnan(conc[i])) {
-          const Mesh hull = convex_hull(parts[i].vertices);
-          conc[i] = concavity(parts[i], hull, engine.hausdorff_resolution, engine.sample_seed);
-        }
+        if (std::isnan(conc[i])) conc[i] = part_concavity(D.E, parts[i]);
         if (conc[i] > best) best = conc[i], idx = i;
       }
       if (best < concavity_threshold) {
@@ -332,8 +374,11 @@ ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
   MeshList hulls;
   for (auto &p : parts) hulls.push_back(convex_hull(p.vertices));
   double final_h = 0.0;
-  for (size_t i = 0; i < parts.size(); ++i)
-    final_h = std::max(final_h, hausdorff_boundary(parts[i], hulls[i], 10000, engine.sample_seed));
+  for (size_t i = 0; i < parts.size(); ++i) {
+    const bool gpu = engine.gpu_distance && !hulls[i].triangles.empty();
+    final_h = std::max(final_h, gpu ? gpu_hausdorff(D.E, parts[i], hulls[i], 10000, engine.sample_seed)
+                                    : hausdorff_boundary(parts[i], hulls[i], 10000, engine.sample_seed));
+  }
   if (config.use_merging) log_line("Hull merging is not part of this engine; parts are kept.");
   for (auto &h : hulls) unnormalize(h, bbox);
   for (auto &p : parts) unnormalize(p, bbox);

@@ -174,6 +174,10 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n,
 void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, uint32_t *d_pairs,
                              int64_t cap, int64_t *d_count, int64_t *d_part_off);
 
+// Largest point-to-mesh distance (distance.hip); writes float bits.
+void launch_max_distance(vacd_ctx *ctx, const float *d_pts, int64_t m, const vacd_bvh *mesh,
+                         unsigned *d_out_bits);
+
 // Scoring (scoring.hip).
 void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P,
                     const float *d_pts, int64_t n, const uint32_t *d_edges,

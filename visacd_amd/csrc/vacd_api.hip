@@ -392,6 +392,38 @@ int vacd_score_planes(vacd_ctx *ctx, const float *planes, int64_t P, const float
   });
 }
 
+static void check_distance_bvh(const vacd_ctx *ctx, const vacd_bvh *mesh) {
+  VACD_REQUIRE(mesh != nullptr, VACD_ERR_INVALID, "NULL BVH");
+  VACD_REQUIRE(mesh->device == ctx->device, VACD_ERR_INVALID, "BVH built on another device");
+  VACD_REQUIRE(3 * (mesh->depth / 2) <= 64, VACD_ERR_LIMIT, "BVH too deep for the distance walk");
+}
+
+int vacd_max_distance_d(vacd_ctx *ctx, const float *d_pts, int64_t m, const vacd_bvh *mesh,
+                        float *d_out) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(m >= 0 && (m == 0 || d_pts) && d_out, VACD_ERR_INVALID, "bad arguments");
+    check_distance_bvh(ctx, mesh);
+    set_device(ctx);
+    vacd::launch_max_distance(ctx, d_pts, m, mesh, reinterpret_cast<unsigned *>(d_out));
+  });
+}
+
+int vacd_max_distance(vacd_ctx *ctx, const float *pts, int64_t m, const vacd_bvh *mesh, float *out) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(m >= 0 && (m == 0 || pts) && out, VACD_ERR_INVALID, "bad arguments");
+    check_distance_bvh(ctx, mesh);
+    set_device(ctx);
+    const float *dp = stage(ctx, SLOT_PTS, pts, (size_t)(3 * m));
+    vacd::DevBuf &o = ctx->buf[SLOT_COUNT];
+    o.ensure(sizeof(int64_t));
+    vacd::launch_max_distance(ctx, dp, m, mesh, o.as<unsigned>());
+    VACD_HIP(hipMemcpyAsync(out, o.ptr, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 void classify_and_rate_planes(const float *planes, const float *points, const unsigned int *edges,
                               float *scores, int numPlanes, int numPoints, int numEdges) {
   thread_local vacd_ctx *tctx = nullptr;
