@@ -9,6 +9,36 @@ from visacd_amd import workloads as wl
 pytestmark = pytest.mark.gpu
 
 
+def _point_tri_dist(p, a, b, c):
+    """Brute force: distance to the plane projection if inside, else to the edges."""
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n)
+    q = p - np.dot(p - a, n) * n
+    bc = [np.dot(np.cross(b - a, q - a), n), np.dot(np.cross(c - b, q - b), n),
+          np.dot(np.cross(a - c, q - c), n)]
+    if min(bc) >= 0:
+        return abs(np.dot(p - a, n))
+    best = np.inf
+    for u, v in ((a, b), (b, c), (c, a)):
+        t = np.clip(np.dot(p - u, v - u) / np.dot(v - u, v - u), 0, 1)
+        best = min(best, np.linalg.norm(p - (u + t * (v - u))))
+    return best
+
+
+def test_max_distance_matches_brute_force(ctx):
+    V, T = wl.perturbed_icosphere(2, 0.3, 31)
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(-1.5, 1.5, size=(200, 3)).astype(np.float32)
+    print("max distance", flush=True)
+    b = ctx.bvh(V, T)
+    got = ctx.max_distance(pts, b)
+    V64 = V.astype(np.float64)
+    ref = max(min(_point_tri_dist(p.astype(np.float64), V64[t[0]], V64[t[1]], V64[t[2]])
+                  for t in T) for p in pts)
+    assert abs(got - ref) <= 1e-5 * max(1.0, ref)
+    assert ctx.max_distance(V, b) < 1e-6  # mesh vertices lie on the mesh
+
+
 def test_batch_matches_oracle_per_part(ctx):
     import torch
 

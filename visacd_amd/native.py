@@ -27,7 +27,7 @@ EXPORTED = [
     "vacd_ctx_set_stream", "vacd_sync", "vacd_ctx_set_option", "vacd_ctx_timing", "vacd_bvh_build", "vacd_bvh_build_d",
     "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_export", "vacd_visibility",
     "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_batch_d", "vacd_score_planes",
-    "vacd_score_planes_d", "classify_and_rate_planes",
+    "vacd_score_planes_d", "vacd_max_distance", "vacd_max_distance_d", "classify_and_rate_planes",
 ]
 
 
@@ -94,6 +94,8 @@ def lib():
         L.vacd_score_planes.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp,
                                         ctypes.POINTER(i32)]
         L.vacd_score_planes_d.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp, vp]
+        L.vacd_max_distance.argtypes = [vp, vp, i64, vp, ctypes.POINTER(ctypes.c_float)]
+        L.vacd_max_distance_d.argtypes = [vp, vp, i64, vp, vp]
         L.classify_and_rate_planes.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int]
         L.classify_and_rate_planes.restype = None
@@ -206,6 +208,14 @@ class Context:
             arr[t] = Part(_p(p), n, _p(m), b.h, c.h if c is not None else None)
         self._check(lib().vacd_visibility_batch_d(self._h, arr, len(parts), _p(d_pairs), cap,
                                                   _p(d_count), _p(d_part_off)))
+
+    def max_distance(self, points, mesh_bvh: "Bvh") -> float:
+        """Largest distance from the host points to the BVH's mesh."""
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = ctypes.c_float()
+        self._check(lib().vacd_max_distance(self._h, _p(pts), pts.shape[0], mesh_bvh.h,
+                                            ctypes.byref(out)))
+        return float(out.value)
 
     def visibility_stats(self) -> dict:
         s = VisStats()
