@@ -171,6 +171,18 @@ def main():
     tb = ctx.timing(reset=True)
     binary_kernel_ms = tb["vis_ms"] / max(1, tb["vis_launches"])
     ctx.set_option(native.VACD_OPT_WIDE_BVH, 1)
+    # ... and with every pair walked (no fan shortcut for pairs starting inside)
+    ctx.set_option(native.VACD_OPT_FAN_PAIRS, 0)
+    vis_step()
+    ctx.timing(reset=True)
+    for _ in range(max(2, args.steps // 4)):
+        vis_step()
+    tf = ctx.timing(reset=True)
+    walk_all_kernel_ms = tf["vis_ms"] / max(1, tf["vis_launches"])
+    ctx.set_option(native.VACD_OPT_FAN_PAIRS, 1)
+    torch.cuda.synchronize()
+    if int(d_count.item()) != E:
+        raise SystemExit("visibility result differs between walk variants")
 
     # algorithmic work per segment (instrumented kernel variant, untimed)
     ctx.set_option(native.VACD_OPT_COUNT_WORK, 1)
@@ -232,6 +244,7 @@ def main():
                    "planes": P, "mode": "midpoint parity", "seed": W.meta.get("seed")},
         "visibility": {"edges": E, "blocked": st["blocked"], "exact_fallback_pairs": st["exact_calls"],
                        "kernel_ms": vis_kernel_ms, "kernel_ms_binary_walk": binary_kernel_ms,
+                       "kernel_ms_walk_every_pair": walk_all_kernel_ms,
                        "kernel_gsegments_per_s": S / (vis_kernel_ms * 1e-3) / 1e9,
                        "bvh_build_ms": bt["bvh_ms"], "bvh_depth": binfo["depth"],
                        "node_visits_per_segment": nodes_per_seg,

@@ -47,6 +47,8 @@ typedef struct vacd_vis_stats {
 #define VACD_OPT_TIMING 1     /* record HIP events around the main kernels  */
 #define VACD_OPT_COUNT_WORK 2 /* visibility counts node visits / tri tests  */
 #define VACD_OPT_WIDE_BVH 3   /* 4-wide node walk (default 1) vs binary    */
+#define VACD_OPT_FAN_PAIRS 4  /* parity mode: settle pairs that leave p     */
+                              /* into the solid from p's fan (default 1)    */
 
 /* Accumulated device time of the main kernels since the last reset
  * (VACD_OPT_TIMING): the visibility traversal kernel, the plane-scoring

@@ -171,12 +171,15 @@ def test_visibility_binary_and_wide_walks_agree(ctx, name):
     _print("walks", name, T.shape)
     o, _ = oracle.visibility(V, V, T)
     for wide in (0, 1):
-        ctx.set_option(native.VACD_OPT_WIDE_BVH, wide)
-        try:
-            g = ctx.visibility(V, ctx.bvh(V, T))
-        finally:
-            ctx.set_option(native.VACD_OPT_WIDE_BVH, 1)
-        np.testing.assert_array_equal(g, o)
+        for fan in (0, 1):
+            ctx.set_option(native.VACD_OPT_WIDE_BVH, wide)
+            ctx.set_option(native.VACD_OPT_FAN_PAIRS, fan)
+            try:
+                g = ctx.visibility(V, ctx.bvh(V, T))
+            finally:
+                ctx.set_option(native.VACD_OPT_WIDE_BVH, 1)
+                ctx.set_option(native.VACD_OPT_FAN_PAIRS, 1)
+            np.testing.assert_array_equal(g, o, err_msg=f"wide={wide} fan={fan}")
 
 
 def test_visibility_open_mesh(ctx):

@@ -111,6 +111,7 @@ enum VacdSlot {
   SLOT_MISC,
   SLOT_FAR,
   SLOT_PARTS,
+  SLOT_FAN,
   SLOT_COUNT_MAX
 };
 
@@ -124,6 +125,7 @@ struct vacd_ctx {
   bool timing = false;
   bool count_work = false;
   bool wide_bvh = true;
+  bool fan_pairs = true;  // decide pairs that start inside from the fan of p
   // recorded (start, stop) event pairs per timed kernel kind
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
 };

@@ -122,6 +122,8 @@ int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value) {
     ctx->count_work = value != 0;
   else if (option == VACD_OPT_WIDE_BVH)
     ctx->wide_bvh = value != 0;
+  else if (option == VACD_OPT_FAN_PAIRS)
+    ctx->fan_pairs = value != 0;
   else
     return fail(ctx, Error{VACD_ERR_INVALID, "unknown option"});
   return VACD_OK;

@@ -386,11 +386,16 @@ __device__ __forceinline__ int walk_parity4(const Node4 *__restrict__ nodes,
 }
 
 // What a pair needs to know about its part (one mesh and its point set).
+constexpr int kFanMax = 16;  // triangles kept per point fan
+
 struct Part {
   const float *pts;
   const uint8_t *mask;
   const uint8_t *far;  // per point: crossing parity of its +x ray with non-incident
                        // triangles (0/1, 2 = undecided); NULL unless the mesh is closed
+  const int *fan_n;    // per point: triangles having the point as a vertex (-1: more
+                       // than kFanMax); NULL together with far
+  const int *fan_idx;  // per point: kFanMax leaf-ordered triangle ids
   const Node *snodes;
   const Node4 *snodes4;
   const Tri *stris;
@@ -463,7 +468,25 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
         Seg s;
 #pragma unroll
         for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
-        if (!same3(s.p, s.q)) {
+        bool decided = same3(s.p, s.q);  // zero length: not an edge
+        if (!decided && P.fan_n && !(CAGE && P.ctris)) {
+          // A pair that leaves p into the solid is blocked or wholly inside:
+          // never an edge.  Decide that from p's fan alone (every lane of a
+          // row shares p, so the fan loads are broadcasts).
+          const int fn = P.fan_n[i];
+          if (fn > 0) {
+            int xp = 0, ni = 0, r = 0;
+            for (int f = 0; f < fn && r == 0; ++f)
+              r = contact_fast(s, P.stris[P.fan_idx[(size_t)i * kFanMax + f]], xp, ni);
+            if (r == 1) {  // the segment is an edge of the mesh
+              ++n_blocked;
+              decided = true;
+            } else if (r == 0 && P.far[i] <= 1 && ((xp ^ P.far[i]) & 1)) {
+              decided = true;  // starts inside
+            }
+          }
+        }
+        if (!decided) {
           make_seg(s);
           int xpar = 0, ninc = 0;
           const int hs = any_hit<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap,
@@ -645,6 +668,69 @@ __global__ __launch_bounds__(64) void k_vis_exact(VisArgs a) {
 
 // Crossing parity of the +x ray from each point with the triangles that do
 // not have the point as a vertex: 0/1, or 2 when a test is undecided.
+// Fan of each point: the leaf-ordered ids of the triangles that have the
+// point as a vertex (bitwise-equal coordinates), found by a point query on
+// the 4-wide BVH; -1 when there are more than kFanMax.
+template <bool BATCH>
+__global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
+                                            int *__restrict__ fan_idx) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= a.npoints) return;
+  Part P = a.one;
+  if (BATCH) {
+    int lo = 0, hi = a.nparts - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.parts[mid].point_begin <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    P = a.parts[lo];
+  }
+  if (!P.fan_n) return;
+  const int64_t i = g - P.point_begin;
+  const float p[3] = {P.pts[3 * i], P.pts[3 * i + 1], P.pts[3 * i + 2]};
+  int stk[64];
+  int node = 0, sp = 0, cnt = 0;
+  bool done = false;
+  int *out = fan_idx + (size_t)g * kFanMax;
+  for (int it = 0; it <= P.scap && !done; ++it) {
+    const Node4 nd = P.snodes4[node];
+    int next = -1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const bool in = p[0] >= lane_of(nd.f[0], c) && p[0] <= lane_of(nd.f[3], c) &&
+                      p[1] >= lane_of(nd.f[1], c) && p[1] <= lane_of(nd.f[4], c) &&
+                      p[2] >= lane_of(nd.f[2], c) && p[2] <= lane_of(nd.f[5], c);
+      if (!in) continue;
+      const int id = __float_as_int(lane_of(nd.f[6], c));
+      if (id < 0) {
+        const Tri t = P.stris[~id];
+        const float a3[3] = {t.a.x, t.a.y, t.a.z}, b3[3] = {t.b.x, t.b.y, t.b.z},
+                    c3[3] = {t.c.x, t.c.y, t.c.z};
+        if (same3(p, a3) || same3(p, b3) || same3(p, c3)) {
+          if (cnt < kFanMax) out[cnt] = ~id;
+          ++cnt;
+        }
+      } else if (next < 0) {
+        next = id;
+      } else if (sp < 64) {
+        stk[sp++] = id;
+      } else {
+        cnt = kFanMax + 1;  // stack exhausted: give up on this point's fan
+      }
+    }
+    if (next >= 0) {
+      node = next;
+    } else if (sp == 0) {
+      done = true;
+    } else {
+      node = stk[--sp];
+    }
+  }
+  // an unfinished walk or an overfull fan leaves the point to the full walk
+  fan_n[g] = (!done || cnt > kFanMax) ? -1 : cnt;
+}
+
 // In a batch, thread g handles global point g of the part whose point range
 // holds it; far is indexed by the global point.
 template <bool BATCH>
@@ -883,6 +969,8 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   P.pts = d_pts;
   P.mask = d_mask;
   P.far = nullptr;
+  P.fan_n = nullptr;
+  P.fan_idx = nullptr;
   P.snodes = self->nodes.as<Node>();
   P.snodes4 = self->nodes4.as<Node4>();
   P.stris = self->tris.as<Tri>();
@@ -921,6 +1009,16 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     else
       k_far_parity<false><<<fg, 64, 0, st>>>(a, farb.as<uint8_t>());
     VACD_HIP(hipGetLastError());
+    if (ctx->fan_pairs) {
+      vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
+      int *fan_n = fanb.as<int>();
+      int *fan_idx = fan_n + a.npoints;
+      if (batch)
+        k_fan<true><<<fg, 64, 0, st>>>(a, fan_n, fan_idx);
+      else
+        k_fan<false><<<fg, 64, 0, st>>>(a, fan_n, fan_idx);
+      VACD_HIP(hipGetLastError());
+    }
   }
   VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
   // stack entries a walk can hold: binary, one pending sibling per level;
@@ -982,12 +1080,18 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
     return;
   }
   const bool any_far = !cage && self->closed;
+  const int depth = std::max(self->depth, cage ? cage->depth : 0);
   if (any_far) {
     vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
     farb.ensure((size_t)n);
     a.one.far = farb.as<uint8_t>();
+    if (ctx->fan_pairs && 3 * (depth / 2) <= 64) {  // the fan query walks the 4-wide tree
+      vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
+      fanb.ensure(sizeof(int) * (size_t)n * (1 + kFanMax));
+      a.one.fan_n = fanb.as<int>();
+      a.one.fan_idx = fanb.as<int>() + n;
+    }
   }
-  const int depth = std::max(self->depth, cage ? cage->depth : 0);
   run(ctx, a, false, cage != nullptr, any_far, depth, d_pairs, cap, d_count, nullptr);
 }
 
@@ -1022,8 +1126,17 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
   if (any_far) {
     vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
     farb.ensure((size_t)std::max<int64_t>(points, 1));
+    const bool fan = ctx->fan_pairs && 3 * (depth / 2) <= 64;
+    vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
+    if (fan) fanb.ensure(sizeof(int) * (size_t)std::max<int64_t>(points, 1) * (1 + kFanMax));
     for (int p = 0; p < nparts; ++p)
-      if (!hp[p].ctris && parts[p].self->closed) hp[p].far = farb.as<uint8_t>() + hp[p].point_begin;
+      if (!hp[p].ctris && parts[p].self->closed) {
+        hp[p].far = farb.as<uint8_t>() + hp[p].point_begin;
+        if (fan) {
+          hp[p].fan_n = fanb.as<int>() + hp[p].point_begin;
+          hp[p].fan_idx = fanb.as<int>() + points + hp[p].point_begin * kFanMax;
+        }
+      }
   }
   vacd::DevBuf &pb = ctx->buf[SLOT_PARTS];
   pb.ensure(sizeof(Part) * (size_t)nparts);

@@ -58,6 +58,7 @@ class Timing(ctypes.Structure):
 VACD_OPT_TIMING = 1
 VACD_OPT_COUNT_WORK = 2
 VACD_OPT_WIDE_BVH = 3
+VACD_OPT_FAN_PAIRS = 4
 
 
 _lib = None
