@@ -6,7 +6,11 @@ Same names and call signatures as the reference's pybind module
 ``VecArray3i``, ``make_vecarray3i`` and ``ProcessResult``.  ``engine`` holds
 the options of this engine that the reference does not have.
 """
-from visacd_amd._visacd import (  # noqa: F401
+from visacd_amd import share_hip_runtime
+
+share_hip_runtime()
+
+from visacd_amd._visacd import (  # noqa: E402,F401
     Config,
     EngineOptions,
     Mesh,

@@ -112,6 +112,7 @@ enum VacdSlot {
   SLOT_FAR,
   SLOT_PARTS,
   SLOT_FAN,
+  SLOT_QUEUE,
   SLOT_COUNT_MAX
 };
 

@@ -416,7 +416,9 @@ struct VisArgs {
   uint64_t *bits;
   uint64_t *defer;
   unsigned long long *stats;  // [0] masked, [1] blocked, [2] deferred, [3] edges, [4] malformed,
-                              // [5] node visits, [6] triangle tests
+                              // [5] node visits, [6] triangle tests, [7] walk-queue length
+  int64_t *queue;             // global pair ids (word * 64 + bit) left to walk
+  unsigned long long *queue_len;
 };
 
 // Part owning global ballot word w (binary search over word_begin).
@@ -448,7 +450,46 @@ __device__ __forceinline__ int ray_parity(const Node *n2, const Node4 *n4, const
   return walk_parity<STACK, COUNT>(n2, tris, s, stk, cap, nv, nt);
 }
 
-template <int STACK, bool CAGE, bool COUNT, bool WIDE, bool BATCH>
+// Walks for one pair the fan could not settle: any contact with the self
+// mesh, then the cage walk (cage mode) or the inside parity (parity mode).
+template <int STACK, bool CAGE, bool COUNT, bool WIDE>
+__device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int *stk, bool &edge,
+                                          bool &defer, unsigned &n_blocked, unsigned &n_bad,
+                                          unsigned &nv, unsigned &nt) {
+  make_seg(s);
+  int xpar = 0, ninc = 0;
+  const int hs =
+      any_hit<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap, nv, nt, xpar, ninc);
+  if (hs == 1) {
+    ++n_blocked;
+  } else if (hs == 2) {
+    defer = true;
+  } else if (hs == 3) {
+    ++n_bad;
+  } else if (CAGE && P.ctris) {
+    int cx = 0, cn = 0;
+    const int hc =
+        any_hit<STACK, WIDE, COUNT>(P.cnodes, P.cnodes4, P.ctris, s, stk, P.ccap, nv, nt, cx, cn);
+    edge = hc == 1;
+    defer = hc == 2;
+    n_bad += hc == 3;
+  } else if (P.far && ninc > 0) {
+    // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
+    const uint8_t fp = P.far[i];
+    if (fp > 1)
+      defer = true;
+    else
+      edge = ((xpar ^ fp) & 1) == 0;
+  } else {
+    const int par =
+        ray_parity<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap, nv, nt);
+    edge = par == 0;
+    defer = par == 2;
+    n_bad += par == 3;
+  }
+}
+
+template <int STACK, bool CAGE, bool COUNT, bool WIDE, bool BATCH, bool QUEUE>
 __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   __shared__ int stk_all[kWaves][STACK][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -458,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < a.nwords; w += total_waves) {
     const Part P = part_of_word<BATCH>(a, w);  // wave-uniform
     const int64_t k = (w - P.word_begin) * 64 + lane;
-    bool edge = false, defer = false;
+    bool edge = false, defer = false, queued = false;
     if (k < P.S) {
       int64_t i, j;
       pair_of(k, P.n, i, j);
@@ -487,37 +528,10 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           }
         }
         if (!decided) {
-          make_seg(s);
-          int xpar = 0, ninc = 0;
-          const int hs = any_hit<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap,
-                                                     nv, nt, xpar, ninc);
-          if (hs == 1) {
-            ++n_blocked;
-          } else if (hs == 2) {
-            defer = true;
-          } else if (hs == 3) {
-            ++n_bad;
-          } else if (CAGE && P.ctris) {
-            int cx = 0, cn = 0;
-            const int hc = any_hit<STACK, WIDE, COUNT>(P.cnodes, P.cnodes4, P.ctris, s, stk,
-                                                       P.ccap, nv, nt, cx, cn);
-            edge = hc == 1;
-            defer = hc == 2;
-            n_bad += hc == 3;
-          } else if (P.far && ninc > 0) {
-            // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
-            const uint8_t fp = P.far[i];
-            if (fp > 1)
-              defer = true;
-            else
-              edge = ((xpar ^ fp) & 1) == 0;
-          } else {
-            const int par =
-                ray_parity<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap, nv, nt);
-            edge = par == 0;
-            defer = par == 2;
-            n_bad += par == 3;
-          }
+          if constexpr (QUEUE)
+            queued = true;
+          else
+            walk_pair<STACK, CAGE, COUNT, WIDE>(P, i, s, stk, edge, defer, n_blocked, n_bad, nv, nt);
         }
       }
     }
@@ -526,6 +540,18 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     if (lane == 0) {
       a.bits[w] = eb;
       a.defer[w] = db;
+    }
+    if (QUEUE) {  // append this wave's undecided pairs (global k) to the walk queue
+      const uint64_t qb = __ballot(queued);
+      if (qb) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(a.queue_len, (unsigned long long)__popcll(qb));
+        base = __shfl(base, 0);
+        if (queued) {
+          const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+          a.queue[base + __popcll(qb & below)] = w * 64 + lane;
+        }
+      }
     }
   }
   // per-wave totals, one atomic per wave
@@ -550,6 +576,53 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     if (lane == 0) {
       atomicAdd(&a.stats[5], v);
       atomicAdd(&a.stats[6], t);
+    }
+  }
+}
+
+// Dense walks over the queue of pairs the fan pass left undecided: one lane
+// per queued pair; results are OR-ed into the ballot words (the fan pass
+// wrote zero for these bits).
+template <int STACK, bool COUNT, bool WIDE, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_walk_queue(VisArgs a) {
+  __shared__ int stk_all[kWaves][STACK][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int *stk = &stk_all[wave][0][lane];
+  unsigned n_blocked = 0, n_defer = 0, n_bad = 0, nv = 0, nt = 0;
+  const int64_t qn = (int64_t)*a.queue_len;
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < qn;
+       e += (int64_t)gridDim.x * kBlock) {
+    const int64_t kg = a.queue[e];
+    const int64_t w = kg >> 6;
+    const int b = (int)(kg & 63);
+    const Part P = part_of_word<BATCH>(a, w);
+    int64_t i, j;
+    pair_of((w - P.word_begin) * 64 + b, P.n, i, j);
+    Seg s;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
+    bool edge = false, defer = false;
+    walk_pair<STACK, false, COUNT, WIDE>(P, i, s, stk, edge, defer, n_blocked, n_bad, nv, nt);
+    if (edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
+    if (defer) {
+      atomicOr(reinterpret_cast<unsigned long long *>(&a.defer[w]), 1ull << b);
+      ++n_defer;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    n_blocked += __shfl_down(n_blocked, off);
+    n_defer += __shfl_down(n_defer, off);
+    n_bad += __shfl_down(n_bad, off);
+    nv += __shfl_down(nv, off);
+    nt += __shfl_down(nt, off);
+  }
+  if (lane == 0) {
+    if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
+    if (n_defer) atomicAdd(&a.stats[2], (unsigned long long)n_defer);
+    if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
+    if (COUNT) {
+      atomicAdd(&a.stats[5], (unsigned long long)nv);
+      atomicAdd(&a.stats[6], (unsigned long long)nt);
     }
   }
 }
@@ -938,29 +1011,41 @@ __global__ void k_fix_part_offsets(VisArgs a, const int64_t *__restrict__ total,
     if (a.parts[p].S == 0) part_off[p] = part_off[p + 1];
 }
 
+struct Launch {
+  unsigned grid, qgrid;
+  hipStream_t st;
+};
+
+// The fan pass queues what it cannot settle (no walk stack needed there);
+// the queue is then walked densely.
 template <int S, bool C, bool N, bool W, bool B>
-void launch_main(unsigned grid, hipStream_t st, const VisArgs &a) {
-  k_visibility<S, C, N, W, B><<<grid, kBlock, 0, st>>>(a);
+void launch_main(bool q, const Launch &L, const VisArgs &a) {
+  if (q && !C) {
+    k_visibility<1, C, N, W, B, true><<<L.grid, kBlock, 0, L.st>>>(a);  // no walk, no stack
+    k_walk_queue<S, N, W, B><<<L.qgrid, kBlock, 0, L.st>>>(a);
+  } else {
+    k_visibility<S, C, N, W, B, false><<<L.grid, kBlock, 0, L.st>>>(a);
+  }
 }
 template <int S, bool C, bool N, bool W>
-void pick_b(bool b, unsigned grid, hipStream_t st, const VisArgs &a) {
-  if (b) launch_main<S, C, N, W, true>(grid, st, a);
-  else launch_main<S, C, N, W, false>(grid, st, a);
+void pick_b(bool b, bool q, const Launch &L, const VisArgs &a) {
+  if (b) launch_main<S, C, N, W, true>(q, L, a);
+  else launch_main<S, C, N, W, false>(q, L, a);
 }
 template <int S, bool C, bool N>
-void pick_w(bool w, bool b, unsigned grid, hipStream_t st, const VisArgs &a) {
-  if (w) pick_b<S, C, N, true>(b, grid, st, a);
-  else pick_b<S, C, N, false>(b, grid, st, a);
+void pick_w(bool w, bool b, bool q, const Launch &L, const VisArgs &a) {
+  if (w) pick_b<S, C, N, true>(b, q, L, a);
+  else pick_b<S, C, N, false>(b, q, L, a);
 }
 template <int S, bool C>
-void pick_n(bool n, bool w, bool b, unsigned grid, hipStream_t st, const VisArgs &a) {
-  if (n) pick_w<S, C, true>(w, b, grid, st, a);
-  else pick_w<S, C, false>(w, b, grid, st, a);
+void pick_n(bool n, bool w, bool b, bool q, const Launch &L, const VisArgs &a) {
+  if (n) pick_w<S, C, true>(w, b, q, L, a);
+  else pick_w<S, C, false>(w, b, q, L, a);
 }
 template <int S>
-void pick_c(bool c, bool n, bool w, bool b, unsigned grid, hipStream_t st, const VisArgs &a) {
-  if (c) pick_n<S, true>(n, w, b, grid, st, a);
-  else pick_n<S, false>(n, w, b, grid, st, a);
+void pick_c(bool c, bool n, bool w, bool b, bool q, const Launch &L, const VisArgs &a) {
+  if (c) pick_n<S, true>(n, w, b, q, L, a);
+  else pick_n<S, false>(n, w, b, q, L, a);
 }
 
 Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_bvh *self,
@@ -1027,12 +1112,24 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   const bool wide = ctx->wide_bvh && need4 <= 64;
   const int need = wide ? need4 : need2;
   const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
-  const unsigned grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
+  Launch L;
+  L.grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
+  L.qgrid = (unsigned)std::min<int64_t>((nwords * 64 + kBlock - 1) / kBlock, 256 * 16);
+  L.st = st;
+  // parity mode with fans: queue the pairs the fan pass leaves open
+  const bool queue = any_far && ctx->fan_pairs && !any_cage && need4 <= 64;
+  a.queue_len = a.stats + 7;
+  a.queue = nullptr;
+  if (queue) {
+    vacd::DevBuf &qb = ctx->buf[SLOT_QUEUE];
+    qb.ensure(8 * (size_t)nwords * 64);
+    a.queue = qb.as<int64_t>();
+  }
   timed_begin(ctx, TIMED_VIS);
   if (need <= 32)
-    pick_c<32>(any_cage, ctx->count_work, wide, batch, grid, st, a);
+    pick_c<32>(any_cage, ctx->count_work, wide, batch, queue, L, a);
   else
-    pick_c<64>(any_cage, ctx->count_work, wide, batch, grid, st, a);
+    pick_c<64>(any_cage, ctx->count_work, wide, batch, queue, L, a);
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_VIS);
   const unsigned egrid = (unsigned)std::min<int64_t>(nwords, 4096);

@@ -71,6 +71,9 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -m visacd_amd.build`")
+        from visacd_amd import share_hip_runtime
+
+        share_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
         L.vacd_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
