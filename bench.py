@@ -162,15 +162,15 @@ def main():
         raise SystemExit("visibility result changed between steps")
     vis_kernel_ms = tv["vis_ms"] / max(1, tv["vis_launches"])
 
-    # the same pass with the binary node walk, for comparison
-    ctx.set_option(native.VACD_OPT_WIDE_BVH, 0)
+    # the same pass with the 4-wide node walk, for comparison
+    ctx.set_option(native.VACD_OPT_WIDE_BVH, 1)
     vis_step()
     ctx.timing(reset=True)
     for _ in range(max(2, args.steps // 4)):
         vis_step()
     tb = ctx.timing(reset=True)
-    binary_kernel_ms = tb["vis_ms"] / max(1, tb["vis_launches"])
-    ctx.set_option(native.VACD_OPT_WIDE_BVH, 1)
+    wide_kernel_ms = tb["vis_ms"] / max(1, tb["vis_launches"])
+    ctx.set_option(native.VACD_OPT_WIDE_BVH, 0)
     # ... and with every pair walked (no fan shortcut for pairs starting inside)
     ctx.set_option(native.VACD_OPT_FAN_PAIRS, 0)
     vis_step()
@@ -192,10 +192,10 @@ def main():
     ctx.timing(reset=True)
     nodes_per_seg = stw["node_visits"] / S
     tris_per_seg = stw["tri_tests"] / S
-    # a 4-wide node visit needs its 4 child boxes and ids (112 B); a
-    # triangle test its 9 coordinates (36 B); a segment its endpoints (24 B)
-    # and one result bit (counted as 1 B)
-    bytes_per_seg = 24 + 1 + 112 * nodes_per_seg + 36 * tris_per_seg
+    # a binary node visit loads one 64-byte node (both child boxes and ids);
+    # a triangle test its 9 coordinates (36 B); a segment its endpoints
+    # (24 B) and one result bit (counted as 1 B)
+    bytes_per_seg = 24 + 1 + 64 * nodes_per_seg + 36 * tris_per_seg
     achieved_gbs = bytes_per_seg * S / (vis_kernel_ms * 1e-3) / 1e9
 
     # plane scoring on the visibility edges of this workload
@@ -243,12 +243,13 @@ def main():
                    "triangles": int(W.triangles.shape[0]), "points": n, "segments": S,
                    "planes": P, "mode": "midpoint parity", "seed": W.meta.get("seed")},
         "visibility": {"edges": E, "blocked": st["blocked"], "exact_fallback_pairs": st["exact_calls"],
-                       "kernel_ms": vis_kernel_ms, "kernel_ms_binary_walk": binary_kernel_ms,
+                       "kernel_ms": vis_kernel_ms, "kernel_ms_wide_walk": wide_kernel_ms,
                        "kernel_ms_walk_every_pair": walk_all_kernel_ms,
                        "kernel_gsegments_per_s": S / (vis_kernel_ms * 1e-3) / 1e9,
                        "bvh_build_ms": bt["bvh_ms"], "bvh_depth": binfo["depth"],
                        "node_visits_per_segment": nodes_per_seg,
                        "tri_tests_per_segment": tris_per_seg,
+                       "walked_pairs": st["walked"],
                        "alg_bytes_per_segment": bytes_per_seg},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,

@@ -41,12 +41,14 @@ typedef struct vacd_vis_stats {
   int64_t exact_calls;/* pairs redone by the exact-predicate fallback     */
   int64_t node_visits;/* BVH nodes loaded (VACD_OPT_COUNT_WORK only)     */
   int64_t tri_tests;  /* triangle tests (VACD_OPT_COUNT_WORK only)       */
+  int64_t walked;     /* pairs that needed a BVH walk (the rest were      */
+                      /* masked, zero-length or settled by p's fan)       */
 } vacd_vis_stats;
 
 /* Context options (vacd_ctx_set_option). */
 #define VACD_OPT_TIMING 1     /* record HIP events around the main kernels  */
 #define VACD_OPT_COUNT_WORK 2 /* visibility counts node visits / tri tests  */
-#define VACD_OPT_WIDE_BVH 3   /* 4-wide node walk (default 1) vs binary    */
+#define VACD_OPT_WIDE_BVH 3   /* 4-wide node walk vs binary (default 0)    */
 #define VACD_OPT_FAN_PAIRS 4  /* parity mode: settle pairs that leave p     */
                               /* into the solid from p's fan (default 1)    */
 

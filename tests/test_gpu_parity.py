@@ -183,7 +183,7 @@ def test_visibility_binary_and_wide_walks_agree(ctx, name):
             try:
                 g = ctx.visibility(V, ctx.bvh(V, T))
             finally:
-                ctx.set_option(native.VACD_OPT_WIDE_BVH, 1)
+                ctx.set_option(native.VACD_OPT_WIDE_BVH, 0)
                 ctx.set_option(native.VACD_OPT_FAN_PAIRS, 1)
             np.testing.assert_array_equal(g, o, err_msg=f"wide={wide} fan={fan}")
 

@@ -125,7 +125,7 @@ struct vacd_ctx {
   int64_t last_segments = 0;
   bool timing = false;
   bool count_work = false;
-  bool wide_bvh = true;
+  bool wide_bvh = false;  // 4-wide node walk (measured slower than binary on gfx950)
   bool fan_pairs = true;  // decide pairs that start inside from the fan of p
   // recorded (start, stop) event pairs per timed kernel kind
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];

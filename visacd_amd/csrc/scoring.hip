@@ -6,9 +6,9 @@
 // Here:
 //   k_edge_prep  one 32-byte record per edge: both endpoints and the edge
 //                length rounded once to Q32.32 (int64).
-//   k_score      a block owns 1024 planes (4 per lane, in registers) and one
-//                chunk of edges; edge records are staged 256 at a time in LDS
-//                and read as wave-wide broadcasts; crossings add the Q32.32
+//   k_score      a block owns 2048 planes (8 per lane, in registers) and one
+//                chunk of edges; every lane reads the same edge record, so the
+//                records come through scalar loads; crossings add the Q32.32
 //                length to per-lane int64 sums, flushed with one integer
 //                atomic per plane per block.  Integer addition makes the
 //                result exact and independent of order.
@@ -24,9 +24,9 @@ namespace vacd {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kPPT = 4;                     // planes per lane
+constexpr int kPPT = 8;                     // planes per lane
 constexpr int kPlanesPerBlock = kBlock * kPPT;
-constexpr int kEdgeTile = 256;              // edge records per LDS tile
+constexpr int kEdgeQuantum = 256;           // edge chunks are multiples of this
 constexpr float kEps = 1e-6f;
 
 struct alignas(16) EdgeRec {
@@ -67,52 +67,71 @@ __global__ __launch_bounds__(256) void k_edge_prep(const float *__restrict__ pts
   rec[e] = r;
 }
 
+// Planes held per lane, oriented per run of edges that share a first
+// endpoint u: the copy (fa, fb, fc, fd) is the plane itself when u is on its
+// + side, the negated plane when u is on its - side, and (0, 0, 0, 1) when u
+// is in the dead zone.  Negation is exact in round-to-nearest, so the
+// oriented value of the second endpoint v is exactly +-val(v), and the edge
+// crosses iff that value is < -1e-6 (with (0, 0, 0, 1) it never is).
+template <int PPT>
+struct Lanes {
+  float pa[PPT], pb[PPT], pc[PPT], pd[PPT];
+  float fa[PPT], fb[PPT], fc[PPT], fd[PPT];
+  unsigned long long sum[PPT];
+};
+
+__device__ __forceinline__ float plane_val(float a, float b, float c, float d, float x, float y,
+                                           float z) {
+  return fmaf(c, z, fmaf(b, y, a * x)) + d;
+}
+
+// Edge records are read with wave-uniform addresses (every lane of a block
+// walks the same chunk), so they arrive through scalar loads and feed the
+// vector ALU as scalar operands; no LDS staging is needed.
 __global__ __launch_bounds__(kBlock) void k_score(const float4 *__restrict__ planes, int64_t P,
                                                   const EdgeRec *__restrict__ rec, int64_t E,
                                                   int64_t chunk, unsigned long long *__restrict__ acc) {
-  __shared__ EdgeRec tile[kEdgeTile];
-  float pa[kPPT], pb[kPPT], pc[kPPT], pd[kPPT];
-  unsigned long long sum[kPPT];
+  Lanes<kPPT> L;
 #pragma unroll
   for (int r = 0; r < kPPT; ++r) {
     const int64_t p = (int64_t)blockIdx.x * kPlanesPerBlock + r * kBlock + threadIdx.x;
     const float4 pl = p < P ? planes[p] : make_float4(0.f, 0.f, 0.f, 0.f);
-    pa[r] = pl.x, pb[r] = pl.y, pc[r] = pl.z, pd[r] = pl.w;
-    sum[r] = 0ull;
+    L.pa[r] = pl.x, L.pb[r] = pl.y, L.pc[r] = pl.z, L.pd[r] = pl.w;
+    L.fa[r] = 0.f, L.fb[r] = 0.f, L.fc[r] = 0.f, L.fd[r] = 1.f;
+    L.sum[r] = 0ull;
   }
   const int64_t e0 = (int64_t)blockIdx.y * chunk;
   const int64_t e1 = e0 + chunk < E ? e0 + chunk : E;
-  float vu[kPPT];
-  bool have_u = false;
-  for (int64_t base = e0; base < e1; base += kEdgeTile) {
-    const int cnt = (int)(e1 - base < kEdgeTile ? e1 - base : kEdgeTile);
-    __syncthreads();
-    if (threadIdx.x < cnt) tile[threadIdx.x] = rec[base + threadIdx.x];
-    __syncthreads();
-    for (int t = 0; t < cnt; ++t) {
-      const float4 ra = tile[t].a, rb = tile[t].b;
-      const unsigned hi = __float_as_uint(rb.w);
-      const unsigned long long lq =
-          ((unsigned long long)(hi & ~kSameFirst) << 32) | (unsigned)__float_as_int(rb.z);
-      // wave-uniform: every lane reads the same edge
-      if (!have_u || !(hi & kSameFirst)) {
-#pragma unroll
-        for (int r = 0; r < kPPT; ++r)
-          vu[r] = fmaf(pc[r], ra.z, fmaf(pb[r], ra.y, pa[r] * ra.x)) + pd[r];
-        have_u = true;
-      }
+  if (e0 >= e1) return;
+  EdgeRec nxt = rec[e0];
+  for (int64_t e = e0; e < e1; ++e) {
+    const EdgeRec rc = nxt;
+    if (e + 1 < e1) nxt = rec[e + 1];  // issued one edge ahead of its use
+    const float4 ra = rc.a, rb = rc.b;
+    const unsigned hi = __float_as_uint(rb.w);
+    const unsigned long long lq =
+        ((unsigned long long)(hi & ~kSameFirst) << 32) | (unsigned)__float_as_int(rb.z);
+    if (e == e0 || !(hi & kSameFirst)) {  // new first endpoint: re-orient the planes
 #pragma unroll
       for (int r = 0; r < kPPT; ++r) {
-        const float vv = fmaf(pc[r], rb.y, fmaf(pb[r], rb.x, pa[r] * ra.w)) + pd[r];
-        const bool cross = fminf(vu[r], vv) < -kEps && fmaxf(vu[r], vv) > kEps;
-        sum[r] += cross ? lq : 0ull;
+        const float vu = plane_val(L.pa[r], L.pb[r], L.pc[r], L.pd[r], ra.x, ra.y, ra.z);
+        const float sg = vu > kEps ? 1.f : (vu < -kEps ? -1.f : 0.f);
+        L.fa[r] = sg * L.pa[r];
+        L.fb[r] = sg * L.pb[r];
+        L.fc[r] = sg * L.pc[r];
+        L.fd[r] = sg != 0.f ? sg * L.pd[r] : 1.f;
       }
+    }
+#pragma unroll
+    for (int r = 0; r < kPPT; ++r) {
+      const float vv = plane_val(L.fa[r], L.fb[r], L.fc[r], L.fd[r], ra.w, rb.x, rb.y);
+      L.sum[r] += vv < -kEps ? lq : 0ull;
     }
   }
 #pragma unroll
   for (int r = 0; r < kPPT; ++r) {
     const int64_t p = (int64_t)blockIdx.x * kPlanesPerBlock + r * kBlock + threadIdx.x;
-    if (p < P && sum[r]) atomicAdd(&acc[p], sum[r]);
+    if (p < P && L.sum[r]) atomicAdd(&acc[p], L.sum[r]);
   }
 }
 
@@ -168,10 +187,10 @@ void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P, const float
     EdgeRec *rec = recb.as<EdgeRec>();
     k_edge_prep<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(d_pts, n, d_edges, E, rec);
     const int64_t ptiles = (P + kPlanesPerBlock - 1) / kPlanesPerBlock;
-    // about 4096 blocks in total, chunks a multiple of the LDS tile
+    // about 4096 blocks in total
     int64_t nchunks = std::max<int64_t>(1, 4096 / ptiles);
     int64_t chunk = (E + nchunks - 1) / nchunks;
-    chunk = std::max<int64_t>(kEdgeTile, (chunk + kEdgeTile - 1) / kEdgeTile * kEdgeTile);
+    chunk = std::max<int64_t>(kEdgeQuantum, (chunk + kEdgeQuantum - 1) / kEdgeQuantum * kEdgeQuantum);
     nchunks = (E + chunk - 1) / chunk;
     VACD_REQUIRE(nchunks < 65536, VACD_ERR_LIMIT, "too many edge chunks");
     dim3 grid((unsigned)ptiles, (unsigned)nchunks);

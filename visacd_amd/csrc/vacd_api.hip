@@ -333,6 +333,7 @@ int vacd_visibility_stats(vacd_ctx *ctx, vacd_vis_stats *out) {
     out->edges = (int64_t)s[3];
     out->node_visits = (int64_t)s[5];
     out->tri_tests = (int64_t)s[6];
+    out->walked = (int64_t)s[7];
   });
 }
 

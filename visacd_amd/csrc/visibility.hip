@@ -494,7 +494,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   __shared__ int stk_all[kWaves][STACK][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int *stk = &stk_all[wave][0][lane];
-  unsigned n_masked = 0, n_blocked = 0, n_defer = 0, n_bad = 0, nv = 0, nt = 0;
+  unsigned n_masked = 0, n_blocked = 0, n_defer = 0, n_bad = 0, nv = 0, nt = 0, n_walk = 0;
   const int64_t total_waves = (int64_t)gridDim.x * kWaves;
   for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < a.nwords; w += total_waves) {
     const Part P = part_of_word<BATCH>(a, w);  // wave-uniform
@@ -528,10 +528,12 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           }
         }
         if (!decided) {
-          if constexpr (QUEUE)
+          if constexpr (QUEUE) {
             queued = true;
-          else
+          } else {
+            ++n_walk;
             walk_pair<STACK, CAGE, COUNT, WIDE>(P, i, s, stk, edge, defer, n_blocked, n_bad, nv, nt);
+          }
         }
       }
     }
@@ -560,12 +562,14 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     n_blocked += __shfl_down(n_blocked, off);
     n_defer += __shfl_down(n_defer, off);
     n_bad += __shfl_down(n_bad, off);
+    n_walk += __shfl_down(n_walk, off);
   }
   if (lane == 0) {
     if (n_masked) atomicAdd(&a.stats[0], (unsigned long long)n_masked);
     if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
     if (n_defer) atomicAdd(&a.stats[2], (unsigned long long)n_defer);
     if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
+    if (n_walk) atomicAdd(&a.stats[7], (unsigned long long)n_walk);  // queue mode: queue_len
   }
   if (COUNT) {
     unsigned long long v = nv, t = nt;
@@ -739,8 +743,6 @@ __global__ __launch_bounds__(64) void k_vis_exact(VisArgs a) {
   }
 }
 
-// Crossing parity of the +x ray from each point with the triangles that do
-// not have the point as a vertex: 0/1, or 2 when a test is undecided.
 // Fan of each point: the leaf-ordered ids of the triangles that have the
 // point as a vertex (bitwise-equal coordinates), found by a point query on
 // the 4-wide BVH; -1 when there are more than kFanMax.
@@ -804,8 +806,11 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
   fan_n[g] = (!done || cnt > kFanMax) ? -1 : cnt;
 }
 
-// In a batch, thread g handles global point g of the part whose point range
-// holds it; far is indexed by the global point.
+// Crossing parity (0/1) of the +x ray from each point, perturbed by
+// (0, e, e^2), with the triangles that do not have the point as a vertex;
+// tests the filter leaves open are decided exactly; 2 only if the walk
+// exceeds its node budget.  In a batch, thread g handles global point g of
+// the part whose point range holds it; far is indexed by the global point.
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restrict__ far) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -831,7 +836,6 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
   make_seg(s);
   int stk[64];
   int node = 0, sp = 0, cnt = 0;
-  bool unsure = false;
   for (int it = 0; it <= cap; ++it) {
     float bx[12];
     int c0, c1;
@@ -845,9 +849,11 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
         const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
                     c3[3] = {tr.c.x, tr.c.y, tr.c.z};
         if (!(same3(s.p, a3) || same3(s.p, b3) || same3(s.p, c3))) {
-          const int r = cross_fast(s, tr);
+          int r = cross_fast(s, tr);
+          // undecided by the filter: settle it exactly here, once per point,
+          // rather than deferring every pair that starts at this point
+          if (r == 2) r = ray_cross_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
           cnt += r == 1;
-          unsure |= r == 2;
         }
         h[t] = false;
       }
@@ -861,7 +867,7 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
       node = ch[1];
     } else {
       if (sp == 0) {
-        far[i] = unsure ? 2 : (uint8_t)(cnt & 1);
+        far[i] = (uint8_t)(cnt & 1);
         return;
       }
       node = stk[--sp];

@@ -41,7 +41,7 @@ class VisStats(ctypes.Structure):
     _fields_ = [("segments", ctypes.c_int64), ("masked", ctypes.c_int64),
                 ("blocked", ctypes.c_int64), ("edges", ctypes.c_int64),
                 ("exact_calls", ctypes.c_int64), ("node_visits", ctypes.c_int64),
-                ("tri_tests", ctypes.c_int64)]
+                ("tri_tests", ctypes.c_int64), ("walked", ctypes.c_int64)]
 
 
 class Part(ctypes.Structure):
