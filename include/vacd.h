@@ -51,6 +51,9 @@ typedef struct vacd_vis_stats {
 #define VACD_OPT_WIDE_BVH 3   /* 4-wide node walk vs binary (default 0)    */
 #define VACD_OPT_FAN_PAIRS 4  /* parity mode: settle pairs that leave p     */
                               /* into the solid from p's fan (default 1)    */
+#define VACD_OPT_SCORE_CHUNK 5 /* edges per scoring block, 0 = automatic    */
+#define VACD_OPT_LANES_PER_PAIR 6 /* lanes sharing one BVH walk of a pair   */
+                                  /* the fan leaves open: 1,2,4,8,16 (4)    */
 
 /* Accumulated device time of the main kernels since the last reset
  * (VACD_OPT_TIMING): the visibility traversal kernel, the plane-scoring

@@ -186,6 +186,14 @@ def test_visibility_binary_and_wide_walks_agree(ctx, name):
                 ctx.set_option(native.VACD_OPT_WIDE_BVH, 0)
                 ctx.set_option(native.VACD_OPT_FAN_PAIRS, 1)
             np.testing.assert_array_equal(g, o, err_msg=f"wide={wide} fan={fan}")
+    # the walks the fan leaves open, split over 1..16 lanes per pair
+    for lpp in (1, 2, 4, 8, 16):
+        ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, lpp)
+        try:
+            g = ctx.visibility(V, ctx.bvh(V, T))
+        finally:
+            ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 4)
+        np.testing.assert_array_equal(g, o, err_msg=f"lanes per pair {lpp}")
 
 
 def test_visibility_open_mesh(ctx):

@@ -26,7 +26,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kPPT = 8;                     // planes per lane
 constexpr int kPlanesPerBlock = kBlock * kPPT;
-constexpr int kEdgeQuantum = 256;           // edge chunks are multiples of this
+constexpr int kEdgeQuantum = 32;            // edge chunks are multiples of this
 constexpr float kEps = 1e-6f;
 
 struct alignas(16) EdgeRec {
@@ -187,9 +187,9 @@ void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P, const float
     EdgeRec *rec = recb.as<EdgeRec>();
     k_edge_prep<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(d_pts, n, d_edges, E, rec);
     const int64_t ptiles = (P + kPlanesPerBlock - 1) / kPlanesPerBlock;
-    // about 4096 blocks in total
+    // about 4096 blocks in total unless the context fixes the chunk length
     int64_t nchunks = std::max<int64_t>(1, 4096 / ptiles);
-    int64_t chunk = (E + nchunks - 1) / nchunks;
+    int64_t chunk = ctx->score_chunk > 0 ? ctx->score_chunk : (E + nchunks - 1) / nchunks;
     chunk = std::max<int64_t>(kEdgeQuantum, (chunk + kEdgeQuantum - 1) / kEdgeQuantum * kEdgeQuantum);
     nchunks = (E + chunk - 1) / chunk;
     VACD_REQUIRE(nchunks < 65536, VACD_ERR_LIMIT, "too many edge chunks");

@@ -124,6 +124,11 @@ int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value) {
     ctx->wide_bvh = value != 0;
   else if (option == VACD_OPT_FAN_PAIRS)
     ctx->fan_pairs = value != 0;
+  else if (option == VACD_OPT_SCORE_CHUNK && value >= 0 && value <= (int64_t(1) << 30))
+    ctx->score_chunk = value;
+  else if (option == VACD_OPT_LANES_PER_PAIR &&
+           (value == 1 || value == 2 || value == 4 || value == 8 || value == 16))
+    ctx->lanes_per_pair_log2 = value == 1 ? 0 : value == 2 ? 1 : value == 4 ? 2 : value == 8 ? 3 : 4;
   else
     return fail(ctx, Error{VACD_ERR_INVALID, "unknown option"});
   return VACD_OK;

@@ -213,16 +213,54 @@ __device__ __forceinline__ void load_node(const Node *__restrict__ nodes, int id
   c1 = __float_as_int(nd.f3.y);
 }
 
+// A walk split over a group of 2^lv lanes: lane g starts at the node reached
+// from the root by the lv bits of g (most significant first), entering a
+// child only if its box passes the query; a leaf met above level lv belongs
+// to the lane whose remaining bits are zero.  Every leaf whose box chain
+// passes is thus reached by exactly one lane.  Returns an internal node id,
+// a leaf (~triangle) or kNoRoot.
+constexpr int kNoRoot = -2147483647 - 1;
+
+template <bool RAY>
+__device__ __forceinline__ int split_root(const Node *__restrict__ nodes, int g, int lv,
+                                          const Seg &s) {
+  int node = 0;
+  for (int l = lv - 1; l >= 0; --l) {
+    if (node < 0) return (g & ((2 << l) - 1)) ? kNoRoot : node;
+    float bx[12];
+    int c0, c1;
+    load_node(nodes, node, bx, c0, c1);
+    const int o = ((g >> l) & 1) ? 6 : 0;
+    const bool h = RAY ? ray_box(s, bx[o], bx[o + 1], bx[o + 2], bx[o + 3], bx[o + 4], bx[o + 5])
+                       : seg_box(s, bx[o], bx[o + 1], bx[o + 2], bx[o + 3], bx[o + 4], bx[o + 5]);
+    if (!h) return kNoRoot;
+    node = o ? c1 : c0;
+  }
+  return node;
+}
+
 // Any contact along the segment: 1 contact, 0 none, 2 none certain but some
-// test undecided.  stk: this lane's LDS stack, stride 64 entries.
+// test undecided, 3 malformed tree.  stk: this lane's LDS stack, stride 64
+// entries.  root: where the walk starts (a split walk's share); stop: a
+// group flag in LDS, set on a contact and polled so the group's other lanes
+// end their share early.
 template <int STACK, bool COUNT>
 __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
                                         const Seg &s, int *stk, int cap, unsigned &nv,
-                                        unsigned &nt, int &xpar, int &ninc) {
-  int node = 0, sp = 0;
+                                        unsigned &nt, int &xpar, int &ninc, int root = 0,
+                                        volatile int *stop = nullptr) {
+  if (root < 0) {
+    if (root == kNoRoot) return 0;
+    if (COUNT) ++nt;
+    const int r = contact_fast(s, tris[~root], xpar, ninc);
+    if (r == 1 && stop) *stop = 1;
+    return r;
+  }
+  int node = root, sp = 0;
   bool unsure = false;
   for (int it = 0;; ++it) {
     if (it > cap) return 3;  // visited more nodes than the tree has: malformed tree
+    if (stop && *stop) return 0;  // another lane of the group found a contact
     float bx[12];
     int c0, c1;
     load_node(nodes, node, bx, c0, c1);
@@ -232,13 +270,19 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
     if (COUNT) nt += (h0 && c0 < 0) + (h1 && c1 < 0);
     if (h0 && c0 < 0) {
       const int r = contact_fast(s, tris[~c0], xpar, ninc);
-      if (r == 1) return 1;
+      if (r == 1) {
+        if (stop) *stop = 1;
+        return 1;
+      }
       unsure |= r == 2;
       h0 = false;
     }
     if (h1 && c1 < 0) {
       const int r = contact_fast(s, tris[~c1], xpar, ninc);
-      if (r == 1) return 1;
+      if (r == 1) {
+        if (stop) *stop = 1;
+        return 1;
+      }
       unsure |= r == 2;
       h1 = false;
     }
@@ -257,12 +301,18 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
   return unsure ? 2 : 0;
 }
 
-// Crossing parity of the midpoint ray: 0 even, 1 odd, 2 undecided.
+// Crossing parity of the midpoint ray: 0 even, 1 odd, 2 undecided, 3
+// malformed tree.  root as for walk_any.
 template <int STACK, bool COUNT>
 __device__ __forceinline__ int walk_parity(const Node *__restrict__ nodes,
                                            const Tri *__restrict__ tris, const Seg &s, int *stk,
-                                           int cap, unsigned &nv, unsigned &nt) {
-  int node = 0, sp = 0, cnt = 0;
+                                           int cap, unsigned &nv, unsigned &nt, int root = 0) {
+  if (root < 0) {
+    if (root == kNoRoot) return 0;
+    if (COUNT) ++nt;
+    return cross_fast(s, tris[~root]);
+  }
+  int node = root, sp = 0, cnt = 0;
   bool unsure = false;
   for (int it = 0;; ++it) {
     if (it > cap) return 3;
@@ -584,18 +634,32 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   }
 }
 
-// Dense walks over the queue of pairs the fan pass left undecided: one lane
-// per queued pair; results are OR-ed into the ballot words (the fan pass
-// wrote zero for these bits).
-template <int STACK, bool COUNT, bool WIDE, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_walk_queue(VisArgs a) {
+// Dense walks over the queue of pairs the fan pass left undecided, 2^LV
+// lanes per pair: each lane walks its share of the tree (split_root), the
+// group combines its lanes' results through ballots (any contact: OR; fan
+// crossings and ray crossings: XOR), and the group's first lane OR-s the
+// outcome into the ballot words (the fan pass wrote zero for these bits).
+// More lanes per pair shorten the longest walk of a wave, which bounds the
+// kernel, at the price of box tests near the root done by several lanes.
+__device__ __forceinline__ uint64_t group_mask(int lane, int G) {
+  return G == 64 ? ~0ull : (((1ull << G) - 1ull) << (lane & ~(G - 1)));
+}
+
+template <int STACK, bool COUNT, bool BATCH, int LV>
+__global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
+  constexpr int G = 1 << LV;
+  constexpr int kGroups = kBlock / G;
   __shared__ int stk_all[kWaves][STACK][64];
+  __shared__ int stop_all[kGroups];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int *stk = &stk_all[wave][0][lane];
+  const int g = lane & (G - 1);
+  const int slot = threadIdx.x / G;
+  const uint64_t gm = group_mask(lane, G);
   unsigned n_blocked = 0, n_defer = 0, n_bad = 0, nv = 0, nt = 0;
   const int64_t qn = (int64_t)*a.queue_len;
-  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < qn;
-       e += (int64_t)gridDim.x * kBlock) {
+  for (int64_t e = (int64_t)blockIdx.x * kGroups + slot; e < qn;
+       e += (int64_t)gridDim.x * kGroups) {
     const int64_t kg = a.queue[e];
     const int64_t w = kg >> 6;
     const int b = (int)(kg & 63);
@@ -605,12 +669,48 @@ __global__ __launch_bounds__(kBlock) void k_walk_queue(VisArgs a) {
     Seg s;
 #pragma unroll
     for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
-    bool edge = false, defer = false;
-    walk_pair<STACK, false, COUNT, WIDE>(P, i, s, stk, edge, defer, n_blocked, n_bad, nv, nt);
-    if (edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
-    if (defer) {
-      atomicOr(reinterpret_cast<unsigned long long *>(&a.defer[w]), 1ull << b);
-      ++n_defer;
+    make_seg(s);
+    volatile int *stop = &stop_all[slot];
+    if (g == 0) *stop = 0;
+    int xpar = 0, ninc = 0;
+    const int root = split_root<false>(P.snodes, g, LV, s);
+    const int hs = walk_any<STACK, COUNT>(P.snodes, P.stris, s, stk, P.scap, nv, nt, xpar, ninc,
+                                          root, G > 1 ? stop : nullptr);
+    const uint64_t m3 = __ballot(hs == 3) & gm, m1 = __ballot(hs == 1) & gm,
+                   m2 = __ballot(hs == 2) & gm, mx = __ballot(xpar & 1) & gm,
+                   mn = __ballot(ninc > 0) & gm;
+    bool edge = false, defer = false, blocked = false, bad = false;
+    if (m3) {
+      bad = true;
+    } else if (m1) {
+      blocked = true;
+    } else if (m2) {
+      defer = true;
+    } else if (P.far && mn) {
+      // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
+      const uint8_t fp = P.far[i];
+      if (fp > 1)
+        defer = true;
+      else
+        edge = (((__popcll(mx) & 1) ^ fp) & 1) == 0;
+    } else {  // group-uniform branch: the midpoint ray, split the same way
+      const int rr = split_root<true>(P.snodes, g, LV, s);
+      const int par = walk_parity<STACK, COUNT>(P.snodes, P.stris, s, stk, P.scap, nv, nt, rr);
+      const uint64_t p3 = __ballot(par == 3) & gm, p2 = __ballot(par == 2) & gm,
+                     p1 = __ballot(par == 1) & gm;
+      if (p3)
+        bad = true;
+      else if (p2)
+        defer = true;
+      else
+        edge = (__popcll(p1) & 1) == 0;
+    }
+    if (g == 0) {
+      if (edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
+      if (defer) atomicOr(reinterpret_cast<unsigned long long *>(&a.defer[w]), 1ull << b);
+      n_blocked += blocked;
+      n_defer += defer;
+      n_bad += bad;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -632,11 +732,35 @@ __global__ __launch_bounds__(kBlock) void k_walk_queue(VisArgs a) {
 }
 
 // ------------------------------------------------------------ exact redo
+// Exact walks from a split root: filtered tests first, exact predicates
+// where the filter cannot decide.  stk: LDS, stride 64.
 __device__ bool exact_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
-                          const Seg &s, int cap) {
-  int stk[64];
-  int node = 0, sp = 0;
-  for (int it = 0; it <= cap; ++it) {
+                          const Seg &s, int root, int *stk, int cap, volatile int *stop,
+                          bool &bad) {
+  auto leaf = [&](int id) {
+    const Tri tr = tris[id];
+    int xp = 0, ni = 0;
+    int r = contact_fast(s, tr, xp, ni);
+    if (r == 2) {
+      const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
+                  c3[3] = {tr.c.x, tr.c.y, tr.c.z};
+      r = contact_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
+    }
+    return r == 1;
+  };
+  if (root == kNoRoot) return false;
+  if (root < 0) {
+    const bool hit = leaf(~root);
+    if (hit) *stop = 1;
+    return hit;
+  }
+  int node = root, sp = 0;
+  for (int it = 0;; ++it) {
+    if (it > cap) {
+      bad = true;
+      return false;
+    }
+    if (*stop) return false;
     float bx[12];
     int c0, c1;
     load_node(nodes, node, bx, c0, c1);
@@ -645,38 +769,50 @@ __device__ bool exact_any(const Node *__restrict__ nodes, const Tri *__restrict_
                  seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11])};
     for (int t = 0; t < 2; ++t) {
       if (h[t] && ch[t] < 0) {
-        const Tri tr = tris[~ch[t]];
-        int xp = 0, ni = 0;
-        int r = contact_fast(s, tr, xp, ni);
-        if (r == 2) {
-          const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
-                      c3[3] = {tr.c.x, tr.c.y, tr.c.z};
-          r = contact_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
+        if (leaf(~ch[t])) {
+          *stop = 1;
+          return true;
         }
-        if (r == 1) return true;
         h[t] = false;
       }
     }
     if (h[0] && h[1]) {
-      stk[sp++] = ch[1];
+      stk[(sp++) * 64] = ch[1];
       node = ch[0];
     } else if (h[0]) {
       node = ch[0];
     } else if (h[1]) {
       node = ch[1];
     } else {
-      if (sp == 0) break;
-      node = stk[--sp];
+      if (sp == 0) return false;
+      node = stk[(--sp) * 64];
     }
   }
-  return false;
 }
 
-__device__ int exact_parity(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
-                            const Seg &s, int cap) {
-  int stk[64];
-  int node = 0, sp = 0, cnt = 0;
-  for (int it = 0; it <= cap; ++it) {
+// Crossings (mod 2) of the perturbed +x ray from the segment's midpoint with
+// the triangles of this lane's share; triangles having `skip` (when not
+// NULL) as a vertex are left out.
+__device__ int exact_ray_parity(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
+                                const Seg &s, int root, int *stk, int cap, const float *skip,
+                                bool &bad) {
+  auto leaf = [&](int id) {
+    const Tri tr = tris[id];
+    const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
+                c3[3] = {tr.c.x, tr.c.y, tr.c.z};
+    if (skip && (same3(skip, a3) || same3(skip, b3) || same3(skip, c3))) return 0;
+    int r = cross_fast(s, tr);
+    if (r == 2) r = ray_cross_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
+    return r;
+  };
+  if (root == kNoRoot) return 0;
+  if (root < 0) return leaf(~root);
+  int node = root, sp = 0, cnt = 0;
+  for (int it = 0;; ++it) {
+    if (it > cap) {
+      bad = true;
+      return 0;
+    }
     float bx[12];
     int c0, c1;
     load_node(nodes, node, bx, c0, c1);
@@ -685,61 +821,83 @@ __device__ int exact_parity(const Node *__restrict__ nodes, const Tri *__restric
                  ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11])};
     for (int t = 0; t < 2; ++t) {
       if (h[t] && ch[t] < 0) {
-        const Tri tr = tris[~ch[t]];
-        int r = cross_fast(s, tr);
-        if (r == 2) {
-          const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
-                      c3[3] = {tr.c.x, tr.c.y, tr.c.z};
-          r = ray_cross_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
-        }
-        cnt += r;
+        cnt += leaf(~ch[t]);
         h[t] = false;
       }
     }
     if (h[0] && h[1]) {
-      stk[sp++] = ch[1];
+      stk[(sp++) * 64] = ch[1];
       node = ch[0];
     } else if (h[0]) {
       node = ch[0];
     } else if (h[1]) {
       node = ch[1];
     } else {
-      if (sp == 0) break;
-      node = stk[--sp];
+      if (sp == 0) return cnt & 1;
+      node = stk[(--sp) * 64];
     }
   }
-  return cnt & 1;
 }
 
-// One wave per ballot word, one lane per deferred pair of that word.
+// Every deferred pair is redone with exact predicates by a whole wave: the
+// 64 lanes split the walks (split_root at level 6), any contact ends them
+// all, crossing parities are combined by xor.  Midpoint parity by
+// definition.  A pair the exact walk cannot finish (malformed tree) counts
+// in the malformed statistic.
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_vis_exact(VisArgs a) {
+  __shared__ int stk_all[64][64];
+  __shared__ int stop_flag;
   const int lane = threadIdx.x;
+  int *stk = &stk_all[0][lane];
+  volatile int *stop = &stop_flag;
+  unsigned n_blocked = 0, n_bad = 0;
   for (int64_t w = blockIdx.x; w < a.nwords; w += gridDim.x) {
-    const uint64_t dm = a.defer[w];
+    uint64_t dm = a.defer[w];
     if (!dm) continue;
     const Part P = part_of_word<BATCH>(a, w);
-    bool edge = false, blocked = false;
-    if ((dm >> lane) & 1ull) {
+    uint64_t eb = 0;
+    while (dm) {
+      const int b = __ffsll((unsigned long long)dm) - 1;
+      dm &= dm - 1;
       int64_t i, j;
-      pair_of((w - P.word_begin) * 64 + lane, P.n, i, j);
+      pair_of((w - P.word_begin) * 64 + b, P.n, i, j);
       Seg s;
       for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
       make_seg(s);
-      if (exact_any(P.snodes, P.stris, s, P.scap)) {
-        blocked = true;
+      if (lane == 0) *stop = 0;
+      bool bad = false;
+      const int root = split_root<false>(P.snodes, lane, 6, s);
+      exact_any(P.snodes, P.stris, s, root, stk, P.scap, stop, bad);
+      bool edge = false;
+      if (__ballot(bad)) {
+        ++n_bad;
+      } else if (*stop) {
+        ++n_blocked;
       } else if (P.ctris) {
-        edge = exact_any(P.cnodes, P.ctris, s, P.ccap);
+        if (lane == 0) *stop = 0;
+        const int rc = split_root<false>(P.cnodes, lane, 6, s);
+        exact_any(P.cnodes, P.ctris, s, rc, stk, P.ccap, stop, bad);
+        if (__ballot(bad))
+          ++n_bad;
+        else
+          edge = *stop != 0;
       } else {
-        edge = exact_parity(P.snodes, P.stris, s, P.scap) == 0;
+        const int rr = split_root<true>(P.snodes, lane, 6, s);
+        const int par = exact_ray_parity(P.snodes, P.stris, s, rr, stk, P.scap, nullptr, bad);
+        const uint64_t odd = __ballot(par & 1);
+        if (__ballot(bad))
+          ++n_bad;
+        else
+          edge = (__popcll(odd) & 1) == 0;
       }
+      if (edge) eb |= 1ull << b;
     }
-    const uint64_t eb = __ballot(edge);
-    const uint64_t bb = __ballot(blocked);
-    if (lane == 0) {
-      a.bits[w] |= eb;
-      if (bb) atomicAdd(&a.stats[1], (unsigned long long)__popcll(bb));
-    }
+    if (lane == 0 && eb) a.bits[w] |= eb;
+  }
+  if (lane == 0) {
+    if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
+    if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
   }
 }
 
@@ -808,72 +966,37 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
 
 // Crossing parity (0/1) of the +x ray from each point, perturbed by
 // (0, e, e^2), with the triangles that do not have the point as a vertex;
-// tests the filter leaves open are decided exactly; 2 only if the walk
-// exceeds its node budget.  In a batch, thread g handles global point g of
-// the part whose point range holds it; far is indexed by the global point.
+// tests the filter leaves open are decided exactly; 2 only if a walk
+// exceeds its node budget.  One wave per point, the 64 lanes splitting the
+// walk (split_root at level 6).  In a batch, far is indexed by the global
+// point.
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restrict__ far) {
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= a.npoints) return;
-  Part P = a.one;
-  if (BATCH) {
-    int lo = 0, hi = a.nparts - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.parts[mid].point_begin <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    P = a.parts[lo];
-  }
-  if (!P.far) return;  // open mesh or cage mode: not needed
-  const int64_t i = g - P.point_begin;
-  const Node *__restrict__ nodes = P.snodes;
-  const Tri *__restrict__ tris = P.stris;
-  const int cap = P.scap;
-  far += P.point_begin;
-  Seg s;
-  for (int c = 0; c < 3; ++c) s.p[c] = s.q[c] = P.pts[3 * i + c];
-  make_seg(s);
-  int stk[64];
-  int node = 0, sp = 0, cnt = 0;
-  for (int it = 0; it <= cap; ++it) {
-    float bx[12];
-    int c0, c1;
-    load_node(nodes, node, bx, c0, c1);
-    int ch[2] = {c0, c1};
-    bool h[2] = {ray_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]),
-                 ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11])};
-    for (int t = 0; t < 2; ++t) {
-      if (h[t] && ch[t] < 0) {
-        const Tri tr = tris[~ch[t]];
-        const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
-                    c3[3] = {tr.c.x, tr.c.y, tr.c.z};
-        if (!(same3(s.p, a3) || same3(s.p, b3) || same3(s.p, c3))) {
-          int r = cross_fast(s, tr);
-          // undecided by the filter: settle it exactly here, once per point,
-          // rather than deferring every pair that starts at this point
-          if (r == 2) r = ray_cross_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
-          cnt += r == 1;
-        }
-        h[t] = false;
+  __shared__ int stk_all[64][64];
+  const int lane = threadIdx.x;
+  int *stk = &stk_all[0][lane];
+  for (int64_t g = blockIdx.x; g < a.npoints; g += gridDim.x) {
+    Part P = a.one;
+    if (BATCH) {
+      int lo = 0, hi = a.nparts - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.parts[mid].point_begin <= g) lo = mid;
+        else hi = mid - 1;
       }
+      P = a.parts[lo];
     }
-    if (h[0] && h[1]) {
-      stk[sp++] = ch[1];
-      node = ch[0];
-    } else if (h[0]) {
-      node = ch[0];
-    } else if (h[1]) {
-      node = ch[1];
-    } else {
-      if (sp == 0) {
-        far[i] = (uint8_t)(cnt & 1);
-        return;
-      }
-      node = stk[--sp];
-    }
+    if (!P.far) continue;  // open mesh or cage mode: not needed
+    const int64_t i = g - P.point_begin;
+    Seg s;
+    for (int c = 0; c < 3; ++c) s.p[c] = s.q[c] = P.pts[3 * i + c];
+    make_seg(s);
+    bool bad = false;
+    const int root = split_root<true>(P.snodes, lane, 6, s);
+    const int par = exact_ray_parity(P.snodes, P.stris, s, root, stk, P.scap, s.p, bad);
+    const uint64_t odd = __ballot(par & 1), mb = __ballot(bad);
+    if (lane == 0) far[g] = mb ? 2 : (uint8_t)(__popcll(odd) & 1);
   }
-  far[i] = 2;  // node budget exceeded
 }
 
 // ------------------------------------------------------------ compaction
@@ -1019,8 +1142,20 @@ __global__ void k_fix_part_offsets(VisArgs a, const int64_t *__restrict__ total,
 
 struct Launch {
   unsigned grid, qgrid;
+  int lv;  // log2 of the lanes per queued pair
   hipStream_t st;
 };
+
+template <int S, bool N, bool B>
+void launch_split(const Launch &L, const VisArgs &a) {
+  switch (L.lv) {
+    case 0: k_walk_split<S, N, B, 0><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+    case 1: k_walk_split<S, N, B, 1><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+    case 2: k_walk_split<S, N, B, 2><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+    case 3: k_walk_split<S, N, B, 3><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+    default: k_walk_split<S, N, B, 4><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+  }
+}
 
 // The fan pass queues what it cannot settle (no walk stack needed there);
 // the queue is then walked densely.
@@ -1028,7 +1163,7 @@ template <int S, bool C, bool N, bool W, bool B>
 void launch_main(bool q, const Launch &L, const VisArgs &a) {
   if (q && !C) {
     k_visibility<1, C, N, W, B, true><<<L.grid, kBlock, 0, L.st>>>(a);  // no walk, no stack
-    k_walk_queue<S, N, W, B><<<L.qgrid, kBlock, 0, L.st>>>(a);
+    launch_split<S, N, B>(L, a);
   } else {
     k_visibility<S, C, N, W, B, false><<<L.grid, kBlock, 0, L.st>>>(a);
   }
@@ -1092,13 +1227,15 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   scan.ensure(8 * (size_t)nscan);
   a.bits = bits.as<uint64_t>();
   a.defer = defer.as<uint64_t>();
+  VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
   if (any_far) {
     const unsigned fg = (unsigned)((a.npoints + 63) / 64);
+    const unsigned pg = (unsigned)std::min<int64_t>(a.npoints, 16384);  // a wave per point
     vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
     if (batch)
-      k_far_parity<true><<<fg, 64, 0, st>>>(a, farb.as<uint8_t>());
+      k_far_parity<true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
     else
-      k_far_parity<false><<<fg, 64, 0, st>>>(a, farb.as<uint8_t>());
+      k_far_parity<false><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
     VACD_HIP(hipGetLastError());
     if (ctx->fan_pairs) {
       vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
@@ -1111,7 +1248,6 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
       VACD_HIP(hipGetLastError());
     }
   }
-  VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
   // stack entries a walk can hold: binary, one pending sibling per level;
   // 4-wide, up to three per visited level (every other binary level)
   const int need2 = depth - 1, need4 = 3 * (depth / 2);
@@ -1120,7 +1256,10 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
   Launch L;
   L.grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
-  L.qgrid = (unsigned)std::min<int64_t>((nwords * 64 + kBlock - 1) / kBlock, 256 * 16);
+  L.lv = ctx->lanes_per_pair_log2;
+  // the queue length is known on the device only: enough groups to cover
+  // every pair once when all are queued, at most 16 blocks per CU
+  L.qgrid = (unsigned)std::min<int64_t>(((nwords * 64) << L.lv) / kBlock + 1, 256 * 16);
   L.st = st;
   // parity mode with fans: queue the pairs the fan pass leaves open
   const bool queue = any_far && ctx->fan_pairs && !any_cage && need4 <= 64;

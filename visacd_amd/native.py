@@ -59,6 +59,8 @@ VACD_OPT_TIMING = 1
 VACD_OPT_COUNT_WORK = 2
 VACD_OPT_WIDE_BVH = 3
 VACD_OPT_FAN_PAIRS = 4
+VACD_OPT_SCORE_CHUNK = 5
+VACD_OPT_LANES_PER_PAIR = 6
 
 
 _lib = None
