@@ -181,7 +181,7 @@ def main():
             vis_step()
         tl = ctx.timing(reset=True)
         lpp_ms[lpp] = tl["vis_ms"] / max(1, tl["vis_launches"])
-    ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 4)
+    ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
     log("lanes per pair -> kernel ms", lpp_ms)
     # ... and with every pair walked (no fan shortcut for pairs starting inside)
     ctx.set_option(native.VACD_OPT_FAN_PAIRS, 0)

@@ -53,7 +53,7 @@ typedef struct vacd_vis_stats {
                               /* into the solid from p's fan (default 1)    */
 #define VACD_OPT_SCORE_CHUNK 5 /* edges per scoring block, 0 = automatic    */
 #define VACD_OPT_LANES_PER_PAIR 6 /* lanes sharing one BVH walk of a pair   */
-                                  /* the fan leaves open: 1,2,4,8,16 (4)    */
+                                  /* the fan leaves open: 1,2,4,8,16 (1)    */
 
 /* Accumulated device time of the main kernels since the last reset
  * (VACD_OPT_TIMING): the visibility traversal kernel, the plane-scoring

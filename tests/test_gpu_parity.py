@@ -192,7 +192,7 @@ def test_visibility_binary_and_wide_walks_agree(ctx, name):
         try:
             g = ctx.visibility(V, ctx.bvh(V, T))
         finally:
-            ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 4)
+            ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
         np.testing.assert_array_equal(g, o, err_msg=f"lanes per pair {lpp}")
 
 
@@ -219,10 +219,13 @@ def test_visibility_points_off_surface(ctx):
     np.testing.assert_array_equal(g, o)
 
 
-@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("seed", [0, 1, 2, 3, 4, 5])
 def test_visibility_degenerate_polycube(ctx, seed):
+    """Axis-aligned meshes: rays from vertices run along grid lines and meet
+    far edges exactly, so far parity must defer rather than guess."""
     rng = np.random.default_rng(seed)
-    cells = {(x, y, z) for x in range(4) for y in range(4) for z in range(2)
+    nz = 2 if seed < 2 else 3
+    cells = {(x, y, z) for x in range(4) for y in range(4) for z in range(nz)
              if rng.uniform() < 0.7}
     cells |= {(0, 0, 0)}
     # keep a face-connected, edge-manifold set: drop cells touching others only by an edge

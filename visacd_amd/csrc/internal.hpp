@@ -128,7 +128,7 @@ struct vacd_ctx {
   bool wide_bvh = false;  // 4-wide node walk (measured slower than binary on gfx950)
   bool fan_pairs = true;  // decide pairs that start inside from the fan of p
   int64_t score_chunk = 0;  // edges per scoring block (0: sized for ~4096 blocks)
-  int lanes_per_pair_log2 = 2;  // queued-pair walks split over 2^k lanes (k <= 4)
+  int lanes_per_pair_log2 = 0;  // queued-pair walks split over 2^k lanes (k <= 4)
   // recorded (start, stop) event pairs per timed kernel kind
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
 };

@@ -295,4 +295,29 @@ __device__ __noinline__ bool ray_cross_exact(const float *p, const float *q, con
   return orient3d_mid_exact(t0, t1, t2, p, q) == s1;
 }
 
+// Unperturbed +x ray from p against a triangle, exactly: 1 crossing, 0 none.
+// Sets degenerate (and returns 0) when the answer would depend on how the
+// ray is perturbed: p projects onto the triangle's boundary in (y, z) while
+// the other edge tests do not exclude it, or p lies in the triangle's plane
+// while projecting inside it.
+__device__ __noinline__ int ray_cross_point_exact(const float *p, const float *t0,
+                                                  const float *t1, const float *t2,
+                                                  bool &degenerate) {
+  const int s1 = orient2d_mid_exact(t0, t1, p, p, 0);
+  const int s2 = orient2d_mid_exact(t1, t2, p, p, 0);
+  const int s3 = orient2d_mid_exact(t2, t0, p, p, 0);
+  const bool pos = s1 > 0 || s2 > 0 || s3 > 0, neg = s1 < 0 || s2 < 0 || s3 < 0;
+  if (pos && neg) return 0;  // outside in projection under any perturbation
+  if (s1 == 0 || s2 == 0 || s3 == 0) {
+    degenerate = true;
+    return 0;
+  }
+  const int o = orient3d_mid_exact(t0, t1, t2, p, p);
+  if (o == 0) {
+    degenerate = true;
+    return 0;
+  }
+  return o == s1 ? 1 : 0;
+}
+
 }  // namespace vacd

@@ -464,11 +464,13 @@ struct VisArgs {
   int nparts;
   int64_t nwords, npoints;
   uint64_t *bits;
-  uint64_t *defer;
   unsigned long long *stats;  // [0] masked, [1] blocked, [2] deferred, [3] edges, [4] malformed,
-                              // [5] node visits, [6] triangle tests, [7] walk-queue length
+                              // [5] node visits, [6] triangle tests, [7] walk-queue length,
+                              // [8] deferred-list length
   int64_t *queue;             // global pair ids (word * 64 + bit) left to walk
   unsigned long long *queue_len;
+  int64_t *dlist;             // global pair ids deferred to the exact redo
+  unsigned long long *dlist_len;
 };
 
 // Part owning global ballot word w (binary search over word_begin).
@@ -539,6 +541,19 @@ __device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int 
   }
 }
 
+// Appends `value` of every lane with `mine` set to a list, in lane order,
+// with one atomic per wave.
+__device__ __forceinline__ void append_lanes(bool mine, int64_t value, int64_t *list,
+                                             unsigned long long *len) {
+  const uint64_t m = __ballot(mine);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(len, (unsigned long long)__popcll(m));
+  base = __shfl(base, 0);
+  if (mine) list[base + __popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull))] = value;
+}
+
 template <int STACK, bool CAGE, bool COUNT, bool WIDE, bool BATCH, bool QUEUE>
 __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   __shared__ int stk_all[kWaves][STACK][64];
@@ -587,24 +602,12 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
         }
       }
     }
-    const uint64_t eb = __ballot(edge), db = __ballot(defer);
+    const uint64_t eb = __ballot(edge);
     n_defer += defer;
-    if (lane == 0) {
-      a.bits[w] = eb;
-      a.defer[w] = db;
-    }
-    if (QUEUE) {  // append this wave's undecided pairs (global k) to the walk queue
-      const uint64_t qb = __ballot(queued);
-      if (qb) {
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(a.queue_len, (unsigned long long)__popcll(qb));
-        base = __shfl(base, 0);
-        if (queued) {
-          const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-          a.queue[base + __popcll(qb & below)] = w * 64 + lane;
-        }
-      }
-    }
+    if (lane == 0) a.bits[w] = eb;
+    append_lanes(defer, w * 64 + lane, a.dlist, a.dlist_len);
+    // append this wave's undecided pairs (global k) to the walk queue
+    if (QUEUE) append_lanes(queued, w * 64 + lane, a.queue, a.queue_len);
   }
   // per-wave totals, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
@@ -707,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
     }
     if (g == 0) {
       if (edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
-      if (defer) atomicOr(reinterpret_cast<unsigned long long *>(&a.defer[w]), 1ull << b);
+      if (defer) a.dlist[atomicAdd(a.dlist_len, 1ull)] = kg;
       n_blocked += blocked;
       n_defer += defer;
       n_bad += bad;
@@ -790,19 +793,25 @@ __device__ bool exact_any(const Node *__restrict__ nodes, const Tri *__restrict_
   }
 }
 
-// Crossings (mod 2) of the perturbed +x ray from the segment's midpoint with
-// the triangles of this lane's share; triangles having `skip` (when not
-// NULL) as a vertex are left out.
+// Crossings (mod 2) of the +x ray from the segment's midpoint with the
+// triangles of this lane's share.  Midpoint mode (skip NULL): the ray is
+// perturbed by (0, e, e^2), the definition of midpoint parity.  Point mode
+// (skip = the point, s.p == s.q): triangles having the point as a vertex are
+// left out and the ray is not perturbed; a test whose answer would depend
+// on a perturbation sets degenerate instead (for pairs starting at the
+// point, the pair's own direction would have to break that tie).
 __device__ int exact_ray_parity(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
                                 const Seg &s, int root, int *stk, int cap, const float *skip,
-                                bool &bad) {
+                                bool &bad, bool &degenerate) {
   auto leaf = [&](int id) {
     const Tri tr = tris[id];
     const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
                 c3[3] = {tr.c.x, tr.c.y, tr.c.z};
     if (skip && (same3(skip, a3) || same3(skip, b3) || same3(skip, c3))) return 0;
     int r = cross_fast(s, tr);
-    if (r == 2) r = ray_cross_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
+    if (r == 2)
+      r = skip ? ray_cross_point_exact(s.p, a3, b3, c3, degenerate)
+               : (ray_cross_exact(s.p, s.q, a3, b3, c3) ? 1 : 0);
     return r;
   };
   if (root == kNoRoot) return 0;
@@ -852,48 +861,46 @@ __global__ __launch_bounds__(64) void k_vis_exact(VisArgs a) {
   int *stk = &stk_all[0][lane];
   volatile int *stop = &stop_flag;
   unsigned n_blocked = 0, n_bad = 0;
-  for (int64_t w = blockIdx.x; w < a.nwords; w += gridDim.x) {
-    uint64_t dm = a.defer[w];
-    if (!dm) continue;
+  const int64_t len = (int64_t)*a.dlist_len;
+  for (int64_t e = blockIdx.x; e < len; e += gridDim.x) {
+    const int64_t kg = a.dlist[e];
+    const int64_t w = kg >> 6;
+    const int b = (int)(kg & 63);
     const Part P = part_of_word<BATCH>(a, w);
-    uint64_t eb = 0;
-    while (dm) {
-      const int b = __ffsll((unsigned long long)dm) - 1;
-      dm &= dm - 1;
-      int64_t i, j;
-      pair_of((w - P.word_begin) * 64 + b, P.n, i, j);
-      Seg s;
-      for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
-      make_seg(s);
+    int64_t i, j;
+    pair_of((w - P.word_begin) * 64 + b, P.n, i, j);
+    Seg s;
+    for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
+    make_seg(s);
+    if (lane == 0) *stop = 0;
+    bool bad = false;
+    const int root = split_root<false>(P.snodes, lane, 6, s);
+    exact_any(P.snodes, P.stris, s, root, stk, P.scap, stop, bad);
+    bool edge = false;
+    if (__ballot(bad)) {
+      ++n_bad;
+    } else if (*stop) {
+      ++n_blocked;
+    } else if (P.ctris) {
       if (lane == 0) *stop = 0;
-      bool bad = false;
-      const int root = split_root<false>(P.snodes, lane, 6, s);
-      exact_any(P.snodes, P.stris, s, root, stk, P.scap, stop, bad);
-      bool edge = false;
-      if (__ballot(bad)) {
+      const int rc = split_root<false>(P.cnodes, lane, 6, s);
+      exact_any(P.cnodes, P.ctris, s, rc, stk, P.ccap, stop, bad);
+      if (__ballot(bad))
         ++n_bad;
-      } else if (*stop) {
-        ++n_blocked;
-      } else if (P.ctris) {
-        if (lane == 0) *stop = 0;
-        const int rc = split_root<false>(P.cnodes, lane, 6, s);
-        exact_any(P.cnodes, P.ctris, s, rc, stk, P.ccap, stop, bad);
-        if (__ballot(bad))
-          ++n_bad;
-        else
-          edge = *stop != 0;
-      } else {
-        const int rr = split_root<true>(P.snodes, lane, 6, s);
-        const int par = exact_ray_parity(P.snodes, P.stris, s, rr, stk, P.scap, nullptr, bad);
-        const uint64_t odd = __ballot(par & 1);
-        if (__ballot(bad))
-          ++n_bad;
-        else
-          edge = (__popcll(odd) & 1) == 0;
-      }
-      if (edge) eb |= 1ull << b;
+      else
+        edge = *stop != 0;
+    } else {
+      const int rr = split_root<true>(P.snodes, lane, 6, s);
+      bool degenerate = false;  // not set in midpoint mode
+      const int par =
+          exact_ray_parity(P.snodes, P.stris, s, rr, stk, P.scap, nullptr, bad, degenerate);
+      const uint64_t odd = __ballot(par & 1);
+      if (__ballot(bad))
+        ++n_bad;
+      else
+        edge = (__popcll(odd) & 1) == 0;
     }
-    if (lane == 0 && eb) a.bits[w] |= eb;
+    if (lane == 0 && edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
   }
   if (lane == 0) {
     if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
@@ -964,12 +971,13 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
   fan_n[g] = (!done || cnt > kFanMax) ? -1 : cnt;
 }
 
-// Crossing parity (0/1) of the +x ray from each point, perturbed by
-// (0, e, e^2), with the triangles that do not have the point as a vertex;
-// tests the filter leaves open are decided exactly; 2 only if a walk
-// exceeds its node budget.  One wave per point, the 64 lanes splitting the
-// walk (split_root at level 6).  In a batch, far is indexed by the global
-// point.
+// Crossing parity (0/1) of the +x ray from each point with the triangles
+// that do not have the point as a vertex; tests the filter leaves open are
+// decided exactly.  2 when a test is degenerate (the ray meets a triangle's
+// boundary exactly: then the parity seen from p + e(q-p) depends on q, and
+// the point's walked pairs go to the exact midpoint redo) or a walk exceeds
+// its node budget.  One wave per point, the 64 lanes splitting the walk
+// (split_root at level 6).  In a batch, far is indexed by the global point.
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restrict__ far) {
   __shared__ int stk_all[64][64];
@@ -991,10 +999,11 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
     Seg s;
     for (int c = 0; c < 3; ++c) s.p[c] = s.q[c] = P.pts[3 * i + c];
     make_seg(s);
-    bool bad = false;
+    bool bad = false, degenerate = false;
     const int root = split_root<true>(P.snodes, lane, 6, s);
-    const int par = exact_ray_parity(P.snodes, P.stris, s, root, stk, P.scap, s.p, bad);
-    const uint64_t odd = __ballot(par & 1), mb = __ballot(bad);
+    const int par =
+        exact_ray_parity(P.snodes, P.stris, s, root, stk, P.scap, s.p, bad, degenerate);
+    const uint64_t odd = __ballot(par & 1), mb = __ballot(bad || degenerate);
     if (lane == 0) far[g] = mb ? 2 : (uint8_t)(__popcll(odd) & 1);
   }
 }
@@ -1219,14 +1228,15 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   hipStream_t st = ctx->stream;
   const int64_t nwords = a.nwords;
   vacd::DevBuf &bits = ctx->buf[SLOT_BITS];
-  vacd::DevBuf &defer = ctx->buf[SLOT_DEFER];
+  vacd::DevBuf &dlist = ctx->buf[SLOT_DEFER];
   vacd::DevBuf &scan = ctx->buf[SLOT_SCAN];
   bits.ensure(8 * (size_t)nwords);
-  defer.ensure(8 * (size_t)nwords);
+  dlist.ensure(8 * (size_t)nwords * 64);
   const int64_t nscan = (nwords + kScanTile - 1) / kScanTile;
   scan.ensure(8 * (size_t)nscan);
   a.bits = bits.as<uint64_t>();
-  a.defer = defer.as<uint64_t>();
+  a.dlist = dlist.as<int64_t>();
+  a.dlist_len = a.stats + 8;
   VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
   if (any_far) {
     const unsigned fg = (unsigned)((a.npoints + 63) / 64);
@@ -1277,7 +1287,7 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     pick_c<64>(any_cage, ctx->count_work, wide, batch, queue, L, a);
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_VIS);
-  const unsigned egrid = (unsigned)std::min<int64_t>(nwords, 4096);
+  const unsigned egrid = 2048;  // a wave per deferred pair, grid-stride over the list
   if (batch)
     k_vis_exact<true><<<egrid, 64, 0, st>>>(a);
   else
@@ -1297,8 +1307,8 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
 
 void reset_stats(vacd_ctx *ctx) {
   vacd::DevBuf &stats = ctx->buf[SLOT_STATS];
-  stats.ensure(8 * sizeof(unsigned long long));
-  VACD_HIP(hipMemsetAsync(stats.ptr, 0, 8 * sizeof(unsigned long long), ctx->stream));
+  stats.ensure(16 * sizeof(unsigned long long));
+  VACD_HIP(hipMemsetAsync(stats.ptr, 0, 16 * sizeof(unsigned long long), ctx->stream));
 }
 
 }  // namespace
