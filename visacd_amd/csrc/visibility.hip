@@ -525,14 +525,10 @@ __device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int 
     edge = hc == 1;
     defer = hc == 2;
     n_bad += hc == 3;
-  } else if (P.far && ninc > 0) {
+  } else if (P.far && ninc > 0 && P.far[i] <= 1) {
     // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
-    const uint8_t fp = P.far[i];
-    if (fp > 1)
-      defer = true;
-    else
-      edge = ((xpar ^ fp) & 1) == 0;
-  } else {
+    edge = ((xpar ^ P.far[i]) & 1) == 0;
+  } else {  // no local parity for this point: the midpoint ray itself
     const int par =
         ray_parity<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap, nv, nt);
     edge = par == 0;
@@ -689,13 +685,9 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
       blocked = true;
     } else if (m2) {
       defer = true;
-    } else if (P.far && mn) {
+    } else if (P.far && mn && P.far[i] <= 1) {
       // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
-      const uint8_t fp = P.far[i];
-      if (fp > 1)
-        defer = true;
-      else
-        edge = (((__popcll(mx) & 1) ^ fp) & 1) == 0;
+      edge = (((__popcll(mx) & 1) ^ P.far[i]) & 1) == 0;
     } else {  // group-uniform branch: the midpoint ray, split the same way
       const int rr = split_root<true>(P.snodes, g, LV, s);
       const int par = walk_parity<STACK, COUNT>(P.snodes, P.stris, s, stk, P.scap, nv, nt, rr);
@@ -975,8 +967,8 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
 // that do not have the point as a vertex; tests the filter leaves open are
 // decided exactly.  2 when a test is degenerate (the ray meets a triangle's
 // boundary exactly: then the parity seen from p + e(q-p) depends on q, and
-// the point's walked pairs go to the exact midpoint redo) or a walk exceeds
-// its node budget.  One wave per point, the 64 lanes splitting the walk
+// the point's walked pairs walk their midpoint rays instead) or a walk
+// exceeds its node budget.  One wave per point, the 64 lanes splitting the walk
 // (split_root at level 6).  In a batch, far is indexed by the global point.
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restrict__ far) {
