@@ -95,6 +95,96 @@ def acd_wall(W):
             "points_per_part": W.n_points, "planes_per_cut": int(W.planes.shape[0])}
 
 
+# kernels of one default visibility pass whose HBM traffic is reported
+TRAFFIC_KERNELS = ("k_visibility<1, false, false, false, false, true>",
+                   "k_walk_split<32, false, false, 0>", "k_walk_split<64, false, false, 0>")
+
+
+def pmc_traffic(workload):
+    """HBM bytes of one default visibility pass (fan pass + queue walk), from
+    two rocprofv3 --pmc runs of this script in probe mode (FETCH_SIZE, then
+    WRITE_SIZE), each scaled by what the counter reads for a 256 MiB copy.
+    Returns (bytes, detail) or (None, reason)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    vals = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(td, counter)
+            cmd = [prof, "--pmc", counter, "-f", "csv", "-d", d, "-o", "pmc", "--",
+                   sys.executable, "-u", os.path.abspath(__file__), "--pmc-probe",
+                   "--workload", workload]
+            env = dict(os.environ, TMPDIR="/tmp")
+            proc = subprocess.Popen(cmd, env=env, stdout=subprocess.DEVNULL,
+                                    stderr=subprocess.DEVNULL, start_new_session=True)
+            try:
+                rc = proc.wait(timeout=90)
+            except subprocess.TimeoutExpired:
+                os.killpg(proc.pid, 9)  # the profiler and the probe it started
+                proc.wait()
+                return None, f"rocprofv3 --pmc {counter} timed out"
+            # the profiler may exit non-zero after the program ended with its
+            # results written; a missing counter file is caught below
+            if rc != 0:
+                log(f"rocprofv3 --pmc {counter} exited with {rc}")
+            per = {"cal": 0.0, "pass": 0.0}
+            for root, _, files in os.walk(d):
+                for fn in files:
+                    if not fn.endswith("counter_collection.csv"):
+                        continue
+                    rows = [r for r in csv.DictReader(open(os.path.join(root, fn)))
+                            if r["Counter_Name"] == counter]
+                    # the last default pass of the probe: its last dispatch of each kernel
+                    last = {}
+                    for r in rows:
+                        name = r["Kernel_Name"]
+                        v = float(r["Counter_Value"])
+                        if "copyBuffer" in name:
+                            per["cal"] = max(per["cal"], v)
+                        for k in TRAFFIC_KERNELS:
+                            if k in name:
+                                last[k] = v
+                    per["pass"] = sum(last.values())
+            if per["cal"] <= 0 or per["pass"] <= 0:
+                return None, f"no counter data in the {counter} run"
+            vals[counter] = per["pass"] * (262144.0 / per["cal"]) * 1024.0  # KiB -> bytes
+    return vals["FETCH_SIZE"] + vals["WRITE_SIZE"], vals
+
+
+def pmc_probe(workload):
+    """Two default visibility passes and a 256 MiB calibration copy."""
+    import numpy as np
+    import torch
+
+    from visacd_amd import native
+
+    W = make_workload(workload)
+    dev = torch.device("cuda:0")
+    d_V = torch.from_numpy(W.vertices).to(dev)
+    d_T = torch.from_numpy(W.triangles.astype(np.int32)).to(dev)
+    d_pts = torch.from_numpy(W.points).to(dev)
+    ctx = native.Context(0)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    bvh = ctx.bvh(d_V, d_T)
+    d_pairs = torch.empty((W.n_segments, 2), dtype=torch.int32, device=dev)
+    d_count = torch.zeros(1, dtype=torch.int64, device=dev)
+    for _ in range(2):
+        ctx.visibility_d(d_pts, W.n_points, bvh, d_pairs=d_pairs, cap=W.n_segments,
+                         d_count=d_count)
+    cal = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+    cal2 = torch.empty_like(cal)
+    cal2.copy_(cal)
+    torch.cuda.synchronize()
+    bvh.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,7 +195,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--score-steps", type=int, default=10)
     ap.add_argument("--no-acd", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the HBM traffic runs")
+    ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_probe:
+        pmc_probe(args.workload)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "this engine has no multi-GPU path in this build.", flush=True)
@@ -271,6 +366,15 @@ def main():
     }
     bvh.close()
     ctx.close()
+    if not args.no_pmc:
+        traffic, detail = pmc_traffic(args.workload)
+        log("pmc traffic", traffic, detail)
+        line["roofline"]["traffic"] = traffic
+        if traffic is None:
+            line["roofline"]["traffic_note"] = detail
+        else:
+            line["roofline"]["traffic_unit"] = "bytes per visibility pass (fan pass + queue walk)"
+            line["roofline"]["alg_bytes_per_pass"] = bytes_per_seg * S
     if not args.no_acd:
         line["acd"] = acd_wall(W)
         log("acd", line["acd"])
