@@ -52,6 +52,12 @@ PYBIND11_MODULE(_visacd, m) {
       .def_readwrite("num_parts", &vhost::ProcessResult::num_parts);
 
   m.def("process", &vhost::process, py::arg("mesh"), py::arg("concavity"), py::arg("num_parts"));
+  m.def("last_profile", [] {
+    py::dict d;
+    for (auto &kv : vhost::last_profile.seconds)
+      d[py::str(kv.first)] = py::make_tuple(kv.second, vhost::last_profile.calls[kv.first]);
+    return d;
+  });
 
   // ---- this build's own knobs and host services
   py::class_<vhost::EngineOptions>(m, "EngineOptions")

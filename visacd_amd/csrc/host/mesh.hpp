@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <random>
+#include <map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -83,6 +84,21 @@ struct EngineOptions {
   bool verbose = true;
 };
 inline EngineOptions engine;
+
+// Wall time per phase of the last process() call, in seconds.
+struct Profile {
+  std::map<std::string, double> seconds;
+  std::map<std::string, long> calls;
+  void add(const std::string &phase, double s) {
+    seconds[phase] += s;
+    calls[phase] += 1;
+  }
+  void clear() {
+    seconds.clear();
+    calls.clear();
+  }
+};
+inline Profile last_profile;
 
 struct ProcessResult {
   MeshList parts;

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <iostream>
 #include <limits>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 
@@ -38,6 +39,16 @@ void log_line(const std::string &msg) {
   std::cout << "[visacd] " << msg << "\n";
   std::cout.flush();
 }
+
+// Adds the lifetime of the object to a phase of last_profile.
+struct PhaseTimer {
+  const char *phase;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit PhaseTimer(const char *p) : phase(p) {}
+  ~PhaseTimer() {
+    last_profile.add(phase, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  }
+};
 
 struct Engine {
   vacd_ctx *ctx = nullptr;
@@ -98,11 +109,26 @@ void choose_points(Mesh &part, bool inherited, uint32_t seed) {
   part.points = pts;
 }
 
+// Host staging buffer for visibility pairs, reused across calls (grown, never
+// zero-filled again).
+struct PairBuffer {
+  std::unique_ptr<uint32_t[]> data;
+  int64_t cap = 0;  // pairs
+  uint32_t *get(int64_t pairs) {
+    if (pairs > cap) {
+      data.reset(new uint32_t[2 * (size_t)pairs]);
+      cap = pairs;
+    }
+    return data.get();
+  }
+};
+
 // Visibility of a part in parity mode; with_mask restricts to pairs touching
 // a new vertex (lib/ptx/ray_segments.cu:61-75).  Edges are appended in k order
 // as pairs of part vertex ids.
-void add_visibility(const Engine &E, Mesh &part, bool with_mask) {
+void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf) {
   if (part.points.size() < 2) return;
+  PhaseTimer timer("visibility");
   std::vector<float> V = to_f32(part.vertices);
   std::vector<uint32_t> T(3 * part.triangles.size());
   for (size_t t = 0; t < part.triangles.size(); ++t)
@@ -125,13 +151,13 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask) {
     vacd_bvh_destroy(bvh);
     return;
   }
-  int64_t cap = std::min<int64_t>(n * (n - 1) / 2, int64_t(1) << 24);
-  std::vector<uint32_t> pairs;
+  int64_t cap = std::max<int64_t>(buf.cap, std::min<int64_t>(n * (n - 1) / 2, int64_t(1) << 22));
+  uint32_t *pairs = nullptr;
   int64_t count = 0;
   for (;;) {
-    pairs.resize(2 * (size_t)std::max<int64_t>(cap, 1));
+    pairs = buf.get(std::max<int64_t>(cap, 1));
     const int rc = vacd_visibility(E.ctx, P.data(), n, use_mask ? mask.data() : nullptr, bvh,
-                                   nullptr, pairs.data(), cap, &count);
+                                   nullptr, pairs, cap, &count);
     if (rc == VACD_ERR_CAPACITY) {
       cap = count;
       continue;
@@ -187,6 +213,7 @@ double gpu_hausdorff(const Engine &E, const Mesh &a, const Mesh &b, int resoluti
 }
 
 double part_concavity(const Engine &E, const Mesh &part) {
+  PhaseTimer timer("concavity");
   const Mesh hull = convex_hull(part.vertices);
   if (!engine.gpu_distance || hull.triangles.empty())
     return concavity(part, hull, engine.hausdorff_resolution, engine.sample_seed);
@@ -221,19 +248,33 @@ std::vector<Plane> bisector_planes(const Mesh &m, int num_planes) {
 }
 
 // Normals uniform on S^2 and offsets uniform over the part's extent along
-// each normal (the benchmark workloads' candidate sets).
+// each normal (the benchmark workloads' candidate sets).  The draws are taken
+// in order first (normal x, y, z, then the offset fraction, per plane), the
+// extents are then computed in parallel.
 std::vector<Plane> sphere_planes(const Mesh &m, int num_planes) {
-  std::vector<Plane> planes;
   std::normal_distribution<double> g(0.0, 1.0);
   std::uniform_real_distribution<double> u(0.0, 1.0);
+  std::vector<Vec3> normals;
+  std::vector<double> frac;
   for (int i = 0; i < num_planes; ++i) {
     Vec3 n{g(random_engine()), g(random_engine()), g(random_engine())};
     const double l = norm(n);
     if (l == 0) continue;
-    n = n * (1.0 / l);
+    normals.push_back(n * (1.0 / l));
+    frac.push_back(u(random_engine()));
+  }
+  std::vector<Vec3> pts(m.points.size());
+  for (size_t i = 0; i < pts.size(); ++i) pts[i] = m.vertices[m.points[i]];
+  std::vector<Plane> planes(normals.size());
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)normals.size(); ++i) {
+    const Vec3 &n = normals[i];
     double lo = 1e300, hi = -1e300;
-    for (int v : m.points) lo = std::min(lo, dot(n, m.vertices[v])), hi = std::max(hi, dot(n, m.vertices[v]));
-    planes.emplace_back(n[0], n[1], n[2], -(lo + u(random_engine()) * (hi - lo)));
+    for (const Vec3 &p : pts) {
+      const double d = dot(n, p);
+      lo = std::min(lo, d), hi = std::max(hi, d);
+    }
+    planes[i] = Plane(n[0], n[1], n[2], -(lo + frac[i] * (hi - lo)));
   }
   return planes;
 }
@@ -247,13 +288,18 @@ double edge_score(const Mesh &m) {
 struct Driver {
   Engine E;
   std::vector<Plane> flat;
+  PairBuffer pair_buf;
   uint32_t next_seed;
   explicit Driver(int device) : E(device), next_seed(engine.sample_seed) {}
 
   // returns true when nothing more can be split (no planes or no edges)
   bool decompose_iteration(Mesh &mesh, MeshList &parts) {
-    std::vector<Plane> planes = engine.planes == "sphere" ? sphere_planes(mesh, engine.num_planes)
-                                                          : bisector_planes(mesh, engine.num_planes);
+    std::vector<Plane> planes;
+    {
+      PhaseTimer timer("planes");
+      planes = engine.planes == "sphere" ? sphere_planes(mesh, engine.num_planes)
+                                         : bisector_planes(mesh, engine.num_planes);
+    }
     const size_t ss_offset = planes.size();
     planes.insert(planes.end(), flat.begin(), flat.end());
     if (planes.empty() || mesh.intersecting_edges.empty()) {
@@ -271,9 +317,12 @@ struct Driver {
       edges[2 * e] = mesh.intersecting_edges[e].first, edges[2 * e + 1] = mesh.intersecting_edges[e].second;
     std::vector<int64_t> q(planes.size());
     int32_t am = -1;
-    E.check(vacd_score_planes(E.ctx, hp.data(), (int64_t)planes.size(), pts.data(),
-                              (int64_t)mesh.vertices.size(), edges.data(),
-                              (int64_t)mesh.intersecting_edges.size(), q.data(), nullptr, &am));
+    {
+      PhaseTimer timer("scoring");
+      E.check(vacd_score_planes(E.ctx, hp.data(), (int64_t)planes.size(), pts.data(),
+                                (int64_t)mesh.vertices.size(), edges.data(),
+                                (int64_t)mesh.intersecting_edges.size(), q.data(), nullptr, &am));
+    }
     size_t best = 0;
     double best_s = -1.0;
     for (size_t i = 0; i < planes.size(); ++i) {
@@ -282,7 +331,11 @@ struct Driver {
       if (s > best_s) best_s = s, best = i;
     }
     std::vector<int> mp, mn;
-    MeshList halves = clip(mesh, planes[best], mp, mn);
+    MeshList halves;
+    {
+      PhaseTimer timer("clip");
+      halves = clip(mesh, planes[best], mp, mn);
+    }
     if (halves.size() < 2) {  // clipping failed
       parts.push_back(std::move(mesh));
       return false;
@@ -298,10 +351,15 @@ struct Driver {
       if (mn[v]) halves[1].points.push_back(mn[v] - 1);
     }
     for (auto &h : halves) {
-      for (Mesh &part : separate_disjoint(h)) {
+      MeshList pieces;
+      {
+        PhaseTimer timer("separate");
+        pieces = separate_disjoint(h);
+      }
+      for (Mesh &part : pieces) {
         if (part.vertices.size() < 10) continue;
         choose_points(part, true, next_seed++);
-        add_visibility(E, part, true);
+        add_visibility(E, part, true, pair_buf);
         parts.push_back(std::move(part));
       }
     }
@@ -312,6 +370,8 @@ struct Driver {
 }  // namespace
 
 ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
+  last_profile.clear();
+  PhaseTimer total("total");
   Driver D(engine.device);
   const std::vector<double> bbox = normalize(mesh);
   log_line("Preprocessing mesh (" + std::to_string(mesh.vertices.size()) + " verts)...");
@@ -325,7 +385,7 @@ ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
   MeshList parts = separate_disjoint(mesh);
   for (auto &part : parts) {
     choose_points(part, false, D.next_seed++);
-    add_visibility(D.E, part, false);
+    add_visibility(D.E, part, false, D.pair_buf);
   }
   std::ostringstream hdr;
   hdr << "Starting decomposition (max parts=" << num_parts << ", concavity threshold="
@@ -371,6 +431,7 @@ ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
     }
   }
   log_line("Computing convex hulls for " + std::to_string(parts.size()) + " parts...");
+  PhaseTimer final_timer("final_hulls");
   MeshList hulls;
   for (auto &p : parts) hulls.push_back(convex_hull(p.vertices));
   double final_h = 0.0;
