@@ -56,6 +56,8 @@ PYBIND11_MODULE(_visacd, m) {
     py::dict d;
     for (auto &kv : vhost::last_profile.seconds)
       d[py::str(kv.first)] = py::make_tuple(kv.second, vhost::last_profile.calls[kv.first]);
+    for (auto &kv : vhost::last_profile.calls)  // counters without a time
+      if (!vhost::last_profile.seconds.count(kv.first)) d[py::str(kv.first)] = kv.second;
     return d;
   });
 

@@ -134,9 +134,13 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
   for (size_t t = 0; t < part.triangles.size(); ++t)
     for (int k = 0; k < 3; ++k) T[3 * t + k] = (uint32_t)part.triangles[t][k];
   vacd_bvh *bvh = nullptr;
-  E.check(vacd_bvh_build(E.ctx, V.data(), (int64_t)part.vertices.size(), T.data(),
-                         (int64_t)part.triangles.size(), &bvh));
+  {
+    PhaseTimer t("visibility.bvh");
+    E.check(vacd_bvh_build(E.ctx, V.data(), (int64_t)part.vertices.size(), T.data(),
+                           (int64_t)part.triangles.size(), &bvh));
+  }
   const int64_t n = (int64_t)part.points.size();
+  last_profile.calls["visibility.points"] += n;
   std::vector<float> P(3 * n);
   std::vector<uint8_t> mask(n, 0);
   bool any_new = false;
@@ -154,6 +158,7 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
   int64_t cap = std::max<int64_t>(buf.cap, std::min<int64_t>(n * (n - 1) / 2, int64_t(1) << 22));
   uint32_t *pairs = nullptr;
   int64_t count = 0;
+  PhaseTimer call_timer("visibility.call");
   for (;;) {
     pairs = buf.get(std::max<int64_t>(cap, 1));
     const int rc = vacd_visibility(E.ctx, P.data(), n, use_mask ? mask.data() : nullptr, bvh,
@@ -169,6 +174,15 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
     break;
   }
   vacd_bvh_destroy(bvh);
+  vacd_vis_stats st;
+  if (vacd_visibility_stats(E.ctx, &st) == VACD_OK) {
+    last_profile.calls["visibility.segments"] += st.segments;
+    last_profile.calls["visibility.masked"] += st.masked;
+    last_profile.calls["visibility.walked"] += st.walked;
+    last_profile.calls["visibility.exact_pairs"] += st.exact_calls;
+  }
+  PhaseTimer append_timer("visibility.append");
+  last_profile.calls["visibility.edges"] += count;
   part.intersecting_edges.reserve(part.intersecting_edges.size() + (size_t)count);
   for (int64_t e = 0; e < count; ++e)
     part.intersecting_edges.push_back(

@@ -327,14 +327,14 @@ int vacd_visibility_stats(vacd_ctx *ctx, vacd_vis_stats *out) {
   if (!ctx || !out) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
   return guarded(ctx, [&] {
     set_device(ctx);
-    unsigned long long s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     VACD_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->buf[SLOT_STATS].ptr)
       VACD_HIP(hipMemcpy(s, ctx->buf[SLOT_STATS].ptr, sizeof(s), hipMemcpyDeviceToHost));
     out->segments = ctx->last_segments;
     out->masked = (int64_t)s[0];
     out->blocked = (int64_t)s[1];
-    out->exact_calls = (int64_t)s[2];
+    out->exact_calls = (int64_t)s[8];
     out->edges = (int64_t)s[3];
     out->node_visits = (int64_t)s[5];
     out->tri_tests = (int64_t)s[6];

@@ -4,23 +4,26 @@
 // the dense uint32 result matrix and the host combine loop of
 // compute_intersection_matrix (lib/src/intersections.cpp:327-385).
 //
-// k_visibility: one pair per lane, 64 consecutive pair ids k per wave
-//   (k = i(2n-i-1)/2 + j-i-1, so a wave's ballot word holds 64 results in k
-//   order).  Each lane walks the self BVH for any contact (short stack of
-//   node ids in LDS, lane-interleaved so the 64 lanes hit 64 distinct
-//   banks), then either walks the cage BVH for any contact (cage mode) or
-//   decides inside/outside (parity mode).  A segment that touches nothing
-//   lies wholly inside or outside, so the crossing parity of the +x ray from
-//   its midpoint equals that from p + e(q-p), e -> 0+.  On a closed mesh
-//   with p one of its vertices that parity is (crossings of p's own fan,
-//   tested during the self walk) xor (crossings of the +x ray from p with
-//   all other triangles, k_far_parity, once per point).  Otherwise the lane
-//   walks the midpoint ray itself.
-//   Triangle tests use float64 filters; a test the filter cannot decide is
-//   not guessed: the lane finishes its walk and, unless a certain contact
-//   blocked it, marks the pair deferred.
-// k_vis_exact: every deferred pair is redone with exact predicates (midpoint
-//   parity by definition).
+// Pairs are numbered k = i(2n-i-1)/2 + j-i-1; a wave owns 64 consecutive k,
+// so its ballot word holds 64 results in k order.  A segment that touches
+// nothing lies wholly inside or outside, so its midpoint parity equals the
+// parity at p + e(q-p), e -> 0+; on a closed mesh with p one of its vertices
+// that is (crossings of p's own fan) xor (crossings of the +x ray from p
+// with all other triangles).
+// k_far_parity: the second term, once per point (a wave per point).
+// k_fan: the triangles having each point as a vertex.
+// k_visibility: one pair per lane.  With fans (closed mesh, parity mode) it
+//   only tests p's fan: a pair leaving p into the solid is never an edge and
+//   is settled there; the rest are queued.  Otherwise (cage mode, open mesh)
+//   each lane walks its pair itself.
+// k_walk_split: the queued pairs, 2^k lanes per pair, any contact with the
+//   self BVH (short stack of node ids in LDS, lane-interleaved so the 64
+//   lanes hit 64 banks), then the local parity, or the midpoint ray when a
+//   fan crossing is degenerate or the point's far parity is.
+// Triangle tests use float64 filters; a pair whose walk meets a test the
+// filter cannot decide is appended to a deferred list, and k_walk_exact
+// redoes those pairs one per lane with exact predicates (expansion
+// arithmetic) in place of the undecided tests.
 // k_word_scan / k_scan_blocks / k_emit: popcount prefix over the ballot
 //   words and emission of (i, j) in k order.
 #include "internal.hpp"
@@ -91,6 +94,8 @@ __device__ __forceinline__ double o2yz(const float *a, const float *b, const flo
 }
 
 // Contact of the segment with a triangle: 0 none, 1 contact, 2 undecided.
+// For a triangle of p's fan (having p, not q, as a vertex): 4 contact
+// undecided, 5 no contact but the crossing below undecided.
 // For a triangle incident to p (and not to q) it also toggles xpar when the
 // +x ray from p + e(q-p), e -> 0+, crosses the triangle, and counts it in
 // ninc.  Over the full fan of p on a closed mesh, xpar is then the inside
@@ -111,7 +116,7 @@ __device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t, int &xpa
   if (ip) {
     double bound;
     const double det = o3_fast(a, b, c, s.q, bound);  // orient3d(p, b', c', q)
-    if (!(det > bound || -det > bound)) return 2;
+    if (!(det > bound || -det > bound)) return 4;
     ++ninc;
     // the triangle rotated to start at p: (p, v1, v2)
     const float *v1 = pa ? b : (pb ? c : a);
@@ -120,7 +125,8 @@ __device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t, int &xpa
     const double o0 = o2yz(s.p, v1, v2, b0);
     const double o1 = o2yz(s.p, v1, s.q, b1);
     const double o2 = o2yz(v2, s.p, s.q, b2);
-    if (!(o0 > b0 || -o0 > b0) || !(o1 > b1 || -o1 > b1) || !(o2 > b2 || -o2 > b2)) return 2;
+    // q is off the triangle's plane: the segment meets it at p only
+    if (!(o0 > b0 || -o0 > b0) || !(o1 > b1 || -o1 > b1) || !(o2 > b2 || -o2 > b2)) return 5;
     const bool s0 = o0 > 0.0;
     if ((o1 > 0.0) == s0 && (o2 > 0.0) == s0 && (det > 0.0) == s0) xpar ^= 1;
     return 0;
@@ -203,6 +209,47 @@ __device__ __forceinline__ int cross_fast(const Seg &s, const Tri &t) {
   return ((det > 0.0) == pos) ? 1 : 0;
 }
 
+// Triangle tests as the walks use them.  Filtered only (EXACT false): a
+// test the float64 filter cannot decide sets unsure and counts as no
+// contact / no crossing for now (the pair is then redone).  EXACT: such a
+// test is decided with exact predicates (counted in nex).  local_ok is
+// cleared when a fan triangle's crossing at p + e(q-p) stays unknown; the
+// pair then needs its midpoint ray.
+template <bool EXACT>
+__device__ __forceinline__ int contact_test(const Seg &s, const Tri &t, int &xpar, int &ninc,
+                                            bool &local_ok, bool &unsure, unsigned &nex) {
+  const int r = contact_fast(s, t, xpar, ninc);
+  if (r <= 1) return r;
+  if (r == 5) {
+    local_ok = false;
+    return 0;
+  }
+  if (r == 4) local_ok = false;
+  if (!EXACT) {
+    unsure = true;
+    return 0;
+  }
+  ++nex;
+  const float a3[3] = {t.a.x, t.a.y, t.a.z}, b3[3] = {t.b.x, t.b.y, t.b.z},
+              c3[3] = {t.c.x, t.c.y, t.c.z};
+  return contact_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
+}
+
+template <bool EXACT>
+__device__ __forceinline__ int cross_test(const Seg &s, const Tri &t, bool &unsure,
+                                          unsigned &nex) {
+  const int r = cross_fast(s, t);
+  if (r != 2) return r;
+  if (!EXACT) {
+    unsure = true;
+    return 0;
+  }
+  ++nex;
+  const float a3[3] = {t.a.x, t.a.y, t.a.z}, b3[3] = {t.b.x, t.b.y, t.b.z},
+              c3[3] = {t.c.x, t.c.y, t.c.z};
+  return ray_cross_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
+}
+
 __device__ __forceinline__ void load_node(const Node *__restrict__ nodes, int idx, float bx[12],
                                           int &c0, int &c1) {
   const Node nd = nodes[idx];
@@ -239,25 +286,28 @@ __device__ __forceinline__ int split_root(const Node *__restrict__ nodes, int g,
   return node;
 }
 
-// Any contact along the segment: 1 contact, 0 none, 2 none certain but some
-// test undecided, 3 malformed tree.  stk: this lane's LDS stack, stride 64
-// entries.  root: where the walk starts (a split walk's share); stop: a
-// group flag in LDS, set on a contact and polled so the group's other lanes
-// end their share early.
-template <int STACK, bool COUNT>
+// Any contact along the segment: 1 contact, 0 none, 2 none but the local
+// parity unknown (see contact_test), 4 none certain yet (filtered walk
+// only), 3 malformed tree.  stk: this lane's LDS stack, stride 64 entries.
+// root: where the walk starts (a split walk's share); stop: a group flag in
+// LDS, set on a contact and polled so the group's other lanes end their
+// share early.
+template <int STACK, bool COUNT, bool EXACT>
 __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
                                         const Seg &s, int *stk, int cap, unsigned &nv,
-                                        unsigned &nt, int &xpar, int &ninc, int root = 0,
-                                        volatile int *stop = nullptr) {
+                                        unsigned &nt, unsigned &nex, int &xpar, int &ninc,
+                                        int root = 0, volatile int *stop = nullptr) {
+  bool local_ok = true, unsure = false;
   if (root < 0) {
     if (root == kNoRoot) return 0;
     if (COUNT) ++nt;
-    const int r = contact_fast(s, tris[~root], xpar, ninc);
-    if (r == 1 && stop) *stop = 1;
-    return r;
+    if (contact_test<EXACT>(s, tris[~root], xpar, ninc, local_ok, unsure, nex)) {
+      if (stop) *stop = 1;
+      return 1;
+    }
+    return unsure ? 4 : (local_ok ? 0 : 2);
   }
   int node = root, sp = 0;
-  bool unsure = false;
   for (int it = 0;; ++it) {
     if (it > cap) return 3;  // visited more nodes than the tree has: malformed tree
     if (stop && *stop) return 0;  // another lane of the group found a contact
@@ -269,21 +319,17 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
     bool h1 = seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
     if (COUNT) nt += (h0 && c0 < 0) + (h1 && c1 < 0);
     if (h0 && c0 < 0) {
-      const int r = contact_fast(s, tris[~c0], xpar, ninc);
-      if (r == 1) {
+      if (contact_test<EXACT>(s, tris[~c0], xpar, ninc, local_ok, unsure, nex)) {
         if (stop) *stop = 1;
         return 1;
       }
-      unsure |= r == 2;
       h0 = false;
     }
     if (h1 && c1 < 0) {
-      const int r = contact_fast(s, tris[~c1], xpar, ninc);
-      if (r == 1) {
+      if (contact_test<EXACT>(s, tris[~c1], xpar, ninc, local_ok, unsure, nex)) {
         if (stop) *stop = 1;
         return 1;
       }
-      unsure |= r == 2;
       h1 = false;
     }
     if (h0 && h1) {
@@ -298,22 +344,24 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
       node = stk[(--sp) * 64];
     }
   }
-  return unsure ? 2 : 0;
+  return unsure ? 4 : (local_ok ? 0 : 2);
 }
 
-// Crossing parity of the midpoint ray: 0 even, 1 odd, 2 undecided, 3
-// malformed tree.  root as for walk_any.
-template <int STACK, bool COUNT>
+// Crossing parity of the midpoint ray: 0 even, 1 odd, 2 undecided (filtered
+// walk only), 3 malformed tree.  root as for walk_any.
+template <int STACK, bool COUNT, bool EXACT>
 __device__ __forceinline__ int walk_parity(const Node *__restrict__ nodes,
                                            const Tri *__restrict__ tris, const Seg &s, int *stk,
-                                           int cap, unsigned &nv, unsigned &nt, int root = 0) {
+                                           int cap, unsigned &nv, unsigned &nt, unsigned &nex,
+                                           int root = 0) {
+  bool unsure = false;
   if (root < 0) {
     if (root == kNoRoot) return 0;
     if (COUNT) ++nt;
-    return cross_fast(s, tris[~root]);
+    const int r = cross_test<EXACT>(s, tris[~root], unsure, nex);
+    return unsure ? 2 : r;
   }
   int node = root, sp = 0, cnt = 0;
-  bool unsure = false;
   for (int it = 0;; ++it) {
     if (it > cap) return 3;
     float bx[12];
@@ -324,15 +372,11 @@ __device__ __forceinline__ int walk_parity(const Node *__restrict__ nodes,
     bool h1 = ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
     if (COUNT) nt += (h0 && c0 < 0) + (h1 && c1 < 0);
     if (h0 && c0 < 0) {
-      const int r = cross_fast(s, tris[~c0]);
-      cnt += r == 1;
-      unsure |= r == 2;
+      cnt += cross_test<EXACT>(s, tris[~c0], unsure, nex);
       h0 = false;
     }
     if (h1 && c1 < 0) {
-      const int r = cross_fast(s, tris[~c1]);
-      cnt += r == 1;
-      unsure |= r == 2;
+      cnt += cross_test<EXACT>(s, tris[~c1], unsure, nex);
       h1 = false;
     }
     if (h0 && h1) {
@@ -347,8 +391,7 @@ __device__ __forceinline__ int walk_parity(const Node *__restrict__ nodes,
       node = stk[(--sp) * 64];
     }
   }
-  if (unsure) return 2;
-  return cnt & 1;
+  return unsure ? 2 : (cnt & 1);
 }
 
 __device__ __forceinline__ float lane_of(const float4 &v, int c) {
@@ -361,9 +404,10 @@ __device__ __forceinline__ float lane_of(const float4 &v, int c) {
 template <int STACK, bool COUNT>
 __device__ __forceinline__ int walk_any4(const Node4 *__restrict__ nodes,
                                          const Tri *__restrict__ tris, const Seg &s, int *stk,
-                                         int cap, unsigned &nv, unsigned &nt, int &xpar, int &ninc) {
+                                         int cap, unsigned &nv, unsigned &nt, unsigned &nex,
+                                         int &xpar, int &ninc) {
   int node = 0, sp = 0;
-  bool unsure = false;
+  bool local_ok = true, unsure = false;
   for (int it = 0;; ++it) {
     if (it > cap) return 3;
     const Node4 nd = nodes[node];
@@ -377,9 +421,7 @@ __device__ __forceinline__ int walk_any4(const Node4 *__restrict__ nodes,
       const int id = __float_as_int(lane_of(nd.f[6], c));
       if (id < 0) {
         if (COUNT) ++nt;
-        const int r = contact_fast(s, tris[~id], xpar, ninc);
-        if (r == 1) return 1;
-        unsure |= r == 2;
+        if (contact_test<false>(s, tris[~id], xpar, ninc, local_ok, unsure, nex)) return 1;
       } else if (next < 0) {
         next = id;
       } else {
@@ -393,13 +435,13 @@ __device__ __forceinline__ int walk_any4(const Node4 *__restrict__ nodes,
       node = stk[(--sp) * 64];
     }
   }
-  return unsure ? 2 : 0;
+  return unsure ? 4 : (local_ok ? 0 : 2);
 }
 
 template <int STACK, bool COUNT>
 __device__ __forceinline__ int walk_parity4(const Node4 *__restrict__ nodes,
                                             const Tri *__restrict__ tris, const Seg &s, int *stk,
-                                            int cap, unsigned &nv, unsigned &nt) {
+                                            int cap, unsigned &nv, unsigned &nt, unsigned &nex) {
   int node = 0, sp = 0, cnt = 0;
   bool unsure = false;
   for (int it = 0;; ++it) {
@@ -415,9 +457,7 @@ __device__ __forceinline__ int walk_parity4(const Node4 *__restrict__ nodes,
       const int id = __float_as_int(lane_of(nd.f[6], c));
       if (id < 0) {
         if (COUNT) ++nt;
-        const int r = cross_fast(s, tris[~id]);
-        cnt += r == 1;
-        unsure |= r == 2;
+        cnt += cross_test<false>(s, tris[~id], unsure, nex);
       } else if (next < 0) {
         next = id;
       } else {
@@ -431,8 +471,7 @@ __device__ __forceinline__ int walk_parity4(const Node4 *__restrict__ nodes,
       node = stk[(--sp) * 64];
     }
   }
-  if (unsure) return 2;
-  return cnt & 1;
+  return unsure ? 2 : (cnt & 1);
 }
 
 // What a pair needs to know about its part (one mesh and its point set).
@@ -464,12 +503,13 @@ struct VisArgs {
   int nparts;
   int64_t nwords, npoints;
   uint64_t *bits;
-  unsigned long long *stats;  // [0] masked, [1] blocked, [2] deferred, [3] edges, [4] malformed,
-                              // [5] node visits, [6] triangle tests, [7] walk-queue length,
-                              // [8] deferred-list length
+  unsigned long long *stats;  // [0] masked, [1] blocked, [2] pairs redone exactly, [3] edges,
+                              // [4] malformed, [5] node visits, [6] triangle tests,
+                              // [7] walk-queue length, [8] deferred-list length,
+                              // [9] exact triangle tests
   int64_t *queue;             // global pair ids (word * 64 + bit) left to walk
   unsigned long long *queue_len;
-  int64_t *dlist;             // global pair ids deferred to the exact redo
+  int64_t *dlist;             // global pair ids whose walks met an undecided test
   unsigned long long *dlist_len;
 };
 
@@ -486,51 +526,53 @@ __device__ __forceinline__ Part part_of_word(const VisArgs &a, int64_t w) {
   return a.parts[lo];
 }
 
-template <int STACK, bool WIDE, bool COUNT>
+template <int STACK, bool WIDE, bool COUNT, bool EXACT>
 __device__ __forceinline__ int any_hit(const Node *n2, const Node4 *n4, const Tri *tris,
                                        const Seg &s, int *stk, int cap, unsigned &nv,
-                                       unsigned &nt, int &xpar, int &ninc) {
-  if (WIDE) return walk_any4<STACK, COUNT>(n4, tris, s, stk, cap, nv, nt, xpar, ninc);
-  return walk_any<STACK, COUNT>(n2, tris, s, stk, cap, nv, nt, xpar, ninc);
+                                       unsigned &nt, unsigned &nex, int &xpar, int &ninc) {
+  if (WIDE && !EXACT) return walk_any4<STACK, COUNT>(n4, tris, s, stk, cap, nv, nt, nex, xpar, ninc);
+  return walk_any<STACK, COUNT, EXACT>(n2, tris, s, stk, cap, nv, nt, nex, xpar, ninc);
 }
 
-template <int STACK, bool WIDE, bool COUNT>
+template <int STACK, bool WIDE, bool COUNT, bool EXACT>
 __device__ __forceinline__ int ray_parity(const Node *n2, const Node4 *n4, const Tri *tris,
                                           const Seg &s, int *stk, int cap, unsigned &nv,
-                                          unsigned &nt) {
-  if (WIDE) return walk_parity4<STACK, COUNT>(n4, tris, s, stk, cap, nv, nt);
-  return walk_parity<STACK, COUNT>(n2, tris, s, stk, cap, nv, nt);
+                                          unsigned &nt, unsigned &nex) {
+  if (WIDE && !EXACT) return walk_parity4<STACK, COUNT>(n4, tris, s, stk, cap, nv, nt, nex);
+  return walk_parity<STACK, COUNT, EXACT>(n2, tris, s, stk, cap, nv, nt, nex);
 }
 
 // Walks for one pair the fan could not settle: any contact with the self
-// mesh, then the cage walk (cage mode) or the inside parity (parity mode).
-template <int STACK, bool CAGE, bool COUNT, bool WIDE>
+// mesh, then the cage walk (cage mode) or the inside parity (parity mode):
+// local when p's fan crossings are all known, else the midpoint ray.  A
+// filtered walk (EXACT false) that meets an undecided test sets defer.
+template <int STACK, bool CAGE, bool COUNT, bool WIDE, bool EXACT>
 __device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int *stk, bool &edge,
                                           bool &defer, unsigned &n_blocked, unsigned &n_bad,
-                                          unsigned &nv, unsigned &nt) {
+                                          unsigned &nv, unsigned &nt, unsigned &nex) {
   make_seg(s);
   int xpar = 0, ninc = 0;
-  const int hs =
-      any_hit<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap, nv, nt, xpar, ninc);
+  const int hs = any_hit<STACK, WIDE, COUNT, EXACT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap,
+                                                    nv, nt, nex, xpar, ninc);
   if (hs == 1) {
     ++n_blocked;
-  } else if (hs == 2) {
-    defer = true;
   } else if (hs == 3) {
     ++n_bad;
+  } else if (hs == 4) {
+    defer = true;
   } else if (CAGE && P.ctris) {
     int cx = 0, cn = 0;
-    const int hc =
-        any_hit<STACK, WIDE, COUNT>(P.cnodes, P.cnodes4, P.ctris, s, stk, P.ccap, nv, nt, cx, cn);
+    const int hc = any_hit<STACK, WIDE, COUNT, EXACT>(P.cnodes, P.cnodes4, P.ctris, s, stk,
+                                                      P.ccap, nv, nt, nex, cx, cn);
     edge = hc == 1;
-    defer = hc == 2;
+    defer = hc == 4;
     n_bad += hc == 3;
-  } else if (P.far && ninc > 0 && P.far[i] <= 1) {
+  } else if (hs == 0 && P.far && ninc > 0 && P.far[i] <= 1) {
     // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
     edge = ((xpar ^ P.far[i]) & 1) == 0;
-  } else {  // no local parity for this point: the midpoint ray itself
-    const int par =
-        ray_parity<STACK, WIDE, COUNT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap, nv, nt);
+  } else {  // no local parity for this pair: the midpoint ray itself
+    const int par = ray_parity<STACK, WIDE, COUNT, EXACT>(P.snodes, P.snodes4, P.stris, s, stk,
+                                                          P.scap, nv, nt, nex);
     edge = par == 0;
     defer = par == 2;
     n_bad += par == 3;
@@ -555,7 +597,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   __shared__ int stk_all[kWaves][STACK][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int *stk = &stk_all[wave][0][lane];
-  unsigned n_masked = 0, n_blocked = 0, n_defer = 0, n_bad = 0, nv = 0, nt = 0, n_walk = 0;
+  unsigned n_masked = 0, n_blocked = 0, n_exact = 0, n_bad = 0, nv = 0, nt = 0, n_walk = 0;
+  (void)n_exact;  // filtered walks make no exact tests
   const int64_t total_waves = (int64_t)gridDim.x * kWaves;
   for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < a.nwords; w += total_waves) {
     const Part P = part_of_word<BATCH>(a, w);  // wave-uniform
@@ -593,13 +636,13 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             queued = true;
           } else {
             ++n_walk;
-            walk_pair<STACK, CAGE, COUNT, WIDE>(P, i, s, stk, edge, defer, n_blocked, n_bad, nv, nt);
+            walk_pair<STACK, CAGE, COUNT, WIDE, false>(P, i, s, stk, edge, defer, n_blocked, n_bad,
+                                                       nv, nt, n_exact);
           }
         }
       }
     }
     const uint64_t eb = __ballot(edge);
-    n_defer += defer;
     if (lane == 0) a.bits[w] = eb;
     append_lanes(defer, w * 64 + lane, a.dlist, a.dlist_len);
     // append this wave's undecided pairs (global k) to the walk queue
@@ -609,14 +652,12 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   for (int off = 32; off > 0; off >>= 1) {
     n_masked += __shfl_down(n_masked, off);
     n_blocked += __shfl_down(n_blocked, off);
-    n_defer += __shfl_down(n_defer, off);
     n_bad += __shfl_down(n_bad, off);
     n_walk += __shfl_down(n_walk, off);
   }
   if (lane == 0) {
     if (n_masked) atomicAdd(&a.stats[0], (unsigned long long)n_masked);
     if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
-    if (n_defer) atomicAdd(&a.stats[2], (unsigned long long)n_defer);
     if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
     if (n_walk) atomicAdd(&a.stats[7], (unsigned long long)n_walk);  // queue mode: queue_len
   }
@@ -633,18 +674,25 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   }
 }
 
-// Dense walks over the queue of pairs the fan pass left undecided, 2^LV
-// lanes per pair: each lane walks its share of the tree (split_root), the
-// group combines its lanes' results through ballots (any contact: OR; fan
-// crossings and ray crossings: XOR), and the group's first lane OR-s the
-// outcome into the ballot words (the fan pass wrote zero for these bits).
-// More lanes per pair shorten the longest walk of a wave, which bounds the
-// kernel, at the price of box tests near the root done by several lanes.
+// Dense walks over a list of pairs, 2^LV lanes per pair: each lane walks its
+// share of the tree (split_root), the group combines its lanes' results
+// through ballots (any contact: OR; fan crossings and ray crossings: XOR),
+// and the group's first lane OR-s the outcome into the ballot words (zero
+// for these bits until then).  More lanes per pair shorten the longest walk
+// of a wave at the price of box tests near the root done by several lanes.
+// EXACT false: the walk queue the fan pass left, filtered tests; a pair
+// meeting an undecided test goes to the deferred list.  EXACT true: the
+// deferred list, every test decided (exact predicates where the filter
+// cannot decide); with LV = 6 (a wave per pair, for latency) the first
+// kExactSplit entries, with LV = 0 (a lane per pair, for throughput) the
+// rest.  Only the EXACT instances carry the exact code's register cost.
 __device__ __forceinline__ uint64_t group_mask(int lane, int G) {
   return G == 64 ? ~0ull : (((1ull << G) - 1ull) << (lane & ~(G - 1)));
 }
 
-template <int STACK, bool COUNT, bool BATCH, int LV>
+constexpr int64_t kExactSplit = 2048;
+
+template <int STACK, bool COUNT, bool BATCH, int LV, bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
   constexpr int G = 1 << LV;
   constexpr int kGroups = kBlock / G;
@@ -655,11 +703,14 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
   const int g = lane & (G - 1);
   const int slot = threadIdx.x / G;
   const uint64_t gm = group_mask(lane, G);
-  unsigned n_blocked = 0, n_defer = 0, n_bad = 0, nv = 0, nt = 0;
-  const int64_t qn = (int64_t)*a.queue_len;
-  for (int64_t e = (int64_t)blockIdx.x * kGroups + slot; e < qn;
+  unsigned n_blocked = 0, n_exact = 0, n_bad = 0, nv = 0, nt = 0;
+  const int64_t *list = EXACT ? a.dlist : a.queue;
+  const int64_t len = (int64_t)(EXACT ? *a.dlist_len : *a.queue_len);
+  const int64_t e_lo = (EXACT && LV == 0) ? kExactSplit : 0;
+  const int64_t e_hi = (EXACT && LV == 6) ? (len < kExactSplit ? len : kExactSplit) : len;
+  for (int64_t e = e_lo + (int64_t)blockIdx.x * kGroups + slot; e < e_hi;
        e += (int64_t)gridDim.x * kGroups) {
-    const int64_t kg = a.queue[e];
+    const int64_t kg = list[e];
     const int64_t w = kg >> 6;
     const int b = (int)(kg & 63);
     const Part P = part_of_word<BATCH>(a, w);
@@ -673,24 +724,40 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
     if (g == 0) *stop = 0;
     int xpar = 0, ninc = 0;
     const int root = split_root<false>(P.snodes, g, LV, s);
-    const int hs = walk_any<STACK, COUNT>(P.snodes, P.stris, s, stk, P.scap, nv, nt, xpar, ninc,
-                                          root, G > 1 ? stop : nullptr);
+    const int hs = walk_any<STACK, COUNT, EXACT>(P.snodes, P.stris, s, stk, P.scap, nv, nt,
+                                                 n_exact, xpar, ninc, root,
+                                                 G > 1 ? stop : nullptr);
     const uint64_t m3 = __ballot(hs == 3) & gm, m1 = __ballot(hs == 1) & gm,
-                   m2 = __ballot(hs == 2) & gm, mx = __ballot(xpar & 1) & gm,
-                   mn = __ballot(ninc > 0) & gm;
-    bool edge = false, defer = false, blocked = false, bad = false;
+                   m2 = __ballot(hs == 2) & gm, m4 = __ballot(hs == 4) & gm,
+                   mx = __ballot(xpar & 1) & gm, mn = __ballot(ninc > 0) & gm;
+    bool edge = false, blocked = false, bad = false, defer = false;
     if (m3) {
       bad = true;
     } else if (m1) {
       blocked = true;
-    } else if (m2) {
+    } else if (m4) {
       defer = true;
-    } else if (P.far && mn && P.far[i] <= 1) {
+    } else if (P.ctris) {  // cage mode (deferred cage pairs only): any contact with the cage
+      if (g == 0) *stop = 0;
+      int cx = 0, cn = 0;
+      const int rc = split_root<false>(P.cnodes, g, LV, s);
+      const int hc = walk_any<STACK, COUNT, EXACT>(P.cnodes, P.ctris, s, stk, P.ccap, nv, nt,
+                                                   n_exact, cx, cn, rc, G > 1 ? stop : nullptr);
+      const uint64_t c3 = __ballot(hc == 3) & gm, c1 = __ballot(hc == 1) & gm,
+                     c4 = __ballot(hc == 4) & gm;
+      if (c3)
+        bad = true;
+      else if (c1)
+        edge = true;
+      else if (c4)
+        defer = true;
+    } else if (!m2 && P.far && mn && P.far[i] <= 1) {
       // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
       edge = (((__popcll(mx) & 1) ^ P.far[i]) & 1) == 0;
     } else {  // group-uniform branch: the midpoint ray, split the same way
       const int rr = split_root<true>(P.snodes, g, LV, s);
-      const int par = walk_parity<STACK, COUNT>(P.snodes, P.stris, s, stk, P.scap, nv, nt, rr);
+      const int par =
+          walk_parity<STACK, COUNT, EXACT>(P.snodes, P.stris, s, stk, P.scap, nv, nt, n_exact, rr);
       const uint64_t p3 = __ballot(par == 3) & gm, p2 = __ballot(par == 2) & gm,
                      p1 = __ballot(par == 1) & gm;
       if (p3)
@@ -702,23 +769,22 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
     }
     if (g == 0) {
       if (edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
-      if (defer) a.dlist[atomicAdd(a.dlist_len, 1ull)] = kg;
+      if (!EXACT && defer) a.dlist[atomicAdd(a.dlist_len, 1ull)] = kg;
       n_blocked += blocked;
-      n_defer += defer;
       n_bad += bad;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
     n_blocked += __shfl_down(n_blocked, off);
-    n_defer += __shfl_down(n_defer, off);
     n_bad += __shfl_down(n_bad, off);
+    n_exact += __shfl_down(n_exact, off);
     nv += __shfl_down(nv, off);
     nt += __shfl_down(nt, off);
   }
   if (lane == 0) {
     if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
-    if (n_defer) atomicAdd(&a.stats[2], (unsigned long long)n_defer);
     if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
+    if (EXACT && n_exact) atomicAdd(&a.stats[9], (unsigned long long)n_exact);
     if (COUNT) {
       atomicAdd(&a.stats[5], (unsigned long long)nv);
       atomicAdd(&a.stats[6], (unsigned long long)nt);
@@ -727,64 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
 }
 
 // ------------------------------------------------------------ exact redo
-// Exact walks from a split root: filtered tests first, exact predicates
-// where the filter cannot decide.  stk: LDS, stride 64.
-__device__ bool exact_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
-                          const Seg &s, int root, int *stk, int cap, volatile int *stop,
-                          bool &bad) {
-  auto leaf = [&](int id) {
-    const Tri tr = tris[id];
-    int xp = 0, ni = 0;
-    int r = contact_fast(s, tr, xp, ni);
-    if (r == 2) {
-      const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
-                  c3[3] = {tr.c.x, tr.c.y, tr.c.z};
-      r = contact_exact(s.p, s.q, a3, b3, c3) ? 1 : 0;
-    }
-    return r == 1;
-  };
-  if (root == kNoRoot) return false;
-  if (root < 0) {
-    const bool hit = leaf(~root);
-    if (hit) *stop = 1;
-    return hit;
-  }
-  int node = root, sp = 0;
-  for (int it = 0;; ++it) {
-    if (it > cap) {
-      bad = true;
-      return false;
-    }
-    if (*stop) return false;
-    float bx[12];
-    int c0, c1;
-    load_node(nodes, node, bx, c0, c1);
-    int ch[2] = {c0, c1};
-    bool h[2] = {seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]),
-                 seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11])};
-    for (int t = 0; t < 2; ++t) {
-      if (h[t] && ch[t] < 0) {
-        if (leaf(~ch[t])) {
-          *stop = 1;
-          return true;
-        }
-        h[t] = false;
-      }
-    }
-    if (h[0] && h[1]) {
-      stk[(sp++) * 64] = ch[1];
-      node = ch[0];
-    } else if (h[0]) {
-      node = ch[0];
-    } else if (h[1]) {
-      node = ch[1];
-    } else {
-      if (sp == 0) return false;
-      node = stk[(--sp) * 64];
-    }
-  }
-}
-
+// ------------------------------------------------------------ far parity
 // Crossings (mod 2) of the +x ray from the segment's midpoint with the
 // triangles of this lane's share.  Midpoint mode (skip NULL): the ray is
 // perturbed by (0, e, e^2), the definition of midpoint parity.  Point mode
@@ -837,66 +846,6 @@ __device__ int exact_ray_parity(const Node *__restrict__ nodes, const Tri *__res
       if (sp == 0) return cnt & 1;
       node = stk[(--sp) * 64];
     }
-  }
-}
-
-// Every deferred pair is redone with exact predicates by a whole wave: the
-// 64 lanes split the walks (split_root at level 6), any contact ends them
-// all, crossing parities are combined by xor.  Midpoint parity by
-// definition.  A pair the exact walk cannot finish (malformed tree) counts
-// in the malformed statistic.
-template <bool BATCH>
-__global__ __launch_bounds__(64) void k_vis_exact(VisArgs a) {
-  __shared__ int stk_all[64][64];
-  __shared__ int stop_flag;
-  const int lane = threadIdx.x;
-  int *stk = &stk_all[0][lane];
-  volatile int *stop = &stop_flag;
-  unsigned n_blocked = 0, n_bad = 0;
-  const int64_t len = (int64_t)*a.dlist_len;
-  for (int64_t e = blockIdx.x; e < len; e += gridDim.x) {
-    const int64_t kg = a.dlist[e];
-    const int64_t w = kg >> 6;
-    const int b = (int)(kg & 63);
-    const Part P = part_of_word<BATCH>(a, w);
-    int64_t i, j;
-    pair_of((w - P.word_begin) * 64 + b, P.n, i, j);
-    Seg s;
-    for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
-    make_seg(s);
-    if (lane == 0) *stop = 0;
-    bool bad = false;
-    const int root = split_root<false>(P.snodes, lane, 6, s);
-    exact_any(P.snodes, P.stris, s, root, stk, P.scap, stop, bad);
-    bool edge = false;
-    if (__ballot(bad)) {
-      ++n_bad;
-    } else if (*stop) {
-      ++n_blocked;
-    } else if (P.ctris) {
-      if (lane == 0) *stop = 0;
-      const int rc = split_root<false>(P.cnodes, lane, 6, s);
-      exact_any(P.cnodes, P.ctris, s, rc, stk, P.ccap, stop, bad);
-      if (__ballot(bad))
-        ++n_bad;
-      else
-        edge = *stop != 0;
-    } else {
-      const int rr = split_root<true>(P.snodes, lane, 6, s);
-      bool degenerate = false;  // not set in midpoint mode
-      const int par =
-          exact_ray_parity(P.snodes, P.stris, s, rr, stk, P.scap, nullptr, bad, degenerate);
-      const uint64_t odd = __ballot(par & 1);
-      if (__ballot(bad))
-        ++n_bad;
-      else
-        edge = (__popcll(odd) & 1) == 0;
-    }
-    if (lane == 0 && edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
-  }
-  if (lane == 0) {
-    if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
-    if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
   }
 }
 
@@ -1150,11 +1099,11 @@ struct Launch {
 template <int S, bool N, bool B>
 void launch_split(const Launch &L, const VisArgs &a) {
   switch (L.lv) {
-    case 0: k_walk_split<S, N, B, 0><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
-    case 1: k_walk_split<S, N, B, 1><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
-    case 2: k_walk_split<S, N, B, 2><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
-    case 3: k_walk_split<S, N, B, 3><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
-    default: k_walk_split<S, N, B, 4><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+    case 0: k_walk_split<S, N, B, 0, false><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+    case 1: k_walk_split<S, N, B, 1, false><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+    case 2: k_walk_split<S, N, B, 2, false><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+    case 3: k_walk_split<S, N, B, 3, false><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
+    default: k_walk_split<S, N, B, 4, false><<<L.qgrid, kBlock, 0, L.st>>>(a); break;
   }
 }
 
@@ -1279,11 +1228,27 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     pick_c<64>(any_cage, ctx->count_work, wide, batch, queue, L, a);
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_VIS);
-  const unsigned egrid = 2048;  // a wave per deferred pair, grid-stride over the list
-  if (batch)
-    k_vis_exact<true><<<egrid, 64, 0, st>>>(a);
-  else
-    k_vis_exact<false><<<egrid, 64, 0, st>>>(a);
+  // the deferred pairs (the list length is read on the device): a wave per
+  // pair for the first kExactSplit, a lane per pair for the rest
+  const unsigned wgrid = (unsigned)(kExactSplit / kWaves);
+  const unsigned xgrid = (unsigned)std::min<int64_t>((nwords * 64 + kBlock - 1) / kBlock, 1024);
+  if (need2 <= 32) {
+    if (batch) {
+      k_walk_split<32, false, true, 6, true><<<wgrid, kBlock, 0, st>>>(a);
+      k_walk_split<32, false, true, 0, true><<<xgrid, kBlock, 0, st>>>(a);
+    } else {
+      k_walk_split<32, false, false, 6, true><<<wgrid, kBlock, 0, st>>>(a);
+      k_walk_split<32, false, false, 0, true><<<xgrid, kBlock, 0, st>>>(a);
+    }
+  } else {
+    if (batch) {
+      k_walk_split<64, false, true, 6, true><<<wgrid, kBlock, 0, st>>>(a);
+      k_walk_split<64, false, true, 0, true><<<xgrid, kBlock, 0, st>>>(a);
+    } else {
+      k_walk_split<64, false, false, 6, true><<<wgrid, kBlock, 0, st>>>(a);
+      k_walk_split<64, false, false, 0, true><<<xgrid, kBlock, 0, st>>>(a);
+    }
+  }
   VACD_HIP(hipGetLastError());
   unsigned long long *blk = scan.as<unsigned long long>();
   k_word_scan<<<(unsigned)nscan, 256, 0, st>>>(a.bits, nwords, blk);
