@@ -127,6 +127,12 @@ int vacd_visibility_d(vacd_ctx *ctx, const float *d_pts, int64_t n,
 /* Statistics of the last visibility call on ctx (waits for it). */
 int vacd_visibility_stats(vacd_ctx *ctx, vacd_vis_stats *out);
 
+/* Diagnostic: the pair ids k (global across the parts of a batch) that the
+ * last visibility call redid with exact predicates, in no particular order;
+ * copies min(count, cap) ids to host memory and sets *count.  No counterpart
+ * in the reference (its any-hit programs have no exact fallback). */
+int vacd_visibility_deferred(vacd_ctx *ctx, int64_t *pair_ids, int64_t cap, int64_t *count);
+
 /* One part of a batched visibility call: device points, optional device
  * new-vertex mask, its own BVH and optional cage BVH. */
 typedef struct vacd_part {

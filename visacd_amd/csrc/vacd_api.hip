@@ -342,6 +342,23 @@ int vacd_visibility_stats(vacd_ctx *ctx, vacd_vis_stats *out) {
   });
 }
 
+int vacd_visibility_deferred(vacd_ctx *ctx, int64_t *pair_ids, int64_t cap, int64_t *count) {
+  if (!ctx || !count) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(cap >= 0 && (cap == 0 || pair_ids), VACD_ERR_INVALID, "bad id buffer");
+    set_device(ctx);
+    unsigned long long len = 0;
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->buf[SLOT_STATS].ptr)
+      VACD_HIP(hipMemcpy(&len, ctx->buf[SLOT_STATS].as<unsigned long long>() + 8, sizeof(len),
+                         hipMemcpyDeviceToHost));
+    *count = (int64_t)len;
+    const int64_t n = (int64_t)len < cap ? (int64_t)len : cap;
+    if (n > 0)
+      VACD_HIP(hipMemcpy(pair_ids, ctx->buf[SLOT_DEFER].ptr, 8 * (size_t)n, hipMemcpyDeviceToHost));
+  });
+}
+
 // ------------------------------------------------------------ scoring
 int vacd_score_planes_d(vacd_ctx *ctx, const float *d_planes, int64_t P, const float *d_pts,
                         int64_t n, const uint32_t *d_edges, int64_t E, int64_t *d_score_q32,

@@ -145,9 +145,13 @@ __device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t, int &xpa
   const double bq = kFilter * (fabs(aqx) * mx + fabs(aqy) * my + fabs(aqz) * mz);
   const bool cp = dp > bp || -dp > bp, cq = dq > bq || -dq > bq;
   if (cp && cq && ((dp > 0.0) == (dq > 0.0))) return 0;  // strictly one side
-  if (!(cp && cq)) return 2;
-  // opposite sides: does the line pq pass through the triangle?
-  // u = D.(BxC), v = D.(CxA), w = D.(AxB) with A = a-p, B = b-p, C = c-p, D = q-p
+  // does the line pq pass through the triangle?  Asked also when a side is
+  // undecided: a line that certainly misses the closed triangle rules out
+  // contact whatever the sides (an endpoint lying in the plane of a
+  // triangle it is not a vertex of, as on a flat face, is the common case).
+  // u = D.(BxC), v = D.(CxA), w = D.(AxB) with A = a-p, B = b-p, C = c-p, D = q-p;
+  // u + v + w = D.N, so (u, v, w) / D.N are the barycentric coordinates of
+  // where the line meets the plane; with D.N = 0 the line is parallel to it.
   const double Ax = apx, Ay = apy, Az = apz;
   const double Bx = (double)b[0] - s.p[0], By = (double)b[1] - s.p[1], Bz = (double)b[2] - s.p[2];
   const double Cx = (double)c[0] - s.p[0], Cy = (double)c[1] - s.p[1], Cz = (double)c[2] - s.p[2];
@@ -169,6 +173,8 @@ __device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t, int &xpa
   const bool pos = u > bu || v > bv || w > bw;
   const bool neg = -u > bu || -v > bv || -w > bw;
   if (pos && neg) return 0;  // the line misses the triangle
+  if (!(cp && cq)) return 2;
+  // opposite sides: contact iff the line passes through the triangle
   const bool all = (u > bu || -u > bu) && (v > bv || -v > bv) && (w > bw || -w > bw);
   return all ? 1 : 2;
 }

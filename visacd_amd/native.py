@@ -26,7 +26,8 @@ EXPORTED = [
     "vacd_ctx_create", "vacd_ctx_destroy", "vacd_last_error", "vacd_ctx_stream",
     "vacd_ctx_set_stream", "vacd_sync", "vacd_ctx_set_option", "vacd_ctx_timing", "vacd_bvh_build", "vacd_bvh_build_d",
     "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_export", "vacd_visibility",
-    "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_batch_d", "vacd_score_planes",
+    "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_deferred",
+    "vacd_visibility_batch_d", "vacd_score_planes",
     "vacd_score_planes_d", "vacd_max_distance", "vacd_max_distance_d", "classify_and_rate_planes",
 ]
 
@@ -95,6 +96,7 @@ def lib():
         L.vacd_visibility.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, ctypes.POINTER(i64)]
         L.vacd_visibility_d.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, vp]
         L.vacd_visibility_stats.argtypes = [vp, ctypes.POINTER(VisStats)]
+        L.vacd_visibility_deferred.argtypes = [vp, vp, i64, ctypes.POINTER(i64)]
         L.vacd_visibility_batch_d.argtypes = [vp, ctypes.POINTER(Part), ctypes.c_int32, vp, i64,
                                               vp, vp]
         L.vacd_score_planes.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp,
@@ -227,6 +229,14 @@ class Context:
         s = VisStats()
         self._check(lib().vacd_visibility_stats(self._h, ctypes.byref(s)))
         return {k: int(getattr(s, k)) for k, _ in VisStats._fields_}
+
+    def visibility_deferred(self) -> np.ndarray:
+        """Pair ids k the last visibility call redid with exact predicates."""
+        cnt = ctypes.c_int64()
+        self._check(lib().vacd_visibility_deferred(self._h, None, 0, ctypes.byref(cnt)))
+        out = np.empty(max(1, cnt.value), dtype=np.int64)
+        self._check(lib().vacd_visibility_deferred(self._h, _p(out), out.size, ctypes.byref(cnt)))
+        return np.sort(out[:cnt.value])
 
     # -- scoring ---------------------------------------------------------------
     def score_planes(self, planes, points, edges):
