@@ -203,7 +203,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "this engine has no multi-GPU path in this build.", flush=True)
+              "this engine runs on one GPU.", flush=True)
         sys.exit(2)
 
     import numpy as np

@@ -100,6 +100,12 @@ int vacd_bvh_info(const vacd_bvh *bvh, int64_t *n_tris, int32_t *depth,
                   float *bounds6);
 /* Copies the node array (64 B per internal node, n_tris-1 nodes) and the
  * leaf-ordered triangle array (48 B per triangle) to host memory. */
+/* Diagnostic: primitives (triangles and slices of long triangles) and depth
+ * of the BVH's contact tree; 0 and 0 when no triangle needed slicing.  No
+ * counterpart in the reference (its any-hit programs run on OptiX's own
+ * acceleration structure). */
+int vacd_bvh_contact_info(const vacd_bvh *bvh, int64_t *n_refs, int32_t *depth);
+
 int vacd_bvh_export(const vacd_bvh *bvh, void *nodes, void *tris);
 
 /* ---- visibility --------------------------------------------------------------
@@ -107,7 +113,10 @@ int vacd_bvh_export(const vacd_bvh *bvh, void *nodes, void *tris);
  * (lib/src/intersections.cpp:75-325, 327-385; device programs
  * lib/ptx/ray_segments.cu:49-125).  Tests every pair (i<j) of the n points
  * pts[n*3]: pairs with new_mask[i]==0 && new_mask[j]==0 are skipped when
- * new_mask is non-NULL; a pair is a visibility edge iff its segment does not
+ * new_mask is non-NULL, and so are pairs with new_mask[i]==2 &&
+ * new_mask[j]==2 (2 marks a new point whose pairs with the other 2-points
+ * are not wanted, such as the points of one planar cap; the reference has
+ * no value 2); a pair is a visibility edge iff its segment does not
  * touch the self mesh and (cage != NULL) touches the cage mesh, or
  * (cage == NULL) its midpoint lies outside the self mesh.  Edges are written
  * to pairs[2*e], pairs[2*e+1] = (i, j) in increasing k = i(2n-i-1)/2 + j-i-1

@@ -487,7 +487,9 @@ struct Ctx {
 };
 
 int pair_detail(const Ctx& C, int64_t i, int64_t j) {
-  if (C.mask && !C.mask[i] && !C.mask[j]) return 2;
+  // 0/0: two old vertices (ray_segments.cu:61-75); 2/2: two points of one
+  // cap, skipped by the caller's choice (no counterpart in the reference)
+  if (C.mask && ((!C.mask[i] && !C.mask[j]) || (C.mask[i] == 2 && C.mask[j] == 2))) return 2;
   P3 p = ld(C.pts + 3 * i), q = ld(C.pts + 3 * j);
   if (same(p, q)) return 3;
   auto self_hit = [&](int t) {

@@ -236,6 +236,51 @@ def test_visibility_degenerate_polycube(ctx, seed):
     _print("exact fallbacks", st["exact_calls"])
 
 
+def _clipped_part(level, seed, plane):
+    from visacd_amd import _visacd as V
+
+    v, t = wl.perturbed_icosphere(level, 0.3, seed)
+    m = V.Mesh()
+    m.vertices = V.VecArray3d(np.asarray(v, np.float64).tolist())
+    m.triangles = V.make_vecarray3i(np.asarray(t, np.int32))
+    part = V.clip(m, plane)[0][0]
+    Pv = np.array([list(x) for x in part.vertices], np.float32)
+    T = np.array([list(x) for x in part.triangles], np.uint32)
+    n = np.asarray(plane[:3])
+    cap = np.abs(Pv.astype(np.float64) @ n + plane[3]) < 1e-6
+    return Pv, T, cap
+
+
+def test_visibility_clipped_part_contact_tree(ctx):
+    """A cut through a fine mesh leaves a cap of slivers spanning the section;
+    they are sliced in the contact tree, and the edges must not change."""
+    n = np.array([0.3, 0.5, -0.2])
+    n /= np.linalg.norm(n)
+    Pv, T, cap = _clipped_part(4, 21, [*n, 0.05])
+    b = ctx.bvh(Pv, T)
+    refs, rdepth = b.contact_info()
+    _print("clipped part", Pv.shape, T.shape, "contact tree", refs, rdepth)
+    assert refs > T.shape[0]
+    rng = np.random.default_rng(4)
+    idx = np.unique(np.concatenate([np.where(cap)[0],
+                                    rng.choice(np.where(~cap)[0], 500, replace=False)]))
+    pts = Pv[idx]
+    o, _ = oracle.visibility(pts, Pv, T)
+    g = ctx.visibility(pts, b)
+    np.testing.assert_array_equal(g, o)
+    mask = cap[idx].astype(np.uint8)
+    om, _ = oracle.visibility(pts, Pv, T, mask=mask)
+    gm = ctx.visibility(pts, b, new_mask=mask)
+    np.testing.assert_array_equal(gm, om)
+    _print("clipped part edges", len(g), "masked", len(gm), ctx.visibility_stats())
+    # cap points as 2: their pairs with each other are skipped
+    m2 = (2 * cap[idx]).astype(np.uint8)
+    o2, ost2 = oracle.visibility(pts, Pv, T, mask=m2)
+    g2 = ctx.visibility(pts, b, new_mask=m2)
+    np.testing.assert_array_equal(g2, o2)
+    assert ctx.visibility_stats()["masked"] == ost2["masked"]
+
+
 def test_visibility_icosphere_c2_sampled(ctx):
     """Full-size C2 run; sampled pairs checked against the oracle."""
     W = wl.icosphere_c2()

@@ -125,6 +125,22 @@ def test_mask_skips_old_pairs():
     assert st["masked"] == nm * (nm - 1) // 2
 
 
+def test_mask_value_two_skips_pairs_among_twos():
+    """Mask 2: a new point whose pairs with the other 2-points are skipped;
+    its pairs with 0- and 1-points are tested as for 1."""
+    V, T = wl.perturbed_icosphere(2, 0.4, 8)
+    n = V.shape[0]
+    rng = np.random.default_rng(1)
+    mask = rng.choice(np.array([0, 1, 2], np.uint8), size=n, p=[0.6, 0.2, 0.2])
+    full, _ = oracle.visibility(V, V, T)
+    part, st = oracle.visibility(V, V, T, mask=mask)
+    keep = [(i, j) for i, j in full.tolist()
+            if (mask[i] or mask[j]) and not (mask[i] == 2 and mask[j] == 2)]
+    assert set(map(tuple, part.tolist())) == set(keep)
+    n0, n2 = int((mask == 0).sum()), int((mask == 2).sum())
+    assert st["masked"] == n0 * (n0 - 1) // 2 + n2 * (n2 - 1) // 2
+
+
 def test_k_order_and_pair_decode():
     n = 1000
     for k in [0, 1, 998, 999, 12345, n * (n - 1) // 2 - 1]:

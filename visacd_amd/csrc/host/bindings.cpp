@@ -1,8 +1,8 @@
 // pybind11 module with the reference's Python surface (lib/bindings.cpp:14-70):
 // VecArray3d, VecArray3i, Mesh, MeshList, Config, make_vecarray3i, set_seed,
 // config (a reference to the process-global Config), ProcessResult, process.
-// Extra names of this build: `engine` (EngineOptions) and a few host
-// geometry services used by its tests.
+// Extra names with no counterpart in the reference: `engine` (EngineOptions),
+// `last_profile` and a few host geometry services used by the tests.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -61,7 +61,7 @@ PYBIND11_MODULE(_visacd, m) {
     return d;
   });
 
-  // ---- this build's own knobs and host services
+  // ---- the engine's own knobs (no counterpart in the reference) and host services
   py::class_<vhost::EngineOptions>(m, "EngineOptions")
       .def(py::init<>())
       .def_readwrite("device", &vhost::EngineOptions::device)
@@ -70,6 +70,8 @@ PYBIND11_MODULE(_visacd, m) {
       .def_readwrite("planes", &vhost::EngineOptions::planes)
       .def_readwrite("hausdorff_resolution", &vhost::EngineOptions::hausdorff_resolution)
       .def_readwrite("sample_seed", &vhost::EngineOptions::sample_seed)
+      .def_readwrite("gpu_distance", &vhost::EngineOptions::gpu_distance)
+      .def_readwrite("skip_cap_pairs", &vhost::EngineOptions::skip_cap_pairs)
       .def_readwrite("verbose", &vhost::EngineOptions::verbose);
   m.attr("engine") = py::cast(&vhost::engine, py::return_value_policy::reference);
 

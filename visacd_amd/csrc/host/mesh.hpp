@@ -2,7 +2,7 @@
 //
 // Mirrors the reference's data model (lib/include/core.hpp:18-65,
 // lib/include/config.hpp:6-30, lib/include/process.hpp:9-15) so the pybind
-// surface is signature-identical; the algorithms are this build's own.
+// surface is signature-identical; the algorithms are written independently.
 #pragma once
 
 #include <array>
@@ -81,6 +81,10 @@ struct EngineOptions {
   int hausdorff_resolution = 3000;
   uint32_t sample_seed = 42;     // seeds subsampling and surface sampling
   bool gpu_distance = true;      // Hausdorff term on the GPU (exact point-to-mesh)
+  // pairs of two vertices of the last cut's cap lie in the cut plane: their
+  // visibility is decided by rounding alone, so they are not tested (the
+  // reference tests every pair touching a new vertex)
+  bool skip_cap_pairs = true;
   bool verbose = true;
 };
 inline EngineOptions engine;

@@ -147,7 +147,8 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
   for (int64_t i = 0; i < n; ++i) {
     const int v = part.points[i];
     for (int k = 0; k < 3; ++k) P[3 * i + k] = V[3 * (size_t)v + k];
-    mask[i] = (v < (int)part.is_new.size() && part.is_new[v]) ? 1 : 0;
+    const bool fresh = v < (int)part.is_new.size() && part.is_new[v];
+    mask[i] = fresh ? (engine.skip_cap_pairs ? 2 : 1) : 0;
     any_new |= mask[i] != 0;
   }
   const bool use_mask = with_mask;

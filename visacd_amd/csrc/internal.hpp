@@ -113,6 +113,8 @@ enum VacdSlot {
   SLOT_PARTS,
   SLOT_FAN,
   SLOT_QUEUE,
+  SLOT_SLICE,
+  SLOT_REFS,
   SLOT_COUNT_MAX
 };
 
@@ -162,6 +164,24 @@ struct vacd_bvh {
   vacd::DevBuf nodes;   // Node[max(1, n_tris-1)]
   vacd::DevBuf nodes4;  // Node4[max(1, n_tris-1)]
   vacd::DevBuf tris;    // Tri[n_tris]
+  // Contact tree: a triangle much longer than the mesh's typical edge is
+  // stored as slices across its length, each slice a leaf with its own box
+  // referring to the whole triangle (b.w = slice | slices << 8 | apex << 16
+  // as int bits), so that a long sliver no longer overlaps every query near
+  // it.  Used by the any-contact walks only (a triangle met twice cannot
+  // change a contact answer); crossing counts use nodes/tris.  n_refs = 0:
+  // no triangle needed slicing, the contact walks use nodes/tris.
+  int64_t n_refs = 0;
+  int32_t depth_refs = 0;
+  vacd::DevBuf rnodes;  // Node[max(1, n_refs-1)]
+  vacd::DevBuf rtris;   // Tri[n_refs]
+  void release() {
+    nodes.release();
+    nodes4.release();
+    tris.release();
+    rnodes.release();
+    rtris.release();
+  }
 };
 
 namespace vacd {

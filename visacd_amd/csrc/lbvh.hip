@@ -20,6 +20,10 @@
 //   7. k_depth    tree depth (longest leaf-to-root chain) for stack sizing.
 // Child boxes are widened by 2^-17 of the scene extent so that the float32
 // slab tests of the traversal stay conservative.
+// Contact tree (build_contact_tree): triangles whose longest edge exceeds
+// twice the mesh's mean longest edge are cut into up to 64 slices parallel
+// to their shortest edge; the slices' boxes go through the same steps 1-7
+// as primitives of their own, each leaf holding its whole triangle.
 #include "internal.hpp"
 
 #include <algorithm>
@@ -276,11 +280,15 @@ __device__ __forceinline__ void put_box(Node *nd, int slot, const float lo[3], c
   f[o + 5] = hi[2];
 }
 
+// blo/bhi (with order, leaf j -> primitive order[j]): the leaves' boxes
+// when they are not the boxes of their triangles (contact tree slices).
 __global__ __launch_bounds__(256) void k_refit(const Tri *__restrict__ tris, int64_t n,
                                                Node *nodes, const int32_t *__restrict__ parent_int,
                                                const int32_t *__restrict__ parent_leaf,
                                                uint32_t *flags, const unsigned *__restrict__ bounds,
-                                               float *root_box) {
+                                               float *root_box, const float *__restrict__ blo,
+                                               const float *__restrict__ bhi,
+                                               const uint32_t *__restrict__ order) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= n) return;
   float ext = 0.0f;
@@ -293,6 +301,11 @@ __global__ __launch_bounds__(256) void k_refit(const Tri *__restrict__ tris, int
                  fminf(t.a.z, fminf(t.b.z, t.c.z)) - eps};
   float hi[3] = {fmaxf(t.a.x, fmaxf(t.b.x, t.c.x)) + eps, fmaxf(t.a.y, fmaxf(t.b.y, t.c.y)) + eps,
                  fmaxf(t.a.z, fmaxf(t.b.z, t.c.z)) + eps};
+  if (blo) {
+    const size_t r = order[j];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lo[k] = blo[3 * r + k] - eps, hi[k] = bhi[3 * r + k] + eps;
+  }
   if (n == 1) {
     Node *nd = &nodes[0];
     put_box(nd, 0, lo, hi);
@@ -422,7 +435,296 @@ __global__ __launch_bounds__(256) void k_collapse4(const Node *__restrict__ node
   out[x] = r;
 }
 
+// ---------------------------------------------------------- contact tree
+constexpr int kMaxSlices = 64;
+constexpr double kLenScale = 1048576.0;  // fixed point of the edge-length sum
+
+__device__ __forceinline__ float dist3(const float *a, const float *b) {
+  const float dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+  return sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+}
+
+__device__ __forceinline__ void load_tri(const float *V, const uint32_t *T, int64_t t,
+                                         float p[3][3]) {
+#pragma unroll
+  for (int v = 0; v < 3; ++v)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p[v][k] = V[3 * (size_t)T[3 * t + v] + k];
+}
+
+// Sum over triangles of the longest edge, in fixed point so that the slice
+// length (and the tree) do not depend on the order of the additions.
+__global__ __launch_bounds__(256) void k_edge_sum(const float *__restrict__ V,
+                                                  const uint32_t *__restrict__ T, int64_t nT,
+                                                  unsigned long long *sum) {
+  unsigned long long acc = 0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nT;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    float p[3][3];
+    load_tri(V, T, t, p);
+    const float L = fmaxf(dist3(p[0], p[1]), fmaxf(dist3(p[1], p[2]), dist3(p[2], p[0])));
+    acc += (unsigned long long)((double)L * kLenScale);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(sum, acc);
+}
+
+// Slicing of a triangle: the apex is the vertex opposite its shortest edge,
+// slice s is the band of the triangle between the lines parallel to that
+// edge at s/k and (s+1)/k of the way to the apex, k = ceil(longest / h).
+struct SliceFrame {
+  float b0[3], b1[3], ap[3];
+  int apex, k;
+};
+
+__device__ __forceinline__ SliceFrame slice_frame(const float *V, const uint32_t *T, int64_t t,
+                                                  float h) {
+  float p[3][3];
+  load_tri(V, T, t, p);
+  const float e0 = dist3(p[0], p[1]), e1 = dist3(p[1], p[2]), e2 = dist3(p[2], p[0]);
+  SliceFrame f;
+  f.apex = 2;  // e0 = |p0 p1| is opposite p2, e1 opposite p0, e2 opposite p1
+  float shortest = e0;
+  if (e1 < shortest) shortest = e1, f.apex = 0;
+  if (e2 < shortest) shortest = e2, f.apex = 1;
+  const float longest = fmaxf(e0, fmaxf(e1, e2));
+  int k = h > 0.0f ? (int)ceilf(longest / h) : 1;
+  f.k = k < 1 ? 1 : (k > kMaxSlices ? kMaxSlices : k);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    f.ap[c] = p[f.apex][c];
+    f.b0[c] = p[(f.apex + 1) % 3][c];
+    f.b1[c] = p[(f.apex + 2) % 3][c];
+  }
+  return f;
+}
+
+__device__ __forceinline__ float slice_h(const unsigned long long *sum, int64_t nT) {
+  return (float)(2.0 * ((double)*sum / kLenScale) / (double)nT);
+}
+
+// counts[t] = slices of triangle t; counts[nT] = 0 (the scan then ends
+// with the total).
+__global__ __launch_bounds__(256) void k_slice_count(const float *__restrict__ V,
+                                                     const uint32_t *__restrict__ T, int64_t nT,
+                                                     const unsigned long long *sum,
+                                                     uint32_t *__restrict__ counts) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t == 0) counts[nT] = 0;
+  if (t >= nT) return;
+  counts[t] = (uint32_t)slice_frame(V, T, t, slice_h(sum, nT)).k;
+}
+
+__global__ __launch_bounds__(256) void k_slices(const float *__restrict__ V,
+                                                const uint32_t *__restrict__ T, int64_t nT,
+                                                const unsigned long long *sum,
+                                                const uint32_t *__restrict__ offsets,
+                                                float *__restrict__ blo, float *__restrict__ bhi,
+                                                uint32_t *__restrict__ ref_tri,
+                                                uint32_t *__restrict__ ref_meta) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nT) return;
+  const SliceFrame f = slice_frame(V, T, t, slice_h(sum, nT));
+  const size_t base = offsets[t];
+  for (int s = 0; s < f.k; ++s) {
+    const float u0 = (float)s / (float)f.k;
+    const float u1 = s + 1 == f.k ? 1.0f : (float)(s + 1) / (float)f.k;
+    const size_t r = base + s;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float q0 = fmaf(f.ap[c] - f.b0[c], u0, f.b0[c]), q1 = fmaf(f.ap[c] - f.b1[c], u0, f.b1[c]);
+      const float q2 = fmaf(f.ap[c] - f.b0[c], u1, f.b0[c]), q3 = fmaf(f.ap[c] - f.b1[c], u1, f.b1[c]);
+      blo[3 * r + c] = fminf(fminf(q0, q1), fminf(q2, q3));
+      bhi[3 * r + c] = fmaxf(fmaxf(q0, q1), fmaxf(q2, q3));
+    }
+    ref_tri[r] = (uint32_t)t;
+    ref_meta[r] = (uint32_t)s | ((uint32_t)f.k << 8) | ((uint32_t)f.apex << 16);
+  }
+}
+
+// bounds of box centroids and of the boxes (same layout as k_bounds)
+__global__ __launch_bounds__(256) void k_bounds_boxes(const float *__restrict__ blo,
+                                                      const float *__restrict__ bhi, int64_t n,
+                                                      unsigned *__restrict__ bounds) {
+  float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float vmin[3] = {INFINITY, INFINITY, INFINITY}, vmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float lo = blo[3 * r + k], hi = bhi[3 * r + k], cc = 0.5f * (lo + hi);
+      cmin[k] = fminf(cmin[k], cc);
+      cmax[k] = fmaxf(cmax[k], cc);
+      vmin[k] = fminf(vmin[k], lo);
+      vmax[k] = fmaxf(vmax[k], hi);
+    }
+  }
+  __shared__ float red[12][256];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    red[k][threadIdx.x] = cmin[k];
+    red[3 + k][threadIdx.x] = cmax[k];
+    red[6 + k][threadIdx.x] = vmin[k];
+    red[9 + k][threadIdx.x] = vmax[k];
+  }
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+#pragma unroll
+      for (int r = 0; r < 12; ++r) {
+        const float a = red[r][threadIdx.x], b = red[r][threadIdx.x + st];
+        red[r][threadIdx.x] = ((r / 3) % 2 == 0) ? fminf(a, b) : fmaxf(a, b);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 12) {
+    const int r = threadIdx.x;
+    const unsigned v = f2ord(red[r][0]);
+    if ((r / 3) % 2 == 0)
+      atomicMin(&bounds[r], v);
+    else
+      atomicMax(&bounds[r], v);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_morton_boxes(const float *__restrict__ blo,
+                                                      const float *__restrict__ bhi, int64_t n,
+                                                      const unsigned *__restrict__ bounds,
+                                                      uint32_t *__restrict__ keys,
+                                                      uint32_t *__restrict__ vals) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float lo = ord2f(bounds[k]), hi = ord2f(bounds[3 + k]);
+    const float cc = 0.5f * (blo[3 * r + k] + bhi[3 * r + k]);
+    const float ext = hi - lo;
+    float u = ext > 0.0f ? (cc - lo) / ext : 0.5f;
+    u = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+    q[k] = (uint32_t)u;
+  }
+  keys[r] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  vals[r] = (uint32_t)r;
+}
+
+__global__ __launch_bounds__(256) void k_leaves_refs(const float *__restrict__ V,
+                                                     const uint32_t *__restrict__ T,
+                                                     const uint32_t *__restrict__ ref_tri,
+                                                     const uint32_t *__restrict__ ref_meta,
+                                                     const uint32_t *__restrict__ order, int64_t n,
+                                                     Tri *__restrict__ tris) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t r = order[j];
+  const uint32_t t = ref_tri[r];
+  const float *a = V + 3 * (size_t)T[3 * (size_t)t];
+  const float *b = V + 3 * (size_t)T[3 * (size_t)t + 1];
+  const float *c = V + 3 * (size_t)T[3 * (size_t)t + 2];
+  Tri x;
+  x.a = make_float4(a[0], a[1], a[2], __uint_as_float(t));
+  x.b = make_float4(b[0], b[1], b[2], __uint_as_float(ref_meta[r]));
+  x.c = make_float4(c[0], c[1], c[2], 0.0f);
+  tris[j] = x;
+}
+
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// The contact tree of out (see vacd_bvh): slices of the long triangles and
+// whole short ones as primitives with their own boxes, then the steps of the
+// plain build.  Left empty (n_refs = 0) when no triangle is sliced, when the
+// slices would exceed four per triangle, or when the tree is too deep for
+// the traversal stacks.
+void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, int64_t nT,
+                        vacd_bvh *out) {
+  out->n_refs = 0;
+  out->depth_refs = 0;
+  if (nT < 2) return;
+  hipStream_t st = ctx->stream;
+  vacd::DevBuf &cb = ctx->buf[SLOT_SLICE];
+  cb.ensure(256 + 4 * (size_t)(nT + 1));
+  unsigned long long *sum = cb.as<unsigned long long>();
+  uint32_t *offsets = reinterpret_cast<uint32_t *>(cb.as<char>() + 256);
+  VACD_HIP(hipMemsetAsync(sum, 0, 8, st));
+  k_edge_sum<<<std::min(grid_for(nT, 256), 1024u), 256, 0, st>>>(d_V, d_T, nT, sum);
+  k_slice_count<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, sum, offsets);
+  k_scan_u32<<<1, 1024, 0, st>>>(offsets, nT + 1);
+  VACD_HIP(hipGetLastError());
+  uint32_t total = 0;
+  VACD_HIP(hipMemcpyAsync(&total, offsets + nT, 4, hipMemcpyDeviceToHost, st));
+  VACD_HIP(hipStreamSynchronize(st));
+  if ((int64_t)total <= nT || (int64_t)total > 4 * nT) return;
+  const int64_t R = total;
+  vacd::DevBuf &rb = ctx->buf[SLOT_REFS];
+  rb.ensure(32 * (size_t)R + 1024);
+  float *blo = rb.as<float>();
+  float *bhi = blo + 3 * R;
+  uint32_t *ref_tri = reinterpret_cast<uint32_t *>(bhi + 3 * R);
+  uint32_t *ref_meta = ref_tri + R;
+  k_slices<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, sum, offsets, blo, bhi, ref_tri,
+                                              ref_meta);
+  const int64_t ntiles = (R + kTile - 1) / kTile;
+  const size_t hist_n = (size_t)256 * ntiles;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  };
+  const size_t o_bounds = take(16 * 4), o_depth = take(4), o_k0 = take(4 * R), o_k1 = take(4 * R),
+               o_v0 = take(4 * R), o_v1 = take(4 * R), o_hist = take(4 * hist_n),
+               o_pint = take(4 * (R - 1)), o_pleaf = take(4 * R), o_flags = take(4 * (R - 1)),
+               o_root = take(6 * 4);
+  vacd::DevBuf &scratch = ctx->buf[SLOT_SORT];
+  scratch.ensure(off);
+  char *base = scratch.as<char>();
+  unsigned *bounds = reinterpret_cast<unsigned *>(base + o_bounds);
+  int32_t *depth = reinterpret_cast<int32_t *>(base + o_depth);
+  uint32_t *k0 = reinterpret_cast<uint32_t *>(base + o_k0);
+  uint32_t *k1 = reinterpret_cast<uint32_t *>(base + o_k1);
+  uint32_t *v0 = reinterpret_cast<uint32_t *>(base + o_v0);
+  uint32_t *v1 = reinterpret_cast<uint32_t *>(base + o_v1);
+  uint32_t *hist = reinterpret_cast<uint32_t *>(base + o_hist);
+  int32_t *pint = reinterpret_cast<int32_t *>(base + o_pint);
+  int32_t *pleaf = reinterpret_cast<int32_t *>(base + o_pleaf);
+  uint32_t *flags = reinterpret_cast<uint32_t *>(base + o_flags);
+  float *root = reinterpret_cast<float *>(base + o_root);
+  out->rnodes.ensure(sizeof(Node) * (size_t)(R - 1));
+  out->rtris.ensure(sizeof(Tri) * (size_t)R);
+  Node *nodes = out->rnodes.as<Node>();
+  Tri *tris = out->rtris.as<Tri>();
+  k_bounds_init<<<1, 64, 0, st>>>(bounds);
+  k_bounds_boxes<<<std::min(grid_for(R, 256), 1024u), 256, 0, st>>>(blo, bhi, R, bounds);
+  k_morton_boxes<<<grid_for(R, 256), 256, 0, st>>>(blo, bhi, R, bounds, k0, v0);
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 8 * pass;
+    k_hist<<<(unsigned)ntiles, kTile, 0, st>>>(k0, R, shift, hist, (int)ntiles);
+    k_scan_u32<<<1, 1024, 0, st>>>(hist, (int64_t)hist_n);
+    k_scatter<<<(unsigned)ntiles, kTile, 0, st>>>(k0, v0, R, shift, hist, (int)ntiles, k1, v1);
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  VACD_HIP(hipMemsetAsync(flags, 0, 4 * (size_t)(R - 1), st));
+  VACD_HIP(hipMemsetAsync(depth, 0, 4, st));
+  VACD_HIP(hipMemsetAsync(nodes, 0, sizeof(Node) * (size_t)(R - 1), st));
+  k_karras<<<grid_for(R - 1, 256), 256, 0, st>>>(k0, R, nodes, pint, pleaf);
+  k_leaves_refs<<<grid_for(R, 256), 256, 0, st>>>(d_V, d_T, ref_tri, ref_meta, v0, R, tris);
+  k_refit<<<grid_for(R, 256), 256, 0, st>>>(tris, R, nodes, pint, pleaf, flags, bounds, root, blo,
+                                            bhi, v0);
+  k_depth<<<grid_for(R, 256), 256, 0, st>>>(R, pint, pleaf, depth);
+  VACD_HIP(hipGetLastError());
+  int32_t h_depth = 0;
+  VACD_HIP(hipMemcpyAsync(&h_depth, depth, 4, hipMemcpyDeviceToHost, st));
+  VACD_HIP(hipStreamSynchronize(st));
+  if (h_depth > 64) {  // the walks' stacks hold 64 entries
+    out->rnodes.release();
+    out->rtris.release();
+    return;
+  }
+  out->n_refs = R;
+  out->depth_refs = h_depth;
+}
 
 }  // namespace
 
@@ -495,7 +797,8 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
     k_karras<<<grid_for(nT - 1, 256), 256, 0, st>>>(k0, nT, nodes, pint, pleaf);
   }
   k_leaves<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, v0, nT, tris);
-  k_refit<<<grid_for(nT, 256), 256, 0, st>>>(tris, nT, nodes, pint, pleaf, flags, bounds, root);
+  k_refit<<<grid_for(nT, 256), 256, 0, st>>>(tris, nT, nodes, pint, pleaf, flags, bounds, root,
+                                             nullptr, nullptr, nullptr);
   k_depth<<<grid_for(nT, 256), 256, 0, st>>>(nT, pint, pleaf, depth);
   out->nodes4.ensure(sizeof(Node4) * (size_t)n_int);
   k_collapse4<<<grid_for(n_int, 256), 256, 0, st>>>(nodes, n_int, out->nodes4.as<Node4>());
@@ -517,6 +820,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   out->depth = h_depth;
   out->closed = h_odd == 0;
   for (int k = 0; k < 6; ++k) out->bounds[k] = h_root[k];
+  build_contact_tree(ctx, d_V, d_T, nT, out);
 }
 
 }  // namespace vacd

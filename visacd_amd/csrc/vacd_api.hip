@@ -192,9 +192,7 @@ static int bvh_build_impl(vacd_ctx *ctx, const float *V, int64_t nV, const uint3
   });
   if (rc != VACD_OK) {
     if (b) {
-      b->nodes.release();
-      b->nodes4.release();
-      b->tris.release();
+      b->release();
       delete b;
     }
     return rc;
@@ -216,9 +214,7 @@ int vacd_bvh_build_d(vacd_ctx *ctx, const float *d_vertices, int64_t n_vertices,
 int vacd_bvh_destroy(vacd_bvh *bvh) {
   if (!bvh) return VACD_OK;
   (void)hipSetDevice(bvh->device);
-  bvh->nodes.release();
-  bvh->nodes4.release();
-  bvh->tris.release();
+  bvh->release();
   delete bvh;
   return VACD_OK;
 }
@@ -229,6 +225,13 @@ int vacd_bvh_info(const vacd_bvh *bvh, int64_t *n_tris, int32_t *depth, float *b
   if (depth) *depth = bvh->depth;
   if (bounds6)
     for (int k = 0; k < 6; ++k) bounds6[k] = bvh->bounds[k];
+  return VACD_OK;
+}
+
+int vacd_bvh_contact_info(const vacd_bvh *bvh, int64_t *n_refs, int32_t *depth) {
+  if (!bvh) return fail(nullptr, Error{VACD_ERR_INVALID, "NULL BVH"});
+  if (n_refs) *n_refs = bvh->n_refs;
+  if (depth) *depth = bvh->depth_refs;
   return VACD_OK;
 }
 

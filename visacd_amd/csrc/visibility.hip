@@ -116,7 +116,15 @@ __device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t, int &xpa
   if (ip) {
     double bound;
     const double det = o3_fast(a, b, c, s.q, bound);  // orient3d(p, b', c', q)
-    if (!(det > bound || -det > bound)) return 4;
+    // a sliced triangle (contact tree) counts its crossing in the slice
+    // holding p only: the apex's slice is the last, the other vertices' the
+    // first; unsliced records have meta 0
+    const unsigned meta = __float_as_uint(t.b.w);
+    const unsigned sl = meta & 0xffu, nsl = (meta >> 8) & 0xffu, apex = (meta >> 16) & 3u;
+    const unsigned vp = pa ? 0u : (pb ? 1u : 2u);
+    const bool counts = nsl == 0u || (vp == apex ? sl == nsl - 1u : sl == 0u);
+    if (!(det > bound || -det > bound)) return counts ? 4 : 2;
+    if (!counts) return 0;  // q is off the plane: the segment meets it at p only
     ++ninc;
     // the triangle rotated to start at p: (p, v1, v2)
     const float *v1 = pa ? b : (pb ? c : a);
@@ -494,13 +502,15 @@ struct Part {
   const Node *snodes;
   const Node4 *snodes4;
   const Tri *stris;
+  const Node *rnodes;  // self contact tree (slices of long triangles; else = snodes)
+  const Tri *rtris;
   const Node *cnodes;  // cage (NULL: parity mode)
   const Node4 *cnodes4;
   const Tri *ctris;
   int64_t n, S;
   int64_t word_begin;   // first ballot word of the part in the batch
   int64_t point_begin;  // first point of the part in the batch
-  int scap, ccap;       // node-visit caps (2 * n_tris + 64)
+  int scap, ccap, rcap;  // node-visit caps (2 * primitives + 64)
 };
 
 struct VisArgs {
@@ -558,8 +568,11 @@ __device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int 
                                           unsigned &nv, unsigned &nt, unsigned &nex) {
   make_seg(s);
   int xpar = 0, ninc = 0;
-  const int hs = any_hit<STACK, WIDE, COUNT, EXACT>(P.snodes, P.snodes4, P.stris, s, stk, P.scap,
-                                                    nv, nt, nex, xpar, ninc);
+  // binary walks use the contact tree; the 4-wide one the plain tree
+  const int hs = WIDE ? any_hit<STACK, WIDE, COUNT, EXACT>(P.snodes, P.snodes4, P.stris, s, stk,
+                                                           P.scap, nv, nt, nex, xpar, ninc)
+                      : any_hit<STACK, WIDE, COUNT, EXACT>(P.rnodes, P.snodes4, P.rtris, s, stk,
+                                                           P.rcap, nv, nt, nex, xpar, ninc);
   if (hs == 1) {
     ++n_blocked;
   } else if (hs == 3) {
@@ -613,7 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     if (k < P.S) {
       int64_t i, j;
       pair_of(k, P.n, i, j);
-      if (P.mask && !P.mask[i] && !P.mask[j]) {
+      if (P.mask && ((!P.mask[i] && !P.mask[j]) || (P.mask[i] == 2 && P.mask[j] == 2))) {
         ++n_masked;
       } else {
         Seg s;
@@ -729,8 +742,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
     volatile int *stop = &stop_all[slot];
     if (g == 0) *stop = 0;
     int xpar = 0, ninc = 0;
-    const int root = split_root<false>(P.snodes, g, LV, s);
-    const int hs = walk_any<STACK, COUNT, EXACT>(P.snodes, P.stris, s, stk, P.scap, nv, nt,
+    const int root = split_root<false>(P.rnodes, g, LV, s);
+    const int hs = walk_any<STACK, COUNT, EXACT>(P.rnodes, P.rtris, s, stk, P.rcap, nv, nt,
                                                  n_exact, xpar, ninc, root,
                                                  G > 1 ? stop : nullptr);
     const uint64_t m3 = __ballot(hs == 3) & gm, m1 = __ballot(hs == 1) & gm,
@@ -1164,6 +1177,15 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   P.word_begin = 0;
   P.point_begin = 0;
   P.scap = (int)std::min<int64_t>(2 * self->n_tris + 64, INT32_MAX);
+  if (self->n_refs > 0) {
+    P.rnodes = self->rnodes.as<Node>();
+    P.rtris = self->rtris.as<Tri>();
+    P.rcap = (int)std::min<int64_t>(2 * self->n_refs + 64, INT32_MAX);
+  } else {
+    P.rnodes = P.snodes;
+    P.rtris = P.stris;
+    P.rcap = P.scap;
+  }
   P.ccap = cage ? (int)std::min<int64_t>(2 * cage->n_tris + 64, INT32_MAX) : 0;
   return P;
 }
@@ -1295,7 +1317,7 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
     return;
   }
   const bool any_far = !cage && self->closed;
-  const int depth = std::max(self->depth, cage ? cage->depth : 0);
+  const int depth = std::max(std::max(self->depth, self->depth_refs), cage ? cage->depth : 0);
   if (any_far) {
     vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
     farb.ensure((size_t)n);
@@ -1332,7 +1354,7 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
     words += (P.S + 63) / 64;
     points += q.n;
     segs += P.S;
-    depth = std::max({depth, q.self->depth, q.cage ? q.cage->depth : 0});
+    depth = std::max({depth, q.self->depth, q.self->depth_refs, q.cage ? q.cage->depth : 0});
     any_cage |= q.cage != nullptr;
     any_far |= !q.cage && q.self->closed;
     hp[p] = P;

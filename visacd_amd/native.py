@@ -25,7 +25,8 @@ VACD_ERR_LIMIT = 4
 EXPORTED = [
     "vacd_ctx_create", "vacd_ctx_destroy", "vacd_last_error", "vacd_ctx_stream",
     "vacd_ctx_set_stream", "vacd_sync", "vacd_ctx_set_option", "vacd_ctx_timing", "vacd_bvh_build", "vacd_bvh_build_d",
-    "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_export", "vacd_visibility",
+    "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_contact_info", "vacd_bvh_export",
+    "vacd_visibility",
     "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_deferred",
     "vacd_visibility_batch_d", "vacd_score_planes",
     "vacd_score_planes_d", "vacd_max_distance", "vacd_max_distance_d", "classify_and_rate_planes",
@@ -93,6 +94,7 @@ def lib():
         L.vacd_bvh_destroy.argtypes = [vp]
         L.vacd_bvh_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i32), vp]
         L.vacd_bvh_export.argtypes = [vp, vp, vp]
+        L.vacd_bvh_contact_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i32)]
         L.vacd_visibility.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, ctypes.POINTER(i64)]
         L.vacd_visibility_d.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, vp]
         L.vacd_visibility_stats.argtypes = [vp, ctypes.POINTER(VisStats)]
@@ -271,6 +273,14 @@ class Bvh:
         if rc != VACD_OK:
             raise VacdError(rc, lib().vacd_last_error(None).decode())
         return {"n_tris": n.value, "depth": d.value, "bounds": b}
+
+    def contact_info(self):
+        """(primitives, depth) of the contact tree; (0, 0) when not built."""
+        n, d = ctypes.c_int64(), ctypes.c_int32()
+        rc = lib().vacd_bvh_contact_info(self.h, ctypes.byref(n), ctypes.byref(d))
+        if rc != VACD_OK:
+            raise VacdError(rc, lib().vacd_last_error(None).decode())
+        return int(n.value), int(d.value)
 
     def export(self):
         """(nodes (N-1, 16) float32 view, tris (N, 12) float32 view)."""
