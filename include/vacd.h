@@ -163,6 +163,14 @@ int vacd_visibility_batch_d(vacd_ctx *ctx, const vacd_part *parts, int32_t npart
                             uint32_t *d_pairs, int64_t cap, int64_t *d_count,
                             int64_t *d_part_off);
 
+/* Host form of vacd_visibility_batch_d: parts[p].pts and parts[p].new_mask
+ * are host arrays; pairs (host, at most cap) and part_off (host, nparts + 1
+ * entries, part p's edges are pairs[part_off[p] .. part_off[p+1])) are
+ * written, *count receives the total; waits.  Returns VACD_ERR_CAPACITY
+ * (after setting *count) if count > cap. */
+int vacd_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int32_t nparts,
+                          uint32_t *pairs, int64_t cap, int64_t *count, int64_t *part_off);
+
 /* ---- plane scoring ------------------------------------------------------------
  * Replaces classify_and_rate_planes + plane_point_side_and_rate_kernel
  * (lib/cuda/plane_intersections.cu:7-133).  planes: float32 [P*4] (a,b,c,d);
@@ -182,6 +190,29 @@ int vacd_score_planes_d(vacd_ctx *ctx, const float *d_planes, int64_t P,
                         const float *d_pts, int64_t n, const uint32_t *d_edges,
                         int64_t E, int64_t *d_score_q32, float *d_score_f,
                         int32_t *d_argmax);
+
+/* One part of a batched scoring call: its planes, points and edges (device
+ * pointers for the _d form, host pointers for the host form). */
+typedef struct vacd_score_part {
+  const float *planes;   /* [P][4] */
+  int64_t P;
+  const float *pts;      /* [n][3] */
+  int64_t n;
+  const uint32_t *edges; /* [E][2], indices into pts */
+  int64_t E;
+} vacd_score_part;
+
+/* Batched scoring: every part's planes on its own edges, with the same
+ * definitions as vacd_score_planes, in one set of launches (the recursion
+ * frontier of process(), lib/src/process.cpp:74-134, one call for all parts
+ * split at once).  Scores of part p start at plane_off[p] = sum of the P of
+ * parts 0..p-1 in score_q32 (nullable); argmax[p] is part p's first maximum
+ * (-1 when it has no planes).  The _d form takes device pointers, writes
+ * device outputs and does not wait; the host form copies and waits. */
+int vacd_score_planes_batch_d(vacd_ctx *ctx, const vacd_score_part *parts, int nparts,
+                              int64_t *d_score_q32, int32_t *d_argmax);
+int vacd_score_planes_batch(vacd_ctx *ctx, const vacd_score_part *parts, int nparts,
+                            int64_t *score_q32, int32_t *argmax);
 
 /* ---- surface distance ---------------------------------------------------------
  * Largest distance from m points to a triangle mesh (its BVH), the one-sided

@@ -79,3 +79,28 @@ def test_batch_matches_oracle_per_part(ctx):
         mesh_V = V if V.shape[0] > 1 else Vm
         o, _ = oracle.visibility(V, mesh_V, T, mask=m)
         np.testing.assert_array_equal(pairs[off[t]:off[t + 1]], o, err_msg=f"part {t}")
+
+
+def test_score_batch_matches_oracle_per_part(ctx):
+    """Parts of very different sizes (none, one plane, no edges, thousands)
+    in one batched scoring call: each part's Q32.32 sums and first-max argmax
+    equal the oracle's."""
+    rng = np.random.default_rng(11)
+    sizes = [(0, 50, 40), (1, 30, 25), (700, 200, 0), (3000, 900, 5000), (5000, 400, 2500),
+             (2049, 300, 777)]
+    parts = []
+    for P, n, E in sizes:
+        nrm = rng.normal(size=(P, 3))
+        nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-12)
+        planes = np.concatenate([nrm, rng.uniform(-0.5, 0.5, (P, 1))], axis=1).astype(np.float32)
+        pts = rng.uniform(-1, 1, size=(n, 3)).astype(np.float32)
+        e = np.sort(rng.integers(0, n, size=(E, 2)), axis=1).astype(np.uint32)
+        e = e[e[:, 0] != e[:, 1]]
+        parts.append((planes, pts, e[np.lexsort((e[:, 1], e[:, 0]))]))
+    print("score batch", [(p[0].shape[0], p[2].shape[0]) for p in parts], flush=True)
+    qs, am = ctx.score_planes_batch(parts)
+    for k, (pl, pt, ed) in enumerate(parts):
+        oq = oracle.score_q32(pl, pt, ed)
+        np.testing.assert_array_equal(qs[k], oq, err_msg=f"part {k}")
+        assert am[k] == (oracle.argmax_i64(oq) if pl.shape[0] else -1), k
+

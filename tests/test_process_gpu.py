@@ -87,3 +87,29 @@ def test_process_with_point_limit():
     finally:
         vis.engine.max_points = 0
         vis.config.return_parts = False
+
+
+def test_process_batched_partitions_and_is_deterministic():
+    """Batched frontier mode: every over-threshold part split per round; the
+    parts still partition the volume, the result repeats exactly, and it
+    lowers concavity like the plain loop."""
+    import visacd as vis
+
+    V, T = wl.perturbed_icosphere(3, 0.4, 1)
+    print("process batched ico3", flush=True)
+    vis.engine.mode = "batched"
+    try:
+        r1 = _run(vis, V, T, 0.02, 8, return_parts=True)
+        r2 = _run(vis, V, T, 0.02, 8, return_parts=True)
+    finally:
+        vis.engine.mode = "plain"
+        vis.config.return_parts = False
+    assert 2 <= r1.num_parts == len(r1.parts)
+    assert r1.num_parts == r2.num_parts and r1.concavity == r2.concavity
+    for a, b in zip(r1.parts, r2.parts):
+        assert [list(v) for v in a.vertices] == [list(v) for v in b.vertices]
+    vol = _volume(_mesh(vis, V, T))
+    total = sum(_volume(p) for p in r1.parts)
+    assert abs(total - vol) < 1e-6 * abs(vol) + 1e-9
+    r0 = _run(vis, V, T, 0.02, 1)
+    assert r1.concavity < r0.concavity

@@ -85,6 +85,9 @@ struct EngineOptions {
   // visibility is decided by rounding alone, so they are not tested (the
   // reference tests every pair touching a new vertex)
   bool skip_cap_pairs = true;
+  // "plain": the reference's loop, one part split per step; "batched": every
+  // part above the threshold split in the same round (up to the part budget)
+  std::string mode = "plain";
   bool verbose = true;
 };
 inline EngineOptions engine;

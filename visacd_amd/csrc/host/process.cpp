@@ -236,14 +236,14 @@ double part_concavity(const Engine &E, const Mesh &part) {
   return std::max(0.3 * rv, gpu_hausdorff(E, part, hull, engine.hausdorff_resolution, engine.sample_seed));
 }
 
-std::vector<Plane> bisector_planes(const Mesh &m, int num_planes) {
+std::vector<Plane> bisector_planes(const Mesh &m, int num_planes, std::mt19937 &rng) {
   std::vector<Plane> planes;
   const auto &E = m.intersecting_edges;
   if (E.empty()) return planes;
   std::uniform_int_distribution<> dis(0, (int)E.size() - 1);
   const double cos_tol = std::cos(5.0 * M_PI / 180.0), dist_tol = 1e-3;
   for (int i = 0; i < num_planes * 5 && (int)planes.size() < num_planes; ++i) {
-    const auto &e = E[dis(random_engine())];
+    const auto &e = E[dis(rng)];
     const Vec3 &p1 = m.vertices[e.first], &p2 = m.vertices[e.second];
     Vec3 n = p2 - p1;
     const double l = norm(n);
@@ -266,17 +266,17 @@ std::vector<Plane> bisector_planes(const Mesh &m, int num_planes) {
 // each normal (the benchmark workloads' candidate sets).  The draws are taken
 // in order first (normal x, y, z, then the offset fraction, per plane), the
 // extents are then computed in parallel.
-std::vector<Plane> sphere_planes(const Mesh &m, int num_planes) {
+std::vector<Plane> sphere_planes(const Mesh &m, int num_planes, std::mt19937 &rng) {
   std::normal_distribution<double> g(0.0, 1.0);
   std::uniform_real_distribution<double> u(0.0, 1.0);
   std::vector<Vec3> normals;
   std::vector<double> frac;
   for (int i = 0; i < num_planes; ++i) {
-    Vec3 n{g(random_engine()), g(random_engine()), g(random_engine())};
+    Vec3 n{g(rng), g(rng), g(rng)};
     const double l = norm(n);
     if (l == 0) continue;
     normals.push_back(n * (1.0 / l));
-    frac.push_back(u(random_engine()));
+    frac.push_back(u(rng));
   }
   std::vector<Vec3> pts(m.points.size());
   for (size_t i = 0; i < pts.size(); ++i) pts[i] = m.vertices[m.points[i]];
@@ -300,6 +300,151 @@ double edge_score(const Mesh &m) {
   return s;
 }
 
+
+// ---------------------------------------------------------------- batched
+// splitmix64: per-part seeds derived from the parent's, independent of the
+// order in which parts are processed.
+uint64_t mix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// Visibility of many parts in one batched call (masked: pairs touching a
+// vertex of the last cut).  Edges are appended to each part in k order.
+void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, bool with_mask,
+                          PairBuffer &buf) {
+  PhaseTimer timer("visibility");
+  const size_t np = parts.size();
+  std::vector<vacd_bvh *> bvhs(np, nullptr);
+  std::vector<std::vector<float>> P(np);
+  std::vector<std::vector<uint8_t>> M(np);
+  std::vector<vacd_part> vp;
+  std::vector<size_t> idx;
+  auto release = [&] {
+    for (auto *b : bvhs) vacd_bvh_destroy(b);
+  };
+  try {
+    for (size_t k = 0; k < np; ++k) {
+      Mesh &part = *parts[k];
+      const int64_t n = (int64_t)part.points.size();
+      if (n < 2) continue;
+      P[k].resize(3 * n);
+      M[k].assign(n, 0);
+      bool any_new = false;
+      for (int64_t i = 0; i < n; ++i) {
+        const int v = part.points[i];
+        for (int c = 0; c < 3; ++c) P[k][3 * i + c] = (float)part.vertices[v][c];
+        const bool fresh = v < (int)part.is_new.size() && part.is_new[v];
+        M[k][i] = fresh ? (engine.skip_cap_pairs ? 2 : 1) : 0;
+        any_new |= fresh;
+      }
+      if (with_mask && !any_new) continue;
+      {
+        PhaseTimer t("visibility.bvh");
+        bvhs[k] = build_bvh(E, part);
+      }
+      vp.push_back(vacd_part{P[k].data(), n, with_mask ? M[k].data() : nullptr, bvhs[k], nullptr});
+      idx.push_back(k);
+      last_profile.calls["visibility.points"] += n;
+    }
+    if (!vp.empty()) {
+      std::vector<int64_t> off(vp.size() + 1, 0);
+      int64_t cap = std::max<int64_t>(buf.cap, int64_t(1) << 22), count = 0;
+      uint32_t *pairs = nullptr;
+      for (;;) {
+        pairs = buf.get(cap);
+        const int rc = vacd_visibility_batch(E.ctx, vp.data(), (int32_t)vp.size(), pairs, cap,
+                                             &count, off.data());
+        if (rc == VACD_ERR_CAPACITY) {
+          cap = count;
+          continue;
+        }
+        E.check(rc);
+        break;
+      }
+      vacd_vis_stats st;
+      if (vacd_visibility_stats(E.ctx, &st) == VACD_OK) {
+        last_profile.calls["visibility.segments"] += st.segments;
+        last_profile.calls["visibility.walked"] += st.walked;
+        last_profile.calls["visibility.exact_pairs"] += st.exact_calls;
+      }
+      last_profile.calls["visibility.edges"] += count;
+      for (size_t j = 0; j < idx.size(); ++j) {
+        Mesh &part = *parts[idx[j]];
+        part.intersecting_edges.reserve(part.intersecting_edges.size() + (size_t)(off[j + 1] - off[j]));
+        for (int64_t e = off[j]; e < off[j + 1]; ++e)
+          part.intersecting_edges.push_back({(unsigned)part.points[pairs[2 * e]],
+                                             (unsigned)part.points[pairs[2 * e + 1]]});
+      }
+    }
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
+}
+
+// Concavities of the parts whose entry is NaN: hulls, volumes and surface
+// samples on all host threads, point-to-mesh distances on the GPU.
+void concavity_batch(const Engine &E, const MeshList &parts, std::vector<double> &conc) {
+  PhaseTimer timer("concavity");
+  std::vector<size_t> idx;
+  for (size_t i = 0; i < parts.size(); ++i)
+    if (std::isnan(conc[i])) idx.push_back(i);
+  struct Prep {
+    Mesh hull;
+    double rv = 0.0, host_value = 0.0;
+    std::vector<Vec3> sa, sb;
+    bool gpu = false;
+  };
+  std::vector<Prep> prep(idx.size());
+#pragma omp parallel for schedule(dynamic)
+  for (long j = 0; j < (long)idx.size(); ++j) {
+    const Mesh &part = parts[idx[j]];
+    Prep &q = prep[j];
+    q.hull = convex_hull(part.vertices);
+    if (!engine.gpu_distance || q.hull.triangles.empty()) {
+      q.host_value = concavity(part, q.hull, engine.hausdorff_resolution, engine.sample_seed);
+      continue;
+    }
+    q.gpu = true;
+    q.rv = std::cbrt(3.0 * std::fabs(mesh_volume(part) - mesh_volume(q.hull)) / (4.0 * M_PI));
+    std::mt19937 rng(engine.sample_seed);
+    q.sa = surface_samples(part, engine.hausdorff_resolution, rng);
+    q.sb = surface_samples(q.hull, engine.hausdorff_resolution, rng);
+  }
+  for (size_t j = 0; j < idx.size(); ++j) {
+    Prep &q = prep[j];
+    if (!q.gpu) {
+      conc[idx[j]] = q.host_value;
+      continue;
+    }
+    if (q.sa.empty() || q.sb.empty()) {
+      conc[idx[j]] = std::max(0.3 * q.rv, 1e300);
+      continue;
+    }
+    vacd_bvh *ba = build_bvh(E, parts[idx[j]]), *bb = nullptr;
+    float da = 0.f, db = 0.f;
+    int rc = VACD_OK;
+    try {
+      bb = build_bvh(E, q.hull);
+      const std::vector<float> fa = to_f32(q.sa), fb = to_f32(q.sb);
+      rc = vacd_max_distance(E.ctx, fa.data(), (int64_t)q.sa.size(), bb, &da);
+      if (rc == VACD_OK) rc = vacd_max_distance(E.ctx, fb.data(), (int64_t)q.sb.size(), ba, &db);
+    } catch (...) {
+      vacd_bvh_destroy(ba);
+      vacd_bvh_destroy(bb);
+      throw;
+    }
+    vacd_bvh_destroy(ba);
+    vacd_bvh_destroy(bb);
+    E.check(rc);
+    conc[idx[j]] = std::max(0.3 * q.rv, (double)std::max(da, db));
+  }
+}
+
 struct Driver {
   Engine E;
   std::vector<Plane> flat;
@@ -312,8 +457,8 @@ struct Driver {
     std::vector<Plane> planes;
     {
       PhaseTimer timer("planes");
-      planes = engine.planes == "sphere" ? sphere_planes(mesh, engine.num_planes)
-                                         : bisector_planes(mesh, engine.num_planes);
+      planes = engine.planes == "sphere" ? sphere_planes(mesh, engine.num_planes, random_engine())
+                                         : bisector_planes(mesh, engine.num_planes, random_engine());
     }
     const size_t ss_offset = planes.size();
     planes.insert(planes.end(), flat.begin(), flat.end());
@@ -382,12 +527,211 @@ struct Driver {
   }
 };
 
+// Batched frontier: every part above the concavity threshold (up to the part
+// budget, worst first) is split in the same round.  Planes, clips, hulls and
+// samples run on all host threads; the GPU scores all those parts in one
+// call and computes the visibility of all new parts in one call.  Each part
+// draws its planes from its own generator, seeded from its parent's seed, so
+// the result does not depend on thread scheduling.
+ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, int num_parts) {
+  const std::vector<double> bbox = normalize(mesh);
+  log_line("Preprocessing mesh (" + std::to_string(mesh.vertices.size()) + " verts), batched...");
+  mesh.is_new.clear();
+  mesh.intersecting_edges.clear();
+  mesh.points.clear();
+  if (config.use_flat_surfaces) D.flat = flat_surface_planes(mesh, config.flat_surface_min_area);
+  MeshList parts = separate_disjoint(mesh);
+  std::vector<uint64_t> seed(parts.size());
+  std::vector<Mesh *> fresh;
+  for (size_t i = 0; i < parts.size(); ++i) {
+    seed[i] = mix64((uint64_t)engine.sample_seed * 0x100000001b3ull + i);
+    choose_points(parts[i], false, (uint32_t)seed[i]);
+    fresh.push_back(&parts[i]);
+  }
+  add_visibility_batch(D.E, fresh, false, D.pair_buf);
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  std::vector<double> conc(parts.size(), nan);
+  std::vector<char> frozen(parts.size(), 0);  // could not be split
+  for (int round = 0;; ++round) {
+    const int budget = num_parts - (int)parts.size();
+    if (budget <= 0) break;
+    std::vector<std::pair<double, size_t>> cand;
+    if (config.score_mode == "edge") {
+      for (size_t i = 0; i < parts.size(); ++i)
+        if (!frozen[i] && !parts[i].intersecting_edges.empty())
+          cand.push_back({edge_score(parts[i]), i});
+    } else {
+      concavity_batch(D.E, parts, conc);
+      for (size_t i = 0; i < parts.size(); ++i)
+        if (!frozen[i] && conc[i] >= concavity_threshold) cand.push_back({conc[i], i});
+    }
+    if (cand.empty()) break;
+    std::stable_sort(cand.begin(), cand.end(),
+                     [](const auto &a, const auto &b) { return a.first > b.first; });
+    if ((int)cand.size() > budget) cand.resize(budget);
+    const size_t F = cand.size();
+    log_line("Round " + std::to_string(round + 1) + ": splitting " + std::to_string(F) +
+             " part(s) (" + std::to_string(parts.size()) + " parts total).");
+    // candidate planes, each part from its own generator
+    std::vector<std::vector<Plane>> planes(F);
+    std::vector<size_t> ss_offset(F);
+    {
+      PhaseTimer t("planes");
+#pragma omp parallel for schedule(dynamic)
+      for (long f = 0; f < (long)F; ++f) {
+        const Mesh &m = parts[cand[f].second];
+        std::mt19937 rng((uint32_t)mix64(seed[cand[f].second] ^ 0x5eedull));
+        planes[f] = engine.planes == "sphere" ? sphere_planes(m, engine.num_planes, rng)
+                                              : bisector_planes(m, engine.num_planes, rng);
+        ss_offset[f] = planes[f].size();
+        planes[f].insert(planes[f].end(), D.flat.begin(), D.flat.end());
+      }
+    }
+    // exact scores of every part's planes in one call
+    std::vector<int> best(F, -1);
+    {
+      PhaseTimer t("scoring");
+      std::vector<std::vector<float>> hp(F), pts(F);
+      std::vector<std::vector<uint32_t>> ed(F);
+      std::vector<vacd_score_part> sp;
+      std::vector<size_t> which;
+      for (size_t f = 0; f < F; ++f) {
+        const Mesh &m = parts[cand[f].second];
+        if (planes[f].empty() || m.intersecting_edges.empty()) continue;
+        hp[f].resize(4 * planes[f].size());
+        for (size_t i = 0; i < planes[f].size(); ++i) {
+          hp[f][4 * i] = (float)planes[f][i].a, hp[f][4 * i + 1] = (float)planes[f][i].b;
+          hp[f][4 * i + 2] = (float)planes[f][i].c, hp[f][4 * i + 3] = (float)planes[f][i].d;
+        }
+        pts[f] = to_f32(m.vertices);
+        ed[f].resize(2 * m.intersecting_edges.size());
+        for (size_t e = 0; e < m.intersecting_edges.size(); ++e)
+          ed[f][2 * e] = m.intersecting_edges[e].first, ed[f][2 * e + 1] = m.intersecting_edges[e].second;
+        sp.push_back(vacd_score_part{hp[f].data(), (int64_t)planes[f].size(), pts[f].data(),
+                                     (int64_t)m.vertices.size(), ed[f].data(),
+                                     (int64_t)m.intersecting_edges.size()});
+        which.push_back(f);
+      }
+      if (!sp.empty()) {
+        int64_t Ptot = 0;
+        for (auto &q : sp) Ptot += q.P;
+        std::vector<int64_t> q32(Ptot);
+        std::vector<int32_t> am(sp.size());
+        D.E.check(vacd_score_planes_batch(D.E.ctx, sp.data(), (int)sp.size(), q32.data(), am.data()));
+        int64_t off = 0;
+        for (size_t j = 0; j < which.size(); ++j) {
+          const size_t f = which[j];
+          double best_s = -1.0;
+          for (size_t i = 0; i < planes[f].size(); ++i) {
+            double sc = (double)q32[off + i] * 0x1p-32;
+            if (i >= ss_offset[f]) sc *= config.flat_surface_k;
+            if (sc > best_s) best_s = sc, best[f] = (int)i;
+          }
+          off += sp[j].P;
+        }
+      }
+    }
+    // clips, edge inheritance and connected components on all host threads
+    std::vector<MeshList> kids(F);
+    std::vector<char> split_ok(F, 0);
+    {
+      PhaseTimer t("clip");
+#pragma omp parallel for schedule(dynamic)
+      for (long f = 0; f < (long)F; ++f) {
+        if (best[f] < 0) continue;
+        const Mesh &mesh_f = parts[cand[f].second];
+        std::vector<int> mp, mn;
+        MeshList halves = clip(mesh_f, planes[f][best[f]], mp, mn);
+        if (halves.size() < 2) continue;
+        for (auto &e : mesh_f.intersecting_edges) {
+          if (mp[e.first] && mp[e.second])
+            halves[0].intersecting_edges.push_back({(unsigned)mp[e.first] - 1, (unsigned)mp[e.second] - 1});
+          if (mn[e.first] && mn[e.second])
+            halves[1].intersecting_edges.push_back({(unsigned)mn[e.first] - 1, (unsigned)mn[e.second] - 1});
+        }
+        for (int v : mesh_f.points) {
+          if (mp[v]) halves[0].points.push_back(mp[v] - 1);
+          if (mn[v]) halves[1].points.push_back(mn[v] - 1);
+        }
+        const uint64_t s0 = seed[cand[f].second];
+        uint32_t k = 0;
+        for (auto &h : halves)
+          for (Mesh &piece : separate_disjoint(h)) {
+            ++k;
+            if (piece.vertices.size() < 10) continue;
+            choose_points(piece, true, (uint32_t)mix64(s0 + k));
+            kids[f].push_back(std::move(piece));
+          }
+        split_ok[f] = 1;
+      }
+    }
+    // next generation: parts not split keep their slot, split ones are replaced
+    std::vector<char> is_cand(parts.size(), 0);
+    for (size_t f = 0; f < F; ++f) is_cand[cand[f].second] = split_ok[f] ? 1 : 2;
+    MeshList next;
+    std::vector<uint64_t> nseed;
+    std::vector<double> nconc;
+    std::vector<char> nfrozen;
+    for (size_t i = 0; i < parts.size(); ++i) {
+      if (is_cand[i] == 1) continue;
+      next.push_back(std::move(parts[i]));
+      nseed.push_back(seed[i]);
+      nconc.push_back(conc[i]);
+      nfrozen.push_back(is_cand[i] == 2 ? 1 : frozen[i]);
+    }
+    const size_t first_kid = next.size();
+    for (size_t f = 0; f < F; ++f) {
+      uint32_t k = 0;
+      for (Mesh &c : kids[f]) {
+        next.push_back(std::move(c));
+        nseed.push_back(mix64(seed[cand[f].second] * 31 + (++k)));
+        nconc.push_back(nan);
+        nfrozen.push_back(0);
+      }
+    }
+    parts = std::move(next);
+    seed = std::move(nseed);
+    conc = std::move(nconc);
+    frozen = std::move(nfrozen);
+    fresh.clear();
+    for (size_t i = first_kid; i < parts.size(); ++i) fresh.push_back(&parts[i]);
+    add_visibility_batch(D.E, fresh, true, D.pair_buf);
+    bool any = false;
+    for (size_t f = 0; f < F; ++f) any |= split_ok[f] != 0;
+    if (!any) break;
+  }
+  log_line("Computing convex hulls for " + std::to_string(parts.size()) + " parts...");
+  PhaseTimer final_timer("final_hulls");
+  MeshList hulls(parts.size());
+#pragma omp parallel for schedule(dynamic)
+  for (long i = 0; i < (long)parts.size(); ++i) hulls[i] = convex_hull(parts[i].vertices);
+  double final_h = 0.0;
+  for (size_t i = 0; i < parts.size(); ++i) {
+    const bool gpu = engine.gpu_distance && !hulls[i].triangles.empty();
+    final_h = std::max(final_h, gpu ? gpu_hausdorff(D.E, parts[i], hulls[i], 10000, engine.sample_seed)
+                                    : hausdorff_boundary(parts[i], hulls[i], 10000, engine.sample_seed));
+  }
+  for (auto &h : hulls) unnormalize(h, bbox);
+  for (auto &p : parts) unnormalize(p, bbox);
+  ProcessResult r;
+  r.parts = config.return_parts ? parts : hulls;
+  r.concavity = final_h;
+  r.num_parts = (int)r.parts.size();
+  std::ostringstream o;
+  o.setf(std::ios::fixed);
+  o.precision(4);
+  o << "Done. parts=" << r.num_parts << "  concavity=" << final_h;
+  log_line(o.str());
+  return r;
+}
+
 }  // namespace
 
 ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
   last_profile.clear();
   PhaseTimer total("total");
   Driver D(engine.device);
+  if (engine.mode == "batched") return process_batched(D, std::move(mesh), concavity_threshold, num_parts);
   const std::vector<double> bbox = normalize(mesh);
   log_line("Preprocessing mesh (" + std::to_string(mesh.vertices.size()) + " verts)...");
   mesh.is_new.clear();

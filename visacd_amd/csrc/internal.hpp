@@ -115,6 +115,7 @@ enum VacdSlot {
   SLOT_QUEUE,
   SLOT_SLICE,
   SLOT_REFS,
+  SLOT_SCORE_TABLES,
   SLOT_COUNT_MAX
 };
 
@@ -133,9 +134,35 @@ struct vacd_ctx {
   int lanes_per_pair_log2 = 0;  // queued-pair walks split over 2^k lanes (k <= 4)
   // recorded (start, stop) event pairs per timed kernel kind
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+  // pinned host staging for descriptor tables copied to the device; a copy
+  // captured into a graph reads it again at every replay
+  void *pinned = nullptr;
+  size_t pinned_bytes = 0;
+  hipEvent_t pinned_free = nullptr;  // recorded after the last copy out of it
 };
 
 namespace vacd {
+// Pinned staging buffer of at least `bytes`, once the copies enqueued from it
+// earlier have completed.
+inline void *pinned_stage(vacd_ctx *ctx, size_t bytes) {
+  if (ctx->pinned_free) VACD_HIP(hipEventSynchronize(ctx->pinned_free));
+  if (bytes > ctx->pinned_bytes) {
+    if (ctx->pinned) VACD_HIP(hipHostFree(ctx->pinned));
+    ctx->pinned = nullptr;
+    ctx->pinned_bytes = 0;
+    const size_t want = bytes < 4096 ? 4096 : bytes;
+    VACD_HIP(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+    ctx->pinned_bytes = want;
+  }
+  return ctx->pinned;
+}
+// Copies `bytes` from the pinned buffer to dst on the context stream.
+inline void pinned_upload(vacd_ctx *ctx, void *dst, size_t bytes) {
+  VACD_HIP(hipMemcpyAsync(dst, ctx->pinned, bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (!ctx->pinned_free) VACD_HIP(hipEventCreateWithFlags(&ctx->pinned_free, hipEventDisableTiming));
+  VACD_HIP(hipEventRecord(ctx->pinned_free, ctx->stream));
+}
+
 enum TimedKind { TIMED_VIS = 0, TIMED_SCORE = 1, TIMED_BVH = 2 };
 // Records a start event when timing is on; returns the index to close.
 inline void timed_begin(vacd_ctx *ctx, int kind) {
@@ -207,5 +234,11 @@ void launch_max_distance(vacd_ctx *ctx, const float *d_pts, int64_t m, const vac
 void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P,
                     const float *d_pts, int64_t n, const uint32_t *d_edges,
                     int64_t E, int64_t *d_q32, float *d_f, int32_t *d_argmax);
+void launch_scoring_batch(vacd_ctx *ctx, const vacd_score_part *parts, int nparts,
+                          int64_t *d_q32, int32_t *d_argmax);
+
+// Fills n 64-bit words with v on the context stream (a kernel, so that it
+// may be captured into a graph).
+void fill_u64(vacd_ctx *ctx, void *p, int64_t n, unsigned long long v);
 
 }  // namespace vacd

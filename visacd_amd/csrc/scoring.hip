@@ -19,6 +19,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace vacd {
 namespace {
@@ -37,11 +38,9 @@ struct alignas(16) EdgeRec {
 // in k order come in runs of one first vertex); lengths stay below 2^36.
 constexpr unsigned kSameFirst = 0x40000000u;
 
-__global__ __launch_bounds__(256) void k_edge_prep(const float *__restrict__ pts, int64_t n,
-                                                   const uint32_t *__restrict__ edges, int64_t E,
-                                                   EdgeRec *__restrict__ rec) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= E) return;
+// Record of edge e of one part (pts[n], edges[E]).
+__device__ __forceinline__ EdgeRec edge_record(const float *__restrict__ pts, int64_t n,
+                                               const uint32_t *__restrict__ edges, int64_t e) {
   const uint2 ij = reinterpret_cast<const uint2 *>(edges)[e];
   EdgeRec r;
   if ((int64_t)ij.x >= n || (int64_t)ij.y >= n) {
@@ -64,7 +63,15 @@ __global__ __launch_bounds__(256) void k_edge_prep(const float *__restrict__ pts
     r.b = make_float4(y1, z1, __int_as_float((int)(q & 0xffffffffll)),
                       __uint_as_float((unsigned)(q >> 32) | (same_first ? kSameFirst : 0u)));
   }
-  rec[e] = r;
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_edge_prep(const float *__restrict__ pts, int64_t n,
+                                                   const uint32_t *__restrict__ edges, int64_t E,
+                                                   EdgeRec *__restrict__ rec) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  rec[e] = edge_record(pts, n, edges, e);
 }
 
 // Planes held per lane, oriented per run of edges that share a first
@@ -85,23 +92,24 @@ __device__ __forceinline__ float plane_val(float a, float b, float c, float d, f
   return fmaf(c, z, fmaf(b, y, a * x)) + d;
 }
 
-// Edge records are read with wave-uniform addresses (every lane of a block
-// walks the same chunk), so they arrive through scalar loads and feed the
-// vector ALU as scalar operands; no LDS staging is needed.
-__global__ __launch_bounds__(kBlock) void k_score(const float4 *__restrict__ planes, int64_t P,
-                                                  const EdgeRec *__restrict__ rec, int64_t E,
-                                                  int64_t chunk, unsigned long long *__restrict__ acc) {
+// One block's work: planes [tile * 2048, +2048) of a plane set (P planes;
+// its sums at acc[0..P)) against edge records [e0, e1).  Edge records are
+// read with wave-uniform addresses (every lane of a block walks the same
+// chunk), so they arrive through scalar loads and feed the vector ALU as
+// scalar operands; no LDS staging is needed.
+__device__ __forceinline__ void score_tile(const float4 *__restrict__ planes, int64_t P,
+                                           int64_t tile, const EdgeRec *__restrict__ rec,
+                                           int64_t e0, int64_t e1,
+                                           unsigned long long *__restrict__ acc) {
   Lanes<kPPT> L;
 #pragma unroll
   for (int r = 0; r < kPPT; ++r) {
-    const int64_t p = (int64_t)blockIdx.x * kPlanesPerBlock + r * kBlock + threadIdx.x;
+    const int64_t p = tile * kPlanesPerBlock + r * kBlock + threadIdx.x;
     const float4 pl = p < P ? planes[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     L.pa[r] = pl.x, L.pb[r] = pl.y, L.pc[r] = pl.z, L.pd[r] = pl.w;
     L.fa[r] = 0.f, L.fb[r] = 0.f, L.fc[r] = 0.f, L.fd[r] = 1.f;
     L.sum[r] = 0ull;
   }
-  const int64_t e0 = (int64_t)blockIdx.y * chunk;
-  const int64_t e1 = e0 + chunk < E ? e0 + chunk : E;
   if (e0 >= e1) return;
   EdgeRec nxt = rec[e0];
   for (int64_t e = e0; e < e1; ++e) {
@@ -130,14 +138,23 @@ __global__ __launch_bounds__(kBlock) void k_score(const float4 *__restrict__ pla
   }
 #pragma unroll
   for (int r = 0; r < kPPT; ++r) {
-    const int64_t p = (int64_t)blockIdx.x * kPlanesPerBlock + r * kBlock + threadIdx.x;
+    const int64_t p = tile * kPlanesPerBlock + r * kBlock + threadIdx.x;
     if (p < P && L.sum[r]) atomicAdd(&acc[p], L.sum[r]);
   }
 }
 
-__global__ __launch_bounds__(1024) void k_finalize(const unsigned long long *__restrict__ acc,
-                                                   int64_t P, int64_t *__restrict__ q32,
-                                                   float *__restrict__ f, int32_t *__restrict__ amax) {
+__global__ __launch_bounds__(kBlock) void k_score(const float4 *__restrict__ planes, int64_t P,
+                                                  const EdgeRec *__restrict__ rec, int64_t E,
+                                                  int64_t chunk, unsigned long long *__restrict__ acc) {
+  const int64_t e0 = (int64_t)blockIdx.y * chunk;
+  const int64_t e1 = e0 + chunk < E ? e0 + chunk : E;
+  score_tile(planes, P, blockIdx.x, rec, e0, e1, acc);
+}
+
+// q32 / float views of P sums and their first maximum (a 1024-thread block).
+__device__ __forceinline__ void finalize_range(const unsigned long long *__restrict__ acc,
+                                               int64_t P, int64_t *__restrict__ q32,
+                                               float *__restrict__ f, int32_t *__restrict__ amax) {
   __shared__ long long bv[1024];
   __shared__ int bi[1024];
   long long best = -1;
@@ -165,6 +182,66 @@ __global__ __launch_bounds__(1024) void k_finalize(const unsigned long long *__r
   if (threadIdx.x == 0 && amax) *amax = bi[0];
 }
 
+__global__ __launch_bounds__(1024) void k_finalize(const unsigned long long *__restrict__ acc,
+                                                   int64_t P, int64_t *__restrict__ q32,
+                                                   float *__restrict__ f, int32_t *__restrict__ amax) {
+  finalize_range(acc, P, q32, f, amax);
+}
+
+// ---------------------------------------------------------------- batch
+struct ScorePartDev {
+  const float4 *planes;
+  int64_t P, plane_base;  // plane_base: index of its first plane in the batch
+  const float *pts;
+  int64_t n;
+  const uint32_t *edges;
+  int64_t E, rec_base;    // rec_base: index of its first edge record
+};
+struct ScoreBlock {
+  int part, tile;
+  int64_t e0, e1;  // edge records of the batch
+};
+
+__global__ __launch_bounds__(256) void k_edge_prep_batch(const ScorePartDev *__restrict__ parts,
+                                                         int nparts, int64_t Etot,
+                                                         EdgeRec *__restrict__ rec) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= Etot) return;
+  int lo = 0, hi = nparts - 1;  // last part with rec_base <= e
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (parts[mid].rec_base <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const ScorePartDev &P = parts[lo];
+  rec[e] = edge_record(P.pts, P.n, P.edges, e - P.rec_base);
+}
+
+__global__ __launch_bounds__(kBlock) void k_score_batch(const ScorePartDev *__restrict__ parts,
+                                                        const ScoreBlock *__restrict__ blocks,
+                                                        const EdgeRec *__restrict__ rec,
+                                                        unsigned long long *__restrict__ acc) {
+  const ScoreBlock b = blocks[blockIdx.x];
+  const ScorePartDev &P = parts[b.part];
+  score_tile(P.planes, P.P, b.tile, rec, b.e0, b.e1, acc + P.plane_base);
+}
+
+__global__ __launch_bounds__(1024) void k_finalize_batch(const unsigned long long *__restrict__ acc,
+                                                         const ScorePartDev *__restrict__ parts,
+                                                         int64_t *__restrict__ q32,
+                                                         int32_t *__restrict__ amax) {
+  const ScorePartDev &P = parts[blockIdx.x];
+  finalize_range(acc + P.plane_base, P.P, q32 ? q32 + P.plane_base : nullptr, nullptr,
+                 amax + blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *__restrict__ p, int64_t n,
+                                                  unsigned long long v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 }  // namespace
 
 void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P, const float *d_pts, int64_t n,
@@ -174,13 +251,19 @@ void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P, const float
   VACD_REQUIRE(P >= 0 && P < (int64_t(1) << 31), VACD_ERR_LIMIT, "scoring needs P < 2^31");
   VACD_REQUIRE(E >= 0 && E < (int64_t(1) << 40), VACD_ERR_LIMIT, "scoring needs E < 2^40");
   if (P == 0) {
-    if (d_argmax) VACD_HIP(hipMemsetAsync(d_argmax, 0xff, sizeof(int32_t), st));
+    if (d_argmax) {
+      // -1: the first 4 bytes of an all-ones word written into the 8-byte slot
+      vacd::DevBuf &tmp = ctx->buf[SLOT_ARGMAX];
+      tmp.ensure(8);
+      fill_u64(ctx, tmp.ptr, 1, ~0ull);
+      VACD_HIP(hipMemcpyAsync(d_argmax, tmp.ptr, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    }
     return;
   }
   vacd::DevBuf &accb = ctx->buf[SLOT_SCORES];
   accb.ensure(8 * (size_t)P);
   unsigned long long *acc = accb.as<unsigned long long>();
-  VACD_HIP(hipMemsetAsync(acc, 0, 8 * (size_t)P, st));
+  fill_u64(ctx, acc, P, 0ull);
   if (E > 0) {
     vacd::DevBuf &recb = ctx->buf[SLOT_EREC];
     recb.ensure(sizeof(EdgeRec) * (size_t)E);
@@ -200,6 +283,79 @@ void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P, const float
     timed_end(ctx, TIMED_SCORE);
   }
   k_finalize<<<1, 1024, 0, st>>>(acc, P, d_q32, d_f, d_argmax);
+  VACD_HIP(hipGetLastError());
+}
+
+void fill_u64(vacd_ctx *ctx, void *p, int64_t n, unsigned long long v) {
+  if (n <= 0) return;
+  const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  k_fill_u64<<<grid, 256, 0, ctx->stream>>>(static_cast<unsigned long long *>(p), n, v);
+  VACD_HIP(hipGetLastError());
+}
+
+void launch_scoring_batch(vacd_ctx *ctx, const vacd_score_part *parts, int nparts,
+                          int64_t *d_q32, int32_t *d_argmax) {
+  hipStream_t st = ctx->stream;
+  VACD_REQUIRE(nparts >= 0, VACD_ERR_INVALID, "negative part count");
+  if (nparts == 0) return;
+  VACD_REQUIRE(d_argmax, VACD_ERR_INVALID, "NULL argmax");
+  std::vector<ScorePartDev> dev(nparts);
+  int64_t Ptot = 0, Etot = 0, work = 0;
+  for (int p = 0; p < nparts; ++p) {
+    const vacd_score_part &q = parts[p];
+    VACD_REQUIRE(q.P >= 0 && q.n >= 0 && q.E >= 0, VACD_ERR_INVALID, "negative size");
+    VACD_REQUIRE(q.P == 0 || (q.planes && ((uintptr_t)q.planes % 16) == 0), VACD_ERR_INVALID,
+                 "planes must be 16-byte aligned");
+    VACD_REQUIRE(q.E == 0 || (q.pts && q.edges && ((uintptr_t)q.edges % 8) == 0),
+                 VACD_ERR_INVALID, "edges must be 8-byte aligned");
+    VACD_REQUIRE(q.P < (int64_t(1) << 31), VACD_ERR_LIMIT, "scoring needs P < 2^31 per part");
+    dev[p] = ScorePartDev{reinterpret_cast<const float4 *>(q.planes), q.P, Ptot, q.pts, q.n,
+                          q.edges, q.E, Etot};
+    Ptot += q.P;
+    Etot += q.E;
+    work += ((q.P + kPlanesPerBlock - 1) / kPlanesPerBlock) * q.E;
+  }
+  VACD_REQUIRE(Etot < (int64_t(1) << 40), VACD_ERR_LIMIT, "scoring needs E < 2^40");
+  // one edge-chunk length for the whole batch: about 4096 blocks in total
+  int64_t chunk = ctx->score_chunk > 0 ? ctx->score_chunk : (work + 4095) / 4096;
+  chunk = std::max<int64_t>(kEdgeQuantum, (chunk + kEdgeQuantum - 1) / kEdgeQuantum * kEdgeQuantum);
+  std::vector<ScoreBlock> blocks;
+  for (int p = 0; p < nparts; ++p) {
+    const ScorePartDev &q = dev[p];
+    const int64_t ptiles = (q.P + kPlanesPerBlock - 1) / kPlanesPerBlock;
+    for (int64_t t = 0; t < ptiles; ++t)
+      for (int64_t e = 0; e < q.E; e += chunk)
+        blocks.push_back(ScoreBlock{p, (int)t, q.rec_base + e, q.rec_base + std::min(e + chunk, q.E)});
+  }
+  VACD_REQUIRE(blocks.size() < (size_t(1) << 31), VACD_ERR_LIMIT, "too many scoring blocks");
+  // tables: parts then blocks, staged through pinned memory
+  const size_t parts_bytes = sizeof(ScorePartDev) * (size_t)nparts;
+  const size_t blocks_off = (parts_bytes + 255) & ~size_t(255);
+  const size_t tables_bytes = blocks_off + sizeof(ScoreBlock) * blocks.size();
+  char *host = static_cast<char *>(pinned_stage(ctx, tables_bytes));
+  std::memcpy(host, dev.data(), parts_bytes);
+  if (!blocks.empty()) std::memcpy(host + blocks_off, blocks.data(), sizeof(ScoreBlock) * blocks.size());
+  vacd::DevBuf &tb = ctx->buf[SLOT_SCORE_TABLES];
+  tb.ensure(tables_bytes);
+  pinned_upload(ctx, tb.ptr, tables_bytes);
+  const ScorePartDev *d_parts = tb.as<ScorePartDev>();
+  const ScoreBlock *d_blocks = reinterpret_cast<const ScoreBlock *>(tb.as<char>() + blocks_off);
+  vacd::DevBuf &accb = ctx->buf[SLOT_SCORES];
+  accb.ensure(8 * (size_t)std::max<int64_t>(Ptot, 1));
+  unsigned long long *acc = accb.as<unsigned long long>();
+  fill_u64(ctx, acc, Ptot, 0ull);
+  if (Etot > 0) {
+    vacd::DevBuf &recb = ctx->buf[SLOT_EREC];
+    recb.ensure(sizeof(EdgeRec) * (size_t)Etot);
+    EdgeRec *rec = recb.as<EdgeRec>();
+    k_edge_prep_batch<<<(unsigned)((Etot + 255) / 256), 256, 0, st>>>(d_parts, nparts, Etot, rec);
+    if (!blocks.empty()) {
+      timed_begin(ctx, TIMED_SCORE);
+      k_score_batch<<<(unsigned)blocks.size(), kBlock, 0, st>>>(d_parts, d_blocks, rec, acc);
+      timed_end(ctx, TIMED_SCORE);
+    }
+  }
+  k_finalize_batch<<<(unsigned)nparts, 1024, 0, st>>>(acc, d_parts, d_q32, d_argmax);
   VACD_HIP(hipGetLastError());
 }
 

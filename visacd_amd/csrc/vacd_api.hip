@@ -85,6 +85,8 @@ int vacd_ctx_destroy(vacd_ctx *ctx) {
       (void)hipEventDestroy(pr.first);
       (void)hipEventDestroy(pr.second);
     }
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->pinned_free) (void)hipEventDestroy(ctx->pinned_free);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return VACD_OK;
@@ -289,6 +291,65 @@ int vacd_visibility_batch_d(vacd_ctx *ctx, const vacd_part *parts, int32_t npart
   });
 }
 
+int vacd_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int32_t nparts,
+                          uint32_t *pairs, int64_t cap, int64_t *count, int64_t *part_off) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  int64_t total = 0;
+  int rc = guarded(ctx, [&] {
+    VACD_REQUIRE(parts && nparts > 0 && count && part_off, VACD_ERR_INVALID, "NULL argument");
+    VACD_REQUIRE(cap >= 0 && (cap == 0 || pairs), VACD_ERR_INVALID, "bad pair buffer");
+    set_device(ctx);
+    // points and masks of every part in one device buffer
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t bytes = 0;
+    for (int p = 0; p < nparts; ++p) {
+      VACD_REQUIRE(parts[p].n >= 0 && parts[p].n <= (int64_t(1) << 26), VACD_ERR_LIMIT,
+                   "visibility needs n <= 2^26");
+      VACD_REQUIRE(parts[p].n < 2 || parts[p].pts, VACD_ERR_INVALID, "NULL points");
+      bytes += up(12 * (size_t)parts[p].n) + (parts[p].new_mask ? up((size_t)parts[p].n) : 0);
+    }
+    vacd::DevBuf &sb = ctx->buf[SLOT_PTS];
+    sb.ensure(bytes + 256);
+    std::vector<vacd_part> dev(parts, parts + nparts);
+    size_t off = 0;
+    for (int p = 0; p < nparts; ++p) {
+      const vacd_part &q = parts[p];
+      char *dp = sb.as<char>() + off;
+      off += up(12 * (size_t)q.n);
+      if (q.n) VACD_HIP(hipMemcpyAsync(dp, q.pts, 12 * (size_t)q.n, hipMemcpyHostToDevice, ctx->stream));
+      dev[p].pts = reinterpret_cast<const float *>(dp);
+      if (q.new_mask) {
+        char *dm = sb.as<char>() + off;
+        off += up((size_t)q.n);
+        if (q.n) VACD_HIP(hipMemcpyAsync(dm, q.new_mask, (size_t)q.n, hipMemcpyHostToDevice, ctx->stream));
+        dev[p].new_mask = reinterpret_cast<const uint8_t *>(dm);
+      }
+    }
+    vacd::DevBuf &pb = ctx->buf[SLOT_PAIRS];
+    pb.ensure(8 * (size_t)(cap > 0 ? cap : 1));
+    vacd::DevBuf &cb = ctx->buf[SLOT_COUNT];
+    cb.ensure(8 * (size_t)(nparts + 2));
+    int64_t *dcount = cb.as<int64_t>(), *doff = dcount + 1;
+    vacd::launch_visibility_batch(ctx, dev.data(), nparts, pb.as<uint32_t>(), cap, dcount, doff);
+    unsigned long long bad = 0;
+    VACD_HIP(hipMemcpyAsync(&total, dcount, 8, hipMemcpyDeviceToHost, ctx->stream));
+    VACD_HIP(hipMemcpyAsync(part_off, doff, 8 * (size_t)(nparts + 1), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    VACD_HIP(hipMemcpyAsync(&bad, ctx->buf[SLOT_STATS].as<unsigned long long>() + 4, sizeof(bad),
+                            hipMemcpyDeviceToHost, ctx->stream));
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+    VACD_REQUIRE(bad == 0, VACD_ERR_HIP, "visibility: BVH walk exceeded its node budget");
+    const int64_t ncopy = total < cap ? total : cap;
+    if (ncopy > 0)
+      VACD_HIP(hipMemcpy(pairs, pb.ptr, 8 * (size_t)ncopy, hipMemcpyDeviceToHost));
+  });
+  if (rc != VACD_OK) return rc;
+  *count = total;
+  if (total > cap)
+    return fail(ctx, Error{VACD_ERR_CAPACITY, "visibility: more edges than the pair buffer holds"});
+  return VACD_OK;
+}
+
 int vacd_visibility(vacd_ctx *ctx, const float *pts, int64_t n, const uint8_t *new_mask,
                     const vacd_bvh *self, const vacd_bvh *cage, uint32_t *pairs, int64_t cap,
                     int64_t *count) {
@@ -387,6 +448,71 @@ int vacd_score_planes_d(vacd_ctx *ctx, const float *d_planes, int64_t P, const f
       ed = b.as<uint32_t>();
     }
     vacd::launch_scoring(ctx, pl, P, d_pts, n, ed, E, d_score_q32, d_score_f, d_argmax);
+  });
+}
+
+int vacd_score_planes_batch_d(vacd_ctx *ctx, const vacd_score_part *parts, int nparts,
+                              int64_t *d_score_q32, int32_t *d_argmax) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(nparts == 0 || parts, VACD_ERR_INVALID, "NULL parts");
+    set_device(ctx);
+    vacd::launch_scoring_batch(ctx, parts, nparts, d_score_q32, d_argmax);
+  });
+}
+
+int vacd_score_planes_batch(vacd_ctx *ctx, const vacd_score_part *parts, int nparts,
+                            int64_t *score_q32, int32_t *argmax) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(nparts >= 0 && (nparts == 0 || (parts && argmax)), VACD_ERR_INVALID,
+                 "NULL argument");
+    if (nparts == 0) return;
+    set_device(ctx);
+    // one device buffer: planes | points | edges of every part, each aligned
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t bytes = 0;
+    int64_t Ptot = 0;
+    for (int p = 0; p < nparts; ++p) {
+      const vacd_score_part &q = parts[p];
+      VACD_REQUIRE(q.P >= 0 && q.n >= 0 && q.E >= 0, VACD_ERR_INVALID, "negative size");
+      VACD_REQUIRE((q.P == 0 || q.planes) && (q.E == 0 || (q.pts && q.edges)), VACD_ERR_INVALID,
+                   "NULL part arrays");
+      for (int64_t e = 0; e < 2 * q.E; ++e)
+        VACD_REQUIRE((int64_t)q.edges[e] < q.n, VACD_ERR_INVALID, "edge index out of range");
+      bytes += up(16 * (size_t)q.P) + up(12 * (size_t)q.n) + up(8 * (size_t)q.E);
+      Ptot += q.P;
+    }
+    vacd::DevBuf &b = ctx->buf[SLOT_EDGES];
+    b.ensure(bytes + 256);
+    char *base = b.as<char>();
+    std::vector<vacd_score_part> dev(nparts);
+    size_t off = 0;
+    for (int p = 0; p < nparts; ++p) {
+      const vacd_score_part &q = parts[p];
+      dev[p] = q;
+      char *pl = base + off;
+      off += up(16 * (size_t)q.P);
+      char *pt = base + off;
+      off += up(12 * (size_t)q.n);
+      char *ed = base + off;
+      off += up(8 * (size_t)q.E);
+      if (q.P) VACD_HIP(hipMemcpyAsync(pl, q.planes, 16 * (size_t)q.P, hipMemcpyHostToDevice, ctx->stream));
+      if (q.n) VACD_HIP(hipMemcpyAsync(pt, q.pts, 12 * (size_t)q.n, hipMemcpyHostToDevice, ctx->stream));
+      if (q.E) VACD_HIP(hipMemcpyAsync(ed, q.edges, 8 * (size_t)q.E, hipMemcpyHostToDevice, ctx->stream));
+      dev[p].planes = reinterpret_cast<const float *>(pl);
+      dev[p].pts = reinterpret_cast<const float *>(pt);
+      dev[p].edges = reinterpret_cast<const uint32_t *>(ed);
+    }
+    vacd::DevBuf &qb = ctx->buf[SLOT_SCORES_F];
+    qb.ensure(8 * (size_t)std::max<int64_t>(Ptot, 1) + 4 * (size_t)nparts + 256);
+    int64_t *dq = qb.as<int64_t>();
+    int32_t *da = reinterpret_cast<int32_t *>(qb.as<char>() + ((8 * (size_t)std::max<int64_t>(Ptot, 1) + 255) & ~size_t(255)));
+    vacd::launch_scoring_batch(ctx, dev.data(), nparts, dq, da);
+    if (score_q32 && Ptot)
+      VACD_HIP(hipMemcpyAsync(score_q32, dq, 8 * (size_t)Ptot, hipMemcpyDeviceToHost, ctx->stream));
+    VACD_HIP(hipMemcpyAsync(argmax, da, 4 * (size_t)nparts, hipMemcpyDeviceToHost, ctx->stream));
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
 

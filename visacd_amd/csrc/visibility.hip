@@ -30,6 +30,7 @@
 #include "predicates.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace vacd {
 namespace {
@@ -1293,7 +1294,7 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
 void reset_stats(vacd_ctx *ctx) {
   vacd::DevBuf &stats = ctx->buf[SLOT_STATS];
   stats.ensure(16 * sizeof(unsigned long long));
-  VACD_HIP(hipMemsetAsync(stats.ptr, 0, 16 * sizeof(unsigned long long), ctx->stream));
+  fill_u64(ctx, stats.ptr, 16, 0ull);  // a kernel: the pass may be captured in a graph
 }
 
 }  // namespace
@@ -1313,7 +1314,7 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
   a.stats = ctx->buf[SLOT_STATS].as<unsigned long long>();
   ctx->last_segments = a.one.S;
   if (a.nwords == 0) {
-    VACD_HIP(hipMemsetAsync(d_count, 0, sizeof(int64_t), ctx->stream));
+    fill_u64(ctx, d_count, 1, 0ull);
     return;
   }
   const bool any_far = !cage && self->closed;
@@ -1375,10 +1376,13 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
         }
       }
   }
+  // the part table goes through pinned memory, which a captured copy reads
+  // again at every replay
   vacd::DevBuf &pb = ctx->buf[SLOT_PARTS];
   pb.ensure(sizeof(Part) * (size_t)nparts);
-  VACD_HIP(hipMemcpyAsync(pb.ptr, hp.data(), sizeof(Part) * (size_t)nparts, hipMemcpyHostToDevice,
-                          ctx->stream));
+  std::memcpy(pinned_stage(ctx, sizeof(Part) * (size_t)nparts), hp.data(),
+              sizeof(Part) * (size_t)nparts);
+  pinned_upload(ctx, pb.ptr, sizeof(Part) * (size_t)nparts);
   VisArgs a;
   a.one = hp[0];
   a.parts = pb.as<Part>();
@@ -1387,8 +1391,8 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
   a.npoints = points;
   a.stats = ctx->buf[SLOT_STATS].as<unsigned long long>();
   if (words == 0) {
-    VACD_HIP(hipMemsetAsync(d_count, 0, sizeof(int64_t), ctx->stream));
-    VACD_HIP(hipMemsetAsync(d_part_off, 0, sizeof(int64_t) * (size_t)(nparts + 1), ctx->stream));
+    fill_u64(ctx, d_count, 1, 0ull);
+    fill_u64(ctx, d_part_off, nparts + 1, 0ull);
     return;
   }
   run(ctx, a, true, any_cage, any_far, depth, d_pairs, cap, d_count, d_part_off);

@@ -28,8 +28,9 @@ EXPORTED = [
     "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_contact_info", "vacd_bvh_export",
     "vacd_visibility",
     "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_deferred",
-    "vacd_visibility_batch_d", "vacd_score_planes",
-    "vacd_score_planes_d", "vacd_max_distance", "vacd_max_distance_d", "classify_and_rate_planes",
+    "vacd_visibility_batch_d", "vacd_visibility_batch", "vacd_score_planes",
+    "vacd_score_planes_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
+    "vacd_max_distance", "vacd_max_distance_d", "classify_and_rate_planes",
 ]
 
 
@@ -49,6 +50,11 @@ class VisStats(ctypes.Structure):
 class Part(ctypes.Structure):
     _fields_ = [("pts", ctypes.c_void_p), ("n", ctypes.c_int64), ("new_mask", ctypes.c_void_p),
                 ("self", ctypes.c_void_p), ("cage", ctypes.c_void_p)]
+
+
+class ScorePart(ctypes.Structure):
+    _fields_ = [("planes", ctypes.c_void_p), ("P", ctypes.c_int64), ("pts", ctypes.c_void_p),
+                ("n", ctypes.c_int64), ("edges", ctypes.c_void_p), ("E", ctypes.c_int64)]
 
 
 class Timing(ctypes.Structure):
@@ -101,9 +107,15 @@ def lib():
         L.vacd_visibility_deferred.argtypes = [vp, vp, i64, ctypes.POINTER(i64)]
         L.vacd_visibility_batch_d.argtypes = [vp, ctypes.POINTER(Part), ctypes.c_int32, vp, i64,
                                               vp, vp]
+        L.vacd_visibility_batch.argtypes = [vp, ctypes.POINTER(Part), ctypes.c_int32, vp, i64,
+                                            ctypes.POINTER(i64), vp]
         L.vacd_score_planes.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp,
                                         ctypes.POINTER(i32)]
         L.vacd_score_planes_d.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp, vp]
+        L.vacd_score_planes_batch.argtypes = [vp, ctypes.POINTER(ScorePart), ctypes.c_int32, vp,
+                                              vp]
+        L.vacd_score_planes_batch_d.argtypes = [vp, ctypes.POINTER(ScorePart), ctypes.c_int32, vp,
+                                                vp]
         L.vacd_max_distance.argtypes = [vp, vp, i64, vp, ctypes.POINTER(ctypes.c_float)]
         L.vacd_max_distance_d.argtypes = [vp, vp, i64, vp, vp]
         L.classify_and_rate_planes.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
@@ -116,6 +128,8 @@ def lib():
 def _p(a):
     if a is None:
         return None
+    if isinstance(a, int):  # a raw device pointer
+        return a
     if hasattr(a, "data_ptr"):
         return a.data_ptr()
     return a.ctypes.data
@@ -259,6 +273,32 @@ class Context:
         self._check(lib().vacd_score_planes_d(self._h, _p(d_planes), P, _p(d_pts), n,
                                               _p(d_edges), E, _p(d_q32), _p(d_f),
                                               _p(d_argmax)))
+
+    def score_planes_batch(self, parts):
+        """parts: list of (planes (P, 4), points (n, 3), edges (E, 2)) host arrays.
+        Returns (list of int64 score arrays, int32 argmax per part)."""
+        keep = []
+        arr = (ScorePart * max(1, len(parts)))()
+        for k, (pl, pt, ed) in enumerate(parts):
+            pl = np.ascontiguousarray(pl, dtype=np.float32).reshape(-1, 4)
+            pt = np.ascontiguousarray(pt, dtype=np.float32).reshape(-1, 3)
+            ed = np.ascontiguousarray(ed, dtype=np.uint32).reshape(-1, 2)
+            keep += [pl, pt, ed]
+            arr[k] = ScorePart(_p(pl), pl.shape[0], _p(pt), pt.shape[0], _p(ed), ed.shape[0])
+        P = [int(keep[3 * k].shape[0]) for k in range(len(parts))]
+        q = np.zeros(max(1, sum(P)), dtype=np.int64)
+        am = np.zeros(max(1, len(parts)), dtype=np.int32)
+        self._check(lib().vacd_score_planes_batch(self._h, arr, len(parts), _p(q), _p(am)))
+        offs = np.cumsum([0] + P)
+        return [q[offs[k]:offs[k + 1]].copy() for k in range(len(parts))], am[:len(parts)].copy()
+
+    def score_planes_batch_d(self, parts, d_q32, d_argmax):
+        """parts: list of (d_planes, P, d_pts, n, d_edges, E) device pointers/tensors."""
+        arr = (ScorePart * max(1, len(parts)))()
+        for k, (pl, P, pt, n, ed, E) in enumerate(parts):
+            arr[k] = ScorePart(_p(pl), P, _p(pt), n, _p(ed), E)
+        self._check(lib().vacd_score_planes_batch_d(self._h, arr, len(parts), _p(d_q32),
+                                                    _p(d_argmax)))
 
 
 class Bvh:
