@@ -73,6 +73,7 @@ PYBIND11_MODULE(_visacd, m) {
       .def_readwrite("gpu_distance", &vhost::EngineOptions::gpu_distance)
       .def_readwrite("skip_cap_pairs", &vhost::EngineOptions::skip_cap_pairs)
       .def_readwrite("mode", &vhost::EngineOptions::mode)
+      .def_readwrite("frontier_fraction", &vhost::EngineOptions::frontier_fraction)
       .def_readwrite("verbose", &vhost::EngineOptions::verbose);
   m.attr("engine") = py::cast(&vhost::engine, py::return_value_policy::reference);
 

@@ -88,6 +88,9 @@ struct EngineOptions {
   // "plain": the reference's loop, one part split per step; "batched": every
   // part above the threshold split in the same round (up to the part budget)
   std::string mode = "plain";
+  // batched rounds split the parts whose concavity is at least this fraction
+  // of the worst part's (1: only the worst, as the plain loop; 0: all)
+  double frontier_fraction = 0.5;
   bool verbose = true;
 };
 inline EngineOptions engine;

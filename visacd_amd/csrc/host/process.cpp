@@ -527,7 +527,8 @@ struct Driver {
   }
 };
 
-// Batched frontier: every part above the concavity threshold (up to the part
+// Batched frontier: every part above the concavity threshold and within
+// engine.frontier_fraction of the worst part's concavity (up to the part
 // budget, worst first) is split in the same round.  Planes, clips, hulls and
 // samples run on all host threads; the GPU scores all those parts in one
 // call and computes the visibility of all new parts in one call.  Each part
@@ -562,8 +563,14 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
           cand.push_back({edge_score(parts[i]), i});
     } else {
       concavity_batch(D.E, parts, conc);
+      double worst = -1.0;
       for (size_t i = 0; i < parts.size(); ++i)
-        if (!frozen[i] && conc[i] >= concavity_threshold) cand.push_back({conc[i], i});
+        if (!frozen[i]) worst = std::max(worst, conc[i]);
+      // the round takes the parts within frontier_fraction of the worst one
+      const double floor = std::max(concavity_threshold, engine.frontier_fraction * worst);
+      for (size_t i = 0; i < parts.size(); ++i)
+        if (!frozen[i] && conc[i] >= concavity_threshold && conc[i] >= floor)
+          cand.push_back({conc[i], i});
     }
     if (cand.empty()) break;
     std::stable_sort(cand.begin(), cand.end(),
@@ -576,8 +583,7 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
     std::vector<std::vector<Plane>> planes(F);
     std::vector<size_t> ss_offset(F);
     {
-      PhaseTimer t("planes");
-#pragma omp parallel for schedule(dynamic)
+      PhaseTimer t("planes");  // each part's plane extents run on all host threads
       for (long f = 0; f < (long)F; ++f) {
         const Mesh &m = parts[cand[f].second];
         std::mt19937 rng((uint32_t)mix64(seed[cand[f].second] ^ 0x5eedull));
