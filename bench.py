@@ -70,16 +70,19 @@ def cpu_baseline(W, seconds):
                       f"(exact predicates, CPU BVH), {dt:.1f} s"}
 
 
-def acd_wall(W):
-    """Full plain decomposition of the workload through visacd.process."""
+def acd_wall(W, mode):
+    """Full decomposition of the workload's mesh through visacd.process:
+    mode "plain" (the reference's loop) or "batched" (frontier rounds)."""
     import numpy as np
 
     import visacd as vis
+    from visacd_amd import _visacd
 
     m = vis.Mesh()
     m.vertices = vis.VecArray3d(W.vertices.astype(np.float64).tolist())
     m.triangles = vis.make_vecarray3i(W.triangles.astype(np.int32))
     vis.engine.verbose = False
+    vis.engine.mode = mode
     vis.engine.max_points = W.n_points
     vis.engine.planes = "sphere"
     vis.engine.num_planes = int(W.planes.shape[0])
@@ -90,14 +93,18 @@ def acd_wall(W):
     t0 = time.perf_counter()
     r = vis.process(m, conc, nparts)
     dt = time.perf_counter() - t0
-    return {"wall_s": dt, "parts": int(r.num_parts), "concavity": float(r.concavity),
-            "concavity_threshold": conc, "max_parts": nparts, "method": "plain",
-            "points_per_part": W.n_points, "planes_per_cut": int(W.planes.shape[0])}
+    prof = {k: round(v[0], 4) for k, v in _visacd.last_profile().items() if isinstance(v, tuple)}
+    vis.engine.mode = "plain"
+    return {"workload": W.name, "mode": mode, "wall_s": dt, "parts": int(r.num_parts),
+            "concavity": float(r.concavity), "concavity_threshold": conc, "max_parts": nparts,
+            "points_per_part": W.n_points, "planes_per_cut": int(W.planes.shape[0]),
+            "phase_seconds": prof}
 
 
 # kernels of one default visibility pass whose HBM traffic is reported
 TRAFFIC_KERNELS = ("k_visibility<1, false, false, false, false, true>",
-                   "k_walk_split<32, false, false, 0>", "k_walk_split<64, false, false, 0>")
+                   "k_walk_split<32, false, false, 0, false>",
+                   "k_walk_split<64, false, false, 0, false>")
 
 
 def pmc_traffic(workload):
@@ -185,6 +192,101 @@ def pmc_probe(workload):
     ctx.close()
 
 
+def c3_batched(steps):
+    """Batched frontier step on 64 independent perturbed icospheres (levels
+    2-5, at most 2,048 points and 1,024 planes each): one batched visibility
+    launch set for all parts, then one batched scoring launch set on each
+    part's edges.  The knobs are autotuned, then the step is captured into a
+    HIP graph and replayed; every replay is checked against the eager step."""
+    import numpy as np
+    import torch
+
+    from visacd_amd import autotune, native
+    from visacd_amd import workloads as wl
+
+    Ws = wl.batched_c3()
+    dev = torch.device("cuda:0")
+    ctx = native.Context(0)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    bvhs, pts, planes = [], [], []
+    for W in Ws:
+        bvhs.append(ctx.bvh(torch.from_numpy(W.vertices).to(dev),
+                            torch.from_numpy(W.triangles.astype(np.int32)).to(dev)))
+        pts.append(torch.from_numpy(W.points).to(dev))
+        planes.append(torch.from_numpy(W.planes).to(dev))
+    ns = [W.n_points for W in Ws]
+    S = sum(n * (n - 1) // 2 for n in ns)
+    cap = S
+    d_pairs = torch.empty((cap, 2), dtype=torch.int32, device=dev)
+    d_count = torch.zeros(1, dtype=torch.int64, device=dev)
+    d_off = torch.zeros(len(Ws) + 1, dtype=torch.int64, device=dev)
+    vis_parts = [(pts[i], ns[i], None, bvhs[i], None) for i in range(len(Ws))]
+
+    def vis():
+        ctx.visibility_batch_d(vis_parts, d_pairs, cap, d_count, d_off)
+
+    vis()
+    torch.cuda.synchronize()
+    off = d_off.cpu().numpy()
+    base = d_pairs.data_ptr()
+    score_parts = [(planes[i], Ws[i].planes.shape[0], pts[i], ns[i], base + 8 * int(off[i]),
+                    int(off[i + 1] - off[i])) for i in range(len(Ws))]
+    P = sum(W.planes.shape[0] for W in Ws)
+    d_q = torch.zeros(P, dtype=torch.int64, device=dev)
+    d_am = torch.zeros(len(Ws), dtype=torch.int32, device=dev)
+
+    def score():
+        ctx.score_planes_batch_d(score_parts, d_q, d_am)
+
+    def step():
+        vis()
+        score()
+
+    best, table = autotune.tune(ctx, step, torch.cuda.synchronize)
+    log("c3 autotune best", best)
+    step()
+    torch.cuda.synchronize()
+    ref = (int(d_count.item()), d_off.cpu().clone(), d_q.cpu().clone(), d_am.cpu().clone())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / steps
+    # capture the step into a graph (buffers and staging regions are sized by
+    # the warm-up calls above) and replay it
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(g, stream=s):
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        step()
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    ok = True
+    for _ in range(3):  # replays interleaved with eager work, each checked
+        d_count.zero_()
+        d_am.fill_(-7)
+        g.replay()
+        torch.cuda.synchronize()
+        ok &= int(d_count.item()) == ref[0] and torch.equal(d_off.cpu(), ref[1])
+        ok &= torch.equal(d_q.cpu(), ref[2]) and torch.equal(d_am.cpu(), ref[3])
+        _ = torch.ones(1024, device=dev).sum().item()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / steps
+    ok &= int(d_count.item()) == ref[0] and torch.equal(d_am.cpu(), ref[3])
+    for b in bvhs:
+        b.close()
+    ctx.close()
+    return {"workload": "64 perturbed icospheres L2-L5, <=2048 points, 1024 planes each",
+            "parts": len(Ws), "segments": S, "edges": ref[0], "planes": P,
+            "eager_ms_per_step": eager_ms, "graph_ms_per_step": graph_ms,
+            "gsegments_per_s_graph": S / (graph_ms * 1e-3) / 1e9,
+            "graph_replays_match_eager": bool(ok), "autotune_best": best,
+            "autotune_table": [{**c, "ms": round(t, 4)} for c, t in table]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +298,7 @@ def main():
     ap.add_argument("--score-steps", type=int, default=10)
     ap.add_argument("--no-acd", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the HBM traffic runs")
+    ap.add_argument("--no-c3", action="store_true", help="skip the batched 64-part step")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_probe:
@@ -375,9 +478,18 @@ def main():
         else:
             line["roofline"]["traffic_unit"] = "bytes per visibility pass (fan pass + queue walk)"
             line["roofline"]["alg_bytes_per_pass"] = bytes_per_seg * S
+    if not args.no_c3:
+        line["batched_frontier"] = c3_batched(max(3, args.steps // 2))
+        log("c3", {k: v for k, v in line["batched_frontier"].items() if k != "autotune_table"})
     if not args.no_acd:
-        line["acd"] = acd_wall(W)
-        log("acd", line["acd"])
+        from visacd_amd import workloads as wl
+
+        runs = []
+        for WW in (W, wl.icosphere_c5()) if args.workload == "c2" else (W,):
+            for mode in ("plain", "batched"):
+                runs.append(acd_wall(WW, mode))
+                log("acd", {k: v for k, v in runs[-1].items() if k != "phase_seconds"})
+        line["acd"] = runs
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
     print(json.dumps(line), flush=True)

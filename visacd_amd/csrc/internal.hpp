@@ -134,33 +134,48 @@ struct vacd_ctx {
   int lanes_per_pair_log2 = 0;  // queued-pair walks split over 2^k lanes (k <= 4)
   // recorded (start, stop) event pairs per timed kernel kind
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
-  // pinned host staging for descriptor tables copied to the device; a copy
-  // captured into a graph reads it again at every replay
-  void *pinned = nullptr;
-  size_t pinned_bytes = 0;
-  hipEvent_t pinned_free = nullptr;  // recorded after the last copy out of it
+  // pinned host staging for descriptor tables copied to the device, one
+  // region per table kind; a copy captured into a graph reads its region
+  // again at every replay
+  void *pinned[2] = {nullptr, nullptr};
+  size_t pinned_bytes[2] = {0, 0};
+  hipEvent_t pinned_free[2] = {nullptr, nullptr};  // after the last copy out of it
 };
 
 namespace vacd {
-// Pinned staging buffer of at least `bytes`, once the copies enqueued from it
-// earlier have completed.
-inline void *pinned_stage(vacd_ctx *ctx, size_t bytes) {
-  if (ctx->pinned_free) VACD_HIP(hipEventSynchronize(ctx->pinned_free));
-  if (bytes > ctx->pinned_bytes) {
-    if (ctx->pinned) VACD_HIP(hipHostFree(ctx->pinned));
-    ctx->pinned = nullptr;
-    ctx->pinned_bytes = 0;
-    const size_t want = bytes < 4096 ? 4096 : bytes;
-    VACD_HIP(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
-    ctx->pinned_bytes = want;
-  }
-  return ctx->pinned;
+enum PinnedKind { PINNED_VIS = 0, PINNED_SCORE = 1 };
+
+inline bool capturing(const vacd_ctx *ctx) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  VACD_HIP(hipStreamIsCapturing(ctx->stream, &st));
+  return st != hipStreamCaptureStatusNone;
 }
-// Copies `bytes` from the pinned buffer to dst on the context stream.
-inline void pinned_upload(vacd_ctx *ctx, void *dst, size_t bytes) {
-  VACD_HIP(hipMemcpyAsync(dst, ctx->pinned, bytes, hipMemcpyHostToDevice, ctx->stream));
-  if (!ctx->pinned_free) VACD_HIP(hipEventCreateWithFlags(&ctx->pinned_free, hipEventDisableTiming));
-  VACD_HIP(hipEventRecord(ctx->pinned_free, ctx->stream));
+
+// Pinned staging region `k` of at least `bytes`, once the copies enqueued
+// from it earlier have completed.  While the stream is being captured the
+// region must already be large enough (a warm-up call sizes it), and it is
+// not waited on: the graph's copy reads it at replay time.
+inline void *pinned_stage(vacd_ctx *ctx, int k, size_t bytes) {
+  const bool cap = capturing(ctx);
+  if (!cap && ctx->pinned_free[k]) VACD_HIP(hipEventSynchronize(ctx->pinned_free[k]));
+  if (bytes > ctx->pinned_bytes[k]) {
+    VACD_REQUIRE(!cap, VACD_ERR_INVALID, "staging buffer too small while capturing: warm up first");
+    if (ctx->pinned[k]) VACD_HIP(hipHostFree(ctx->pinned[k]));
+    ctx->pinned[k] = nullptr;
+    ctx->pinned_bytes[k] = 0;
+    const size_t want = bytes < 4096 ? 4096 : bytes;
+    VACD_HIP(hipHostMalloc(&ctx->pinned[k], want, hipHostMallocDefault));
+    ctx->pinned_bytes[k] = want;
+  }
+  return ctx->pinned[k];
+}
+// Copies `bytes` from pinned region k to dst on the context stream.
+inline void pinned_upload(vacd_ctx *ctx, int k, void *dst, size_t bytes) {
+  VACD_HIP(hipMemcpyAsync(dst, ctx->pinned[k], bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (capturing(ctx)) return;
+  if (!ctx->pinned_free[k])
+    VACD_HIP(hipEventCreateWithFlags(&ctx->pinned_free[k], hipEventDisableTiming));
+  VACD_HIP(hipEventRecord(ctx->pinned_free[k], ctx->stream));
 }
 
 enum TimedKind { TIMED_VIS = 0, TIMED_SCORE = 1, TIMED_BVH = 2 };

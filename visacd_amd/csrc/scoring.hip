@@ -332,12 +332,12 @@ void launch_scoring_batch(vacd_ctx *ctx, const vacd_score_part *parts, int npart
   const size_t parts_bytes = sizeof(ScorePartDev) * (size_t)nparts;
   const size_t blocks_off = (parts_bytes + 255) & ~size_t(255);
   const size_t tables_bytes = blocks_off + sizeof(ScoreBlock) * blocks.size();
-  char *host = static_cast<char *>(pinned_stage(ctx, tables_bytes));
+  char *host = static_cast<char *>(pinned_stage(ctx, PINNED_SCORE, tables_bytes));
   std::memcpy(host, dev.data(), parts_bytes);
   if (!blocks.empty()) std::memcpy(host + blocks_off, blocks.data(), sizeof(ScoreBlock) * blocks.size());
   vacd::DevBuf &tb = ctx->buf[SLOT_SCORE_TABLES];
   tb.ensure(tables_bytes);
-  pinned_upload(ctx, tb.ptr, tables_bytes);
+  pinned_upload(ctx, PINNED_SCORE, tb.ptr, tables_bytes);
   const ScorePartDev *d_parts = tb.as<ScorePartDev>();
   const ScoreBlock *d_blocks = reinterpret_cast<const ScoreBlock *>(tb.as<char>() + blocks_off);
   vacd::DevBuf &accb = ctx->buf[SLOT_SCORES];

@@ -85,8 +85,10 @@ int vacd_ctx_destroy(vacd_ctx *ctx) {
       (void)hipEventDestroy(pr.first);
       (void)hipEventDestroy(pr.second);
     }
-  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-  if (ctx->pinned_free) (void)hipEventDestroy(ctx->pinned_free);
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->pinned[k]) (void)hipHostFree(ctx->pinned[k]);
+    if (ctx->pinned_free[k]) (void)hipEventDestroy(ctx->pinned_free[k]);
+  }
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return VACD_OK;

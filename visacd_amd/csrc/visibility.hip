@@ -1380,9 +1380,9 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
   // again at every replay
   vacd::DevBuf &pb = ctx->buf[SLOT_PARTS];
   pb.ensure(sizeof(Part) * (size_t)nparts);
-  std::memcpy(pinned_stage(ctx, sizeof(Part) * (size_t)nparts), hp.data(),
+  std::memcpy(pinned_stage(ctx, PINNED_VIS, sizeof(Part) * (size_t)nparts), hp.data(),
               sizeof(Part) * (size_t)nparts);
-  pinned_upload(ctx, pb.ptr, sizeof(Part) * (size_t)nparts);
+  pinned_upload(ctx, PINNED_VIS, pb.ptr, sizeof(Part) * (size_t)nparts);
   VisArgs a;
   a.one = hp[0];
   a.parts = pb.as<Part>();
