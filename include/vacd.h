@@ -214,6 +214,37 @@ int vacd_score_planes_batch_d(vacd_ctx *ctx, const vacd_score_part *parts, int n
 int vacd_score_planes_batch(vacd_ctx *ctx, const vacd_score_part *parts, int nparts,
                             int64_t *score_q32, int32_t *argmax);
 
+/* ---- learned plane proposal ----------------------------------------------------
+ * No counterpart in the reference.  vacd_gemm_bf16_d is the product the model
+ * is built from, exported for its tests: C[M][N] = op(A) op(B) (+ bias[N],
+ * relu), A, B bf16 device arrays, A(m,k) = ta ? A[k*lda+m] : A[m*lda+k],
+ * B(k,n) = tb ? B[n*ldb+k] : B[k*ldb+n], fp32 output C (ldc), accumulate
+ * into C when acc != 0. */
+int vacd_gemm_bf16_d(vacd_ctx *ctx, int ta, int tb, int M, int N, int K, const void *d_A,
+                     int lda, const void *d_B, int ldb, const float *d_bias, int relu, int acc,
+                     float *d_C, int ldc);
+
+/* The model: a set encoder over E sampled visibility edges of a part (edge
+ * features (u, v, |v - u|), 7 floats) with max-pooling, and a head that
+ * predicts, for each plane (n, d) of the part, its cut length divided by the
+ * part's total edge length.  Weights are FP32 masters with bf16 shadows on
+ * MFMA.  Batches hold B parts with E edges and P planes each: d_feat
+ * [B][E][7], d_planes [B][P][4], d_out / d_target [B][P] (device).
+ * Parameters travel as vacd_mlp_param_count() floats. */
+typedef struct vacd_mlp vacd_mlp;
+int64_t vacd_mlp_param_count(void);
+int vacd_mlp_create(vacd_ctx *ctx, uint64_t seed, vacd_mlp **out);
+int vacd_mlp_destroy(vacd_mlp *mlp);
+int vacd_mlp_set_params(vacd_ctx *ctx, vacd_mlp *mlp, const float *params);
+int vacd_mlp_get_params(vacd_ctx *ctx, vacd_mlp *mlp, float *params);
+int vacd_mlp_forward_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *d_feat, int P,
+                       const float *d_planes, float *d_out);
+/* One Adam step (lr; betas 0.9, 0.999) on the mean squared error; waits and
+ * writes the batch loss (before the step) to *loss. */
+int vacd_mlp_train_step_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *d_feat,
+                          int P, const float *d_planes, const float *d_target, float lr,
+                          float *loss);
+
 /* ---- surface distance ---------------------------------------------------------
  * Largest distance from m points to a triangle mesh (its BVH), the one-sided
  * term of the sampled Hausdorff distance in the concavity metric

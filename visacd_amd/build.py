@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBVACD = os.path.join(HERE, "libvacd.so")
 
-HIP_SOURCES = ["vacd_api.hip", "lbvh.hip", "visibility.hip", "scoring.hip", "distance.hip"]
+HIP_SOURCES = ["vacd_api.hip", "lbvh.hip", "visibility.hip", "scoring.hip", "distance.hip", "mlp.hip"]
 HOST_SOURCES = []  # filled in by the host driver sources when present
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

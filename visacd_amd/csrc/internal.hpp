@@ -252,6 +252,25 @@ void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P,
 void launch_scoring_batch(vacd_ctx *ctx, const vacd_score_part *parts, int nparts,
                           int64_t *d_q32, int32_t *d_argmax);
 
+// bf16 GEMM on MFMA (mlp.hip): C = op(A) op(B) (+ bias, relu), fp32 and/or
+// bf16 output; A(m, k) = ta ? A[k * lda + m] : A[m * lda + k], B(k, n) = tb ?
+// B[n * ldb + k] : B[k * ldb + n]; acc adds into the fp32 output.
+void gemm_bf16(vacd_ctx *ctx, bool ta, bool tb, int M, int N, int K, const void *A, int lda,
+               const void *B, int ldb, const float *bias, bool relu, bool acc, float *Cf,
+               void *Cb, int ldc);
+void to_bf16(vacd_ctx *ctx, const float *x, int64_t n, void *y);
+
+// Learned plane proposal model (mlp.hip).
+int64_t MlpLayoutCount();
+void mlp_destroy(vacd_mlp *M);
+void mlp_create(vacd_ctx *ctx, uint64_t seed, vacd_mlp **out);
+void mlp_set(vacd_ctx *ctx, vacd_mlp *M, const float *host);
+void mlp_get(vacd_ctx *ctx, vacd_mlp *M, float *host);
+void mlp_forward(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
+                 const float *planes, float *out);
+float mlp_train_step(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
+                     const float *planes, const float *target, float lr);
+
 // Fills n 64-bit words with v on the context stream (a kernel, so that it
 // may be captured into a graph).
 void fill_u64(vacd_ctx *ctx, void *p, int64_t n, unsigned long long v);

@@ -1,0 +1,543 @@
+// Learned plane proposal for gfx950: a set-encoder MLP with BF16 weights on
+// MFMA and FP32 accumulation, with hand-written forward, backward and Adam.
+//
+// Model (per part; D = 256, E = 512 sampled visibility edges, P planes):
+//   edge features x_e = (u, v, |v - u|) in R^7, zero-padded to 8;
+//   encoder   h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), h3 = relu(W3 h2 + b3)
+//             (shared over the edges), z = max over the edges of h3;
+//   head      g1 = relu(Wz z + Wp q + c1) for plane q = (n, d): the pooled part
+//             Wz z + c1 is computed once per part, Wp q per plane;
+//             g2 = relu(V2 g1 + c2);  y = v3 . g2 + c3, the predicted cut length
+//             of the plane divided by the part's total edge length.
+// Every layer product runs through k_gemm: C = op(A) op(B) with bf16 operands,
+// v_mfma_f32_32x32x8_bf16 tiles and FP32 accumulation; a block computes a
+// 64 x 64 tile with four waves (32 x 32 each), K staged through LDS 16 at a
+// time.  Activations and gradients that feed products are kept in bf16,
+// weights in FP32 master copies with bf16 shadows for the products.
+#include "internal.hpp"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace vacd {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  // round to nearest even (NaN is not produced by these layers)
+  const unsigned u = __float_as_uint(f);
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+constexpr int kTM = 64, kTN = 64, kTK = 16;
+
+// C[m][n] (+)= sum_k A(m, k) B(k, n) (+ bias[n]), optionally relu'd.
+// A(m, k) = TA ? A[k * lda + m] : A[m * lda + k]; B(k, n) = TB ? B[n * ldb + k]
+// : B[k * ldb + n].  Outputs: Cf (fp32, nullable) and Cb (bf16, nullable);
+// ACC adds into Cf instead of storing.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const unsigned short *__restrict__ A,
+                                              int lda, const unsigned short *__restrict__ B, int ldb,
+                                              const float *__restrict__ bias, int relu, int acc,
+                                              float *__restrict__ Cf, unsigned short *__restrict__ Cb,
+                                              int ldc) {
+  __shared__ unsigned short As[kTM][kTK + 4];  // As[m][k]
+  __shared__ unsigned short Bs[kTN][kTK + 4];  // Bs[n][k]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x16 c;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) c[r] = 0.0f;
+  for (int k0 = 0; k0 < K; k0 += kTK) {
+    // 64 x 16 elements of each operand, 4 per thread
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int e = tid * 4 + t;
+      const int r = e / kTK, k = e % kTK;
+      const int gm = m0 + r, gn = n0 + r, gk = k0 + k;
+      unsigned short av = 0, bv = 0;
+      if (gm < M && gk < K) av = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+      if (gn < N && gk < K) bv = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+      As[r][k] = av;
+      Bs[r][k] = bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kTK; kk += 8) {
+      // lane l: row / column l % 32, k = kk + 4 * (l / 32) .. + 3
+      const int rc = lane & 31, kb = kk + 4 * (lane >> 5);
+      bf16x4 a, b;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = (short)As[wm + rc][kb + i];
+        b[i] = (short)Bs[wn + rc][kb + i];
+      }
+      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // D layout: element r of lane l is row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+  const int gn = n0 + wn + (lane & 31);
+  if (gn >= N) return;
+  const float bn = bias ? bias[gn] : 0.0f;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int gm = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (gm >= M) continue;
+    float v = c[r] + bn;
+    const size_t o = (size_t)gm * ldc + gn;
+    if (acc) v += Cf[o];
+    if (relu) v = fmaxf(v, 0.0f);
+    if (Cf) Cf[o] = v;
+    if (Cb) Cb[o] = f2bf(v);
+  }
+}
+
+__global__ void k_f2bf(const float *__restrict__ x, int64_t n, unsigned short *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = f2bf(x[i]);
+}
+
+constexpr int kD = 256;  // hidden width
+
+// edge features (R rows of 7 floats) -> bf16 rows of 8 (zero-padded)
+__global__ void k_edge_in(const float *__restrict__ x, int64_t R, unsigned short *__restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= R * 8) return;
+  const int64_t r = i >> 3;
+  const int c = (int)(i & 7);
+  y[i] = c < 7 ? f2bf(x[r * 7 + c]) : (unsigned short)0;
+}
+
+// z[b][f] = max over part b's E rows of h[.][f]; arg = that row (first max)
+__global__ void k_maxpool(const unsigned short *__restrict__ h, int E, float *__restrict__ z,
+                          int *__restrict__ arg) {
+  const int b = blockIdx.x, f = threadIdx.x;
+  float best = -1e30f;
+  int bi = 0;
+  for (int e = 0; e < E; ++e) {
+    const float v = bf2f(h[((size_t)b * E + e) * kD + f]);
+    if (v > best) best = v, bi = e;
+  }
+  z[b * kD + f] = best;
+  arg[b * kD + f] = bi;
+}
+
+// g1[p][f] = relu(u[b][f] + Wp[f] . q[p]) for the P planes of each part b
+__global__ void k_head1(const float *__restrict__ u, const float *__restrict__ Wp,
+                        const float *__restrict__ q, int64_t BP, int P,
+                        unsigned short *__restrict__ g1) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= BP * kD) return;
+  const int64_t p = i / kD;
+  const int f = (int)(i % kD);
+  const int64_t b = p / P;
+  const float *w = Wp + 4 * f;
+  const float *x = q + 4 * p;
+  const float v = u[b * kD + f] + w[0] * x[0] + w[1] * x[1] + w[2] * x[2] + w[3] * x[3];
+  g1[i] = f2bf(fmaxf(v, 0.0f));
+}
+
+// out[p] = v3 . g2[p] + c3, a wave per plane
+__global__ void k_head_out(const unsigned short *__restrict__ g2, const float *__restrict__ v3,
+                           const float *__restrict__ c3, int64_t BP, float *__restrict__ out) {
+  const int64_t p = blockIdx.x * (int64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= BP) return;
+  float s = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) s += v3[lane + 64 * t] * bf2f(g2[p * kD + lane + 64 * t]);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if (lane == 0) out[p] = s + c3[0];
+}
+
+// loss gradient and the last layer: dout = 2 (out - t) / BP; dg2 = relu'(g2) dout v3;
+// dv3 += dout g2 and dc3 += dout (atomics); loss += (out - t)^2 / BP
+__global__ void k_head_back(const float *__restrict__ out, const float *__restrict__ t,
+                            const unsigned short *__restrict__ g2, const float *__restrict__ v3,
+                            int64_t BP, unsigned short *__restrict__ dg2, float *__restrict__ dv3,
+                            float *__restrict__ dc3, float *__restrict__ loss) {
+  const int64_t p = blockIdx.x * (int64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= BP) return;
+  const float r = out[p] - t[p];
+  const float d = 2.0f * r / (float)BP;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int f = lane + 64 * k;
+    const float g = bf2f(g2[p * kD + f]);
+    dg2[p * kD + f] = f2bf(g > 0.0f ? d * v3[f] : 0.0f);
+    atomicAdd(&dv3[f], d * g);
+  }
+  if (lane == 0) {
+    atomicAdd(dc3, d);
+    atomicAdd(loss, r * r / (float)BP);
+  }
+}
+
+// out[r][c] = bf16(keep[r][c] > 0 ? x[r][c] : 0) (backward through relu)
+__global__ void k_relu_back(const float *__restrict__ x, const unsigned short *__restrict__ keep,
+                            int64_t n, unsigned short *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = f2bf(bf2f(keep[i]) > 0.0f ? x[i] : 0.0f);
+}
+
+// column sums of R rows of a bf16 [R][256] array into out (atomics), rows
+// grouped by `group` (out[(r / group) * 256 + c]) when group > 0
+__global__ void k_colsum(const unsigned short *__restrict__ x, int64_t R, int64_t group,
+                         float *__restrict__ out) {
+  const int c = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int64_t r1 = r0 + 64 < R ? r0 + 64 : R;
+  float s = 0.0f;
+  int64_t g = group > 0 ? r0 / group : 0;
+  for (int64_t r = r0; r < r1; ++r) {
+    const int64_t gr = group > 0 ? r / group : 0;
+    if (gr != g) {
+      atomicAdd(&out[g * kD + c], s);
+      s = 0.0f;
+      g = gr;
+    }
+    s += bf2f(x[r * kD + c]);
+  }
+  atomicAdd(&out[g * kD + c], s);
+}
+
+// dWp[f][j] += sum_p dg1[p][f] q[p][j]
+__global__ void k_wp_grad(const unsigned short *__restrict__ dg1, const float *__restrict__ q,
+                          int64_t BP, float *__restrict__ dWp) {
+  const int f = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int64_t r1 = r0 + 64 < BP ? r0 + 64 : BP;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t p = r0; p < r1; ++p) {
+    const float d = bf2f(dg1[p * kD + f]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s[j] += d * q[4 * p + j];
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) atomicAdd(&dWp[4 * f + j], s[j]);
+}
+
+// gradient through the max-pool: row arg[b][f] of part b gets dz[b][f]
+// (when the pooled value is positive), every other row 0
+__global__ void k_unpool(const float *__restrict__ dz, const int *__restrict__ arg,
+                         const float *__restrict__ z, int B, int E,
+                         unsigned short *__restrict__ dh) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)B * E * kD) return;
+  const int64_t row = i / kD;
+  const int f = (int)(i % kD);
+  const int b = (int)(row / E), e = (int)(row % E);
+  const bool hit = arg[b * kD + f] == e && z[b * kD + f] > 0.0f;
+  dh[i] = f2bf(hit ? dz[b * kD + f] : 0.0f);
+}
+
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+}  // namespace
+
+void gemm_bf16(vacd_ctx *ctx, bool ta, bool tb, int M, int N, int K, const void *A, int lda,
+               const void *B, int ldb, const float *bias, bool relu, bool acc, float *Cf,
+               void *Cb, int ldc) {
+  if (M <= 0 || N <= 0) return;
+  const dim3 grid((unsigned)((N + kTN - 1) / kTN), (unsigned)((M + kTM - 1) / kTM));
+  auto a = static_cast<const unsigned short *>(A);
+  auto b = static_cast<const unsigned short *>(B);
+  auto cb = static_cast<unsigned short *>(Cb);
+  hipStream_t st = ctx->stream;
+  if (!ta && !tb)
+    k_gemm<false, false><<<grid, 256, 0, st>>>(M, N, K, a, lda, b, ldb, bias, relu, acc, Cf, cb, ldc);
+  else if (!ta && tb)
+    k_gemm<false, true><<<grid, 256, 0, st>>>(M, N, K, a, lda, b, ldb, bias, relu, acc, Cf, cb, ldc);
+  else if (ta && !tb)
+    k_gemm<true, false><<<grid, 256, 0, st>>>(M, N, K, a, lda, b, ldb, bias, relu, acc, Cf, cb, ldc);
+  else
+    k_gemm<true, true><<<grid, 256, 0, st>>>(M, N, K, a, lda, b, ldb, bias, relu, acc, Cf, cb, ldc);
+  VACD_HIP(hipGetLastError());
+}
+
+void to_bf16(vacd_ctx *ctx, const float *x, int64_t n, void *y) {
+  if (n <= 0) return;
+  k_f2bf<<<(unsigned)std::min<int64_t>((n + 255) / 256, 4096), 256, 0, ctx->stream>>>(
+      x, n, static_cast<unsigned short *>(y));
+  VACD_HIP(hipGetLastError());
+}
+
+// Parameter layout (fp32, in this order): W1[256][8] b1[256] W2[256][256] b2
+// W3[256][256] b3 Wz[256][256] Wp[256][4] c1[256] V2[256][256] c2 v3[256] c3[1].
+struct MlpLayout {
+  int64_t W1, b1, W2, b2, W3, b3, Wz, Wp, c1, V2, c2, v3, c3, n;
+  MlpLayout() {
+    int64_t o = 0;
+    auto take = [&](int64_t k) {
+      const int64_t r = o;
+      o += k;
+      return r;
+    };
+    W1 = take(kD * 8), b1 = take(kD), W2 = take(kD * kD), b2 = take(kD), W3 = take(kD * kD);
+    b3 = take(kD), Wz = take(kD * kD), Wp = take(kD * 4), c1 = take(kD), V2 = take(kD * kD);
+    c2 = take(kD), v3 = take(kD), c3 = take(1), n = o;
+  }
+};
+// bf16 shadows: W1, W2, W3, Wz, V2 at these element offsets
+constexpr int64_t kB_W1 = 0, kB_W2 = kD * 8, kB_W3 = kB_W2 + kD * kD, kB_Wz = kB_W3 + kD * kD,
+                  kB_V2 = kB_Wz + kD * kD, kB_n = kB_V2 + kD * kD;
+
+}  // namespace vacd
+
+struct vacd_mlp {
+  int device = 0;
+  vacd::MlpLayout L;
+  vacd::DevBuf w, g, m, v, wb, act;
+  int64_t steps = 0;
+};
+
+namespace vacd {
+namespace {
+
+void refresh_shadows(vacd_ctx *ctx, vacd_mlp *M) {
+  const float *w = M->w.as<float>();
+  unsigned short *wb = M->wb.as<unsigned short>();
+  to_bf16(ctx, w + M->L.W1, kD * 8, wb + kB_W1);
+  to_bf16(ctx, w + M->L.W2, kD * kD, wb + kB_W2);
+  to_bf16(ctx, w + M->L.W3, kD * kD, wb + kB_W3);
+  to_bf16(ctx, w + M->L.Wz, kD * kD, wb + kB_Wz);
+  to_bf16(ctx, w + M->L.V2, kD * kD, wb + kB_V2);
+}
+
+// Activation buffers of one forward pass (bf16 unless noted)
+struct Acts {
+  unsigned short *x0, *h1, *h2, *h3, *zb, *g1, *g2;
+  float *z, *u, *out;  // fp32
+  int *arg;
+  // backward
+  unsigned short *dg2, *dg1, *dub, *dh3, *dh2, *dh1;
+  float *tmp, *du, *dz, *loss;
+};
+
+Acts carve(vacd_mlp *M, int64_t R, int64_t B, int64_t BP, bool backward) {
+  size_t off = 0;
+  std::vector<std::pair<void **, size_t>> want;
+  Acts a{};
+  auto add = [&](void **p, size_t bytes) { want.push_back({p, bytes}); };
+  add((void **)&a.x0, 2 * R * 8);
+  add((void **)&a.h1, 2 * R * kD);
+  add((void **)&a.h2, 2 * R * kD);
+  add((void **)&a.h3, 2 * R * kD);
+  add((void **)&a.zb, 2 * B * kD);
+  add((void **)&a.g1, 2 * BP * kD);
+  add((void **)&a.g2, 2 * BP * kD);
+  add((void **)&a.z, 4 * B * kD);
+  add((void **)&a.u, 4 * B * kD);
+  add((void **)&a.out, 4 * BP);
+  add((void **)&a.arg, 4 * B * kD);
+  if (backward) {
+    add((void **)&a.dg2, 2 * BP * kD);
+    add((void **)&a.dg1, 2 * BP * kD);
+    add((void **)&a.dub, 2 * B * kD);
+    add((void **)&a.dh3, 2 * R * kD);
+    add((void **)&a.dh2, 2 * R * kD);
+    add((void **)&a.dh1, 2 * R * kD);
+    add((void **)&a.tmp, 4 * std::max(BP, R) * kD);
+    add((void **)&a.du, 4 * B * kD);
+    add((void **)&a.dz, 4 * B * kD);
+    add((void **)&a.loss, 256);
+  }
+  for (auto &x : want) off += (x.second + 255) & ~size_t(255);
+  M->act.ensure(off);
+  char *base = M->act.as<char>();
+  off = 0;
+  for (auto &x : want) {
+    *x.first = base + off;
+    off += (x.second + 255) & ~size_t(255);
+  }
+  return a;
+}
+
+unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+void forward(vacd_ctx *ctx, vacd_mlp *M, const Acts &a, int B, int E, const float *feat, int P,
+             const float *planes) {
+  hipStream_t st = ctx->stream;
+  const float *w = M->w.as<float>();
+  const unsigned short *wb = M->wb.as<unsigned short>();
+  const int64_t R = (int64_t)B * E, BP = (int64_t)B * P;
+  k_edge_in<<<blocks_for(R * 8), 256, 0, st>>>(feat, R, a.x0);
+  gemm_bf16(ctx, false, true, (int)R, kD, 8, a.x0, 8, wb + kB_W1, 8, w + M->L.b1, true, false,
+            nullptr, a.h1, kD);
+  gemm_bf16(ctx, false, true, (int)R, kD, kD, a.h1, kD, wb + kB_W2, kD, w + M->L.b2, true, false,
+            nullptr, a.h2, kD);
+  gemm_bf16(ctx, false, true, (int)R, kD, kD, a.h2, kD, wb + kB_W3, kD, w + M->L.b3, true, false,
+            nullptr, a.h3, kD);
+  k_maxpool<<<(unsigned)B, kD, 0, st>>>(a.h3, E, a.z, a.arg);
+  to_bf16(ctx, a.z, (int64_t)B * kD, a.zb);
+  gemm_bf16(ctx, false, true, B, kD, kD, a.zb, kD, wb + kB_Wz, kD, w + M->L.c1, false, false, a.u,
+            nullptr, kD);
+  k_head1<<<blocks_for(BP * kD), 256, 0, st>>>(a.u, w + M->L.Wp, planes, BP, P, a.g1);
+  gemm_bf16(ctx, false, true, (int)BP, kD, kD, a.g1, kD, wb + kB_V2, kD, w + M->L.c2, true, false,
+            nullptr, a.g2, kD);
+  k_head_out<<<blocks_for(BP, 4), 256, 0, st>>>(a.g2, w + M->L.v3, w + M->L.c3, BP, a.out);
+  VACD_HIP(hipGetLastError());
+}
+
+void check_dims(int B, int E, int P) {
+  VACD_REQUIRE(B >= 1 && E >= 1 && P >= 1, VACD_ERR_INVALID, "empty MLP batch");
+  VACD_REQUIRE((int64_t)B * E < (int64_t(1) << 26) && (int64_t)B * P < (int64_t(1) << 26),
+               VACD_ERR_LIMIT, "MLP batch too large");
+}
+
+}  // namespace
+
+void mlp_create(vacd_ctx *ctx, uint64_t seed, vacd_mlp **out) {
+  vacd_mlp *M = new vacd_mlp();
+  try {
+    M->device = ctx->device;
+    const int64_t n = M->L.n;
+    // padded to whole 8-byte words: the buffers are cleared with 64-bit fills
+    M->w.ensure(8 * ((n + 1) / 2));
+    M->g.ensure(8 * ((n + 1) / 2));
+    M->m.ensure(8 * ((n + 1) / 2));
+    M->v.ensure(8 * ((n + 1) / 2));
+    M->wb.ensure(2 * kB_n);
+    // He-uniform weights from a seeded generator, zero biases
+    std::vector<float> h(n, 0.0f);
+    std::mt19937_64 rng(seed);
+    auto fill = [&](int64_t o, int64_t count, int fan_in) {
+      const double lim = std::sqrt(6.0 / fan_in);
+      std::uniform_real_distribution<double> u(-lim, lim);
+      for (int64_t i = 0; i < count; ++i) h[o + i] = (float)u(rng);
+    };
+    fill(M->L.W1, kD * 8, 7);
+    fill(M->L.W2, kD * kD, kD);
+    fill(M->L.W3, kD * kD, kD);
+    fill(M->L.Wz, kD * kD, kD);
+    fill(M->L.Wp, kD * 4, 4);
+    fill(M->L.V2, kD * kD, kD);
+    fill(M->L.v3, kD, kD);
+    VACD_HIP(hipMemcpyAsync(M->w.ptr, h.data(), 4 * n, hipMemcpyHostToDevice, ctx->stream));
+    fill_u64(ctx, M->m.ptr, (n + 1) / 2, 0ull);
+    fill_u64(ctx, M->v.ptr, (n + 1) / 2, 0ull);
+    refresh_shadows(ctx, M);
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    M->w.release(), M->g.release(), M->m.release(), M->v.release(), M->wb.release();
+    delete M;
+    throw;
+  }
+  *out = M;
+}
+
+int64_t MlpLayoutCount() { return MlpLayout().n; }
+
+void mlp_destroy(vacd_mlp *M) {
+  if (!M) return;
+  (void)hipSetDevice(M->device);
+  M->w.release(), M->g.release(), M->m.release(), M->v.release(), M->wb.release();
+  M->act.release();
+  delete M;
+}
+
+void mlp_set(vacd_ctx *ctx, vacd_mlp *M, const float *host) {
+  VACD_HIP(hipMemcpyAsync(M->w.ptr, host, 4 * M->L.n, hipMemcpyHostToDevice, ctx->stream));
+  refresh_shadows(ctx, M);
+  VACD_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void mlp_get(vacd_ctx *ctx, vacd_mlp *M, float *host) {
+  VACD_HIP(hipMemcpyAsync(host, M->w.ptr, 4 * M->L.n, hipMemcpyDeviceToHost, ctx->stream));
+  VACD_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void mlp_forward(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
+                 const float *planes, float *out) {
+  check_dims(B, E, P);
+  const Acts a = carve(M, (int64_t)B * E, B, (int64_t)B * P, false);
+  forward(ctx, M, a, B, E, feat, P, planes);
+  VACD_HIP(hipMemcpyAsync(out, a.out, 4 * (size_t)B * P, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+float mlp_train_step(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
+                     const float *planes, const float *target, float lr) {
+  check_dims(B, E, P);
+  hipStream_t st = ctx->stream;
+  const int64_t R = (int64_t)B * E, BP = (int64_t)B * P;
+  const Acts a = carve(M, R, B, BP, true);
+  forward(ctx, M, a, B, E, feat, P, planes);
+  const MlpLayout &L = M->L;
+  float *w = M->w.as<float>(), *g = M->g.as<float>();
+  const unsigned short *wb = M->wb.as<unsigned short>();
+  fill_u64(ctx, g, (L.n + 1) / 2, 0ull);
+  fill_u64(ctx, a.loss, 1, 0ull);
+  fill_u64(ctx, a.du, (int64_t)B * kD / 2, 0ull);
+  // head
+  k_head_back<<<blocks_for(BP, 4), 256, 0, st>>>(a.out, target, a.g2, w + L.v3, BP, a.dg2,
+                                                  g + L.v3, g + L.c3, a.loss);
+  gemm_bf16(ctx, true, false, kD, kD, (int)BP, a.dg2, kD, a.g1, kD, nullptr, false, false,
+            g + L.V2, nullptr, kD);                                   // dV2 = dg2^T g1
+  k_colsum<<<blocks_for(BP, 64), kD, 0, st>>>(a.dg2, BP, 0, g + L.c2);  // dc2
+  gemm_bf16(ctx, false, false, (int)BP, kD, kD, a.dg2, kD, wb + kB_V2, kD, nullptr, false, false,
+            a.tmp, nullptr, kD);                                      // dg1 = dg2 V2
+  k_relu_back<<<blocks_for(BP * kD), 256, 0, st>>>(a.tmp, a.g1, BP * kD, a.dg1);
+  k_wp_grad<<<blocks_for(BP, 64), kD, 0, st>>>(a.dg1, planes, BP, g + L.Wp);
+  k_colsum<<<blocks_for(BP, 64), kD, 0, st>>>(a.dg1, BP, P, a.du);    // du per part
+  k_colsum<<<blocks_for(BP, 64), kD, 0, st>>>(a.dg1, BP, 0, g + L.c1);
+  to_bf16(ctx, a.du, (int64_t)B * kD, a.dub);
+  gemm_bf16(ctx, true, false, kD, kD, B, a.dub, kD, a.zb, kD, nullptr, false, false, g + L.Wz,
+            nullptr, kD);                                             // dWz = du^T z
+  gemm_bf16(ctx, false, false, B, kD, kD, a.dub, kD, wb + kB_Wz, kD, nullptr, false, false, a.dz,
+            nullptr, kD);                                             // dz = du Wz
+  // encoder
+  k_unpool<<<blocks_for(R * kD), 256, 0, st>>>(a.dz, a.arg, a.z, B, E, a.dh3);
+  gemm_bf16(ctx, true, false, kD, kD, (int)R, a.dh3, kD, a.h2, kD, nullptr, false, false,
+            g + L.W3, nullptr, kD);
+  k_colsum<<<blocks_for(R, 64), kD, 0, st>>>(a.dh3, R, 0, g + L.b3);
+  gemm_bf16(ctx, false, false, (int)R, kD, kD, a.dh3, kD, wb + kB_W3, kD, nullptr, false, false,
+            a.tmp, nullptr, kD);
+  k_relu_back<<<blocks_for(R * kD), 256, 0, st>>>(a.tmp, a.h2, R * kD, a.dh2);
+  gemm_bf16(ctx, true, false, kD, kD, (int)R, a.dh2, kD, a.h1, kD, nullptr, false, false,
+            g + L.W2, nullptr, kD);
+  k_colsum<<<blocks_for(R, 64), kD, 0, st>>>(a.dh2, R, 0, g + L.b2);
+  gemm_bf16(ctx, false, false, (int)R, kD, kD, a.dh2, kD, wb + kB_W2, kD, nullptr, false, false,
+            a.tmp, nullptr, kD);
+  k_relu_back<<<blocks_for(R * kD), 256, 0, st>>>(a.tmp, a.h1, R * kD, a.dh1);
+  gemm_bf16(ctx, true, false, kD, 8, (int)R, a.dh1, kD, a.x0, 8, nullptr, false, false, g + L.W1,
+            nullptr, 8);
+  k_colsum<<<blocks_for(R, 64), kD, 0, st>>>(a.dh1, R, 0, g + L.b1);
+  // Adam
+  M->steps += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - std::pow(b1, (float)M->steps), c2 = 1.0f - std::pow(b2, (float)M->steps);
+  k_adam<<<blocks_for(L.n), 256, 0, st>>>(w, g, M->m.as<float>(), M->v.as<float>(), L.n, lr, b1, b2,
+                                          1e-8f, c1, c2);
+  refresh_shadows(ctx, M);
+  float loss = 0.0f;
+  VACD_HIP(hipMemcpyAsync(&loss, a.loss, 4, hipMemcpyDeviceToHost, st));
+  VACD_HIP(hipStreamSynchronize(st));
+  return loss;
+}
+
+}  // namespace vacd

@@ -518,6 +518,72 @@ int vacd_score_planes_batch(vacd_ctx *ctx, const vacd_score_part *parts, int npa
   });
 }
 
+int vacd_gemm_bf16_d(vacd_ctx *ctx, int ta, int tb, int M, int N, int K, const void *d_A,
+                     int lda, const void *d_B, int ldb, const float *d_bias, int relu, int acc,
+                     float *d_C, int ldc) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(M >= 0 && N >= 0 && K >= 0 && d_A && d_B && d_C, VACD_ERR_INVALID,
+                 "bad GEMM arguments");
+    set_device(ctx);
+    vacd::gemm_bf16(ctx, ta != 0, tb != 0, M, N, K, d_A, lda, d_B, ldb, d_bias, relu != 0,
+                    acc != 0, d_C, nullptr, ldc);
+  });
+}
+
+int64_t vacd_mlp_param_count(void) { return vacd::MlpLayoutCount(); }
+
+int vacd_mlp_create(vacd_ctx *ctx, uint64_t seed, vacd_mlp **out) {
+  if (!ctx || !out) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    vacd::mlp_create(ctx, seed, out);
+  });
+}
+
+int vacd_mlp_destroy(vacd_mlp *mlp) {
+  vacd::mlp_destroy(mlp);
+  return VACD_OK;
+}
+
+int vacd_mlp_set_params(vacd_ctx *ctx, vacd_mlp *mlp, const float *params) {
+  if (!ctx || !mlp || !params) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    vacd::mlp_set(ctx, mlp, params);
+  });
+}
+
+int vacd_mlp_get_params(vacd_ctx *ctx, vacd_mlp *mlp, float *params) {
+  if (!ctx || !mlp || !params) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    vacd::mlp_get(ctx, mlp, params);
+  });
+}
+
+int vacd_mlp_forward_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *d_feat, int P,
+                       const float *d_planes, float *d_out) {
+  if (!ctx || !mlp || !d_feat || !d_planes || !d_out)
+    return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    vacd::mlp_forward(ctx, mlp, B, E, d_feat, P, d_planes, d_out);
+  });
+}
+
+int vacd_mlp_train_step_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *d_feat,
+                          int P, const float *d_planes, const float *d_target, float lr,
+                          float *loss) {
+  if (!ctx || !mlp || !d_feat || !d_planes || !d_target || !loss)
+    return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    *loss = vacd::mlp_train_step(ctx, mlp, B, E, d_feat, P, d_planes, d_target, lr);
+  });
+}
+
 int vacd_score_planes(vacd_ctx *ctx, const float *planes, int64_t P, const float *pts, int64_t n,
                       const uint32_t *edges, int64_t E, int64_t *score_q32, float *score_f,
                       int32_t *argmax) {

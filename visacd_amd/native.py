@@ -30,7 +30,9 @@ EXPORTED = [
     "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_deferred",
     "vacd_visibility_batch_d", "vacd_visibility_batch", "vacd_score_planes",
     "vacd_score_planes_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
-    "vacd_max_distance", "vacd_max_distance_d", "classify_and_rate_planes",
+    "vacd_max_distance", "vacd_max_distance_d", "vacd_gemm_bf16_d", "vacd_mlp_param_count",
+    "vacd_mlp_create", "vacd_mlp_destroy", "vacd_mlp_set_params", "vacd_mlp_get_params",
+    "vacd_mlp_forward_d", "vacd_mlp_train_step_d", "classify_and_rate_planes",
 ]
 
 
@@ -118,6 +120,16 @@ def lib():
                                                 vp]
         L.vacd_max_distance.argtypes = [vp, vp, i64, vp, ctypes.POINTER(ctypes.c_float)]
         L.vacd_max_distance_d.argtypes = [vp, vp, i64, vp, vp]
+        ci = ctypes.c_int
+        L.vacd_gemm_bf16_d.argtypes = [vp, ci, ci, ci, ci, ci, vp, ci, vp, ci, vp, ci, ci, vp, ci]
+        L.vacd_mlp_param_count.restype = i64
+        L.vacd_mlp_create.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.vacd_mlp_destroy.argtypes = [vp]
+        L.vacd_mlp_set_params.argtypes = [vp, vp, vp]
+        L.vacd_mlp_get_params.argtypes = [vp, vp, vp]
+        L.vacd_mlp_forward_d.argtypes = [vp, vp, ci, ci, vp, ci, vp, vp]
+        L.vacd_mlp_train_step_d.argtypes = [vp, vp, ci, ci, vp, ci, vp, vp, ctypes.c_float,
+                                            ctypes.POINTER(ctypes.c_float)]
         L.classify_and_rate_planes.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int]
         L.classify_and_rate_planes.restype = None
@@ -299,6 +311,73 @@ class Context:
             arr[k] = ScorePart(_p(pl), P, _p(pt), n, _p(ed), E)
         self._check(lib().vacd_score_planes_batch_d(self._h, arr, len(parts), _p(d_q32),
                                                     _p(d_argmax)))
+
+
+    def gemm_bf16_d(self, ta, tb, M, N, K, d_A, lda, d_B, ldb, d_C, ldc, d_bias=None,
+                    relu=False, acc=False):
+        self._check(lib().vacd_gemm_bf16_d(self._h, int(ta), int(tb), M, N, K, _p(d_A), lda,
+                                           _p(d_B), ldb, _p(d_bias), int(relu), int(acc),
+                                           _p(d_C), ldc))
+
+
+class Mlp:
+    """The learned plane proposal model of a context (see include/vacd.h)."""
+
+    def __init__(self, ctx: "Context", seed: int = 0):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        ctx._check(lib().vacd_mlp_create(ctx._h, seed, ctypes.byref(self.h)))
+
+    @staticmethod
+    def param_count() -> int:
+        return int(lib().vacd_mlp_param_count())
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.param_count(), dtype=np.float32)
+        self.ctx._check(lib().vacd_mlp_get_params(self.ctx._h, self.h, _p(out)))
+        return out
+
+    def set(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.param_count()
+        self.ctx._check(lib().vacd_mlp_set_params(self.ctx._h, self.h, _p(p)))
+
+    def forward_d(self, B, E, d_feat, P, d_planes, d_out) -> None:
+        self.ctx._check(lib().vacd_mlp_forward_d(self.ctx._h, self.h, B, E, _p(d_feat), P,
+                                                 _p(d_planes), _p(d_out)))
+
+    def train_step_d(self, B, E, d_feat, P, d_planes, d_target, lr) -> float:
+        loss = ctypes.c_float()
+        self.ctx._check(lib().vacd_mlp_train_step_d(self.ctx._h, self.h, B, E, _p(d_feat), P,
+                                                    _p(d_planes), _p(d_target), lr,
+                                                    ctypes.byref(loss)))
+        return float(loss.value)
+
+    def close(self):
+        if self.h:
+            lib().vacd_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mlp_unpack(p: np.ndarray) -> dict:
+    """Named views of a parameter vector (layout of the C side)."""
+    D = 256
+    shapes = [("W1", (D, 8)), ("b1", (D,)), ("W2", (D, D)), ("b2", (D,)), ("W3", (D, D)),
+              ("b3", (D,)), ("Wz", (D, D)), ("Wp", (D, 4)), ("c1", (D,)), ("V2", (D, D)),
+              ("c2", (D,)), ("v3", (D,)), ("c3", (1,))]
+    out, o = {}, 0
+    for name, shp in shapes:
+        k = int(np.prod(shp))
+        out[name] = p[o:o + k].reshape(shp)
+        o += k
+    assert o == p.size
+    return out
 
 
 class Bvh:
