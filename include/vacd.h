@@ -239,6 +239,10 @@ int vacd_mlp_set_params(vacd_ctx *ctx, vacd_mlp *mlp, const float *params);
 int vacd_mlp_get_params(vacd_ctx *ctx, vacd_mlp *mlp, float *params);
 int vacd_mlp_forward_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *d_feat, int P,
                        const float *d_planes, float *d_out);
+/* Same on host buffers (feat [B][E][7], planes [B][P][4], out [B][P]);
+ * returns when out is written. */
+int vacd_mlp_forward(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *feat, int P,
+                     const float *planes, float *out);
 /* One Adam step (lr; betas 0.9, 0.999) on the mean squared error; waits and
  * writes the batch loss (before the step) to *loss. */
 int vacd_mlp_train_step_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *d_feat,

@@ -9,6 +9,7 @@
 #include <pybind11/stl_bind.h>
 
 #include "mesh.hpp"
+#include "proposal.hpp"
 
 namespace py = pybind11;
 
@@ -74,6 +75,11 @@ PYBIND11_MODULE(_visacd, m) {
       .def_readwrite("skip_cap_pairs", &vhost::EngineOptions::skip_cap_pairs)
       .def_readwrite("mode", &vhost::EngineOptions::mode)
       .def_readwrite("frontier_fraction", &vhost::EngineOptions::frontier_fraction)
+      .def_readwrite("proposal", &vhost::EngineOptions::proposal)
+      .def_readwrite("proposal_weights", &vhost::EngineOptions::proposal_weights)
+      .def_readwrite("proposal_top", &vhost::EngineOptions::proposal_top)
+      .def_readwrite("proposal_audit", &vhost::EngineOptions::proposal_audit)
+      .def_readwrite("proposal_audit_ratio", &vhost::EngineOptions::proposal_audit_ratio)
       .def_readwrite("verbose", &vhost::EngineOptions::verbose);
   m.attr("engine") = py::cast(&vhost::engine, py::return_value_policy::reference);
 
@@ -91,6 +97,35 @@ PYBIND11_MODULE(_visacd, m) {
   m.def("concavity", [](const vhost::Mesh &part, const vhost::Mesh &hull, int res, uint32_t seed) {
     return vhost::concavity(part, hull, res, seed);
   });
+  // inputs of the learned plane proposal, as the driver computes them
+  m.def("proposal_inputs",
+        [](py::array_t<double, py::array::c_style | py::array::forcecast> verts,
+           py::array_t<uint32_t, py::array::c_style | py::array::forcecast> edges,
+           py::array_t<double, py::array::c_style | py::array::forcecast> planes, uint64_t seed) {
+          if (verts.ndim() != 2 || verts.shape(1) != 3) throw std::invalid_argument("verts must be (n, 3)");
+          if (edges.ndim() != 2 || edges.shape(1) != 2) throw std::invalid_argument("edges must be (E, 2)");
+          if (planes.ndim() != 2 || planes.shape(1) != 4) throw std::invalid_argument("planes must be (P, 4)");
+          const size_t n = (size_t)verts.shape(0), E = (size_t)edges.shape(0), P = (size_t)planes.shape(0);
+          std::vector<vhost::Vec3> v(n);
+          const double *pv = verts.data();
+          for (size_t i = 0; i < n; ++i) v[i] = {pv[3 * i], pv[3 * i + 1], pv[3 * i + 2]};
+          std::vector<std::pair<unsigned, unsigned>> e(E);
+          const uint32_t *pe = edges.data();
+          for (size_t i = 0; i < E; ++i) {
+            if (pe[2 * i] >= n || pe[2 * i + 1] >= n) throw std::out_of_range("edge endpoint out of range");
+            e[i] = {pe[2 * i], pe[2 * i + 1]};
+          }
+          std::vector<vhost::Plane> pl(P);
+          const double *pp = planes.data();
+          for (size_t i = 0; i < P; ++i) pl[i] = vhost::Plane(pp[4 * i], pp[4 * i + 1], pp[4 * i + 2], pp[4 * i + 3]);
+          const vhost::PartFrame fr = vhost::part_frame(v);
+          py::array_t<float> feat({(py::ssize_t)vhost::kProposalEdges, (py::ssize_t)vhost::kEdgeFeatures});
+          py::array_t<float> q({(py::ssize_t)P, (py::ssize_t)4});
+          vhost::edge_features(v, e, seed, fr, feat.mutable_data());
+          vhost::plane_inputs(pl, P, fr, q.mutable_data());
+          return py::make_tuple(feat, q, py::make_tuple(fr.c[0], fr.c[1], fr.c[2], fr.s));
+        },
+        py::arg("verts"), py::arg("edges"), py::arg("planes"), py::arg("seed"));
   m.def("flat_surface_planes", [](const vhost::Mesh &mesh, double min_area) {
     std::vector<std::array<double, 4>> out;
     for (auto &p : vhost::flat_surface_planes(mesh, min_area)) out.push_back({p.a, p.b, p.c, p.d});

@@ -91,6 +91,16 @@ struct EngineOptions {
   // batched rounds split the parts whose concavity is at least this fraction
   // of the worst part's (1: only the worst, as the plain loop; 0: all)
   double frontier_fraction = 0.5;
+  // plane selection: "exact" scores every candidate plane (the reference);
+  // "learned" ranks the candidates with the proposal model, scores the
+  // proposal_top best-ranked and proposal_audit random ones exactly, and
+  // scores them all when the best of the top is under proposal_audit_ratio
+  // times the best of the audit
+  std::string proposal = "exact";
+  std::string proposal_weights;  // float32 parameter file of the model
+  int proposal_top = 256;
+  int proposal_audit = 256;
+  double proposal_audit_ratio = 0.98;
   bool verbose = true;
 };
 inline EngineOptions engine;

@@ -22,13 +22,16 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <fstream>
 #include <iostream>
 #include <limits>
 #include <memory>
+#include <numeric>
 #include <sstream>
 #include <stdexcept>
 
 #include "mesh.hpp"
+#include "proposal.hpp"
 #include "vacd.h"
 
 namespace vhost {
@@ -52,13 +55,39 @@ struct PhaseTimer {
 
 struct Engine {
   vacd_ctx *ctx = nullptr;
+  vacd_mlp *mlp = nullptr;  // proposal model, loaded on first use
   explicit Engine(int device) {
     if (vacd_ctx_create(device, &ctx) != VACD_OK)
       throw std::runtime_error(std::string("visacd: ") + vacd_last_error(nullptr));
   }
-  ~Engine() { vacd_ctx_destroy(ctx); }
+  ~Engine() {
+    vacd_mlp_destroy(mlp);
+    vacd_ctx_destroy(ctx);
+  }
+  Engine(const Engine &) = delete;
+  Engine &operator=(const Engine &) = delete;
   void check(int rc) const {
     if (rc != VACD_OK) throw std::runtime_error(std::string("visacd: ") + vacd_last_error(ctx));
+  }
+  // the proposal model with the parameters of engine.proposal_weights
+  vacd_mlp *proposal_model() {
+    if (mlp) return mlp;
+    const std::string &path = engine.proposal_weights;
+    if (path.empty()) throw std::runtime_error("visacd: engine.proposal_weights is not set");
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) throw std::runtime_error("visacd: cannot open proposal weights " + path);
+    const int64_t want = vacd_mlp_param_count();
+    const std::streamoff bytes = f.tellg();
+    if (bytes != (std::streamoff)(4 * want))
+      throw std::runtime_error("visacd: " + path + " holds " + std::to_string(bytes) +
+                               " bytes, the proposal model has " + std::to_string(4 * want));
+    std::vector<float> w((size_t)want);
+    f.seekg(0);
+    f.read(reinterpret_cast<char *>(w.data()), bytes);
+    if (!f) throw std::runtime_error("visacd: cannot read proposal weights " + path);
+    check(vacd_mlp_create(ctx, 0, &mlp));
+    check(vacd_mlp_set_params(ctx, mlp, w.data()));
+    return mlp;
   }
 };
 
@@ -302,15 +331,6 @@ double edge_score(const Mesh &m) {
 
 
 // ---------------------------------------------------------------- batched
-// splitmix64: per-part seeds derived from the parent's, independent of the
-// order in which parts are processed.
-uint64_t mix64(uint64_t x) {
-  x += 0x9e3779b97f4a7c15ull;
-  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
-  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
-  return x ^ (x >> 31);
-}
-
 // Visibility of many parts in one batched call (masked: pairs touching a
 // vertex of the last cut).  Edges are appended to each part in k order.
 void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, bool with_mask,
@@ -445,11 +465,187 @@ void concavity_batch(const Engine &E, const MeshList &parts, std::vector<double>
   }
 }
 
+// ------------------------------------------------------- plane selection
+// Scoring inputs of a part, converted once per selection.
+struct ScoreInput {
+  std::vector<float> pts;
+  std::vector<uint32_t> ed;
+  void fill(const Mesh &m) {
+    if (!pts.empty() || m.intersecting_edges.empty()) return;
+    pts = to_f32(m.vertices);
+    ed.resize(2 * m.intersecting_edges.size());
+    for (size_t e = 0; e < m.intersecting_edges.size(); ++e)
+      ed[2 * e] = m.intersecting_edges[e].first, ed[2 * e + 1] = m.intersecting_edges[e].second;
+  }
+};
+
+// Exact Q32.32 scores of the listed planes of the listed parts in one batched
+// call: q[f][j] is the score of planes[f][sub[f][j]].
+void score_subsets(const Engine &E, const std::vector<const Mesh *> &ms,
+                   const std::vector<std::vector<Plane>> &planes,
+                   const std::vector<std::vector<int>> &sub, const std::vector<size_t> &parts,
+                   std::vector<ScoreInput> &in, std::vector<std::vector<int64_t>> &q) {
+  std::vector<std::vector<float>> hp(parts.size());
+  std::vector<vacd_score_part> sp;
+  int64_t total = 0;
+  for (size_t j = 0; j < parts.size(); ++j) {
+    const size_t f = parts[j];
+    in[f].fill(*ms[f]);
+    const std::vector<int> &idx = sub[f];
+    hp[j].resize(4 * idx.size());
+    for (size_t i = 0; i < idx.size(); ++i) {
+      const Plane &p = planes[f][idx[i]];
+      hp[j][4 * i] = (float)p.a, hp[j][4 * i + 1] = (float)p.b;
+      hp[j][4 * i + 2] = (float)p.c, hp[j][4 * i + 3] = (float)p.d;
+    }
+    sp.push_back(vacd_score_part{hp[j].data(), (int64_t)idx.size(), in[f].pts.data(),
+                                 (int64_t)ms[f]->vertices.size(), in[f].ed.data(),
+                                 (int64_t)ms[f]->intersecting_edges.size()});
+    total += (int64_t)idx.size();
+  }
+  if (sp.empty()) return;
+  std::vector<int64_t> q32(total);
+  std::vector<int32_t> am(sp.size());
+  E.check(vacd_score_planes_batch(E.ctx, sp.data(), (int)sp.size(), q32.data(), am.data()));
+  int64_t off = 0;
+  for (size_t j = 0; j < parts.size(); ++j) {
+    q[parts[j]].assign(q32.begin() + off, q32.begin() + off + sp[j].P);
+    off += sp[j].P;
+  }
+  last_profile.calls["scoring.planes"] += total;
+}
+
+// First maximum of the scores in plane order; planes from ss_offset on (the
+// flat-surface planes) count config.flat_surface_k times
+// (lib/src/process.cpp:128-134).
+int first_max(const std::vector<int> &idx, const std::vector<int64_t> &q, size_t ss_offset) {
+  int best = -1;
+  double best_s = -1.0;
+  for (size_t j = 0; j < idx.size(); ++j) {
+    double sc = (double)q[j] * 0x1p-32;
+    if ((size_t)idx[j] >= ss_offset) sc *= config.flat_surface_k;
+    if (sc > best_s) best_s = sc, best = idx[j];
+  }
+  return best;
+}
+
+// The plane each part is cut with (index into planes[f], or -1 when the part
+// has no planes or no edges).  engine.proposal "exact" scores every plane;
+// "learned" scores the model's top-ranked planes, an audit sample and the
+// flat-surface planes, and falls back to every plane when the best of the top
+// is under engine.proposal_audit_ratio times the best of the audit.
+// seeds[f] seeds the part's edge sample and audit draw.
+std::vector<int> choose_planes(Engine &E, const std::vector<const Mesh *> &ms,
+                               const std::vector<std::vector<Plane>> &planes,
+                               const std::vector<size_t> &ss_offset,
+                               const std::vector<uint64_t> &seeds) {
+  const size_t F = ms.size();
+  std::vector<int> best(F, -1);
+  std::vector<ScoreInput> in(F);
+  std::vector<std::vector<int>> sub(F);
+  std::vector<std::vector<int64_t>> q(F);
+  std::vector<size_t> live, ranked;
+  const bool learned = engine.proposal == "learned";
+  if (!learned && engine.proposal != "exact")
+    throw std::invalid_argument("visacd: engine.proposal must be \"exact\" or \"learned\"");
+  const size_t T = (size_t)std::max(1, engine.proposal_top);
+  const size_t A = (size_t)std::max(1, engine.proposal_audit);
+  for (size_t f = 0; f < F; ++f) {
+    if (planes[f].empty() || ms[f]->intersecting_edges.empty()) continue;
+    live.push_back(f);
+    if (learned && ss_offset[f] > T + A) {
+      ranked.push_back(f);
+    } else {
+      sub[f].resize(planes[f].size());
+      std::iota(sub[f].begin(), sub[f].end(), 0);
+    }
+  }
+  std::vector<std::vector<uint8_t>> role(F);  // 1 = top-ranked, 2 = audit
+  if (!ranked.empty()) {
+    PhaseTimer t("proposal");
+    size_t Pmax = 0;
+    for (size_t f : ranked) Pmax = std::max(Pmax, ss_offset[f]);
+    const size_t chunk = std::max<size_t>(1, (size_t(1) << 21) / Pmax);
+    std::vector<float> feat, pin, pred;
+    for (size_t c0 = 0; c0 < ranked.size(); c0 += chunk) {
+      const size_t B = std::min(chunk, ranked.size() - c0);
+      feat.assign(B * kProposalEdges * kEdgeFeatures, 0.0f);
+      pin.assign(B * Pmax * 4, 0.0f);
+      pred.resize(B * Pmax);
+#pragma omp parallel for schedule(dynamic)
+      for (long b = 0; b < (long)B; ++b) {
+        const size_t f = ranked[c0 + b];
+        const PartFrame fr = part_frame(ms[f]->vertices);
+        edge_features(ms[f]->vertices, ms[f]->intersecting_edges, mix64(seeds[f] ^ 0xfea7ull), fr,
+                      feat.data() + b * kProposalEdges * kEdgeFeatures);
+        float *pq = pin.data() + b * Pmax * 4;
+        plane_inputs(planes[f], ss_offset[f], fr, pq);
+        for (size_t i = ss_offset[f]; i < Pmax; ++i)  // padding rows: copies of plane 0
+          for (int k = 0; k < 4; ++k) pq[4 * i + k] = pq[k];
+      }
+      E.check(vacd_mlp_forward(E.ctx, E.proposal_model(), (int)B, kProposalEdges, feat.data(),
+                               (int)Pmax, pin.data(), pred.data()));
+#pragma omp parallel for schedule(dynamic)
+      for (long b = 0; b < (long)B; ++b) {
+        const size_t f = ranked[c0 + b], n = ss_offset[f];
+        std::vector<float> y(pred.begin() + b * Pmax, pred.begin() + b * Pmax + n);
+        for (float &v : y)
+          if (!(v == v)) v = -std::numeric_limits<float>::infinity();
+        std::vector<int> order(n);
+        std::iota(order.begin(), order.end(), 0);
+        std::partial_sort(order.begin(), order.begin() + T, order.end(),
+                          [&](int i, int j) { return y[i] > y[j] || (y[i] == y[j] && i < j); });
+        std::vector<uint8_t> &r = role[f];
+        r.assign(planes[f].size(), 0);
+        for (size_t i = 0; i < T; ++i) r[order[i]] |= 1;
+        std::mt19937_64 rng(mix64(seeds[f] ^ 0xa0d17ull));
+        std::iota(order.begin(), order.end(), 0);
+        for (size_t i = 0; i < A; ++i) {  // partial Fisher-Yates: A distinct planes
+          std::uniform_int_distribution<size_t> d(i, n - 1);
+          std::swap(order[i], order[d(rng)]);
+          r[order[i]] |= 2;
+        }
+        for (size_t i = 0; i < planes[f].size(); ++i)
+          if (r[i] || i >= n) sub[f].push_back((int)i);
+      }
+    }
+    last_profile.calls["proposal.parts"] += (long)ranked.size();
+    for (size_t f : ranked) last_profile.calls["proposal.ranked_planes"] += (long)ss_offset[f];
+  }
+  {
+    PhaseTimer t("scoring");
+    score_subsets(E, ms, planes, sub, live, in, q);
+    std::vector<size_t> redo;
+    for (size_t f : ranked) {
+      double top = -1.0, audit = -1.0;
+      for (size_t j = 0; j < sub[f].size(); ++j) {
+        const int i = sub[f][j];
+        if ((size_t)i >= ss_offset[f]) continue;
+        const double sc = (double)q[f][j];
+        if (role[f][i] & 1) top = std::max(top, sc);
+        if (role[f][i] & 2) audit = std::max(audit, sc);
+      }
+      if (top < engine.proposal_audit_ratio * audit) redo.push_back(f);
+    }
+    if (!redo.empty()) {  // full sweep of the parts whose ranking missed
+      for (size_t f : redo) {
+        sub[f].resize(planes[f].size());
+        std::iota(sub[f].begin(), sub[f].end(), 0);
+      }
+      score_subsets(E, ms, planes, sub, redo, in, q);
+      last_profile.calls["proposal.fallbacks"] += (long)redo.size();
+    }
+  }
+  for (size_t f : live) best[f] = first_max(sub[f], q[f], ss_offset[f]);
+  return best;
+}
+
 struct Driver {
   Engine E;
   std::vector<Plane> flat;
   PairBuffer pair_buf;
   uint32_t next_seed;
+  uint64_t step = 0;  // splits so far (seeds the proposal's draws)
   explicit Driver(int device) : E(device), next_seed(engine.sample_seed) {}
 
   // returns true when nothing more can be split (no planes or no edges)
@@ -466,29 +662,14 @@ struct Driver {
       parts.push_back(std::move(mesh));
       return true;
     }
-    std::vector<float> hp(4 * planes.size());
-    for (size_t i = 0; i < planes.size(); ++i) {
-      hp[4 * i] = (float)planes[i].a, hp[4 * i + 1] = (float)planes[i].b;
-      hp[4 * i + 2] = (float)planes[i].c, hp[4 * i + 3] = (float)planes[i].d;
-    }
-    const std::vector<float> pts = to_f32(mesh.vertices);
-    std::vector<uint32_t> edges(2 * mesh.intersecting_edges.size());
-    for (size_t e = 0; e < mesh.intersecting_edges.size(); ++e)
-      edges[2 * e] = mesh.intersecting_edges[e].first, edges[2 * e + 1] = mesh.intersecting_edges[e].second;
-    std::vector<int64_t> q(planes.size());
-    int32_t am = -1;
-    {
-      PhaseTimer timer("scoring");
-      E.check(vacd_score_planes(E.ctx, hp.data(), (int64_t)planes.size(), pts.data(),
-                                (int64_t)mesh.vertices.size(), edges.data(),
-                                (int64_t)mesh.intersecting_edges.size(), q.data(), nullptr, &am));
-    }
     size_t best = 0;
-    double best_s = -1.0;
-    for (size_t i = 0; i < planes.size(); ++i) {
-      double s = (double)q[i] * 0x1p-32;
-      if (i >= ss_offset) s *= config.flat_surface_k;
-      if (s > best_s) best_s = s, best = i;
+    {
+      std::vector<std::vector<Plane>> pl(1);
+      pl[0].swap(planes);
+      const std::vector<int> b = choose_planes(E, {&mesh}, pl, {ss_offset},
+                                               {mix64((uint64_t)engine.sample_seed * 7919u + step++)});
+      planes.swap(pl[0]);
+      best = (size_t)b[0];
     }
     std::vector<int> mp, mn;
     MeshList halves;
@@ -593,49 +774,13 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
         planes[f].insert(planes[f].end(), D.flat.begin(), D.flat.end());
       }
     }
-    // exact scores of every part's planes in one call
-    std::vector<int> best(F, -1);
+    // one plane per part: every part's planes scored in one batched call
+    std::vector<int> best;
     {
-      PhaseTimer t("scoring");
-      std::vector<std::vector<float>> hp(F), pts(F);
-      std::vector<std::vector<uint32_t>> ed(F);
-      std::vector<vacd_score_part> sp;
-      std::vector<size_t> which;
-      for (size_t f = 0; f < F; ++f) {
-        const Mesh &m = parts[cand[f].second];
-        if (planes[f].empty() || m.intersecting_edges.empty()) continue;
-        hp[f].resize(4 * planes[f].size());
-        for (size_t i = 0; i < planes[f].size(); ++i) {
-          hp[f][4 * i] = (float)planes[f][i].a, hp[f][4 * i + 1] = (float)planes[f][i].b;
-          hp[f][4 * i + 2] = (float)planes[f][i].c, hp[f][4 * i + 3] = (float)planes[f][i].d;
-        }
-        pts[f] = to_f32(m.vertices);
-        ed[f].resize(2 * m.intersecting_edges.size());
-        for (size_t e = 0; e < m.intersecting_edges.size(); ++e)
-          ed[f][2 * e] = m.intersecting_edges[e].first, ed[f][2 * e + 1] = m.intersecting_edges[e].second;
-        sp.push_back(vacd_score_part{hp[f].data(), (int64_t)planes[f].size(), pts[f].data(),
-                                     (int64_t)m.vertices.size(), ed[f].data(),
-                                     (int64_t)m.intersecting_edges.size()});
-        which.push_back(f);
-      }
-      if (!sp.empty()) {
-        int64_t Ptot = 0;
-        for (auto &q : sp) Ptot += q.P;
-        std::vector<int64_t> q32(Ptot);
-        std::vector<int32_t> am(sp.size());
-        D.E.check(vacd_score_planes_batch(D.E.ctx, sp.data(), (int)sp.size(), q32.data(), am.data()));
-        int64_t off = 0;
-        for (size_t j = 0; j < which.size(); ++j) {
-          const size_t f = which[j];
-          double best_s = -1.0;
-          for (size_t i = 0; i < planes[f].size(); ++i) {
-            double sc = (double)q32[off + i] * 0x1p-32;
-            if (i >= ss_offset[f]) sc *= config.flat_surface_k;
-            if (sc > best_s) best_s = sc, best[f] = (int)i;
-          }
-          off += sp[j].P;
-        }
-      }
+      std::vector<const Mesh *> ms(F);
+      std::vector<uint64_t> ps(F);
+      for (size_t f = 0; f < F; ++f) ms[f] = &parts[cand[f].second], ps[f] = seed[cand[f].second];
+      best = choose_planes(D.E, ms, planes, ss_offset, ps);
     }
     // clips, edge inheritance and connected components on all host threads
     std::vector<MeshList> kids(F);

@@ -116,6 +116,7 @@ enum VacdSlot {
   SLOT_SLICE,
   SLOT_REFS,
   SLOT_SCORE_TABLES,
+  SLOT_MLP_IN,
   SLOT_COUNT_MAX
 };
 
@@ -268,6 +269,8 @@ void mlp_set(vacd_ctx *ctx, vacd_mlp *M, const float *host);
 void mlp_get(vacd_ctx *ctx, vacd_mlp *M, float *host);
 void mlp_forward(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
                  const float *planes, float *out);
+void mlp_forward_host(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
+                      const float *planes, float *out);
 float mlp_train_step(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
                      const float *planes, const float *target, float lr);
 

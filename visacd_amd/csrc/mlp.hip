@@ -480,6 +480,21 @@ void mlp_forward(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, in
   VACD_HIP(hipMemcpyAsync(out, a.out, 4 * (size_t)B * P, hipMemcpyDeviceToDevice, ctx->stream));
 }
 
+void mlp_forward_host(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
+                      const float *planes, float *out) {
+  check_dims(B, E, P);
+  const int64_t R = (int64_t)B * E, BP = (int64_t)B * P;
+  DevBuf &din = ctx->buf[SLOT_MLP_IN];
+  din.ensure(4 * (size_t)(7 * R + 4 * BP));
+  float *dfeat = din.as<float>(), *dplanes = dfeat + 7 * R;
+  VACD_HIP(hipMemcpyAsync(dfeat, feat, 28 * (size_t)R, hipMemcpyHostToDevice, ctx->stream));
+  VACD_HIP(hipMemcpyAsync(dplanes, planes, 16 * (size_t)BP, hipMemcpyHostToDevice, ctx->stream));
+  const Acts a = carve(M, R, B, BP, false);
+  forward(ctx, M, a, B, E, dfeat, P, dplanes);
+  VACD_HIP(hipMemcpyAsync(out, a.out, 4 * (size_t)BP, hipMemcpyDeviceToHost, ctx->stream));
+  VACD_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 float mlp_train_step(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
                      const float *planes, const float *target, float lr) {
   check_dims(B, E, P);

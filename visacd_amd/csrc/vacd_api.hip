@@ -573,6 +573,16 @@ int vacd_mlp_forward_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *
   });
 }
 
+int vacd_mlp_forward(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *feat, int P,
+                     const float *planes, float *out) {
+  if (!ctx || !mlp || !feat || !planes || !out)
+    return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    vacd::mlp_forward_host(ctx, mlp, B, E, feat, P, planes, out);
+  });
+}
+
 int vacd_mlp_train_step_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *d_feat,
                           int P, const float *d_planes, const float *d_target, float lr,
                           float *loss) {

@@ -32,7 +32,7 @@ EXPORTED = [
     "vacd_score_planes_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
     "vacd_max_distance", "vacd_max_distance_d", "vacd_gemm_bf16_d", "vacd_mlp_param_count",
     "vacd_mlp_create", "vacd_mlp_destroy", "vacd_mlp_set_params", "vacd_mlp_get_params",
-    "vacd_mlp_forward_d", "vacd_mlp_train_step_d", "classify_and_rate_planes",
+    "vacd_mlp_forward_d", "vacd_mlp_forward", "vacd_mlp_train_step_d", "classify_and_rate_planes",
 ]
 
 
@@ -128,6 +128,7 @@ def lib():
         L.vacd_mlp_set_params.argtypes = [vp, vp, vp]
         L.vacd_mlp_get_params.argtypes = [vp, vp, vp]
         L.vacd_mlp_forward_d.argtypes = [vp, vp, ci, ci, vp, ci, vp, vp]
+        L.vacd_mlp_forward.argtypes = [vp, vp, ci, ci, vp, ci, vp, vp]
         L.vacd_mlp_train_step_d.argtypes = [vp, vp, ci, ci, vp, ci, vp, vp, ctypes.c_float,
                                             ctypes.POINTER(ctypes.c_float)]
         L.classify_and_rate_planes.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
@@ -345,6 +346,18 @@ class Mlp:
     def forward_d(self, B, E, d_feat, P, d_planes, d_out) -> None:
         self.ctx._check(lib().vacd_mlp_forward_d(self.ctx._h, self.h, B, E, _p(d_feat), P,
                                                  _p(d_planes), _p(d_out)))
+
+    def forward(self, feat: np.ndarray, planes: np.ndarray) -> np.ndarray:
+        """Host arrays feat [B][E][7], planes [B][P][4] -> predictions [B][P]."""
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        q = np.ascontiguousarray(planes, dtype=np.float32)
+        B, E = f.shape[0], f.shape[1]
+        P = q.shape[1]
+        assert f.shape == (B, E, 7) and q.shape == (B, P, 4)
+        out = np.empty((B, P), dtype=np.float32)
+        self.ctx._check(lib().vacd_mlp_forward(self.ctx._h, self.h, B, E, _p(f), P, _p(q),
+                                               _p(out)))
+        return out
 
     def train_step_d(self, B, E, d_feat, P, d_planes, d_target, lr) -> float:
         loss = ctypes.c_float()
