@@ -92,6 +92,19 @@ int vacd_ctx_timing(vacd_ctx *ctx, vacd_timing *out, int reset);
  * Limits: 1 <= n_tris < 2^30. */
 int vacd_bvh_build(vacd_ctx *ctx, const float *vertices, int64_t n_vertices,
                    const uint32_t *tris, int64_t n_tris, vacd_bvh **out);
+/* Same as vacd_bvh_build, with planar groups: tri_group[t] in [0, n_groups)
+ * tags triangle t as lying in the plane planes[4g..4g+3] (a, b, c, d with
+ * a x + b y + c z + d = 0; -1: no group), n_groups <= 7.  The contact walks
+ * then test a group's boxes only against the part of a segment within the
+ * group's measured distance of its plane; results are the same as without
+ * groups (a group whose triangles lie farther than 1e-4 of the mesh extent
+ * from its plane is ignored).  Meant for the planar caps that clipping
+ * leaves, whose long triangles overlap the boxes around every cap vertex.
+ * No counterpart in the reference (OptiX builds from the soup only,
+ * lib/src/intersections.cpp:131-180). */
+int vacd_bvh_build_planar(vacd_ctx *ctx, const float *vertices, int64_t n_vertices,
+                          const uint32_t *tris, int64_t n_tris, const int8_t *tri_group,
+                          const double *planes, int n_groups, vacd_bvh **out);
 int vacd_bvh_build_d(vacd_ctx *ctx, const float *d_vertices, int64_t n_vertices,
                      const uint32_t *d_tris, int64_t n_tris, vacd_bvh **out);
 int vacd_bvh_destroy(vacd_bvh *bvh);
@@ -226,8 +239,9 @@ int vacd_gemm_bf16_d(vacd_ctx *ctx, int ta, int tb, int M, int N, int K, const v
 
 /* The model: a set encoder over E sampled visibility edges of a part (edge
  * features (u, v, |v - u|), 7 floats) with max-pooling, and a head that
- * predicts, for each plane (n, d) of the part, its cut length divided by the
- * part's total edge length.  Weights are FP32 masters with bf16 shadows on
+ * predicts, for each plane (n, d) of the part, its cut length relative to the
+ * part's other planes (trained on cut length / largest cut length among the
+ * part's planes; only the order is used).  Weights are FP32 masters with bf16 shadows on
  * MFMA.  Batches hold B parts with E edges and P planes each: d_feat
  * [B][E][7], d_planes [B][P][4], d_out / d_target [B][P] (device).
  * Parameters travel as vacd_mlp_param_count() floats. */

@@ -365,3 +365,83 @@ def test_reference_shim(ctx):
     s = native.classify_and_rate_planes(W.planes[:32], W.points, edges)
     oq = oracle.score_q32(W.planes[:32], W.points, edges)
     np.testing.assert_array_equal(s, (oq.astype(np.float64) * 2.0 ** -32).astype(np.float32))
+
+
+def _clip_twice(level, seed, planes):
+    """A part cut by each plane in turn (positive side kept); returns float32
+    vertices, triangles and, per triangle, the plane it lies on (-1: none)."""
+    from visacd_amd import _visacd as V
+
+    v, t = wl.perturbed_icosphere(level, 0.3, seed)
+    m = V.Mesh()
+    m.vertices = V.VecArray3d(np.asarray(v, np.float64).tolist())
+    m.triangles = V.make_vecarray3i(np.asarray(t, np.int32))
+    for pl in planes:
+        m = V.clip(m, pl)[0][0]
+    Pv = np.array([list(x) for x in m.vertices], np.float64)
+    T = np.array([list(x) for x in m.triangles], np.int64)
+    grp = np.full(T.shape[0], -1, np.int8)
+    on = np.zeros(Pv.shape[0], bool)
+    for g, pl in enumerate(planes):
+        d = np.abs(Pv @ np.asarray(pl[:3]) + pl[3]) <= 1e-9
+        on |= d
+        grp[(grp < 0) & d[T].all(1)] = g
+    return Pv.astype(np.float32), T.astype(np.uint32), grp, on
+
+
+def _planes2():
+    a = np.array([0.3, 0.5, -0.2])
+    b = np.array([-0.6, 0.2, 0.4])
+    a /= np.linalg.norm(a)
+    b /= np.linalg.norm(b)
+    return [[*a, 0.05], [*b, -0.1]]
+
+
+def test_visibility_planar_groups_match_oracle(ctx):
+    """Caps of two cuts as planar groups: the pruned contact walks give the
+    oracle's edges, with and without the cap mask."""
+    planes = _planes2()
+    Pv, T, grp, on = _clip_twice(4, 23, planes)
+    b = ctx.bvh(Pv, T, tri_group=grp, planes=np.array(planes))
+    _print("planar part", Pv.shape, T.shape, "grouped", int((grp >= 0).sum()), b.contact_info())
+    assert (grp >= 0).sum() > 32
+    rng = np.random.default_rng(5)
+    off = np.where(~on)[0]
+    idx = np.unique(np.concatenate([np.where(on)[0],
+                                    rng.choice(off, min(500, off.size), replace=False)]))
+    pts = Pv[idx]
+    o, _ = oracle.visibility(pts, Pv, T)
+    g = ctx.visibility(pts, b)
+    np.testing.assert_array_equal(g, o)
+    m2 = (2 * on[idx]).astype(np.uint8)
+    o2, _ = oracle.visibility(pts, Pv, T, mask=m2)
+    g2 = ctx.visibility(pts, b, new_mask=m2)
+    np.testing.assert_array_equal(g2, o2)
+
+
+def test_visibility_planar_groups_prune_and_keep_edges(ctx):
+    """On a finer part the grouped tree must give exactly the ungrouped
+    tree's edges while testing fewer triangles."""
+    from visacd_amd import native
+
+    planes = _planes2()
+    Pv, T, grp, on = _clip_twice(5, 29, planes)
+    rng = np.random.default_rng(6)
+    off = np.where(~on)[0]
+    idx = np.unique(np.concatenate([np.where(on)[0],
+                                    rng.choice(off, min(3000, off.size), replace=False)]))
+    pts = Pv[idx]
+    mask = (2 * on[idx]).astype(np.uint8)
+    plain = ctx.bvh(Pv, T)
+    grouped = ctx.bvh(Pv, T, tri_group=grp, planes=np.array(planes))
+    ctx.set_option(native.VACD_OPT_COUNT_WORK, 1)
+    try:
+        e0 = ctx.visibility(pts, plain, new_mask=mask)
+        s0 = ctx.visibility_stats()
+        e1 = ctx.visibility(pts, grouped, new_mask=mask)
+        s1 = ctx.visibility_stats()
+    finally:
+        ctx.set_option(native.VACD_OPT_COUNT_WORK, 0)
+    _print("planar prune", s0["tri_tests"], s1["tri_tests"], s0["node_visits"], s1["node_visits"])
+    np.testing.assert_array_equal(e1, e0)
+    assert s1["tri_tests"] < s0["tri_tests"]

@@ -113,3 +113,70 @@ def test_process_batched_partitions_and_is_deterministic():
     assert abs(total - vol) < 1e-6 * abs(vol) + 1e-9
     r0 = _run(vis, V, T, 0.02, 1)
     assert r1.concavity < r0.concavity
+
+
+def _parts(r):
+    return [[list(v) for v in p.vertices] for p in r.parts]
+
+
+def test_learned_proposal_with_full_sweep_matches_exact():
+    """With an audit ratio no ranking can meet, every ranked part falls back to
+    the full sweep, so the learned mode must cut exactly like the exact one."""
+    import visacd as vis
+    from visacd_amd import _visacd
+
+    V, T = wl.perturbed_icosphere(4, 0.4, 5)
+    print("process learned vs exact ico4", flush=True)
+    vis.engine.planes = "sphere"
+    vis.engine.num_planes = 1024
+    try:
+        for mode in ("plain", "batched"):
+            vis.engine.mode = mode
+            vis.engine.proposal = "exact"
+            r0 = _run(vis, V, T, 0.03, 6, return_parts=True)
+            vis.engine.proposal = "learned"
+            vis.engine.proposal_audit_ratio = 1e30
+            r1 = _run(vis, V, T, 0.03, 6, return_parts=True)
+            prof = _visacd.last_profile()
+            assert prof["proposal.parts"] == prof["proposal.fallbacks"] >= 1, mode
+            assert r1.num_parts == r0.num_parts and r1.concavity == r0.concavity, mode
+            assert _parts(r1) == _parts(r0), mode
+    finally:
+        vis.engine.proposal = "exact"
+        vis.engine.proposal_audit_ratio = 0.98
+        vis.engine.mode = "plain"
+        vis.engine.planes = "bisector"
+        vis.engine.num_planes = 2500
+        vis.config.return_parts = False
+
+
+def test_learned_proposal_with_trained_model_is_deterministic():
+    """The shipped model ranks the planes; 512 of 4096 are scored exactly per
+    ranked cut, the result repeats, and the parts still partition the volume."""
+    import visacd as vis
+    from visacd_amd import _visacd
+
+    V, T = wl.perturbed_icosphere(4, 0.4, 6)
+    print("process learned ico4", flush=True)
+    vis.engine.planes = "sphere"
+    vis.engine.num_planes = 4096
+    vis.engine.proposal = "learned"
+    try:
+        r1 = _run(vis, V, T, 0.03, 6, return_parts=True)
+        prof = _visacd.last_profile()
+        r2 = _run(vis, V, T, 0.03, 6, return_parts=True)
+    finally:
+        vis.engine.proposal = "exact"
+        vis.engine.planes = "bisector"
+        vis.engine.num_planes = 2500
+        vis.config.return_parts = False
+    assert prof["proposal.parts"] >= 1
+    ranked = prof["proposal.parts"]
+    fb = prof.get("proposal.fallbacks", 0)
+    # ranked cuts score top 256 + audit 256 (+ flat planes), fallbacks all 4096
+    if fb < ranked:
+        assert prof["scoring.planes"] < prof["proposal.ranked_planes"] + fb * 4096
+    assert r1.num_parts == r2.num_parts >= 2 and r1.concavity == r2.concavity
+    assert _parts(r1) == _parts(r2)
+    vol = _volume(_mesh(vis, V, T))
+    assert abs(sum(_volume(p) for p in r1.parts) - vol) < 1e-6 * abs(vol) + 1e-9

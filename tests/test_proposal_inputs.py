@@ -26,6 +26,9 @@ def test_frame_and_features_on_a_hand_example():
     np.testing.assert_array_equal(feat[3::2], np.repeat(feat[1:2], 255, 0))
     # x = 1 is the frame's x' = 0 (scaled input normalised); z = 0 is z' = -0.5
     np.testing.assert_allclose(q, [[1, 0, 0, 0], [0, 0, 1, 0.5]], atol=1e-7)
+    # the same planes with the opposite orientation give the same inputs
+    _, q2, _ = _inputs(v, [[1, 0], [2, 3]], [[-1, 0, 0, 1], [0, 0, -3, 0]])
+    np.testing.assert_allclose(q2, q, atol=1e-7)
 
 
 def test_stratified_sample_takes_one_edge_per_stratum():

@@ -80,6 +80,7 @@ PYBIND11_MODULE(_visacd, m) {
       .def_readwrite("proposal_top", &vhost::EngineOptions::proposal_top)
       .def_readwrite("proposal_audit", &vhost::EngineOptions::proposal_audit)
       .def_readwrite("proposal_audit_ratio", &vhost::EngineOptions::proposal_audit_ratio)
+      .def_readwrite("planar_caps", &vhost::EngineOptions::planar_caps)
       .def_readwrite("verbose", &vhost::EngineOptions::verbose);
   m.attr("engine") = py::cast(&vhost::engine, py::return_value_policy::reference);
 

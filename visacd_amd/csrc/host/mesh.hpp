@@ -50,6 +50,9 @@ struct Mesh {
   std::vector<std::pair<unsigned, unsigned>> intersecting_edges;
   // vertex ids whose pairs are tested; empty = all vertices
   std::vector<int> points;
+  // planes of the cuts that made this part, oldest first (their caps are
+  // planar groups of the part's BVH)
+  std::vector<Plane> cut_planes;
 
   Mesh copy() const {  // geometry only (lib/src/core.cpp:239-246)
     Mesh m;
@@ -101,6 +104,8 @@ struct EngineOptions {
   int proposal_top = 256;
   int proposal_audit = 256;
   double proposal_audit_ratio = 0.98;
+  // cap triangles of earlier cuts go to the visibility BVH as planar groups
+  bool planar_caps = true;
   bool verbose = true;
 };
 inline EngineOptions engine;

@@ -152,6 +152,8 @@ struct PairBuffer {
   }
 };
 
+vacd_bvh *build_bvh(const Engine &E, const Mesh &m, bool planar = false);
+
 // Visibility of a part in parity mode; with_mask restricts to pairs touching
 // a new vertex (lib/ptx/ray_segments.cu:61-75).  Edges are appended in k order
 // as pairs of part vertex ids.
@@ -159,14 +161,10 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
   if (part.points.size() < 2) return;
   PhaseTimer timer("visibility");
   std::vector<float> V = to_f32(part.vertices);
-  std::vector<uint32_t> T(3 * part.triangles.size());
-  for (size_t t = 0; t < part.triangles.size(); ++t)
-    for (int k = 0; k < 3; ++k) T[3 * t + k] = (uint32_t)part.triangles[t][k];
   vacd_bvh *bvh = nullptr;
   {
     PhaseTimer t("visibility.bvh");
-    E.check(vacd_bvh_build(E.ctx, V.data(), (int64_t)part.vertices.size(), T.data(),
-                           (int64_t)part.triangles.size(), &bvh));
+    bvh = build_bvh(E, part, true);
   }
   const int64_t n = (int64_t)part.points.size();
   last_profile.calls["visibility.points"] += n;
@@ -219,14 +217,47 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
         {(unsigned)part.points[pairs[2 * e]], (unsigned)part.points[pairs[2 * e + 1]]});
 }
 
-vacd_bvh *build_bvh(const Engine &E, const Mesh &m) {
+// BVH of a part.  planar: the caps of the part's cuts (triangles with all
+// three vertices on one of its last 7 cut planes, at least 16 of them) are
+// passed as planar groups, which the contact walks prune by plane distance.
+vacd_bvh *build_bvh(const Engine &E, const Mesh &m, bool planar) {
   const std::vector<float> V = to_f32(m.vertices);
-  std::vector<uint32_t> T(3 * m.triangles.size());
-  for (size_t t = 0; t < m.triangles.size(); ++t)
+  const size_t nT = m.triangles.size();
+  std::vector<uint32_t> T(3 * nT);
+  for (size_t t = 0; t < nT; ++t)
     for (int k = 0; k < 3; ++k) T[3 * t + k] = (uint32_t)m.triangles[t][k];
+  std::vector<int8_t> grp;
+  std::vector<double> pl;
+  if (planar && engine.planar_caps && !m.cut_planes.empty()) {
+    grp.assign(nT, -1);
+    std::vector<size_t> mine;
+    for (int c = (int)m.cut_planes.size() - 1; c >= 0 && pl.size() < 4 * 7; --c) {
+      const Plane &q = m.cut_planes[c];
+      const double l = std::sqrt(q.a * q.a + q.b * q.b + q.c * q.c);
+      if (!(l > 0.0)) continue;
+      const double tol = 1e-9 * l;
+      mine.clear();
+      for (size_t t = 0; t < nT; ++t) {
+        if (grp[t] >= 0) continue;
+        const auto &tr = m.triangles[t];
+        if (std::fabs(q.eval(m.vertices[tr[0]])) <= tol && std::fabs(q.eval(m.vertices[tr[1]])) <= tol &&
+            std::fabs(q.eval(m.vertices[tr[2]])) <= tol)
+          mine.push_back(t);
+      }
+      if (mine.size() < 16) continue;
+      const int8_t g = (int8_t)(pl.size() / 4);
+      for (size_t t : mine) grp[t] = g;
+      pl.insert(pl.end(), {q.a, q.b, q.c, q.d});
+    }
+  }
   vacd_bvh *b = nullptr;
-  E.check(vacd_bvh_build(E.ctx, V.data(), (int64_t)m.vertices.size(), T.data(),
-                         (int64_t)m.triangles.size(), &b));
+  if (!pl.empty()) {
+    E.check(vacd_bvh_build_planar(E.ctx, V.data(), (int64_t)m.vertices.size(), T.data(), (int64_t)nT,
+                                  grp.data(), pl.data(), (int)(pl.size() / 4), &b));
+    last_profile.calls["visibility.planar_groups"] += (long)(pl.size() / 4);
+  } else {
+    E.check(vacd_bvh_build(E.ctx, V.data(), (int64_t)m.vertices.size(), T.data(), (int64_t)nT, &b));
+  }
   return b;
 }
 
@@ -363,7 +394,7 @@ void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, boo
       if (with_mask && !any_new) continue;
       {
         PhaseTimer t("visibility.bvh");
-        bvhs[k] = build_bvh(E, part);
+        bvhs[k] = build_bvh(E, part, true);
       }
       vp.push_back(vacd_part{P[k].data(), n, with_mask ? M[k].data() : nullptr, bvhs[k], nullptr});
       idx.push_back(k);
@@ -699,6 +730,8 @@ struct Driver {
       }
       for (Mesh &part : pieces) {
         if (part.vertices.size() < 10) continue;
+        part.cut_planes = mesh.cut_planes;
+        part.cut_planes.push_back(planes[best]);
         choose_points(part, true, next_seed++);
         add_visibility(E, part, true, pair_buf);
         parts.push_back(std::move(part));
@@ -810,6 +843,8 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
           for (Mesh &piece : separate_disjoint(h)) {
             ++k;
             if (piece.vertices.size() < 10) continue;
+            piece.cut_planes = mesh_f.cut_planes;
+            piece.cut_planes.push_back(planes[f][best[f]]);
             choose_points(piece, true, (uint32_t)mix64(s0 + k));
             kids[f].push_back(std::move(piece));
           }

@@ -57,9 +57,14 @@ void plane_inputs(const std::vector<Plane> &planes, size_t count, const PartFram
     const Plane &p = planes[i];
     double l = std::sqrt(p.a * p.a + p.b * p.b + p.c * p.c);
     if (!(l > 0.0)) l = 1.0;
-    const double a = p.a / l, b = p.b / l, c = p.c / l, d = p.d / l;
+    double a = p.a / l, b = p.b / l, c = p.c / l, d = p.d / l;
+    d = (a * fr.c[0] + b * fr.c[1] + c * fr.c[2] + d) / fr.s;
+    // (n, d) and (-n, -d) are one plane: the sign is fixed by d > 0, or by the
+    // first non-zero normal component when d = 0
+    const bool flip = d < 0.0 || (d == 0.0 && (a < 0.0 || (a == 0.0 && (b < 0.0 || (b == 0.0 && c < 0.0)))));
+    if (flip) a = -a, b = -b, c = -c, d = -d;
     out[4 * i] = (float)a, out[4 * i + 1] = (float)b, out[4 * i + 2] = (float)c;
-    out[4 * i + 3] = (float)((a * fr.c[0] + b * fr.c[1] + c * fr.c[2] + d) / fr.s);
+    out[4 * i + 3] = (float)d;
   }
 }
 

@@ -35,7 +35,9 @@ void edge_features(const std::vector<Vec3> &v, const std::vector<std::pair<unsig
                    uint64_t seed, const PartFrame &fr, float *out);
 
 // count planes -> (n, d') per plane in the frame, n of unit length:
-// n.x + d = 0 with x = s x' + c gives n.x' + (n.c + d) / s = 0.
+// n.x + d = 0 with x = s x' + c gives n.x' + (n.c + d) / s = 0; the sign of
+// (n, d') is chosen so that d' > 0 (or, for d' = 0, the first non-zero
+// component of n is positive).
 void plane_inputs(const std::vector<Plane> &planes, size_t count, const PartFrame &fr, float *out);
 
 uint64_t mix64(uint64_t x);

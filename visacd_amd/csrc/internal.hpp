@@ -117,6 +117,7 @@ enum VacdSlot {
   SLOT_REFS,
   SLOT_SCORE_TABLES,
   SLOT_MLP_IN,
+  SLOT_GROUPS,
   SLOT_COUNT_MAX
 };
 
@@ -218,20 +219,30 @@ struct vacd_bvh {
   int32_t depth_refs = 0;
   vacd::DevBuf rnodes;  // Node[max(1, n_refs-1)]
   vacd::DevBuf rtris;   // Tri[n_refs]
+  // Planar groups of the contact tree: group g's triangles lie within tau of
+  // the plane n.x + d = 0 (|n| = 1); float4 pairs (n.x, n.y, n.z, d),
+  // (tau, 0, 0, 0).  A segment can touch them only where it is within tau of
+  // that plane, so the walks test the boxes of a group's subtree against
+  // that part of the segment only.
+  int n_groups = 0;
+  vacd::DevBuf groups;
   void release() {
     nodes.release();
     nodes4.release();
     tris.release();
     rnodes.release();
     rtris.release();
+    groups.release();
   }
 };
 
 namespace vacd {
 
-// LBVH build from device vertices/triangles (lbvh.hip).
+// LBVH build from device vertices/triangles (lbvh.hip).  d_grp (nullable):
+// per triangle its planar group code, 0 = none, 1..7 = group 0..6.
 void build_lbvh(vacd_ctx *ctx, const float *d_vertices, int64_t n_vertices,
-                const uint32_t *d_tris, int64_t n_tris, vacd_bvh *out);
+                const uint32_t *d_tris, int64_t n_tris, vacd_bvh *out,
+                const uint8_t *d_grp = nullptr);
 
 // Visibility (visibility.hip): enqueue everything; writes pairs and count.
 void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n,

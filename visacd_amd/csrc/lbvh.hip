@@ -12,7 +12,7 @@
 //                 (match on the 8 digit bits) and per-wave digit counts in LDS.
 //   4. k_karras   Karras (HPG 2012) hierarchy: each internal node finds its key
 //                 range and split, with the key index breaking ties between
-//                 equal codes.
+//                 equal codes; the child with fewer leaves is stored first.
 //   5. k_leaves   leaf-ordered 48-byte triangles.
 //   6. k_refit    bottom-up child boxes: every leaf walks to the root; at each
 //                 node the second arrival (agent-scope acq_rel counter) merges
@@ -24,6 +24,11 @@
 // twice the mesh's mean longest edge are cut into up to 64 slices parallel
 // to their shortest edge; the slices' boxes go through the same steps 1-7
 // as primitives of their own, each leaf holding its whole triangle.
+// Planar groups: triangles the caller tags as lying in one of up to 7 planes
+// (the caps of earlier cuts) get the group code (1..7) as the top 3 bits of
+// their 30-bit key (27 Morton bits below), so each group fills whole
+// subtrees; k_karras stores each child's group code in the node's spare
+// words (0: no single group) for the traversal's slab pruning.
 #include "internal.hpp"
 
 #include <algorithm>
@@ -207,10 +212,12 @@ __device__ __forceinline__ int delta(const uint32_t *__restrict__ k, int64_t n, 
   return 32 + __clz((uint32_t)(i ^ j));
 }
 
+// gshift: keys carry a planar group code in bits gshift.. (0: no groups)
 __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ keys, int64_t n,
                                                 Node *__restrict__ nodes,
                                                 int32_t *__restrict__ parent_int,
-                                                int32_t *__restrict__ parent_leaf) {
+                                                int32_t *__restrict__ parent_leaf,
+                                                int gshift = 0) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -247,7 +254,23 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ key
     c1 = (int32_t)(g + 1);
     parent_int[g + 1] = (int32_t)i;
   }
-  nodes[i].f3 = make_float4(__int_as_float(c0), __int_as_float(c1), 0.0f, 0.0f);
+  int g0 = 0, g1 = 0;  // group code of each child's key range, 0 unless one group
+  if (gshift > 0) {
+    const uint32_t a = keys[lo] >> gshift, b = keys[g] >> gshift, c = keys[g + 1] >> gshift,
+                   d = keys[hi] >> gshift;
+    g0 = a == b ? (int)a : 0;
+    g1 = c == d ? (int)c : 0;
+  }
+  // child 0 is the one with fewer leaves: the walks push child 1 and descend
+  // into child 0, so a walk's stack never holds more than log2(n) entries
+  if (g - lo + 1 > hi - g) {
+    const int32_t tc = c0;
+    c0 = c1, c1 = tc;
+    const int tg = g0;
+    g0 = g1, g1 = tg;
+  }
+  nodes[i].f3 = make_float4(__int_as_float(c0), __int_as_float(c1), __int_as_float(g0),
+                            __int_as_float(g1));
   if (i == 0) parent_int[0] = -1;
 }
 
@@ -478,7 +501,7 @@ struct SliceFrame {
 };
 
 __device__ __forceinline__ SliceFrame slice_frame(const float *V, const uint32_t *T, int64_t t,
-                                                  float h) {
+                                                  float h, int max_k) {
   float p[3][3];
   load_tri(V, T, t, p);
   const float e0 = dist3(p[0], p[1]), e1 = dist3(p[1], p[2]), e2 = dist3(p[2], p[0]);
@@ -489,7 +512,7 @@ __device__ __forceinline__ SliceFrame slice_frame(const float *V, const uint32_t
   if (e2 < shortest) shortest = e2, f.apex = 1;
   const float longest = fmaxf(e0, fmaxf(e1, e2));
   int k = h > 0.0f ? (int)ceilf(longest / h) : 1;
-  f.k = k < 1 ? 1 : (k > kMaxSlices ? kMaxSlices : k);
+  f.k = k < 1 ? 1 : (k > max_k ? max_k : k);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     f.ap[c] = p[f.apex][c];
@@ -508,11 +531,11 @@ __device__ __forceinline__ float slice_h(const unsigned long long *sum, int64_t 
 __global__ __launch_bounds__(256) void k_slice_count(const float *__restrict__ V,
                                                      const uint32_t *__restrict__ T, int64_t nT,
                                                      const unsigned long long *sum,
-                                                     uint32_t *__restrict__ counts) {
+                                                     uint32_t *__restrict__ counts, int max_k) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t == 0) counts[nT] = 0;
   if (t >= nT) return;
-  counts[t] = (uint32_t)slice_frame(V, T, t, slice_h(sum, nT)).k;
+  counts[t] = (uint32_t)slice_frame(V, T, t, slice_h(sum, nT), max_k).k;
 }
 
 __global__ __launch_bounds__(256) void k_slices(const float *__restrict__ V,
@@ -521,10 +544,10 @@ __global__ __launch_bounds__(256) void k_slices(const float *__restrict__ V,
                                                 const uint32_t *__restrict__ offsets,
                                                 float *__restrict__ blo, float *__restrict__ bhi,
                                                 uint32_t *__restrict__ ref_tri,
-                                                uint32_t *__restrict__ ref_meta) {
+                                                uint32_t *__restrict__ ref_meta, int max_k) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nT) return;
-  const SliceFrame f = slice_frame(V, T, t, slice_h(sum, nT));
+  const SliceFrame f = slice_frame(V, T, t, slice_h(sum, nT), max_k);
   const size_t base = offsets[t];
   for (int s = 0; s < f.k; ++s) {
     const float u0 = (float)s / (float)f.k;
@@ -592,7 +615,9 @@ __global__ __launch_bounds__(256) void k_morton_boxes(const float *__restrict__ 
                                                       const float *__restrict__ bhi, int64_t n,
                                                       const unsigned *__restrict__ bounds,
                                                       uint32_t *__restrict__ keys,
-                                                      uint32_t *__restrict__ vals) {
+                                                      uint32_t *__restrict__ vals,
+                                                      const uint8_t *__restrict__ grp,
+                                                      const uint32_t *__restrict__ ref_tri) {
   const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (r >= n) return;
   uint32_t q[3];
@@ -605,7 +630,8 @@ __global__ __launch_bounds__(256) void k_morton_boxes(const float *__restrict__ 
     u = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
     q[k] = (uint32_t)u;
   }
-  keys[r] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  const uint32_t code = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  keys[r] = grp ? ((uint32_t)grp[ref_tri[r]] << 27) | (code >> 3) : code;
   vals[r] = (uint32_t)r;
 }
 
@@ -637,7 +663,7 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 // slices would exceed four per triangle, or when the tree is too deep for
 // the traversal stacks.
 void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, int64_t nT,
-                        vacd_bvh *out) {
+                        vacd_bvh *out, const uint8_t *d_grp) {
   out->n_refs = 0;
   out->depth_refs = 0;
   if (nT < 2) return;
@@ -648,13 +674,23 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
   uint32_t *offsets = reinterpret_cast<uint32_t *>(cb.as<char>() + 256);
   VACD_HIP(hipMemsetAsync(sum, 0, 8, st));
   k_edge_sum<<<std::min(grid_for(nT, 256), 1024u), 256, 0, st>>>(d_V, d_T, nT, sum);
-  k_slice_count<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, sum, offsets);
-  k_scan_u32<<<1, 1024, 0, st>>>(offsets, nT + 1);
-  VACD_HIP(hipGetLastError());
+  int max_k = kMaxSlices;
   uint32_t total = 0;
-  VACD_HIP(hipMemcpyAsync(&total, offsets + nT, 4, hipMemcpyDeviceToHost, st));
-  VACD_HIP(hipStreamSynchronize(st));
-  if ((int64_t)total <= nT || (int64_t)total > 4 * nT) return;
+  for (;;) {
+    k_slice_count<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, sum, offsets, max_k);
+    k_scan_u32<<<1, 1024, 0, st>>>(offsets, nT + 1);
+    VACD_HIP(hipGetLastError());
+    VACD_HIP(hipMemcpyAsync(&total, offsets + nT, 4, hipMemcpyDeviceToHost, st));
+    VACD_HIP(hipStreamSynchronize(st));
+    // with planar groups the tree is built even without slices (for the
+    // groups' subtrees); too many slices fall back to whole triangles
+    if ((int64_t)total > 4 * nT && d_grp && max_k > 1) {
+      max_k = 1;
+      continue;
+    }
+    break;
+  }
+  if ((int64_t)total > 4 * nT || ((int64_t)total <= nT && !d_grp)) return;
   const int64_t R = total;
   vacd::DevBuf &rb = ctx->buf[SLOT_REFS];
   rb.ensure(32 * (size_t)R + 1024);
@@ -663,7 +699,7 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
   uint32_t *ref_tri = reinterpret_cast<uint32_t *>(bhi + 3 * R);
   uint32_t *ref_meta = ref_tri + R;
   k_slices<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, sum, offsets, blo, bhi, ref_tri,
-                                              ref_meta);
+                                              ref_meta, max_k);
   const int64_t ntiles = (R + kTile - 1) / kTile;
   const size_t hist_n = (size_t)256 * ntiles;
   size_t off = 0;
@@ -696,7 +732,7 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
   Tri *tris = out->rtris.as<Tri>();
   k_bounds_init<<<1, 64, 0, st>>>(bounds);
   k_bounds_boxes<<<std::min(grid_for(R, 256), 1024u), 256, 0, st>>>(blo, bhi, R, bounds);
-  k_morton_boxes<<<grid_for(R, 256), 256, 0, st>>>(blo, bhi, R, bounds, k0, v0);
+  k_morton_boxes<<<grid_for(R, 256), 256, 0, st>>>(blo, bhi, R, bounds, k0, v0, d_grp, ref_tri);
   for (int pass = 0; pass < 4; ++pass) {
     const int shift = 8 * pass;
     k_hist<<<(unsigned)ntiles, kTile, 0, st>>>(k0, R, shift, hist, (int)ntiles);
@@ -708,7 +744,7 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
   VACD_HIP(hipMemsetAsync(flags, 0, 4 * (size_t)(R - 1), st));
   VACD_HIP(hipMemsetAsync(depth, 0, 4, st));
   VACD_HIP(hipMemsetAsync(nodes, 0, sizeof(Node) * (size_t)(R - 1), st));
-  k_karras<<<grid_for(R - 1, 256), 256, 0, st>>>(k0, R, nodes, pint, pleaf);
+  k_karras<<<grid_for(R - 1, 256), 256, 0, st>>>(k0, R, nodes, pint, pleaf, d_grp ? 27 : 0);
   k_leaves_refs<<<grid_for(R, 256), 256, 0, st>>>(d_V, d_T, ref_tri, ref_meta, v0, R, tris);
   k_refit<<<grid_for(R, 256), 256, 0, st>>>(tris, R, nodes, pint, pleaf, flags, bounds, root, blo,
                                             bhi, v0);
@@ -729,7 +765,7 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
 }  // namespace
 
 void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T, int64_t nT,
-                vacd_bvh *out) {
+                vacd_bvh *out, const uint8_t *d_grp) {
   (void)nV;
   VACD_REQUIRE(nT >= 1 && nT < (int64_t(1) << 30), VACD_ERR_LIMIT,
                "BVH needs 1 <= n_tris < 2^30");
@@ -820,7 +856,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   out->depth = h_depth;
   out->closed = h_odd == 0;
   for (int k = 0; k < 6; ++k) out->bounds[k] = h_root[k];
-  build_contact_tree(ctx, d_V, d_T, nT, out);
+  build_contact_tree(ctx, d_V, d_T, nT, out, d_grp);
 }
 
 }  // namespace vacd

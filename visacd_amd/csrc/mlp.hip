@@ -8,7 +8,7 @@
 //   head      g1 = relu(Wz z + Wp q + c1) for plane q = (n, d): the pooled part
 //             Wz z + c1 is computed once per part, Wp q per plane;
 //             g2 = relu(V2 g1 + c2);  y = v3 . g2 + c3, the predicted cut length
-//             of the plane divided by the part's total edge length.
+//             of the plane relative to the largest one among the part's planes.
 // Every layer product runs through k_gemm: C = op(A) op(B) with bf16 operands,
 // v_mfma_f32_32x32x8_bf16 tiles and FP32 accumulation; a block computes a
 // 64 x 64 tile with four waves (32 x 32 each), K staged through LDS 16 at a

@@ -7,8 +7,11 @@
 // (lib/include/optixUtils.hpp:56-71).
 #include "internal.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
+#include <vector>
 
 using vacd::Error;
 
@@ -170,8 +173,60 @@ int vacd_ctx_timing(vacd_ctx *ctx, vacd_timing *out, int reset) {
 }
 
 // ------------------------------------------------------------------ BVH
+// Planar group codes of the triangles (0 none, g + 1) and the group table
+// (float4 pairs (n, d), (tau, 0, 0, 0)) from host arrays; groups whose
+// triangles do not lie within 1e-4 of the extent of their plane are dropped.
+// Returns the number of triangles left in a group.
+static int64_t planar_groups(const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+                             const int8_t *tri_group, const double *planes, int n_groups,
+                             std::vector<uint8_t> &code, std::vector<float> &table) {
+  VACD_REQUIRE(n_groups >= 0 && n_groups <= 7, VACD_ERR_INVALID, "at most 7 planar groups");
+  VACD_REQUIRE(n_groups == 0 || (tri_group && planes), VACD_ERR_INVALID, "NULL group arrays");
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t v = 0; v < nV; ++v)
+    for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], V[3 * v + k]), hi[k] = std::max(hi[k], V[3 * v + k]);
+  double ext = 0.0;
+  for (int k = 0; k < 3; ++k) ext = std::max(ext, (double)hi[k] - (double)lo[k]);
+  std::vector<float> nf(4 * (size_t)n_groups);
+  std::vector<double> tau((size_t)n_groups, 0.0);
+  for (int g = 0; g < n_groups; ++g) {
+    const double *p = planes + 4 * g;
+    const double l = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    VACD_REQUIRE(l > 0.0 && std::isfinite(l) && std::isfinite(p[3]), VACD_ERR_INVALID,
+                 "planar group plane has no normal");
+    for (int k = 0; k < 4; ++k) nf[4 * g + k] = (float)(p[k] / l);
+  }
+  for (int64_t t = 0; t < nT; ++t) {
+    const int g = tri_group[t];
+    VACD_REQUIRE(g >= -1 && g < n_groups, VACD_ERR_INVALID, "triangle group out of range");
+    if (g < 0) continue;
+    const float *n = &nf[4 * g];
+    for (int c = 0; c < 3; ++c) {
+      const float *x = V + 3 * (size_t)T[3 * t + c];
+      const double r = (double)n[0] * x[0] + (double)n[1] * x[1] + (double)n[2] * x[2] + (double)n[3];
+      tau[g] = std::max(tau[g], std::fabs(r));
+    }
+  }
+  code.assign((size_t)nT, 0);
+  table.assign(8 * (size_t)n_groups, 0.0f);
+  int64_t kept = 0;
+  for (int64_t t = 0; t < nT; ++t) {
+    const int g = tri_group[t];
+    if (g < 0 || tau[g] > 1e-4 * ext) continue;
+    code[t] = (uint8_t)(g + 1);
+    ++kept;
+  }
+  for (int g = 0; g < n_groups; ++g) {
+    for (int k = 0; k < 4; ++k) table[8 * g + k] = nf[4 * g + k];
+    // rounded up: the slab must hold every vertex of the group
+    table[8 * g + 4] = std::nextafter((float)(tau[g] * (1.0 + 1e-6)), INFINITY);
+  }
+  return kept;
+}
+
 static int bvh_build_impl(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
-                          bool device_ptrs, vacd_bvh **out) {
+                          bool device_ptrs, vacd_bvh **out, const int8_t *tri_group = nullptr,
+                          const double *planes = nullptr, int n_groups = 0) {
   if (!ctx || !out) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
   *out = nullptr;
   vacd_bvh *b = nullptr;
@@ -190,9 +245,21 @@ static int bvh_build_impl(vacd_ctx *ctx, const float *V, int64_t nV, const uint3
       dV = stage(ctx, SLOT_VERTS, V, (size_t)(3 * nV));
       dT = stage(ctx, SLOT_TRIS, T, (size_t)(3 * nT));
     }
+    std::vector<uint8_t> code;
+    std::vector<float> table;
+    const int64_t grouped =
+        n_groups > 0 ? planar_groups(V, nV, T, nT, tri_group, planes, n_groups, code, table) : 0;
     b = new vacd_bvh();
     b->device = ctx->device;
-    vacd::build_lbvh(ctx, dV, nV, dT, nT, b);
+    const uint8_t *dgrp = grouped > 0 ? stage(ctx, SLOT_GROUPS, code.data(), code.size()) : nullptr;
+    vacd::build_lbvh(ctx, dV, nV, dT, nT, b, dgrp);
+    if (dgrp && b->n_refs > 0) {
+      b->groups.ensure(4 * table.size());
+      VACD_HIP(hipMemcpyAsync(b->groups.ptr, table.data(), 4 * table.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+      VACD_HIP(hipStreamSynchronize(ctx->stream));
+      b->n_groups = n_groups;
+    }
   });
   if (rc != VACD_OK) {
     if (b) {
@@ -208,6 +275,13 @@ static int bvh_build_impl(vacd_ctx *ctx, const float *V, int64_t nV, const uint3
 int vacd_bvh_build(vacd_ctx *ctx, const float *vertices, int64_t n_vertices, const uint32_t *tris,
                    int64_t n_tris, vacd_bvh **out) {
   return bvh_build_impl(ctx, vertices, n_vertices, tris, n_tris, false, out);
+}
+
+int vacd_bvh_build_planar(vacd_ctx *ctx, const float *vertices, int64_t n_vertices,
+                          const uint32_t *tris, int64_t n_tris, const int8_t *tri_group,
+                          const double *planes, int n_groups, vacd_bvh **out) {
+  return bvh_build_impl(ctx, vertices, n_vertices, tris, n_tris, false, out, tri_group, planes,
+                        n_groups);
 }
 
 int vacd_bvh_build_d(vacd_ctx *ctx, const float *d_vertices, int64_t n_vertices,

@@ -82,6 +82,47 @@ __device__ __forceinline__ bool seg_box(const Seg &s, float lx, float ly, float 
   return t0 <= t1;
 }
 
+// seg_box over the parameter range [t0, t1] of the segment only
+__device__ __forceinline__ bool seg_box_span(const Seg &s, float lx, float ly, float lz, float hx,
+                                             float hy, float hz, float t0, float t1) {
+  const float ax = fmaf(lx, s.inv[0], -s.pinv[0]), bx = fmaf(hx, s.inv[0], -s.pinv[0]);
+  const float ay = fmaf(ly, s.inv[1], -s.pinv[1]), by = fmaf(hy, s.inv[1], -s.pinv[1]);
+  const float az = fmaf(lz, s.inv[2], -s.pinv[2]), bz = fmaf(hz, s.inv[2], -s.pinv[2]);
+  const float u0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), t0));
+  const float u1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), t1));
+  return u0 <= u1;
+}
+
+// Parameters [t0, t1] (within [0, 1]) of the part of the segment within tau
+// of planar group `code`'s plane (grp: float4 pairs (n, d), (tau, ...)), with
+// room for the float32 rounding of this evaluation: every point of the
+// segment on a triangle of the group lies in that part.  False when empty.
+__device__ __forceinline__ bool group_span(const Seg &s, const float4 *__restrict__ grp, int code,
+                                           float &t0, float &t1) {
+  const float4 pl = grp[2 * (code - 1)];
+  const float tau = grp[2 * (code - 1) + 1].x;
+  const float dx = s.q[0] - s.p[0], dy = s.q[1] - s.p[1], dz = s.q[2] - s.p[2];
+  const float a = fmaf(pl.x, s.p[0], fmaf(pl.y, s.p[1], fmaf(pl.z, s.p[2], pl.w)));
+  const float b = fmaf(pl.x, dx, fmaf(pl.y, dy, pl.z * dz));
+  // |n| = 1: each rounding error is below 2^-23 of these magnitudes
+  const float err = 1e-6f * (fabsf(s.p[0]) + fabsf(s.p[1]) + fabsf(s.p[2]) + fabsf(pl.w) +
+                             fabsf(dx) + fabsf(dy) + fabsf(dz));
+  const float T = tau + err;
+  if (!(fabsf(b) > 1e-30f)) {
+    t0 = 0.0f, t1 = 1.0f;
+    return fabsf(a) <= T;
+  }
+  float u0 = (-T - a) / b, u1 = (T - a) / b;
+  if (u0 > u1) {
+    const float x = u0;
+    u0 = u1, u1 = x;
+  }
+  u0 -= 1e-6f * (1.0f + fabsf(u0));
+  u1 += 1e-6f * (1.0f + fabsf(u1));
+  t0 = fmaxf(u0, 0.0f), t1 = fminf(u1, 1.0f);
+  return t0 <= t1;
+}
+
 __device__ __forceinline__ bool ray_box(const Seg &s, float lx, float ly, float lz, float hx,
                                         float hy, float hz) {
   (void)lx;
@@ -275,6 +316,19 @@ __device__ __forceinline__ void load_node(const Node *__restrict__ nodes, int id
   c1 = __float_as_int(nd.f3.y);
 }
 
+// load_node plus the planar group codes of the children (0: none)
+__device__ __forceinline__ void load_node_g(const Node *__restrict__ nodes, int idx, float bx[12],
+                                            int &c0, int &c1, int &g0, int &g1) {
+  const Node nd = nodes[idx];
+  bx[0] = nd.f0.x, bx[1] = nd.f0.y, bx[2] = nd.f0.z, bx[3] = nd.f0.w;
+  bx[4] = nd.f1.x, bx[5] = nd.f1.y, bx[6] = nd.f1.z, bx[7] = nd.f1.w;
+  bx[8] = nd.f2.x, bx[9] = nd.f2.y, bx[10] = nd.f2.z, bx[11] = nd.f2.w;
+  c0 = __float_as_int(nd.f3.x);
+  c1 = __float_as_int(nd.f3.y);
+  g0 = __float_as_int(nd.f3.z);
+  g1 = __float_as_int(nd.f3.w);
+}
+
 // A walk split over a group of 2^lv lanes: lane g starts at the node reached
 // from the root by the lv bits of g (most significant first), entering a
 // child only if its box passes the query; a leaf met above level lv belongs
@@ -306,12 +360,14 @@ __device__ __forceinline__ int split_root(const Node *__restrict__ nodes, int g,
 // only), 3 malformed tree.  stk: this lane's LDS stack, stride 64 entries.
 // root: where the walk starts (a split walk's share); stop: a group flag in
 // LDS, set on a contact and polled so the group's other lanes end their
-// share early.
+// share early.  grp: planar group table of a contact tree (nullable); a child
+// of a group is tested against the group's part of the segment only.
 template <int STACK, bool COUNT, bool EXACT>
 __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
                                         const Seg &s, int *stk, int cap, unsigned &nv,
                                         unsigned &nt, unsigned &nex, int &xpar, int &ninc,
-                                        int root = 0, volatile int *stop = nullptr) {
+                                        int root = 0, volatile int *stop = nullptr,
+                                        const float4 *__restrict__ grp = nullptr) {
   bool local_ok = true, unsure = false;
   if (root < 0) {
     if (root == kNoRoot) return 0;
@@ -323,15 +379,30 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
     return unsure ? 4 : (local_ok ? 0 : 2);
   }
   int node = root, sp = 0;
+  int gc = 0;  // group whose span [gt0, gt1] was computed last
+  float gt0 = 0.0f, gt1 = 1.0f;
+  bool gin = true;
   for (int it = 0;; ++it) {
     if (it > cap) return 3;  // visited more nodes than the tree has: malformed tree
     if (stop && *stop) return 0;  // another lane of the group found a contact
     float bx[12];
-    int c0, c1;
-    load_node(nodes, node, bx, c0, c1);
+    int c0, c1, g0, g1;
+    load_node_g(nodes, node, bx, c0, c1, g0, g1);
     if (COUNT) ++nv;
-    bool h0 = seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
-    bool h1 = seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+    if (!grp) g0 = g1 = 0;
+    bool h0, h1;
+    if (g0) {
+      if (g0 != gc) gc = g0, gin = group_span(s, grp, g0, gt0, gt1);
+      h0 = gin && seg_box_span(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], gt0, gt1);
+    } else {
+      h0 = seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+    }
+    if (g1) {
+      if (g1 != gc) gc = g1, gin = group_span(s, grp, g1, gt0, gt1);
+      h1 = gin && seg_box_span(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11], gt0, gt1);
+    } else {
+      h1 = seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+    }
     if (COUNT) nt += (h0 && c0 < 0) + (h1 && c1 < 0);
     if (h0 && c0 < 0) {
       if (contact_test<EXACT>(s, tris[~c0], xpar, ninc, local_ok, unsure, nex)) {
@@ -505,6 +576,7 @@ struct Part {
   const Tri *stris;
   const Node *rnodes;  // self contact tree (slices of long triangles; else = snodes)
   const Tri *rtris;
+  const float4 *rgroups;  // planar groups of the contact tree (NULL: none)
   const Node *cnodes;  // cage (NULL: parity mode)
   const Node4 *cnodes4;
   const Tri *ctris;
@@ -546,9 +618,11 @@ __device__ __forceinline__ Part part_of_word(const VisArgs &a, int64_t w) {
 template <int STACK, bool WIDE, bool COUNT, bool EXACT>
 __device__ __forceinline__ int any_hit(const Node *n2, const Node4 *n4, const Tri *tris,
                                        const Seg &s, int *stk, int cap, unsigned &nv,
-                                       unsigned &nt, unsigned &nex, int &xpar, int &ninc) {
+                                       unsigned &nt, unsigned &nex, int &xpar, int &ninc,
+                                       const float4 *grp = nullptr) {
   if (WIDE && !EXACT) return walk_any4<STACK, COUNT>(n4, tris, s, stk, cap, nv, nt, nex, xpar, ninc);
-  return walk_any<STACK, COUNT, EXACT>(n2, tris, s, stk, cap, nv, nt, nex, xpar, ninc);
+  return walk_any<STACK, COUNT, EXACT>(n2, tris, s, stk, cap, nv, nt, nex, xpar, ninc, 0, nullptr,
+                                       grp);
 }
 
 template <int STACK, bool WIDE, bool COUNT, bool EXACT>
@@ -573,7 +647,8 @@ __device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int 
   const int hs = WIDE ? any_hit<STACK, WIDE, COUNT, EXACT>(P.snodes, P.snodes4, P.stris, s, stk,
                                                            P.scap, nv, nt, nex, xpar, ninc)
                       : any_hit<STACK, WIDE, COUNT, EXACT>(P.rnodes, P.snodes4, P.rtris, s, stk,
-                                                           P.rcap, nv, nt, nex, xpar, ninc);
+                                                           P.rcap, nv, nt, nex, xpar, ninc,
+                                                           P.rgroups);
   if (hs == 1) {
     ++n_blocked;
   } else if (hs == 3) {
@@ -746,7 +821,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
     const int root = split_root<false>(P.rnodes, g, LV, s);
     const int hs = walk_any<STACK, COUNT, EXACT>(P.rnodes, P.rtris, s, stk, P.rcap, nv, nt,
                                                  n_exact, xpar, ninc, root,
-                                                 G > 1 ? stop : nullptr);
+                                                 G > 1 ? stop : nullptr, P.rgroups);
     const uint64_t m3 = __ballot(hs == 3) & gm, m1 = __ballot(hs == 1) & gm,
                    m2 = __ballot(hs == 2) & gm, m4 = __ballot(hs == 4) & gm,
                    mx = __ballot(xpar & 1) & gm, mn = __ballot(ninc > 0) & gm;
@@ -941,7 +1016,8 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
 // (split_root at level 6).  In a batch, far is indexed by the global point.
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restrict__ far) {
-  __shared__ int stk_all[64][64];
+  // 32 entries: a binary walk holds at most log2(n_tris) - 1 < 30 (binary_stack)
+  __shared__ int stk_all[32][64];
   const int lane = threadIdx.x;
   int *stk = &stk_all[0][lane];
   for (int64_t g = blockIdx.x; g < a.npoints; g += gridDim.x) {
@@ -1178,10 +1254,12 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   P.word_begin = 0;
   P.point_begin = 0;
   P.scap = (int)std::min<int64_t>(2 * self->n_tris + 64, INT32_MAX);
+  P.rgroups = nullptr;
   if (self->n_refs > 0) {
     P.rnodes = self->rnodes.as<Node>();
     P.rtris = self->rtris.as<Tri>();
     P.rcap = (int)std::min<int64_t>(2 * self->n_refs + 64, INT32_MAX);
+    if (self->n_groups > 0) P.rgroups = self->groups.as<float4>();
   } else {
     P.rnodes = P.snodes;
     P.rtris = P.stris;
@@ -1191,9 +1269,20 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   return P;
 }
 
+// Stack entries a binary walk of a tree can hold.  Child 0 of every node has
+// no more leaves than child 1 and the walks push child 1 while descending
+// into child 0, so with s entries pushed the current subtree holds at most
+// n / 2^s leaves: s <= log2(n) - 1.  Never more than depth - 1 either.
+int binary_stack(int depth, int64_t leaves) {
+  if (leaves <= 0) return 0;
+  int lg = 0;
+  while ((int64_t(2) << lg) <= leaves) ++lg;  // floor(log2(leaves))
+  return std::min(depth - 1, lg);
+}
+
 // Enqueues far parity, the main kernel, the exact redo and the compaction for
 // a prepared VisArgs (a.nwords > 0).
-void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int depth,
+void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int depth, int stack2,
          uint32_t *d_pairs, int64_t cap, int64_t *d_count, int64_t *d_part_off) {
   hipStream_t st = ctx->stream;
   const int64_t nwords = a.nwords;
@@ -1228,9 +1317,9 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
       VACD_HIP(hipGetLastError());
     }
   }
-  // stack entries a walk can hold: binary, one pending sibling per level;
+  // stack entries a walk can hold: binary, stack2 (see binary_stack);
   // 4-wide, up to three per visited level (every other binary level)
-  const int need2 = depth - 1, need4 = 3 * (depth / 2);
+  const int need2 = stack2, need4 = 3 * (depth / 2);
   const bool wide = ctx->wide_bvh && need4 <= 64;
   const int need = wide ? need4 : need2;
   const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
@@ -1330,7 +1419,10 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
       a.one.fan_idx = fanb.as<int>() + n;
     }
   }
-  run(ctx, a, false, cage != nullptr, any_far, depth, d_pairs, cap, d_count, nullptr);
+  const int stack2 = std::max({binary_stack(self->depth, self->n_tris),
+                               binary_stack(self->depth_refs, self->n_refs),
+                               cage ? binary_stack(cage->depth, cage->n_tris) : 0});
+  run(ctx, a, false, cage != nullptr, any_far, depth, stack2, d_pairs, cap, d_count, nullptr);
 }
 
 void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, uint32_t *d_pairs,
@@ -1339,7 +1431,7 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
   reset_stats(ctx);
   std::vector<Part> hp((size_t)nparts);
   int64_t words = 0, points = 0, segs = 0;
-  int depth = 0;
+  int depth = 0, stack2 = 0;
   bool any_cage = false, any_far = false;
   for (int p = 0; p < nparts; ++p) {
     const vacd_part &q = parts[p];
@@ -1356,6 +1448,9 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
     points += q.n;
     segs += P.S;
     depth = std::max({depth, q.self->depth, q.self->depth_refs, q.cage ? q.cage->depth : 0});
+    stack2 = std::max({stack2, binary_stack(q.self->depth, q.self->n_tris),
+                       binary_stack(q.self->depth_refs, q.self->n_refs),
+                       q.cage ? binary_stack(q.cage->depth, q.cage->n_tris) : 0});
     any_cage |= q.cage != nullptr;
     any_far |= !q.cage && q.self->closed;
     hp[p] = P;
@@ -1395,7 +1490,7 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
     fill_u64(ctx, d_part_off, nparts + 1, 0ull);
     return;
   }
-  run(ctx, a, true, any_cage, any_far, depth, d_pairs, cap, d_count, d_part_off);
+  run(ctx, a, true, any_cage, any_far, depth, stack2, d_pairs, cap, d_count, d_part_off);
 }
 
 }  // namespace vacd

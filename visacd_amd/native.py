@@ -24,7 +24,8 @@ VACD_ERR_LIMIT = 4
 # Every symbol include/vacd.h declares (checked by tests/test_abi.py).
 EXPORTED = [
     "vacd_ctx_create", "vacd_ctx_destroy", "vacd_last_error", "vacd_ctx_stream",
-    "vacd_ctx_set_stream", "vacd_sync", "vacd_ctx_set_option", "vacd_ctx_timing", "vacd_bvh_build", "vacd_bvh_build_d",
+    "vacd_ctx_set_stream", "vacd_sync", "vacd_ctx_set_option", "vacd_ctx_timing", "vacd_bvh_build", "vacd_bvh_build_planar",
+    "vacd_bvh_build_d",
     "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_contact_info", "vacd_bvh_export",
     "vacd_visibility",
     "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_deferred",
@@ -99,6 +100,8 @@ def lib():
         L.vacd_ctx_timing.argtypes = [vp, ctypes.POINTER(Timing), ctypes.c_int]
         L.vacd_bvh_build.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(vp)]
         L.vacd_bvh_build_d.argtypes = [vp, vp, i64, vp, i64, ctypes.POINTER(vp)]
+        L.vacd_bvh_build_planar.argtypes = [vp, vp, i64, vp, i64, vp, vp, ctypes.c_int,
+                                            ctypes.POINTER(vp)]
         L.vacd_bvh_destroy.argtypes = [vp]
         L.vacd_bvh_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i32), vp]
         L.vacd_bvh_export.argtypes = [vp, vp, vp]
@@ -197,9 +200,21 @@ class Context:
         return {k: getattr(t, k) for k, _ in Timing._fields_}
 
     # -- BVH -----------------------------------------------------------------
-    def bvh(self, vertices, tris) -> "Bvh":
-        """BVH from numpy (host) or torch (device) arrays."""
+    def bvh(self, vertices, tris, tri_group=None, planes=None) -> "Bvh":
+        """BVH from numpy (host) or torch (device) arrays.  tri_group (int8
+        per triangle, -1 = none) and planes ((G, 4) float64) give planar
+        groups (host arrays only; see vacd_bvh_build_planar)."""
         h = ctypes.c_void_p()
+        if tri_group is not None:
+            V = np.ascontiguousarray(vertices, dtype=np.float32)
+            T = np.ascontiguousarray(tris, dtype=np.uint32)
+            g = np.ascontiguousarray(tri_group, dtype=np.int8)
+            pl = np.ascontiguousarray(planes, dtype=np.float64).reshape(-1, 4)
+            assert g.shape == (T.shape[0],)
+            rc = lib().vacd_bvh_build_planar(self._h, _p(V), V.shape[0], _p(T), T.shape[0],
+                                             _p(g), _p(pl), pl.shape[0], ctypes.byref(h))
+            self._check(rc)
+            return Bvh(h)
         if hasattr(vertices, "data_ptr"):
             rc = lib().vacd_bvh_build_d(self._h, _p(vertices), vertices.shape[0], _p(tris),
                                         tris.shape[0], ctypes.byref(h))

@@ -3,9 +3,10 @@
 The model reads 512 visibility edges of a part and a candidate plane and
 predicts the plane's cut length (the sum of the lengths of the edges whose
 endpoints lie strictly on opposite sides, the score of lib/cuda/
-plane_intersections.cu:43-66) divided by the part's total edge length.  Its
-training targets are the engine's own exact scores (vacd_score_planes,
-Q32.32):
+plane_intersections.cu:43-66) divided by the largest cut length among the
+sample's planes: only the order of a part's planes is used, so the target is
+normalised per part.  Its training targets are the engine's own exact scores
+(vacd_score_planes, Q32.32):
 
 * sample i (seed 10000 + i) is a perturbed icosphere of level U{3, 4, 5} and
   amplitude U(0.1, 0.5), normalised (longest bbox side 2), then cut down to
@@ -82,9 +83,7 @@ def make_sample(ctx, seed: int):
     planes[N_SPHERE:, :3] = nrm
     planes[N_SPHERE:, 3] = -np.einsum("ij,ij->i", nrm, 0.5 * (a + b))
     q, _, _ = ctx.score_planes(planes.astype(np.float32), pts, pairs)
-    d = pts[pairs[:, 1]].astype(np.float64) - pts[pairs[:, 0]].astype(np.float64)
-    total = float(np.linalg.norm(d, axis=1).sum())
-    target = (q.astype(np.float64) * 2.0 ** -32 / total).astype(np.float32)
+    target = (q.astype(np.float64) / max(1, int(q.max()))).astype(np.float32)
     feat, pin, _ = _visacd.proposal_inputs(v[crop], edges, planes.astype(np.float32).astype(np.float64),
                                            int(native_seed(seed)))
     info = {"level": level, "amplitude": amp, "cuts": n_cuts, "points": int(sub.size),
@@ -178,7 +177,7 @@ def predict(m, feat: np.ndarray, planes: np.ndarray, chunk: int = 64) -> np.ndar
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--samples", type=int, default=10000)
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", required=True, help="float32 parameter file to write")
