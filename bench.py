@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense
 
 
 def log(*a):
@@ -70,9 +71,11 @@ def cpu_baseline(W, seconds):
                       f"(exact predicates, CPU BVH), {dt:.1f} s"}
 
 
-def acd_wall(W, mode):
+def acd_wall(W, mode, proposal="exact"):
     """Full decomposition of the workload's mesh through visacd.process:
-    mode "plain" (the reference's loop) or "batched" (frontier rounds)."""
+    mode "plain" (the reference's loop) or "batched" (frontier rounds);
+    proposal "exact" (every candidate plane scored) or "learned" (the
+    proposal model ranks them, the top 256 and 256 audit planes are scored)."""
     import numpy as np
 
     import visacd as vis
@@ -86,6 +89,7 @@ def acd_wall(W, mode):
     vis.engine.max_points = W.n_points
     vis.engine.planes = "sphere"
     vis.engine.num_planes = int(W.planes.shape[0])
+    vis.engine.proposal = proposal
     vis.config.return_parts = False
     vis.set_seed(int(W.meta.get("seed", 42)))
     conc = float(W.meta.get("concavity", 0.04))
@@ -93,9 +97,13 @@ def acd_wall(W, mode):
     t0 = time.perf_counter()
     r = vis.process(m, conc, nparts)
     dt = time.perf_counter() - t0
-    prof = {k: round(v[0], 4) for k, v in _visacd.last_profile().items() if isinstance(v, tuple)}
+    raw = _visacd.last_profile()
+    prof = {k: round(v[0], 4) for k, v in raw.items() if isinstance(v, tuple)}
+    counters = {k: v for k, v in raw.items() if not isinstance(v, tuple)}
     vis.engine.mode = "plain"
-    return {"workload": W.name, "mode": mode, "wall_s": dt, "parts": int(r.num_parts),
+    vis.engine.proposal = "exact"
+    return {"workload": W.name, "mode": mode, "proposal": proposal, "wall_s": dt,
+            "parts": int(r.num_parts), "counters": counters,
             "concavity": float(r.concavity), "concavity_threshold": conc, "max_parts": nparts,
             "points_per_part": W.n_points, "planes_per_cut": int(W.planes.shape[0]),
             "phase_seconds": prof}
@@ -190,6 +198,66 @@ def pmc_probe(workload):
     torch.cuda.synchronize()
     bvh.close()
     ctx.close()
+
+
+def mlp_flops_per_part(E, P, D=256):
+    """Multiply-adds x 2 of one part's forward pass: the shared edge layers
+    (input padded to 8), the pooled projection once, and per plane the plane
+    term, the 256 x 256 layer and the output dot product."""
+    enc = 2 * E * (8 * D + 2 * D * D)
+    pooled = 2 * D * D
+    head = P * (2 * 4 * D + 2 * D * D + 2 * D)
+    return enc + pooled + head
+
+
+def proposal_bench(steps):
+    """Forward pass of the shipped proposal model for B parts x 512 edges x
+    16,384 planes (B = 1, 16, 64), timed with events on the launch stream."""
+    import numpy as np
+    import torch
+
+    import visacd
+    from visacd_amd import native
+
+    dev = torch.device("cuda:0")
+    ctx = native.Context(0)
+    stream = torch.cuda.Stream()  # the kernels and the events on one stream
+    ctx.set_stream(stream.cuda_stream)
+    m = native.Mlp(ctx, seed=0)
+    m.set(np.fromfile(visacd.engine.proposal_weights, dtype="<f4"))
+    E, P = 512, 16384
+    res = {}
+    torch.cuda.synchronize()
+    for B in (1, 16, 64):
+        g = torch.Generator().manual_seed(B)
+        feat = (torch.rand((B, E, 7), generator=g) * 2 - 1).to(dev)
+        nrm = torch.randn((B, P, 3), generator=g)
+        nrm = nrm / nrm.norm(dim=-1, keepdim=True)
+        planes = torch.cat([nrm, torch.rand((B, P, 1), generator=g)], -1).to(dev).contiguous()
+        out = torch.empty(B * P, device=dev)
+        torch.cuda.synchronize()
+        for _ in range(2):
+            m.forward_d(B, E, feat, P, planes, out)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            m.forward_d(B, E, feat, P, planes, out)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        fl = B * mlp_flops_per_part(E, P)
+        res[str(B)] = {"ms": ms, "parts_per_s": B / (ms * 1e-3),
+                       "gflop": fl / 1e9, "tflops": fl / (ms * 1e-3) / 1e12}
+    m.close()
+    ctx.close()
+    best = res["64"]
+    return {"edges_per_part": E, "planes_per_part": P, "by_parts": res,
+            "roofline": {"bound": "mfma", "achieved": best["tflops"],
+                         "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": best["tflops"] / MFMA_BF16_PEAK_TFLOPS,
+                         "flop_per_part": mlp_flops_per_part(E, P), "parts": 64}}
 
 
 def c3_batched(steps):
@@ -299,6 +367,7 @@ def main():
     ap.add_argument("--no-acd", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the HBM traffic runs")
     ap.add_argument("--no-c3", action="store_true", help="skip the batched 64-part step")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the proposal model timing")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_probe:
@@ -478,6 +547,9 @@ def main():
         else:
             line["roofline"]["traffic_unit"] = "bytes per visibility pass (fan pass + queue walk)"
             line["roofline"]["alg_bytes_per_pass"] = bytes_per_seg * S
+    if not args.no_mlp:
+        line["proposal_mlp"] = proposal_bench(max(3, args.steps // 2))
+        log("mlp", line["proposal_mlp"])
     if not args.no_c3:
         line["batched_frontier"] = c3_batched(max(3, args.steps // 2))
         log("c3", {k: v for k, v in line["batched_frontier"].items() if k != "autotune_table"})
@@ -486,8 +558,8 @@ def main():
 
         runs = []
         for WW in (W, wl.icosphere_c5()) if args.workload == "c2" else (W,):
-            for mode in ("plain", "batched"):
-                runs.append(acd_wall(WW, mode))
+            for mode, prop in (("plain", "exact"), ("batched", "exact"), ("batched", "learned")):
+                runs.append(acd_wall(WW, mode, prop))
                 log("acd", {k: v for k, v in runs[-1].items() if k != "phase_seconds"})
         line["acd"] = runs
     if not args.no_cpu:

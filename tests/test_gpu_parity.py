@@ -445,3 +445,58 @@ def test_visibility_planar_groups_prune_and_keep_edges(ctx):
     _print("planar prune", s0["tri_tests"], s1["tri_tests"], s0["node_visits"], s1["node_visits"])
     np.testing.assert_array_equal(e1, e0)
     assert s1["tri_tests"] < s0["tri_tests"]
+
+
+def _split_edges_at_endpoint(V, T, n_splits, seed):
+    """Splits edges (u, v) by a new vertex w at u's float32 position (moved by
+    1e-12, below float32 resolution): the two triangles at u then have zero
+    area in float32 and lie on edges of their positive-area neighbours."""
+    V = [tuple(map(float, v)) for v in V]
+    T = [list(map(int, t)) for t in T]
+    rng = np.random.default_rng(seed)
+    done = set()
+    for _ in range(n_splits):
+        t0 = int(rng.integers(len(T)))
+        u, v = T[t0][0], T[t0][1]
+        if (u, v) in done or (v, u) in done:
+            continue
+        done.add((u, v))
+        w = len(V)
+        V.append((V[u][0] + 1e-12, V[u][1], V[u][2]))
+        for t in T[:]:
+            for k in range(3):
+                a, b, c = t[k], t[(k + 1) % 3], t[(k + 2) % 3]
+                if {a, b} == {u, v}:
+                    # (a, b, c) -> (a, w, c) and (w, b, c), same orientation
+                    i = T.index(t)
+                    T[i] = [a, w, c]
+                    T.append([w, b, c])
+                    break
+    return np.array(V, np.float32), np.array(T, np.uint32)
+
+
+def test_visibility_zero_area_triangles(ctx):
+    """Zero-area triangles (two corners at one float32 position) on a closed
+    mesh: the contact walks pass them over, the edges stay the oracle's and
+    fewer pairs need the exact predicates than without the build's flags."""
+    import torch
+
+    V, T = wl.perturbed_icosphere(3, 0.3, 31)
+    Vf, Tf = _split_edges_at_endpoint(V, T, 40, 7)
+    zero = np.zeros(len(Tf), bool)
+    for k in range(3):
+        zero |= (Vf[Tf[:, k]] == Vf[Tf[:, (k + 1) % 3]]).all(1)
+    _print("zero-area triangles", int(zero.sum()), "of", len(Tf))
+    assert zero.sum() >= 40
+    flagged = ctx.bvh(Vf, Tf)  # host build: flags the covered zero-area triangles
+    plain = ctx.bvh(torch.from_numpy(Vf).cuda(), torch.from_numpy(Tf.astype(np.int32)).cuda())
+    pts = Vf
+    o, _ = oracle.visibility(pts, Vf, Tf)
+    g1 = ctx.visibility(pts, flagged)
+    s1 = ctx.visibility_stats()
+    g0 = ctx.visibility(pts, plain)
+    s0 = ctx.visibility_stats()
+    _print("exact pairs with / without flags", s1["exact_calls"], s0["exact_calls"])
+    np.testing.assert_array_equal(g1, o)
+    np.testing.assert_array_equal(g0, o)
+    assert s1["exact_calls"] <= s0["exact_calls"]

@@ -120,3 +120,45 @@ def test_flat_surfaces_of_a_box():
     assert len(planes) == 6
     for a, b, c, d in planes:
         assert abs(np.linalg.norm([a, b, c]) - 1) < 1e-12
+
+
+def _cone(n=64, h=2e-6):
+    """Closed cone: apex just above z = 0 (outside the 1e-6 dead zone), base
+    ring at z = -1, base centre below.  The plane z = 0 crosses the n slanted
+    edges within about 2e-6 of the apex, so the crossings lie closer together
+    than float32 resolution at the mesh's scale."""
+    ang = 2 * np.pi * np.arange(n) / n
+    V = [[0.0, 0.0, h]] + [[np.cos(a), np.sin(a), -1.0] for a in ang] + [[0.0, 0.0, -1.0]]
+    T = []
+    for i in range(n):
+        a, b = 1 + i, 1 + (i + 1) % n
+        T.append([0, a, b])           # side, outward
+        T.append([n + 1, b, a])       # base, outward (down)
+    return np.array(V), np.array(T)
+
+
+def test_clip_welds_crossings_closer_than_float32_resolution():
+    V, T = _cone()
+    m = _mesh(V, T)
+    vol = hs.mesh_volume(m)
+    assert vol > 0
+    parts, mp, mn, flags = hs.clip(m, [0.0, 0.0, 1.0, 0.0])
+    assert len(parts) == 2
+    total = 0.0
+    for p, f in zip(parts, flags):
+        pv, pt = _arrays(p)
+        assert _closed(pt.tolist())
+        assert len(f) == len(pv)
+        n_old = sum(1 for x in f if not x)
+        assert all(not x for x in f[:n_old]) and all(f[n_old:])
+        new = np.flatnonzero(f)
+        # no two new vertices left within the weld distance, nor at one float32 position
+        q = pv[new]
+        d = np.linalg.norm(q[:, None] - q[None], axis=2) + np.eye(len(q))
+        assert d.min() >= 4e-7 * np.abs(pv).max()
+        assert len(np.unique(q.astype(np.float32), axis=0)) == len(q)
+        total += hs.mesh_volume(p)
+    # welding moves cap points by less than 1e-6 next to the tiny cross-section
+    assert abs(total - vol) < 1e-6 * vol
+    # the maps of old vertices are unchanged by the weld
+    assert mp[0] == 1 and all(mn[i] == i for i in range(1, len(V)))

@@ -7,9 +7,11 @@
 // them and flagged is_new (lib/src/clip.cpp:664-673).  The cap polygons are
 // triangulated here by ear clipping with hole bridging (D. Eberly,
 // "Triangulation by Ear Clipping", Geometric Tools, 2002) instead of the
-// reference's constrained Delaunay triangulation.
+// reference's constrained Delaunay triangulation.  New vertices closer than
+// float32 resolution are then welded (weld_short_new_edges).
 #include <algorithm>
 #include <cmath>
+#include <iterator>
 #include <map>
 #include <unordered_map>
 
@@ -217,6 +219,98 @@ bool build_cap(const std::vector<std::pair<int, int>> &edges, const std::vector<
   return true;
 }
 
+// Collapses the edges between two new (cap) vertices shorter than 4e-7 of the
+// part's largest coordinate: crossing points that close round to one float32
+// position, and the triangles between them would have zero area in the
+// visibility engine's float32 geometry.  An edge (u, v) is collapsed into u
+// when its two triangles are the only ones holding both (the link
+// condition), which keeps the mesh closed and manifold; the two triangles go.
+// Only new vertices are removed, and they follow the old ones, so the parent
+// maps stay valid.  Returns the number of collapsed edges.
+int weld_short_new_edges(Mesh &m) {
+  const int nv = (int)m.vertices.size();
+  double scale = 0.0;
+  for (const Vec3 &v : m.vertices)
+    for (int k = 0; k < 3; ++k) scale = std::max(scale, std::fabs(v[k]));
+  const double delta = 4e-7 * scale;
+  auto fresh = [&](int v) { return v < (int)m.is_new.size() && m.is_new[v]; };
+  std::vector<std::pair<double, std::pair<int, int>>> cand;
+  for (const auto &t : m.triangles)
+    for (int k = 0; k < 3; ++k) {
+      const int u = t[k], v = t[(k + 1) % 3];
+      if (u < v && fresh(u) && fresh(v)) {
+        const double d = norm(m.vertices[u] - m.vertices[v]);
+        if (d < delta) cand.push_back({d, {u, v}});
+      }
+    }
+  if (cand.empty()) return 0;
+  std::sort(cand.begin(), cand.end());
+  std::vector<std::vector<int>> vt(nv);  // vertex -> triangles (may hold dead ones)
+  for (int t = 0; t < (int)m.triangles.size(); ++t)
+    for (int k = 0; k < 3; ++k) vt[m.triangles[t][k]].push_back(t);
+  std::vector<char> dead_t(m.triangles.size(), 0), dead_v(nv, 0);
+  auto holds = [&](int t, int x) {
+    const auto &r = m.triangles[t];
+    return r[0] == x || r[1] == x || r[2] == x;
+  };
+  auto ring = [&](int u) {
+    std::vector<int> r;
+    for (int t : vt[u])
+      if (!dead_t[t] && holds(t, u))
+        for (int k = 0; k < 3; ++k)
+          if (m.triangles[t][k] != u) r.push_back(m.triangles[t][k]);
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+    return r;
+  };
+  int collapsed = 0;
+  for (const auto &c : cand) {
+    const int u = c.second.first, v = c.second.second;
+    if (dead_v[u] || dead_v[v]) continue;
+    std::vector<int> both;  // live triangles holding u and v
+    for (int t : vt[u])
+      if (!dead_t[t] && holds(t, u) && holds(t, v)) both.push_back(t);
+    std::sort(both.begin(), both.end());
+    both.erase(std::unique(both.begin(), both.end()), both.end());
+    if (both.size() != 2) continue;
+    std::vector<int> opp;
+    for (int t : both)
+      for (int k = 0; k < 3; ++k)
+        if (m.triangles[t][k] != u && m.triangles[t][k] != v) opp.push_back(m.triangles[t][k]);
+    std::sort(opp.begin(), opp.end());
+    if (opp.size() != 2 || opp[0] == opp[1]) continue;
+    const std::vector<int> ru = ring(u), rv = ring(v);
+    std::vector<int> common;
+    std::set_intersection(ru.begin(), ru.end(), rv.begin(), rv.end(), std::back_inserter(common));
+    if (common != opp) continue;  // link condition
+    for (int t : both) dead_t[t] = 1;
+    for (int t : vt[v]) {
+      if (dead_t[t] || !holds(t, v)) continue;
+      for (int k = 0; k < 3; ++k)
+        if (m.triangles[t][k] == v) m.triangles[t][k] = u;
+      vt[u].push_back(t);
+    }
+    dead_v[v] = 1;
+    ++collapsed;
+  }
+  if (!collapsed) return 0;
+  std::vector<int> id(nv, -1);
+  Mesh out;
+  for (int i = 0; i < nv; ++i)
+    if (!dead_v[i]) {
+      id[i] = (int)out.vertices.size();
+      out.vertices.push_back(m.vertices[i]);
+      out.is_new.push_back(fresh(i));
+    }
+  for (int t = 0; t < (int)m.triangles.size(); ++t)
+    if (!dead_t[t])
+      out.triangles.push_back({id[m.triangles[t][0]], id[m.triangles[t][1]], id[m.triangles[t][2]]});
+  m.vertices.swap(out.vertices);
+  m.triangles.swap(out.triangles);
+  m.is_new.swap(out.is_new);
+  return collapsed;
+}
+
 }  // namespace
 
 MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::vector<int> &map_neg) {
@@ -311,6 +405,9 @@ MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::ve
       return out;
     }
     for (auto &t : cap) part.triangles.push_back({local(~t[0]), local(~t[1]), local(~t[2])});
+    // a collapse can leave a new short edge next to it: repeat until none
+    for (int pass = 0; pass < 8 && weld_short_new_edges(part) > 0; ++pass) {
+    }
     out.push_back(std::move(part));
   }
   return out;

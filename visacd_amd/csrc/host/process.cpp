@@ -364,6 +364,33 @@ double edge_score(const Mesh &m) {
 // ---------------------------------------------------------------- batched
 // Visibility of many parts in one batched call (masked: pairs touching a
 // vertex of the last cut).  Edges are appended to each part in k order.
+// Verbose diagnostic: which part of a batch most of the exactly redone pairs
+// came from, with its size and cut planes.
+void log_exact_hotspot(const Engine &E, const std::vector<Mesh *> &parts,
+                       const std::vector<size_t> &idx, const std::vector<vacd_part> &vp,
+                       const vacd_vis_stats &st) {
+  std::vector<int64_t> ids((size_t)st.exact_calls);
+  int64_t cnt = 0;
+  if (vacd_visibility_deferred(E.ctx, ids.data(), (int64_t)ids.size(), &cnt) != VACD_OK) return;
+  std::vector<int64_t> first(vp.size() + 1, 0);  // first pair id of each part
+  for (size_t j = 0; j < vp.size(); ++j)
+    first[j + 1] = first[j] + 64 * ((vp[j].n * (vp[j].n - 1) / 2 + 63) / 64);
+  std::vector<int64_t> per(vp.size(), 0);
+  for (int64_t t = 0; t < std::min<int64_t>(cnt, (int64_t)ids.size()); ++t) {
+    const size_t j = (size_t)(std::upper_bound(first.begin(), first.end(), ids[t]) - first.begin()) - 1;
+    if (j < per.size()) ++per[j];
+  }
+  const size_t j = (size_t)(std::max_element(per.begin(), per.end()) - per.begin());
+  const Mesh &m = *parts[idx[j]];
+  std::ostringstream o;
+  o << "visibility: " << st.exact_calls << " pairs redone exactly; " << per[j] << " in a part of "
+    << m.vertices.size() << " vertices, " << m.triangles.size() << " triangles, " << vp[j].n
+    << " points; cut planes:";
+  o.precision(9);
+  for (const Plane &q : m.cut_planes) o << " (" << q.a << ", " << q.b << ", " << q.c << ", " << q.d << ")";
+  log_line(o.str());
+}
+
 void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, bool with_mask,
                           PairBuffer &buf) {
   PhaseTimer timer("visibility");
@@ -420,6 +447,7 @@ void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, boo
         last_profile.calls["visibility.segments"] += st.segments;
         last_profile.calls["visibility.walked"] += st.walked;
         last_profile.calls["visibility.exact_pairs"] += st.exact_calls;
+        if (engine.verbose && st.exact_calls > 100000) log_exact_hotspot(E, parts, idx, vp, st);
       }
       last_profile.calls["visibility.edges"] += count;
       for (size_t j = 0; j < idx.size(); ++j) {

@@ -277,7 +277,8 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ key
 __global__ __launch_bounds__(256) void k_leaves(const float *__restrict__ V,
                                                 const uint32_t *__restrict__ T,
                                                 const uint32_t *__restrict__ order, int64_t n,
-                                                Tri *__restrict__ tris) {
+                                                Tri *__restrict__ tris,
+                                                const uint8_t *__restrict__ skip) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t t = order[j];
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(256) void k_leaves(const float *__restrict__ V,
   Tri r;
   r.a = make_float4(a[0], a[1], a[2], __uint_as_float(t));
   r.b = make_float4(b[0], b[1], b[2], 0.0f);
-  r.c = make_float4(c[0], c[1], c[2], 0.0f);
+  r.c = make_float4(c[0], c[1], c[2], __int_as_float(skip ? (int)skip[t] : 0));
   tris[j] = r;
 }
 
@@ -640,7 +641,8 @@ __global__ __launch_bounds__(256) void k_leaves_refs(const float *__restrict__ V
                                                      const uint32_t *__restrict__ ref_tri,
                                                      const uint32_t *__restrict__ ref_meta,
                                                      const uint32_t *__restrict__ order, int64_t n,
-                                                     Tri *__restrict__ tris) {
+                                                     Tri *__restrict__ tris,
+                                                     const uint8_t *__restrict__ skip) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t r = order[j];
@@ -651,7 +653,7 @@ __global__ __launch_bounds__(256) void k_leaves_refs(const float *__restrict__ V
   Tri x;
   x.a = make_float4(a[0], a[1], a[2], __uint_as_float(t));
   x.b = make_float4(b[0], b[1], b[2], __uint_as_float(ref_meta[r]));
-  x.c = make_float4(c[0], c[1], c[2], 0.0f);
+  x.c = make_float4(c[0], c[1], c[2], __int_as_float(skip ? (int)skip[t] : 0));
   tris[j] = x;
 }
 
@@ -663,7 +665,7 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 // slices would exceed four per triangle, or when the tree is too deep for
 // the traversal stacks.
 void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, int64_t nT,
-                        vacd_bvh *out, const uint8_t *d_grp) {
+                        vacd_bvh *out, const uint8_t *d_grp, const uint8_t *d_skip) {
   out->n_refs = 0;
   out->depth_refs = 0;
   if (nT < 2) return;
@@ -745,7 +747,8 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
   VACD_HIP(hipMemsetAsync(depth, 0, 4, st));
   VACD_HIP(hipMemsetAsync(nodes, 0, sizeof(Node) * (size_t)(R - 1), st));
   k_karras<<<grid_for(R - 1, 256), 256, 0, st>>>(k0, R, nodes, pint, pleaf, d_grp ? 27 : 0);
-  k_leaves_refs<<<grid_for(R, 256), 256, 0, st>>>(d_V, d_T, ref_tri, ref_meta, v0, R, tris);
+  k_leaves_refs<<<grid_for(R, 256), 256, 0, st>>>(d_V, d_T, ref_tri, ref_meta, v0, R, tris,
+                                                  d_skip);
   k_refit<<<grid_for(R, 256), 256, 0, st>>>(tris, R, nodes, pint, pleaf, flags, bounds, root, blo,
                                             bhi, v0);
   k_depth<<<grid_for(R, 256), 256, 0, st>>>(R, pint, pleaf, depth);
@@ -765,7 +768,7 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
 }  // namespace
 
 void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T, int64_t nT,
-                vacd_bvh *out, const uint8_t *d_grp) {
+                vacd_bvh *out, const uint8_t *d_grp, const uint8_t *d_skip) {
   (void)nV;
   VACD_REQUIRE(nT >= 1 && nT < (int64_t(1) << 30), VACD_ERR_LIMIT,
                "BVH needs 1 <= n_tris < 2^30");
@@ -832,7 +835,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
     VACD_HIP(hipMemsetAsync(nodes, 0, sizeof(Node) * (size_t)n_int, st));
     k_karras<<<grid_for(nT - 1, 256), 256, 0, st>>>(k0, nT, nodes, pint, pleaf);
   }
-  k_leaves<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, v0, nT, tris);
+  k_leaves<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, v0, nT, tris, d_skip);
   k_refit<<<grid_for(nT, 256), 256, 0, st>>>(tris, nT, nodes, pint, pleaf, flags, bounds, root,
                                              nullptr, nullptr, nullptr);
   k_depth<<<grid_for(nT, 256), 256, 0, st>>>(nT, pint, pleaf, depth);
@@ -856,7 +859,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   out->depth = h_depth;
   out->closed = h_odd == 0;
   for (int k = 0; k < 6; ++k) out->bounds[k] = h_root[k];
-  build_contact_tree(ctx, d_V, d_T, nT, out, d_grp);
+  build_contact_tree(ctx, d_V, d_T, nT, out, d_grp, d_skip);
 }
 
 }  // namespace vacd

@@ -149,6 +149,9 @@ __device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t, int &xpa
   const bool pa = same3(s.p, a), pb = same3(s.p, b), pc = same3(s.p, c);
   const bool ip = pa || pb || pc;
   const bool iq = same3(s.q, a) || same3(s.q, b) || same3(s.q, c);
+  // a zero-area triangle flagged at the build: its points lie on an edge of
+  // a neighbour of positive area, which is tested on its own
+  if (!ip && !iq && (__float_as_int(t.c.w) & 1)) return 0;
   if (ip && iq) return 1;  // the segment is an edge of the triangle
   if (iq) {
     double bound;
