@@ -38,8 +38,7 @@ static_assert(sizeof(Node4) == 128, "node4 is 128 bytes");
 // reject it (an inverted box would not be rejected by the slab test).
 constexpr float kEmpty = 1e30f;
 
-// Leaf-ordered triangle (48 B): a.xyz + original id bits, b.xyz, c.xyz; c.w
-// bit 0 (int bits): a zero-area triangle the contact tests may pass over.
+// Leaf-ordered triangle (48 B): a.xyz + original id bits, b.xyz, c.xyz.
 struct alignas(16) Tri {
   float4 a, b, c;
 };
@@ -119,7 +118,6 @@ enum VacdSlot {
   SLOT_SCORE_TABLES,
   SLOT_MLP_IN,
   SLOT_GROUPS,
-  SLOT_SKIP,
   SLOT_COUNT_MAX
 };
 
@@ -202,6 +200,8 @@ struct vacd_bvh {
   int device = 0;
   int64_t n_tris = 0;
   int32_t depth = 0;
+  // trees deeper than 33 levels store the child with fewer leaves first
+  bool small_first = false, small_first_refs = false;
   // every undirected edge (by vertex index) has an even number of incident
   // triangles: crossing parity is then the same from any point of an open
   // segment that touches no triangle
@@ -242,11 +242,9 @@ namespace vacd {
 
 // LBVH build from device vertices/triangles (lbvh.hip).  d_grp (nullable):
 // per triangle its planar group code, 0 = none, 1..7 = group 0..6.
-// d_skip (nullable): per triangle 1 if contact tests may pass it over
-// (stored as bit 0 of the leaf's c.w).
 void build_lbvh(vacd_ctx *ctx, const float *d_vertices, int64_t n_vertices,
                 const uint32_t *d_tris, int64_t n_tris, vacd_bvh *out,
-                const uint8_t *d_grp = nullptr, const uint8_t *d_skip = nullptr);
+                const uint8_t *d_grp = nullptr);
 
 // Visibility (visibility.hip): enqueue everything; writes pairs and count.
 void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n,

@@ -11,7 +11,6 @@
 #include <cmath>
 #include <cstring>
 #include <new>
-#include <unordered_map>
 #include <vector>
 
 using vacd::Error;
@@ -225,54 +224,6 @@ static int64_t planar_groups(const float *V, int64_t nV, const uint32_t *T, int6
   return kept;
 }
 
-// Triangles with two corners at the same float32 position (zero area) whose
-// points all lie on an edge shared with a triangle of positive area: a
-// contact with them is a contact with that neighbour, so the contact walks
-// may pass them over (their tests are all degenerate and would otherwise go
-// to the exact predicates).  Returns how many; skip[t] = 1 marks them.
-static int64_t contact_skips(const float *V, int64_t nV, const uint32_t *T, int64_t nT,
-                             std::vector<uint8_t> &skip) {
-  auto same = [&](uint32_t a, uint32_t b) {
-    return V[3 * (size_t)a] == V[3 * (size_t)b] && V[3 * (size_t)a + 1] == V[3 * (size_t)b + 1] &&
-           V[3 * (size_t)a + 2] == V[3 * (size_t)b + 2];
-  };
-  auto degenerate = [&](int64_t t) {
-    const uint32_t a = T[3 * t], b = T[3 * t + 1], c = T[3 * t + 2];
-    return same(a, b) || same(b, c) || same(c, a);
-  };
-  std::vector<int64_t> deg;
-  for (int64_t t = 0; t < nT; ++t)
-    if (degenerate(t)) deg.push_back(t);
-  if (deg.empty()) return 0;
-  // edges of the triangles around the degenerate ones: key (lo, hi) -> triangles
-  std::vector<uint8_t> near((size_t)nV, 0);
-  for (int64_t t : deg)
-    for (int k = 0; k < 3; ++k) near[T[3 * t + k]] = 1;
-  std::unordered_map<uint64_t, std::vector<int64_t>> around;
-  for (int64_t t = 0; t < nT; ++t)
-    for (int k = 0; k < 3; ++k) {
-      const uint32_t u = T[3 * t + k], v = T[3 * t + (k + 1) % 3];
-      if (!near[u] || !near[v]) continue;
-      const uint64_t key = ((uint64_t)std::min(u, v) << 32) | std::max(u, v);
-      around[key].push_back(t);
-    }
-  skip.assign((size_t)nT, 0);
-  int64_t n = 0;
-  for (int64_t t : deg) {
-    bool covered = false;
-    for (int k = 0; k < 3 && !covered; ++k) {
-      const uint32_t u = T[3 * t + k], v = T[3 * t + (k + 1) % 3];
-      if (same(u, v)) continue;  // a zero-length edge covers nothing
-      // every point of the triangle lies on its edges of positive length
-      const uint64_t key = ((uint64_t)std::min(u, v) << 32) | std::max(u, v);
-      for (int64_t o : around[key])
-        if (o != t && !degenerate(o)) covered = true;
-    }
-    if (covered) skip[t] = 1, ++n;
-  }
-  return n;
-}
-
 static int bvh_build_impl(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
                           bool device_ptrs, vacd_bvh **out, const int8_t *tri_group = nullptr,
                           const double *planes = nullptr, int n_groups = 0) {
@@ -301,11 +252,7 @@ static int bvh_build_impl(vacd_ctx *ctx, const float *V, int64_t nV, const uint3
     b = new vacd_bvh();
     b->device = ctx->device;
     const uint8_t *dgrp = grouped > 0 ? stage(ctx, SLOT_GROUPS, code.data(), code.size()) : nullptr;
-    std::vector<uint8_t> skip;
-    const uint8_t *dskip = nullptr;
-    if (!device_ptrs && contact_skips(V, nV, T, nT, skip) > 0)
-      dskip = stage(ctx, SLOT_SKIP, skip.data(), skip.size());
-    vacd::build_lbvh(ctx, dV, nV, dT, nT, b, dgrp, dskip);
+    vacd::build_lbvh(ctx, dV, nV, dT, nT, b, dgrp);
     if (dgrp && b->n_refs > 0) {
       b->groups.ensure(4 * table.size());
       VACD_HIP(hipMemcpyAsync(b->groups.ptr, table.data(), 4 * table.size(), hipMemcpyHostToDevice,

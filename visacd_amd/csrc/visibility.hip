@@ -149,9 +149,6 @@ __device__ __forceinline__ int contact_fast(const Seg &s, const Tri &t, int &xpa
   const bool pa = same3(s.p, a), pb = same3(s.p, b), pc = same3(s.p, c);
   const bool ip = pa || pb || pc;
   const bool iq = same3(s.q, a) || same3(s.q, b) || same3(s.q, c);
-  // a zero-area triangle flagged at the build: its points lie on an edge of
-  // a neighbour of positive area, which is tested on its own
-  if (!ip && !iq && (__float_as_int(t.c.w) & 1)) return 0;
   if (ip && iq) return 1;  // the segment is an edge of the triangle
   if (iq) {
     double bound;
@@ -365,12 +362,12 @@ __device__ __forceinline__ int split_root(const Node *__restrict__ nodes, int g,
 // LDS, set on a contact and polled so the group's other lanes end their
 // share early.  grp: planar group table of a contact tree (nullable); a child
 // of a group is tested against the group's part of the segment only.
-template <int STACK, bool COUNT, bool EXACT>
-__device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
-                                        const Seg &s, int *stk, int cap, unsigned &nv,
-                                        unsigned &nt, unsigned &nex, int &xpar, int &ninc,
-                                        int root = 0, volatile int *stop = nullptr,
-                                        const float4 *__restrict__ grp = nullptr) {
+template <int STACK, bool COUNT, bool EXACT, bool GROUPS>
+__device__ __forceinline__ int walk_any_t(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
+                                          const Seg &s, int *stk, int cap, unsigned &nv,
+                                          unsigned &nt, unsigned &nex, int &xpar, int &ninc,
+                                          int root, volatile int *stop,
+                                          const float4 *__restrict__ grp) {
   bool local_ok = true, unsure = false;
   if (root < 0) {
     if (root == kNoRoot) return 0;
@@ -389,18 +386,20 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
     if (it > cap) return 3;  // visited more nodes than the tree has: malformed tree
     if (stop && *stop) return 0;  // another lane of the group found a contact
     float bx[12];
-    int c0, c1, g0, g1;
-    load_node_g(nodes, node, bx, c0, c1, g0, g1);
+    int c0, c1, g0 = 0, g1 = 0;
+    if (GROUPS)
+      load_node_g(nodes, node, bx, c0, c1, g0, g1);
+    else
+      load_node(nodes, node, bx, c0, c1);
     if (COUNT) ++nv;
-    if (!grp) g0 = g1 = 0;
     bool h0, h1;
-    if (g0) {
+    if (GROUPS && g0) {
       if (g0 != gc) gc = g0, gin = group_span(s, grp, g0, gt0, gt1);
       h0 = gin && seg_box_span(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], gt0, gt1);
     } else {
       h0 = seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
     }
-    if (g1) {
+    if (GROUPS && g1) {
       if (g1 != gc) gc = g1, gin = group_span(s, grp, g1, gt0, gt1);
       h1 = gin && seg_box_span(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11], gt0, gt1);
     } else {
@@ -434,6 +433,21 @@ __device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tr
     }
   }
   return unsure ? 4 : (local_ok ? 0 : 2);
+}
+
+// walk_any_t with the planar-group pruning compiled in only when the tree
+// has groups (a uniform branch outside the walk loop)
+template <int STACK, bool COUNT, bool EXACT>
+__device__ __forceinline__ int walk_any(const Node *__restrict__ nodes, const Tri *__restrict__ tris,
+                                        const Seg &s, int *stk, int cap, unsigned &nv,
+                                        unsigned &nt, unsigned &nex, int &xpar, int &ninc,
+                                        int root = 0, volatile int *stop = nullptr,
+                                        const float4 *__restrict__ grp = nullptr) {
+  if (grp)
+    return walk_any_t<STACK, COUNT, EXACT, true>(nodes, tris, s, stk, cap, nv, nt, nex, xpar, ninc,
+                                                 root, stop, grp);
+  return walk_any_t<STACK, COUNT, EXACT, false>(nodes, tris, s, stk, cap, nv, nt, nex, xpar, ninc,
+                                                root, stop, nullptr);
 }
 
 // Crossing parity of the midpoint ray: 0 even, 1 odd, 2 undecided (filtered
@@ -1019,7 +1033,8 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
 // (split_root at level 6).  In a batch, far is indexed by the global point.
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restrict__ far) {
-  // 32 entries: a binary walk holds at most log2(n_tris) - 1 < 30 (binary_stack)
+  // a walk holds at most 32 entries: depth - 1 <= 32, or small-first order
+  // (binary_stack)
   __shared__ int stk_all[32][64];
   const int lane = threadIdx.x;
   int *stk = &stk_all[0][lane];
@@ -1272,12 +1287,14 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   return P;
 }
 
-// Stack entries a binary walk of a tree can hold.  Child 0 of every node has
-// no more leaves than child 1 and the walks push child 1 while descending
-// into child 0, so with s entries pushed the current subtree holds at most
-// n / 2^s leaves: s <= log2(n) - 1.  Never more than depth - 1 either.
-int binary_stack(int depth, int64_t leaves) {
+// Stack entries a binary walk of a tree can hold: depth - 1, one pending
+// sibling per level.  In a small-first tree child 0 of every node has no more
+// leaves than child 1 and the walks push child 1 while descending into child
+// 0, so with s entries pushed the current subtree holds at most n / 2^s
+// leaves: s <= log2(n) - 1.
+int binary_stack(int depth, int64_t leaves, bool small_first) {
   if (leaves <= 0) return 0;
+  if (!small_first) return depth - 1;
   int lg = 0;
   while ((int64_t(2) << lg) <= leaves) ++lg;  // floor(log2(leaves))
   return std::min(depth - 1, lg);
@@ -1422,9 +1439,9 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
       a.one.fan_idx = fanb.as<int>() + n;
     }
   }
-  const int stack2 = std::max({binary_stack(self->depth, self->n_tris),
-                               binary_stack(self->depth_refs, self->n_refs),
-                               cage ? binary_stack(cage->depth, cage->n_tris) : 0});
+  const int stack2 = std::max({binary_stack(self->depth, self->n_tris, self->small_first),
+                               binary_stack(self->depth_refs, self->n_refs, self->small_first_refs),
+                               cage ? binary_stack(cage->depth, cage->n_tris, cage->small_first) : 0});
   run(ctx, a, false, cage != nullptr, any_far, depth, stack2, d_pairs, cap, d_count, nullptr);
 }
 
@@ -1451,9 +1468,9 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
     points += q.n;
     segs += P.S;
     depth = std::max({depth, q.self->depth, q.self->depth_refs, q.cage ? q.cage->depth : 0});
-    stack2 = std::max({stack2, binary_stack(q.self->depth, q.self->n_tris),
-                       binary_stack(q.self->depth_refs, q.self->n_refs),
-                       q.cage ? binary_stack(q.cage->depth, q.cage->n_tris) : 0});
+    stack2 = std::max({stack2, binary_stack(q.self->depth, q.self->n_tris, q.self->small_first),
+                       binary_stack(q.self->depth_refs, q.self->n_refs, q.self->small_first_refs),
+                       q.cage ? binary_stack(q.cage->depth, q.cage->n_tris, q.cage->small_first) : 0});
     any_cage |= q.cage != nullptr;
     any_far |= !q.cage && q.self->closed;
     hp[p] = P;
