@@ -742,6 +742,26 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
               decided = true;  // starts inside
             }
           }
+          // The same from q's end: a pair that enters the solid at q (the
+          // segment seen from q towards p starts inside) is blocked or
+          // wholly inside too.  Most pairs that leave p outwards and are not
+          // edges pass through the solid and are settled here, so only
+          // pairs that leave both ends outwards are walked.
+          const int fq = decided ? 0 : P.fan_n[j];
+          if (fq > 0) {
+            Seg r_s;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) r_s.p[c] = s.q[c], r_s.q[c] = s.p[c];
+            int xq = 0, nq = 0, r = 0;
+            for (int f = 0; f < fq && r == 0; ++f)
+              r = contact_fast(r_s, P.stris[P.fan_idx[(size_t)j * kFanMax + f]], xq, nq);
+            if (r == 1) {
+              ++n_blocked;
+              decided = true;
+            } else if (r == 0 && P.far[j] <= 1 && ((xq ^ P.far[j]) & 1)) {
+              decided = true;  // enters the solid at q
+            }
+          }
         }
         if (!decided) {
           if constexpr (QUEUE) {
