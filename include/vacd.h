@@ -276,6 +276,25 @@ int vacd_max_distance(vacd_ctx *ctx, const float *pts, int64_t m,
 int vacd_max_distance_d(vacd_ctx *ctx, const float *d_pts, int64_t m,
                         const vacd_bvh *mesh, float *d_out);
 
+/* Batched form for the concavity metric: for each query, the largest
+ * distance from its m points to its triangle mesh (float32 vertices
+ * [nV][3], triangles [nT][3] of vertex ids), every point measured against
+ * every triangle (no BVH to build; same value as the BVH form).  One launch
+ * for all queries; out[nq] is written when the call returns (+inf for a
+ * query with points but no triangles, 0 for one without points).  Replaces,
+ * for a whole decomposition round at once, the per-part Hausdorff sampling
+ * of lib/src/cost.cpp:69-105. */
+typedef struct vacd_distance_query {
+  const float *pts;
+  int64_t m;
+  const float *V;
+  int64_t nV;
+  const uint32_t *T;
+  int64_t nT;
+} vacd_distance_query;
+int vacd_max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int32_t nq,
+                            float *out);
+
 /* Reference-signature shim: same arguments and behaviour as
  * classify_and_rate_planes (lib/cuda/plane_intersections.cu:71-78,
  * declared at lib/src/process.cpp:74-79): host pointers, synchronous,

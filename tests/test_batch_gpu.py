@@ -39,6 +39,40 @@ def test_max_distance_matches_brute_force(ctx):
     assert ctx.max_distance(V, b) < 1e-6  # mesh vertices lie on the mesh
 
 
+def test_max_distance_batch_matches_bvh_form_and_brute_force(ctx):
+    """The batched brute-force distance equals the BVH walk's value for every
+    query of a mixed batch: meshes of several sizes (more than one triangle
+    chunk), points inside, outside and on the mesh, an empty query and one
+    with no triangles."""
+    rng = np.random.default_rng(5)
+    queries, want = [], []
+    for s, (level, m) in enumerate(((1, 40), (3, 700), (4, 300), (2, 1))):
+        V, T = wl.perturbed_icosphere(level, 0.3, 40 + s)
+        pts = rng.uniform(-1.4, 1.4, size=(m, 3)).astype(np.float32)
+        pts[: m // 4] = V[rng.integers(0, V.shape[0], m // 4)]  # some on the mesh
+        print("distance batch query", s, "level", level, "points", m, flush=True)
+        b = ctx.bvh(V, T)
+        want.append(ctx.max_distance(pts, b))
+        b.close()
+        queries.append((pts, V, T))
+    V, T = wl.perturbed_icosphere(1, 0.2, 3)
+    queries.append((np.zeros((0, 3), np.float32), V, T))  # no points: 0
+    queries.append((V[:5], V, np.zeros((0, 3), np.uint32)))  # no triangles: inf
+    got = ctx.max_distance_batch(queries)
+    # same minimum over the same float32 triangle distances (up to the
+    # compiler's contraction choices in the two kernels)
+    assert np.allclose(got[:4], np.array(want, dtype=np.float32), rtol=1e-6, atol=1e-7), (got, want)
+    assert got[4] == 0.0 and np.isinf(got[5])
+    # brute force in float64 on the small query
+    pts, V, T = queries[0]
+    V64 = V.astype(np.float64)
+    ref = max(min(_point_tri_dist(p.astype(np.float64), V64[t[0]], V64[t[1]], V64[t[2]])
+                  for t in T) for p in pts)
+    assert abs(got[0] - ref) <= 1e-5 * max(1.0, ref)
+    with pytest.raises(Exception):
+        ctx.max_distance_batch([(pts, V, T + 1000)])  # vertex id out of range
+
+
 def test_batch_matches_oracle_per_part(ctx):
     import torch
 

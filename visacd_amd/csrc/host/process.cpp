@@ -48,9 +48,13 @@ struct PhaseTimer {
   const char *phase;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   explicit PhaseTimer(const char *p) : phase(p) {}
-  ~PhaseTimer() {
+  bool done = false;
+  void stop() {
+    if (done) return;
+    done = true;
     last_profile.add(phase, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   }
+  ~PhaseTimer() { stop(); }
 };
 
 struct Engine {
@@ -261,6 +265,32 @@ vacd_bvh *build_bvh(const Engine &E, const Mesh &m, bool planar) {
   return b;
 }
 
+// One point-to-mesh distance query in the layout of vacd_max_distance_batch.
+struct DistQuery {
+  std::vector<float> pts, V;
+  std::vector<uint32_t> T;
+  void set(const std::vector<Vec3> &samples, const Mesh &mesh) {
+    pts = to_f32(samples);
+    V = to_f32(mesh.vertices);
+    T.resize(3 * mesh.triangles.size());
+    for (size_t t = 0; t < mesh.triangles.size(); ++t)
+      for (int k = 0; k < 3; ++k) T[3 * t + k] = (uint32_t)mesh.triangles[t][k];
+  }
+};
+
+// Largest distance of each query's samples to its mesh, all queries in one
+// batched GPU call (every sample against every triangle of its mesh).
+std::vector<double> max_distances(const Engine &E, const std::vector<DistQuery> &qs) {
+  std::vector<vacd_distance_query> q(qs.size());
+  for (size_t k = 0; k < qs.size(); ++k)
+    q[k] = vacd_distance_query{qs[k].pts.data(), (int64_t)(qs[k].pts.size() / 3), qs[k].V.data(),
+                               (int64_t)(qs[k].V.size() / 3), qs[k].T.data(),
+                               (int64_t)(qs[k].T.size() / 3)};
+  std::vector<float> out(qs.size(), 0.0f);
+  if (!q.empty()) E.check(vacd_max_distance_batch(E.ctx, q.data(), (int32_t)q.size(), out.data()));
+  return std::vector<double>(out.begin(), out.end());
+}
+
 // Symmetric sampled surface distance with exact point-to-mesh distances
 // measured on the GPU (same samples as the host form).
 double gpu_hausdorff(const Engine &E, const Mesh &a, const Mesh &b, int resolution, uint32_t seed) {
@@ -268,23 +298,41 @@ double gpu_hausdorff(const Engine &E, const Mesh &a, const Mesh &b, int resoluti
   const std::vector<Vec3> sa = surface_samples(a, resolution, rng);
   const std::vector<Vec3> sb = surface_samples(b, resolution, rng);
   if (sa.empty() || sb.empty()) return 1e300;
-  vacd_bvh *ba = build_bvh(E, a), *bb = nullptr;
-  float da = 0.f, db = 0.f;
-  int rc = VACD_OK;
-  try {
-    bb = build_bvh(E, b);
-    const std::vector<float> fa = to_f32(sa), fb = to_f32(sb);
-    rc = vacd_max_distance(E.ctx, fa.data(), (int64_t)sa.size(), bb, &da);
-    if (rc == VACD_OK) rc = vacd_max_distance(E.ctx, fb.data(), (int64_t)sb.size(), ba, &db);
-  } catch (...) {
-    vacd_bvh_destroy(ba);
-    vacd_bvh_destroy(bb);
-    throw;
+  std::vector<DistQuery> qs(2);
+  qs[0].set(sa, b);
+  qs[1].set(sb, a);
+  const std::vector<double> d = max_distances(E, qs);
+  return std::max(d[0], d[1]);
+}
+
+// Final concavity of a decomposition (lib/src/process.cpp:318-327): the
+// largest symmetric sampled surface distance (10,000 samples) between a
+// part and its hull.  Samples on all host threads, distances of every part
+// in one batched call.
+double final_concavity(const Engine &E, const MeshList &parts, const MeshList &hulls) {
+  const size_t N = parts.size();
+  std::vector<DistQuery> qs(2 * N);
+  std::vector<double> host(N, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (long i = 0; i < (long)N; ++i) {
+    if (!engine.gpu_distance || hulls[i].triangles.empty()) {
+      host[i] = hausdorff_boundary(parts[i], hulls[i], 10000, engine.sample_seed);
+      continue;
+    }
+    std::mt19937 rng(engine.sample_seed);
+    const std::vector<Vec3> sa = surface_samples(parts[i], 10000, rng);
+    const std::vector<Vec3> sb = surface_samples(hulls[i], 10000, rng);
+    if (sa.empty() || sb.empty()) {
+      host[i] = 1e300;
+      continue;
+    }
+    qs[2 * i].set(sa, hulls[i]);
+    qs[2 * i + 1].set(sb, parts[i]);
   }
-  vacd_bvh_destroy(ba);
-  vacd_bvh_destroy(bb);
-  E.check(rc);
-  return std::max((double)da, (double)db);
+  const std::vector<double> d = max_distances(E, qs);
+  double h = 0.0;
+  for (size_t i = 0; i < N; ++i) h = std::max(h, std::max(host[i], std::max(d[2 * i], d[2 * i + 1])));
+  return h;
 }
 
 double part_concavity(const Engine &E, const Mesh &part) {
@@ -431,6 +479,7 @@ void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, boo
       std::vector<int64_t> off(vp.size() + 1, 0);
       int64_t cap = std::max<int64_t>(buf.cap, int64_t(1) << 22), count = 0;
       uint32_t *pairs = nullptr;
+      PhaseTimer call_timer("visibility.call");
       for (;;) {
         pairs = buf.get(cap);
         const int rc = vacd_visibility_batch(E.ctx, vp.data(), (int32_t)vp.size(), pairs, cap,
@@ -442,6 +491,7 @@ void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, boo
         E.check(rc);
         break;
       }
+      call_timer.stop();
       vacd_vis_stats st;
       if (vacd_visibility_stats(E.ctx, &st) == VACD_OK) {
         last_profile.calls["visibility.segments"] += st.segments;
@@ -479,6 +529,7 @@ void concavity_batch(const Engine &E, const MeshList &parts, std::vector<double>
     bool gpu = false;
   };
   std::vector<Prep> prep(idx.size());
+  PhaseTimer host_timer("concavity.host");
 #pragma omp parallel for schedule(dynamic)
   for (long j = 0; j < (long)idx.size(); ++j) {
     const Mesh &part = parts[idx[j]];
@@ -494,33 +545,28 @@ void concavity_batch(const Engine &E, const MeshList &parts, std::vector<double>
     q.sa = surface_samples(part, engine.hausdorff_resolution, rng);
     q.sb = surface_samples(q.hull, engine.hausdorff_resolution, rng);
   }
+  host_timer.stop();
+  PhaseTimer gpu_timer("concavity.distance");
+  // every gpu part's two one-sided distances in one batched call
+  std::vector<DistQuery> qs;
+  std::vector<size_t> at(idx.size(), SIZE_MAX);
+  for (size_t j = 0; j < idx.size(); ++j)
+    if (prep[j].gpu && !prep[j].sa.empty() && !prep[j].sb.empty()) at[j] = qs.size(), qs.resize(qs.size() + 2);
+#pragma omp parallel for schedule(dynamic)
+  for (long j = 0; j < (long)idx.size(); ++j) {
+    if (at[j] == SIZE_MAX) continue;
+    qs[at[j]].set(prep[j].sa, prep[j].hull);
+    qs[at[j] + 1].set(prep[j].sb, parts[idx[j]]);
+  }
+  const std::vector<double> d = max_distances(E, qs);
   for (size_t j = 0; j < idx.size(); ++j) {
-    Prep &q = prep[j];
-    if (!q.gpu) {
+    const Prep &q = prep[j];
+    if (!q.gpu)
       conc[idx[j]] = q.host_value;
-      continue;
-    }
-    if (q.sa.empty() || q.sb.empty()) {
+    else if (at[j] == SIZE_MAX)
       conc[idx[j]] = std::max(0.3 * q.rv, 1e300);
-      continue;
-    }
-    vacd_bvh *ba = build_bvh(E, parts[idx[j]]), *bb = nullptr;
-    float da = 0.f, db = 0.f;
-    int rc = VACD_OK;
-    try {
-      bb = build_bvh(E, q.hull);
-      const std::vector<float> fa = to_f32(q.sa), fb = to_f32(q.sb);
-      rc = vacd_max_distance(E.ctx, fa.data(), (int64_t)q.sa.size(), bb, &da);
-      if (rc == VACD_OK) rc = vacd_max_distance(E.ctx, fb.data(), (int64_t)q.sb.size(), ba, &db);
-    } catch (...) {
-      vacd_bvh_destroy(ba);
-      vacd_bvh_destroy(bb);
-      throw;
-    }
-    vacd_bvh_destroy(ba);
-    vacd_bvh_destroy(bb);
-    E.check(rc);
-    conc[idx[j]] = std::max(0.3 * q.rv, (double)std::max(da, db));
+    else
+      conc[idx[j]] = std::max(0.3 * q.rv, std::max(d[at[j]], d[at[j] + 1]));
   }
 }
 
@@ -917,14 +963,12 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
   log_line("Computing convex hulls for " + std::to_string(parts.size()) + " parts...");
   PhaseTimer final_timer("final_hulls");
   MeshList hulls(parts.size());
+  PhaseTimer fh_timer("final_hulls.hulls");
 #pragma omp parallel for schedule(dynamic)
   for (long i = 0; i < (long)parts.size(); ++i) hulls[i] = convex_hull(parts[i].vertices);
-  double final_h = 0.0;
-  for (size_t i = 0; i < parts.size(); ++i) {
-    const bool gpu = engine.gpu_distance && !hulls[i].triangles.empty();
-    final_h = std::max(final_h, gpu ? gpu_hausdorff(D.E, parts[i], hulls[i], 10000, engine.sample_seed)
-                                    : hausdorff_boundary(parts[i], hulls[i], 10000, engine.sample_seed));
-  }
+  fh_timer.stop();
+  PhaseTimer fd_timer("final_hulls.distance");
+  const double final_h = final_concavity(D.E, parts, hulls);
   for (auto &h : hulls) unnormalize(h, bbox);
   for (auto &p : parts) unnormalize(p, bbox);
   ProcessResult r;
@@ -1007,12 +1051,7 @@ ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
   PhaseTimer final_timer("final_hulls");
   MeshList hulls;
   for (auto &p : parts) hulls.push_back(convex_hull(p.vertices));
-  double final_h = 0.0;
-  for (size_t i = 0; i < parts.size(); ++i) {
-    const bool gpu = engine.gpu_distance && !hulls[i].triangles.empty();
-    final_h = std::max(final_h, gpu ? gpu_hausdorff(D.E, parts[i], hulls[i], 10000, engine.sample_seed)
-                                    : hausdorff_boundary(parts[i], hulls[i], 10000, engine.sample_seed));
-  }
+  const double final_h = final_concavity(D.E, parts, hulls);
   if (config.use_merging) log_line("Hull merging is not part of this engine; parts are kept.");
   for (auto &h : hulls) unnormalize(h, bbox);
   for (auto &p : parts) unnormalize(p, bbox);

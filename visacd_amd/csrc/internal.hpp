@@ -118,6 +118,7 @@ enum VacdSlot {
   SLOT_SCORE_TABLES,
   SLOT_MLP_IN,
   SLOT_GROUPS,
+  SLOT_DIST,
   SLOT_COUNT_MAX
 };
 
@@ -139,13 +140,13 @@ struct vacd_ctx {
   // pinned host staging for descriptor tables copied to the device, one
   // region per table kind; a copy captured into a graph reads its region
   // again at every replay
-  void *pinned[2] = {nullptr, nullptr};
-  size_t pinned_bytes[2] = {0, 0};
-  hipEvent_t pinned_free[2] = {nullptr, nullptr};  // after the last copy out of it
+  void *pinned[3] = {nullptr, nullptr, nullptr};
+  size_t pinned_bytes[3] = {0, 0, 0};
+  hipEvent_t pinned_free[3] = {nullptr, nullptr, nullptr};  // after the last copy out of it
 };
 
 namespace vacd {
-enum PinnedKind { PINNED_VIS = 0, PINNED_SCORE = 1 };
+enum PinnedKind { PINNED_VIS = 0, PINNED_SCORE = 1, PINNED_DIST = 2 };
 
 inline bool capturing(const vacd_ctx *ctx) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
@@ -258,6 +259,9 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
 // Largest point-to-mesh distance (distance.hip); writes float bits.
 void launch_max_distance(vacd_ctx *ctx, const float *d_pts, int64_t m, const vacd_bvh *mesh,
                          unsigned *d_out_bits);
+// Same for many (points, mesh) queries at once, every point against every
+// triangle of its mesh (no BVH); host arrays in, out[nq] written on return.
+void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, float *out);
 
 // Scoring (scoring.hip).
 void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P,

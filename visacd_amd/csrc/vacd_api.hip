@@ -88,7 +88,7 @@ int vacd_ctx_destroy(vacd_ctx *ctx) {
       (void)hipEventDestroy(pr.first);
       (void)hipEventDestroy(pr.second);
     }
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 3; ++k) {
     if (ctx->pinned[k]) (void)hipHostFree(ctx->pinned[k]);
     if (ctx->pinned_free[k]) (void)hipEventDestroy(ctx->pinned_free[k]);
   }
@@ -727,6 +727,16 @@ int vacd_max_distance(vacd_ctx *ctx, const float *pts, int64_t m, const vacd_bvh
     vacd::launch_max_distance(ctx, dp, m, mesh, o.as<unsigned>());
     VACD_HIP(hipMemcpyAsync(out, o.ptr, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     VACD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int vacd_max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int32_t nq, float *out) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(nq >= 0 && (nq == 0 || (q && out)), VACD_ERR_INVALID, "bad arguments");
+    if (nq == 0) return;
+    set_device(ctx);
+    vacd::max_distance_batch(ctx, q, nq, out);
   });
 }
 

@@ -31,7 +31,7 @@ EXPORTED = [
     "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_deferred",
     "vacd_visibility_batch_d", "vacd_visibility_batch", "vacd_score_planes",
     "vacd_score_planes_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
-    "vacd_max_distance", "vacd_max_distance_d", "vacd_gemm_bf16_d", "vacd_mlp_param_count",
+    "vacd_max_distance", "vacd_max_distance_d", "vacd_max_distance_batch", "vacd_gemm_bf16_d", "vacd_mlp_param_count",
     "vacd_mlp_create", "vacd_mlp_destroy", "vacd_mlp_set_params", "vacd_mlp_get_params",
     "vacd_mlp_forward_d", "vacd_mlp_forward", "vacd_mlp_train_step_d", "classify_and_rate_planes",
 ]
@@ -53,6 +53,11 @@ class VisStats(ctypes.Structure):
 class Part(ctypes.Structure):
     _fields_ = [("pts", ctypes.c_void_p), ("n", ctypes.c_int64), ("new_mask", ctypes.c_void_p),
                 ("self", ctypes.c_void_p), ("cage", ctypes.c_void_p)]
+
+
+class DistanceQuery(ctypes.Structure):
+    _fields_ = [("pts", ctypes.c_void_p), ("m", ctypes.c_int64), ("V", ctypes.c_void_p),
+                ("nV", ctypes.c_int64), ("T", ctypes.c_void_p), ("nT", ctypes.c_int64)]
 
 
 class ScorePart(ctypes.Structure):
@@ -123,6 +128,8 @@ def lib():
                                                 vp]
         L.vacd_max_distance.argtypes = [vp, vp, i64, vp, ctypes.POINTER(ctypes.c_float)]
         L.vacd_max_distance_d.argtypes = [vp, vp, i64, vp, vp]
+        L.vacd_max_distance_batch.argtypes = [vp, ctypes.POINTER(DistanceQuery), ctypes.c_int32,
+                                              vp]
         ci = ctypes.c_int
         L.vacd_gemm_bf16_d.argtypes = [vp, ci, ci, ci, ci, ci, vp, ci, vp, ci, vp, ci, ci, vp, ci]
         L.vacd_mlp_param_count.restype = i64
@@ -268,6 +275,20 @@ class Context:
         self._check(lib().vacd_max_distance(self._h, _p(pts), pts.shape[0], mesh_bvh.h,
                                             ctypes.byref(out)))
         return float(out.value)
+
+    def max_distance_batch(self, queries) -> np.ndarray:
+        """Largest distance of each query's points to its mesh, in one call.
+        queries: list of (points [m,3], V [nV,3], T [nT,3]) host arrays."""
+        keep, arr = [], (DistanceQuery * max(1, len(queries)))()
+        for k, (pts, V, T) in enumerate(queries):
+            pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+            V = np.ascontiguousarray(V, dtype=np.float32).reshape(-1, 3)
+            T = np.ascontiguousarray(T, dtype=np.uint32).reshape(-1, 3)
+            keep += [pts, V, T]
+            arr[k] = DistanceQuery(_p(pts), pts.shape[0], _p(V), V.shape[0], _p(T), T.shape[0])
+        out = np.zeros(len(queries), dtype=np.float32)
+        self._check(lib().vacd_max_distance_batch(self._h, arr, len(queries), _p(out)))
+        return out
 
     def visibility_stats(self) -> dict:
         s = VisStats()
