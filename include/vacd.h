@@ -295,6 +295,23 @@ typedef struct vacd_distance_query {
 int vacd_max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int32_t nq,
                             float *out);
 
+/* ---- candidate-plane extents ---------------------------------------------
+ * For each query, the extent [min, max] of n.p over its points p for each of
+ * its P normals n, in float64 with the products and sums rounded as the host
+ * evaluates ((n0 p0 + n1 p1) + n2 p2) without fused multiply-add, so the
+ * values equal a host loop's bit for bit.  Used to place the benchmark
+ * workloads' candidate planes (normals uniform on the sphere, offsets uniform
+ * over the part's extent along the normal) for many parts in one launch.
+ * lo_hi receives [P][2] doubles per query, queries back to back. */
+typedef struct vacd_extent_query {
+  const double *normals; /* [P][3] */
+  int64_t P;
+  const double *pts; /* [n][3] */
+  int64_t n;
+} vacd_extent_query;
+int vacd_plane_extents_batch(vacd_ctx *ctx, const vacd_extent_query *q, int32_t nq,
+                             double *lo_hi);
+
 /* Reference-signature shim: same arguments and behaviour as
  * classify_and_rate_planes (lib/cuda/plane_intersections.cu:71-78,
  * declared at lib/src/process.cpp:74-79): host pointers, synchronous,

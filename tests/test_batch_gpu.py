@@ -73,6 +73,27 @@ def test_max_distance_batch_matches_bvh_form_and_brute_force(ctx):
         ctx.max_distance_batch([(pts, V, T + 1000)])  # vertex id out of range
 
 
+def test_plane_extents_batch_equal_host_float64(ctx):
+    """Extents along unit normals equal numpy's float64 ((n0 p0 + n1 p1) + n2 p2)
+    bit for bit, for queries of several sizes (more than one LDS tile of
+    points, more than one block of normals), an empty point set included."""
+    rng = np.random.default_rng(9)
+    queries = []
+    for P, n in ((5, 3), (300, 2500), (1000, 700), (7, 0)):
+        N = rng.normal(size=(P, 3))
+        N /= np.linalg.norm(N, axis=1, keepdims=True)
+        queries.append((N, rng.uniform(-1, 1, size=(n, 3))))
+    print("extent batch", [(q[0].shape[0], q[1].shape[0]) for q in queries], flush=True)
+    got = ctx.plane_extents_batch(queries)
+    for (N, pts), g in zip(queries, got):
+        if pts.shape[0] == 0:
+            assert np.all(g[:, 0] == 1e300) and np.all(g[:, 1] == -1e300)
+            continue
+        d = N[:, 0, None] * pts[None, :, 0] + N[:, 1, None] * pts[None, :, 1]
+        d = d + N[:, 2, None] * pts[None, :, 2]
+        assert np.array_equal(g[:, 0], d.min(axis=1)) and np.array_equal(g[:, 1], d.max(axis=1))
+
+
 def test_batch_matches_oracle_per_part(ctx):
     import torch
 

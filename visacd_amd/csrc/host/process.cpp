@@ -372,32 +372,54 @@ std::vector<Plane> bisector_planes(const Mesh &m, int num_planes, std::mt19937 &
 
 // Normals uniform on S^2 and offsets uniform over the part's extent along
 // each normal (the benchmark workloads' candidate sets).  The draws are taken
-// in order first (normal x, y, z, then the offset fraction, per plane), the
-// extents are then computed in parallel.
-std::vector<Plane> sphere_planes(const Mesh &m, int num_planes, std::mt19937 &rng) {
+// in order (normal x, y, z, then the offset fraction, per plane); the
+// extents of every part of a round come from one GPU call
+// (vacd_plane_extents_batch, float64 rounded as a host loop would).
+struct SphereDraws {
+  std::vector<double> normals;  // [P][3]
+  std::vector<double> frac;
+};
+
+SphereDraws sphere_draws(int num_planes, std::mt19937 &rng) {
   std::normal_distribution<double> g(0.0, 1.0);
   std::uniform_real_distribution<double> u(0.0, 1.0);
-  std::vector<Vec3> normals;
-  std::vector<double> frac;
+  SphereDraws d;
   for (int i = 0; i < num_planes; ++i) {
     Vec3 n{g(rng), g(rng), g(rng)};
     const double l = norm(n);
     if (l == 0) continue;
-    normals.push_back(n * (1.0 / l));
-    frac.push_back(u(rng));
+    n = n * (1.0 / l);
+    d.normals.insert(d.normals.end(), {n[0], n[1], n[2]});
+    d.frac.push_back(u(rng));
   }
-  std::vector<Vec3> pts(m.points.size());
-  for (size_t i = 0; i < pts.size(); ++i) pts[i] = m.vertices[m.points[i]];
-  std::vector<Plane> planes(normals.size());
-#pragma omp parallel for schedule(static)
-  for (long i = 0; i < (long)normals.size(); ++i) {
-    const Vec3 &n = normals[i];
-    double lo = 1e300, hi = -1e300;
-    for (const Vec3 &p : pts) {
-      const double d = dot(n, p);
-      lo = std::min(lo, d), hi = std::max(hi, d);
+  return d;
+}
+
+std::vector<std::vector<Plane>> sphere_planes(const Engine &E, const std::vector<const Mesh *> &ms,
+                                              const std::vector<SphereDraws> &dr) {
+  const size_t F = ms.size();
+  std::vector<std::vector<double>> pts(F);
+  std::vector<vacd_extent_query> q(F);
+  int64_t total = 0;
+  for (size_t f = 0; f < F; ++f) {
+    const Mesh &m = *ms[f];
+    pts[f].resize(3 * m.points.size());
+    for (size_t i = 0; i < m.points.size(); ++i)
+      for (int k = 0; k < 3; ++k) pts[f][3 * i + k] = m.vertices[m.points[i]][k];
+    q[f] = vacd_extent_query{dr[f].normals.data(), (int64_t)dr[f].frac.size(), pts[f].data(),
+                             (int64_t)m.points.size()};
+    total += q[f].P;
+  }
+  std::vector<double> lh(2 * (size_t)total);
+  E.check(vacd_plane_extents_batch(E.ctx, q.data(), (int32_t)F, lh.data()));
+  std::vector<std::vector<Plane>> planes(F);
+  size_t o = 0;
+  for (size_t f = 0; f < F; ++f) {
+    for (size_t i = 0; i < dr[f].frac.size(); ++i, ++o) {
+      const double lo = lh[2 * o], hi = lh[2 * o + 1];
+      const double *n = &dr[f].normals[3 * i];
+      planes[f].emplace_back(n[0], n[1], n[2], -(lo + dr[f].frac[i] * (hi - lo)));
     }
-    planes[i] = Plane(n[0], n[1], n[2], -(lo + frac[i] * (hi - lo)));
   }
   return planes;
 }
@@ -758,8 +780,10 @@ struct Driver {
     std::vector<Plane> planes;
     {
       PhaseTimer timer("planes");
-      planes = engine.planes == "sphere" ? sphere_planes(mesh, engine.num_planes, random_engine())
-                                         : bisector_planes(mesh, engine.num_planes, random_engine());
+      if (engine.planes == "sphere")
+        planes = sphere_planes(E, {&mesh}, {sphere_draws(engine.num_planes, random_engine())})[0];
+      else
+        planes = bisector_planes(mesh, engine.num_planes, random_engine());
     }
     const size_t ss_offset = planes.size();
     planes.insert(planes.end(), flat.begin(), flat.end());
@@ -871,12 +895,19 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
     std::vector<std::vector<Plane>> planes(F);
     std::vector<size_t> ss_offset(F);
     {
-      PhaseTimer t("planes");  // each part's plane extents run on all host threads
+      PhaseTimer t("planes");  // every part's draws, then all extents in one GPU call
+      std::vector<SphereDraws> dr(F);
+      std::vector<const Mesh *> ms(F);
+#pragma omp parallel for schedule(dynamic)
       for (long f = 0; f < (long)F; ++f) {
         const Mesh &m = parts[cand[f].second];
+        ms[f] = &m;
         std::mt19937 rng((uint32_t)mix64(seed[cand[f].second] ^ 0x5eedull));
-        planes[f] = engine.planes == "sphere" ? sphere_planes(m, engine.num_planes, rng)
-                                              : bisector_planes(m, engine.num_planes, rng);
+        if (engine.planes == "sphere") dr[f] = sphere_draws(engine.num_planes, rng);
+        else planes[f] = bisector_planes(m, engine.num_planes, rng);
+      }
+      if (engine.planes == "sphere") planes = sphere_planes(D.E, ms, dr);
+      for (size_t f = 0; f < F; ++f) {
         ss_offset[f] = planes[f].size();
         planes[f].insert(planes[f].end(), D.flat.begin(), D.flat.end());
       }

@@ -119,6 +119,7 @@ enum VacdSlot {
   SLOT_MLP_IN,
   SLOT_GROUPS,
   SLOT_DIST,
+  SLOT_EXTENT,
   SLOT_COUNT_MAX
 };
 
@@ -262,6 +263,8 @@ void launch_max_distance(vacd_ctx *ctx, const float *d_pts, int64_t m, const vac
 // Same for many (points, mesh) queries at once, every point against every
 // triangle of its mesh (no BVH); host arrays in, out[nq] written on return.
 void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, float *out);
+// Extents of point sets along unit normals, float64 (scoring.hip).
+void plane_extents_batch(vacd_ctx *ctx, const vacd_extent_query *q, int nq, double *lo_hi);
 
 // Scoring (scoring.hip).
 void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P,

@@ -360,3 +360,98 @@ void launch_scoring_batch(vacd_ctx *ctx, const vacd_score_part *parts, int npart
 }
 
 }  // namespace vacd
+
+// ------------------------------------------------- candidate-plane extents
+// Extent of each part's points along each of its candidate normals (the
+// sphere-sampled candidate planes put their offset uniformly in it).  A
+// thread owns one normal and scans the part's points, staged 1024 at a time
+// in LDS; float64 products and sums are rounded one by one (contraction into
+// fused multiply-adds is switched off) in the host's order, so the extents
+// equal the host loop's.
+namespace vacd {
+namespace {
+
+struct ExtQ {
+  int64_t n0, P;    // first normal, normal count
+  int64_t p0, n;    // first point, point count
+  int64_t blk0;     // first block
+};
+
+constexpr int kExtPts = 1024;
+
+__global__ __launch_bounds__(256) void k_extents(const ExtQ *__restrict__ qs, int nq,
+                                                 const double *__restrict__ normals,
+                                                 const double *__restrict__ pts,
+                                                 double *__restrict__ lo_hi) {
+  __shared__ double sp[3 * kExtPts];
+  const int64_t blk = blockIdx.x;
+  int lo = 0, hi = nq - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (qs[mid].blk0 <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  const ExtQ q = qs[lo];
+  const int64_t i = (blk - q.blk0) * 256 + threadIdx.x;
+  const bool valid = i < q.P;
+  double nx = 0.0, ny = 0.0, nz = 0.0;
+  if (valid) {
+    nx = normals[3 * (q.n0 + i)], ny = normals[3 * (q.n0 + i) + 1], nz = normals[3 * (q.n0 + i) + 2];
+  }
+  double mn = 1e300, mx = -1e300;
+  for (int64_t b = 0; b < q.n; b += kExtPts) {
+    const int cnt = (int)(q.n - b < kExtPts ? q.n - b : kExtPts);
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3 * cnt; k += 256) sp[k] = pts[3 * (q.p0 + b) + k];
+    __syncthreads();
+    if (!valid) continue;
+    for (int k = 0; k < cnt; ++k) {
+#pragma clang fp contract(off)  // a fused multiply-add would round differently from the host
+      const double d = (nx * sp[3 * k] + ny * sp[3 * k + 1]) + nz * sp[3 * k + 2];
+      mn = fmin(mn, d), mx = fmax(mx, d);
+    }
+  }
+  if (valid) lo_hi[2 * (q.n0 + i)] = mn, lo_hi[2 * (q.n0 + i) + 1] = mx;
+}
+
+}  // namespace
+
+void plane_extents_batch(vacd_ctx *ctx, const vacd_extent_query *q, int nq, double *lo_hi) {
+  hipStream_t st = ctx->stream;
+  std::vector<ExtQ> qs((size_t)nq);
+  int64_t tn = 0, tp = 0, nb = 0;
+  for (int i = 0; i < nq; ++i) {
+    VACD_REQUIRE(q[i].P >= 0 && q[i].n >= 0, VACD_ERR_INVALID, "negative size in an extent query");
+    VACD_REQUIRE((q[i].P == 0 || q[i].normals) && (q[i].n == 0 || q[i].pts), VACD_ERR_INVALID,
+                 "NULL array in an extent query");
+    qs[(size_t)i] = ExtQ{tn, q[i].P, tp, q[i].n, nb};
+    nb += (q[i].P + 255) / 256;
+    tn += q[i].P, tp += q[i].n;
+  }
+  VACD_REQUIRE(nb < (int64_t(1) << 31), VACD_ERR_LIMIT, "extent batch too large");
+  const size_t b_q = (sizeof(ExtQ) * (size_t)nq + 255) & ~size_t(255);
+  const size_t b_n = (24 * (size_t)tn + 255) & ~size_t(255);
+  const size_t b_p = (24 * (size_t)tp + 255) & ~size_t(255);
+  const size_t bytes = b_q + b_n + b_p;
+  char *h = static_cast<char *>(pinned_stage(ctx, PINNED_DIST, bytes));
+  std::memcpy(h, qs.data(), sizeof(ExtQ) * (size_t)nq);
+  for (int i = 0; i < nq; ++i) {
+    const ExtQ &e = qs[(size_t)i];
+    if (e.P) std::memcpy(h + b_q + 24 * e.n0, q[i].normals, 24 * (size_t)e.P);
+    if (e.n) std::memcpy(h + b_q + b_n + 24 * e.p0, q[i].pts, 24 * (size_t)e.n);
+  }
+  DevBuf &b = ctx->buf[SLOT_EXTENT];
+  b.ensure(bytes + 16 * (size_t)std::max<int64_t>(tn, 1));
+  char *d = b.as<char>();
+  pinned_upload(ctx, PINNED_DIST, d, bytes);
+  double *dout = reinterpret_cast<double *>(d + bytes);
+  if (nb)
+    k_extents<<<(unsigned)nb, 256, 0, st>>>(reinterpret_cast<const ExtQ *>(d), nq,
+                                            reinterpret_cast<const double *>(d + b_q),
+                                            reinterpret_cast<const double *>(d + b_q + b_n), dout);
+  VACD_HIP(hipGetLastError());
+  if (tn) VACD_HIP(hipMemcpyAsync(lo_hi, dout, 16 * (size_t)tn, hipMemcpyDeviceToHost, st));
+  VACD_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace vacd

@@ -740,6 +740,16 @@ int vacd_max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int32_t
   });
 }
 
+int vacd_plane_extents_batch(vacd_ctx *ctx, const vacd_extent_query *q, int32_t nq, double *lo_hi) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(nq >= 0 && (nq == 0 || (q && lo_hi)), VACD_ERR_INVALID, "bad arguments");
+    if (nq == 0) return;
+    set_device(ctx);
+    vacd::plane_extents_batch(ctx, q, nq, lo_hi);
+  });
+}
+
 void classify_and_rate_planes(const float *planes, const float *points, const unsigned int *edges,
                               float *scores, int numPlanes, int numPoints, int numEdges) {
   thread_local vacd_ctx *tctx = nullptr;

@@ -31,7 +31,8 @@ EXPORTED = [
     "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_deferred",
     "vacd_visibility_batch_d", "vacd_visibility_batch", "vacd_score_planes",
     "vacd_score_planes_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
-    "vacd_max_distance", "vacd_max_distance_d", "vacd_max_distance_batch", "vacd_gemm_bf16_d", "vacd_mlp_param_count",
+    "vacd_max_distance", "vacd_max_distance_d", "vacd_max_distance_batch", "vacd_plane_extents_batch",
+    "vacd_gemm_bf16_d", "vacd_mlp_param_count",
     "vacd_mlp_create", "vacd_mlp_destroy", "vacd_mlp_set_params", "vacd_mlp_get_params",
     "vacd_mlp_forward_d", "vacd_mlp_forward", "vacd_mlp_train_step_d", "classify_and_rate_planes",
 ]
@@ -58,6 +59,11 @@ class Part(ctypes.Structure):
 class DistanceQuery(ctypes.Structure):
     _fields_ = [("pts", ctypes.c_void_p), ("m", ctypes.c_int64), ("V", ctypes.c_void_p),
                 ("nV", ctypes.c_int64), ("T", ctypes.c_void_p), ("nT", ctypes.c_int64)]
+
+
+class ExtentQuery(ctypes.Structure):
+    _fields_ = [("normals", ctypes.c_void_p), ("P", ctypes.c_int64), ("pts", ctypes.c_void_p),
+                ("n", ctypes.c_int64)]
 
 
 class ScorePart(ctypes.Structure):
@@ -130,6 +136,7 @@ def lib():
         L.vacd_max_distance_d.argtypes = [vp, vp, i64, vp, vp]
         L.vacd_max_distance_batch.argtypes = [vp, ctypes.POINTER(DistanceQuery), ctypes.c_int32,
                                               vp]
+        L.vacd_plane_extents_batch.argtypes = [vp, ctypes.POINTER(ExtentQuery), ctypes.c_int32, vp]
         ci = ctypes.c_int
         L.vacd_gemm_bf16_d.argtypes = [vp, ci, ci, ci, ci, ci, vp, ci, vp, ci, vp, ci, ci, vp, ci]
         L.vacd_mlp_param_count.restype = i64
@@ -289,6 +296,20 @@ class Context:
         out = np.zeros(len(queries), dtype=np.float32)
         self._check(lib().vacd_max_distance_batch(self._h, arr, len(queries), _p(out)))
         return out
+
+    def plane_extents_batch(self, queries) -> list:
+        """[min, max] of n.p over each query's points for each of its normals.
+        queries: list of (normals [P,3], points [n,3]) float64 host arrays."""
+        keep, arr, sizes = [], (ExtentQuery * max(1, len(queries)))(), []
+        for k, (N, pts) in enumerate(queries):
+            N = np.ascontiguousarray(N, dtype=np.float64).reshape(-1, 3)
+            pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+            keep += [N, pts]
+            arr[k] = ExtentQuery(_p(N), N.shape[0], _p(pts), pts.shape[0])
+            sizes.append(N.shape[0])
+        out = np.zeros((max(1, sum(sizes)), 2), dtype=np.float64)
+        self._check(lib().vacd_plane_extents_batch(self._h, arr, len(queries), _p(out)))
+        return np.split(out[: sum(sizes)], np.cumsum(sizes)[:-1])
 
     def visibility_stats(self) -> dict:
         s = VisStats()
