@@ -6,7 +6,7 @@
 // float64 and decided with exact orientation predicates:
 //
 //   * lib/ptx/ray_segments.cu:49-111 and lib/src/intersections.cpp:67-73,
-//     327-385: pair order k -> (i<j), the new-vertex mask (state 2),
+//     327-385: pair order k -> (i<j), the new-vertex mask (plus cap bits),
 //     zero-length segments, and the combine rule
 //     edge <=> hits(cage) && !hits(self) in cage mode;
 //   * the midpoint-parity form of the same rule (cage absent):
@@ -486,10 +486,17 @@ struct Ctx {
   Bvh bself, bcage;
 };
 
+// Mask of a point: bit 0 new, bit 1 new and on the last cut's cap, bits 2-7
+// on one of six older cut planes.  A pair is skipped when neither point is
+// new (ray_segments.cu:61-75) or when both lie on one cut plane (bits 1-7
+// shared: the segment lies in that plane; the caller's choice, no
+// counterpart in the reference).
+bool mask_skips(uint8_t a, uint8_t b) {
+  return (!(a & 3) && !(b & 3)) || ((a >> 1) & (b >> 1)) != 0;
+}
+
 int pair_detail(const Ctx& C, int64_t i, int64_t j) {
-  // 0/0: two old vertices (ray_segments.cu:61-75); 2/2: two points of one
-  // cap, skipped by the caller's choice (no counterpart in the reference)
-  if (C.mask && ((!C.mask[i] && !C.mask[j]) || (C.mask[i] == 2 && C.mask[j] == 2))) return 2;
+  if (C.mask && mask_skips(C.mask[i], C.mask[j])) return 2;
   P3 p = ld(C.pts + 3 * i), q = ld(C.pts + 3 * j);
   if (same(p, q)) return 3;
   auto self_hit = [&](int t) {

@@ -196,6 +196,38 @@ def test_visibility_binary_and_wide_walks_agree(ctx, name):
         np.testing.assert_array_equal(g, o, err_msg=f"lanes per pair {lpp}")
 
 
+@pytest.mark.parametrize("level,amp,seed", [(3, 0.5, 23), (3, 0.3, 5)])
+def test_inside_grid_settles_pairs_without_changing_edges(ctx, level, amp, seed):
+    """Pairs that leave both ends outwards and cross a grid cell lying inside
+    the solid are settled without a walk: the oracle's edges with the grid on
+    and off, with fewer walks when it is on; points off the surface too."""
+    from visacd_amd import native
+
+    V, T = wl.perturbed_icosphere(level, amp, seed)
+    rng = np.random.default_rng(seed)
+    off = rng.uniform(-0.9, 0.9, size=(60, 3)).astype(np.float32)
+    _print("inside grid", level, amp, seed)
+    ctx.set_option(native.VACD_OPT_INSIDE_GRID, 1)
+    try:
+        b = ctx.bvh(V, T)  # built with its grid
+        walked = {}
+        for pts in (V, np.concatenate([V[::3], off], 0)):
+            o, _ = oracle.visibility(pts, V, T)
+            for grid in (1, 0):
+                ctx.set_option(native.VACD_OPT_INSIDE_GRID, grid)
+                ctx.set_option(native.VACD_OPT_COUNT_WORK, 1)
+                g = ctx.visibility(pts, b)
+                walked[grid] = ctx.visibility_stats()["walked"]
+                settled = ctx.visibility_counters()["settled_by_grid"]
+                ctx.set_option(native.VACD_OPT_COUNT_WORK, 0)
+                np.testing.assert_array_equal(g, o, err_msg=f"inside grid {grid}")
+                assert (settled > 0) == (grid == 1)
+            assert walked[1] < walked[0], walked
+    finally:
+        ctx.set_option(native.VACD_OPT_INSIDE_GRID, 0)
+        ctx.set_option(native.VACD_OPT_COUNT_WORK, 0)
+
+
 def test_visibility_open_mesh(ctx):
     """An open mesh takes the midpoint-ray path (fan parity needs a closed mesh)."""
     V, T = wl.perturbed_icosphere(3, 0.3, 13)

@@ -141,6 +141,26 @@ def test_mask_value_two_skips_pairs_among_twos():
     assert st["masked"] == n0 * (n0 - 1) // 2 + n2 * (n2 - 1) // 2
 
 
+def test_mask_cap_bits_skip_pairs_on_one_cut_plane():
+    """Bits 2-7 mark points on older cut planes: 4 = old point on plane 1,
+    6 = new point on the last cap and on plane 1.  Skipped: old-old pairs
+    (0 and 4 are old), pairs of two 6s (shared last cap) and 4-6 pairs
+    (shared plane 1); 1-x, 0-6 and the rest are tested as usual."""
+    V, T = wl.perturbed_icosphere(2, 0.4, 8)
+    n = V.shape[0]
+    rng = np.random.default_rng(3)
+    mask = rng.choice(np.array([0, 1, 4, 6], np.uint8), size=n, p=[0.4, 0.2, 0.2, 0.2])
+    full, _ = oracle.visibility(V, V, T)
+    part, st = oracle.visibility(V, V, T, mask=mask)
+    n0, n4, n6 = (int((mask == v).sum()) for v in (0, 4, 6))
+    old = n0 + n4
+    assert st["masked"] == old * (old - 1) // 2 + n6 * (n6 - 1) // 2 + n4 * n6
+    tested = {(0, 1), (1, 1), (1, 4), (1, 6), (0, 6)}
+    keep = [(i, j) for i, j in full.tolist()
+            if tuple(sorted((int(mask[i]), int(mask[j])))) in tested]
+    assert set(map(tuple, part.tolist())) == set(keep)
+
+
 def test_k_order_and_pair_decode():
     n = 1000
     for k in [0, 1, 998, 999, 12345, n * (n - 1) // 2 - 1]:

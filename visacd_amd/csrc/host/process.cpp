@@ -158,6 +158,32 @@ struct PairBuffer {
 
 vacd_bvh *build_bvh(const Engine &E, const Mesh &m, bool planar = false);
 
+// Visibility mask of a part's points (vacd_visibility): bit 0 new vertex;
+// with engine.skip_cap_pairs, bit 1 = new vertex (on the last cut's cap)
+// and bits 2-7 = vertex on one of the six cut planes before it, so that the
+// pairs of two points on one cut plane are skipped (their segment lies in
+// that plane).
+std::vector<uint8_t> point_mask(const Mesh &part) {
+  const size_t n = part.points.size();
+  std::vector<uint8_t> mask(n, 0);
+  std::vector<Plane> older;
+  if (engine.skip_cap_pairs)
+    for (int c = (int)part.cut_planes.size() - 2; c >= 0 && older.size() < 6; --c)
+      older.push_back(part.cut_planes[c]);
+  for (size_t i = 0; i < n; ++i) {
+    const int v = part.points[i];
+    const bool fresh = v < (int)part.is_new.size() && part.is_new[v];
+    uint8_t m = fresh ? (engine.skip_cap_pairs ? 2 : 1) : 0;
+    for (size_t k = 0; k < older.size(); ++k) {
+      const Plane &q = older[k];
+      const double tol = 1e-9 * std::sqrt(q.a * q.a + q.b * q.b + q.c * q.c);
+      if (std::fabs(q.eval(part.vertices[v])) <= tol) m |= (uint8_t)(4u << k);
+    }
+    mask[i] = m;
+  }
+  return mask;
+}
+
 // Visibility of a part in parity mode; with_mask restricts to pairs touching
 // a new vertex (lib/ptx/ray_segments.cu:61-75).  Edges are appended in k order
 // as pairs of part vertex ids.
@@ -173,14 +199,12 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
   const int64_t n = (int64_t)part.points.size();
   last_profile.calls["visibility.points"] += n;
   std::vector<float> P(3 * n);
-  std::vector<uint8_t> mask(n, 0);
+  const std::vector<uint8_t> mask = point_mask(part);
   bool any_new = false;
   for (int64_t i = 0; i < n; ++i) {
     const int v = part.points[i];
     for (int k = 0; k < 3; ++k) P[3 * i + k] = V[3 * (size_t)v + k];
-    const bool fresh = v < (int)part.is_new.size() && part.is_new[v];
-    mask[i] = fresh ? (engine.skip_cap_pairs ? 2 : 1) : 0;
-    any_new |= mask[i] != 0;
+    any_new |= (mask[i] & 3) != 0;
   }
   const bool use_mask = with_mask;
   if (use_mask && !any_new) {
@@ -479,14 +503,12 @@ void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, boo
       const int64_t n = (int64_t)part.points.size();
       if (n < 2) continue;
       P[k].resize(3 * n);
-      M[k].assign(n, 0);
+      M[k] = point_mask(part);
       bool any_new = false;
       for (int64_t i = 0; i < n; ++i) {
         const int v = part.points[i];
         for (int c = 0; c < 3; ++c) P[k][3 * i + c] = (float)part.vertices[v][c];
-        const bool fresh = v < (int)part.is_new.size() && part.is_new[v];
-        M[k][i] = fresh ? (engine.skip_cap_pairs ? 2 : 1) : 0;
-        any_new |= fresh;
+        any_new |= (M[k][i] & 3) != 0;
       }
       if (with_mask && !any_new) continue;
       {

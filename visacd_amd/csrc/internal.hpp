@@ -44,6 +44,21 @@ struct alignas(16) Tri {
 };
 static_assert(sizeof(Tri) == 48, "triangle is 48 bytes");
 
+// Inside grid of a closed mesh: kGrid^3 cells over its box, one 64-bit word
+// per (y, z) column with bit x set when cell (x, y, z) lies wholly inside the
+// solid.  Cell of coordinate c along axis k: floor((c - lo[k]) * inv[k]).
+constexpr int kGrid = 64;
+struct GridBox {
+  float lo[3];
+  float inv[3];
+  float eps;  // distance kept between inside cells and the surface
+};
+
+__device__ __forceinline__ int grid_cell(float x, float lo, float inv) {
+  const int i = (int)floorf((x - lo) * inv);
+  return i < 0 ? 0 : (i >= kGrid ? kGrid - 1 : i);
+}
+
 // ---------------------------------------------------------------- errors
 struct Error {
   int code;
@@ -120,6 +135,7 @@ enum VacdSlot {
   SLOT_GROUPS,
   SLOT_DIST,
   SLOT_EXTENT,
+  SLOT_GRID,
   SLOT_COUNT_MAX
 };
 
@@ -134,6 +150,9 @@ struct vacd_ctx {
   bool count_work = false;
   bool wide_bvh = false;  // 4-wide node walk (measured slower than binary on gfx950)
   bool fan_pairs = true;  // decide pairs that start inside from the fan of p
+  // settle pairs crossing a cell inside the solid (the grid is built with
+  // each closed BVH while this is on)
+  bool inside_grid = false;
   int64_t score_chunk = 0;  // edges per scoring block (0: sized for ~4096 blocks)
   int lanes_per_pair_log2 = 0;  // queued-pair walks split over 2^k lanes (k <= 4)
   // recorded (start, stop) event pairs per timed kernel kind
@@ -230,7 +249,12 @@ struct vacd_bvh {
   // that part of the segment only.
   int n_groups = 0;
   vacd::DevBuf groups;
+  // Inside grid (closed meshes): cells wholly inside the solid.
+  bool has_grid = false;
+  vacd::GridBox grid{};
+  vacd::DevBuf inside;  // uint64[kGrid * kGrid]
   void release() {
+    inside.release();
     nodes.release();
     nodes4.release();
     tris.release();

@@ -129,6 +129,8 @@ int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value) {
     ctx->count_work = value != 0;
   else if (option == VACD_OPT_WIDE_BVH)
     ctx->wide_bvh = value != 0;
+  else if (option == VACD_OPT_INSIDE_GRID)
+    ctx->inside_grid = value != 0;
   else if (option == VACD_OPT_FAN_PAIRS)
     ctx->fan_pairs = value != 0;
   else if (option == VACD_OPT_SCORE_CHUNK && value >= 0 && value <= (int64_t(1) << 30))
@@ -479,6 +481,19 @@ int vacd_visibility_stats(vacd_ctx *ctx, vacd_vis_stats *out) {
     out->node_visits = (int64_t)s[5];
     out->tri_tests = (int64_t)s[6];
     out->walked = (int64_t)s[7];
+  });
+}
+
+int vacd_visibility_counters(vacd_ctx *ctx, int64_t *out, int32_t n) {
+  if (!ctx || (n > 0 && !out)) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(n >= 0 && n <= 16, VACD_ERR_INVALID, "at most 16 counters");
+    set_device(ctx);
+    unsigned long long s[16] = {};
+    VACD_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->buf[SLOT_STATS].ptr)
+      VACD_HIP(hipMemcpy(s, ctx->buf[SLOT_STATS].ptr, sizeof(s), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; ++k) out[k] = (int64_t)s[k];
   });
 }
 

@@ -588,6 +588,8 @@ struct Part {
   const int *fan_n;    // per point: triangles having the point as a vertex (-1: more
                        // than kFanMax); NULL together with far
   const int *fan_idx;  // per point: kFanMax leaf-ordered triangle ids
+  const unsigned long long *inside;  // inside grid of the closed self mesh (NULL: none)
+  GridBox grid;
   const Node *snodes;
   const Node4 *snodes4;
   const Tri *stris;
@@ -612,7 +614,12 @@ struct VisArgs {
   unsigned long long *stats;  // [0] masked, [1] blocked, [2] pairs redone exactly, [3] edges,
                               // [4] malformed, [5] node visits, [6] triangle tests,
                               // [7] walk-queue length, [8] deferred-list length,
-                              // [9] exact triangle tests
+                              // [9] exact triangle tests; work counters of
+                              // the fan pass (COUNT): pairs left undecided
+                              // with [10] no usable fan at p, [11] p's fan
+                              // test undecided, [12] no usable fan at q,
+                              // [13] q's undecided, [14] both ends leaving
+                              // outwards; [15] pairs settled by the grid
   int64_t *queue;             // global pair ids (word * 64 + bit) left to walk
   unsigned long long *queue_len;
   int64_t *dlist;             // global pair ids whose walks met an undecided test
@@ -691,6 +698,42 @@ __device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int 
   }
 }
 
+// Whether the segment has a sample point in a cell of the inside grid (the
+// cell lies wholly inside the solid, away from the surface by more than the
+// sample's rounding): then the segment has a point strictly inside the
+// solid and is not an edge.  Samples at most half a cell apart along every
+// axis, taken coarse to fine (midpoint first): a segment that leaves both
+// ends outwards and crosses the solid does so away from its ends.
+__device__ __forceinline__ bool meets_inside(const Part &P, const Seg &s) {
+  const GridBox &g = P.grid;
+  float d[3], span = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    d[k] = s.q[k] - s.p[k];
+    span = fmaxf(span, fabsf(d[k]) * g.inv[k]);
+  }
+  int lg = 1;
+  while ((float)(1 << lg) < 2.0f * span && lg < 8) ++lg;
+  const float invK = 1.0f / (float)(1 << lg);
+  for (int i = 0; i < (1 << lg); ++i) {
+    const int k = (int)(__brev((unsigned)i) >> (32 - lg));
+    const float t = ((float)k + 0.5f) * invK;
+    const int cx = grid_cell(fmaf(t, d[0], s.p[0]), g.lo[0], g.inv[0]);
+    const int cy = grid_cell(fmaf(t, d[1], s.p[1]), g.lo[1], g.inv[1]);
+    const int cz = grid_cell(fmaf(t, d[2], s.p[2]), g.lo[2], g.inv[2]);
+    if ((P.inside[cz * kGrid + cy] >> cx) & 1ull) return true;
+  }
+  return false;
+}
+
+// Mask of a point: bit 0 new, bit 1 new and on the last cut's cap, bits 2-7
+// on one of six older cut planes.  A pair is skipped when neither point is
+// new (lib/ptx/ray_segments.cu:61-75) or when both lie on one cut plane: its
+// segment then lies in that plane, where only rounding decides contact.
+__device__ __forceinline__ bool mask_skips(uint8_t a, uint8_t b) {
+  return (!(a & 3) && !(b & 3)) || ((a >> 1) & (b >> 1)) != 0;
+}
+
 // Appends `value` of every lane with `mine` set to a list, in lane order,
 // with one atomic per wave.
 __device__ __forceinline__ void append_lanes(bool mine, int64_t value, int64_t *list,
@@ -711,6 +754,9 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   int *stk = &stk_all[wave][0][lane];
   unsigned n_masked = 0, n_blocked = 0, n_exact = 0, n_bad = 0, nv = 0, nt = 0, n_walk = 0;
   (void)n_exact;  // filtered walks make no exact tests
+  // work counters (COUNT): undecided pairs by the state of each end's fan
+  // test, and pairs settled by the inside grid
+  unsigned n_pno = 0, n_pund = 0, n_qno = 0, n_qund = 0, n_out = 0, n_grid = 0;
   const int64_t total_waves = (int64_t)gridDim.x * kWaves;
   for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < a.nwords; w += total_waves) {
     const Part P = part_of_word<BATCH>(a, w);  // wave-uniform
@@ -719,13 +765,14 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     if (k < P.S) {
       int64_t i, j;
       pair_of(k, P.n, i, j);
-      if (P.mask && ((!P.mask[i] && !P.mask[j]) || (P.mask[i] == 2 && P.mask[j] == 2))) {
+      if (P.mask && mask_skips(P.mask[i], P.mask[j])) {
         ++n_masked;
       } else {
         Seg s;
 #pragma unroll
         for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
         bool decided = same3(s.p, s.q);  // zero length: not an edge
+        int pstate = 2, qstate = 2;  // work counters: 1 outwards, 2 no fan, 3 undecided
         if (!decided && P.fan_n && !(CAGE && P.ctris)) {
           // A pair that leaves p into the solid is blocked or wholly inside:
           // never an edge.  Decide that from p's fan alone (every lane of a
@@ -741,6 +788,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             } else if (r == 0 && P.far[i] <= 1 && ((xp ^ P.far[i]) & 1)) {
               decided = true;  // starts inside
             }
+            if (COUNT) pstate = (r == 0 && P.far[i] <= 1) ? 1 : 3;
           }
           // The same from q's end: a pair that enters the solid at q (the
           // segment seen from q towards p starts inside) is blocked or
@@ -761,7 +809,18 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             } else if (r == 0 && P.far[j] <= 1 && ((xq ^ P.far[j]) & 1)) {
               decided = true;  // enters the solid at q
             }
+            if (COUNT) qstate = (r == 0 && P.far[j] <= 1) ? 1 : 3;
           }
+          // Leaves both ends outwards: settled when it crosses a cell that
+          // lies inside the solid.
+          if (!decided && P.inside) {
+            decided = meets_inside(P, s);
+            if (COUNT && decided) ++n_grid;
+          }
+        }
+        if (COUNT && !decided) {
+          n_pno += pstate == 2, n_pund += pstate == 3, n_qno += qstate == 2, n_qund += qstate == 3;
+          n_out += pstate == 1 && qstate == 1;
         }
         if (!decided) {
           if constexpr (QUEUE) {
@@ -795,13 +854,19 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   }
   if (COUNT) {
     unsigned long long v = nv, t = nt;
+    unsigned c[6] = {n_pno, n_pund, n_qno, n_qund, n_out, n_grid};
     for (int off = 32; off > 0; off >>= 1) {
       v += __shfl_down(v, off);
       t += __shfl_down(t, off);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) c[k] += __shfl_down(c[k], off);
     }
     if (lane == 0) {
       atomicAdd(&a.stats[5], v);
       atomicAdd(&a.stats[6], t);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        if (c[k]) atomicAdd(&a.stats[10 + k], (unsigned long long)c[k]);
     }
   }
 }
@@ -1281,6 +1346,8 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   P.far = nullptr;
   P.fan_n = nullptr;
   P.fan_idx = nullptr;
+  P.inside = nullptr;
+  P.grid = GridBox{};
   P.snodes = self->nodes.as<Node>();
   P.snodes4 = self->nodes4.as<Node4>();
   P.stris = self->tris.as<Tri>();
@@ -1458,6 +1525,10 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
       a.one.fan_n = fanb.as<int>();
       a.one.fan_idx = fanb.as<int>() + n;
     }
+    if (ctx->inside_grid && self->has_grid) {
+      a.one.inside = self->inside.as<unsigned long long>();
+      a.one.grid = self->grid;
+    }
   }
   const int stack2 = std::max({binary_stack(self->depth, self->n_tris, self->small_first),
                                binary_stack(self->depth_refs, self->n_refs, self->small_first_refs),
@@ -1508,6 +1579,10 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
         if (fan) {
           hp[p].fan_n = fanb.as<int>() + hp[p].point_begin;
           hp[p].fan_idx = fanb.as<int>() + points + hp[p].point_begin * kFanMax;
+        }
+        if (ctx->inside_grid && parts[p].self->has_grid) {
+          hp[p].inside = parts[p].self->inside.as<unsigned long long>();
+          hp[p].grid = parts[p].self->grid;
         }
       }
   }

@@ -28,7 +28,7 @@ EXPORTED = [
     "vacd_bvh_build_d",
     "vacd_bvh_destroy", "vacd_bvh_info", "vacd_bvh_contact_info", "vacd_bvh_export",
     "vacd_visibility",
-    "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_deferred",
+    "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_counters", "vacd_visibility_deferred",
     "vacd_visibility_batch_d", "vacd_visibility_batch", "vacd_score_planes",
     "vacd_score_planes_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
     "vacd_max_distance", "vacd_max_distance_d", "vacd_max_distance_batch", "vacd_plane_extents_batch",
@@ -83,6 +83,7 @@ VACD_OPT_WIDE_BVH = 3
 VACD_OPT_FAN_PAIRS = 4
 VACD_OPT_SCORE_CHUNK = 5
 VACD_OPT_LANES_PER_PAIR = 6
+VACD_OPT_INSIDE_GRID = 7
 
 
 _lib = None
@@ -120,6 +121,7 @@ def lib():
         L.vacd_visibility.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, ctypes.POINTER(i64)]
         L.vacd_visibility_d.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, vp]
         L.vacd_visibility_stats.argtypes = [vp, ctypes.POINTER(VisStats)]
+        L.vacd_visibility_counters.argtypes = [vp, vp, ctypes.c_int32]
         L.vacd_visibility_deferred.argtypes = [vp, vp, i64, ctypes.POINTER(i64)]
         L.vacd_visibility_batch_d.argtypes = [vp, ctypes.POINTER(Part), ctypes.c_int32, vp, i64,
                                               vp, vp]
@@ -310,6 +312,18 @@ class Context:
         out = np.zeros((max(1, sum(sizes)), 2), dtype=np.float64)
         self._check(lib().vacd_plane_extents_batch(self._h, arr, len(queries), _p(out)))
         return np.split(out[: sum(sizes)], np.cumsum(sizes)[:-1])
+
+    COUNTER_NAMES = ("masked", "blocked", "unused2", "edges", "malformed", "node_visits",
+                     "tri_tests", "walked", "exact_calls", "exact_tri_tests", "undecided_no_fan_p",
+                     "undecided_fan_p", "undecided_no_fan_q", "undecided_fan_q",
+                     "undecided_both_outwards", "settled_by_grid")
+
+    def visibility_counters(self) -> dict:
+        """Raw work counters of the last visibility call (fan-pass reasons
+        only with VACD_OPT_COUNT_WORK)."""
+        out = np.zeros(16, dtype=np.int64)
+        self._check(lib().vacd_visibility_counters(self._h, _p(out), 16))
+        return {k: int(v) for k, v in zip(self.COUNTER_NAMES, out)}
 
     def visibility_stats(self) -> dict:
         s = VisStats()
