@@ -148,27 +148,47 @@ __global__ __launch_bounds__(kTile) void k_hist(const uint32_t *__restrict__ key
   hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of m entries with one block of 1024 threads.
+// Exclusive scan of m entries with one block of 1024 threads, 4096
+// consecutive entries per step (4 per thread): thread sums, a wave scan by
+// shuffles, the 16 wave totals scanned in LDS, and a running carry.
 __global__ __launch_bounds__(1024) void k_scan_u32(uint32_t *__restrict__ data, int64_t m) {
-  __shared__ uint32_t part[1024];
-  const int64_t per = (m + 1023) / 1024;
-  const int64_t beg = threadIdx.x * per;
-  const int64_t end = beg + per < m ? beg + per : m;
-  uint32_t s = 0;
-  for (int64_t i = beg; i < end; ++i) s += data[i];
-  part[threadIdx.x] = s;
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t carry_s;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry_s = 0u;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+  for (int64_t base = 0; base < m; base += 4096) {
+    const int64_t i0 = base + 4 * (int64_t)threadIdx.x;
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < m ? data[i0 + k] : 0u;
+    const uint32_t tsum = v[0] + v[1] + v[2] + v[3];
+    uint32_t inc = tsum;  // inclusive scan over the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(inc, off);
+      if (lane >= off) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
     __syncthreads();
-    part[threadIdx.x] += v;
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (int w = 0; w < 16; ++w) {
+        const uint32_t x = wsum[w];
+        wsum[w] = run;
+        run += x;
+      }
+    }
     __syncthreads();
-  }
-  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-  for (int64_t i = beg; i < end; ++i) {
-    const uint32_t v = data[i];
-    data[i] = run;
-    run += v;
+    uint32_t run = carry_s + wsum[wave] + (inc - tsum);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (i0 + k < m) data[i0 + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry_s = run;  // the last thread ends on the step's total
+    __syncthreads();
   }
 }
 
@@ -397,39 +417,42 @@ __global__ __launch_bounds__(256) void k_depth(int64_t n, const int32_t *__restr
   atomicMax(depth, d);
 }
 
-// Closedness (mod 2): every undirected edge key lo<<32|hi toggles a parity bit
-// in an open-addressing table (linear probing, load factor <= 3/8).
-__global__ __launch_bounds__(256) void k_edge_parity(const uint32_t *__restrict__ T, int64_t nT,
-                                                     unsigned long long *keys, uint32_t *par,
-                                                     uint64_t mask) {
-  const int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (h >= 3 * nT) return;
-  const int64_t t = h / 3;
-  const int e = (int)(h % 3);
-  const uint32_t u = T[3 * t + e], v = T[3 * t + (e + 1) % 3];
-  if (u == v) return;
-  const unsigned long long key =
-      ((unsigned long long)(u < v ? u : v) << 32) | (unsigned long long)(u < v ? v : u);
-  uint64_t s = (key * 0x9E3779B97F4A7C15ull) >> 17;
-  for (uint64_t probe = 0; probe <= mask; ++probe) {
-    s &= mask;
-    const unsigned long long prev = atomicCAS(&keys[s], ~0ull, key);
-    if (prev == ~0ull || prev == key) {
-      atomicXor(&par[s], 1u);
-      return;
-    }
-    ++s;
-  }
+// Closedness (mod 2): every undirected edge must belong to an even number
+// of triangles.  Each edge key lo<<32|hi is hashed by two independent
+// 64-bit mixers (splitmix64 finalizer, D. Lemire / S. Vigna) and all hashes
+// are XOR-ed: an edge met an even number of times cancels, so a closed mesh
+// gives zero; an open one gives zero only if the hashes of its odd edges
+// happen to XOR to zero in both halves (probability 2^-128 for a random
+// function).  One pass over the triangles, no table.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
 }
 
-__global__ __launch_bounds__(256) void k_any_odd(const uint32_t *__restrict__ par, uint64_t n,
-                                                 int32_t *odd) {
-  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n;
-       s += (uint64_t)gridDim.x * blockDim.x)
-    if (par[s] & 1u) {
-      *odd = 1;
-      return;
+__global__ __launch_bounds__(256) void k_edge_xor(const uint32_t *__restrict__ T, int64_t nT,
+                                                  unsigned long long *__restrict__ acc) {
+  unsigned long long h1 = 0, h2 = 0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nT;
+       t += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      const uint32_t u = T[3 * t + e], v = T[3 * t + (e + 1) % 3];
+      if (u == v) continue;
+      const unsigned long long key =
+          ((unsigned long long)(u < v ? u : v) << 32) | (unsigned long long)(u < v ? v : u);
+      h1 ^= mix64(key + 0x9E3779B97F4A7C15ull);
+      h2 ^= mix64((key ^ 0xD6E8FEB86659FD93ull) * 0xA0761D6478BD642Full + 0x2545F4914F6CDD1Dull);
     }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    h1 ^= __shfl_down(h1, off);
+    h2 ^= __shfl_down(h2, off);
+  }
+  if ((threadIdx.x & 63) == 0 && (h1 | h2)) {
+    atomicXor(&acc[0], h1);
+    atomicXor(&acc[1], h2);
+  }
 }
 
 // Node4[x] = the (up to four) grandchildren of binary node x.
@@ -945,7 +968,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   const size_t o_bounds = take(16 * 4), o_depth = take(4), o_k0 = take(4 * nT),
                o_k1 = take(4 * nT), o_v0 = take(4 * nT), o_v1 = take(4 * nT),
                o_hist = take(4 * hist_n), o_pint = take(4 * n_int), o_pleaf = take(4 * nT),
-               o_flags = take(4 * n_int), o_root = take(6 * 4), o_odd = take(4),
+               o_flags = take(4 * n_int), o_root = take(6 * 4), o_odd = take(16),
                o_larger = take(n_int);
   vacd::DevBuf &scratch = ctx->buf[SLOT_SORT];
   scratch.ensure(off);
@@ -963,12 +986,6 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   float *root = reinterpret_cast<float *>(base + o_root);
   int32_t *odd = reinterpret_cast<int32_t *>(base + o_odd);
   uint8_t *larger0 = reinterpret_cast<uint8_t *>(base + o_larger);
-  uint64_t hsize = 1;
-  while (hsize < (uint64_t)(8 * nT)) hsize <<= 1;
-  vacd::DevBuf &table = ctx->buf[SLOT_MISC];
-  table.ensure(12 * (size_t)hsize);
-  unsigned long long *hkeys = table.as<unsigned long long>();
-  uint32_t *hpar = reinterpret_cast<uint32_t *>(table.as<char>() + 8 * (size_t)hsize);
 
   out->n_tris = nT;
   out->nodes.ensure(sizeof(Node) * (size_t)n_int);
@@ -1005,22 +1022,20 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   k_collapse4<<<grid_for(n_int, 256), 256, 0, st>>>(nodes, n_int, out->nodes4.as<Node4>());
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_BVH);
-  VACD_HIP(hipMemsetAsync(hkeys, 0xff, 8 * (size_t)hsize, st));
-  VACD_HIP(hipMemsetAsync(hpar, 0, 4 * (size_t)hsize, st));
-  VACD_HIP(hipMemsetAsync(odd, 0, 4, st));
-  k_edge_parity<<<grid_for(3 * nT, 256), 256, 0, st>>>(d_T, nT, hkeys, hpar, hsize - 1);
-  k_any_odd<<<(unsigned)std::min<uint64_t>((hsize + 255) / 256, 2048), 256, 0, st>>>(hpar, hsize,
-                                                                                     odd);
+  VACD_HIP(hipMemsetAsync(odd, 0, 16, st));
+  unsigned long long *edge_hash = reinterpret_cast<unsigned long long *>(odd);
+  k_edge_xor<<<std::min(grid_for(nT, 256), 1024u), 256, 0, st>>>(d_T, nT, edge_hash);
   VACD_HIP(hipGetLastError());
-  int32_t h_depth = 0, h_odd = 1;
+  int32_t h_depth = 0;
+  unsigned long long h_hash[2] = {1, 1};
   float h_root[6];
-  VACD_HIP(hipMemcpyAsync(&h_odd, odd, 4, hipMemcpyDeviceToHost, st));
+  VACD_HIP(hipMemcpyAsync(h_hash, edge_hash, 16, hipMemcpyDeviceToHost, st));
   VACD_HIP(hipMemcpyAsync(&h_depth, depth, 4, hipMemcpyDeviceToHost, st));
   VACD_HIP(hipMemcpyAsync(h_root, root, 24, hipMemcpyDeviceToHost, st));
   VACD_HIP(hipStreamSynchronize(st));
   out->depth = h_depth;
   out->small_first = true;
-  out->closed = h_odd == 0;
+  out->closed = h_hash[0] == 0 && h_hash[1] == 0;
   for (int k = 0; k < 6; ++k) out->bounds[k] = h_root[k];
   out->has_grid = false;
   if (out->closed && ctx->inside_grid) build_inside_grid(ctx, d_V, d_T, nT, out);
