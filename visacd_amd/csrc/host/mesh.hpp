@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <random>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -111,14 +112,23 @@ struct EngineOptions {
 inline EngineOptions engine;
 
 // Wall time per phase of the last process() call, in seconds.
+// Phase times and counters of the last process() call; add and count may be
+// called from several host threads.
 struct Profile {
   std::map<std::string, double> seconds;
   std::map<std::string, long> calls;
+  std::mutex mu;
   void add(const std::string &phase, double s) {
+    std::lock_guard<std::mutex> g(mu);
     seconds[phase] += s;
     calls[phase] += 1;
   }
+  void count(const std::string &name, long v) {
+    std::lock_guard<std::mutex> g(mu);
+    calls[name] += v;
+  }
   void clear() {
+    std::lock_guard<std::mutex> g(mu);
     seconds.clear();
     calls.clear();
   }

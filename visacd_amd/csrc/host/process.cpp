@@ -29,6 +29,7 @@
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 #include "mesh.hpp"
 #include "proposal.hpp"
@@ -197,7 +198,7 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
     bvh = build_bvh(E, part, true);
   }
   const int64_t n = (int64_t)part.points.size();
-  last_profile.calls["visibility.points"] += n;
+  last_profile.count("visibility.points", (long)(n));
   std::vector<float> P(3 * n);
   const std::vector<uint8_t> mask = point_mask(part);
   bool any_new = false;
@@ -232,13 +233,13 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
   vacd_bvh_destroy(bvh);
   vacd_vis_stats st;
   if (vacd_visibility_stats(E.ctx, &st) == VACD_OK) {
-    last_profile.calls["visibility.segments"] += st.segments;
-    last_profile.calls["visibility.masked"] += st.masked;
-    last_profile.calls["visibility.walked"] += st.walked;
-    last_profile.calls["visibility.exact_pairs"] += st.exact_calls;
+    last_profile.count("visibility.segments", (long)(st.segments));
+    last_profile.count("visibility.masked", (long)(st.masked));
+    last_profile.count("visibility.walked", (long)(st.walked));
+    last_profile.count("visibility.exact_pairs", (long)(st.exact_calls));
   }
   PhaseTimer append_timer("visibility.append");
-  last_profile.calls["visibility.edges"] += count;
+  last_profile.count("visibility.edges", (long)(count));
   part.intersecting_edges.reserve(part.intersecting_edges.size() + (size_t)count);
   for (int64_t e = 0; e < count; ++e)
     part.intersecting_edges.push_back(
@@ -282,7 +283,7 @@ vacd_bvh *build_bvh(const Engine &E, const Mesh &m, bool planar) {
   if (!pl.empty()) {
     E.check(vacd_bvh_build_planar(E.ctx, V.data(), (int64_t)m.vertices.size(), T.data(), (int64_t)nT,
                                   grp.data(), pl.data(), (int)(pl.size() / 4), &b));
-    last_profile.calls["visibility.planar_groups"] += (long)(pl.size() / 4);
+    last_profile.count("visibility.planar_groups", (long)((long)(pl.size() / 4)));
   } else {
     E.check(vacd_bvh_build(E.ctx, V.data(), (int64_t)m.vertices.size(), T.data(), (int64_t)nT, &b));
   }
@@ -517,7 +518,7 @@ void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, boo
       }
       vp.push_back(vacd_part{P[k].data(), n, with_mask ? M[k].data() : nullptr, bvhs[k], nullptr});
       idx.push_back(k);
-      last_profile.calls["visibility.points"] += n;
+      last_profile.count("visibility.points", (long)(n));
     }
     if (!vp.empty()) {
       std::vector<int64_t> off(vp.size() + 1, 0);
@@ -538,12 +539,12 @@ void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, boo
       call_timer.stop();
       vacd_vis_stats st;
       if (vacd_visibility_stats(E.ctx, &st) == VACD_OK) {
-        last_profile.calls["visibility.segments"] += st.segments;
-        last_profile.calls["visibility.walked"] += st.walked;
-        last_profile.calls["visibility.exact_pairs"] += st.exact_calls;
+        last_profile.count("visibility.segments", (long)(st.segments));
+        last_profile.count("visibility.walked", (long)(st.walked));
+        last_profile.count("visibility.exact_pairs", (long)(st.exact_calls));
         if (engine.verbose && st.exact_calls > 100000) log_exact_hotspot(E, parts, idx, vp, st);
       }
-      last_profile.calls["visibility.edges"] += count;
+      last_profile.count("visibility.edges", (long)(count));
       for (size_t j = 0; j < idx.size(); ++j) {
         Mesh &part = *parts[idx[j]];
         part.intersecting_edges.reserve(part.intersecting_edges.size() + (size_t)(off[j + 1] - off[j]));
@@ -561,57 +562,68 @@ void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, boo
 
 // Concavities of the parts whose entry is NaN: hulls, volumes and surface
 // samples on all host threads, point-to-mesh distances on the GPU.
-void concavity_batch(const Engine &E, const MeshList &parts, std::vector<double> &conc) {
-  PhaseTimer timer("concavity");
+// Host half of the concavity of the parts whose entry is NaN: hulls,
+// volumes, surface samples and the two distance queries of each part, on
+// all host threads.
+struct ConcavityPrep {
   std::vector<size_t> idx;
-  for (size_t i = 0; i < parts.size(); ++i)
-    if (std::isnan(conc[i])) idx.push_back(i);
-  struct Prep {
-    Mesh hull;
-    double rv = 0.0, host_value = 0.0;
-    std::vector<Vec3> sa, sb;
-    bool gpu = false;
-  };
-  std::vector<Prep> prep(idx.size());
+  std::vector<double> rv, host_value;
+  std::vector<char> gpu;  // 1: distances from the queries, 2: no samples
+  std::vector<DistQuery> qs;  // 2 per part (empty for host-measured parts)
+};
+
+ConcavityPrep concavity_prep(const MeshList &parts, const std::vector<double> &conc) {
   PhaseTimer host_timer("concavity.host");
+  ConcavityPrep cp;
+  for (size_t i = 0; i < parts.size(); ++i)
+    if (std::isnan(conc[i])) cp.idx.push_back(i);
+  const size_t J = cp.idx.size();
+  cp.rv.assign(J, 0.0);
+  cp.host_value.assign(J, 0.0);
+  cp.gpu.assign(J, 0);
+  cp.qs.resize(2 * J);
 #pragma omp parallel for schedule(dynamic)
-  for (long j = 0; j < (long)idx.size(); ++j) {
-    const Mesh &part = parts[idx[j]];
-    Prep &q = prep[j];
-    q.hull = convex_hull(part.vertices);
-    if (!engine.gpu_distance || q.hull.triangles.empty()) {
-      q.host_value = concavity(part, q.hull, engine.hausdorff_resolution, engine.sample_seed);
+  for (long j = 0; j < (long)J; ++j) {
+    const Mesh &part = parts[cp.idx[j]];
+    const Mesh hull = convex_hull(part.vertices);
+    if (!engine.gpu_distance || hull.triangles.empty()) {
+      cp.host_value[j] = concavity(part, hull, engine.hausdorff_resolution, engine.sample_seed);
       continue;
     }
-    q.gpu = true;
-    q.rv = std::cbrt(3.0 * std::fabs(mesh_volume(part) - mesh_volume(q.hull)) / (4.0 * M_PI));
+    cp.rv[j] = std::cbrt(3.0 * std::fabs(mesh_volume(part) - mesh_volume(hull)) / (4.0 * M_PI));
     std::mt19937 rng(engine.sample_seed);
-    q.sa = surface_samples(part, engine.hausdorff_resolution, rng);
-    q.sb = surface_samples(q.hull, engine.hausdorff_resolution, rng);
+    const std::vector<Vec3> sa = surface_samples(part, engine.hausdorff_resolution, rng);
+    const std::vector<Vec3> sb = surface_samples(hull, engine.hausdorff_resolution, rng);
+    if (sa.empty() || sb.empty()) {
+      cp.gpu[j] = 2;
+      continue;
+    }
+    cp.gpu[j] = 1;
+    cp.qs[2 * j].set(sa, hull);
+    cp.qs[2 * j + 1].set(sb, part);
   }
-  host_timer.stop();
+  return cp;
+}
+
+// GPU half: every part's two one-sided distances in one batched call.
+void concavity_finish(const Engine &E, const ConcavityPrep &cp, std::vector<double> &conc) {
   PhaseTimer gpu_timer("concavity.distance");
-  // every gpu part's two one-sided distances in one batched call
-  std::vector<DistQuery> qs;
-  std::vector<size_t> at(idx.size(), SIZE_MAX);
-  for (size_t j = 0; j < idx.size(); ++j)
-    if (prep[j].gpu && !prep[j].sa.empty() && !prep[j].sb.empty()) at[j] = qs.size(), qs.resize(qs.size() + 2);
-#pragma omp parallel for schedule(dynamic)
-  for (long j = 0; j < (long)idx.size(); ++j) {
-    if (at[j] == SIZE_MAX) continue;
-    qs[at[j]].set(prep[j].sa, prep[j].hull);
-    qs[at[j] + 1].set(prep[j].sb, parts[idx[j]]);
-  }
-  const std::vector<double> d = max_distances(E, qs);
-  for (size_t j = 0; j < idx.size(); ++j) {
-    const Prep &q = prep[j];
-    if (!q.gpu)
-      conc[idx[j]] = q.host_value;
-    else if (at[j] == SIZE_MAX)
-      conc[idx[j]] = std::max(0.3 * q.rv, 1e300);
+  const std::vector<double> d = max_distances(E, cp.qs);
+  for (size_t j = 0; j < cp.idx.size(); ++j) {
+    if (cp.gpu[j] == 0)
+      conc[cp.idx[j]] = cp.host_value[j];
+    else if (cp.gpu[j] == 2)
+      conc[cp.idx[j]] = std::max(0.3 * cp.rv[j], 1e300);
     else
-      conc[idx[j]] = std::max(0.3 * q.rv, std::max(d[at[j]], d[at[j] + 1]));
+      conc[cp.idx[j]] = std::max(0.3 * cp.rv[j], std::max(d[2 * j], d[2 * j + 1]));
   }
+}
+
+// Concavities of the parts whose entry is NaN.
+void concavity_batch(const Engine &E, const MeshList &parts, std::vector<double> &conc) {
+  PhaseTimer timer("concavity");
+  const ConcavityPrep cp = concavity_prep(parts, conc);
+  concavity_finish(E, cp, conc);
 }
 
 // ------------------------------------------------------- plane selection
@@ -661,7 +673,7 @@ void score_subsets(const Engine &E, const std::vector<const Mesh *> &ms,
     q[parts[j]].assign(q32.begin() + off, q32.begin() + off + sp[j].P);
     off += sp[j].P;
   }
-  last_profile.calls["scoring.planes"] += total;
+  last_profile.count("scoring.planes", (long)(total));
 }
 
 // First maximum of the scores in plane order; planes from ss_offset on (the
@@ -758,8 +770,8 @@ std::vector<int> choose_planes(Engine &E, const std::vector<const Mesh *> &ms,
           if (r[i] || i >= n) sub[f].push_back((int)i);
       }
     }
-    last_profile.calls["proposal.parts"] += (long)ranked.size();
-    for (size_t f : ranked) last_profile.calls["proposal.ranked_planes"] += (long)ss_offset[f];
+    last_profile.count("proposal.parts", (long)((long)ranked.size()));
+    for (size_t f : ranked) last_profile.count("proposal.ranked_planes", (long)((long)ss_offset[f]));
   }
   {
     PhaseTimer t("scoring");
@@ -782,7 +794,7 @@ std::vector<int> choose_planes(Engine &E, const std::vector<const Mesh *> &ms,
         std::iota(sub[f].begin(), sub[f].end(), 0);
       }
       score_subsets(E, ms, planes, sub, redo, in, q);
-      last_profile.calls["proposal.fallbacks"] += (long)redo.size();
+      last_profile.count("proposal.fallbacks", (long)((long)redo.size()));
     }
   }
   for (size_t f : live) best[f] = first_max(sub[f], q[f], ss_offset[f]);
@@ -861,6 +873,39 @@ struct Driver {
   }
 };
 
+// Visibility of the fresh parts on a helper thread (BVH builds and GPU
+// passes) while the host threads prepare the concavity of every part whose
+// entry is NaN (the fresh ones), then their distances on the GPU.  The
+// helper only appends to the fresh parts' edge lists; the concavity reads
+// their vertices and triangles.
+void visibility_and_concavity(Driver &D, const std::vector<Mesh *> &fresh, bool with_mask,
+                              const MeshList &parts, std::vector<double> &conc) {
+  if (config.score_mode != "concavity") {
+    add_visibility_batch(D.E, fresh, with_mask, D.pair_buf);
+    return;
+  }
+  std::exception_ptr err;
+  std::thread helper([&] {
+    try {
+      add_visibility_batch(D.E, fresh, with_mask, D.pair_buf);
+    } catch (...) {
+      err = std::current_exception();
+    }
+  });
+  ConcavityPrep cp;
+  try {
+    PhaseTimer timer("concavity");
+    cp = concavity_prep(parts, conc);
+  } catch (...) {
+    helper.join();
+    throw;
+  }
+  helper.join();
+  if (err) std::rethrow_exception(err);
+  PhaseTimer timer("concavity");
+  concavity_finish(D.E, cp, conc);
+}
+
 // Batched frontier: every part above the concavity threshold and within
 // engine.frontier_fraction of the worst part's concavity (up to the part
 // budget, worst first) is split in the same round.  Planes, clips, hulls and
@@ -883,9 +928,9 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
     choose_points(parts[i], false, (uint32_t)seed[i]);
     fresh.push_back(&parts[i]);
   }
-  add_visibility_batch(D.E, fresh, false, D.pair_buf);
   const double nan = std::numeric_limits<double>::quiet_NaN();
   std::vector<double> conc(parts.size(), nan);
+  visibility_and_concavity(D, fresh, false, parts, conc);
   std::vector<char> frozen(parts.size(), 0);  // could not be split
   for (int round = 0;; ++round) {
     const int budget = num_parts - (int)parts.size();
@@ -1008,10 +1053,13 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
     frozen = std::move(nfrozen);
     fresh.clear();
     for (size_t i = first_kid; i < parts.size(); ++i) fresh.push_back(&parts[i]);
-    add_visibility_batch(D.E, fresh, true, D.pair_buf);
     bool any = false;
     for (size_t f = 0; f < F; ++f) any |= split_ok[f] != 0;
     if (!any) break;
+    // a full part budget ends the loop: the last parts need no edges and
+    // no concavity (the final concavity is measured on their hulls below)
+    if ((int)parts.size() >= num_parts) break;
+    visibility_and_concavity(D, fresh, true, parts, conc);
   }
   log_line("Computing convex hulls for " + std::to_string(parts.size()) + " parts...");
   PhaseTimer final_timer("final_hulls");
