@@ -200,53 +200,70 @@ std::vector<Plane> flat_surface_planes(const Mesh &m, double min_area) {
     const double l = norm(n);
     nrm[i] = l > 0 ? n * (1.0 / l) : Vec3{0, 0, 0};
   }
-  std::unordered_map<uint64_t, std::vector<int>> by_edge;
-  for (int i = 0; i < T; ++i)
+  // triangle adjacency across shared edges, from a vertex -> triangle index
+  // (for each edge (a, b) of t: the other triangles of a that contain b)
+  const int nv = (int)m.vertices.size();
+  std::vector<int> inc_off(nv + 1, 0), inc;
+  for (const auto &t : m.triangles)
+    for (int k = 0; k < 3; ++k) ++inc_off[t[k] + 1];
+  for (int v = 0; v < nv; ++v) inc_off[v + 1] += inc_off[v];
+  inc.resize(inc_off[nv]);
+  {
+    std::vector<int> fill(inc_off.begin(), inc_off.end() - 1);
+    for (int i = 0; i < T; ++i)
+      for (int k = 0; k < 3; ++k) inc[fill[m.triangles[i][k]]++] = i;
+  }
+  std::vector<int> adj_off(T + 1, 0), adj;
+  adj.reserve(3 * (size_t)T);
+  for (int i = 0; i < T; ++i) {
+    const auto &f = m.triangles[i];
     for (int e = 0; e < 3; ++e) {
-      uint32_t a = m.triangles[i][e], b = m.triangles[i][(e + 1) % 3];
-      if (a > b) std::swap(a, b);
-      by_edge[((uint64_t)a << 32) | b].push_back(i);
+      const int a = f[e], b = f[(e + 1) % 3];
+      if (a == b) continue;
+      for (int x = inc_off[a]; x < inc_off[a + 1]; ++x) {
+        const int u = inc[x];
+        if (u == i) continue;
+        const auto &g = m.triangles[u];
+        if (g[0] == b || g[1] == b || g[2] == b) adj.push_back(u);
+      }
     }
-  std::vector<std::vector<int>> adj(T);
-  for (auto &kv : by_edge)
-    for (int a : kv.second)
-      for (int b : kv.second)
-        if (a != b) adj[a].push_back(b);
+    adj_off[i + 1] = (int)adj.size();
+  }
   struct Region {
     std::vector<int> tris;
     double area = 0;
   };
   std::vector<Region> regions;
   std::vector<char> used(T, 0);
+  std::vector<int> grown, queue;  // region being grown (reused)
   for (int s = 0; s < T; ++s) {
     if (used[s]) continue;
-    Region r;
-    r.tris.push_back(s);
+    grown.assign(1, s);
     used[s] = 1;
     Vec3 mean = nrm[s];
-    std::queue<int> q;
-    q.push(s);
-    while (!q.empty()) {
-      const int t = q.front();
-      q.pop();
-      for (int nb : adj[t]) {
+    queue.assign(1, s);
+    for (size_t qh = 0; qh < queue.size(); ++qh) {
+      const int t = queue[qh];
+      for (int x = adj_off[t]; x < adj_off[t + 1]; ++x) {
+        const int nb = adj[x];
         if (used[nb]) continue;
         const double lm = norm(mean), ln = norm(nrm[nb]);
         const double c = (lm > 0 && ln > 0) ? dot(mean, nrm[nb]) / (lm * ln) : 0.0;
         if (c > 0.999) {
           used[nb] = 1;
-          r.tris.push_back(nb);
-          const double k = (double)r.tris.size();
+          grown.push_back(nb);
+          const double k = (double)grown.size();
           mean = (mean * (k - 1) + nrm[nb]) * (1.0 / k);
-          q.push(nb);
+          queue.push_back(nb);
         }
       }
     }
-    for (int t : r.tris) {
+    double area = 0.0;
+    for (int t : grown) {
       const auto &f = m.triangles[t];
-      r.area += tri_area(m.vertices[f[0]], m.vertices[f[1]], m.vertices[f[2]]);
+      area += tri_area(m.vertices[f[0]], m.vertices[f[1]], m.vertices[f[2]]);
     }
-    if (r.area > min_area) regions.push_back(std::move(r));
+    if (area > min_area) regions.push_back(Region{grown, area});
   }
   if (regions.size() > 40) {
     std::stable_sort(regions.begin(), regions.end(),

@@ -38,9 +38,15 @@
 namespace vhost {
 namespace {
 
+// start of the running process() call, for the verbose log's timestamps
+std::chrono::steady_clock::time_point process_start = std::chrono::steady_clock::now();
+
 void log_line(const std::string &msg) {
   if (!engine.verbose) return;
-  std::cout << "[visacd] " << msg << "\n";
+  const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - process_start).count();
+  char stamp[32];
+  std::snprintf(stamp, sizeof(stamp), "[visacd %8.3f s] ", t);
+  std::cout << stamp << msg << "\n";
   std::cout.flush();
 }
 
@@ -920,6 +926,7 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
   mesh.intersecting_edges.clear();
   mesh.points.clear();
   if (config.use_flat_surfaces) D.flat = flat_surface_planes(mesh, config.flat_surface_min_area);
+  log_line("  " + std::to_string(D.flat.size()) + " flat surface(s)");
   MeshList parts = separate_disjoint(mesh);
   std::vector<uint64_t> seed(parts.size());
   std::vector<Mesh *> fresh;
@@ -930,7 +937,9 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
   }
   const double nan = std::numeric_limits<double>::quiet_NaN();
   std::vector<double> conc(parts.size(), nan);
+  log_line("  " + std::to_string(parts.size()) + " connected part(s)");
   visibility_and_concavity(D, fresh, false, parts, conc);
+  log_line("  initial visibility and concavity done");
   std::vector<char> frozen(parts.size(), 0);  // could not be split
   for (int round = 0;; ++round) {
     const int budget = num_parts - (int)parts.size();
@@ -987,6 +996,7 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
       for (size_t f = 0; f < F; ++f) ms[f] = &parts[cand[f].second], ps[f] = seed[cand[f].second];
       best = choose_planes(D.E, ms, planes, ss_offset, ps);
     }
+    log_line("  planes chosen");
     // clips, edge inheritance and connected components on all host threads
     std::vector<MeshList> kids(F);
     std::vector<char> split_ok(F, 0);
@@ -1047,6 +1057,7 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
         nfrozen.push_back(0);
       }
     }
+    log_line("  parts clipped");
     parts = std::move(next);
     seed = std::move(nseed);
     conc = std::move(nconc);
@@ -1060,6 +1071,7 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
     // no concavity (the final concavity is measured on their hulls below)
     if ((int)parts.size() >= num_parts) break;
     visibility_and_concavity(D, fresh, true, parts, conc);
+    log_line("  visibility and concavity of " + std::to_string(fresh.size()) + " new part(s) done");
   }
   log_line("Computing convex hulls for " + std::to_string(parts.size()) + " parts...");
   PhaseTimer final_timer("final_hulls");
@@ -1088,6 +1100,7 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
 
 ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
   last_profile.clear();
+  process_start = std::chrono::steady_clock::now();
   PhaseTimer total("total");
   Driver D(engine.device);
   if (engine.mode == "batched") return process_batched(D, std::move(mesh), concavity_threshold, num_parts);
