@@ -141,47 +141,131 @@ void launch_max_distance(vacd_ctx *ctx, const float *d_pts, int64_t m, const vac
 // Many (points, mesh) queries in one launch, without a BVH: the point-to-mesh
 // distance is needed for a few thousand surface samples against every part
 // of a decomposition round and against its hull, and building a BVH per
-// mesh (a dozen launches, allocations and a host wait each) costs far more
-// than measuring every sample against every triangle.  A block takes 256
-// samples of one query and a chunk of kDistChunk of its triangles, staged
-// 256 at a time in LDS with their bounding spheres; a triangle whose sphere
-// lies farther than the best distance so far (with a relative slack well
-// above float32 rounding) cannot hold the minimum and is skipped.  Per-sample
-// minima (squared, as float bits) meet through atomicMin in a device array;
-// a second kernel takes each query's maximum.  The minimum over all
-// triangles equals that of the BVH walk, which prunes the same way.
+// mesh (a dozen launches, allocations and a host wait each) costs more than
+// this.  Three steps:
+//   k_dist_bound: an upper bound per sample, its distance to the nearest of
+//     up to kRep vertices spread over the query's triangles (points of the
+//     mesh), staged in LDS;
+//   k_dist_chunks: a bounding sphere per chunk of kDistChunk triangles (a
+//     mesh lists neighbouring triangles near each other);
+//   k_dist_batch: a block takes 256 samples and one chunk; lanes whose bound
+//     the chunk's sphere cannot beat skip it (the block skips it when all
+//     do); the rest stage its triangles 256 at a time in LDS with their own
+//     spheres and measure exactly those that can come closer than the best
+//     so far (relative slack well above float32 rounding).
+// Per-sample minima (squared, as float bits) meet through atomicMin; a last
+// kernel takes each query's maximum.  The minimum over all triangles equals
+// that of the BVH walk, which prunes the same way.
 namespace vacd {
 namespace {
 
 constexpr int kDistChunk = 1024;  // triangles per block
+constexpr int kRep = 512;         // vertices of the upper-bound pass
 
 struct DistQ {
-  int64_t pt0, m;     // first sample, sample count
-  int64_t v0, t0, nT; // first vertex, first triangle, triangle count
-  int64_t blk0;       // first block of the query
-  int64_t tchunks;    // triangle chunks of the query
+  int64_t pt0, m;      // first sample, sample count
+  int64_t v0, t0, nT;  // first vertex, first triangle, triangle count
+  int64_t ch0;         // first chunk (kDistChunk triangles)
+  int64_t blk0;        // first block of the query
+  int64_t tchunks;     // triangle chunks of the query
+  int64_t sblk0;       // first block of the bound pass
 };
 
-__global__ __launch_bounds__(256) void k_dist_fill(unsigned *__restrict__ d, int64_t n) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) d[i] = 0x7f800000u;  // +inf
+__device__ __forceinline__ int find_query(const DistQ *__restrict__ qs, int nq, int64_t key,
+                                          int which) {
+  int lo = 0, hi = nq - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t v = which == 0 ? qs[mid].blk0 : (which == 1 ? qs[mid].ch0 : qs[mid].sblk0);
+    if (v <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// dmin[s] = squared distance to the nearest of the query's kRep spread
+// vertices (+inf without triangles); one block per 256 samples.
+__global__ __launch_bounds__(256) void k_dist_bound(const DistQ *__restrict__ qs, int nq,
+                                                    const float *__restrict__ pts,
+                                                    const float *__restrict__ V,
+                                                    const uint32_t *__restrict__ T,
+                                                    unsigned *__restrict__ dmin) {
+  __shared__ float4 s_rep[kRep];
+  const DistQ q = qs[find_query(qs, nq, blockIdx.x, 2)];
+  const int64_t s = (blockIdx.x - q.sblk0) * 256 + threadIdx.x;
+  const int nr = (int)(q.nT < kRep ? q.nT : kRep);
+  for (int r = threadIdx.x; r < nr; r += 256) {
+    const int64_t t = (int64_t)r * q.nT / nr;  // spread over the triangle list
+    const float *X = V + 3 * (q.v0 + T[3 * (q.t0 + t)]);
+    s_rep[r] = make_float4(X[0], X[1], X[2], 0.f);
+  }
+  __syncthreads();
+  if (s >= q.m) return;
+  const float p[3] = {pts[3 * (q.pt0 + s)], pts[3 * (q.pt0 + s) + 1], pts[3 * (q.pt0 + s) + 2]};
+  float best = INFINITY;
+  for (int r = 0; r < nr; ++r) {
+    const float4 x = s_rep[r];
+    const float dx = p[0] - x.x, dy = p[1] - x.y, dz = p[2] - x.z;
+    best = fminf(best, dx * dx + dy * dy + dz * dz);
+  }
+  dmin[q.pt0 + s] = __float_as_uint(best);
+}
+
+// Bounding sphere (box centre, farthest vertex) of each chunk.
+__global__ __launch_bounds__(256) void k_dist_chunks(const DistQ *__restrict__ qs, int nq,
+                                                     int64_t nch, const float *__restrict__ V,
+                                                     const uint32_t *__restrict__ T,
+                                                     float4 *__restrict__ sph) {
+  __shared__ float red[6][256];
+  const int64_t c = blockIdx.x;  // one block per chunk
+  const DistQ q = qs[find_query(qs, nq, c, 1)];
+  const int64_t t0 = (c - q.ch0) * kDistChunk;
+  const int64_t t1 = q.nT < t0 + kDistChunk ? q.nT : t0 + kDistChunk;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += 256)
+    for (int v = 0; v < 3; ++v) {
+      const float *X = V + 3 * (q.v0 + T[3 * (q.t0 + t) + v]);
+      for (int k = 0; k < 3; ++k) lo[k] = fminf(lo[k], X[k]), hi[k] = fmaxf(hi[k], X[k]);
+    }
+  for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = lo[k], red[3 + k][threadIdx.x] = hi[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 3; ++k) {
+        red[k][threadIdx.x] = fminf(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+        red[3 + k][threadIdx.x] = fmaxf(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  const float cx = 0.5f * (red[0][0] + red[3][0]), cy = 0.5f * (red[1][0] + red[4][0]),
+              cz = 0.5f * (red[2][0] + red[5][0]);
+  __syncthreads();
+  float r2 = 0.0f;
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += 256)
+    for (int v = 0; v < 3; ++v) {
+      const float *X = V + 3 * (q.v0 + T[3 * (q.t0 + t) + v]);
+      const float dx = X[0] - cx, dy = X[1] - cy, dz = X[2] - cz;
+      r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
+    }
+  red[0][threadIdx.x] = r2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[0][threadIdx.x] = fmaxf(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sph[c] = make_float4(cx, cy, cz, sqrtf(red[0][0]));
 }
 
 __global__ __launch_bounds__(256) void k_dist_batch(const DistQ *__restrict__ qs, int nq,
                                                     const float *__restrict__ pts,
                                                     const float *__restrict__ V,
                                                     const uint32_t *__restrict__ T,
+                                                    const float4 *__restrict__ csph,
                                                     unsigned *__restrict__ dmin) {
   __shared__ float4 sa[256], sb[256], sc[256];
   __shared__ float sr[256];
   const int64_t blk = blockIdx.x;
-  int lo = 0, hi = nq - 1;
-  while (lo < hi) {  // query owning this block (block-uniform)
-    const int mid = (lo + hi + 1) >> 1;
-    if (qs[mid].blk0 <= blk) lo = mid;
-    else hi = mid - 1;
-  }
-  const DistQ q = qs[lo];
+  const DistQ q = qs[find_query(qs, nq, blk, 0)];  // block-uniform
   const int64_t local = blk - q.blk0;
   const int64_t st = local / q.tchunks, tc = local % q.tchunks;
   const int64_t s = st * 256 + threadIdx.x;
@@ -190,9 +274,16 @@ __global__ __launch_bounds__(256) void k_dist_batch(const DistQ *__restrict__ qs
   float best = 0.f;
   if (valid) {
     p[0] = pts[3 * (q.pt0 + s)], p[1] = pts[3 * (q.pt0 + s) + 1], p[2] = pts[3 * (q.pt0 + s) + 2];
-    best = __uint_as_float(dmin[q.pt0 + s]);  // what other chunks found already
+    best = __uint_as_float(dmin[q.pt0 + s]);
   }
   float reach = sqrtf(best);
+  {  // the whole chunk first: skipped when no lane can improve on its bound
+    const float4 b = csph[q.ch0 + tc];
+    const float dx = p[0] - b.x, dy = p[1] - b.y, dz = p[2] - b.z;
+    const float lim = (reach + b.w) * 1.0001f + 1e-30f;
+    const bool need = valid && dx * dx + dy * dy + dz * dz <= lim * lim;
+    if (!__syncthreads_or(need)) return;
+  }
   const int64_t t_begin = tc * kDistChunk;
   const int64_t t_end = q.nT < t_begin + kDistChunk ? q.nT : t_begin + kDistChunk;
   for (int64_t t0 = t_begin; t0 < t_end; t0 += 256) {
@@ -247,22 +338,58 @@ __global__ __launch_bounds__(256) void k_dist_max(const DistQ *__restrict__ qs,
 
 }  // namespace
 
+namespace {
+// Copies m samples to dst in the order of their cell in a 16^3 grid over
+// their box (Morton order of the cells, counting sort), so that a block's
+// 256 samples lie close together and skip the same far chunks.  A query's
+// result, a maximum over its samples, does not depend on their order.
+void sort_samples(const float *src, int64_t m, float *dst, std::vector<uint32_t> &cell) {
+  float lo[3] = {src[0], src[1], src[2]}, hi[3] = {src[0], src[1], src[2]};
+  for (int64_t i = 1; i < m; ++i)
+    for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], src[3 * i + k]), hi[k] = std::max(hi[k], src[3 * i + k]);
+  float inv[3];
+  for (int k = 0; k < 3; ++k) inv[k] = hi[k] > lo[k] ? 16.0f / (hi[k] - lo[k]) : 0.0f;
+  cell.resize((size_t)m);
+  std::vector<uint32_t> start(4097, 0);
+  for (int64_t i = 0; i < m; ++i) {
+    uint32_t key = 0;
+    for (int k = 0; k < 3; ++k) {
+      int c = (int)((src[3 * i + k] - lo[k]) * inv[k]);
+      c = c < 0 ? 0 : (c > 15 ? 15 : c);
+      for (int b = 0; b < 4; ++b) key |= (uint32_t)((c >> b) & 1) << (3 * b + k);
+    }
+    cell[(size_t)i] = key;
+    ++start[key + 1];
+  }
+  for (int c = 0; c < 4096; ++c) start[c + 1] += start[c];
+  for (int64_t i = 0; i < m; ++i) {
+    const uint32_t at = start[cell[(size_t)i]]++;
+    for (int k = 0; k < 3; ++k) dst[3 * (size_t)at + k] = src[3 * i + k];
+  }
+}
+}  // namespace
+
 void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, float *out) {
   hipStream_t st = ctx->stream;
+  std::vector<uint32_t> cell;
   std::vector<DistQ> qs((size_t)nq);
-  int64_t np = 0, nv = 0, nt = 0, nb = 0;
+  int64_t np = 0, nv = 0, nt = 0, nch = 0, nb = 0, nsb = 0;
   for (int i = 0; i < nq; ++i) {
     VACD_REQUIRE(q[i].m >= 0 && q[i].nV >= 0 && q[i].nT >= 0, VACD_ERR_INVALID,
                  "negative size in a distance query");
     VACD_REQUIRE((q[i].m == 0 || q[i].pts) && (q[i].nV == 0 || q[i].V) && (q[i].nT == 0 || q[i].T),
                  VACD_ERR_INVALID, "NULL array in a distance query");
     DistQ &d = qs[(size_t)i];
-    d.pt0 = np, d.m = q[i].m, d.v0 = nv, d.t0 = nt, d.nT = q[i].nT, d.blk0 = nb;
+    d.pt0 = np, d.m = q[i].m, d.v0 = nv, d.t0 = nt, d.nT = q[i].nT, d.blk0 = nb, d.ch0 = nch;
+    d.sblk0 = nsb;
     d.tchunks = std::max<int64_t>(1, (q[i].nT + kDistChunk - 1) / kDistChunk);
     if (q[i].m > 0 && q[i].nT > 0) nb += ((q[i].m + 255) / 256) * d.tchunks;
+    nsb += (q[i].m + 255) / 256;
     np += q[i].m, nv += q[i].nV, nt += q[i].nT;
+    nch += (q[i].nT + kDistChunk - 1) / kDistChunk;
   }
-  VACD_REQUIRE(nb < (int64_t(1) << 31), VACD_ERR_LIMIT, "distance batch too large");
+  VACD_REQUIRE(nb < (int64_t(1) << 31) && nch < (int64_t(1) << 31), VACD_ERR_LIMIT,
+               "distance batch too large");
   // one pinned region: query table | points | vertices | triangles
   const size_t b_q = ((sizeof(DistQ) * (size_t)nq + 255) & ~size_t(255));
   const size_t b_p = ((12 * (size_t)np + 255) & ~size_t(255));
@@ -274,7 +401,7 @@ void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, flo
   char *hp = h + b_q, *hv = hp + b_p, *ht = hv + b_v;
   for (int i = 0; i < nq; ++i) {
     const DistQ &d = qs[(size_t)i];
-    if (d.m) std::memcpy(hp + 12 * d.pt0, q[i].pts, 12 * (size_t)d.m);
+    if (d.m) sort_samples(q[i].pts, d.m, reinterpret_cast<float *>(hp + 12 * d.pt0), cell);
     if (q[i].nV) std::memcpy(hv + 12 * d.v0, q[i].V, 12 * (size_t)q[i].nV);
     if (d.nT) {
       const uint32_t *src = q[i].T;
@@ -287,7 +414,8 @@ void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, flo
   DevBuf &b = ctx->buf[SLOT_DIST];
   const size_t b_min = ((4 * (size_t)np + 255) & ~size_t(255));
   const size_t b_out = ((4 * (size_t)nq + 255) & ~size_t(255));
-  b.ensure(bytes + b_min + b_out);
+  const size_t b_ch = 16 * (size_t)std::max<int64_t>(nch, 1);
+  b.ensure(bytes + b_min + b_out + b_ch);
   char *d = b.as<char>();
   pinned_upload(ctx, PINNED_DIST, d, bytes);
   const DistQ *dq = reinterpret_cast<const DistQ *>(d);
@@ -296,8 +424,10 @@ void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, flo
   const uint32_t *dt = reinterpret_cast<const uint32_t *>(d + b_q + b_p + b_v);
   unsigned *dmin = reinterpret_cast<unsigned *>(d + bytes);
   float *dout = reinterpret_cast<float *>(d + bytes + b_min);
-  if (np) k_dist_fill<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(dmin, np);
-  if (nb) k_dist_batch<<<(unsigned)nb, 256, 0, st>>>(dq, nq, dp, dv, dt, dmin);
+  float4 *csph = reinterpret_cast<float4 *>(d + bytes + b_min + b_out);
+  if (nsb) k_dist_bound<<<(unsigned)nsb, 256, 0, st>>>(dq, nq, dp, dv, dt, dmin);
+  if (nch) k_dist_chunks<<<(unsigned)nch, 256, 0, st>>>(dq, nq, nch, dv, dt, csph);
+  if (nb) k_dist_batch<<<(unsigned)nb, 256, 0, st>>>(dq, nq, dp, dv, dt, csph, dmin);
   if (nq) k_dist_max<<<(unsigned)nq, 256, 0, st>>>(dq, dmin, dout);
   VACD_HIP(hipGetLastError());
   VACD_HIP(hipMemcpyAsync(out, dout, 4 * (size_t)nq, hipMemcpyDeviceToHost, st));
