@@ -369,9 +369,15 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the batched 64-part step")
     ap.add_argument("--no-mlp", action="store_true", help="skip the proposal model timing")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--acd-run", metavar="WORKLOAD:MODE:PROPOSAL",
+                    help="only one decomposition, e.g. c5:batched:exact (for profiling)")
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe(args.workload)
+        return
+    if args.acd_run:
+        wname, mode, prop = args.acd_run.split(":")
+        print(json.dumps(acd_wall(make_workload(wname), mode, prop)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
