@@ -88,6 +88,34 @@ def test_mlp_forward_matches_fp32_reference(ctx):
     m.close()
 
 
+def test_fused_inference_head_matches_the_training_forward(ctx):
+    """The fused inference head (g1 and g2 kept in registers) gives the same
+    predictions as the unfused forward of a training step, up to fp32
+    summation order: the step's loss (lr 0) equals the loss of the fused
+    outputs.  P is not a multiple of 32, so row tiles span parts."""
+    import torch
+
+    from visacd_amd import native
+
+    print("mlp fused head", flush=True)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    m = native.Mlp(ctx, seed=5)
+    B, E, P = 4, 64, 200
+    feat, planes, target = _batch(2, B, E, P)
+    out = torch.empty(B * P, device="cuda")
+    m.forward_d(B, E, feat.cuda(), P, planes.cuda(), out)
+    torch.cuda.synchronize()
+    loss_fused = float(((out.cpu() - target.reshape(-1)) ** 2).mean())
+    loss_step = m.train_step_d(B, E, feat.cuda(), P, planes.cuda(), target.reshape(-1).cuda(), 0.0)
+    assert abs(loss_fused - loss_step) <= 1e-4 * loss_step, (loss_fused, loss_step)
+    # and the same values again (deterministic)
+    out2 = torch.empty_like(out)
+    m.forward_d(B, E, feat.cuda(), P, planes.cuda(), out2)
+    torch.cuda.synchronize()
+    assert torch.equal(out, out2)
+    m.close()
+
+
 def test_mlp_first_adam_step_follows_reference_gradient(ctx):
     """Adam's first step moves each weight by -lr * sign(g) (|g| >> eps): the
     update's signs must agree with the fp32 reference gradient."""

@@ -161,6 +161,188 @@ __global__ void k_head_out(const unsigned short *__restrict__ g2, const float *_
   if (lane == 0) out[p] = s + c3[0];
 }
 
+// Inference head, fused: out[p] = v3 . bf16(relu(V2 g1[p] + c2)) + c3 with
+// g1[p] = bf16(relu(u[b] + Wp q[p])) built in registers, in the layout of the
+// MFMA A operand, so neither g1 nor g2 goes to memory (the unfused training
+// path writes both).  V2 (bf16, [n][k], rows padded by 16 bytes so the 32
+// lanes reading one k-slice of 32 rows hit distinct banks), Wp, c2 and v3
+// sit in LDS for the whole kernel; one block per CU, eight waves, each wave
+// takes 32 planes at a time: 16 k-steps of v_mfma_f32_32x32x16_bf16 over the
+// eight 32-column tiles of the 256 outputs (operands: lane l holds row or
+// column l % 32 and k = 8 (l / 32) .. +7; result element r of lane l is row
+// 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32).  The epilogue adds c2,
+// applies relu and the bf16 rounding of the unfused path, dots with v3 and
+// sums the 32 lanes of each half-wave in a fixed order (deterministic).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+constexpr int kHeadWaves = 8;
+constexpr int kV2Stride = kD + 8;  // bf16 per padded LDS row
+
+__global__ __launch_bounds__(64 * kHeadWaves) void k_head_fused(
+    const float *__restrict__ u, const float *__restrict__ Wp, const float *__restrict__ q,
+    int64_t BP, int P, const unsigned short *__restrict__ V2, const float *__restrict__ c2,
+    const float *__restrict__ v3, const float *__restrict__ c3, float *__restrict__ out) {
+  __shared__ __attribute__((aligned(16))) unsigned short sV[kD * kV2Stride];
+  __shared__ float4 sWp[kD];
+  __shared__ float sc2[kD], sv3[kD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // V2 into LDS, 16 bytes per load
+  for (int c = tid; c < kD * kD / 8; c += 64 * kHeadWaves) {
+    const int n = c / (kD / 8), k8 = c % (kD / 8);
+    *reinterpret_cast<uint4 *>(&sV[n * kV2Stride + 8 * k8]) =
+        reinterpret_cast<const uint4 *>(V2)[c];
+  }
+  for (int k = tid; k < kD; k += 64 * kHeadWaves) {
+    sWp[k] = make_float4(Wp[4 * k], Wp[4 * k + 1], Wp[4 * k + 2], Wp[4 * k + 3]);
+    sc2[k] = c2[k];
+    sv3[k] = v3[k];
+  }
+  __syncthreads();
+  const float bias3 = c3[0];
+  const int half = lane >> 5, col = lane & 31;
+  const int64_t tiles = (BP + 31) / 32;
+  for (int64_t t = (int64_t)blockIdx.x * kHeadWaves + wave; t < tiles;
+       t += (int64_t)gridDim.x * kHeadWaves) {
+    const int64_t row = t * 32 + col;  // this lane's A-operand row
+    const bool valid = row < BP;
+    const float4 qr = valid ? make_float4(q[4 * row], q[4 * row + 1], q[4 * row + 2], q[4 * row + 3])
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float *ub = u + (valid ? row / P : 0) * kD;
+    f32x16 acc[8];
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[nt][r] = 0.0f;
+    for (int ks = 0; ks < kD / 16; ++ks) {
+      const int k0 = ks * 16 + 8 * half;
+      bf16x8_t a;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const float4 w = sWp[k0 + i];
+        const float v = ub[k0 + i] + w.x * qr.x + w.y * qr.y + w.z * qr.z + w.w * qr.w;
+        a[i] = (__bf16)fmaxf(v, 0.0f);
+      }
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) {
+        const bf16x8_t b =
+            *reinterpret_cast<const bf16x8_t *>(&sV[(nt * 32 + col) * kV2Stride + k0]);
+        acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nt], 0, 0, 0);
+      }
+    }
+    // epilogue: the 16 rows this lane holds, partial dots over its columns
+    float part[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) part[r] = 0.0f;
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) {
+      const int n = nt * 32 + col;
+      const float bn = sc2[n], vn = sv3[n];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float g = (float)(__bf16)fmaxf(acc[nt][r] + bn, 0.0f);
+        part[r] = fmaf(vn, g, part[r]);
+      }
+    }
+    // sum over the 32 columns held by the 32 lanes of this half-wave
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) part[r] += __shfl_xor(part[r], off);
+    if (col == 0) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t orow = t * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+        if (orow < BP) out[orow] = part[r] + bias3;
+      }
+    }
+  }
+}
+
+// C[m][:] = bf16(relu(W A[m][:] + bias)) for a 256-wide layer (A bf16
+// [M][256], W bf16 [256][256] as [out][in]): the inference form of the
+// encoder's 256 -> 256 layers, laid out like k_head_fused (W in padded LDS
+// rows, eight waves, 32 rows per wave at a time) with the A operand read
+// straight from memory (lane l: row l % 32, 16 bytes at k = 8 (l / 32)).
+// POOL: instead of storing C, max-pool it over each part's E rows into
+// zbits[b][n] (float bits of the non-negative relu outputs, which order like
+// unsigned integers; zbits must start at 0).
+template <bool POOL>
+__global__ __launch_bounds__(64 * kHeadWaves) void k_layer256(
+    const unsigned short *__restrict__ A, int64_t M, const unsigned short *__restrict__ W,
+    const float *__restrict__ bias, unsigned short *__restrict__ C, int E,
+    unsigned *__restrict__ zbits) {
+  __shared__ __attribute__((aligned(16))) unsigned short sV[kD * kV2Stride];
+  __shared__ float sb[kD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int c = tid; c < kD * kD / 8; c += 64 * kHeadWaves) {
+    const int n = c / (kD / 8), k8 = c % (kD / 8);
+    *reinterpret_cast<uint4 *>(&sV[n * kV2Stride + 8 * k8]) = reinterpret_cast<const uint4 *>(W)[c];
+  }
+  for (int k = tid; k < kD; k += 64 * kHeadWaves) sb[k] = bias[k];
+  __syncthreads();
+  const int half = lane >> 5, col = lane & 31;
+  const int64_t tiles = (M + 31) / 32;
+  for (int64_t t = (int64_t)blockIdx.x * kHeadWaves + wave; t < tiles;
+       t += (int64_t)gridDim.x * kHeadWaves) {
+    const int64_t row = t * 32 + col;
+    const bool valid = row < M;
+    f32x16 acc[8];
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[nt][r] = 0.0f;
+    for (int ks = 0; ks < kD / 16; ++ks) {
+      const int k0 = ks * 16 + 8 * half;
+      bf16x8_t a;
+      if (valid) {
+        a = *reinterpret_cast<const bf16x8_t *>(A + row * kD + k0);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = (__bf16)0.0f;
+      }
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) {
+        const bf16x8_t b =
+            *reinterpret_cast<const bf16x8_t *>(&sV[(nt * 32 + col) * kV2Stride + k0]);
+        acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nt], 0, 0, 0);
+      }
+    }
+    if (!POOL) {
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) {
+        const int n = nt * 32 + col;
+        const float bn = sb[n];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t orow = t * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+          if (orow < M) C[orow * kD + n] = f2bf(fmaxf(acc[nt][r] + bn, 0.0f));
+        }
+      }
+    } else {
+      const int64_t last = (t * 32 + 31 < M ? t * 32 + 31 : M - 1);
+      const bool one_part = (t * 32) / E == last / E;
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) {
+        const int n = nt * 32 + col;
+        const float bn = sb[n];
+        float mx = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t orow = t * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+          if (orow >= M) continue;
+          // the value the unfused path pools: bf16 of the relu output
+          const float v = bf2f(f2bf(fmaxf(acc[nt][r] + bn, 0.0f)));
+          if (one_part) mx = fmaxf(mx, v);
+          else atomicMax(&zbits[(orow / E) * kD + n], __float_as_uint(v));
+        }
+        if (one_part) {
+          mx = fmaxf(mx, __shfl_xor(mx, 32));  // the other half-wave's rows
+          if (half == 0) atomicMax(&zbits[((t * 32) / E) * kD + n], __float_as_uint(mx));
+        }
+      }
+    }
+  }
+}
+
 // loss gradient and the last layer: dout = 2 (out - t) / BP; dg2 = relu'(g2) dout v3;
 // dv3 += dout g2 and dc3 += dout (atomics); loss += (out - t)^2 / BP
 __global__ void k_head_back(const float *__restrict__ out, const float *__restrict__ t,
@@ -339,7 +521,8 @@ struct Acts {
   float *tmp, *du, *dz, *loss;
 };
 
-Acts carve(vacd_mlp *M, int64_t R, int64_t B, int64_t BP, bool backward) {
+// keep_head false: no g1 / g2 (the fused inference head keeps them in registers)
+Acts carve(vacd_mlp *M, int64_t R, int64_t B, int64_t BP, bool backward, bool keep_head = true) {
   size_t off = 0;
   std::vector<std::pair<void **, size_t>> want;
   Acts a{};
@@ -349,8 +532,10 @@ Acts carve(vacd_mlp *M, int64_t R, int64_t B, int64_t BP, bool backward) {
   add((void **)&a.h2, 2 * R * kD);
   add((void **)&a.h3, 2 * R * kD);
   add((void **)&a.zb, 2 * B * kD);
-  add((void **)&a.g1, 2 * BP * kD);
-  add((void **)&a.g2, 2 * BP * kD);
+  if (keep_head) {
+    add((void **)&a.g1, 2 * BP * kD);
+    add((void **)&a.g2, 2 * BP * kD);
+  }
   add((void **)&a.z, 4 * B * kD);
   add((void **)&a.u, 4 * B * kD);
   add((void **)&a.out, 4 * BP);
@@ -380,8 +565,10 @@ Acts carve(vacd_mlp *M, int64_t R, int64_t B, int64_t BP, bool backward) {
 
 unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
 
+// fused: the inference head (k_head_fused) in place of g1, g2 and the output
+// layer as separate passes (training needs them)
 void forward(vacd_ctx *ctx, vacd_mlp *M, const Acts &a, int B, int E, const float *feat, int P,
-             const float *planes) {
+             const float *planes, bool fused = false) {
   hipStream_t st = ctx->stream;
   const float *w = M->w.as<float>();
   const unsigned short *wb = M->wb.as<unsigned short>();
@@ -389,14 +576,34 @@ void forward(vacd_ctx *ctx, vacd_mlp *M, const Acts &a, int B, int E, const floa
   k_edge_in<<<blocks_for(R * 8), 256, 0, st>>>(feat, R, a.x0);
   gemm_bf16(ctx, false, true, (int)R, kD, 8, a.x0, 8, wb + kB_W1, 8, w + M->L.b1, true, false,
             nullptr, a.h1, kD);
-  gemm_bf16(ctx, false, true, (int)R, kD, kD, a.h1, kD, wb + kB_W2, kD, w + M->L.b2, true, false,
-            nullptr, a.h2, kD);
-  gemm_bf16(ctx, false, true, (int)R, kD, kD, a.h2, kD, wb + kB_W3, kD, w + M->L.b3, true, false,
-            nullptr, a.h3, kD);
-  k_maxpool<<<(unsigned)B, kD, 0, st>>>(a.h3, E, a.z, a.arg);
+  if (fused) {  // the 256 -> 256 encoder layers with the weights in LDS,
+                // the last one max-pooled in its epilogue
+    const unsigned grid =
+        (unsigned)std::min<int64_t>(((R + 31) / 32 + kHeadWaves - 1) / kHeadWaves, 256);
+    k_layer256<false><<<grid, 64 * kHeadWaves, 0, st>>>(a.h1, R, wb + kB_W2, w + M->L.b2, a.h2, E,
+                                                        nullptr);
+    fill_u64(ctx, a.z, (int64_t)B * kD / 2, 0ull);
+    k_layer256<true><<<grid, 64 * kHeadWaves, 0, st>>>(a.h2, R, wb + kB_W3, w + M->L.b3, nullptr,
+                                                       E, reinterpret_cast<unsigned *>(a.z));
+    VACD_HIP(hipGetLastError());
+  } else {
+    gemm_bf16(ctx, false, true, (int)R, kD, kD, a.h1, kD, wb + kB_W2, kD, w + M->L.b2, true, false,
+              nullptr, a.h2, kD);
+    gemm_bf16(ctx, false, true, (int)R, kD, kD, a.h2, kD, wb + kB_W3, kD, w + M->L.b3, true, false,
+              nullptr, a.h3, kD);
+  }
+  if (!fused) k_maxpool<<<(unsigned)B, kD, 0, st>>>(a.h3, E, a.z, a.arg);
   to_bf16(ctx, a.z, (int64_t)B * kD, a.zb);
   gemm_bf16(ctx, false, true, B, kD, kD, a.zb, kD, wb + kB_Wz, kD, w + M->L.c1, false, false, a.u,
             nullptr, kD);
+  if (fused) {
+    const int64_t tiles = (BP + 31) / 32;
+    const unsigned grid = (unsigned)std::min<int64_t>((tiles + kHeadWaves - 1) / kHeadWaves, 256);
+    k_head_fused<<<grid, 64 * kHeadWaves, 0, st>>>(a.u, w + M->L.Wp, planes, BP, P, wb + kB_V2,
+                                                     w + M->L.c2, w + M->L.v3, w + M->L.c3, a.out);
+    VACD_HIP(hipGetLastError());
+    return;
+  }
   k_head1<<<blocks_for(BP * kD), 256, 0, st>>>(a.u, w + M->L.Wp, planes, BP, P, a.g1);
   gemm_bf16(ctx, false, true, (int)BP, kD, kD, a.g1, kD, wb + kB_V2, kD, w + M->L.c2, true, false,
             nullptr, a.g2, kD);
@@ -475,8 +682,8 @@ void mlp_get(vacd_ctx *ctx, vacd_mlp *M, float *host) {
 void mlp_forward(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
                  const float *planes, float *out) {
   check_dims(B, E, P);
-  const Acts a = carve(M, (int64_t)B * E, B, (int64_t)B * P, false);
-  forward(ctx, M, a, B, E, feat, P, planes);
+  const Acts a = carve(M, (int64_t)B * E, B, (int64_t)B * P, false, false);
+  forward(ctx, M, a, B, E, feat, P, planes, true);
   VACD_HIP(hipMemcpyAsync(out, a.out, 4 * (size_t)B * P, hipMemcpyDeviceToDevice, ctx->stream));
 }
 
@@ -489,8 +696,8 @@ void mlp_forward_host(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *fea
   float *dfeat = din.as<float>(), *dplanes = dfeat + 7 * R;
   VACD_HIP(hipMemcpyAsync(dfeat, feat, 28 * (size_t)R, hipMemcpyHostToDevice, ctx->stream));
   VACD_HIP(hipMemcpyAsync(dplanes, planes, 16 * (size_t)BP, hipMemcpyHostToDevice, ctx->stream));
-  const Acts a = carve(M, R, B, BP, false);
-  forward(ctx, M, a, B, E, dfeat, P, dplanes);
+  const Acts a = carve(M, R, B, BP, false, false);
+  forward(ctx, M, a, B, E, dfeat, P, dplanes, true);
   VACD_HIP(hipMemcpyAsync(out, a.out, 4 * (size_t)BP, hipMemcpyDeviceToHost, ctx->stream));
   VACD_HIP(hipStreamSynchronize(ctx->stream));
 }
