@@ -20,6 +20,7 @@
 // part's concavity is computed once and cached; engine.max_points bounds the
 // points whose pairs are tested by a seeded subsample.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <fstream>
@@ -65,15 +66,30 @@ struct PhaseTimer {
 };
 
 struct Engine {
+  int device = 0;
   vacd_ctx *ctx = nullptr;
   vacd_mlp *mlp = nullptr;  // proposal model, loaded on first use
-  explicit Engine(int device) {
+  // extra contexts (own streams) for BVH builds run from several host threads
+  std::vector<vacd_ctx *> aux;
+  explicit Engine(int dev) : device(dev) {
     if (vacd_ctx_create(device, &ctx) != VACD_OK)
       throw std::runtime_error(std::string("visacd: ") + vacd_last_error(nullptr));
   }
   ~Engine() {
     vacd_mlp_destroy(mlp);
+    for (vacd_ctx *c : aux) vacd_ctx_destroy(c);
     vacd_ctx_destroy(ctx);
+  }
+  // context i: 0 is ctx, the others are created on first use (call from one thread)
+  vacd_ctx *context(size_t i) {
+    if (i == 0) return ctx;
+    while (aux.size() < i) {
+      vacd_ctx *c = nullptr;
+      if (vacd_ctx_create(device, &c) != VACD_OK)
+        throw std::runtime_error(std::string("visacd: ") + vacd_last_error(nullptr));
+      aux.push_back(c);
+    }
+    return aux[i - 1];
   }
   Engine(const Engine &) = delete;
   Engine &operator=(const Engine &) = delete;
@@ -255,7 +271,14 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
 // BVH of a part.  planar: the caps of the part's cuts (triangles with all
 // three vertices on one of its last 7 cut planes, at least 16 of them) are
 // passed as planar groups, which the contact walks prune by plane distance.
-vacd_bvh *build_bvh(const Engine &E, const Mesh &m, bool planar) {
+vacd_bvh *build_bvh_on(vacd_ctx *ctx, const Mesh &m, bool planar);
+
+vacd_bvh *build_bvh(const Engine &E, const Mesh &m, bool planar) { return build_bvh_on(E.ctx, m, planar); }
+
+vacd_bvh *build_bvh_on(vacd_ctx *ctx, const Mesh &m, bool planar) {
+  auto check = [ctx](int rc) {
+    if (rc != VACD_OK) throw std::runtime_error(std::string("visacd: ") + vacd_last_error(ctx));
+  };
   const std::vector<float> V = to_f32(m.vertices);
   const size_t nT = m.triangles.size();
   std::vector<uint32_t> T(3 * nT);
@@ -287,11 +310,11 @@ vacd_bvh *build_bvh(const Engine &E, const Mesh &m, bool planar) {
   }
   vacd_bvh *b = nullptr;
   if (!pl.empty()) {
-    E.check(vacd_bvh_build_planar(E.ctx, V.data(), (int64_t)m.vertices.size(), T.data(), (int64_t)nT,
-                                  grp.data(), pl.data(), (int)(pl.size() / 4), &b));
+    check(vacd_bvh_build_planar(ctx, V.data(), (int64_t)m.vertices.size(), T.data(), (int64_t)nT,
+                                grp.data(), pl.data(), (int)(pl.size() / 4), &b));
     last_profile.count("visibility.planar_groups", (long)((long)(pl.size() / 4)));
   } else {
-    E.check(vacd_bvh_build(E.ctx, V.data(), (int64_t)m.vertices.size(), T.data(), (int64_t)nT, &b));
+    check(vacd_bvh_build(ctx, V.data(), (int64_t)m.vertices.size(), T.data(), (int64_t)nT, &b));
   }
   return b;
 }
@@ -492,7 +515,7 @@ void log_exact_hotspot(const Engine &E, const std::vector<Mesh *> &parts,
   log_line(o.str());
 }
 
-void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, bool with_mask,
+void add_visibility_batch(Engine &E, const std::vector<Mesh *> &parts, bool with_mask,
                           PairBuffer &buf) {
   PhaseTimer timer("visibility");
   const size_t np = parts.size();
@@ -518,14 +541,34 @@ void add_visibility_batch(const Engine &E, const std::vector<Mesh *> &parts, boo
         any_new |= (M[k][i] & 3) != 0;
       }
       if (with_mask && !any_new) continue;
-      {
-        PhaseTimer t("visibility.bvh");
-        bvhs[k] = build_bvh(E, part, true);
-      }
-      vp.push_back(vacd_part{P[k].data(), n, with_mask ? M[k].data() : nullptr, bvhs[k], nullptr});
       idx.push_back(k);
       last_profile.count("visibility.points", (long)(n));
     }
+    {  // the BVHs, from up to four host threads with a context each
+      PhaseTimer t("visibility.bvh");
+      const size_t W = std::min<size_t>(4, idx.size());
+      std::vector<vacd_ctx *> cs(W);
+      for (size_t w = 0; w < W; ++w) cs[w] = E.context(w);
+      std::atomic<size_t> next{0};
+      std::vector<std::exception_ptr> errs(W);
+      auto work = [&](size_t w) {
+        try {
+          for (size_t j = next++; j < idx.size(); j = next++)
+            bvhs[idx[j]] = build_bvh_on(cs[w], *parts[idx[j]], true);
+        } catch (...) {
+          errs[w] = std::current_exception();
+        }
+      };
+      std::vector<std::thread> pool;
+      for (size_t w = 1; w < W; ++w) pool.emplace_back(work, w);
+      if (W > 0) work(0);
+      for (auto &th : pool) th.join();
+      for (auto &e : errs)
+        if (e) std::rethrow_exception(e);
+    }
+    for (size_t k : idx)
+      vp.push_back(vacd_part{P[k].data(), (int64_t)parts[k]->points.size(),
+                             with_mask ? M[k].data() : nullptr, bvhs[k], nullptr});
     if (!vp.empty()) {
       std::vector<int64_t> off(vp.size() + 1, 0);
       int64_t cap = std::max<int64_t>(buf.cap, int64_t(1) << 22), count = 0;
@@ -906,7 +949,9 @@ void visibility_and_concavity(Driver &D, const std::vector<Mesh *> &fresh, bool 
     helper.join();
     throw;
   }
+  log_line("    concavity host work done");
   helper.join();
+  log_line("    visibility done");
   if (err) std::rethrow_exception(err);
   PhaseTimer timer("concavity");
   concavity_finish(D.E, cp, conc);
