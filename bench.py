@@ -446,6 +446,7 @@ def main():
     ctx.set_option(native.VACD_OPT_WIDE_BVH, 0)
     # the walks the fan leaves open, split over 1..16 lanes per pair
     lpp_ms = {}
+    ctx.set_option(native.VACD_OPT_FRONT_LANES, 0)  # depth-first walks for this sweep
     for lpp in (1, 2, 4, 8, 16):
         ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, lpp)
         vis_step()
@@ -456,6 +457,19 @@ def main():
         lpp_ms[lpp] = tl["vis_ms"] / max(1, tl["vis_launches"])
     ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
     log("lanes per pair -> kernel ms", lpp_ms)
+    # the short walk queue walked breadth-first with 4..64 lanes per pair, or
+    # depth-first (0: the lanes-per-pair walks above)
+    front_ms = {}
+    for fl in (0, 4, 8, 16, 32, 64):
+        ctx.set_option(native.VACD_OPT_FRONT_LANES, fl)
+        vis_step()
+        ctx.timing(reset=True)
+        for _ in range(max(2, args.steps // 4)):
+            vis_step()
+        tl = ctx.timing(reset=True)
+        front_ms[fl] = tl["vis_ms"] / max(1, tl["vis_launches"])
+    ctx.set_option(native.VACD_OPT_FRONT_LANES, 8)
+    log("breadth-first lanes per pair -> kernel ms", front_ms)
     # ... and with every pair walked (no fan shortcut for pairs starting inside)
     ctx.set_option(native.VACD_OPT_FAN_PAIRS, 0)
     vis_step()
@@ -530,6 +544,7 @@ def main():
         "visibility": {"edges": E, "blocked": st["blocked"], "exact_fallback_pairs": st["exact_calls"],
                        "kernel_ms": vis_kernel_ms, "kernel_ms_wide_walk": wide_kernel_ms,
                        "kernel_ms_by_lanes_per_pair": lpp_ms,
+                       "kernel_ms_by_front_lanes": front_ms,
                        "kernel_ms_walk_every_pair": walk_all_kernel_ms,
                        "kernel_gsegments_per_s": S / (vis_kernel_ms * 1e-3) / 1e9,
                        "bvh_build_ms": bt["bvh_ms"], "bvh_depth": binfo["depth"],

@@ -65,6 +65,10 @@ int vacd_visibility_counters(vacd_ctx *ctx, int64_t *out, int32_t n);
 #define VACD_OPT_SCORE_CHUNK 5 /* edges per scoring block, 0 = automatic    */
 #define VACD_OPT_LANES_PER_PAIR 6 /* lanes sharing one BVH walk of a pair   */
                                   /* the fan leaves open: 1,2,4,8,16 (1)    */
+#define VACD_OPT_FRONT_LANES 8 /* lanes per pair of the breadth-first walks */
+                               /* of short walk queues: 0, 4, 8, 16, 32, 64 */
+                               /* (8; 0 = depth-first walks only)           */
+#define VACD_OPT_FRONT_MAX 9   /* longest queue walked breadth-first (131072)*/
 #define VACD_OPT_INSIDE_GRID 7 /* parity mode: settle pairs that cross a    */
                                /* cell wholly inside a closed solid; the    */
                                /* grid is built with BVHs built while on (0)*/

@@ -186,14 +186,22 @@ def test_visibility_binary_and_wide_walks_agree(ctx, name):
                 ctx.set_option(native.VACD_OPT_WIDE_BVH, 0)
                 ctx.set_option(native.VACD_OPT_FAN_PAIRS, 1)
             np.testing.assert_array_equal(g, o, err_msg=f"wide={wide} fan={fan}")
-    # the walks the fan leaves open, split over 1..16 lanes per pair
-    for lpp in (1, 2, 4, 8, 16):
-        ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, lpp)
-        try:
+    # the walks the fan leaves open, split over 1..16 lanes per pair (depth
+    # first), and breadth first with 4..64 lanes per pair
+    ctx.set_option(native.VACD_OPT_FRONT_LANES, 0)
+    try:
+        for lpp in (1, 2, 4, 8, 16):
+            ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, lpp)
             g = ctx.visibility(V, ctx.bvh(V, T))
-        finally:
-            ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
-        np.testing.assert_array_equal(g, o, err_msg=f"lanes per pair {lpp}")
+            np.testing.assert_array_equal(g, o, err_msg=f"lanes per pair {lpp}")
+        ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
+        for fl in (4, 8, 16, 32, 64):
+            ctx.set_option(native.VACD_OPT_FRONT_LANES, fl)
+            g = ctx.visibility(V, ctx.bvh(V, T))
+            np.testing.assert_array_equal(g, o, err_msg=f"breadth-first lanes {fl}")
+    finally:
+        ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
+        ctx.set_option(native.VACD_OPT_FRONT_LANES, 8)
 
 
 @pytest.mark.parametrize("level,amp,seed", [(3, 0.5, 23), (3, 0.3, 5)])

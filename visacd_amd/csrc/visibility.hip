@@ -624,6 +624,7 @@ struct VisArgs {
   unsigned long long *queue_len;
   int64_t *dlist;             // global pair ids whose walks met an undecided test
   unsigned long long *dlist_len;
+  int64_t front_max;          // walk queues up to this long go to k_walk_front
 };
 
 // Part owning global ballot word w (binary search over word_begin).
@@ -903,6 +904,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
   unsigned n_blocked = 0, n_exact = 0, n_bad = 0, nv = 0, nt = 0;
   const int64_t *list = EXACT ? a.dlist : a.queue;
   const int64_t len = (int64_t)(EXACT ? *a.dlist_len : *a.queue_len);
+  if (!EXACT && len <= a.front_max) return;  // k_walk_front took this queue
   const int64_t e_lo = (EXACT && LV == 0) ? kExactSplit : 0;
   const int64_t e_hi = (EXACT && LV == 6) ? (len < kExactSplit ? len : kExactSplit) : len;
   for (int64_t e = e_lo + (int64_t)blockIdx.x * kGroups + slot; e < e_hi;
@@ -982,6 +984,149 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
     if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
     if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
     if (EXACT && n_exact) atomicAdd(&a.stats[9], (unsigned long long)n_exact);
+    if (COUNT) {
+      atomicAdd(&a.stats[5], (unsigned long long)nv);
+      atomicAdd(&a.stats[6], (unsigned long long)nt);
+    }
+  }
+}
+
+// Breadth-first walks of a short walk queue: G lanes per pair (a group),
+// the pair's frontier of tree nodes in LDS.  At each level the group's lanes
+// take the frontier's nodes, test both children, test the triangles of the
+// leaf children, and append the internal children that pass to the next
+// frontier (ballot offsets).  A walk then takes about as many dependent
+// steps as the tree is deep, instead of one per node visited: a short queue
+// does not fill the chip, and its time is the latency of its longest walks
+// (edges, which cannot stop early, visit a few hundred nodes).  Outcome as
+// in walk_pair for the any-contact walk and the local parity; a pair whose
+// local parity is unknown, whose tests were undecided or whose frontier
+// would overflow goes to the deferred list (the exact redo decides it,
+// midpoint ray included).  Only queues of at most a.front_max pairs are
+// taken; k_walk_split walks longer ones.
+template <bool COUNT, bool BATCH, int G>
+__global__ __launch_bounds__(kBlock) void k_walk_front(VisArgs a) {
+  constexpr int kGroups = kBlock / G;
+  constexpr int kFront = G >= 8 ? 16 * G : 128;  // frontier entries per group and buffer
+  __shared__ int front_all[kGroups][2][kFront];
+  const int lane = threadIdx.x & 63;
+  const int slot = threadIdx.x / G;
+  const int g = lane & (G - 1);
+  const uint64_t gm = group_mask(lane, G);
+  const uint64_t below = gm & ((1ull << lane) - 1ull);  // lanes of the group before this one
+  unsigned n_blocked = 0, n_bad = 0, nv = 0, nt = 0, n_exact = 0;
+  const int64_t len = (int64_t)*a.queue_len;
+  if (len > a.front_max) return;
+  for (int64_t e = (int64_t)blockIdx.x * kGroups + slot; e < len;
+       e += (int64_t)gridDim.x * kGroups) {
+    const int64_t kg = a.queue[e];
+    const int64_t w = kg >> 6;
+    const int b = (int)(kg & 63);
+    const Part P = part_of_word<BATCH>(a, w);
+    int64_t i, j;
+    pair_of((w - P.word_begin) * 64 + b, P.n, i, j);
+    Seg s;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
+    make_seg(s);
+    int *cur = front_all[slot][0], *nxt = front_all[slot][1];
+    if (g == 0) cur[0] = 0;  // the root
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int n = 1;
+    int64_t visits = 0;
+    int xpar = 0, ninc = 0;
+    bool local_ok = true, unsure = false, contact = false, bad = false, over = false;
+    while (n > 0) {
+      visits += n;
+      if (visits > P.rcap) {  // more nodes than the tree has: malformed tree
+        bad = true;
+        break;
+      }
+      int nn = 0;
+      for (int base = 0; base < n; base += G) {
+        const int idx = base + g;
+        bool w0 = false, w1 = false, hit = false;
+        int c0 = 0, c1 = 0;
+        if (idx < n) {
+          float bx[12];
+          int g0 = 0, g1 = 0;
+          if (P.rgroups) load_node_g(P.rnodes, cur[idx], bx, c0, c1, g0, g1);
+          else load_node(P.rnodes, cur[idx], bx, c0, c1);
+          if (COUNT) ++nv;
+          float t0, t1;
+          const bool h0 = g0 ? (group_span(s, P.rgroups, g0, t0, t1) &&
+                                seg_box_span(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], t0, t1))
+                             : seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+          const bool h1 = g1 ? (group_span(s, P.rgroups, g1, t0, t1) &&
+                                seg_box_span(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11], t0, t1))
+                             : seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+          if (h0 && c0 < 0) {
+            if (COUNT) ++nt;
+            hit |= contact_test<false>(s, P.rtris[~c0], xpar, ninc, local_ok, unsure, n_exact) == 1;
+          }
+          if (h1 && c1 < 0) {
+            if (COUNT) ++nt;
+            hit |= contact_test<false>(s, P.rtris[~c1], xpar, ninc, local_ok, unsure, n_exact) == 1;
+          }
+          w0 = h0 && c0 >= 0;
+          w1 = h1 && c1 >= 0;
+        }
+        const uint64_t m0 = __ballot(w0) & gm, m1 = __ballot(w1) & gm, mh = __ballot(hit) & gm;
+        if (mh) {
+          contact = true;
+          break;
+        }
+        const int cnt0 = __popcll(m0), tot = cnt0 + __popcll(m1);
+        if (nn + tot > kFront) {
+          over = true;
+          break;
+        }
+        if (w0) nxt[nn + __popcll(m0 & below)] = c0;
+        if (w1) nxt[nn + cnt0 + __popcll(m1 & below)] = c1;
+        nn += tot;
+      }
+      if (contact || over) break;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      int *t = cur;
+      cur = nxt;
+      nxt = t;
+      n = nn;
+    }
+    const uint64_t mu = __ballot(unsure) & gm, ml = __ballot(!local_ok) & gm,
+                   mx = __ballot(xpar & 1) & gm, mn = __ballot(ninc > 0) & gm;
+    bool edge = false, defer = false;
+    if (bad) {
+      if (g == 0) ++n_bad;
+    } else if (contact) {
+      if (g == 0) ++n_blocked;
+    } else if (over || mu || ml || !P.far || !mn || P.far[i] > 1) {
+      defer = true;  // undecided tests or no local parity: the exact redo
+    } else {
+      edge = (((__popcll(mx) & 1) ^ P.far[i]) & 1) == 0;
+    }
+    if (g == 0) {
+      if (edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
+      if (defer) a.dlist[atomicAdd(a.dlist_len, 1ull)] = kg;
+    }
+    // the next pair of this group reuses the frontier buffers: keep its
+    // writes after every lane's last read of this pair's frontier
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    n_blocked += __shfl_down(n_blocked, off);
+    n_bad += __shfl_down(n_bad, off);
+    nv += __shfl_down(nv, off);
+    nt += __shfl_down(nt, off);
+  }
+  if (lane == 0) {
+    if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
+    if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
     if (COUNT) {
       atomicAdd(&a.stats[5], (unsigned long long)nv);
       atomicAdd(&a.stats[6], (unsigned long long)nt);
@@ -1291,9 +1436,23 @@ __global__ void k_fix_part_offsets(VisArgs a, const int64_t *__restrict__ total,
 
 struct Launch {
   unsigned grid, qgrid;
-  int lv;  // log2 of the lanes per queued pair
+  int lv;     // log2 of the lanes per queued pair
+  int front;  // lanes per pair of the breadth-first walks of short queues (0: off)
   hipStream_t st;
 };
+
+template <bool N, bool B>
+void launch_front(const Launch &L, const VisArgs &a) {
+  constexpr unsigned kFrontGrid = 1024;
+  switch (L.front) {
+    case 0: break;
+    case 4: k_walk_front<N, B, 4><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+    case 8: k_walk_front<N, B, 8><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+    case 16: k_walk_front<N, B, 16><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+    case 32: k_walk_front<N, B, 32><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+    default: k_walk_front<N, B, 64><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+  }
+}
 
 template <int S, bool N, bool B>
 void launch_split(const Launch &L, const VisArgs &a) {
@@ -1312,7 +1471,8 @@ template <int S, bool C, bool N, bool W, bool B>
 void launch_main(bool q, const Launch &L, const VisArgs &a) {
   if (q && !C) {
     k_visibility<1, C, N, W, B, true><<<L.grid, kBlock, 0, L.st>>>(a);  // no walk, no stack
-    launch_split<S, N, B>(L, a);
+    launch_front<N, B>(L, a);  // a short queue: breadth-first walks
+    launch_split<S, N, B>(L, a);  // a long one: depth-first walks
   } else {
     k_visibility<S, C, N, W, B, false><<<L.grid, kBlock, 0, L.st>>>(a);
   }
@@ -1433,6 +1593,8 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   Launch L;
   L.grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
   L.lv = ctx->lanes_per_pair_log2;
+  L.front = ctx->front_lanes;
+  a.front_max = L.front ? ctx->front_max : -1;
   // the queue length is known on the device only: enough groups to cover
   // every pair once when all are queued, at most 16 blocks per CU
   L.qgrid = (unsigned)std::min<int64_t>(((nwords * 64) << L.lv) / kBlock + 1, 256 * 16);

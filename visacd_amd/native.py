@@ -84,6 +84,8 @@ VACD_OPT_FAN_PAIRS = 4
 VACD_OPT_SCORE_CHUNK = 5
 VACD_OPT_LANES_PER_PAIR = 6
 VACD_OPT_INSIDE_GRID = 7
+VACD_OPT_FRONT_LANES = 8
+VACD_OPT_FRONT_MAX = 9
 
 
 _lib = None
