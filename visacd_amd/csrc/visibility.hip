@@ -999,11 +999,91 @@ __global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
 // steps as the tree is deep, instead of one per node visited: a short queue
 // does not fill the chip, and its time is the latency of its longest walks
 // (edges, which cannot stop early, visit a few hundred nodes).  Outcome as
-// in walk_pair for the any-contact walk and the local parity; a pair whose
-// local parity is unknown, whose tests were undecided or whose frontier
-// would overflow goes to the deferred list (the exact redo decides it,
-// midpoint ray included).  Only queues of at most a.front_max pairs are
-// taken; k_walk_split walks longer ones.
+// in walk_pair: the any-contact walk, then the local parity, or the
+// midpoint ray (breadth first as well) when the local parity is unknown; a
+// pair whose tests were undecided or whose frontier would overflow goes to
+// the deferred list (the exact redo decides it).  Only queues of at most
+// a.front_max pairs are taken; k_walk_split walks longer ones.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One breadth-first walk by a group of G lanes (cur/nxt: the group's LDS
+// frontier buffers of KF entries).  RAY false: any contact of the segment
+// with the tree's triangles (the contact tree, planar groups pruned), with
+// p's fan crossings met on the way in xpar/ninc/local_ok as in walk_any.
+// RAY true: crossings of the +x ray from the midpoint (plain tree), parity
+// in xpar.  Per-lane results; returns (group-uniform) 0 done, 1 contact,
+// 3 more visits than the tree has nodes, 5 frontier overflow.
+template <int G, int KF, bool RAY, bool COUNT>
+__device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
+                                          const Tri *__restrict__ tris,
+                                          const float4 *__restrict__ grp, int cap, const Seg &s,
+                                          int *cur, int *nxt, int g, uint64_t gm, uint64_t below,
+                                          int &xpar, int &ninc, bool &local_ok, bool &unsure,
+                                          unsigned &nv, unsigned &nt) {
+  unsigned nex = 0;
+  if (g == 0) cur[0] = 0;  // the root
+  wave_lds_sync();
+  int n = 1;
+  int64_t visits = 0;
+  while (n > 0) {
+    visits += n;
+    if (visits > cap) return 3;
+    int nn = 0;
+    for (int base = 0; base < n; base += G) {
+      const int idx = base + g;
+      bool w0 = false, w1 = false, hit = false;
+      int c0 = 0, c1 = 0;
+      if (idx < n) {
+        float bx[12];
+        int g0 = 0, g1 = 0;
+        if (!RAY && grp) load_node_g(nodes, cur[idx], bx, c0, c1, g0, g1);
+        else load_node(nodes, cur[idx], bx, c0, c1);
+        if (COUNT) ++nv;
+        bool h0, h1;
+        if (RAY) {
+          h0 = ray_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+          h1 = ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+        } else {
+          float t0, t1;
+          h0 = g0 ? (group_span(s, grp, g0, t0, t1) &&
+                     seg_box_span(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], t0, t1))
+                  : seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+          h1 = g1 ? (group_span(s, grp, g1, t0, t1) &&
+                     seg_box_span(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11], t0, t1))
+                  : seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const int ch = c ? c1 : c0;
+          if (!(c ? h1 : h0) || ch >= 0) continue;
+          if (COUNT) ++nt;
+          if (RAY) xpar ^= cross_test<false>(s, tris[~ch], unsure, nex);
+          else hit |= contact_test<false>(s, tris[~ch], xpar, ninc, local_ok, unsure, nex) == 1;
+        }
+        w0 = h0 && c0 >= 0;
+        w1 = h1 && c1 >= 0;
+      }
+      const uint64_t m0 = __ballot(w0) & gm, m1 = __ballot(w1) & gm;
+      if (!RAY && (__ballot(hit) & gm)) return 1;
+      const int cnt0 = __popcll(m0), tot = cnt0 + __popcll(m1);
+      if (nn + tot > KF) return 5;
+      if (w0) nxt[nn + __popcll(m0 & below)] = c0;
+      if (w1) nxt[nn + cnt0 + __popcll(m1 & below)] = c1;
+      nn += tot;
+    }
+    wave_lds_sync();
+    int *t = cur;
+    cur = nxt;
+    nxt = t;
+    n = nn;
+  }
+  return 0;
+}
+
 template <bool COUNT, bool BATCH, int G>
 __global__ __launch_bounds__(kBlock) void k_walk_front(VisArgs a) {
   constexpr int kGroups = kBlock / G;
@@ -1014,9 +1094,10 @@ __global__ __launch_bounds__(kBlock) void k_walk_front(VisArgs a) {
   const int g = lane & (G - 1);
   const uint64_t gm = group_mask(lane, G);
   const uint64_t below = gm & ((1ull << lane) - 1ull);  // lanes of the group before this one
-  unsigned n_blocked = 0, n_bad = 0, nv = 0, nt = 0, n_exact = 0;
+  unsigned n_blocked = 0, n_bad = 0, nv = 0, nt = 0;
   const int64_t len = (int64_t)*a.queue_len;
   if (len > a.front_max) return;
+  int *fa = front_all[slot][0], *fb = front_all[slot][1];
   for (int64_t e = (int64_t)blockIdx.x * kGroups + slot; e < len;
        e += (int64_t)gridDim.x * kGroups) {
     const int64_t kg = a.queue[e];
@@ -1029,84 +1110,41 @@ __global__ __launch_bounds__(kBlock) void k_walk_front(VisArgs a) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
     make_seg(s);
-    int *cur = front_all[slot][0], *nxt = front_all[slot][1];
-    if (g == 0) cur[0] = 0;  // the root
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    int n = 1;
-    int64_t visits = 0;
     int xpar = 0, ninc = 0;
-    bool local_ok = true, unsure = false, contact = false, bad = false, over = false;
-    while (n > 0) {
-      visits += n;
-      if (visits > P.rcap) {  // more nodes than the tree has: malformed tree
-        bad = true;
-        break;
-      }
-      int nn = 0;
-      for (int base = 0; base < n; base += G) {
-        const int idx = base + g;
-        bool w0 = false, w1 = false, hit = false;
-        int c0 = 0, c1 = 0;
-        if (idx < n) {
-          float bx[12];
-          int g0 = 0, g1 = 0;
-          if (P.rgroups) load_node_g(P.rnodes, cur[idx], bx, c0, c1, g0, g1);
-          else load_node(P.rnodes, cur[idx], bx, c0, c1);
-          if (COUNT) ++nv;
-          float t0, t1;
-          const bool h0 = g0 ? (group_span(s, P.rgroups, g0, t0, t1) &&
-                                seg_box_span(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], t0, t1))
-                             : seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
-          const bool h1 = g1 ? (group_span(s, P.rgroups, g1, t0, t1) &&
-                                seg_box_span(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11], t0, t1))
-                             : seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
-          if (h0 && c0 < 0) {
-            if (COUNT) ++nt;
-            hit |= contact_test<false>(s, P.rtris[~c0], xpar, ninc, local_ok, unsure, n_exact) == 1;
-          }
-          if (h1 && c1 < 0) {
-            if (COUNT) ++nt;
-            hit |= contact_test<false>(s, P.rtris[~c1], xpar, ninc, local_ok, unsure, n_exact) == 1;
-          }
-          w0 = h0 && c0 >= 0;
-          w1 = h1 && c1 >= 0;
-        }
-        const uint64_t m0 = __ballot(w0) & gm, m1 = __ballot(w1) & gm, mh = __ballot(hit) & gm;
-        if (mh) {
-          contact = true;
-          break;
-        }
-        const int cnt0 = __popcll(m0), tot = cnt0 + __popcll(m1);
-        if (nn + tot > kFront) {
-          over = true;
-          break;
-        }
-        if (w0) nxt[nn + __popcll(m0 & below)] = c0;
-        if (w1) nxt[nn + cnt0 + __popcll(m1 & below)] = c1;
-        nn += tot;
-      }
-      if (contact || over) break;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      int *t = cur;
-      cur = nxt;
-      nxt = t;
-      n = nn;
-    }
-    const uint64_t mu = __ballot(unsure) & gm, ml = __ballot(!local_ok) & gm,
-                   mx = __ballot(xpar & 1) & gm, mn = __ballot(ninc > 0) & gm;
+    bool local_ok = true, unsure = false;
+    const int hs = front_walk<G, kFront, false, COUNT>(P.rnodes, P.rtris, P.rgroups, P.rcap, s, fa,
+                                                       fb, g, gm, below, xpar, ninc, local_ok,
+                                                       unsure, nv, nt);
+    uint64_t mu = __ballot(unsure) & gm;
+    const uint64_t ml = __ballot(!local_ok) & gm, mx = __ballot(xpar & 1) & gm,
+                   mn = __ballot(ninc > 0) & gm;
     bool edge = false, defer = false;
-    if (bad) {
+    if (hs == 3) {
       if (g == 0) ++n_bad;
-    } else if (contact) {
+    } else if (hs == 1) {
       if (g == 0) ++n_blocked;
-    } else if (over || mu || ml || !P.far || !mn || P.far[i] > 1) {
-      defer = true;  // undecided tests or no local parity: the exact redo
-    } else {
+    } else if (hs == 5 || mu) {
+      defer = true;  // undecided tests or a frontier overflow: the exact redo
+    } else if (!ml && P.far && mn && P.far[i] <= 1) {
+      // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
       edge = (((__popcll(mx) & 1) ^ P.far[i]) & 1) == 0;
+    } else {  // no local parity for this pair: the midpoint ray, breadth first too
+      wave_lds_sync();
+      int cross = 0, unused = 0;
+      bool ok = true;
+      unsure = false;
+      const int hr = front_walk<G, kFront, true, COUNT>(P.snodes, P.stris, nullptr, P.scap, s, fa,
+                                                        fb, g, gm, below, cross, unused, ok,
+                                                        unsure, nv, nt);
+      mu = __ballot(unsure) & gm;
+      const uint64_t mc = __ballot(cross & 1) & gm;
+      if (hr == 3) {
+        if (g == 0) ++n_bad;
+      } else if (hr == 5 || mu) {
+        defer = true;
+      } else {
+        edge = (__popcll(mc) & 1) == 0;
+      }
     }
     if (g == 0) {
       if (edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
@@ -1114,9 +1152,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_front(VisArgs a) {
     }
     // the next pair of this group reuses the frontier buffers: keep its
     // writes after every lane's last read of this pair's frontier
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_lds_sync();
   }
   for (int off = 32; off > 0; off >>= 1) {
     n_blocked += __shfl_down(n_blocked, off);
