@@ -136,6 +136,7 @@ enum VacdSlot {
   SLOT_DIST,
   SLOT_EXTENT,
   SLOT_GRID,
+  SLOT_FANREC,
   SLOT_COUNT_MAX
 };
 

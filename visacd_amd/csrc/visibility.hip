@@ -580,6 +580,48 @@ __device__ __forceinline__ int walk_parity4(const Node4 *__restrict__ nodes,
 // What a pair needs to know about its part (one mesh and its point set).
 constexpr int kFanMax = 16;  // triangles kept per point fan
 
+// A fan triangle (p, v1, v2) of point p (rotated to start at p), prepared
+// once per point so that its test against a pair (p, q) is linear in
+// d = q - p: orient3d(p, v1, v2, q) = -d . N with N = e1 x e2, e1 = v1 - p,
+// e2 = v2 - p; the (y, z) orientations of the local crossing test are
+// orient2d(p, v1, q) = e1y dz - e1z dy, orient2d(v2, p, q) = e2z dy - e2y dz
+// and orient2d(p, v1, v2) = N.x (its sign kept, decided as before).
+// Filters: |error| of -d . N is below 9 u sum |d_k| A_k (A_k: the absolute
+// products of N_k; u = 2^-53, counting the rounding of e1, e2, d, N and the
+// dot product), of the 2-D forms below 5 u (|e1y dz| + |e1z dy|); kFilter
+// (1e-14 = 90 u) keeps a tenfold margin.  Undecided tests are left to the
+// walks, as before.
+struct alignas(16) FanRec {
+  double nx, ny, nz, ax, ay, az;
+  double e1y, e1z, e2y, e2z;
+  float v1[3], v2[3];
+  int o0;  // sign of orient2d(p, v1, v2) in (y, z): 1, -1, 0 undecided
+  int pad;
+};
+static_assert(sizeof(FanRec) == 112, "fan record is 112 bytes");
+
+// contact_fast for a fan triangle of the pair's first point, from its
+// record: 1 the segment is an edge of the triangle, 0 no contact (xpar
+// toggled by a crossing at p + e d, ninc counted), 4 contact undecided,
+// 5 crossing undecided.  other: the pair's other point; d = other - p.
+__device__ __forceinline__ int fan_test(const FanRec &r, const float other[3], const double d[3],
+                                        int &xpar, int &ninc) {
+  const float w1[3] = {r.v1[0], r.v1[1], r.v1[2]}, w2[3] = {r.v2[0], r.v2[1], r.v2[2]};
+  if (same3(other, w1) || same3(other, w2)) return 1;
+  const double det = -(d[0] * r.nx + d[1] * r.ny + d[2] * r.nz);
+  const double bd = kFilter * (fabs(d[0]) * r.ax + fabs(d[1]) * r.ay + fabs(d[2]) * r.az);
+  if (!(det > bd || -det > bd)) return 4;
+  ++ninc;
+  const double o1 = r.e1y * d[2] - r.e1z * d[1];
+  const double b1 = kFilter * (fabs(r.e1y * d[2]) + fabs(r.e1z * d[1]));
+  const double o2 = r.e2z * d[1] - r.e2y * d[2];
+  const double b2 = kFilter * (fabs(r.e2z * d[1]) + fabs(r.e2y * d[2]));
+  if (r.o0 == 0 || !(o1 > b1 || -o1 > b1) || !(o2 > b2 || -o2 > b2)) return 5;
+  const bool s0 = r.o0 > 0;
+  if ((o1 > 0.0) == s0 && (o2 > 0.0) == s0 && (det > 0.0) == s0) xpar ^= 1;
+  return 0;
+}
+
 struct Part {
   const float *pts;
   const uint8_t *mask;
@@ -588,6 +630,7 @@ struct Part {
   const int *fan_n;    // per point: triangles having the point as a vertex (-1: more
                        // than kFanMax); NULL together with far
   const int *fan_idx;  // per point: kFanMax leaf-ordered triangle ids
+  const FanRec *fan_rec;  // per point: kFanMax prepared fan triangles
   const unsigned long long *inside;  // inside grid of the closed self mesh (NULL: none)
   GridBox grid;
   const Node *snodes;
@@ -779,10 +822,12 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           // never an edge.  Decide that from p's fan alone (every lane of a
           // row shares p, so the fan loads are broadcasts).
           const int fn = P.fan_n[i];
+          const double dq[3] = {(double)s.q[0] - s.p[0], (double)s.q[1] - s.p[1],
+                                (double)s.q[2] - s.p[2]};
           if (fn > 0) {
             int xp = 0, ni = 0, r = 0;
-            for (int f = 0; f < fn && r == 0; ++f)
-              r = contact_fast(s, P.stris[P.fan_idx[(size_t)i * kFanMax + f]], xp, ni);
+            const FanRec *fr = P.fan_rec + (size_t)i * kFanMax;
+            for (int f = 0; f < fn && r == 0; ++f) r = fan_test(fr[f], s.q, dq, xp, ni);
             if (r == 1) {  // the segment is an edge of the mesh
               ++n_blocked;
               decided = true;
@@ -798,12 +843,10 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           // pairs that leave both ends outwards are walked.
           const int fq = decided ? 0 : P.fan_n[j];
           if (fq > 0) {
-            Seg r_s;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) r_s.p[c] = s.q[c], r_s.q[c] = s.p[c];
+            const double dp[3] = {-dq[0], -dq[1], -dq[2]};  // p - q (negation is exact)
             int xq = 0, nq = 0, r = 0;
-            for (int f = 0; f < fq && r == 0; ++f)
-              r = contact_fast(r_s, P.stris[P.fan_idx[(size_t)j * kFanMax + f]], xq, nq);
+            const FanRec *fr = P.fan_rec + (size_t)j * kFanMax;
+            for (int f = 0; f < fq && r == 0; ++f) r = fan_test(fr[f], s.p, dp, xq, nq);
             if (r == 1) {
               ++n_blocked;
               decided = true;
@@ -1232,7 +1275,8 @@ __device__ int exact_ray_parity(const Node *__restrict__ nodes, const Tri *__res
 // the 4-wide BVH; -1 when there are more than kFanMax.
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
-                                            int *__restrict__ fan_idx) {
+                                            int *__restrict__ fan_idx,
+                                            FanRec *__restrict__ fan_rec) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (g >= a.npoints) return;
   Part P = a.one;
@@ -1288,6 +1332,32 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
   }
   // an unfinished walk or an overfull fan leaves the point to the full walk
   fan_n[g] = (!done || cnt > kFanMax) ? -1 : cnt;
+  if (!done || cnt > kFanMax) return;
+  // the fan's records (see FanRec)
+  for (int f = 0; f < cnt; ++f) {
+    const Tri t = P.stris[out[f]];
+    const float a3[3] = {t.a.x, t.a.y, t.a.z}, b3[3] = {t.b.x, t.b.y, t.b.z},
+                c3[3] = {t.c.x, t.c.y, t.c.z};
+    const bool pa = same3(p, a3), pb = same3(p, b3);
+    const float *v1 = pa ? b3 : (pb ? c3 : a3);
+    const float *v2 = pa ? c3 : (pb ? a3 : b3);
+    FanRec r;
+    const double e1x = (double)v1[0] - p[0], e1y = (double)v1[1] - p[1], e1z = (double)v1[2] - p[2];
+    const double e2x = (double)v2[0] - p[0], e2y = (double)v2[1] - p[1], e2z = (double)v2[2] - p[2];
+    r.nx = e1y * e2z - e1z * e2y;
+    r.ny = e1z * e2x - e1x * e2z;
+    r.nz = e1x * e2y - e1y * e2x;
+    r.ax = fabs(e1y * e2z) + fabs(e1z * e2y);
+    r.ay = fabs(e1z * e2x) + fabs(e1x * e2z);
+    r.az = fabs(e1x * e2y) + fabs(e1y * e2x);
+    r.e1y = e1y, r.e1z = e1z, r.e2y = e2y, r.e2z = e2z;
+    for (int k = 0; k < 3; ++k) r.v1[k] = v1[k], r.v2[k] = v2[k];
+    double b0;
+    const double o0 = o2yz(p, v1, v2, b0);
+    r.o0 = o0 > b0 ? 1 : (-o0 > b0 ? -1 : 0);
+    r.pad = 0;
+    fan_rec[(size_t)g * kFanMax + f] = r;
+  }
 }
 
 // Crossing parity (0/1) of the +x ray from each point with the triangles
@@ -1542,6 +1612,7 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   P.far = nullptr;
   P.fan_n = nullptr;
   P.fan_idx = nullptr;
+  P.fan_rec = nullptr;
   P.inside = nullptr;
   P.grid = GridBox{};
   P.snodes = self->nodes.as<Node>();
@@ -1613,10 +1684,11 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
       vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
       int *fan_n = fanb.as<int>();
       int *fan_idx = fan_n + a.npoints;
+      FanRec *fan_rec = ctx->buf[SLOT_FANREC].as<FanRec>();
       if (batch)
-        k_fan<true><<<fg, 64, 0, st>>>(a, fan_n, fan_idx);
+        k_fan<true><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec);
       else
-        k_fan<false><<<fg, 64, 0, st>>>(a, fan_n, fan_idx);
+        k_fan<false><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec);
       VACD_HIP(hipGetLastError());
     }
   }
@@ -1722,6 +1794,9 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
       fanb.ensure(sizeof(int) * (size_t)n * (1 + kFanMax));
       a.one.fan_n = fanb.as<int>();
       a.one.fan_idx = fanb.as<int>() + n;
+      vacd::DevBuf &recb = ctx->buf[SLOT_FANREC];
+      recb.ensure(sizeof(FanRec) * (size_t)n * kFanMax);
+      a.one.fan_rec = recb.as<FanRec>();
     }
     if (ctx->inside_grid && self->has_grid) {
       a.one.inside = self->inside.as<unsigned long long>();
@@ -1770,13 +1845,18 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
     farb.ensure((size_t)std::max<int64_t>(points, 1));
     const bool fan = ctx->fan_pairs && 3 * (depth / 2) <= 64;
     vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
-    if (fan) fanb.ensure(sizeof(int) * (size_t)std::max<int64_t>(points, 1) * (1 + kFanMax));
+    vacd::DevBuf &recb = ctx->buf[SLOT_FANREC];
+    if (fan) {
+      fanb.ensure(sizeof(int) * (size_t)std::max<int64_t>(points, 1) * (1 + kFanMax));
+      recb.ensure(sizeof(FanRec) * (size_t)std::max<int64_t>(points, 1) * kFanMax);
+    }
     for (int p = 0; p < nparts; ++p)
       if (!hp[p].ctris && parts[p].self->closed) {
         hp[p].far = farb.as<uint8_t>() + hp[p].point_begin;
         if (fan) {
           hp[p].fan_n = fanb.as<int>() + hp[p].point_begin;
           hp[p].fan_idx = fanb.as<int>() + points + hp[p].point_begin * kFanMax;
+          hp[p].fan_rec = recb.as<FanRec>() + hp[p].point_begin * kFanMax;
         }
         if (ctx->inside_grid && parts[p].self->has_grid) {
           hp[p].inside = parts[p].self->inside.as<unsigned long long>();
