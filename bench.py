@@ -111,6 +111,7 @@ def acd_wall(W, mode, proposal="exact"):
 
 # kernels of one default visibility pass whose HBM traffic is reported
 TRAFFIC_KERNELS = ("k_visibility<1, false, false, false, false, true>",
+                   "k_walk_front<false, false, 8>",
                    "k_walk_split<32, false, false, 0, false>",
                    "k_walk_split<64, false, false, 0, false>")
 
