@@ -3,11 +3,13 @@
 Times a step function under every configuration of a search space and
 keeps the fastest.  The knobs are the context options of the engine:
 
-* ``lanes_per_pair`` (VACD_OPT_LANES_PER_PAIR): lanes that share one BVH
-  walk of a pair the fan pass left open (1, 2, 4, 8, 16);
+* ``lanes_per_pair`` (VACD_OPT_LANES_PER_PAIR): lanes that share one
+  depth-first BVH walk of a pair the fan pass left open (1, 2, 4, 8, 16);
+* ``front_max`` (VACD_OPT_FRONT_MAX): the longest walk queue walked breadth
+  first (8 lanes per pair) instead of depth first (131072; 2^40 = always);
 * ``score_chunk`` (VACD_OPT_SCORE_CHUNK): edges per scoring block, i.e. the
-  edge tile a block streams through its 2048 planes (0 = sized for about
-  4096 blocks; 256 to 4096 otherwise).
+  edge tile a block streams through its 512 planes (0 = sized for about
+  8192 blocks; 64 or 256 otherwise).
 
 The traversal stack depth is not a knob: it is sized from the tree depth
 (32 or 64 entries), the smallest that cannot overflow.
@@ -21,15 +23,17 @@ from visacd_amd import native
 
 SPACE = {
     "lanes_per_pair": (1, 2, 4, 8, 16),
-    "score_chunk": (0, 256, 1024, 4096),
+    "front_max": (131072, 1 << 40),
+    "score_chunk": (0, 64, 256),
 }
 
 _OPTION = {
     "lanes_per_pair": native.VACD_OPT_LANES_PER_PAIR,
+    "front_max": native.VACD_OPT_FRONT_MAX,
     "score_chunk": native.VACD_OPT_SCORE_CHUNK,
 }
 
-DEFAULT = {"lanes_per_pair": 1, "score_chunk": 0}
+DEFAULT = {"lanes_per_pair": 1, "front_max": 131072, "score_chunk": 0}
 
 
 def apply(ctx, cfg: dict) -> None:
