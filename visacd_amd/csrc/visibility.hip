@@ -1367,7 +1367,7 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
 // the point's walked pairs walk their midpoint rays instead) or a walk
 // exceeds its node budget.  One wave per point, the 64 lanes splitting the walk
 // (split_root at level 6).  In a batch, far is indexed by the global point.
-template <bool BATCH>
+template <bool BATCH, bool REDO = false>  // REDO: only the points marked 3
 __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restrict__ far) {
   // a walk holds at most 32 entries: depth - 1 <= 32, or small-first order
   // (binary_stack)
@@ -1386,6 +1386,7 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
       P = a.parts[lo];
     }
     if (!P.far) continue;  // open mesh or cage mode: not needed
+    if (REDO && far[g] != 3) continue;
     const int64_t i = g - P.point_begin;
     Seg s;
     for (int c = 0; c < 3; ++c) s.p[c] = s.q[c] = P.pts[3 * i + c];
@@ -1396,6 +1397,98 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
         exact_ray_parity(P.snodes, P.stris, s, root, stk, P.scap, s.p, bad, degenerate);
     const uint64_t odd = __ballot(par & 1), mb = __ballot(bad || degenerate);
     if (lane == 0) far[g] = mb ? 2 : (uint8_t)(__popcll(odd) & 1);
+  }
+}
+
+// The same, breadth first: G lanes per point (the group), the point's
+// frontier of tree nodes in LDS (as in k_walk_front), so that a wave walks
+// 64 / G points at once; a point whose frontier overflows is marked 3 and
+// redone depth first by k_far_parity<.., true>.
+template <bool BATCH, int G>
+__global__ __launch_bounds__(kBlock) void k_far_front(VisArgs a, uint8_t *__restrict__ far) {
+  constexpr int kGroups = kBlock / G;
+  constexpr int kF = 16 * G;
+  __shared__ int front_all[kGroups][2][kF];
+  const int lane = threadIdx.x & 63;
+  const int slot = threadIdx.x / G;
+  const int g = lane & (G - 1);
+  const uint64_t gm = group_mask(lane, G);
+  const uint64_t below = gm & ((1ull << lane) - 1ull);
+  for (int64_t pt = (int64_t)blockIdx.x * kGroups + slot; pt < a.npoints;
+       pt += (int64_t)gridDim.x * kGroups) {
+    Part P = a.one;
+    if (BATCH) {
+      int lo = 0, hi = a.nparts - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.parts[mid].point_begin <= pt) lo = mid;
+        else hi = mid - 1;
+      }
+      P = a.parts[lo];
+    }
+    if (!P.far) continue;  // open mesh or cage mode: not needed (group-uniform)
+    const int64_t i = pt - P.point_begin;
+    Seg s;
+    for (int c = 0; c < 3; ++c) s.p[c] = s.q[c] = P.pts[3 * i + c];
+    make_seg(s);
+    int *cur = front_all[slot][0], *nxt = front_all[slot][1];
+    if (g == 0) cur[0] = 0;
+    wave_lds_sync();
+    int n = 1, cnt = 0;
+    int64_t visits = 0;
+    bool bad = false, degenerate = false, over = false;
+    while (n > 0) {
+      visits += n;
+      if (visits > P.scap) {
+        bad = true;
+        break;
+      }
+      int nn = 0;
+      for (int base = 0; base < n; base += G) {
+        const int idx = base + g;
+        bool w0 = false, w1 = false;
+        int c0 = 0, c1 = 0;
+        if (idx < n) {
+          float bx[12];
+          load_node(P.snodes, cur[idx], bx, c0, c1);
+          const bool h0 = ray_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+          const bool h1 = ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            const int ch = c ? c1 : c0;
+            if (!(c ? h1 : h0) || ch >= 0) continue;
+            const Tri tr = P.stris[~ch];
+            const float a3[3] = {tr.a.x, tr.a.y, tr.a.z}, b3[3] = {tr.b.x, tr.b.y, tr.b.z},
+                        c3[3] = {tr.c.x, tr.c.y, tr.c.z};
+            if (same3(s.p, a3) || same3(s.p, b3) || same3(s.p, c3)) continue;
+            int r = cross_fast(s, tr);
+            if (r == 2) r = ray_cross_point_exact(s.p, a3, b3, c3, degenerate);
+            cnt += r;
+          }
+          w0 = h0 && c0 >= 0;
+          w1 = h1 && c1 >= 0;
+        }
+        const uint64_t m0 = __ballot(w0) & gm, m1 = __ballot(w1) & gm;
+        const int cnt0 = __popcll(m0), tot = cnt0 + __popcll(m1);
+        if (nn + tot > kF) {
+          over = true;
+          break;
+        }
+        if (w0) nxt[nn + __popcll(m0 & below)] = c0;
+        if (w1) nxt[nn + cnt0 + __popcll(m1 & below)] = c1;
+        nn += tot;
+      }
+      if (over) break;
+      wave_lds_sync();
+      int *t = cur;
+      cur = nxt;
+      nxt = t;
+      n = nn;
+    }
+    const uint64_t odd = __ballot(cnt & 1) & gm, mb = __ballot(degenerate) & gm;
+    // 3: the frontier overflowed; k_far_parity<.., true> redoes the point
+    if (g == 0) far[pt] = over ? 3 : ((bad || mb) ? 2 : (uint8_t)(__popcll(odd) & 1));
+    wave_lds_sync();
   }
 }
 
@@ -1675,7 +1768,16 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     const unsigned fg = (unsigned)((a.npoints + 63) / 64);
     const unsigned pg = (unsigned)std::min<int64_t>(a.npoints, 16384);  // a wave per point
     vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
-    if (batch)
+    // breadth first, 8 lanes per point (a wave per point, the walk split
+    // by subtree, when the breadth-first walks are off)
+    const unsigned fpg = (unsigned)std::min<int64_t>((a.npoints + kBlock / 8 - 1) / (kBlock / 8), 4096);
+    if (ctx->front_lanes && batch) {
+      k_far_front<true, 8><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+      k_far_parity<true, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
+    } else if (ctx->front_lanes) {
+      k_far_front<false, 8><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+      k_far_parity<false, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
+    } else if (batch)
       k_far_parity<true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
     else
       k_far_parity<false><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
