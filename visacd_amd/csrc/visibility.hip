@@ -934,7 +934,7 @@ __device__ __forceinline__ uint64_t group_mask(int lane, int G) {
 constexpr int64_t kExactSplit = 2048;
 
 template <int STACK, bool COUNT, bool BATCH, int LV, bool EXACT>
-__global__ __launch_bounds__(kBlock) void k_walk_split(VisArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 4))) void k_walk_split(VisArgs a) {
   constexpr int G = 1 << LV;
   constexpr int kGroups = kBlock / G;
   __shared__ int stk_all[kWaves][STACK][64];
