@@ -1128,7 +1128,7 @@ __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
 }
 
 template <bool COUNT, bool BATCH, int G>
-__global__ __launch_bounds__(kBlock) void k_walk_front(VisArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_front(VisArgs a) {
   constexpr int kGroups = kBlock / G;
   constexpr int kFront = G >= 8 ? 16 * G : 128;  // frontier entries per group and buffer
   __shared__ int front_all[kGroups][2][kFront];
@@ -1292,7 +1292,10 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
   if (!P.fan_n) return;
   const int64_t i = g - P.point_begin;
   const float p[3] = {P.pts[3 * i], P.pts[3 * i + 1], P.pts[3 * i + 2]};
-  int stk[64];
+  // the walk stack in LDS, lane-interleaved (a private array would live in
+  // scratch memory)
+  __shared__ int stk_all[64][64];
+  int *stk = &stk_all[0][threadIdx.x];
   int node = 0, sp = 0, cnt = 0;
   bool done = false;
   int *out = fan_idx + (size_t)g * kFanMax;
@@ -1317,7 +1320,7 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
       } else if (next < 0) {
         next = id;
       } else if (sp < 64) {
-        stk[sp++] = id;
+        stk[(sp++) * 64] = id;
       } else {
         cnt = kFanMax + 1;  // stack exhausted: give up on this point's fan
       }
@@ -1327,7 +1330,7 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
     } else if (sp == 0) {
       done = true;
     } else {
-      node = stk[--sp];
+      node = stk[(--sp) * 64];
     }
   }
   // an unfinished walk or an overfull fan leaves the point to the full walk
