@@ -12,6 +12,14 @@
 // (three factors, remainder via fma).  The terms are summed exactly into a
 // nonoverlapping expansion (Grow-Expansion with zero elimination, Shewchuk
 // section 2.4); the sign of its largest component is the exact sign.
+//
+// Before the expansions, the exact forms try integer arithmetic: float32
+// values are integers m * 2^e (|m| < 2^24); when the nonzero inputs of one
+// predicate have exponents e within kGridWindow bits of each other, all of
+// them are integers on the grid 2^emin below 2^39, their differences below
+// 2^40, and a 3x3 determinant of differences (or the sum of two) is below
+// 2^124: computed exactly in signed 128-bit integers in registers.  Only
+// inputs of very different magnitude take the expansion path.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -99,6 +107,54 @@ __device__ __forceinline__ void exp_det3(Exp &e, const float *a, const float *b,
   }
 }
 
+// ------------------------------------------------------------ integer path
+constexpr int kGridWindow = 15;
+using i128 = __int128;
+
+// f[i] = m[i] * 2^emin with one emin for all; false when the nonzero values'
+// exponents span more than kGridWindow.
+template <int N>
+__device__ __forceinline__ bool int_grid(const float (&f)[N], int64_t (&m)[N]) {
+  int e[N];
+  int lo = 1 << 20, hi = -(1 << 20);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const unsigned b = __float_as_uint(f[i]);
+    const int be = (int)((b >> 23) & 0xffu);
+    const int mm = (int)(b & 0x7fffffu) | (be ? 0x800000 : 0);
+    m[i] = (b >> 31) ? -(int64_t)mm : (int64_t)mm;
+    e[i] = be ? be - 150 : -149;
+    if (mm) {
+      lo = e[i] < lo ? e[i] : lo;
+      hi = e[i] > hi ? e[i] : hi;
+    }
+  }
+  if (hi - lo > kGridWindow) return false;
+#pragma unroll
+  for (int i = 0; i < N; ++i) m[i] *= (int64_t)1 << (m[i] ? e[i] - lo : 0);
+  return true;
+}
+
+__device__ __forceinline__ int sign128(i128 v) { return v > 0 ? 1 : (v < 0 ? -1 : 0); }
+
+// det[a-d; b-d; c-d] on grid integers (a, b, c, d at m[0..2], m[3..5], ...).
+__device__ __forceinline__ i128 det3_int(const int64_t *m) {
+  const int64_t ax = m[0] - m[9], ay = m[1] - m[10], az = m[2] - m[11];
+  const int64_t bx = m[3] - m[9], by = m[4] - m[10], bz = m[5] - m[11];
+  const int64_t cx = m[6] - m[9], cy = m[7] - m[10], cz = m[8] - m[11];
+  return (i128)ax * ((i128)by * cz - (i128)bz * cy) + (i128)bx * ((i128)cy * az - (i128)cz * ay) +
+         (i128)cx * ((i128)ay * bz - (i128)az * by);
+}
+
+// (a_u - c_u)(b_v - c_v) - (a_v - c_v)(b_u - c_u) on grid integers
+// (a, b, c at m[0..1], m[2..3], m[4..5]).
+__device__ __forceinline__ i128 det2_int(const int64_t *m) {
+  return (i128)(m[0] - m[4]) * (m[3] - m[5]) - (i128)(m[1] - m[5]) * (m[2] - m[4]);
+}
+
+__device__ __forceinline__ int proj_u(int k) { return k == 0 ? 1 : (k == 1 ? 2 : 0); }
+__device__ __forceinline__ int proj_v(int k) { return k == 0 ? 2 : (k == 1 ? 0 : 1); }
+
 // ------------------------------------------------------------ float64 filters
 // det[a-d, b-d, c-d] (orient3d), float64, with its error bound.
 __device__ __forceinline__ double o3_fast(const float *a, const float *b, const float *c,
@@ -138,6 +194,11 @@ __device__ __noinline__ int orient3d_exact(const float *a, const float *b, const
   const double det = o3_fast(a, b, c, d, bound);
   if (det > bound) return 1;
   if (-det > bound) return -1;
+  {
+    const float f[12] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]};
+    int64_t m[12];
+    if (int_grid(f, m)) return sign128(det3_int(m));
+  }
   Exp e;
   e.n = 0;
   exp_det4(e, a, b, c, d, 1);
@@ -154,6 +215,16 @@ __device__ __noinline__ int orient3d_mid_exact(const float *a, const float *b, c
   const double bound = (b1 + b2) * 1.0000001 + 1e-15 * fabs(s);
   if (s > bound) return 1;
   if (-s > bound) return -1;
+  {
+    const float f[15] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2],
+                         p[0], p[1], p[2], q[0], q[1], q[2]};
+    int64_t m[15];
+    if (int_grid(f, m)) {
+      const i128 dp = det3_int(m);
+      m[9] = m[12], m[10] = m[13], m[11] = m[14];
+      return sign128(dp + det3_int(m));
+    }
+  }
   Exp e;
   e.n = 0;
   exp_det4(e, a, b, c, p, 1);
@@ -167,6 +238,12 @@ __device__ __noinline__ int orient2d_exact(const float *a, const float *b, const
   const double det = o2_fast(a, b, c, k, bound);
   if (det > bound) return 1;
   if (-det > bound) return -1;
+  {
+    const int u = proj_u(k), v = proj_v(k);
+    const float f[6] = {a[u], a[v], b[u], b[v], c[u], c[v]};
+    int64_t m[6];
+    if (int_grid(f, m)) return sign128(det2_int(m));
+  }
   Exp e;
   e.n = 0;
   exp_det3(e, a, b, c, k, 1);
@@ -183,6 +260,16 @@ __device__ __noinline__ int orient2d_mid_exact(const float *a, const float *b, c
   const double bound = (b1 + b2) * 1.0000001 + 1e-15 * fabs(s);
   if (s > bound) return 1;
   if (-s > bound) return -1;
+  {
+    const int u = proj_u(k), v = proj_v(k);
+    const float f[8] = {a[u], a[v], b[u], b[v], p[u], p[v], q[u], q[v]};
+    int64_t m[8];
+    if (int_grid(f, m)) {
+      const i128 dp = det2_int(m);
+      m[4] = m[6], m[5] = m[7];
+      return sign128(dp + det2_int(m));
+    }
+  }
   Exp e;
   e.n = 0;
   exp_det3(e, a, b, p, k, 1);

@@ -953,6 +953,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
   for (int64_t e = e_lo + (int64_t)blockIdx.x * kGroups + slot; e < e_hi;
        e += (int64_t)gridDim.x * kGroups) {
     const int64_t kg = list[e];
+    if (EXACT && kg < 0) continue;  // settled by the breadth-first exact walk
     const int64_t w = kg >> 6;
     const int b = (int)(kg & 63);
     const Part P = part_of_word<BATCH>(a, w);
@@ -1059,15 +1060,16 @@ __device__ __forceinline__ void wave_lds_sync() {
 // p's fan crossings met on the way in xpar/ninc/local_ok as in walk_any.
 // RAY true: crossings of the +x ray from the midpoint (plain tree), parity
 // in xpar.  Per-lane results; returns (group-uniform) 0 done, 1 contact,
-// 3 more visits than the tree has nodes, 5 frontier overflow.
-template <int G, int KF, bool RAY, bool COUNT>
+// 3 more visits than the tree has nodes, 5 frontier overflow.  EXACT: tests
+// the float64 filter cannot decide are decided exactly (counted in nex)
+// instead of setting unsure.
+template <int G, int KF, bool RAY, bool COUNT, bool EXACT = false>
 __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
                                           const Tri *__restrict__ tris,
                                           const float4 *__restrict__ grp, int cap, const Seg &s,
                                           int *cur, int *nxt, int g, uint64_t gm, uint64_t below,
                                           int &xpar, int &ninc, bool &local_ok, bool &unsure,
-                                          unsigned &nv, unsigned &nt) {
-  unsigned nex = 0;
+                                          unsigned &nv, unsigned &nt, unsigned &nex) {
   if (g == 0) cur[0] = 0;  // the root
   wave_lds_sync();
   int n = 1;
@@ -1104,8 +1106,8 @@ __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
           const int ch = c ? c1 : c0;
           if (!(c ? h1 : h0) || ch >= 0) continue;
           if (COUNT) ++nt;
-          if (RAY) xpar ^= cross_test<false>(s, tris[~ch], unsure, nex);
-          else hit |= contact_test<false>(s, tris[~ch], xpar, ninc, local_ok, unsure, nex) == 1;
+          if (RAY) xpar ^= cross_test<EXACT>(s, tris[~ch], unsure, nex);
+          else hit |= contact_test<EXACT>(s, tris[~ch], xpar, ninc, local_ok, unsure, nex) == 1;
         }
         w0 = h0 && c0 >= 0;
         w1 = h1 && c1 >= 0;
@@ -1127,8 +1129,13 @@ __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
   return 0;
 }
 
-template <bool COUNT, bool BATCH, int G>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_front(VisArgs a) {
+// EXACT false: the walk queue (only when it holds at most a.front_max
+// pairs).  EXACT true: the first kExactSplit entries of the deferred list,
+// every test decided; an entry this kernel settles is overwritten with -1
+// (k_walk_split skips it), one whose frontier overflows or that has a cage
+// is left to k_walk_split.
+template <bool COUNT, bool BATCH, int G, bool EXACT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 4))) void k_walk_front(VisArgs a) {
   constexpr int kGroups = kBlock / G;
   constexpr int kFront = G >= 8 ? 16 * G : 128;  // frontier entries per group and buffer
   __shared__ int front_all[kGroups][2][kFront];
@@ -1137,16 +1144,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   const int g = lane & (G - 1);
   const uint64_t gm = group_mask(lane, G);
   const uint64_t below = gm & ((1ull << lane) - 1ull);  // lanes of the group before this one
-  unsigned n_blocked = 0, n_bad = 0, nv = 0, nt = 0;
-  const int64_t len = (int64_t)*a.queue_len;
-  if (len > a.front_max) return;
+  unsigned n_blocked = 0, n_bad = 0, nv = 0, nt = 0, n_exact = 0;
+  int64_t *list = EXACT ? a.dlist : a.queue;
+  int64_t len = (int64_t)(EXACT ? *a.dlist_len : *a.queue_len);
+  if (EXACT && len > kExactSplit) len = kExactSplit;
+  if (!EXACT && len > a.front_max) return;
   int *fa = front_all[slot][0], *fb = front_all[slot][1];
   for (int64_t e = (int64_t)blockIdx.x * kGroups + slot; e < len;
        e += (int64_t)gridDim.x * kGroups) {
-    const int64_t kg = a.queue[e];
+    const int64_t kg = list[e];
     const int64_t w = kg >> 6;
     const int b = (int)(kg & 63);
     const Part P = part_of_word<BATCH>(a, w);
+    if (EXACT && P.ctris) continue;  // cage pairs: k_walk_split
     int64_t i, j;
     pair_of((w - P.word_begin) * 64 + b, P.n, i, j);
     Seg s;
@@ -1155,19 +1165,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     make_seg(s);
     int xpar = 0, ninc = 0;
     bool local_ok = true, unsure = false;
-    const int hs = front_walk<G, kFront, false, COUNT>(P.rnodes, P.rtris, P.rgroups, P.rcap, s, fa,
-                                                       fb, g, gm, below, xpar, ninc, local_ok,
-                                                       unsure, nv, nt);
+    const int hs = front_walk<G, kFront, false, COUNT, EXACT>(
+        P.rnodes, P.rtris, P.rgroups, P.rcap, s, fa, fb, g, gm, below, xpar, ninc, local_ok,
+        unsure, nv, nt, n_exact);
     uint64_t mu = __ballot(unsure) & gm;
     const uint64_t ml = __ballot(!local_ok) & gm, mx = __ballot(xpar & 1) & gm,
                    mn = __ballot(ninc > 0) & gm;
-    bool edge = false, defer = false;
+    bool edge = false, defer = false, left = false;
     if (hs == 3) {
       if (g == 0) ++n_bad;
     } else if (hs == 1) {
       if (g == 0) ++n_blocked;
     } else if (hs == 5 || mu) {
-      defer = true;  // undecided tests or a frontier overflow: the exact redo
+      // undecided tests or a frontier overflow: the exact redo (EXACT: the
+      // depth-first one)
+      if (EXACT) left = true;
+      else defer = true;
     } else if (!ml && P.far && mn && P.far[i] <= 1) {
       // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
       edge = (((__popcll(mx) & 1) ^ P.far[i]) & 1) == 0;
@@ -1176,15 +1189,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       int cross = 0, unused = 0;
       bool ok = true;
       unsure = false;
-      const int hr = front_walk<G, kFront, true, COUNT>(P.snodes, P.stris, nullptr, P.scap, s, fa,
-                                                        fb, g, gm, below, cross, unused, ok,
-                                                        unsure, nv, nt);
+      const int hr = front_walk<G, kFront, true, COUNT, EXACT>(
+          P.snodes, P.stris, nullptr, P.scap, s, fa, fb, g, gm, below, cross, unused, ok, unsure,
+          nv, nt, n_exact);
       mu = __ballot(unsure) & gm;
       const uint64_t mc = __ballot(cross & 1) & gm;
       if (hr == 3) {
         if (g == 0) ++n_bad;
       } else if (hr == 5 || mu) {
-        defer = true;
+        if (EXACT) left = true;
+        else defer = true;
       } else {
         edge = (__popcll(mc) & 1) == 0;
       }
@@ -1192,6 +1206,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (g == 0) {
       if (edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
       if (defer) a.dlist[atomicAdd(a.dlist_len, 1ull)] = kg;
+      if (EXACT && !left) list[e] = -1;  // settled here
     }
     // the next pair of this group reuses the frontier buffers: keep its
     // writes after every lane's last read of this pair's frontier
@@ -1200,12 +1215,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   for (int off = 32; off > 0; off >>= 1) {
     n_blocked += __shfl_down(n_blocked, off);
     n_bad += __shfl_down(n_bad, off);
+    n_exact += __shfl_down(n_exact, off);
     nv += __shfl_down(nv, off);
     nt += __shfl_down(nt, off);
   }
   if (lane == 0) {
     if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
     if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
+    if (EXACT && n_exact) atomicAdd(&a.stats[9], (unsigned long long)n_exact);
     if (COUNT) {
       atomicAdd(&a.stats[5], (unsigned long long)nv);
       atomicAdd(&a.stats[6], (unsigned long long)nt);
@@ -1648,11 +1665,11 @@ void launch_front(const Launch &L, const VisArgs &a) {
   constexpr unsigned kFrontGrid = 1024;
   switch (L.front) {
     case 0: break;
-    case 4: k_walk_front<N, B, 4><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
-    case 8: k_walk_front<N, B, 8><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
-    case 16: k_walk_front<N, B, 16><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
-    case 32: k_walk_front<N, B, 32><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
-    default: k_walk_front<N, B, 64><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+    case 4: k_walk_front<N, B, 4, false><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+    case 8: k_walk_front<N, B, 8, false><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+    case 16: k_walk_front<N, B, 16, false><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+    case 32: k_walk_front<N, B, 32, false><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
+    default: k_walk_front<N, B, 64, false><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
   }
 }
 
@@ -1832,6 +1849,10 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   // pair for the first kExactSplit, a lane per pair for the rest
   const unsigned wgrid = (unsigned)(kExactSplit / kWaves);
   const unsigned xgrid = (unsigned)std::min<int64_t>((nwords * 64 + kBlock - 1) / kBlock, 1024);
+  if (L.front) {  // breadth first (a wave per pair) before the depth-first kernels
+    if (batch) k_walk_front<false, true, 64, true><<<wgrid, kBlock, 0, st>>>(a);
+    else k_walk_front<false, false, 64, true><<<wgrid, kBlock, 0, st>>>(a);
+  }
   if (need2 <= 32) {
     if (batch) {
       k_walk_split<32, false, true, 6, true><<<wgrid, kBlock, 0, st>>>(a);
