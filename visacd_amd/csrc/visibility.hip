@@ -600,6 +600,57 @@ struct alignas(16) FanRec {
 };
 static_assert(sizeof(FanRec) == 112, "fan record is 112 bytes");
 
+// Two cones of directions at a point with a complete fan, one around an
+// axis a and one around -a, each free of the fan's triangles: the point
+// p + e u (small e > 0) then lies on one side of the surface for every u in
+// the cone, the side of the axis, so the fan test's crossing parity for
+// such a u is the axis's (xp for a, xm for -a).  A unit direction u is in
+// the first cone when u . a > tp, in the second when u . a < -tm; tp and tm
+// are the cosines of the cones' half-angles (the angular distance of +-a
+// from the fan's triangles) shrunk by 1e-3 rad, plus 1e-5: far more than
+// the float32 error of u . a (below 1e-6).  tp = tm = 2 disables a cone.
+// A pair whose direction falls in a cone skips its fan tests at that end.
+struct alignas(16) FanCone {
+  float ax, ay, az, tp;
+  float tm;
+  int xp, xm, pad;
+};
+
+// Crossing parity of the fan test for direction d from the point, from its
+// cones; -1 when d is in neither.
+__device__ __forceinline__ int cone_parity(const FanCone &c, float dx, float dy, float dz) {
+  const float dd = dx * dx + dy * dy + dz * dz;
+  if (!(dd > 1e-30f && dd < 1e30f)) return -1;
+  const float cs = (dx * c.ax + dy * c.ay + dz * c.az) * rsqrtf(dd);
+  if (cs > c.tp) return c.xp;
+  if (cs < -c.tm) return c.xm;
+  return -1;
+}
+
+// Angle between unit u and the cone {l1 e1 + l2 e2 : l1, l2 >= 0}.
+__device__ __forceinline__ double sector_angle(const double u[3], const double e1[3],
+                                               const double e2[3]) {
+  const double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+  const double l2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+  const double c1 = (u[0] * e1[0] + u[1] * e1[1] + u[2] * e1[2]) / l1;
+  const double c2 = (u[0] * e2[0] + u[1] * e2[1] + u[2] * e2[2]) / l2;
+  double best = acos(fmin(1.0, fmax(-1.0, fmax(c1, c2))));  // nearer edge ray
+  const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                       e1[0] * e2[1] - e1[1] * e2[0]};
+  const double ln = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (ln > 1e-12 * l1 * l2) {
+    const double un = (u[0] * n[0] + u[1] * n[1] + u[2] * n[2]) / ln;
+    const double w[3] = {u[0] - un * n[0] / ln, u[1] - un * n[1] / ln, u[2] - un * n[2] / ln};
+    // w inside the sector: on e2's side of e1 and on e1's side of e2
+    const double s1 = (e1[1] * w[2] - e1[2] * w[1]) * n[0] + (e1[2] * w[0] - e1[0] * w[2]) * n[1] +
+                      (e1[0] * w[1] - e1[1] * w[0]) * n[2];
+    const double s2 = (w[1] * e2[2] - w[2] * e2[1]) * n[0] + (w[2] * e2[0] - w[0] * e2[2]) * n[1] +
+                      (w[0] * e2[1] - w[1] * e2[0]) * n[2];
+    if (s1 >= 0.0 && s2 >= 0.0) best = fmin(best, asin(fmin(1.0, fabs(un))));
+  }
+  return best;
+}
+
 // contact_fast for a fan triangle of the pair's first point, from its
 // record: 1 the segment is an edge of the triangle, 0 no contact (xpar
 // toggled by a crossing at p + e d, ninc counted), 4 contact undecided,
@@ -631,6 +682,7 @@ struct Part {
                        // than kFanMax); NULL together with far
   const int *fan_idx;  // per point: kFanMax leaf-ordered triangle ids
   const FanRec *fan_rec;  // per point: kFanMax prepared fan triangles
+  const FanCone *fan_cone;  // per point: the fan's cones
   const unsigned long long *inside;  // inside grid of the closed self mesh (NULL: none)
   GridBox grid;
   const Node *snodes;
@@ -665,6 +717,8 @@ struct VisArgs {
                               // outwards; [15] pairs settled by the grid
   int64_t *queue;             // global pair ids (word * 64 + bit) left to walk
   unsigned long long *queue_len;
+  uint64_t *qbits;            // per ballot word: its pairs left to walk (compacted into
+                              // queue by a scan, in k order: no atomics in the fan pass)
   int64_t *dlist;             // global pair ids whose walks met an undecided test
   unsigned long long *dlist_len;
   int64_t front_max;          // walk queues up to this long go to k_walk_front
@@ -825,9 +879,14 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           const double dq[3] = {(double)s.q[0] - s.p[0], (double)s.q[1] - s.p[1],
                                 (double)s.q[2] - s.p[2]};
           if (fn > 0) {
-            int xp = 0, ni = 0, r = 0;
-            const FanRec *fr = P.fan_rec + (size_t)i * kFanMax;
-            for (int f = 0; f < fn && r == 0; ++f) r = fan_test(fr[f], s.q, dq, xp, ni);
+            int xp = cone_parity(P.fan_cone[i], s.q[0] - s.p[0], s.q[1] - s.p[1],
+                                 s.q[2] - s.p[2]),
+                ni = 0, r = 0;
+            if (xp < 0) {
+              xp = 0;
+              const FanRec *fr = P.fan_rec + (size_t)i * kFanMax;
+              for (int f = 0; f < fn && r == 0; ++f) r = fan_test(fr[f], s.q, dq, xp, ni);
+            }
             if (r == 1) {  // the segment is an edge of the mesh
               ++n_blocked;
               decided = true;
@@ -844,9 +903,14 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           const int fq = decided ? 0 : P.fan_n[j];
           if (fq > 0) {
             const double dp[3] = {-dq[0], -dq[1], -dq[2]};  // p - q (negation is exact)
-            int xq = 0, nq = 0, r = 0;
-            const FanRec *fr = P.fan_rec + (size_t)j * kFanMax;
-            for (int f = 0; f < fq && r == 0; ++f) r = fan_test(fr[f], s.p, dp, xq, nq);
+            int xq = cone_parity(P.fan_cone[j], s.p[0] - s.q[0], s.p[1] - s.q[1],
+                                 s.p[2] - s.q[2]),
+                nq = 0, r = 0;
+            if (xq < 0) {
+              xq = 0;
+              const FanRec *fr = P.fan_rec + (size_t)j * kFanMax;
+              for (int f = 0; f < fq && r == 0; ++f) r = fan_test(fr[f], s.p, dp, xq, nq);
+            }
             if (r == 1) {
               ++n_blocked;
               decided = true;
@@ -880,8 +944,10 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     const uint64_t eb = __ballot(edge);
     if (lane == 0) a.bits[w] = eb;
     append_lanes(defer, w * 64 + lane, a.dlist, a.dlist_len);
-    // append this wave's undecided pairs (global k) to the walk queue
-    if (QUEUE) append_lanes(queued, w * 64 + lane, a.queue, a.queue_len);
+    if (QUEUE) {  // this wave's undecided pairs, for the walk queue
+      const uint64_t qb = __ballot(queued);
+      if (lane == 0) a.qbits[w] = qb;
+    }
   }
   // per-wave totals, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
@@ -1293,7 +1359,8 @@ __device__ int exact_ray_parity(const Node *__restrict__ nodes, const Tri *__res
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
                                             int *__restrict__ fan_idx,
-                                            FanRec *__restrict__ fan_rec) {
+                                            FanRec *__restrict__ fan_rec,
+                                            FanCone *__restrict__ fan_cone) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (g >= a.npoints) return;
   Part P = a.one;
@@ -1378,6 +1445,52 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
     r.pad = 0;
     fan_rec[(size_t)g * kFanMax + f] = r;
   }
+  // the cones: axis = sum of the unit normals, each turned to the first's side
+  double ax = 0.0, ay = 0.0, az = 0.0, n0[3] = {0.0, 0.0, 0.0};
+  for (int f = 0; f < cnt; ++f) {
+    const FanRec &r = fan_rec[(size_t)g * kFanMax + f];
+    const double ln = sqrt(r.nx * r.nx + r.ny * r.ny + r.nz * r.nz);
+    if (!(ln > 0.0)) continue;
+    double u[3] = {r.nx / ln, r.ny / ln, r.nz / ln};
+    if (n0[0] == 0.0 && n0[1] == 0.0 && n0[2] == 0.0) n0[0] = u[0], n0[1] = u[1], n0[2] = u[2];
+    if (u[0] * n0[0] + u[1] * n0[1] + u[2] * n0[2] < 0.0) u[0] = -u[0], u[1] = -u[1], u[2] = -u[2];
+    ax += u[0], ay += u[1], az += u[2];
+  }
+  FanCone cone;
+  cone.tp = cone.tm = 2.0f;
+  cone.xp = cone.xm = 0;
+  cone.pad = 0;
+  const double la = sqrt(ax * ax + ay * ay + az * az);
+  const float fa[3] = {(float)(ax / la), (float)(ay / la), (float)(az / la)};
+  cone.ax = fa[0], cone.ay = fa[1], cone.az = fa[2];
+  if (la > 0.0 && cnt > 0) {
+    // the float axis as used by the pairs, unit in double
+    const double lf = sqrt((double)fa[0] * fa[0] + (double)fa[1] * fa[1] + (double)fa[2] * fa[2]);
+    const double up[3] = {fa[0] / lf, fa[1] / lf, fa[2] / lf};
+    const double um[3] = {-up[0], -up[1], -up[2]};
+    double thp = 10.0, thm = 10.0;
+    int xpp = 0, xpm = 0, np = 0, nm = 0, rp = 0, rm = 0;
+    for (int f = 0; f < cnt; ++f) {
+      const FanRec &r = fan_rec[(size_t)g * kFanMax + f];
+      const double e1[3] = {(double)r.v1[0] - p[0], (double)r.v1[1] - p[1], (double)r.v1[2] - p[2]};
+      const double e2[3] = {(double)r.v2[0] - p[0], (double)r.v2[1] - p[1], (double)r.v2[2] - p[2]};
+      thp = fmin(thp, sector_angle(up, e1, e2));
+      thm = fmin(thm, sector_angle(um, e1, e2));
+      // the axes' own crossing parities (p itself stands for the other
+      // point: it is no vertex of the fan but its apex)
+      if (rp == 0) rp = fan_test(r, p, up, xpp, np);
+      if (rm == 0) rm = fan_test(r, p, um, xpm, nm);
+    }
+    if (rp == 0 && thp > 2e-3 && thp < 3.1) {
+      cone.tp = (float)(cos(thp - 1e-3) + 1e-5);
+      cone.xp = xpp & 1;
+    }
+    if (rm == 0 && thm > 2e-3 && thm < 3.1) {
+      cone.tm = (float)(cos(thm - 1e-3) + 1e-5);
+      cone.xm = xpm & 1;
+    }
+  }
+  fan_cone[g] = cone;
 }
 
 // Crossing parity (0/1) of the +x ray from each point with the triangles
@@ -1547,6 +1660,7 @@ __global__ __launch_bounds__(256) void k_word_scan(const uint64_t *__restrict__ 
 __global__ __launch_bounds__(1024) void k_scan_blocks(unsigned long long *__restrict__ v,
                                                       int64_t m, int64_t *__restrict__ count,
                                                       unsigned long long *stats_edges) {
+  // count may be NULL (the walk queue: only its length in stats)
   __shared__ unsigned long long part[1024];
   const int64_t per = (m + 1023) / 1024;
   const int64_t beg = threadIdx.x * per;
@@ -1568,8 +1682,32 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(unsigned long long *__rest
     run += x;
   }
   if (threadIdx.x == 1023) {
-    *count = (int64_t)part[1023];
+    if (count) *count = (int64_t)part[1023];
     *stats_edges = part[1023];
+  }
+}
+
+// The walk queue from the fan pass's per-word bits: global pair ids
+// (word * 64 + bit) in k order, at offsets from k_word_scan/k_scan_blocks.
+__global__ __launch_bounds__(256) void k_queue_emit(VisArgs a,
+                                                    const unsigned long long *__restrict__ block_off) {
+  __shared__ unsigned long long sh[256];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
+  uint64_t w4[4];
+  unsigned long long c = 0;
+  for (int t = 0; t < 4; ++t) {
+    w4[t] = base + t < a.nwords ? a.qbits[base + t] : 0ull;
+    c += __popcll(w4[t]);
+  }
+  unsigned long long total;
+  unsigned long long out = block_excl_scan(c, sh, total) + block_off[blockIdx.x];
+  for (int t = 0; t < 4; ++t) {
+    uint64_t m = w4[t];
+    while (m) {
+      const int b = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      a.queue[out++] = (base + t) * 64 + b;
+    }
   }
 }
 
@@ -1655,6 +1793,8 @@ __global__ void k_fix_part_offsets(VisArgs a, const int64_t *__restrict__ total,
 
 struct Launch {
   unsigned grid, qgrid;
+  unsigned long long *qscan;  // block offsets of the queue compaction
+  int64_t nscan;              // its blocks
   int lv;     // log2 of the lanes per queued pair
   int front;  // lanes per pair of the breadth-first walks of short queues (0: off)
   hipStream_t st;
@@ -1690,6 +1830,10 @@ template <int S, bool C, bool N, bool W, bool B>
 void launch_main(bool q, const Launch &L, const VisArgs &a) {
   if (q && !C) {
     k_visibility<1, C, N, W, B, true><<<L.grid, kBlock, 0, L.st>>>(a);  // no walk, no stack
+    // compact the per-word bits into the queue
+    k_word_scan<<<(unsigned)L.nscan, 256, 0, L.st>>>(a.qbits, a.nwords, L.qscan);
+    k_scan_blocks<<<1, 1024, 0, L.st>>>(L.qscan, L.nscan, nullptr, a.queue_len);
+    k_queue_emit<<<(unsigned)L.nscan, 256, 0, L.st>>>(a, L.qscan);
     launch_front<N, B>(L, a);  // a short queue: breadth-first walks
     launch_split<S, N, B>(L, a);  // a long one: depth-first walks
   } else {
@@ -1726,6 +1870,7 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   P.fan_n = nullptr;
   P.fan_idx = nullptr;
   P.fan_rec = nullptr;
+  P.fan_cone = nullptr;
   P.inside = nullptr;
   P.grid = GridBox{};
   P.snodes = self->nodes.as<Node>();
@@ -1807,10 +1952,11 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
       int *fan_n = fanb.as<int>();
       int *fan_idx = fan_n + a.npoints;
       FanRec *fan_rec = ctx->buf[SLOT_FANREC].as<FanRec>();
+      FanCone *fan_cone = reinterpret_cast<FanCone *>(fan_rec + (size_t)a.npoints * kFanMax);
       if (batch)
-        k_fan<true><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec);
+        k_fan<true><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
       else
-        k_fan<false><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec);
+        k_fan<false><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
       VACD_HIP(hipGetLastError());
     }
   }
@@ -1833,10 +1979,14 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   const bool queue = any_far && ctx->fan_pairs && !any_cage && need4 <= 64;
   a.queue_len = a.stats + 7;
   a.queue = nullptr;
+  a.qbits = nullptr;
+  L.qscan = scan.as<unsigned long long>();  // free until the edge scan
+  L.nscan = nscan;
   if (queue) {
     vacd::DevBuf &qb = ctx->buf[SLOT_QUEUE];
-    qb.ensure(8 * (size_t)nwords * 64);
+    qb.ensure(8 * (size_t)nwords * 65);  // the queue, then the per-word bits
     a.queue = qb.as<int64_t>();
+    a.qbits = reinterpret_cast<uint64_t *>(a.queue + nwords * 64);
   }
   timed_begin(ctx, TIMED_VIS);
   if (need <= 32)
@@ -1921,8 +2071,9 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
       a.one.fan_n = fanb.as<int>();
       a.one.fan_idx = fanb.as<int>() + n;
       vacd::DevBuf &recb = ctx->buf[SLOT_FANREC];
-      recb.ensure(sizeof(FanRec) * (size_t)n * kFanMax);
+      recb.ensure((sizeof(FanRec) * kFanMax + sizeof(FanCone)) * (size_t)n);  // records, cones
       a.one.fan_rec = recb.as<FanRec>();
+      a.one.fan_cone = reinterpret_cast<const FanCone *>(a.one.fan_rec + (size_t)n * kFanMax);
     }
     if (ctx->inside_grid && self->has_grid) {
       a.one.inside = self->inside.as<unsigned long long>();
@@ -1974,7 +2125,8 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
     vacd::DevBuf &recb = ctx->buf[SLOT_FANREC];
     if (fan) {
       fanb.ensure(sizeof(int) * (size_t)std::max<int64_t>(points, 1) * (1 + kFanMax));
-      recb.ensure(sizeof(FanRec) * (size_t)std::max<int64_t>(points, 1) * kFanMax);
+      recb.ensure((sizeof(FanRec) * kFanMax + sizeof(FanCone)) *
+                  (size_t)std::max<int64_t>(points, 1));
     }
     for (int p = 0; p < nparts; ++p)
       if (!hp[p].ctris && parts[p].self->closed) {
@@ -1983,6 +2135,9 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
           hp[p].fan_n = fanb.as<int>() + hp[p].point_begin;
           hp[p].fan_idx = fanb.as<int>() + points + hp[p].point_begin * kFanMax;
           hp[p].fan_rec = recb.as<FanRec>() + hp[p].point_begin * kFanMax;
+          hp[p].fan_cone = reinterpret_cast<const FanCone *>(recb.as<FanRec>() +
+                                                             (size_t)points * kFanMax) +
+                           hp[p].point_begin;
         }
         if (ctx->inside_grid && parts[p].self->has_grid) {
           hp[p].inside = parts[p].self->inside.as<unsigned long long>();
