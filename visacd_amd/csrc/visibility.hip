@@ -949,18 +949,26 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
       if (lane == 0) a.qbits[w] = qb;
     }
   }
-  // per-wave totals, one atomic per wave
+  // per-block totals, one atomic per block and counter (atomics on one
+  // address from every wave of the grid serialize)
   for (int off = 32; off > 0; off >>= 1) {
     n_masked += __shfl_down(n_masked, off);
     n_blocked += __shfl_down(n_blocked, off);
     n_bad += __shfl_down(n_bad, off);
     n_walk += __shfl_down(n_walk, off);
   }
+  __shared__ unsigned wave_tot[kWaves][4];
   if (lane == 0) {
-    if (n_masked) atomicAdd(&a.stats[0], (unsigned long long)n_masked);
-    if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
-    if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
-    if (n_walk) atomicAdd(&a.stats[7], (unsigned long long)n_walk);  // queue mode: queue_len
+    wave_tot[wave][0] = n_masked, wave_tot[wave][1] = n_blocked;
+    wave_tot[wave][2] = n_bad, wave_tot[wave][3] = n_walk;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long t = 0;
+    for (int v = 0; v < kWaves; ++v) t += wave_tot[v][threadIdx.x];
+    // stats slots: masked, blocked, malformed, walked (queue mode: queue_len)
+    const int slot = threadIdx.x == 0 ? 0 : (threadIdx.x == 1 ? 1 : (threadIdx.x == 2 ? 4 : 7));
+    if (t) atomicAdd(&a.stats[slot], t);
   }
   if (COUNT) {
     unsigned long long v = nv, t = nt;
@@ -1508,6 +1516,7 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
   const int lane = threadIdx.x;
   int *stk = &stk_all[0][lane];
   for (int64_t g = blockIdx.x; g < a.npoints; g += gridDim.x) {
+    if (REDO && far[g] != 3) continue;  // (the part is checked below)
     Part P = a.one;
     if (BATCH) {
       int lo = 0, hi = a.nparts - 1;
@@ -1519,7 +1528,6 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
       P = a.parts[lo];
     }
     if (!P.far) continue;  // open mesh or cage mode: not needed
-    if (REDO && far[g] != 3) continue;
     const int64_t i = g - P.point_begin;
     Seg s;
     for (int c = 0; c < 3; ++c) s.p[c] = s.q[c] = P.pts[3 * i + c];
@@ -1537,7 +1545,7 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
 // frontier of tree nodes in LDS (as in k_walk_front), so that a wave walks
 // 64 / G points at once; a point whose frontier overflows is marked 3 and
 // redone depth first by k_far_parity<.., true>.
-template <bool BATCH, int G>
+template <bool BATCH, int G, bool AGAIN = false>  // AGAIN: only the points marked 3
 __global__ __launch_bounds__(kBlock) void k_far_front(VisArgs a, uint8_t *__restrict__ far) {
   constexpr int kGroups = kBlock / G;
   constexpr int kF = 16 * G;
@@ -1549,6 +1557,7 @@ __global__ __launch_bounds__(kBlock) void k_far_front(VisArgs a, uint8_t *__rest
   const uint64_t below = gm & ((1ull << lane) - 1ull);
   for (int64_t pt = (int64_t)blockIdx.x * kGroups + slot; pt < a.npoints;
        pt += (int64_t)gridDim.x * kGroups) {
+    if (AGAIN && far[pt] != 3) continue;  // (the part is checked below)
     Part P = a.one;
     if (BATCH) {
       int lo = 0, hi = a.nparts - 1;
@@ -1914,7 +1923,8 @@ int binary_stack(int depth, int64_t leaves, bool small_first) {
 
 // Enqueues far parity, the main kernel, the exact redo and the compaction for
 // a prepared VisArgs (a.nwords > 0).
-void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int depth, int stack2,
+void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int64_t max_tris,
+         int depth, int stack2,
          uint32_t *d_pairs, int64_t cap, int64_t *d_count, int64_t *d_part_off) {
   hipStream_t st = ctx->stream;
   const int64_t nwords = a.nwords;
@@ -1935,12 +1945,25 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
     // breadth first, 8 lanes per point (a wave per point, the walk split
     // by subtree, when the breadth-first walks are off)
-    const unsigned fpg = (unsigned)std::min<int64_t>((a.npoints + kBlock / 8 - 1) / (kBlock / 8), 4096);
+    // A ray across a mesh of T triangles meets about T^(2/3) boxes at the
+    // deeper levels: meshes above 131,072 triangles take 16 lanes per point
+    // (a frontier of 256 nodes) so that fewer points overflow to the
+    // depth-first redo.
+    const int fl = max_tris > 131072 ? 16 : 8;
+    const unsigned fpg =
+        (unsigned)std::min<int64_t>((a.npoints + kBlock / fl - 1) / (kBlock / fl), 4096);
+    // points whose frontier overflows: again with a wave per point (a
+    // frontier of 1024 nodes), then depth first
+    const unsigned fpg64 = (unsigned)std::min<int64_t>((a.npoints + kWaves - 1) / kWaves, 4096);
     if (ctx->front_lanes && batch) {
-      k_far_front<true, 8><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+      if (fl == 16) k_far_front<true, 16><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+      else k_far_front<true, 8><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+      k_far_front<true, 64, true><<<fpg64, kBlock, 0, st>>>(a, farb.as<uint8_t>());
       k_far_parity<true, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
     } else if (ctx->front_lanes) {
-      k_far_front<false, 8><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+      if (fl == 16) k_far_front<false, 16><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+      else k_far_front<false, 8><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+      k_far_front<false, 64, true><<<fpg64, kBlock, 0, st>>>(a, farb.as<uint8_t>());
       k_far_parity<false, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
     } else if (batch)
       k_far_parity<true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
@@ -2083,7 +2106,8 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
   const int stack2 = std::max({binary_stack(self->depth, self->n_tris, self->small_first),
                                binary_stack(self->depth_refs, self->n_refs, self->small_first_refs),
                                cage ? binary_stack(cage->depth, cage->n_tris, cage->small_first) : 0});
-  run(ctx, a, false, cage != nullptr, any_far, depth, stack2, d_pairs, cap, d_count, nullptr);
+  run(ctx, a, false, cage != nullptr, any_far, self->n_tris, depth, stack2, d_pairs, cap, d_count,
+      nullptr);
 }
 
 void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, uint32_t *d_pairs,
@@ -2093,6 +2117,7 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
   std::vector<Part> hp((size_t)nparts);
   int64_t words = 0, points = 0, segs = 0;
   int depth = 0, stack2 = 0;
+  int64_t max_tris = 0;
   bool any_cage = false, any_far = false;
   for (int p = 0; p < nparts; ++p) {
     const vacd_part &q = parts[p];
@@ -2114,6 +2139,7 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
                        q.cage ? binary_stack(q.cage->depth, q.cage->n_tris, q.cage->small_first) : 0});
     any_cage |= q.cage != nullptr;
     any_far |= !q.cage && q.self->closed;
+    max_tris = std::max(max_tris, q.self->n_tris);
     hp[p] = P;
   }
   ctx->last_segments = segs;
@@ -2164,7 +2190,7 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
     fill_u64(ctx, d_part_off, nparts + 1, 0ull);
     return;
   }
-  run(ctx, a, true, any_cage, any_far, depth, stack2, d_pairs, cap, d_count, d_part_off);
+  run(ctx, a, true, any_cage, any_far, max_tris, depth, stack2, d_pairs, cap, d_count, d_part_off);
 }
 
 }  // namespace vacd
