@@ -1943,8 +1943,25 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     const unsigned fg = (unsigned)((a.npoints + 63) / 64);
     const unsigned pg = (unsigned)std::min<int64_t>(a.npoints, 16384);  // a wave per point
     vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
-    // breadth first, 8 lanes per point (a wave per point, the walk split
-    // by subtree, when the breadth-first walks are off)
+    // The fans (independent of the far parity) on the side stream, forked
+    // here and joined before the fan pass.
+    if (ctx->fan_pairs) {
+      vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
+      int *fan_n = fanb.as<int>();
+      int *fan_idx = fan_n + a.npoints;
+      FanRec *fan_rec = ctx->buf[SLOT_FANREC].as<FanRec>();
+      FanCone *fan_cone = reinterpret_cast<FanCone *>(fan_rec + (size_t)a.npoints * kFanMax);
+      VACD_HIP(hipEventRecord(ctx->fork, st));
+      VACD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+      if (batch)
+        k_fan<true><<<fg, 64, 0, ctx->side>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
+      else
+        k_fan<false><<<fg, 64, 0, ctx->side>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
+      VACD_HIP(hipGetLastError());
+      VACD_HIP(hipEventRecord(ctx->join, ctx->side));
+    }
+    // far parity breadth first, 8 lanes per point (a wave per point, the
+    // walk split by subtree, when the breadth-first walks are off)
     // A ray across a mesh of T triangles meets about T^(2/3) boxes at the
     // deeper levels: meshes above 131,072 triangles take 16 lanes per point
     // (a frontier of 256 nodes) so that fewer points overflow to the
@@ -1970,18 +1987,7 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     else
       k_far_parity<false><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
     VACD_HIP(hipGetLastError());
-    if (ctx->fan_pairs) {
-      vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
-      int *fan_n = fanb.as<int>();
-      int *fan_idx = fan_n + a.npoints;
-      FanRec *fan_rec = ctx->buf[SLOT_FANREC].as<FanRec>();
-      FanCone *fan_cone = reinterpret_cast<FanCone *>(fan_rec + (size_t)a.npoints * kFanMax);
-      if (batch)
-        k_fan<true><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
-      else
-        k_fan<false><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
-      VACD_HIP(hipGetLastError());
-    }
+    if (ctx->fan_pairs) VACD_HIP(hipStreamWaitEvent(st, ctx->join, 0));  // the fan query
   }
   // stack entries a walk can hold: binary, stack2 (see binary_stack);
   // 4-wide, up to three per visited level (every other binary level)
