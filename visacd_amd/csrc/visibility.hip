@@ -627,14 +627,15 @@ __device__ __forceinline__ int cone_parity(const FanCone &c, float dx, float dy,
   return -1;
 }
 
-// Angle between unit u and the cone {l1 e1 + l2 e2 : l1, l2 >= 0}.
-__device__ __forceinline__ double sector_angle(const double u[3], const double e1[3],
-                                               const double e2[3]) {
+// Cosine of the angle between unit u and the cone {l1 e1 + l2 e2 : l1, l2 >= 0}
+// (the nearer edge ray's, or the plane's when u projects inside the cone).
+__device__ __forceinline__ double sector_cos(const double u[3], const double e1[3],
+                                             const double e2[3]) {
   const double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
   const double l2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
   const double c1 = (u[0] * e1[0] + u[1] * e1[1] + u[2] * e1[2]) / l1;
   const double c2 = (u[0] * e2[0] + u[1] * e2[1] + u[2] * e2[2]) / l2;
-  double best = acos(fmin(1.0, fmax(-1.0, fmax(c1, c2))));  // nearer edge ray
+  double best = fmin(1.0, fmax(-1.0, fmax(c1, c2)));  // nearer edge ray
   const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
                        e1[0] * e2[1] - e1[1] * e2[0]};
   const double ln = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
@@ -646,9 +647,17 @@ __device__ __forceinline__ double sector_angle(const double u[3], const double e
                       (e1[0] * w[1] - e1[1] * w[0]) * n[2];
     const double s2 = (w[1] * e2[2] - w[2] * e2[1]) * n[0] + (w[2] * e2[0] - w[0] * e2[2]) * n[1] +
                       (w[0] * e2[1] - w[1] * e2[0]) * n[2];
-    if (s1 >= 0.0 && s2 >= 0.0) best = fmin(best, asin(fmin(1.0, fabs(un))));
+    if (s1 >= 0.0 && s2 >= 0.0) best = fmax(best, sqrt(fmax(0.0, 1.0 - un * un)));
   }
   return best;
+}
+
+// cos(theta - 1e-3) + 1e-5 for a cone whose half-angle theta has cosine cm
+// (2: no cone when theta <= 2e-3 or theta >= 3.1).
+__device__ __forceinline__ float cone_threshold(double cm) {
+  if (!(cm < 0.999998 && cm > -0.99913)) return 2.0f;  // cos(2e-3), cos(3.1)
+  const double sm = sqrt(fmax(0.0, 1.0 - cm * cm));
+  return (float)(cm * 0.9999995000000417 + sm * 9.999998333333417e-4 + 1e-5);  // cos, sin of 1e-3
 }
 
 // contact_fast for a fan triangle of the pair's first point, from its
@@ -1476,25 +1485,25 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
     const double lf = sqrt((double)fa[0] * fa[0] + (double)fa[1] * fa[1] + (double)fa[2] * fa[2]);
     const double up[3] = {fa[0] / lf, fa[1] / lf, fa[2] / lf};
     const double um[3] = {-up[0], -up[1], -up[2]};
-    double thp = 10.0, thm = 10.0;
+    double cp = -1.0, cm = -1.0;  // cosines of the cones' half-angles
     int xpp = 0, xpm = 0, np = 0, nm = 0, rp = 0, rm = 0;
     for (int f = 0; f < cnt; ++f) {
       const FanRec &r = fan_rec[(size_t)g * kFanMax + f];
       const double e1[3] = {(double)r.v1[0] - p[0], (double)r.v1[1] - p[1], (double)r.v1[2] - p[2]};
       const double e2[3] = {(double)r.v2[0] - p[0], (double)r.v2[1] - p[1], (double)r.v2[2] - p[2]};
-      thp = fmin(thp, sector_angle(up, e1, e2));
-      thm = fmin(thm, sector_angle(um, e1, e2));
+      cp = fmax(cp, sector_cos(up, e1, e2));
+      cm = fmax(cm, sector_cos(um, e1, e2));
       // the axes' own crossing parities (p itself stands for the other
       // point: it is no vertex of the fan but its apex)
       if (rp == 0) rp = fan_test(r, p, up, xpp, np);
       if (rm == 0) rm = fan_test(r, p, um, xpm, nm);
     }
-    if (rp == 0 && thp > 2e-3 && thp < 3.1) {
-      cone.tp = (float)(cos(thp - 1e-3) + 1e-5);
+    if (rp == 0) {
+      cone.tp = cone_threshold(cp);
       cone.xp = xpp & 1;
     }
-    if (rm == 0 && thm > 2e-3 && thm < 3.1) {
-      cone.tm = (float)(cos(thm - 1e-3) + 1e-5);
+    if (rm == 0) {
+      cone.tm = cone_threshold(cm);
       cone.xm = xpm & 1;
     }
   }
