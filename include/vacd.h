@@ -94,13 +94,7 @@ int vacd_ctx_destroy(vacd_ctx *ctx);
 const char *vacd_last_error(const vacd_ctx *ctx);
 /* The hipStream_t the context enqueues on (returned as void*). */
 int vacd_ctx_stream(vacd_ctx *ctx, void **stream);
-/* Enqueue on a caller-owned hipStream_t instead (NULL restores the own one).
- * A visibility call also runs its fan query on a second stream of the
- * context, forked from the context stream by an event and joined back to it
- * before the fan pass: every kernel of the call stays ordered after what
- * the context stream held and before what it is given next, and the fork
- * and join are captured along when the context stream is being captured
- * into a HIP graph. */
+/* Enqueue on a caller-owned hipStream_t instead (NULL restores the own one). */
 int vacd_ctx_set_stream(vacd_ctx *ctx, void *stream);
 int vacd_sync(vacd_ctx *ctx);
 int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value);

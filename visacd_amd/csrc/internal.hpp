@@ -144,10 +144,6 @@ struct vacd_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
-  // a second stream for work independent of the stream's own (the fan
-  // query beside the far parity), forked from and joined to it by events
-  hipStream_t side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
   std::string last_error;
   vacd::DevBuf buf[SLOT_COUNT_MAX];
   int64_t last_segments = 0;

@@ -69,9 +69,6 @@ int vacd_ctx_create(int device, vacd_ctx **out) {
     VACD_HIP(hipSetDevice(device));
     VACD_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
     ctx->stream = ctx->own_stream;
-    VACD_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
-    VACD_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
-    VACD_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
   });
   if (rc != VACD_OK) {
     delete ctx;
@@ -95,10 +92,6 @@ int vacd_ctx_destroy(vacd_ctx *ctx) {
     if (ctx->pinned[k]) (void)hipHostFree(ctx->pinned[k]);
     if (ctx->pinned_free[k]) (void)hipEventDestroy(ctx->pinned_free[k]);
   }
-  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
-  if (ctx->fork) (void)hipEventDestroy(ctx->fork);
-  if (ctx->join) (void)hipEventDestroy(ctx->join);
-  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return VACD_OK;

@@ -1373,13 +1373,14 @@ __device__ int exact_ray_parity(const Node *__restrict__ nodes, const Tri *__res
 // Fan of each point: the leaf-ordered ids of the triangles that have the
 // point as a vertex (bitwise-equal coordinates), found by a point query on
 // the 4-wide BVH; -1 when there are more than kFanMax.
+// One point g per lane; stk: the lane's walk stack in LDS, 64 entries at a
+// stride of 64 ints (lane-interleaved: a private array would live in
+// scratch memory).
 template <bool BATCH>
-__global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
-                                            int *__restrict__ fan_idx,
-                                            FanRec *__restrict__ fan_rec,
-                                            FanCone *__restrict__ fan_cone) {
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= a.npoints) return;
+__device__ __forceinline__ void fan_point(const VisArgs &a, int64_t g, int *stk,
+                                          int *__restrict__ fan_n, int *__restrict__ fan_idx,
+                                          FanRec *__restrict__ fan_rec,
+                                          FanCone *__restrict__ fan_cone) {
   Part P = a.one;
   if (BATCH) {
     int lo = 0, hi = a.nparts - 1;
@@ -1393,10 +1394,6 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
   if (!P.fan_n) return;
   const int64_t i = g - P.point_begin;
   const float p[3] = {P.pts[3 * i], P.pts[3 * i + 1], P.pts[3 * i + 2]};
-  // the walk stack in LDS, lane-interleaved (a private array would live in
-  // scratch memory)
-  __shared__ int stk_all[64][64];
-  int *stk = &stk_all[0][threadIdx.x];
   int node = 0, sp = 0, cnt = 0;
   bool done = false;
   int *out = fan_idx + (size_t)g * kFanMax;
@@ -1510,6 +1507,16 @@ __global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
   fan_cone[g] = cone;
 }
 
+template <bool BATCH>
+__global__ __launch_bounds__(64) void k_fan(VisArgs a, int *__restrict__ fan_n,
+                                            int *__restrict__ fan_idx,
+                                            FanRec *__restrict__ fan_rec,
+                                            FanCone *__restrict__ fan_cone) {
+  __shared__ int stk_all[64][64];
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g < a.npoints) fan_point<BATCH>(a, g, &stk_all[0][threadIdx.x], fan_n, fan_idx, fan_rec, fan_cone);
+}
+
 // Crossing parity (0/1) of the +x ray from each point with the triangles
 // that do not have the point as a vertex; tests the filter leaves open are
 // decided exactly.  2 when a test is degenerate (the ray meets a triangle's
@@ -1554,18 +1561,20 @@ __global__ __launch_bounds__(64) void k_far_parity(VisArgs a, uint8_t *__restric
 // frontier of tree nodes in LDS (as in k_walk_front), so that a wave walks
 // 64 / G points at once; a point whose frontier overflows is marked 3 and
 // redone depth first by k_far_parity<.., true>.
-template <bool BATCH, int G, bool AGAIN = false>  // AGAIN: only the points marked 3
-__global__ __launch_bounds__(kBlock) void k_far_front(VisArgs a, uint8_t *__restrict__ far) {
-  constexpr int kGroups = kBlock / G;
+// The walks of block `block` of `nblocks` blocks of NT threads; lds: the
+// block's frontiers, (NT / G) * 2 * 16 * G ints.
+template <bool BATCH, int G, bool AGAIN, int NT>
+__device__ __forceinline__ void far_front_block(const VisArgs &a, uint8_t *__restrict__ far,
+                                                int64_t block, int64_t nblocks, int *lds) {
+  constexpr int kGroups = NT / G;
   constexpr int kF = 16 * G;
-  __shared__ int front_all[kGroups][2][kF];
+  int(*front_all)[2][kF] = reinterpret_cast<int(*)[2][kF]>(lds);
   const int lane = threadIdx.x & 63;
   const int slot = threadIdx.x / G;
   const int g = lane & (G - 1);
   const uint64_t gm = group_mask(lane, G);
   const uint64_t below = gm & ((1ull << lane) - 1ull);
-  for (int64_t pt = (int64_t)blockIdx.x * kGroups + slot; pt < a.npoints;
-       pt += (int64_t)gridDim.x * kGroups) {
+  for (int64_t pt = block * kGroups + slot; pt < a.npoints; pt += nblocks * kGroups) {
     if (AGAIN && far[pt] != 3) continue;  // (the part is checked below)
     Part P = a.one;
     if (BATCH) {
@@ -1640,6 +1649,35 @@ __global__ __launch_bounds__(kBlock) void k_far_front(VisArgs a, uint8_t *__rest
     // 3: the frontier overflowed; k_far_parity<.., true> redoes the point
     if (g == 0) far[pt] = over ? 3 : ((bad || mb) ? 2 : (uint8_t)(__popcll(odd) & 1));
     wave_lds_sync();
+  }
+}
+
+template <bool BATCH, int G, bool AGAIN = false>  // AGAIN: only the points marked 3
+__global__ __launch_bounds__(kBlock) void k_far_front(VisArgs a, uint8_t *__restrict__ far) {
+  __shared__ int lds[(kBlock / G) * 2 * 16 * G];
+  far_front_block<BATCH, G, AGAIN, kBlock>(a, far, blockIdx.x, gridDim.x, lds);
+}
+
+// The fan query and the first far-parity walks in one launch (they are
+// independent): blocks below fan_blocks take 64 points' fans, the others
+// the far walks of 8 (G = 8) or 4 (G = 16) points each.  One launch instead
+// of two back to back; the fan blocks come first so that their longer
+// walks start at once.
+template <bool BATCH, int G>
+__global__ __launch_bounds__(64) void k_far_fan(VisArgs a, uint8_t *__restrict__ far,
+                                                unsigned fan_blocks, int *__restrict__ fan_n,
+                                                int *__restrict__ fan_idx,
+                                                FanRec *__restrict__ fan_rec,
+                                                FanCone *__restrict__ fan_cone) {
+  // the fan query's stacks (64 x 64) or the far walks' frontiers (2048)
+  __shared__ int lds[64 * 64];
+  static_assert((64 / G) * 2 * 16 * G <= 64 * 64, "far frontiers fit the stacks' LDS");
+  if (blockIdx.x < fan_blocks) {
+    const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (g < a.npoints) fan_point<BATCH>(a, g, lds + threadIdx.x, fan_n, fan_idx, fan_rec, fan_cone);
+  } else {
+    far_front_block<BATCH, G, false, 64>(a, far, blockIdx.x - fan_blocks,
+                                         gridDim.x - fan_blocks, lds);
   }
 }
 
@@ -1952,51 +1990,54 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     const unsigned fg = (unsigned)((a.npoints + 63) / 64);
     const unsigned pg = (unsigned)std::min<int64_t>(a.npoints, 16384);  // a wave per point
     vacd::DevBuf &farb = ctx->buf[SLOT_FAR];
-    // The fans (independent of the far parity) on the side stream, forked
-    // here and joined before the fan pass.
-    if (ctx->fan_pairs) {
-      vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
-      int *fan_n = fanb.as<int>();
-      int *fan_idx = fan_n + a.npoints;
-      FanRec *fan_rec = ctx->buf[SLOT_FANREC].as<FanRec>();
-      FanCone *fan_cone = reinterpret_cast<FanCone *>(fan_rec + (size_t)a.npoints * kFanMax);
-      VACD_HIP(hipEventRecord(ctx->fork, st));
-      VACD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
-      if (batch)
-        k_fan<true><<<fg, 64, 0, ctx->side>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
-      else
-        k_fan<false><<<fg, 64, 0, ctx->side>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
-      VACD_HIP(hipGetLastError());
-      VACD_HIP(hipEventRecord(ctx->join, ctx->side));
-    }
-    // far parity breadth first, 8 lanes per point (a wave per point, the
-    // walk split by subtree, when the breadth-first walks are off)
+    vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
+    int *fan_n = fanb.as<int>();
+    int *fan_idx = fan_n + a.npoints;
+    FanRec *fan_rec = ctx->buf[SLOT_FANREC].as<FanRec>();
+    FanCone *fan_cone = reinterpret_cast<FanCone *>(fan_rec + (size_t)a.npoints * kFanMax);
+    // far parity breadth first, 8 lanes per point, with the fan query in
+    // the same launch (a wave per point, the walk split by subtree, and the
+    // fan query on its own, when the breadth-first walks are off).
     // A ray across a mesh of T triangles meets about T^(2/3) boxes at the
     // deeper levels: meshes above 131,072 triangles take 16 lanes per point
     // (a frontier of 256 nodes) so that fewer points overflow to the
     // depth-first redo.
     const int fl = max_tris > 131072 ? 16 : 8;
-    const unsigned fpg =
-        (unsigned)std::min<int64_t>((a.npoints + kBlock / fl - 1) / (kBlock / fl), 4096);
+    const unsigned far_blocks =
+        (unsigned)std::min<int64_t>((a.npoints + 64 / fl - 1) / (64 / fl), 16384);
+    const unsigned fan_blocks = ctx->fan_pairs ? fg : 0;
     // points whose frontier overflows: again with a wave per point (a
     // frontier of 1024 nodes), then depth first
     const unsigned fpg64 = (unsigned)std::min<int64_t>((a.npoints + kWaves - 1) / kWaves, 4096);
-    if (ctx->front_lanes && batch) {
-      if (fl == 16) k_far_front<true, 16><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
-      else k_far_front<true, 8><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
-      k_far_front<true, 64, true><<<fpg64, kBlock, 0, st>>>(a, farb.as<uint8_t>());
-      k_far_parity<true, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
-    } else if (ctx->front_lanes) {
-      if (fl == 16) k_far_front<false, 16><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
-      else k_far_front<false, 8><<<fpg, kBlock, 0, st>>>(a, farb.as<uint8_t>());
-      k_far_front<false, 64, true><<<fpg64, kBlock, 0, st>>>(a, farb.as<uint8_t>());
-      k_far_parity<false, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
+    if (ctx->front_lanes) {
+      const unsigned grid = fan_blocks + far_blocks;  // > 0: a closed part has points
+      if (batch && fl == 16)
+        k_far_fan<true, 16><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
+      else if (batch)
+        k_far_fan<true, 8><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
+      else if (fl == 16)
+        k_far_fan<false, 16><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
+      else
+        k_far_fan<false, 8><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
+      if (batch) {
+        k_far_front<true, 64, true><<<fpg64, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+        k_far_parity<true, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
+      } else {
+        k_far_front<false, 64, true><<<fpg64, kBlock, 0, st>>>(a, farb.as<uint8_t>());
+        k_far_parity<false, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
+      }
     } else if (batch)
       k_far_parity<true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
     else
       k_far_parity<false><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
     VACD_HIP(hipGetLastError());
-    if (ctx->fan_pairs) VACD_HIP(hipStreamWaitEvent(st, ctx->join, 0));  // the fan query
+    if (ctx->fan_pairs && !ctx->front_lanes) {
+      if (batch)
+        k_fan<true><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
+      else
+        k_fan<false><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
+      VACD_HIP(hipGetLastError());
+    }
   }
   // stack entries a walk can hold: binary, stack2 (see binary_stack);
   // 4-wide, up to three per visited level (every other binary level)
