@@ -341,6 +341,14 @@ def test_visibility_icosphere_c2_sampled(ctx):
     det = oracle.visibility_sample(W.points, W.vertices, W.triangles, ks)
     is_edge = np.isin(ks, k)
     np.testing.assert_array_equal(is_edge, det == 1)
+    # the pairs redone exactly (degenerate on this symmetric mesh): every one
+    # is listed, as a valid pair id, and decided as the oracle decides it
+    dk = ctx.visibility_deferred()
+    _print("c2 deferred", len(dk))
+    assert len(dk) == st["exact_calls"] > 0
+    assert ((dk >= 0) & (dk < W.n_segments)).all()
+    det = oracle.visibility_sample(W.points, W.vertices, W.triangles, dk)
+    np.testing.assert_array_equal(np.isin(dk, k), det == 1)
 
 
 # ----------------------------------------------------------------- scoring

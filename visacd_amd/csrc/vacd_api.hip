@@ -515,6 +515,8 @@ int vacd_visibility_deferred(vacd_ctx *ctx, int64_t *pair_ids, int64_t cap, int6
     const int64_t n = (int64_t)len < cap ? (int64_t)len : cap;
     if (n > 0)
       VACD_HIP(hipMemcpy(pair_ids, ctx->buf[SLOT_DEFER].ptr, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    // bit 62 marks entries the breadth-first exact walk settled
+    for (int64_t t = 0; t < n; ++t) pair_ids[t] &= ~(int64_t(1) << 62);
   });
 }
 

@@ -1015,6 +1015,10 @@ __device__ __forceinline__ uint64_t group_mask(int lane, int G) {
 }
 
 constexpr int64_t kExactSplit = 2048;
+// Deferred-list entries the breadth-first exact walk settled carry this bit
+// (the depth-first exact kernels skip them; the list still names every pair
+// redone exactly).
+constexpr int64_t kSettled = int64_t(1) << 62;
 
 template <int STACK, bool COUNT, bool BATCH, int LV, bool EXACT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 4))) void k_walk_split(VisArgs a) {
@@ -1036,7 +1040,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
   for (int64_t e = e_lo + (int64_t)blockIdx.x * kGroups + slot; e < e_hi;
        e += (int64_t)gridDim.x * kGroups) {
     const int64_t kg = list[e];
-    if (EXACT && kg < 0) continue;  // settled by the breadth-first exact walk
+    if (EXACT && (kg & kSettled)) continue;  // settled by the breadth-first exact walk
     const int64_t w = kg >> 6;
     const int b = (int)(kg & 63);
     const Part P = part_of_word<BATCH>(a, w);
@@ -1214,7 +1218,7 @@ __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
 
 // EXACT false: the walk queue (only when it holds at most a.front_max
 // pairs).  EXACT true: the first kExactSplit entries of the deferred list,
-// every test decided; an entry this kernel settles is overwritten with -1
+// every test decided; an entry this kernel settles gets the kSettled bit
 // (k_walk_split skips it), one whose frontier overflows or that has a cage
 // is left to k_walk_split.
 template <bool COUNT, bool BATCH, int G, bool EXACT>
@@ -1289,7 +1293,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
     if (g == 0) {
       if (edge) atomicOr(reinterpret_cast<unsigned long long *>(&a.bits[w]), 1ull << b);
       if (defer) a.dlist[atomicAdd(a.dlist_len, 1ull)] = kg;
-      if (EXACT && !left) list[e] = -1;  // settled here
+      if (EXACT && !left) list[e] = kg | kSettled;  // settled here
     }
     // the next pair of this group reuses the frontier buffers: keep its
     // writes after every lane's last read of this pair's frontier
