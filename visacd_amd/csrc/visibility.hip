@@ -1374,70 +1374,12 @@ __device__ int exact_ray_parity(const Node *__restrict__ nodes, const Tri *__res
   }
 }
 
-// Fan of each point: the leaf-ordered ids of the triangles that have the
-// point as a vertex (bitwise-equal coordinates), found by a point query on
-// the 4-wide BVH; -1 when there are more than kFanMax.
-// One point g per lane; stk: the lane's walk stack in LDS, 64 entries at a
-// stride of 64 ints (lane-interleaved: a private array would live in
-// scratch memory).
-template <bool BATCH>
-__device__ __forceinline__ void fan_point(const VisArgs &a, int64_t g, int *stk,
-                                          int *__restrict__ fan_n, int *__restrict__ fan_idx,
-                                          FanRec *__restrict__ fan_rec,
-                                          FanCone *__restrict__ fan_cone) {
-  Part P = a.one;
-  if (BATCH) {
-    int lo = 0, hi = a.nparts - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.parts[mid].point_begin <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    P = a.parts[lo];
-  }
-  if (!P.fan_n) return;
-  const int64_t i = g - P.point_begin;
-  const float p[3] = {P.pts[3 * i], P.pts[3 * i + 1], P.pts[3 * i + 2]};
-  int node = 0, sp = 0, cnt = 0;
-  bool done = false;
-  int *out = fan_idx + (size_t)g * kFanMax;
-  for (int it = 0; it <= P.scap && !done; ++it) {
-    const Node4 nd = P.snodes4[node];
-    int next = -1;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const bool in = p[0] >= lane_of(nd.f[0], c) && p[0] <= lane_of(nd.f[3], c) &&
-                      p[1] >= lane_of(nd.f[1], c) && p[1] <= lane_of(nd.f[4], c) &&
-                      p[2] >= lane_of(nd.f[2], c) && p[2] <= lane_of(nd.f[5], c);
-      if (!in) continue;
-      const int id = __float_as_int(lane_of(nd.f[6], c));
-      if (id < 0) {
-        const Tri t = P.stris[~id];
-        const float a3[3] = {t.a.x, t.a.y, t.a.z}, b3[3] = {t.b.x, t.b.y, t.b.z},
-                    c3[3] = {t.c.x, t.c.y, t.c.z};
-        if (same3(p, a3) || same3(p, b3) || same3(p, c3)) {
-          if (cnt < kFanMax) out[cnt] = ~id;
-          ++cnt;
-        }
-      } else if (next < 0) {
-        next = id;
-      } else if (sp < 64) {
-        stk[(sp++) * 64] = id;
-      } else {
-        cnt = kFanMax + 1;  // stack exhausted: give up on this point's fan
-      }
-    }
-    if (next >= 0) {
-      node = next;
-    } else if (sp == 0) {
-      done = true;
-    } else {
-      node = stk[(--sp) * 64];
-    }
-  }
-  // an unfinished walk or an overfull fan leaves the point to the full walk
-  fan_n[g] = (!done || cnt > kFanMax) ? -1 : cnt;
-  if (!done || cnt > kFanMax) return;
+// The records and cones of point g's fan (cnt triangle ids at out; p the
+// point).
+__device__ __forceinline__ void fan_finish(const Part &P, int64_t g, const float p[3],
+                                           const int *out, int cnt,
+                                           FanRec *__restrict__ fan_rec,
+                                           FanCone *__restrict__ fan_cone) {
   // the fan's records (see FanRec)
   for (int f = 0; f < cnt; ++f) {
     const Tri t = P.stris[out[f]];
@@ -1509,6 +1451,204 @@ __device__ __forceinline__ void fan_point(const VisArgs &a, int64_t g, int *stk,
     }
   }
   fan_cone[g] = cone;
+}
+
+// Fan of each point: the leaf-ordered ids of the triangles that have the
+// point as a vertex (bitwise-equal coordinates), found by a point query on
+// the 4-wide BVH; -1 when there are more than kFanMax.  One point g per
+// lane; stk: the lane's walk stack in LDS, 64 entries at a stride of 64 ints
+// (lane-interleaved: a private array would live in scratch memory).
+template <bool BATCH>
+__device__ __forceinline__ void fan_point(const VisArgs &a, int64_t g, int *stk,
+                                          int *__restrict__ fan_n, int *__restrict__ fan_idx,
+                                          FanRec *__restrict__ fan_rec,
+                                          FanCone *__restrict__ fan_cone) {
+  Part P = a.one;
+  if (BATCH) {
+    int lo = 0, hi = a.nparts - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.parts[mid].point_begin <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    P = a.parts[lo];
+  }
+  if (!P.fan_n) return;
+  const int64_t i = g - P.point_begin;
+  const float p[3] = {P.pts[3 * i], P.pts[3 * i + 1], P.pts[3 * i + 2]};
+  int node = 0, sp = 0, cnt = 0;
+  bool done = false;
+  int *out = fan_idx + (size_t)g * kFanMax;
+  for (int it = 0; it <= P.scap && !done; ++it) {
+    const Node4 nd = P.snodes4[node];
+    int next = -1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const bool in = p[0] >= lane_of(nd.f[0], c) && p[0] <= lane_of(nd.f[3], c) &&
+                      p[1] >= lane_of(nd.f[1], c) && p[1] <= lane_of(nd.f[4], c) &&
+                      p[2] >= lane_of(nd.f[2], c) && p[2] <= lane_of(nd.f[5], c);
+      if (!in) continue;
+      const int id = __float_as_int(lane_of(nd.f[6], c));
+      if (id < 0) {
+        const Tri t = P.stris[~id];
+        const float a3[3] = {t.a.x, t.a.y, t.a.z}, b3[3] = {t.b.x, t.b.y, t.b.z},
+                    c3[3] = {t.c.x, t.c.y, t.c.z};
+        if (same3(p, a3) || same3(p, b3) || same3(p, c3)) {
+          if (cnt < kFanMax) out[cnt] = ~id;
+          ++cnt;
+        }
+      } else if (next < 0) {
+        next = id;
+      } else if (sp < 64) {
+        stk[(sp++) * 64] = id;
+      } else {
+        cnt = kFanMax + 1;  // stack exhausted: give up on this point's fan
+      }
+    }
+    if (next >= 0) {
+      node = next;
+    } else if (sp == 0) {
+      done = true;
+    } else {
+      node = stk[(--sp) * 64];
+    }
+  }
+  // an unfinished walk or an overfull fan leaves the point to the full walk
+  fan_n[g] = (!done || cnt > kFanMax) ? -1 : cnt;
+  if (!done || cnt > kFanMax) return;
+  fan_finish(P, g, p, out, cnt, fan_rec, fan_cone);
+}
+
+// The fan query breadth first: G lanes per point (the group), the point's
+// frontier of tree nodes (binary tree: boxes containing the point) in LDS as
+// in the far walks, fan triangles appended to the point's list in ballot
+// order (in LDS) and then sorted; a group per point of block `block` (NT
+// threads, lds: (NT / G) * (2 * 16 * G + kFanMax) ints).  About as many
+// dependent steps as the tree is deep instead of one per visited node.
+template <bool BATCH, int G, int NT>
+__device__ __forceinline__ void fan_front_block(const VisArgs &a, int64_t block, int *lds,
+                                                int *__restrict__ fan_n,
+                                                int *__restrict__ fan_idx,
+                                                FanRec *__restrict__ fan_rec,
+                                                FanCone *__restrict__ fan_cone) {
+  constexpr int kGroups = NT / G;
+  constexpr int kF = 16 * G;
+  int(*front_all)[2][kF] = reinterpret_cast<int(*)[2][kF]>(lds);
+  const int lane = threadIdx.x & 63;
+  const int slot = threadIdx.x / G;
+  const int g = lane & (G - 1);
+  const uint64_t gm = group_mask(lane, G);
+  const uint64_t below = gm & ((1ull << lane) - 1ull);
+  const int64_t pt = block * kGroups + slot;
+  if (pt >= a.npoints) return;  // group-uniform; no block-wide barrier below
+  Part P = a.one;
+  if (BATCH) {
+    int lo = 0, hi = a.nparts - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.parts[mid].point_begin <= pt) lo = mid;
+      else hi = mid - 1;
+    }
+    P = a.parts[lo];
+  }
+  if (!P.fan_n) return;
+  const int64_t i = pt - P.point_begin;
+  const float p[3] = {P.pts[3 * i], P.pts[3 * i + 1], P.pts[3 * i + 2]};
+  int *out = fan_idx + (size_t)pt * kFanMax;
+  int *fl = lds + kGroups * 2 * kF + slot * kFanMax;  // the group's fan list
+  int *cur = front_all[slot][0], *nxt = front_all[slot][1];
+  if (g == 0) cur[0] = 0;
+  wave_lds_sync();
+  int n = 1, cnt = 0;
+  int64_t visits = 0;
+  bool fail = false;
+  while (n > 0 && !fail) {
+    visits += n;
+    if (visits > P.scap) {
+      fail = true;
+      break;
+    }
+    int nn = 0;
+    for (int base = 0; base < n; base += G) {
+      const int idx = base + g;
+      bool w0 = false, w1 = false, f0 = false, f1 = false;
+      int c0 = 0, c1 = 0;
+      if (idx < n) {
+        float bx[12];
+        load_node(P.snodes, cur[idx], bx, c0, c1);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const float *b = bx + 6 * c;
+          if (!(p[0] >= b[0] && p[0] <= b[3] && p[1] >= b[1] && p[1] <= b[4] && p[2] >= b[2] &&
+                p[2] <= b[5]))
+            continue;
+          const int ch = c ? c1 : c0;
+          bool in_fan = false;
+          if (ch < 0) {
+            const Tri t = P.stris[~ch];
+            const float a3[3] = {t.a.x, t.a.y, t.a.z}, b3[3] = {t.b.x, t.b.y, t.b.z},
+                        c3[3] = {t.c.x, t.c.y, t.c.z};
+            in_fan = same3(p, a3) || same3(p, b3) || same3(p, c3);
+          }
+          if (c) f1 = in_fan, w1 = ch >= 0;
+          else f0 = in_fan, w0 = ch >= 0;
+        }
+      }
+      const uint64_t mf0 = __ballot(f0) & gm, mf1 = __ballot(f1) & gm;
+      const int nf0 = __popcll(mf0);
+      if (f0 && cnt + __popcll(mf0 & below) < kFanMax) fl[cnt + __popcll(mf0 & below)] = ~c0;
+      if (f1 && cnt + nf0 + __popcll(mf1 & below) < kFanMax)
+        fl[cnt + nf0 + __popcll(mf1 & below)] = ~c1;
+      cnt += nf0 + __popcll(mf1);
+      const uint64_t m0 = __ballot(w0) & gm, m1 = __ballot(w1) & gm;
+      const int cnt0 = __popcll(m0), tot = cnt0 + __popcll(m1);
+      if (nn + tot > kF) {
+        fail = true;  // frontier overflow: k_fan_redo queries the point depth first
+        break;
+      }
+      if (w0) nxt[nn + __popcll(m0 & below)] = c0;
+      if (w1) nxt[nn + cnt0 + __popcll(m1 & below)] = c1;
+      nn += tot;
+    }
+    wave_lds_sync();
+    int *t = cur;
+    cur = nxt;
+    nxt = t;
+    n = nn;
+  }
+  wave_lds_sync();  // the group's frontier buffers are free after this
+  if (g != 0) return;
+  if (fail || cnt > kFanMax) {
+    fan_n[pt] = fail ? -2 : -1;  // -2: k_fan_redo
+    return;
+  }
+  // canonical order: ascending triangle id
+  for (int u = 1; u < cnt; ++u) {
+    const int v = fl[u];
+    int w = u - 1;
+    while (w >= 0 && fl[w] > v) {
+      fl[w + 1] = fl[w];
+      --w;
+    }
+    fl[w + 1] = v;
+  }
+  for (int u = 0; u < cnt; ++u) out[u] = fl[u];
+  fan_n[pt] = cnt;
+  fan_finish(P, pt, p, fl, cnt, fan_rec, fan_cone);
+}
+
+
+// Points whose breadth-first fan query overflowed its frontier (fan_n -2),
+// queried again depth first, a lane per point.
+template <bool BATCH>
+__global__ __launch_bounds__(64) void k_fan_redo(VisArgs a, int *__restrict__ fan_n,
+                                                 int *__restrict__ fan_idx,
+                                                 FanRec *__restrict__ fan_rec,
+                                                 FanCone *__restrict__ fan_cone) {
+  __shared__ int stk_all[64][64];
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g < a.npoints && fan_n[g] == -2)
+    fan_point<BATCH>(a, g, &stk_all[0][threadIdx.x], fan_n, fan_idx, fan_rec, fan_cone);
 }
 
 template <bool BATCH>
@@ -1663,22 +1803,20 @@ __global__ __launch_bounds__(kBlock) void k_far_front(VisArgs a, uint8_t *__rest
 }
 
 // The fan query and the first far-parity walks in one launch (they are
-// independent): blocks below fan_blocks take 64 points' fans, the others
-// the far walks of 8 (G = 8) or 4 (G = 16) points each.  One launch instead
-// of two back to back; the fan blocks come first so that their longer
-// walks start at once.
+// independent): blocks below fan_blocks take 8 points' fans (breadth
+// first, 8 lanes each), the others the far walks of 8 (G = 8) or 4
+// (G = 16) points each.  One launch instead of two back to back.
 template <bool BATCH, int G>
 __global__ __launch_bounds__(64) void k_far_fan(VisArgs a, uint8_t *__restrict__ far,
                                                 unsigned fan_blocks, int *__restrict__ fan_n,
                                                 int *__restrict__ fan_idx,
                                                 FanRec *__restrict__ fan_rec,
                                                 FanCone *__restrict__ fan_cone) {
-  // the fan query's stacks (64 x 64) or the far walks' frontiers (2048)
-  __shared__ int lds[64 * 64];
-  static_assert((64 / G) * 2 * 16 * G <= 64 * 64, "far frontiers fit the stacks' LDS");
+  // the frontiers and fan lists of the fan queries, or the far walks' frontiers
+  __shared__ int lds[(64 / 8) * (2 * 16 * 8 + kFanMax)];
+  static_assert((64 / G) * 2 * 16 * G <= (64 / 8) * 2 * 16 * 8, "far frontiers fit");
   if (blockIdx.x < fan_blocks) {
-    const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (g < a.npoints) fan_point<BATCH>(a, g, lds + threadIdx.x, fan_n, fan_idx, fan_rec, fan_cone);
+    fan_front_block<BATCH, 8, 64>(a, blockIdx.x, lds, fan_n, fan_idx, fan_rec, fan_cone);
   } else {
     far_front_block<BATCH, G, false, 64>(a, far, blockIdx.x - fan_blocks,
                                          gridDim.x - fan_blocks, lds);
@@ -2009,7 +2147,7 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     const int fl = max_tris > 131072 ? 16 : 8;
     const unsigned far_blocks =
         (unsigned)std::min<int64_t>((a.npoints + 64 / fl - 1) / (64 / fl), 16384);
-    const unsigned fan_blocks = ctx->fan_pairs ? fg : 0;
+    const unsigned fan_blocks = ctx->fan_pairs ? (unsigned)((a.npoints + 7) / 8) : 0;
     // points whose frontier overflows: again with a wave per point (a
     // frontier of 1024 nodes), then depth first
     const unsigned fpg64 = (unsigned)std::min<int64_t>((a.npoints + kWaves - 1) / kWaves, 4096);
@@ -2023,6 +2161,10 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
         k_far_fan<false, 16><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
       else
         k_far_fan<false, 8><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
+      if (fan_blocks) {
+        if (batch) k_fan_redo<true><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
+        else k_fan_redo<false><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
+      }
       if (batch) {
         k_far_front<true, 64, true><<<fpg64, kBlock, 0, st>>>(a, farb.as<uint8_t>());
         k_far_parity<true, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
