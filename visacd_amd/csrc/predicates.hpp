@@ -188,8 +188,45 @@ __device__ __forceinline__ int sgn_or_zero(double v, double bound, bool &certain
 }
 
 // ------------------------------------------------------------ exact signs
-__device__ __noinline__ int orient3d_exact(const float *a, const float *b, const float *c,
-                                           const float *d) {
+// The expansion forms (rare: inputs of very different magnitude) are
+// separate, out-of-line functions so that the filters and the integer path
+// inline into their callers.
+__device__ __noinline__ int orient3d_expansion(const float *a, const float *b, const float *c,
+                                               const float *d) {
+  Exp e;
+  e.n = 0;
+  exp_det4(e, a, b, c, d, 1);
+  return exp_sign(e);
+}
+
+__device__ __noinline__ int orient3d_mid_expansion(const float *a, const float *b, const float *c,
+                                                   const float *p, const float *q) {
+  Exp e;
+  e.n = 0;
+  exp_det4(e, a, b, c, p, 1);
+  exp_det4(e, a, b, c, q, 1);
+  return exp_sign(e);
+}
+
+__device__ __noinline__ int orient2d_expansion(const float *a, const float *b, const float *c,
+                                               int k) {
+  Exp e;
+  e.n = 0;
+  exp_det3(e, a, b, c, k, 1);
+  return exp_sign(e);
+}
+
+__device__ __noinline__ int orient2d_mid_expansion(const float *a, const float *b, const float *p,
+                                                   const float *q, int k) {
+  Exp e;
+  e.n = 0;
+  exp_det3(e, a, b, p, k, 1);
+  exp_det3(e, a, b, q, k, 1);
+  return exp_sign(e);
+}
+
+__device__ __forceinline__ int orient3d_exact(const float *a, const float *b, const float *c,
+                                              const float *d) {
   double bound;
   const double det = o3_fast(a, b, c, d, bound);
   if (det > bound) return 1;
@@ -199,15 +236,12 @@ __device__ __noinline__ int orient3d_exact(const float *a, const float *b, const
     int64_t m[12];
     if (int_grid(f, m)) return sign128(det3_int(m));
   }
-  Exp e;
-  e.n = 0;
-  exp_det4(e, a, b, c, d, 1);
-  return exp_sign(e);
+  return orient3d_expansion(a, b, c, d);
 }
 
 // sign of orient3d(a, b, c, (p+q)/2) = sign(orient3d(.., p) + orient3d(.., q)).
-__device__ __noinline__ int orient3d_mid_exact(const float *a, const float *b, const float *c,
-                                               const float *p, const float *q) {
+__device__ __forceinline__ int orient3d_mid_exact(const float *a, const float *b, const float *c,
+                                                  const float *p, const float *q) {
   double b1, b2;
   const double d1 = o3_fast(a, b, c, p, b1);
   const double d2 = o3_fast(a, b, c, q, b2);
@@ -225,15 +259,11 @@ __device__ __noinline__ int orient3d_mid_exact(const float *a, const float *b, c
       return sign128(dp + det3_int(m));
     }
   }
-  Exp e;
-  e.n = 0;
-  exp_det4(e, a, b, c, p, 1);
-  exp_det4(e, a, b, c, q, 1);
-  return exp_sign(e);
+  return orient3d_mid_expansion(a, b, c, p, q);
 }
 
-__device__ __noinline__ int orient2d_exact(const float *a, const float *b, const float *c,
-                                           int k) {
+__device__ __forceinline__ int orient2d_exact(const float *a, const float *b, const float *c,
+                                              int k) {
   double bound;
   const double det = o2_fast(a, b, c, k, bound);
   if (det > bound) return 1;
@@ -244,15 +274,12 @@ __device__ __noinline__ int orient2d_exact(const float *a, const float *b, const
     int64_t m[6];
     if (int_grid(f, m)) return sign128(det2_int(m));
   }
-  Exp e;
-  e.n = 0;
-  exp_det3(e, a, b, c, k, 1);
-  return exp_sign(e);
+  return orient2d_expansion(a, b, c, k);
 }
 
 // sign of orient2d(a, b, (p+q)/2) in projection k.
-__device__ __noinline__ int orient2d_mid_exact(const float *a, const float *b, const float *p,
-                                               const float *q, int k) {
+__device__ __forceinline__ int orient2d_mid_exact(const float *a, const float *b, const float *p,
+                                                  const float *q, int k) {
   double b1, b2;
   const double d1 = o2_fast(a, b, p, k, b1);
   const double d2 = o2_fast(a, b, q, k, b2);
@@ -270,11 +297,7 @@ __device__ __noinline__ int orient2d_mid_exact(const float *a, const float *b, c
       return sign128(dp + det2_int(m));
     }
   }
-  Exp e;
-  e.n = 0;
-  exp_det3(e, a, b, p, k, 1);
-  exp_det3(e, a, b, q, k, 1);
-  return exp_sign(e);
+  return orient2d_mid_expansion(a, b, p, q, k);
 }
 
 // ------------------------------------------------- exact contact / crossing
