@@ -1035,7 +1035,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
   const int64_t *list = EXACT ? a.dlist : a.queue;
   const int64_t len = (int64_t)(EXACT ? *a.dlist_len : *a.queue_len);
   if (!EXACT && len <= a.front_max) return;  // k_walk_front took this queue
-  const int64_t e_lo = (EXACT && LV == 0) ? kExactSplit : 0;
+  // with the breadth-first exact walk (a.front_max >= 0) the lane-per-pair
+  // instance takes every entry it left (kSettled clear), the wave-per-pair
+  // one is not launched
+  const int64_t e_lo = (EXACT && LV == 0 && a.front_max < 0) ? kExactSplit : 0;
   const int64_t e_hi = (EXACT && LV == 6) ? (len < kExactSplit ? len : kExactSplit) : len;
   for (int64_t e = e_lo + (int64_t)blockIdx.x * kGroups + slot; e < e_hi;
        e += (int64_t)gridDim.x * kGroups) {
@@ -2228,20 +2231,21 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
     if (batch) k_walk_front<false, true, 64, true><<<wgrid, kBlock, 0, st>>>(a);
     else k_walk_front<false, false, 64, true><<<wgrid, kBlock, 0, st>>>(a);
   }
+  const bool wave = !L.front;  // the wave-per-pair instance (without the breadth-first one)
   if (need2 <= 32) {
     if (batch) {
-      k_walk_split<32, false, true, 6, true><<<wgrid, kBlock, 0, st>>>(a);
+      if (wave) k_walk_split<32, false, true, 6, true><<<wgrid, kBlock, 0, st>>>(a);
       k_walk_split<32, false, true, 0, true><<<xgrid, kBlock, 0, st>>>(a);
     } else {
-      k_walk_split<32, false, false, 6, true><<<wgrid, kBlock, 0, st>>>(a);
+      if (wave) k_walk_split<32, false, false, 6, true><<<wgrid, kBlock, 0, st>>>(a);
       k_walk_split<32, false, false, 0, true><<<xgrid, kBlock, 0, st>>>(a);
     }
   } else {
     if (batch) {
-      k_walk_split<64, false, true, 6, true><<<wgrid, kBlock, 0, st>>>(a);
+      if (wave) k_walk_split<64, false, true, 6, true><<<wgrid, kBlock, 0, st>>>(a);
       k_walk_split<64, false, true, 0, true><<<xgrid, kBlock, 0, st>>>(a);
     } else {
-      k_walk_split<64, false, false, 6, true><<<wgrid, kBlock, 0, st>>>(a);
+      if (wave) k_walk_split<64, false, false, 6, true><<<wgrid, kBlock, 0, st>>>(a);
       k_walk_split<64, false, false, 0, true><<<xgrid, kBlock, 0, st>>>(a);
     }
   }
