@@ -225,6 +225,26 @@ int vacd_score_planes_d(vacd_ctx *ctx, const float *d_planes, int64_t P,
                         int64_t E, int64_t *d_score_q32, float *d_score_f,
                         int32_t *d_argmax);
 
+/* Allocation-free scoring.  No counterpart in the reference: meant for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * definitions and results as vacd_score_planes_d (device pointers, enqueued
+ * on the context stream, returns without waiting), but the plane sums and
+ * edge records live in a fixed device array of the library (one per GPU,
+ * held until the process ends, about 32.5 MiB) instead of the context's
+ * pool, so the call makes no allocation of any kind.  Limits: P <=
+ * VACD_NOALLOC_MAX_PLANES, E <= VACD_NOALLOC_MAX_EDGES (VACD_ERR_LIMIT
+ * beyond); d_planes must be 16-byte and d_edges 8-byte aligned
+ * (VACD_ERR_INVALID otherwise).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (two streams, two contexts, two threads)
+ * race and must be serialised by the caller. */
+#define VACD_NOALLOC_MAX_PLANES 65536
+#define VACD_NOALLOC_MAX_EDGES 1048576
+int vacd_score_planes_noalloc_d(vacd_ctx *ctx, const float *d_planes, int64_t P,
+                                const float *d_pts, int64_t n, const uint32_t *d_edges,
+                                int64_t E, int64_t *d_score_q32, float *d_score_f,
+                                int32_t *d_argmax);
+
 /* One part of a batched scoring call: its planes, points and edges (device
  * pointers for the _d form, host pointers for the host form). */
 typedef struct vacd_score_part {

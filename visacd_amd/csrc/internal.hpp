@@ -301,6 +301,13 @@ void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P,
                     int64_t E, int64_t *d_q32, float *d_f, int32_t *d_argmax);
 void launch_scoring_batch(vacd_ctx *ctx, const vacd_score_part *parts, int nparts,
                           int64_t *d_q32, int32_t *d_argmax);
+// Allocation-free scoring (scoring.hip): noalloc_setup looks up, once per
+// device and with that device current, the address of the library's fixed
+// scoring scratch array; launch_scoring_noalloc then scores with it.
+void noalloc_setup(int device);
+void launch_scoring_noalloc(vacd_ctx *ctx, const float *d_planes, int64_t P, const float *d_pts,
+                            int64_t n, const uint32_t *d_edges, int64_t E, int64_t *d_q32,
+                            float *d_f, int32_t *d_argmax);
 
 // bf16 GEMM on MFMA (mlp.hip): C = op(A) op(B) (+ bias, relu), fp32 and/or
 // bf16 output; A(m, k) = ta ? A[k * lda + m] : A[m * lda + k], B(k, n) = tb ?

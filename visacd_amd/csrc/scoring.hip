@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace vacd {
 namespace {
@@ -260,6 +261,33 @@ __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *__restrict
 
 }  // namespace
 
+// The launches of one scoring call, given its scratch: acc (P sums) and
+// rec (E edge records).  Nothing here allocates.
+static void score_launches(vacd_ctx *ctx, const float *d_planes, int64_t P, const float *d_pts,
+                           int64_t n, const uint32_t *d_edges, int64_t E,
+                           unsigned long long *acc, EdgeRec *rec, int64_t *d_q32, float *d_f,
+                           int32_t *d_argmax, bool timed) {
+  hipStream_t st = ctx->stream;
+  fill_u64(ctx, acc, P, 0ull);
+  if (E > 0) {
+    k_edge_prep<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(d_pts, n, d_edges, E, rec);
+    const int64_t ptiles = (P + kPlanesPerBlock - 1) / kPlanesPerBlock;
+    // about kTargetBlocks blocks in total unless the context fixes the chunk length
+    int64_t nchunks = std::max<int64_t>(1, kTargetBlocks / ptiles);
+    int64_t chunk = ctx->score_chunk > 0 ? ctx->score_chunk : (E + nchunks - 1) / nchunks;
+    chunk = std::max<int64_t>(kEdgeQuantum, (chunk + kEdgeQuantum - 1) / kEdgeQuantum * kEdgeQuantum);
+    nchunks = (E + chunk - 1) / chunk;
+    VACD_REQUIRE(nchunks < 65536, VACD_ERR_LIMIT, "too many edge chunks");
+    dim3 grid((unsigned)ptiles, (unsigned)nchunks);
+    if (timed) timed_begin(ctx, TIMED_SCORE);
+    k_score<<<grid, kBlock, 0, st>>>(reinterpret_cast<const float4 *>(d_planes), P, rec, E, chunk,
+                                     acc);
+    if (timed) timed_end(ctx, TIMED_SCORE);
+  }
+  k_finalize<<<1, 1024, 0, st>>>(acc, P, d_q32, d_f, d_argmax);
+  VACD_HIP(hipGetLastError());
+}
+
 void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P, const float *d_pts, int64_t n,
                     const uint32_t *d_edges, int64_t E, int64_t *d_q32, float *d_f,
                     int32_t *d_argmax) {
@@ -278,28 +306,75 @@ void launch_scoring(vacd_ctx *ctx, const float *d_planes, int64_t P, const float
   }
   vacd::DevBuf &accb = ctx->buf[SLOT_SCORES];
   accb.ensure(8 * (size_t)P);
-  unsigned long long *acc = accb.as<unsigned long long>();
-  fill_u64(ctx, acc, P, 0ull);
+  EdgeRec *rec = nullptr;
   if (E > 0) {
     vacd::DevBuf &recb = ctx->buf[SLOT_EREC];
     recb.ensure(sizeof(EdgeRec) * (size_t)E);
-    EdgeRec *rec = recb.as<EdgeRec>();
-    k_edge_prep<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(d_pts, n, d_edges, E, rec);
-    const int64_t ptiles = (P + kPlanesPerBlock - 1) / kPlanesPerBlock;
-    // about kTargetBlocks blocks in total unless the context fixes the chunk length
-    int64_t nchunks = std::max<int64_t>(1, kTargetBlocks / ptiles);
-    int64_t chunk = ctx->score_chunk > 0 ? ctx->score_chunk : (E + nchunks - 1) / nchunks;
-    chunk = std::max<int64_t>(kEdgeQuantum, (chunk + kEdgeQuantum - 1) / kEdgeQuantum * kEdgeQuantum);
-    nchunks = (E + chunk - 1) / chunk;
-    VACD_REQUIRE(nchunks < 65536, VACD_ERR_LIMIT, "too many edge chunks");
-    dim3 grid((unsigned)ptiles, (unsigned)nchunks);
-    timed_begin(ctx, TIMED_SCORE);
-    k_score<<<grid, kBlock, 0, st>>>(reinterpret_cast<const float4 *>(d_planes), P, rec, E, chunk,
-                                     acc);
-    timed_end(ctx, TIMED_SCORE);
+    rec = recb.as<EdgeRec>();
   }
-  k_finalize<<<1, 1024, 0, st>>>(acc, P, d_q32, d_f, d_argmax);
-  VACD_HIP(hipGetLastError());
+  score_launches(ctx, d_planes, P, d_pts, n, d_edges, E, accb.as<unsigned long long>(), rec,
+                 d_q32, d_f, d_argmax, true);
+}
+
+// ------------------------------------------------------- allocation-free form
+// Scratch of vacd_score_planes_noalloc_d: the Q32.32 sums of up to
+// VACD_NOALLOC_MAX_PLANES planes, then the records of up to
+// VACD_NOALLOC_MAX_EDGES edges.  Each GPU holds its own copy from module load
+// until the process ends.  Host code never names the array itself (its name
+// is a host-side stand-in, not a device address): noalloc_setup() looks up
+// the device address once per device and every launch uses that.
+namespace {
+constexpr size_t kNoAllocAccBytes = 8 * (size_t)VACD_NOALLOC_MAX_PLANES;
+constexpr size_t kNoAllocBytes = kNoAllocAccBytes + sizeof(EdgeRec) * (size_t)VACD_NOALLOC_MAX_EDGES;
+static_assert(kNoAllocAccBytes % 256 == 0, "edge records start 256-byte aligned");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoAllocDevices] = {};
+
+__global__ void k_set_i32(int32_t *p, int32_t v) { *p = v; }
+}  // namespace
+
+void noalloc_setup(int device) {
+  VACD_REQUIRE(device >= 0 && device < kNoAllocDevices, VACD_ERR_LIMIT,
+               "allocation-free scoring supports device ordinals below 64");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return;
+  int cur = -1;
+  VACD_HIP(hipGetDevice(&cur));
+  VACD_REQUIRE(cur == device, VACD_ERR_INVALID, "noalloc_setup: device is not current");
+  void *p = nullptr;
+  VACD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  VACD_REQUIRE(p != nullptr, VACD_ERR_HIP, "no device address for the scoring scratch array");
+  g_noalloc_ptr[device] = p;
+}
+
+void launch_scoring_noalloc(vacd_ctx *ctx, const float *d_planes, int64_t P, const float *d_pts,
+                            int64_t n, const uint32_t *d_edges, int64_t E, int64_t *d_q32,
+                            float *d_f, int32_t *d_argmax) {
+  VACD_REQUIRE(P <= VACD_NOALLOC_MAX_PLANES, VACD_ERR_LIMIT,
+               "allocation-free scoring: more planes than VACD_NOALLOC_MAX_PLANES (65,536)");
+  VACD_REQUIRE(E <= VACD_NOALLOC_MAX_EDGES, VACD_ERR_LIMIT,
+               "allocation-free scoring: more edges than VACD_NOALLOC_MAX_EDGES (1,048,576)");
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    base = g_noalloc_ptr[ctx->device];
+  }
+  VACD_REQUIRE(base != nullptr, VACD_ERR_INVALID,
+               "allocation-free scoring: the context's device has no scratch address");
+  if (P == 0) {
+    if (d_argmax) k_set_i32<<<1, 1, 0, ctx->stream>>>(d_argmax, -1);
+    VACD_HIP(hipGetLastError());
+    return;
+  }
+  unsigned long long *acc = static_cast<unsigned long long *>(base);
+  EdgeRec *rec = reinterpret_cast<EdgeRec *>(static_cast<char *>(base) + kNoAllocAccBytes);
+  score_launches(ctx, d_planes, P, d_pts, n, d_edges, E, acc, rec, d_q32, d_f, d_argmax, false);
 }
 
 void fill_u64(vacd_ctx *ctx, void *p, int64_t n, unsigned long long v) {

@@ -69,6 +69,9 @@ int vacd_ctx_create(int device, vacd_ctx **out) {
     VACD_HIP(hipSetDevice(device));
     VACD_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
     ctx->stream = ctx->own_stream;
+    // once per device, with it current: the scratch address of the
+    // allocation-free scoring entry point
+    vacd::noalloc_setup(device);
   });
   if (rc != VACD_OK) {
     delete ctx;
@@ -545,6 +548,24 @@ int vacd_score_planes_d(vacd_ctx *ctx, const float *d_planes, int64_t P, const f
       ed = b.as<uint32_t>();
     }
     vacd::launch_scoring(ctx, pl, P, d_pts, n, ed, E, d_score_q32, d_score_f, d_argmax);
+  });
+}
+
+int vacd_score_planes_noalloc_d(vacd_ctx *ctx, const float *d_planes, int64_t P,
+                                const float *d_pts, int64_t n, const uint32_t *d_edges, int64_t E,
+                                int64_t *d_score_q32, float *d_score_f, int32_t *d_argmax) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(P >= 0 && n >= 0 && E >= 0, VACD_ERR_INVALID, "negative size");
+    VACD_REQUIRE(P == 0 || d_planes, VACD_ERR_INVALID, "NULL planes");
+    VACD_REQUIRE(E == 0 || (d_edges && d_pts), VACD_ERR_INVALID, "NULL edges or points");
+    VACD_REQUIRE(P == 0 || aligned(d_planes, 16), VACD_ERR_INVALID,
+                 "allocation-free scoring needs 16-byte aligned planes");
+    VACD_REQUIRE(E == 0 || aligned(d_edges, 8), VACD_ERR_INVALID,
+                 "allocation-free scoring needs 8-byte aligned edges");
+    set_device(ctx);
+    vacd::launch_scoring_noalloc(ctx, d_planes, P, d_pts, n, d_edges, E, d_score_q32, d_score_f,
+                                 d_argmax);
   });
 }
 

@@ -20,6 +20,8 @@ VACD_ERR_INVALID = 1
 VACD_ERR_HIP = 2
 VACD_ERR_CAPACITY = 3
 VACD_ERR_LIMIT = 4
+VACD_NOALLOC_MAX_PLANES = 65536
+VACD_NOALLOC_MAX_EDGES = 1048576
 
 # Every symbol include/vacd.h declares (checked by tests/test_abi.py).
 EXPORTED = [
@@ -30,7 +32,7 @@ EXPORTED = [
     "vacd_visibility",
     "vacd_visibility_d", "vacd_visibility_stats", "vacd_visibility_counters", "vacd_visibility_deferred",
     "vacd_visibility_batch_d", "vacd_visibility_batch", "vacd_score_planes",
-    "vacd_score_planes_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
+    "vacd_score_planes_d", "vacd_score_planes_noalloc_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
     "vacd_max_distance", "vacd_max_distance_d", "vacd_max_distance_batch", "vacd_plane_extents_batch",
     "vacd_gemm_bf16_d", "vacd_mlp_param_count",
     "vacd_mlp_create", "vacd_mlp_destroy", "vacd_mlp_set_params", "vacd_mlp_get_params",
@@ -132,6 +134,7 @@ def lib():
         L.vacd_score_planes.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp,
                                         ctypes.POINTER(i32)]
         L.vacd_score_planes_d.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp, vp]
+        L.vacd_score_planes_noalloc_d.argtypes = [vp, vp, i64, vp, i64, vp, i64, vp, vp, vp]
         L.vacd_score_planes_batch.argtypes = [vp, ctypes.POINTER(ScorePart), ctypes.c_int32, vp,
                                               vp]
         L.vacd_score_planes_batch_d.argtypes = [vp, ctypes.POINTER(ScorePart), ctypes.c_int32, vp,
@@ -359,6 +362,14 @@ class Context:
         self._check(lib().vacd_score_planes_d(self._h, _p(d_planes), P, _p(d_pts), n,
                                               _p(d_edges), E, _p(d_q32), _p(d_f),
                                               _p(d_argmax)))
+
+    def score_planes_noalloc_d(self, d_planes, P, d_pts, n, d_edges, E, d_q32=None, d_f=None,
+                               d_argmax=None):
+        """vacd_score_planes_noalloc_d: as score_planes_d, with the library's
+        fixed scratch array instead of the context's pool."""
+        self._check(lib().vacd_score_planes_noalloc_d(self._h, _p(d_planes), P, _p(d_pts), n,
+                                                      _p(d_edges), E, _p(d_q32), _p(d_f),
+                                                      _p(d_argmax)))
 
     def score_planes_batch(self, parts):
         """parts: list of (planes (P, 4), points (n, 3), edges (E, 2)) host arrays.
