@@ -47,16 +47,19 @@ def lib():
         L.orc_k_to_ij.argtypes = [i64, i64, vp, vp]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_visibility.argtypes = [vp, i64, vp, vp, i64, vp, i64, vp, i64, vp, i64,
-                                     ctypes.c_int, vp, i64, vp]
+                                     ctypes.c_int, vp, i64, vp, vp, vp, ctypes.c_int]
         L.orc_visibility.restype = i64
         L.orc_visibility_sample.argtypes = [vp, i64, vp, vp, i64, vp, i64, vp, i64, vp,
-                                            i64, ctypes.c_int, vp, i64, vp]
+                                            i64, ctypes.c_int, vp, i64, vp, vp, vp,
+                                            ctypes.c_int]
         L.orc_score_fp64.argtypes = [vp, i64, vp, i64, vp, i64, vp]
         L.orc_score_q32.argtypes = [vp, i64, vp, i64, vp, i64, vp]
         L.orc_argmax_i64.argtypes = [vp, i64]
         L.orc_argmax_i64.restype = i64
         L.orc_bisector_planes.argtypes = [vp, vp, i64, ctypes.c_int, ctypes.c_uint32, vp]
         L.orc_bisector_planes.restype = i64
+        L.orc_mt19937.argtypes = [ctypes.c_uint32, i64, vp]
+        L.orc_uniform_int.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, i64, vp]
         del fp
         _lib = L
     return _lib
@@ -115,9 +118,20 @@ def _mesh_args(V, T):
     return _ptr(V), V.shape[0], _ptr(T), T.shape[0], V, T
 
 
-def visibility(points, V, T, mask=None, cage=None, brute=False, cap=None):
+def _groups(groups):
+    """(tri_group int8 [T] (-1 none), planes float64 [G, 4]) -> ctypes args."""
+    if groups is None:
+        return None, None, 0, ()
+    g = np.ascontiguousarray(groups[0], dtype=np.int8)
+    pl = np.ascontiguousarray(groups[1], dtype=np.float64).reshape(-1, 4)
+    return _ptr(g), _ptr(pl), int(pl.shape[0]), (g, pl)
+
+
+def visibility(points, V, T, mask=None, cage=None, brute=False, cap=None, groups=None):
     """Visibility edges (E, 2) uint32 in k order and detail counts.
 
+    groups = (tri_group, planes): the triangle groups of mask bits 1-7 (the
+    cap rule), as given to the engine's planar BVH build.
     Returns (pairs, stats) with stats = {none, edge, masked, zero, blocked}.
     """
     pts = _f32(points)
@@ -132,15 +146,16 @@ def visibility(points, V, T, mask=None, cage=None, brute=False, cap=None):
         cap = max(1, n * (n - 1) // 2)
     out = np.zeros((cap, 2), dtype=np.uint32)
     stats = np.zeros(5, dtype=np.int64)
+    gp, pp, ng, _keep = _groups(groups)
     e = lib().orc_visibility(_ptr(pts), n, _ptr(m), sv, snv, st, snt, cv, cnv, ct, cnt,
-                             1 if brute else 0, _ptr(out), cap, _ptr(stats))
+                             1 if brute else 0, _ptr(out), cap, _ptr(stats), gp, pp, ng)
     if e < 0:
         raise RuntimeError("oracle: edge buffer too small")
     names = ["none", "edge", "masked", "zero", "blocked"]
     return out[:e].copy(), dict(zip(names, stats.tolist()))
 
 
-def visibility_sample(points, V, T, ks, mask=None, cage=None, brute=False):
+def visibility_sample(points, V, T, ks, mask=None, cage=None, brute=False, groups=None):
     """Per-pair detail codes (0 none, 1 edge, 2 masked, 3 zero, 4 blocked) for pair ids ks."""
     pts = _f32(points)
     n = pts.shape[0]
@@ -152,8 +167,9 @@ def visibility_sample(points, V, T, ks, mask=None, cage=None, brute=False):
     else:
         cv, cnv, ct, cnt = None, 0, None, 0
     out = np.zeros(ks.shape[0], dtype=np.uint8)
+    gp, pp, ng, _keep = _groups(groups)
     lib().orc_visibility_sample(_ptr(pts), n, _ptr(m), sv, snv, st, snt, cv, cnv, ct, cnt,
-                                1 if brute else 0, _ptr(ks), ks.shape[0], _ptr(out))
+                                1 if brute else 0, _ptr(ks), ks.shape[0], _ptr(out), gp, pp, ng)
     return out
 
 
@@ -192,3 +208,17 @@ def bisector_planes(vertices64, edges, num_planes: int, seed: int):
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+def mt19937(seed: int, n: int) -> np.ndarray:
+    """The first n raw outputs of std::mt19937(seed)."""
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_mt19937(seed, n, _ptr(out))
+    return out
+
+
+def uniform_int(seed: int, lo: int, hi: int, n: int) -> np.ndarray:
+    """n draws of std::uniform_int_distribution<int>(lo, hi) on std::mt19937(seed)."""
+    out = np.zeros(n, dtype=np.int32)
+    lib().orc_uniform_int(seed, lo, hi, n, _ptr(out))
+    return out

@@ -6,7 +6,8 @@
 // float64 and decided with exact orientation predicates:
 //
 //   * lib/ptx/ray_segments.cu:49-111 and lib/src/intersections.cpp:67-73,
-//     327-385: pair order k -> (i<j), the new-vertex mask (plus cap bits),
+//     327-385: pair order k -> (i<j), the new-vertex mask (plus the cap
+//     rule of mask bits 1-7, no counterpart in the reference),
 //     zero-length segments, and the combine rule
 //     edge <=> hits(cage) && !hits(self) in cage mode;
 //   * the midpoint-parity form of the same rule (cage absent):
@@ -42,6 +43,7 @@
 // atomicAdd order of the reference are not reproducible and are not pinned.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -476,6 +478,8 @@ void k_to_ij(int64_t k, int64_t n, int64_t& i, int64_t& j) {
 enum { ST_NOEDGE = 0, ST_EDGE = 1, ST_SKIP = 2 };
 // Per-pair outcome detail: 0 none, 1 edge, 2 skipped by mask, 3 zero length,
 // 4 blocked by self.
+constexpr int kFanMax = 16;  // incident triangles above which an end is not tested
+
 struct Ctx {
   const float* pts;
   int64_t n;
@@ -484,21 +488,55 @@ struct Ctx {
   bool has_cage;
   bool brute;
   Bvh bself, bcage;
+  // cap rule (mask bits 1-7): triangle groups, the axis each group's 2-D
+  // projection drops, whether the self mesh is closed, and each point's fan
+  const int8_t* tri_group = nullptr;
+  int axis[7] = {3, 3, 3, 3, 3, 3, 3};
+  bool closed = false;
+  std::vector<std::vector<int>> fan;
 };
 
-// Mask of a point: bit 0 new, bit 1 new and on the last cut's cap, bits 2-7
-// on one of six older cut planes.  A pair is skipped when neither point is
-// new (ray_segments.cu:61-75) or when both lie on one cut plane (bits 1-7
-// shared: the segment lies in that plane; the caller's choice, no
-// counterpart in the reference).
-bool mask_skips(uint8_t a, uint8_t b) {
-  return (!(a & 3) && !(b & 3)) || ((a >> 1) & (b >> 1)) != 0;
+// Mask of a point: bit 0 new (ray_segments.cu:61-75: a pair is skipped when
+// neither point is new); bit 1 + g: the point lies on the plane of triangle
+// group g (no counterpart in the reference; see enters_cap).
+bool mask_skips(uint8_t a, uint8_t b) { return !((a | b) & 1); }
+
+// The cap rule: for a pair whose points both lie on the plane of group g
+// (shared mask bit 1 + g), in parity mode on a closed self mesh, the pair is
+// not an edge when its direction at either end enters the closed 2-D angle,
+// in the projection that drops the axis of the plane normal's largest
+// component, of a group-g triangle incident to that end (vertex identity:
+// equal coordinates; an end with more than kFanMax incident triangles is not
+// tested).  With the group's triangles exactly in their plane this is the
+// contact of the segment with them; with rounded (cut) vertices it decides
+// the in-plane pairs that rounding alone would decide.
+bool enters_cap(const Ctx& C, int64_t i, const P3& p, const P3& q, unsigned groups) {
+  const std::vector<int>& fan = C.fan[i];
+  if ((int)fan.size() > kFanMax) return false;
+  for (int t : fan) {
+    const int g = C.tri_group[t];
+    if (g < 0 || g >= 7 || !((groups >> g) & 1u) || C.axis[g] == 3) continue;
+    const int k = C.axis[g];
+    const P3 &a = C.bself.a[t], &b = C.bself.b[t], &c = C.bself.c[t];
+    const bool pa = same(p, a), pb = same(p, b);
+    const P3& v1 = pa ? b : (pb ? c : a);
+    const P3& v2 = pa ? c : (pb ? a : b);
+    const int o = orient2d(p, v1, v2, k);
+    if (o == 0) continue;
+    const int o1 = orient2d(p, v1, q, k), o2 = orient2d(p, q, v2, k);
+    if ((o1 != 0 || o2 != 0) && o * o1 >= 0 && o * o2 >= 0) return true;
+  }
+  return false;
 }
 
 int pair_detail(const Ctx& C, int64_t i, int64_t j) {
   if (C.mask && mask_skips(C.mask[i], C.mask[j])) return 2;
   P3 p = ld(C.pts + 3 * i), q = ld(C.pts + 3 * j);
   if (same(p, q)) return 3;
+  if (C.mask && C.tri_group && C.closed && !C.has_cage) {
+    const unsigned shared = (unsigned)(C.mask[i] & C.mask[j]) >> 1;
+    if (shared && (enters_cap(C, i, p, q, shared) || enters_cap(C, j, q, p, shared))) return 4;
+  }
   auto self_hit = [&](int t) {
     return contact(p, q, C.bself.a[t], C.bself.b[t], C.bself.c[t]);
   };
@@ -535,9 +573,19 @@ int pair_detail(const Ctx& C, int64_t i, int64_t j) {
   return (cnt & 1) ? 0 : 1;
 }
 
+// Axis dropped by the 2-D projection of a plane with normal (a, b, c): the
+// largest |component|, ties to the lower axis.
+int plane_axis(const double* p) {
+  const double a = std::fabs(p[0]), b = std::fabs(p[1]), c = std::fabs(p[2]);
+  int k = b > a ? 1 : 0;
+  if (c > std::max(a, b)) k = 2;
+  return k;
+}
+
 void setup(Ctx& C, const float* pts, int64_t n, const uint8_t* mask, const float* V,
            const uint32_t* T, int64_t nT, const float* cV, const uint32_t* cT, int64_t cnT,
-           int brute) {
+           int brute, const int8_t* tri_group = nullptr, const double* group_planes = nullptr,
+           int n_groups = 0) {
   C.pts = pts;
   C.n = n;
   C.mask = mask;
@@ -547,6 +595,44 @@ void setup(Ctx& C, const float* pts, int64_t n, const uint8_t* mask, const float
   C.brute = brute != 0;
   C.bself.build(V, T, nT);
   if (C.has_cage) C.bcage.build(cV, cT, cnT);
+  if (!tri_group || n_groups <= 0 || !mask) return;
+  C.tri_group = tri_group;
+  for (int g = 0; g < n_groups && g < 7; ++g) C.axis[g] = plane_axis(group_planes + 4 * g);
+  // closed: every undirected edge (by vertex index) has an even number of triangles
+  std::vector<std::pair<uint32_t, uint32_t>> es;
+  es.reserve(3 * (size_t)nT);
+  for (int64_t t = 0; t < nT; ++t)
+    for (int e = 0; e < 3; ++e) {
+      uint32_t u = T[3 * t + e], v = T[3 * t + (e + 1) % 3];
+      es.push_back({std::min(u, v), std::max(u, v)});
+    }
+  std::sort(es.begin(), es.end());
+  C.closed = true;
+  for (size_t a = 0; a < es.size();) {
+    size_t b = a;
+    while (b < es.size() && es[b] == es[a]) ++b;
+    if ((b - a) & 1) C.closed = false;
+    a = b;
+  }
+  // each point's fan: the triangles with a vertex of equal coordinates
+  std::vector<std::pair<std::array<uint32_t, 3>, int>> keyed;
+  keyed.reserve(3 * (size_t)nT);
+  auto key = [](const float* v) {
+    std::array<uint32_t, 3> k;
+    std::memcpy(k.data(), v, 12);
+    for (auto& x : k) x = (x == 0x80000000u) ? 0u : x;  // -0 equals +0
+    return k;
+  };
+  for (int64_t t = 0; t < nT; ++t)
+    for (int c = 0; c < 3; ++c) keyed.push_back({key(V + 3 * (size_t)T[3 * t + c]), (int)t});
+  std::sort(keyed.begin(), keyed.end());
+  C.fan.assign((size_t)n, {});
+  for (int64_t i = 0; i < n; ++i) {
+    const auto k = key(pts + 3 * i);
+    auto it = std::lower_bound(keyed.begin(), keyed.end(), std::make_pair(k, -1));
+    for (; it != keyed.end() && it->first == k; ++it)
+      if (C.fan[i].empty() || C.fan[i].back() != it->second) C.fan[i].push_back(it->second);
+  }
 }
 
 }  // namespace
@@ -587,14 +673,17 @@ int orc_num_threads(void) {
 // cap pairs) and returns the edge count; stats[0..4] receive the counts of
 // details 0..4 (none, edge, masked, zero length, blocked).  Returns -1 if
 // the edge count exceeds cap.
+// tri_group (nullable): per triangle its group (-1 none, 0..6) for the cap
+// rule of mask bits 1-7, with group_planes [n_groups][4] (a, b, c, d).
 int64_t orc_visibility(const float* pts, int64_t n, const uint8_t* mask, const float* V,
                        int64_t nV, const uint32_t* T, int64_t nT, const float* cV,
                        int64_t cnV, const uint32_t* cT, int64_t cnT, int brute,
-                       uint32_t* pairs_out, int64_t cap, int64_t* stats) {
+                       uint32_t* pairs_out, int64_t cap, int64_t* stats,
+                       const int8_t* tri_group, const double* group_planes, int n_groups) {
   (void)nV;
   (void)cnV;
   Ctx C;
-  setup(C, pts, n, mask, V, T, nT, cV, cT, cnT, brute);
+  setup(C, pts, n, mask, V, T, nT, cV, cT, cnT, brute, tri_group, group_planes, n_groups);
   std::vector<std::vector<uint32_t>> rows(n > 0 ? n : 0);
   int64_t st[5] = {0, 0, 0, 0, 0};
 #pragma omp parallel
@@ -630,11 +719,12 @@ int64_t orc_visibility(const float* pts, int64_t n, const uint8_t* mask, const f
 void orc_visibility_sample(const float* pts, int64_t n, const uint8_t* mask, const float* V,
                            int64_t nV, const uint32_t* T, int64_t nT, const float* cV,
                            int64_t cnV, const uint32_t* cT, int64_t cnT, int brute,
-                           const int64_t* ks, int64_t m, uint8_t* detail_out) {
+                           const int64_t* ks, int64_t m, uint8_t* detail_out,
+                           const int8_t* tri_group, const double* group_planes, int n_groups) {
   (void)nV;
   (void)cnV;
   Ctx C;
-  setup(C, pts, n, mask, V, T, nT, cV, cT, cnT, brute);
+  setup(C, pts, n, mask, V, T, nT, cV, cT, cnT, brute, tri_group, group_planes, n_groups);
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t s = 0; s < m; ++s) {
     int64_t i, j;
@@ -718,6 +808,19 @@ int64_t orc_argmax_i64(const int64_t* s, int64_t P) {
   for (int64_t p = 1; p < P; ++p)
     if (s[p] > s[best]) best = p;
   return P > 0 ? best : -1;
+}
+
+// Raw outputs of std::mt19937(seed) (the reference's random_engine,
+// lib/include/core.hpp:13) and draws of std::uniform_int_distribution<int>
+// (lo, hi) on it (lib/src/process.cpp:38-43), for the known-answer tests.
+void orc_mt19937(uint32_t seed, int64_t n, uint32_t* out) {
+  std::mt19937 rng(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = (uint32_t)rng();
+}
+void orc_uniform_int(uint32_t seed, int lo, int hi, int64_t n, int32_t* out) {
+  std::mt19937 rng(seed);
+  std::uniform_int_distribution<> dis(lo, hi);
+  for (int64_t i = 0; i < n; ++i) out[i] = dis(rng);
 }
 
 // Bisector candidate planes (lib/src/process.cpp:28-72) with a fresh

@@ -276,6 +276,71 @@ def test_visibility_degenerate_polycube(ctx, seed):
     _print("exact fallbacks", st["exact_calls"])
 
 
+def test_exact_redo_beyond_the_wave_per_pair_share(ctx):
+    """A polycube large enough that more than 2,048 pairs (the share of the
+    wave-per-pair exact walk) are redone exactly: with breadth-first walks (8
+    lanes) and with depth-first walks only (0), every deferred pair must be
+    decided as the oracle decides it, and the whole edge list must match."""
+    from visacd_amd import native
+
+    rng = np.random.default_rng(77)
+    cells = {(x, y, z) for x in range(20) for y in range(20) for z in range(4)
+             if z == 0 or rng.uniform() < 0.55}
+    V, T = meshes.polycube(sorted(cells))
+    _print("big polycube", V.shape, T.shape)
+    o, _ = oracle.visibility(V, V, T)
+    oset = set(map(tuple, o.tolist()))
+    b = ctx.bvh(V, T)
+    n = V.shape[0]
+    for fl in (8, 0):
+        ctx.set_option(native.VACD_OPT_FRONT_LANES, fl)
+        try:
+            g = ctx.visibility(V, b)
+            st = ctx.visibility_stats()
+            ids = ctx.visibility_deferred()
+        finally:
+            ctx.set_option(native.VACD_OPT_FRONT_LANES, 8)
+        _print("front lanes", fl, "deferred", len(ids), st)
+        assert len(ids) > 2048, f"only {len(ids)} deferred pairs"
+        np.testing.assert_array_equal(g, o, err_msg=f"front lanes {fl}")
+        gset = set(map(tuple, g.tolist()))
+        det = oracle.visibility_sample(V, V, T, ids)
+        for k, d in zip(ids.tolist(), det.tolist()):
+            i, j = oracle.k_to_ij(k, n)
+            assert ((i, j) in gset) == (d == 1) == ((i, j) in oset), (fl, k)
+
+
+def test_cap_rule_u_channel_cut_through_the_arms(ctx):
+    """U-channel cut through both arms at y = 0.375 (exact in float32): the
+    cap rule (cap points new and on group 0, cap triangles as group 0) gives
+    the oracle's edges, which equal the plain mask's, gap-crossing pairs
+    included."""
+    from visacd_amd import _visacd as H
+
+    V, T = meshes.u_channel()
+    plane = [0.0, -1.0, 0.0, 0.375]
+    m = H.Mesh()
+    m.vertices = H.VecArray3d(np.asarray(V, np.float64).tolist())
+    m.triangles = H.make_vecarray3i(np.asarray(T, np.int32))
+    parts, _mp, _mn, flags = H.clip(m, plane)
+    Pv = np.array([list(x) for x in parts[0].vertices], np.float32)
+    Pt = np.array([list(x) for x in parts[0].triangles], np.uint32)
+    new = np.asarray(flags[0], bool)
+    on = np.abs(Pv[:, 1].astype(np.float64) - 0.375) == 0.0
+    grp = np.where(on[Pt].all(1), 0, -1).astype(np.int8)
+    b = ctx.bvh(Pv, Pt, tri_group=grp, planes=np.array([plane]))
+    m1 = new.astype(np.uint8)
+    m2 = m1 | (2 * on).astype(np.uint8)
+    o1, _ = oracle.visibility(Pv, Pv, Pt, mask=m1)
+    o2, _ = oracle.visibility(Pv, Pv, Pt, mask=m2, groups=(grp, np.array([plane])))
+    g1 = ctx.visibility(Pv, b, new_mask=m1)
+    g2 = ctx.visibility(Pv, b, new_mask=m2)
+    _print("u channel", len(o1), len(o2), len(g1), len(g2))
+    np.testing.assert_array_equal(o2, o1)
+    np.testing.assert_array_equal(g1, o1)
+    np.testing.assert_array_equal(g2, o2)
+
+
 def _clipped_part(level, seed, plane):
     from visacd_amd import _visacd as V
 
@@ -313,12 +378,19 @@ def test_visibility_clipped_part_contact_tree(ctx):
     gm = ctx.visibility(pts, b, new_mask=mask)
     np.testing.assert_array_equal(gm, om)
     _print("clipped part edges", len(g), "masked", len(gm), ctx.visibility_stats())
-    # cap points as 2: their pairs with each other are skipped
-    m2 = (2 * cap[idx]).astype(np.uint8)
-    o2, ost2 = oracle.visibility(pts, Pv, T, mask=m2)
-    g2 = ctx.visibility(pts, b, new_mask=m2)
+    # the cap rule: cap points new and on group 0's plane, the cap triangles
+    # as planar group 0 of the BVH and of the oracle
+    grp = np.where(cap[T].all(1), 0, -1).astype(np.int8)
+    plane = np.array([[*n, 0.05]])
+    bg = ctx.bvh(Pv, T, tri_group=grp, planes=plane)
+    m2 = (cap[idx] * 3).astype(np.uint8)
+    o2, ost2 = oracle.visibility(pts, Pv, T, mask=m2, groups=(grp, plane))
+    g2 = ctx.visibility(pts, bg, new_mask=m2)
     np.testing.assert_array_equal(g2, o2)
-    assert ctx.visibility_stats()["masked"] == ost2["masked"]
+    st2 = ctx.visibility_stats()
+    assert st2["masked"] == ost2["masked"]
+    assert set(map(tuple, g2.tolist())) <= set(map(tuple, gm.tolist()))
+    _print("cap rule", len(gm), "->", len(g2), st2)
 
 
 def test_visibility_icosphere_c2_sampled(ctx):
@@ -430,11 +502,13 @@ def _clip_twice(level, seed, planes):
     T = np.array([list(x) for x in m.triangles], np.int64)
     grp = np.full(T.shape[0], -1, np.int8)
     on = np.zeros(Pv.shape[0], bool)
+    bits = np.zeros(Pv.shape[0], np.uint8)  # mask bit 1 + g: on plane g
     for g, pl in enumerate(planes):
         d = np.abs(Pv @ np.asarray(pl[:3]) + pl[3]) <= 1e-9
         on |= d
+        bits |= (d * (2 << g)).astype(np.uint8)
         grp[(grp < 0) & d[T].all(1)] = g
-    return Pv.astype(np.float32), T.astype(np.uint32), grp, on
+    return Pv.astype(np.float32), T.astype(np.uint32), grp, on, bits
 
 
 def _planes2():
@@ -449,7 +523,7 @@ def test_visibility_planar_groups_match_oracle(ctx):
     """Caps of two cuts as planar groups: the pruned contact walks give the
     oracle's edges, with and without the cap mask."""
     planes = _planes2()
-    Pv, T, grp, on = _clip_twice(4, 23, planes)
+    Pv, T, grp, on, bits = _clip_twice(4, 23, planes)
     b = ctx.bvh(Pv, T, tri_group=grp, planes=np.array(planes))
     _print("planar part", Pv.shape, T.shape, "grouped", int((grp >= 0).sum()), b.contact_info())
     assert (grp >= 0).sum() > 32
@@ -461,10 +535,16 @@ def test_visibility_planar_groups_match_oracle(ctx):
     o, _ = oracle.visibility(pts, Pv, T)
     g = ctx.visibility(pts, b)
     np.testing.assert_array_equal(g, o)
-    m2 = (2 * on[idx]).astype(np.uint8)
-    o2, _ = oracle.visibility(pts, Pv, T, mask=m2)
+    m1 = on[idx].astype(np.uint8)
+    o1, _ = oracle.visibility(pts, Pv, T, mask=m1)
+    g1 = ctx.visibility(pts, b, new_mask=m1)
+    np.testing.assert_array_equal(g1, o1)
+    # the cap rule on both cut planes (points of both caps new)
+    m2 = m1 | bits[idx]
+    o2, _ = oracle.visibility(pts, Pv, T, mask=m2, groups=(grp, np.array(planes)))
     g2 = ctx.visibility(pts, b, new_mask=m2)
     np.testing.assert_array_equal(g2, o2)
+    _print("planar cap rule", len(g1), "->", len(g2))
 
 
 def test_visibility_planar_groups_prune_and_keep_edges(ctx):
@@ -473,13 +553,13 @@ def test_visibility_planar_groups_prune_and_keep_edges(ctx):
     from visacd_amd import native
 
     planes = _planes2()
-    Pv, T, grp, on = _clip_twice(5, 29, planes)
+    Pv, T, grp, on, _bits = _clip_twice(5, 29, planes)
     rng = np.random.default_rng(6)
     off = np.where(~on)[0]
     idx = np.unique(np.concatenate([np.where(on)[0],
                                     rng.choice(off, min(3000, off.size), replace=False)]))
     pts = Pv[idx]
-    mask = (2 * on[idx]).astype(np.uint8)
+    mask = on[idx].astype(np.uint8)
     plain = ctx.bvh(Pv, T)
     grouped = ctx.bvh(Pv, T, tri_group=grp, planes=np.array(planes))
     ctx.set_option(native.VACD_OPT_COUNT_WORK, 1)

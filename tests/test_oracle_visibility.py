@@ -125,40 +125,116 @@ def test_mask_skips_old_pairs():
     assert st["masked"] == nm * (nm - 1) // 2
 
 
-def test_mask_value_two_skips_pairs_among_twos():
-    """Mask 2: a new point whose pairs with the other 2-points are skipped;
-    its pairs with 0- and 1-points are tested as for 1."""
+def test_mask_bits_above_zero_do_not_skip_without_groups():
+    """Only bit 0 marks a new point: a pair is skipped when neither point has
+    it; bits 1-7 (points on a group plane) change nothing without groups."""
     V, T = wl.perturbed_icosphere(2, 0.4, 8)
     n = V.shape[0]
     rng = np.random.default_rng(1)
-    mask = rng.choice(np.array([0, 1, 2], np.uint8), size=n, p=[0.6, 0.2, 0.2])
+    mask = rng.choice(np.array([0, 1, 2, 3, 6], np.uint8), size=n, p=[0.4, 0.15, 0.15, 0.15, 0.15])
     full, _ = oracle.visibility(V, V, T)
     part, st = oracle.visibility(V, V, T, mask=mask)
-    keep = [(i, j) for i, j in full.tolist()
-            if (mask[i] or mask[j]) and not (mask[i] == 2 and mask[j] == 2)]
+    keep = [(i, j) for i, j in full.tolist() if (mask[i] | mask[j]) & 1]
     assert set(map(tuple, part.tolist())) == set(keep)
-    n0, n2 = int((mask == 0).sum()), int((mask == 2).sum())
-    assert st["masked"] == n0 * (n0 - 1) // 2 + n2 * (n2 - 1) // 2
+    old = int(((mask & 1) == 0).sum())
+    assert st["masked"] == old * (old - 1) // 2
 
 
-def test_mask_cap_bits_skip_pairs_on_one_cut_plane():
-    """Bits 2-7 mark points on older cut planes: 4 = old point on plane 1,
-    6 = new point on the last cap and on plane 1.  Skipped: old-old pairs
-    (0 and 4 are old), pairs of two 6s (shared last cap) and 4-6 pairs
-    (shared plane 1); 1-x, 0-6 and the rest are tested as usual."""
-    V, T = wl.perturbed_icosphere(2, 0.4, 8)
-    n = V.shape[0]
-    rng = np.random.default_rng(3)
-    mask = rng.choice(np.array([0, 1, 4, 6], np.uint8), size=n, p=[0.4, 0.2, 0.2, 0.2])
-    full, _ = oracle.visibility(V, V, T)
-    part, st = oracle.visibility(V, V, T, mask=mask)
-    n0, n4, n6 = (int((mask == v).sum()) for v in (0, 4, 6))
-    old = n0 + n4
-    assert st["masked"] == old * (old - 1) // 2 + n6 * (n6 - 1) // 2 + n4 * n6
-    tested = {(0, 1), (1, 1), (1, 4), (1, 6), (0, 6)}
-    keep = [(i, j) for i, j in full.tolist()
-            if tuple(sorted((int(mask[i]), int(mask[j])))) in tested]
-    assert set(map(tuple, part.tolist())) == set(keep)
+def _clipped(V, T, plane):
+    """The positive-side part of the host clip, float32, with its new
+    (cap) vertices and the cap triangles as group 0."""
+    from visacd_amd import _visacd as H
+
+    m = H.Mesh()
+    m.vertices = H.VecArray3d(np.asarray(V, np.float64).tolist())
+    m.triangles = H.make_vecarray3i(np.asarray(T, np.int32))
+    parts, _mp, _mn, flags = H.clip(m, list(plane))
+    part = parts[0]
+    Pv = np.array([list(x) for x in part.vertices], np.float64)
+    Pt = np.array([list(x) for x in part.triangles], np.int64)
+    new = np.asarray(flags[0], bool)
+    on = np.abs(Pv @ np.asarray(plane[:3]) + plane[3]) <= 1e-9 * np.linalg.norm(plane[:3])
+    grp = np.where(on[Pt].all(1), 0, -1).astype(np.int8)
+    return Pv.astype(np.float32), Pt.astype(np.uint32), new, on, grp
+
+
+def test_cap_rule_keeps_edges_across_the_gap_of_a_u_channel():
+    """The U-channel cut through both arms (y = 0.375, exact in float32)
+    leaves two rectangular caps in one plane.  Pairs of cap points across
+    the gap between the arms cross air and are edges; pairs within one cap
+    run into it.  With exactly planar caps the cap rule (mask bit 1 on the
+    cap points, the cap triangles as group 0) must give exactly the edges of
+    the plain mask (new points as 1), and the gap-crossing pairs must be
+    among them."""
+    V, T = meshes.u_channel()
+    plane = (0.0, -1.0, 0.0, 0.375)  # positive side: y < 0.375 (the lower part)
+    Pv, Pt, new, on, grp = _clipped(V, T, plane)
+    assert new.sum() == 16 and (on & new).sum() == 16 and (grp == 0).sum() >= 4
+    m1 = new.astype(np.uint8)
+    e1, _ = oracle.visibility(Pv, Pv, Pt, mask=m1)
+    m_rule = (new.astype(np.uint8) | (2 * on).astype(np.uint8))
+    e2, st2 = oracle.visibility(Pv, Pv, Pt, mask=m_rule, groups=(grp, [plane]))
+    np.testing.assert_array_equal(e2, e1)
+    cap = np.where(new)[0]
+    left = [i for i in cap if Pv[i, 0] < 0.3]
+    right = [i for i in cap if Pv[i, 0] > 0.4]
+    assert len(left) == 8 and len(right) == 8
+    edges = set(map(tuple, e2.tolist()))
+    across = {(min(a, b), max(a, b)) for a in left for b in right}
+    assert len(across & edges) >= 9
+    within = {(min(a, b), max(a, b)) for a in left for b in left if a != b}
+    # diagonals of one rectangular cap run through it: never edges
+    assert not (within & edges) - set(map(tuple, e1.tolist()))
+
+
+def test_cap_rule_only_removes_in_plane_pairs_that_enter_the_cap():
+    """On a rounded cut (a general plane: the cap vertices are rounded to
+    float32) the cap rule may only turn edges into non-edges, and only pairs
+    of two points on the cap plane; it removes every in-plane pair whose
+    direction enters a cap triangle at either end (checked on the chords of
+    the cap that pass through the part's section)."""
+    V, T = wl.perturbed_icosphere(3, 0.3, 21)
+    nrm = np.array([0.3, 0.5, -0.2])
+    nrm /= np.linalg.norm(nrm)
+    plane = (*nrm, 0.05)
+    Pv, Pt, new, on, grp = _clipped(V, T, plane)
+    assert (grp == 0).sum() > 16
+    m1 = new.astype(np.uint8)
+    m_rule = (m1 | (2 * on).astype(np.uint8))
+    e1, _ = oracle.visibility(Pv, Pv, Pt, mask=m1)
+    e2, _ = oracle.visibility(Pv, Pv, Pt, mask=m_rule, groups=(grp, [plane]))
+    s1, s2 = set(map(tuple, e1.tolist())), set(map(tuple, e2.tolist()))
+    assert s2 <= s1
+    for i, j in s1 - s2:
+        assert on[i] and on[j]
+    # no in-plane pair left an edge enters a cap triangle at either end
+    # (wedge test restated with the Fraction-pinned orient2d)
+    k = int(np.argmax(np.abs(nrm)))
+    caps = [t for t in range(Pt.shape[0]) if grp[t] == 0]
+
+    def fan_size(a):
+        return int(np.all(Pv[Pt] == Pv[a], axis=2).any(axis=1).sum())
+
+    def enters(a, b):  # an end with more than 16 incident triangles is not tested
+        p, q = Pv[a], Pv[b]
+        if fan_size(a) > 16:
+            return False
+        for t in caps:
+            tri = [Pv[v] for v in Pt[t]]
+            at = [x for x in range(3) if np.array_equal(tri[x], p)]
+            if not at:
+                continue
+            v1, v2 = tri[(at[0] + 1) % 3], tri[(at[0] + 2) % 3]
+            o = oracle.orient2d(p, v1, v2, k)
+            o1, o2 = oracle.orient2d(p, v1, q, k), oracle.orient2d(p, q, v2, k)
+            if o != 0 and (o1 or o2) and o * o1 >= 0 and o * o2 >= 0:
+                return True
+        return False
+
+    in_plane = [(i, j) for i, j in s2 if on[i] and on[j]]
+    assert not any(enters(i, j) or enters(j, i) for i, j in in_plane)
+    removed = [(i, j) for i, j in s1 - s2]
+    assert all(enters(i, j) or enters(j, i) for i, j in removed)
 
 
 def test_k_order_and_pair_decode():

@@ -89,34 +89,43 @@ def test_process_with_point_limit():
         vis.config.return_parts = False
 
 
-def test_process_batched_partitions_and_is_deterministic():
-    """Batched frontier mode: every over-threshold part split per round; the
-    parts still partition the volume, the result repeats exactly, and it
-    lowers concavity like the plain loop."""
-    import visacd as vis
-
-    V, T = wl.perturbed_icosphere(3, 0.4, 1)
-    print("process batched ico3", flush=True)
-    vis.engine.mode = "batched"
-    try:
-        r1 = _run(vis, V, T, 0.02, 8, return_parts=True)
-        r2 = _run(vis, V, T, 0.02, 8, return_parts=True)
-    finally:
-        vis.engine.mode = "plain"
-        vis.config.return_parts = False
-    assert 2 <= r1.num_parts == len(r1.parts)
-    assert r1.num_parts == r2.num_parts and r1.concavity == r2.concavity
-    for a, b in zip(r1.parts, r2.parts):
-        assert [list(v) for v in a.vertices] == [list(v) for v in b.vertices]
-    vol = _volume(_mesh(vis, V, T))
-    total = sum(_volume(p) for p in r1.parts)
-    assert abs(total - vol) < 1e-6 * abs(vol) + 1e-9
-    r0 = _run(vis, V, T, 0.02, 1)
-    assert r1.concavity < r0.concavity
-
-
 def _parts(r):
     return [[list(v) for v in p.vertices] for p in r.parts]
+
+
+@pytest.mark.parametrize("conc,n", [(0.001, 9), (0.06, 40)])
+def test_batched_frontier_gives_the_plain_decomposition(conc, n):
+    """The batched frontier replays the plain loop on the tree of parts it has
+    split (speculatively, several per round): the parts, their order and the
+    final concavity must be exactly the plain loop's, whether the part budget
+    (conc 0.001) or the threshold (conc 0.06) ends the run, for any round
+    width and with or without eager visibility."""
+    import visacd as vis
+    from visacd_amd import _visacd
+
+    V, T = wl.perturbed_icosphere(3, 0.4, 1)
+    print("batched vs plain ico3", conc, n, flush=True)
+    try:
+        vis.engine.mode = "plain"
+        r0 = _run(vis, V, T, conc, n, return_parts=True)
+        for width, eager in ((64, True), (3, False), (1, True)):
+            vis.engine.mode = "batched"
+            vis.engine.batch_width = width
+            vis.engine.eager_visibility = eager
+            r1 = _run(vis, V, T, conc, n, return_parts=True)
+            prof = _visacd.last_profile()
+            print("  width", width, "eager", eager, "rounds", prof.get("batched.rounds"),
+                  "split", prof.get("batched.parts_split"), flush=True)
+            assert r1.num_parts == r0.num_parts and r1.concavity == r0.concavity, (width, eager)
+            assert _parts(r1) == _parts(r0), (width, eager)
+    finally:
+        vis.engine.mode = "plain"
+        vis.engine.batch_width = 64
+        vis.engine.eager_visibility = False
+        vis.config.return_parts = False
+    assert r0.num_parts >= 3
+    vol = _volume(_mesh(vis, V, T))
+    assert abs(sum(_volume(p) for p in r0.parts) - vol) < 1e-6 * abs(vol) + 1e-9
 
 
 def test_learned_proposal_with_full_sweep_matches_exact():

@@ -72,9 +72,10 @@ PYBIND11_MODULE(_visacd, m) {
       .def_readwrite("hausdorff_resolution", &vhost::EngineOptions::hausdorff_resolution)
       .def_readwrite("sample_seed", &vhost::EngineOptions::sample_seed)
       .def_readwrite("gpu_distance", &vhost::EngineOptions::gpu_distance)
-      .def_readwrite("skip_cap_pairs", &vhost::EngineOptions::skip_cap_pairs)
+      .def_readwrite("cap_rule", &vhost::EngineOptions::cap_rule)
       .def_readwrite("mode", &vhost::EngineOptions::mode)
-      .def_readwrite("frontier_fraction", &vhost::EngineOptions::frontier_fraction)
+      .def_readwrite("batch_width", &vhost::EngineOptions::batch_width)
+      .def_readwrite("eager_visibility", &vhost::EngineOptions::eager_visibility)
       .def_readwrite("proposal", &vhost::EngineOptions::proposal)
       .def_readwrite("proposal_weights", &vhost::EngineOptions::proposal_weights)
       .def_readwrite("proposal_top", &vhost::EngineOptions::proposal_top)
@@ -127,6 +128,28 @@ PYBIND11_MODULE(_visacd, m) {
           return py::make_tuple(feat, q, py::make_tuple(fr.c[0], fr.c[1], fr.c[2], fr.s));
         },
         py::arg("verts"), py::arg("edges"), py::arg("planes"), py::arg("seed"));
+  m.def("bisector_planes",
+        [](py::array_t<double, py::array::c_style | py::array::forcecast> verts,
+           py::array_t<uint32_t, py::array::c_style | py::array::forcecast> edges, int num_planes,
+           uint32_t seed) {
+          if (verts.ndim() != 2 || verts.shape(1) != 3) throw std::invalid_argument("verts must be (n, 3)");
+          if (edges.ndim() != 2 || edges.shape(1) != 2) throw std::invalid_argument("edges must be (E, 2)");
+          vhost::Mesh m;
+          const size_t n = (size_t)verts.shape(0);
+          m.vertices.resize(n);
+          for (size_t i = 0; i < n; ++i)
+            m.vertices[i] = {verts.data()[3 * i], verts.data()[3 * i + 1], verts.data()[3 * i + 2]};
+          for (py::ssize_t e = 0; e < edges.shape(0); ++e) {
+            const uint32_t a = edges.data()[2 * e], b = edges.data()[2 * e + 1];
+            if (a >= n || b >= n) throw std::out_of_range("edge endpoint out of range");
+            m.intersecting_edges.push_back({a, b});
+          }
+          std::vector<std::array<double, 4>> out;
+          for (auto &p : vhost::bisector_planes_seeded(m, num_planes, seed)) out.push_back({p.a, p.b, p.c, p.d});
+          return out;
+        },
+        py::arg("verts"), py::arg("edges"), py::arg("num_planes"), py::arg("seed"));
+  m.def("select_plane", &vhost::select_plane, py::arg("score_q32"), py::arg("ss_offset"));
   m.def("flat_surface_planes", [](const vhost::Mesh &mesh, double min_area) {
     std::vector<std::array<double, 4>> out;
     for (auto &p : vhost::flat_surface_planes(mesh, min_area)) out.push_back({p.a, p.b, p.c, p.d});

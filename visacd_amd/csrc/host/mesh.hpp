@@ -85,16 +85,20 @@ struct EngineOptions {
   int hausdorff_resolution = 3000;
   uint32_t sample_seed = 42;     // seeds subsampling and surface sampling
   bool gpu_distance = true;      // Hausdorff term on the GPU (exact point-to-mesh)
-  // pairs of two vertices of the last cut's cap lie in the cut plane: their
-  // visibility is decided by rounding alone, so they are not tested (the
-  // reference tests every pair touching a new vertex)
-  bool skip_cap_pairs = true;
-  // "plain": the reference's loop, one part split per step; "batched": every
-  // part above the threshold split in the same round (up to the part budget)
+  // cap rule: a pair of two points on one cut plane whose segment runs into
+  // that cut's cap at an end touches the cap and is not an edge, decided by
+  // exact 2-D orientations in the plane instead of by the rounded cap
+  // vertices (needs planar_caps; the reference tests such pairs like any)
+  bool cap_rule = true;
+  // "plain": the reference's loop, one part split per step; "batched": the
+  // same decomposition with the parts the loop will split next split
+  // together, up to batch_width per round
   std::string mode = "plain";
-  // batched rounds split the parts whose concavity is at least this fraction
-  // of the worst part's (1: only the worst, as the plain loop; 0: all)
-  double frontier_fraction = 0.5;
+  int batch_width = 64;
+  // batched: compute the new parts' visibility as soon as they are made,
+  // overlapped with their concavity on the host (true), or only when a part
+  // is about to be split (false: no visibility for parts never split)
+  bool eager_visibility = false;
   // plane selection: "exact" scores every candidate plane (the reference);
   // "learned" ranks the candidates with the proposal model, scores the
   // proposal_top best-ranked and proposal_audit random ones exactly, and
@@ -165,5 +169,11 @@ double hausdorff_boundary(const Mesh &a, const Mesh &b, int resolution, uint32_t
 std::vector<Plane> flat_surface_planes(const Mesh &m, double min_area);
 
 ProcessResult process(Mesh mesh, double concavity, int num_parts);
+// The driver's candidate sampler with a fresh std::mt19937(seed) over
+// m.intersecting_edges (lib/src/process.cpp:28-72), and its plane choice
+// over Q32.32 scores: first maximum, planes from ss_offset on (flat
+// surfaces) times config.flat_surface_k (lib/src/process.cpp:128-134).
+std::vector<Plane> bisector_planes_seeded(const Mesh &m, int num_planes, uint32_t seed);
+int select_plane(const std::vector<int64_t> &q32, size_t ss_offset);
 
 }  // namespace vhost

@@ -181,26 +181,55 @@ struct PairBuffer {
 
 vacd_bvh *build_bvh(const Engine &E, const Mesh &m, bool planar = false);
 
-// Visibility mask of a part's points (vacd_visibility): bit 0 new vertex;
-// with engine.skip_cap_pairs, bit 1 = new vertex (on the last cut's cap)
-// and bits 2-7 = vertex on one of the six cut planes before it, so that the
-// pairs of two points on one cut plane are skipped (their segment lies in
-// that plane).
+// Planar groups of a part's BVH: the caps of its last 7 cuts, each the
+// triangles with all three vertices on the cut plane (|n.v + d| <= 1e-9 |n|),
+// kept when at least 16 triangles lie on it.  grp[t] = group of triangle t
+// (-1 none); planes = (a, b, c, d) per group.
+void cap_groups(const Mesh &m, std::vector<int8_t> &grp, std::vector<double> &planes) {
+  const size_t nT = m.triangles.size();
+  grp.assign(nT, -1);
+  planes.clear();
+  std::vector<size_t> mine;
+  for (int c = (int)m.cut_planes.size() - 1; c >= 0 && planes.size() < 4 * 7; --c) {
+    const Plane &q = m.cut_planes[c];
+    const double l = std::sqrt(q.a * q.a + q.b * q.b + q.c * q.c);
+    if (!(l > 0.0)) continue;
+    const double tol = 1e-9 * l;
+    mine.clear();
+    for (size_t t = 0; t < nT; ++t) {
+      if (grp[t] >= 0) continue;
+      const auto &tr = m.triangles[t];
+      if (std::fabs(q.eval(m.vertices[tr[0]])) <= tol && std::fabs(q.eval(m.vertices[tr[1]])) <= tol &&
+          std::fabs(q.eval(m.vertices[tr[2]])) <= tol)
+        mine.push_back(t);
+    }
+    if (mine.size() < 16) continue;
+    const int8_t g = (int8_t)(planes.size() / 4);
+    for (size_t t : mine) grp[t] = g;
+    planes.insert(planes.end(), {q.a, q.b, q.c, q.d});
+  }
+}
+
+// Visibility mask of a part's points (vacd_visibility): bit 0 = new vertex
+// (the reference's mask); with engine.cap_rule (and planar caps), bit 1 + g
+// = the vertex lies on the plane of the part's planar group g, so that pairs
+// of two points on one cap plane whose segment runs into the cap at an end
+// are decided by the engine's cap rule.
 std::vector<uint8_t> point_mask(const Mesh &part) {
   const size_t n = part.points.size();
   std::vector<uint8_t> mask(n, 0);
-  std::vector<Plane> older;
-  if (engine.skip_cap_pairs)
-    for (int c = (int)part.cut_planes.size() - 2; c >= 0 && older.size() < 6; --c)
-      older.push_back(part.cut_planes[c]);
+  std::vector<int8_t> grp;
+  std::vector<double> pl;
+  if (engine.cap_rule && engine.planar_caps) cap_groups(part, grp, pl);
+  const size_t G = pl.size() / 4;
   for (size_t i = 0; i < n; ++i) {
     const int v = part.points[i];
     const bool fresh = v < (int)part.is_new.size() && part.is_new[v];
-    uint8_t m = fresh ? (engine.skip_cap_pairs ? 2 : 1) : 0;
-    for (size_t k = 0; k < older.size(); ++k) {
-      const Plane &q = older[k];
+    uint8_t m = fresh ? 1 : 0;
+    for (size_t g = 0; g < G; ++g) {
+      const Plane q(pl[4 * g], pl[4 * g + 1], pl[4 * g + 2], pl[4 * g + 3]);
       const double tol = 1e-9 * std::sqrt(q.a * q.a + q.b * q.b + q.c * q.c);
-      if (std::fabs(q.eval(part.vertices[v])) <= tol) m |= (uint8_t)(4u << k);
+      if (std::fabs(q.eval(part.vertices[v])) <= tol) m |= (uint8_t)(2u << g);
     }
     mask[i] = m;
   }
@@ -227,7 +256,7 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
   for (int64_t i = 0; i < n; ++i) {
     const int v = part.points[i];
     for (int k = 0; k < 3; ++k) P[3 * i + k] = V[3 * (size_t)v + k];
-    any_new |= (mask[i] & 3) != 0;
+    any_new |= (mask[i] & 1) != 0;
   }
   const bool use_mask = with_mask;
   if (use_mask && !any_new) {
@@ -286,28 +315,7 @@ vacd_bvh *build_bvh_on(vacd_ctx *ctx, const Mesh &m, bool planar) {
     for (int k = 0; k < 3; ++k) T[3 * t + k] = (uint32_t)m.triangles[t][k];
   std::vector<int8_t> grp;
   std::vector<double> pl;
-  if (planar && engine.planar_caps && !m.cut_planes.empty()) {
-    grp.assign(nT, -1);
-    std::vector<size_t> mine;
-    for (int c = (int)m.cut_planes.size() - 1; c >= 0 && pl.size() < 4 * 7; --c) {
-      const Plane &q = m.cut_planes[c];
-      const double l = std::sqrt(q.a * q.a + q.b * q.b + q.c * q.c);
-      if (!(l > 0.0)) continue;
-      const double tol = 1e-9 * l;
-      mine.clear();
-      for (size_t t = 0; t < nT; ++t) {
-        if (grp[t] >= 0) continue;
-        const auto &tr = m.triangles[t];
-        if (std::fabs(q.eval(m.vertices[tr[0]])) <= tol && std::fabs(q.eval(m.vertices[tr[1]])) <= tol &&
-            std::fabs(q.eval(m.vertices[tr[2]])) <= tol)
-          mine.push_back(t);
-      }
-      if (mine.size() < 16) continue;
-      const int8_t g = (int8_t)(pl.size() / 4);
-      for (size_t t : mine) grp[t] = g;
-      pl.insert(pl.end(), {q.a, q.b, q.c, q.d});
-    }
-  }
+  if (planar && engine.planar_caps && !m.cut_planes.empty()) cap_groups(m, grp, pl);
   vacd_bvh *b = nullptr;
   if (!pl.empty()) {
     check(vacd_bvh_build_planar(ctx, V.data(), (int64_t)m.vertices.size(), T.data(), (int64_t)nT,
@@ -410,7 +418,7 @@ std::vector<Plane> bisector_planes(const Mesh &m, int num_planes, std::mt19937 &
     Vec3 n = p2 - p1;
     const double l = norm(n);
     if (l < 1e-6) continue;
-    n = n * (1.0 / l);
+    n = Vec3{n[0] / l, n[1] / l, n[2] / l};  // normalize_vector (lib/include/core.hpp:102-108)
     const Vec3 mid = (p1 + p2) * 0.5;
     const double d = -dot(n, mid);
     bool similar = false;
@@ -538,7 +546,7 @@ void add_visibility_batch(Engine &E, const std::vector<Mesh *> &parts, bool with
       for (int64_t i = 0; i < n; ++i) {
         const int v = part.points[i];
         for (int c = 0; c < 3; ++c) P[k][3 * i + c] = (float)part.vertices[v][c];
-        any_new |= (M[k][i] & 3) != 0;
+        any_new |= (M[k][i] & 1) != 0;
       }
       if (with_mask && !any_new) continue;
       idx.push_back(k);
@@ -609,31 +617,25 @@ void add_visibility_batch(Engine &E, const std::vector<Mesh *> &parts, bool with
   release();
 }
 
-// Concavities of the parts whose entry is NaN: hulls, volumes and surface
-// samples on all host threads, point-to-mesh distances on the GPU.
-// Host half of the concavity of the parts whose entry is NaN: hulls,
-// volumes, surface samples and the two distance queries of each part, on
-// all host threads.
+// Host half of the concavity of a list of parts: hulls, volumes, surface
+// samples and the two distance queries of each part, on all host threads.
 struct ConcavityPrep {
-  std::vector<size_t> idx;
   std::vector<double> rv, host_value;
   std::vector<char> gpu;  // 1: distances from the queries, 2: no samples
   std::vector<DistQuery> qs;  // 2 per part (empty for host-measured parts)
 };
 
-ConcavityPrep concavity_prep(const MeshList &parts, const std::vector<double> &conc) {
+ConcavityPrep concavity_prep(const std::vector<const Mesh *> &ms) {
   PhaseTimer host_timer("concavity.host");
   ConcavityPrep cp;
-  for (size_t i = 0; i < parts.size(); ++i)
-    if (std::isnan(conc[i])) cp.idx.push_back(i);
-  const size_t J = cp.idx.size();
+  const size_t J = ms.size();
   cp.rv.assign(J, 0.0);
   cp.host_value.assign(J, 0.0);
   cp.gpu.assign(J, 0);
   cp.qs.resize(2 * J);
 #pragma omp parallel for schedule(dynamic)
   for (long j = 0; j < (long)J; ++j) {
-    const Mesh &part = parts[cp.idx[j]];
+    const Mesh &part = *ms[j];
     const Mesh hull = convex_hull(part.vertices);
     if (!engine.gpu_distance || hull.triangles.empty()) {
       cp.host_value[j] = concavity(part, hull, engine.hausdorff_resolution, engine.sample_seed);
@@ -655,24 +657,19 @@ ConcavityPrep concavity_prep(const MeshList &parts, const std::vector<double> &c
 }
 
 // GPU half: every part's two one-sided distances in one batched call.
-void concavity_finish(const Engine &E, const ConcavityPrep &cp, std::vector<double> &conc) {
+std::vector<double> concavity_finish(const Engine &E, const ConcavityPrep &cp) {
   PhaseTimer gpu_timer("concavity.distance");
   const std::vector<double> d = max_distances(E, cp.qs);
-  for (size_t j = 0; j < cp.idx.size(); ++j) {
+  std::vector<double> out(cp.gpu.size());
+  for (size_t j = 0; j < cp.gpu.size(); ++j) {
     if (cp.gpu[j] == 0)
-      conc[cp.idx[j]] = cp.host_value[j];
+      out[j] = cp.host_value[j];
     else if (cp.gpu[j] == 2)
-      conc[cp.idx[j]] = std::max(0.3 * cp.rv[j], 1e300);
+      out[j] = std::max(0.3 * cp.rv[j], 1e300);
     else
-      conc[cp.idx[j]] = std::max(0.3 * cp.rv[j], std::max(d[2 * j], d[2 * j + 1]));
+      out[j] = std::max(0.3 * cp.rv[j], std::max(d[2 * j], d[2 * j + 1]));
   }
-}
-
-// Concavities of the parts whose entry is NaN.
-void concavity_batch(const Engine &E, const MeshList &parts, std::vector<double> &conc) {
-  PhaseTimer timer("concavity");
-  const ConcavityPrep cp = concavity_prep(parts, conc);
-  concavity_finish(E, cp, conc);
+  return out;
 }
 
 // ------------------------------------------------------- plane selection
@@ -850,274 +847,99 @@ std::vector<int> choose_planes(Engine &E, const std::vector<const Mesh *> &ms,
   return best;
 }
 
+// ------------------------------------------------------------ part lineage
+// Every part carries a 64-bit seed.  The parts of the input take
+// mix64(root + i) (root: two draws of the global generator, so set_seed
+// fixes a run); the k-th piece of a split (1-based over the pieces of both
+// halves, dropped ones included) takes kid_seed(parent, k).  A part's
+// candidate planes, point subsample and proposal draws come from its own
+// seed, so a part splits the same way whenever it is split: the plain loop
+// and the batched frontier, which splits parts in another order, make the
+// same parts.
+uint64_t kid_seed(uint64_t parent, uint32_t k) { return mix64(parent * 31 + k); }
+uint32_t point_seed(uint64_t s) { return (uint32_t)mix64(s ^ 0x9017ull); }
+// generator of the candidate planes of a part's attempt-th split (a failed
+// clip is retried with fresh planes, as the reference's next step draws
+// fresh ones from its global generator)
+uint32_t plane_seed(uint64_t s, int attempt) { return (uint32_t)mix64(s ^ (0x5eedull + (uint64_t)attempt)); }
+
 struct Driver {
   Engine E;
   std::vector<Plane> flat;
   PairBuffer pair_buf;
-  uint32_t next_seed;
-  uint64_t step = 0;  // splits so far (seeds the proposal's draws)
-  explicit Driver(int device) : E(device), next_seed(engine.sample_seed) {}
+  explicit Driver(int device) : E(device) {}
 
-  // returns true when nothing more can be split (no planes or no edges)
-  bool decompose_iteration(Mesh &mesh, MeshList &parts) {
-    std::vector<Plane> planes;
-    {
-      PhaseTimer timer("planes");
-      if (engine.planes == "sphere")
-        planes = sphere_planes(E, {&mesh}, {sphere_draws(engine.num_planes, random_engine())})[0];
-      else
-        planes = bisector_planes(mesh, engine.num_planes, random_engine());
+  // Candidate planes of each part (its own generator), then the flat-surface
+  // planes (lib/src/process.cpp:87-94); ss_offset[f] = first flat plane.
+  std::vector<std::vector<Plane>> candidates(const std::vector<const Mesh *> &ms,
+                                             const std::vector<uint64_t> &seeds,
+                                             const std::vector<int> &attempts,
+                                             std::vector<size_t> &ss_offset) {
+    PhaseTimer t("planes");
+    const size_t F = ms.size();
+    std::vector<std::vector<Plane>> planes(F);
+    std::vector<SphereDraws> dr(F);
+#pragma omp parallel for schedule(dynamic)
+    for (long f = 0; f < (long)F; ++f) {
+      std::mt19937 rng(plane_seed(seeds[f], attempts[f]));
+      if (engine.planes == "sphere") dr[f] = sphere_draws(engine.num_planes, rng);
+      else planes[f] = bisector_planes(*ms[f], engine.num_planes, rng);
     }
-    const size_t ss_offset = planes.size();
-    planes.insert(planes.end(), flat.begin(), flat.end());
-    if (planes.empty() || mesh.intersecting_edges.empty()) {
-      parts.push_back(std::move(mesh));
-      return true;
+    if (engine.planes == "sphere" && F) planes = sphere_planes(E, ms, dr);
+    ss_offset.assign(F, 0);
+    for (size_t f = 0; f < F; ++f) {
+      ss_offset[f] = planes[f].size();
+      planes[f].insert(planes[f].end(), flat.begin(), flat.end());
     }
-    size_t best = 0;
-    {
-      std::vector<std::vector<Plane>> pl(1);
-      pl[0].swap(planes);
-      const std::vector<int> b = choose_planes(E, {&mesh}, pl, {ss_offset},
-                                               {mix64((uint64_t)engine.sample_seed * 7919u + step++)});
-      planes.swap(pl[0]);
-      best = (size_t)b[0];
-    }
-    std::vector<int> mp, mn;
-    MeshList halves;
-    {
-      PhaseTimer timer("clip");
-      halves = clip(mesh, planes[best], mp, mn);
-    }
-    if (halves.size() < 2) {  // clipping failed
-      parts.push_back(std::move(mesh));
-      return false;
-    }
-    for (auto &e : mesh.intersecting_edges) {
-      if (mp[e.first] && mp[e.second])
-        halves[0].intersecting_edges.push_back({(unsigned)mp[e.first] - 1, (unsigned)mp[e.second] - 1});
-      if (mn[e.first] && mn[e.second])
-        halves[1].intersecting_edges.push_back({(unsigned)mn[e.first] - 1, (unsigned)mn[e.second] - 1});
-    }
-    for (int v : mesh.points) {
-      if (mp[v]) halves[0].points.push_back(mp[v] - 1);
-      if (mn[v]) halves[1].points.push_back(mn[v] - 1);
-    }
-    for (auto &h : halves) {
-      MeshList pieces;
-      {
-        PhaseTimer timer("separate");
-        pieces = separate_disjoint(h);
-      }
-      for (Mesh &part : pieces) {
-        if (part.vertices.size() < 10) continue;
-        part.cut_planes = mesh.cut_planes;
-        part.cut_planes.push_back(planes[best]);
-        choose_points(part, true, next_seed++);
-        add_visibility(E, part, true, pair_buf);
-        parts.push_back(std::move(part));
-      }
-    }
-    return false;
+    return planes;
   }
 };
 
-// Visibility of the fresh parts on a helper thread (BVH builds and GPU
-// passes) while the host threads prepare the concavity of every part whose
-// entry is NaN (the fresh ones), then their distances on the GPU.  The
-// helper only appends to the fresh parts' edge lists; the concavity reads
-// their vertices and triangles.
-void visibility_and_concavity(Driver &D, const std::vector<Mesh *> &fresh, bool with_mask,
-                              const MeshList &parts, std::vector<double> &conc) {
-  if (config.score_mode != "concavity") {
-    add_visibility_batch(D.E, fresh, with_mask, D.pair_buf);
-    return;
+// Clips a part by a plane (lib/src/process.cpp:136-168): the pieces of both
+// halves with at least 10 vertices, each with the parent's edges whose both
+// ends it keeps (1-based maps), its points, cut planes and seed.  False when
+// the clip fails.
+bool split_part(const Mesh &mesh, uint64_t seed, const Plane &plane, MeshList &kids,
+                std::vector<uint64_t> &kid_seeds) {
+  std::vector<int> mp, mn;
+  MeshList halves = clip(mesh, plane, mp, mn);
+  if (halves.size() < 2) return false;
+  for (auto &e : mesh.intersecting_edges) {
+    if (mp[e.first] && mp[e.second])
+      halves[0].intersecting_edges.push_back({(unsigned)mp[e.first] - 1, (unsigned)mp[e.second] - 1});
+    if (mn[e.first] && mn[e.second])
+      halves[1].intersecting_edges.push_back({(unsigned)mn[e.first] - 1, (unsigned)mn[e.second] - 1});
   }
-  std::exception_ptr err;
-  std::thread helper([&] {
-    try {
-      add_visibility_batch(D.E, fresh, with_mask, D.pair_buf);
-    } catch (...) {
-      err = std::current_exception();
+  for (int v : mesh.points) {
+    if (mp[v]) halves[0].points.push_back(mp[v] - 1);
+    if (mn[v]) halves[1].points.push_back(mn[v] - 1);
+  }
+  uint32_t k = 0;
+  for (auto &h : halves)
+    for (Mesh &piece : separate_disjoint(h)) {
+      ++k;
+      if (piece.vertices.size() < 10) continue;
+      piece.cut_planes = mesh.cut_planes;
+      piece.cut_planes.push_back(plane);
+      const uint64_t ks = kid_seed(seed, k);
+      choose_points(piece, true, point_seed(ks));
+      kids.push_back(std::move(piece));
+      kid_seeds.push_back(ks);
     }
-  });
-  ConcavityPrep cp;
-  try {
-    PhaseTimer timer("concavity");
-    cp = concavity_prep(parts, conc);
-  } catch (...) {
-    helper.join();
-    throw;
-  }
-  log_line("    concavity host work done");
-  helper.join();
-  log_line("    visibility done");
-  if (err) std::rethrow_exception(err);
-  PhaseTimer timer("concavity");
-  concavity_finish(D.E, cp, conc);
+  return true;
 }
 
-// Batched frontier: every part above the concavity threshold and within
-// engine.frontier_fraction of the worst part's concavity (up to the part
-// budget, worst first) is split in the same round.  Planes, clips, hulls and
-// samples run on all host threads; the GPU scores all those parts in one
-// call and computes the visibility of all new parts in one call.  Each part
-// draws its planes from its own generator, seeded from its parent's seed, so
-// the result does not depend on thread scheduling.
-ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, int num_parts) {
-  const std::vector<double> bbox = normalize(mesh);
-  log_line("Preprocessing mesh (" + std::to_string(mesh.vertices.size()) + " verts), batched...");
-  mesh.is_new.clear();
-  mesh.intersecting_edges.clear();
-  mesh.points.clear();
-  if (config.use_flat_surfaces) D.flat = flat_surface_planes(mesh, config.flat_surface_min_area);
-  log_line("  " + std::to_string(D.flat.size()) + " flat surface(s)");
-  MeshList parts = separate_disjoint(mesh);
-  std::vector<uint64_t> seed(parts.size());
-  std::vector<Mesh *> fresh;
-  for (size_t i = 0; i < parts.size(); ++i) {
-    seed[i] = mix64((uint64_t)engine.sample_seed * 0x100000001b3ull + i);
-    choose_points(parts[i], false, (uint32_t)seed[i]);
-    fresh.push_back(&parts[i]);
-  }
-  const double nan = std::numeric_limits<double>::quiet_NaN();
-  std::vector<double> conc(parts.size(), nan);
-  log_line("  " + std::to_string(parts.size()) + " connected part(s)");
-  visibility_and_concavity(D, fresh, false, parts, conc);
-  log_line("  initial visibility and concavity done");
-  std::vector<char> frozen(parts.size(), 0);  // could not be split
-  for (int round = 0;; ++round) {
-    const int budget = num_parts - (int)parts.size();
-    if (budget <= 0) break;
-    std::vector<std::pair<double, size_t>> cand;
-    if (config.score_mode == "edge") {
-      for (size_t i = 0; i < parts.size(); ++i)
-        if (!frozen[i] && !parts[i].intersecting_edges.empty())
-          cand.push_back({edge_score(parts[i]), i});
-    } else {
-      concavity_batch(D.E, parts, conc);
-      double worst = -1.0;
-      for (size_t i = 0; i < parts.size(); ++i)
-        if (!frozen[i]) worst = std::max(worst, conc[i]);
-      // the round takes the parts within frontier_fraction of the worst one
-      const double floor = std::max(concavity_threshold, engine.frontier_fraction * worst);
-      for (size_t i = 0; i < parts.size(); ++i)
-        if (!frozen[i] && conc[i] >= concavity_threshold && conc[i] >= floor)
-          cand.push_back({conc[i], i});
-    }
-    if (cand.empty()) break;
-    std::stable_sort(cand.begin(), cand.end(),
-                     [](const auto &a, const auto &b) { return a.first > b.first; });
-    if ((int)cand.size() > budget) cand.resize(budget);
-    const size_t F = cand.size();
-    log_line("Round " + std::to_string(round + 1) + ": splitting " + std::to_string(F) +
-             " part(s) (" + std::to_string(parts.size()) + " parts total).");
-    // candidate planes, each part from its own generator
-    std::vector<std::vector<Plane>> planes(F);
-    std::vector<size_t> ss_offset(F);
-    {
-      PhaseTimer t("planes");  // every part's draws, then all extents in one GPU call
-      std::vector<SphereDraws> dr(F);
-      std::vector<const Mesh *> ms(F);
-#pragma omp parallel for schedule(dynamic)
-      for (long f = 0; f < (long)F; ++f) {
-        const Mesh &m = parts[cand[f].second];
-        ms[f] = &m;
-        std::mt19937 rng((uint32_t)mix64(seed[cand[f].second] ^ 0x5eedull));
-        if (engine.planes == "sphere") dr[f] = sphere_draws(engine.num_planes, rng);
-        else planes[f] = bisector_planes(m, engine.num_planes, rng);
-      }
-      if (engine.planes == "sphere") planes = sphere_planes(D.E, ms, dr);
-      for (size_t f = 0; f < F; ++f) {
-        ss_offset[f] = planes[f].size();
-        planes[f].insert(planes[f].end(), D.flat.begin(), D.flat.end());
-      }
-    }
-    // one plane per part: every part's planes scored in one batched call
-    std::vector<int> best;
-    {
-      std::vector<const Mesh *> ms(F);
-      std::vector<uint64_t> ps(F);
-      for (size_t f = 0; f < F; ++f) ms[f] = &parts[cand[f].second], ps[f] = seed[cand[f].second];
-      best = choose_planes(D.E, ms, planes, ss_offset, ps);
-    }
-    log_line("  planes chosen");
-    // clips, edge inheritance and connected components on all host threads
-    std::vector<MeshList> kids(F);
-    std::vector<char> split_ok(F, 0);
-    {
-      PhaseTimer t("clip");
-#pragma omp parallel for schedule(dynamic)
-      for (long f = 0; f < (long)F; ++f) {
-        if (best[f] < 0) continue;
-        const Mesh &mesh_f = parts[cand[f].second];
-        std::vector<int> mp, mn;
-        MeshList halves = clip(mesh_f, planes[f][best[f]], mp, mn);
-        if (halves.size() < 2) continue;
-        for (auto &e : mesh_f.intersecting_edges) {
-          if (mp[e.first] && mp[e.second])
-            halves[0].intersecting_edges.push_back({(unsigned)mp[e.first] - 1, (unsigned)mp[e.second] - 1});
-          if (mn[e.first] && mn[e.second])
-            halves[1].intersecting_edges.push_back({(unsigned)mn[e.first] - 1, (unsigned)mn[e.second] - 1});
-        }
-        for (int v : mesh_f.points) {
-          if (mp[v]) halves[0].points.push_back(mp[v] - 1);
-          if (mn[v]) halves[1].points.push_back(mn[v] - 1);
-        }
-        const uint64_t s0 = seed[cand[f].second];
-        uint32_t k = 0;
-        for (auto &h : halves)
-          for (Mesh &piece : separate_disjoint(h)) {
-            ++k;
-            if (piece.vertices.size() < 10) continue;
-            piece.cut_planes = mesh_f.cut_planes;
-            piece.cut_planes.push_back(planes[f][best[f]]);
-            choose_points(piece, true, (uint32_t)mix64(s0 + k));
-            kids[f].push_back(std::move(piece));
-          }
-        split_ok[f] = 1;
-      }
-    }
-    // next generation: parts not split keep their slot, split ones are replaced
-    std::vector<char> is_cand(parts.size(), 0);
-    for (size_t f = 0; f < F; ++f) is_cand[cand[f].second] = split_ok[f] ? 1 : 2;
-    MeshList next;
-    std::vector<uint64_t> nseed;
-    std::vector<double> nconc;
-    std::vector<char> nfrozen;
-    for (size_t i = 0; i < parts.size(); ++i) {
-      if (is_cand[i] == 1) continue;
-      next.push_back(std::move(parts[i]));
-      nseed.push_back(seed[i]);
-      nconc.push_back(conc[i]);
-      nfrozen.push_back(is_cand[i] == 2 ? 1 : frozen[i]);
-    }
-    const size_t first_kid = next.size();
-    for (size_t f = 0; f < F; ++f) {
-      uint32_t k = 0;
-      for (Mesh &c : kids[f]) {
-        next.push_back(std::move(c));
-        nseed.push_back(mix64(seed[cand[f].second] * 31 + (++k)));
-        nconc.push_back(nan);
-        nfrozen.push_back(0);
-      }
-    }
-    log_line("  parts clipped");
-    parts = std::move(next);
-    seed = std::move(nseed);
-    conc = std::move(nconc);
-    frozen = std::move(nfrozen);
-    fresh.clear();
-    for (size_t i = first_kid; i < parts.size(); ++i) fresh.push_back(&parts[i]);
-    bool any = false;
-    for (size_t f = 0; f < F; ++f) any |= split_ok[f] != 0;
-    if (!any) break;
-    // a full part budget ends the loop: the last parts need no edges and
-    // no concavity (the final concavity is measured on their hulls below)
-    if ((int)parts.size() >= num_parts) break;
-    visibility_and_concavity(D, fresh, true, parts, conc);
-    log_line("  visibility and concavity of " + std::to_string(fresh.size()) + " new part(s) done");
-  }
+// Index of the part the loop splits next (lib/src/process.cpp:286-299): the
+// first maximum of the concavity (or of the total edge length); -1 for none.
+int pick_part(const std::vector<double> &key) {
+  int idx = -1;
+  double best = -1.0;
+  for (int i = 0; i < (int)key.size(); ++i)
+    if (key[i] > best) best = key[i], idx = i;
+  return idx;
+}
+
+ProcessResult finish(Driver &D, MeshList parts, const std::vector<double> &bbox) {
   log_line("Computing convex hulls for " + std::to_string(parts.size()) + " parts...");
   PhaseTimer final_timer("final_hulls");
   MeshList hulls(parts.size());
@@ -1127,6 +949,8 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
   fh_timer.stop();
   PhaseTimer fd_timer("final_hulls.distance");
   const double final_h = final_concavity(D.E, parts, hulls);
+  fd_timer.stop();
+  if (config.use_merging) log_line("Hull merging is not part of this engine; parts are kept.");
   for (auto &h : hulls) unnormalize(h, bbox);
   for (auto &p : parts) unnormalize(p, bbox);
   ProcessResult r;
@@ -1141,14 +965,10 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
   return r;
 }
 
-}  // namespace
-
-ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
-  last_profile.clear();
-  process_start = std::chrono::steady_clock::now();
-  PhaseTimer total("total");
-  Driver D(engine.device);
-  if (engine.mode == "batched") return process_batched(D, std::move(mesh), concavity_threshold, num_parts);
+// The reference's loop (lib/src/process.cpp:238-347): one part per step,
+// its planes scored, clipped, and the new parts' visibility computed at once.
+ProcessResult process_plain(Driver &D, Mesh mesh, double concavity_threshold, int num_parts,
+                            uint64_t root) {
   const std::vector<double> bbox = normalize(mesh);
   log_line("Preprocessing mesh (" + std::to_string(mesh.vertices.size()) + " verts)...");
   mesh.is_new.clear();
@@ -1159,71 +979,316 @@ ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
     log_line("Detected " + std::to_string(D.flat.size()) + " flat surface(s).");
   }
   MeshList parts = separate_disjoint(mesh);
-  for (auto &part : parts) {
-    choose_points(part, false, D.next_seed++);
-    add_visibility(D.E, part, false, D.pair_buf);
+  std::vector<uint64_t> seed;
+  std::vector<int> attempt;
+  for (size_t i = 0; i < parts.size(); ++i) {
+    seed.push_back(mix64(root + i));
+    attempt.push_back(0);
+    choose_points(parts[i], false, point_seed(seed[i]));
+    add_visibility(D.E, parts[i], false, D.pair_buf);
   }
   std::ostringstream hdr;
   hdr << "Starting decomposition (max parts=" << num_parts << ", concavity threshold="
       << std::to_string(concavity_threshold) << ", mode=" << config.score_mode << ").";
   log_line(hdr.str());
-  // cached concavity per part (NaN: not computed yet)
   const double nan = std::numeric_limits<double>::quiet_NaN();
-  std::vector<double> conc(parts.size(), nan);
+  std::vector<double> conc(parts.size(), nan);  // cached per part
   for (int step = 0; step < num_parts - 1; ++step) {
-    int idx = -1;
+    std::vector<double> key(parts.size());
     if (config.score_mode == "edge") {
-      double best = -1.0;
-      for (int i = 0; i < (int)parts.size(); ++i) {
-        const double s = edge_score(parts[i]);
-        if (s > best) best = s, idx = i;
-      }
+      for (size_t i = 0; i < parts.size(); ++i) key[i] = edge_score(parts[i]);
     } else if (config.score_mode == "concavity") {
-      double best = -1.0;
-      for (int i = 0; i < (int)parts.size(); ++i) {
+      for (size_t i = 0; i < parts.size(); ++i) {
         if (std::isnan(conc[i])) conc[i] = part_concavity(D.E, parts[i]);
-        if (conc[i] > best) best = conc[i], idx = i;
+        key[i] = conc[i];
       }
-      if (best < concavity_threshold) {
-        log_line("Step " + std::to_string(step + 1) + ": concavity " +
-                 std::to_string(best).substr(0, 6) + " < threshold, stopping.");
-        break;
-      }
+    }
+    const int idx = pick_part(key);
+    if (config.score_mode == "concavity" && idx >= 0 && key[idx] < concavity_threshold) {
+      log_line("Step " + std::to_string(step + 1) + ": concavity " +
+               std::to_string(key[idx]).substr(0, 6) + " < threshold, stopping.");
+      break;
     }
     if (idx < 0) break;
     log_line("Step " + std::to_string(step + 1) + "/" + std::to_string(num_parts - 1) +
              ": splitting part " + std::to_string(idx) + " (" + std::to_string(parts.size()) +
              " parts total).");
     Mesh part = std::move(parts[idx]);
+    const uint64_t s = seed[idx];
+    const int at = attempt[idx];
+    const double c = conc[idx];
     parts.erase(parts.begin() + idx);
+    seed.erase(seed.begin() + idx);
+    attempt.erase(attempt.begin() + idx);
     conc.erase(conc.begin() + idx);
-    const size_t before = parts.size();
-    const bool stop = D.decompose_iteration(part, parts);
-    conc.resize(parts.size(), nan);
-    for (size_t i = before; i < parts.size(); ++i) conc[i] = nan;
-    if (stop) {
+    std::vector<size_t> ss;
+    std::vector<std::vector<Plane>> planes = D.candidates({&part}, {s}, {at}, ss);
+    if (planes[0].empty() || part.intersecting_edges.empty()) {
+      parts.push_back(std::move(part));
+      seed.push_back(s), attempt.push_back(at), conc.push_back(c);
       log_line("No more intersecting edges, stopping early.");
       break;
     }
+    const int best = choose_planes(D.E, {&part}, planes, ss, {s})[0];
+    MeshList kids;
+    std::vector<uint64_t> ks;
+    bool ok;
+    {
+      PhaseTimer timer("clip");
+      ok = split_part(part, s, planes[0][best], kids, ks);
+    }
+    if (!ok) {  // clipping failed: the part goes back, to be retried with fresh planes
+      parts.push_back(std::move(part));
+      seed.push_back(s), attempt.push_back(at + 1), conc.push_back(c);
+      continue;
+    }
+    for (size_t k = 0; k < kids.size(); ++k) {
+      add_visibility(D.E, kids[k], true, D.pair_buf);
+      parts.push_back(std::move(kids[k]));
+      seed.push_back(ks[k]), attempt.push_back(0), conc.push_back(nan);
+    }
   }
-  log_line("Computing convex hulls for " + std::to_string(parts.size()) + " parts...");
-  PhaseTimer final_timer("final_hulls");
-  MeshList hulls;
-  for (auto &p : parts) hulls.push_back(convex_hull(p.vertices));
-  const double final_h = final_concavity(D.E, parts, hulls);
-  if (config.use_merging) log_line("Hull merging is not part of this engine; parts are kept.");
-  for (auto &h : hulls) unnormalize(h, bbox);
-  for (auto &p : parts) unnormalize(p, bbox);
-  ProcessResult r;
-  r.parts = config.return_parts ? parts : hulls;
-  r.concavity = final_h;
-  r.num_parts = (int)r.parts.size();
-  std::ostringstream s;
-  s.setf(std::ios::fixed);
-  s.precision(4);
-  s << "Done. parts=" << r.num_parts << "  concavity=" << final_h;
-  log_line(s.str());
-  return r;
+  return finish(D, std::move(parts), bbox);
+}
+
+// ------------------------------------------------------------- batched
+// The batched frontier gives exactly the plain loop's decomposition.  It
+// keeps the tree of parts split so far and replays the plain loop on it
+// (which part is split at each step, in which order parts are listed); when
+// the replay reaches a part not yet split, it splits in one round that part
+// and, speculatively, the other unsplit parts the loop is likely to split
+// next (the highest concavities of its list, up to engine.batch_width and
+// the steps left): their visibility, planes, scores, clips and the new
+// parts' concavities each in one batched call or on all host threads.  A
+// speculative split the loop never reaches is discarded work; parts split
+// the same way whenever they are split, so the result does not depend on
+// which parts a round took.
+struct Node {
+  Mesh mesh;
+  uint64_t seed = 0;
+  int attempt = 0;
+  double conc = std::numeric_limits<double>::quiet_NaN();
+  int state = 0;  // 0 not split yet, 1 split (kids), 2 nothing to split (no planes or edges)
+  std::vector<int> kids;
+  bool edges = false;  // visibility of its new pairs computed
+};
+
+// Visibility (masked when with_mask) of the listed nodes on a helper thread,
+// overlapped with the host half of the concavity of the nodes in `conc_of`;
+// then their distances on the GPU.
+void visibility_and_concavity(Driver &D, std::vector<Node> &nodes, const std::vector<int> &vis_of,
+                              bool with_mask, const std::vector<int> &conc_of) {
+  std::vector<Mesh *> vm;
+  for (int x : vis_of) vm.push_back(&nodes[x].mesh);
+  std::vector<const Mesh *> cm;
+  for (int x : conc_of) cm.push_back(&nodes[x].mesh);
+  std::exception_ptr err;
+  std::thread helper;
+  if (!vm.empty())
+    helper = std::thread([&] {
+      try {
+        add_visibility_batch(D.E, vm, with_mask, D.pair_buf);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+  ConcavityPrep cp;
+  try {
+    PhaseTimer timer("concavity");
+    if (!cm.empty()) cp = concavity_prep(cm);
+  } catch (...) {
+    if (helper.joinable()) helper.join();
+    throw;
+  }
+  if (helper.joinable()) helper.join();
+  if (err) std::rethrow_exception(err);
+  for (int x : vis_of) nodes[x].edges = true;
+  if (!cm.empty()) {
+    PhaseTimer timer("concavity");
+    const std::vector<double> c = concavity_finish(D.E, cp);
+    for (size_t j = 0; j < conc_of.size(); ++j) nodes[conc_of[j]].conc = c[j];
+  }
+}
+
+ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, int num_parts,
+                              uint64_t root) {
+  const std::vector<double> bbox = normalize(mesh);
+  log_line("Preprocessing mesh (" + std::to_string(mesh.vertices.size()) + " verts), batched...");
+  mesh.is_new.clear();
+  mesh.intersecting_edges.clear();
+  mesh.points.clear();
+  if (config.use_flat_surfaces) D.flat = flat_surface_planes(mesh, config.flat_surface_min_area);
+  log_line("  " + std::to_string(D.flat.size()) + " flat surface(s)");
+  const bool by_edges = config.score_mode == "edge";
+  const bool by_conc = config.score_mode == "concavity";
+  std::vector<Node> nodes;
+  std::vector<int> L;  // the plain loop's list of parts
+  {
+    MeshList roots = separate_disjoint(mesh);
+    for (size_t i = 0; i < roots.size(); ++i) {
+      Node nd;
+      nd.mesh = std::move(roots[i]);
+      nd.seed = mix64(root + i);
+      choose_points(nd.mesh, false, point_seed(nd.seed));
+      nodes.push_back(std::move(nd));
+      L.push_back((int)i);
+    }
+  }
+  log_line("  " + std::to_string(L.size()) + " connected part(s)");
+  visibility_and_concavity(D, nodes, L, false, by_conc ? L : std::vector<int>{});
+  auto key_of = [&](int x) {
+    if (by_edges) return edge_score(nodes[x].mesh);
+    if (by_conc) return nodes[x].conc;
+    return 0.0;
+  };
+  int step = 0, round = 0;
+  for (;;) {
+    // replay the plain loop as far as the tree goes
+    int need = -1;
+    while (step < num_parts - 1) {
+      std::vector<double> key(L.size());
+      for (size_t i = 0; i < L.size(); ++i) key[i] = key_of(L[i]);
+      const int idx = pick_part(key);
+      if (idx < 0 || (by_conc && key[idx] < concavity_threshold)) break;
+      const int x = L[idx];
+      if (nodes[x].state == 0) {
+        need = x;
+        break;
+      }
+      L.erase(L.begin() + idx);
+      if (nodes[x].state == 2) {  // no planes or no edges: the loop stops
+        L.push_back(x);
+        step = num_parts;
+        break;
+      }
+      for (int k : nodes[x].kids) L.push_back(k);
+      nodes[x].mesh = Mesh();  // split for good: its mesh is not needed again
+      ++step;
+    }
+    if (need < 0) break;
+    // this round: the part the loop needs and the unsplit parts most likely next
+    const int left = num_parts - 1 - step;
+    const int width = std::max(1, std::min(engine.batch_width, left));
+    std::vector<std::pair<double, int>> cand;
+    for (int x : L)
+      if (x != need && nodes[x].state == 0 && (!by_conc || nodes[x].conc >= concavity_threshold))
+        cand.push_back({key_of(x), x});
+    std::stable_sort(cand.begin(), cand.end(),
+                     [](const auto &p, const auto &q) { return p.first > q.first; });
+    std::vector<int> batch{need};
+    for (size_t c = 0; c < cand.size() && (int)batch.size() < width; ++c) batch.push_back(cand[c].second);
+    ++round;
+    log_line("Round " + std::to_string(round) + ": step " + std::to_string(step + 1) + ", splitting " +
+             std::to_string(batch.size()) + " part(s) (" + std::to_string(L.size()) + " listed).");
+    // visibility of the parts whose new pairs are still untested
+    std::vector<int> novis;
+    for (int x : batch)
+      if (!nodes[x].edges) novis.push_back(x);
+    if (!novis.empty()) visibility_and_concavity(D, nodes, novis, true, {});
+    // planes and the cut of every part, in batched calls
+    const size_t F = batch.size();
+    std::vector<const Mesh *> ms(F);
+    std::vector<uint64_t> seeds(F);
+    std::vector<int> attempts(F);
+    for (size_t f = 0; f < F; ++f) {
+      ms[f] = &nodes[batch[f]].mesh;
+      seeds[f] = nodes[batch[f]].seed;
+      attempts[f] = nodes[batch[f]].attempt;
+    }
+    std::vector<size_t> ss;
+    const std::vector<std::vector<Plane>> planes = D.candidates(ms, seeds, attempts, ss);
+    std::vector<int> best(F, -1);
+    {
+      std::vector<const Mesh *> live_m;
+      std::vector<std::vector<Plane>> live_p;
+      std::vector<size_t> live_ss, live_f;
+      std::vector<uint64_t> live_s;
+      for (size_t f = 0; f < F; ++f) {
+        if (planes[f].empty() || ms[f]->intersecting_edges.empty()) continue;
+        live_m.push_back(ms[f]), live_p.push_back(planes[f]), live_ss.push_back(ss[f]);
+        live_s.push_back(seeds[f]), live_f.push_back(f);
+      }
+      const std::vector<int> b = choose_planes(D.E, live_m, live_p, live_ss, live_s);
+      for (size_t j = 0; j < live_f.size(); ++j) best[live_f[j]] = b[j];
+    }
+    // clips on all host threads
+    std::vector<MeshList> kids(F);
+    std::vector<std::vector<uint64_t>> kseeds(F);
+    std::vector<char> ok(F, 0);
+    {
+      PhaseTimer t("clip");
+#pragma omp parallel for schedule(dynamic)
+      for (long f = 0; f < (long)F; ++f)
+        if (best[f] >= 0) ok[f] = split_part(*ms[f], seeds[f], planes[f][best[f]], kids[f], kseeds[f]);
+    }
+    std::vector<int> fresh;
+    for (size_t f = 0; f < F; ++f) {
+      const int x = batch[f];
+      if (best[f] < 0) {
+        nodes[x].state = 2;
+        continue;
+      }
+      nodes[x].state = 1;
+      if (!ok[f]) {  // clipping failed: retried later with fresh planes
+        Node retry;
+        retry.mesh = std::move(nodes[x].mesh);
+        retry.seed = nodes[x].seed;
+        retry.attempt = nodes[x].attempt + 1;
+        retry.conc = nodes[x].conc;
+        retry.edges = true;
+        nodes.push_back(std::move(retry));
+        nodes[x].kids = {(int)nodes.size() - 1};
+        continue;
+      }
+      for (size_t k = 0; k < kids[f].size(); ++k) {
+        Node kid;
+        kid.mesh = std::move(kids[f][k]);
+        kid.seed = kseeds[f][k];
+        nodes.push_back(std::move(kid));
+        nodes[x].kids.push_back((int)nodes.size() - 1);
+        fresh.push_back((int)nodes.size() - 1);
+      }
+    }
+    log_line("  parts clipped: " + std::to_string(fresh.size()) + " new");
+    // the new parts' concavity; their visibility now (overlapped) when
+    // engine.eager_visibility or when the edge score needs it
+    std::vector<int> vis_now;
+    if (engine.eager_visibility || by_edges) vis_now = fresh;
+    visibility_and_concavity(D, nodes, vis_now, true, by_conc ? fresh : std::vector<int>{});
+    log_line("  visibility and concavity done");
+  }
+  MeshList parts;
+  for (int x : L) parts.push_back(std::move(nodes[x].mesh));
+  last_profile.count("batched.rounds", (long)round);
+  last_profile.count("batched.parts_split", (long)std::count_if(nodes.begin(), nodes.end(),
+                                                                [](const Node &n) { return n.state == 1; }));
+  return finish(D, std::move(parts), bbox);
+}
+
+}  // namespace
+
+std::vector<Plane> bisector_planes_seeded(const Mesh &m, int num_planes, uint32_t seed) {
+  std::mt19937 rng(seed);
+  return bisector_planes(m, num_planes, rng);
+}
+
+int select_plane(const std::vector<int64_t> &q32, size_t ss_offset) {
+  std::vector<int> idx(q32.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  return first_max(idx, q32, ss_offset);
+}
+
+ProcessResult process(Mesh mesh, double concavity_threshold, int num_parts) {
+  last_profile.clear();
+  process_start = std::chrono::steady_clock::now();
+  PhaseTimer total("total");
+  // the run's root seed: two draws of the global generator (set_seed)
+  const uint64_t root = ((uint64_t)random_engine()() << 32) | (uint64_t)random_engine()();
+  Driver D(engine.device);
+  if (engine.mode == "batched")
+    return process_batched(D, std::move(mesh), concavity_threshold, num_parts, root);
+  if (engine.mode != "plain") throw std::invalid_argument("visacd: engine.mode must be \"plain\" or \"batched\"");
+  return process_plain(D, std::move(mesh), concavity_threshold, num_parts, root);
 }
 
 }  // namespace vhost

@@ -38,7 +38,8 @@ static_assert(sizeof(Node4) == 128, "node4 is 128 bytes");
 // reject it (an inverted box would not be rejected by the slab test).
 constexpr float kEmpty = 1e30f;
 
-// Leaf-ordered triangle (48 B): a.xyz + original id bits, b.xyz, c.xyz.
+// Leaf-ordered triangle (48 B): a.xyz + original id bits, b.xyz, c.xyz +
+// planar group code bits (0 none, g + 1: group g; plain tree only).
 struct alignas(16) Tri {
   float4 a, b, c;
 };
@@ -254,6 +255,10 @@ struct vacd_bvh {
   // that part of the segment only.
   int n_groups = 0;
   vacd::DevBuf groups;
+  // Projection of each planar group's plane for 2-D tests within it: 2 bits
+  // per group g at bit 2g, the axis dropped (the largest component of the
+  // plane normal: 0 x, 1 y, 2 z); 3 for a group dropped by its tolerance.
+  uint32_t group_axis = 0xffffffffu;
   // Inside grid (closed meshes): cells wholly inside the solid.
   bool has_grid = false;
   vacd::GridBox grid{};

@@ -309,9 +309,12 @@ __global__ __launch_bounds__(256) void k_small_first(Node *__restrict__ nodes, i
   nodes[i] = nd;
 }
 
+// Leaf-ordered triangles; c.w holds the triangle's planar group code (0
+// none, g + 1 for group g) as integer bits.
 __global__ __launch_bounds__(256) void k_leaves(const float *__restrict__ V,
                                                 const uint32_t *__restrict__ T,
                                                 const uint32_t *__restrict__ order, int64_t n,
+                                                const uint8_t *__restrict__ grp,
                                                 Tri *__restrict__ tris) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(256) void k_leaves(const float *__restrict__ V,
   Tri r;
   r.a = make_float4(a[0], a[1], a[2], __uint_as_float(t));
   r.b = make_float4(b[0], b[1], b[2], 0.0f);
-  r.c = make_float4(c[0], c[1], c[2], 0.0f);
+  r.c = make_float4(c[0], c[1], c[2], __uint_as_float(grp ? (unsigned)grp[t] : 0u));
   tris[j] = r;
 }
 
@@ -1013,7 +1016,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
     VACD_HIP(hipMemsetAsync(nodes, 0, sizeof(Node) * (size_t)n_int, st));
     k_karras<<<grid_for(nT - 1, 256), 256, 0, st>>>(k0, nT, nodes, pint, pleaf, larger0);
   }
-  k_leaves<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, v0, nT, tris);
+  k_leaves<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, v0, nT, d_grp, tris);
   k_refit<<<grid_for(nT, 256), 256, 0, st>>>(tris, nT, nodes, pint, pleaf, flags, bounds, root,
                                              nullptr, nullptr, nullptr);
   k_depth<<<grid_for(nT, 256), 256, 0, st>>>(nT, pint, pleaf, depth);

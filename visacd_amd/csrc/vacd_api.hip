@@ -186,9 +186,19 @@ int vacd_ctx_timing(vacd_ctx *ctx, vacd_timing *out, int reset) {
 // (float4 pairs (n, d), (tau, 0, 0, 0)) from host arrays; groups whose
 // triangles do not lie within 1e-4 of the extent of their plane are dropped.
 // Returns the number of triangles left in a group.
+// Axis dropped by the 2-D projection of a plane with normal (a, b, c): the
+// largest |component|, ties to the lower axis.
+static int plane_axis(const double *p) {
+  const double a = std::fabs(p[0]), b = std::fabs(p[1]), c = std::fabs(p[2]);
+  int k = b > a ? 1 : 0;
+  if (c > std::max(a, b)) k = 2;
+  return k;
+}
+
 static int64_t planar_groups(const float *V, int64_t nV, const uint32_t *T, int64_t nT,
                              const int8_t *tri_group, const double *planes, int n_groups,
-                             std::vector<uint8_t> &code, std::vector<float> &table) {
+                             std::vector<uint8_t> &code, std::vector<float> &table,
+                             uint32_t &axis) {
   VACD_REQUIRE(n_groups >= 0 && n_groups <= 7, VACD_ERR_INVALID, "at most 7 planar groups");
   VACD_REQUIRE(n_groups == 0 || (tri_group && planes), VACD_ERR_INVALID, "NULL group arrays");
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -225,7 +235,12 @@ static int64_t planar_groups(const float *V, int64_t nV, const uint32_t *T, int6
     code[t] = (uint8_t)(g + 1);
     ++kept;
   }
+  axis = 0xffffffffu;
   for (int g = 0; g < n_groups; ++g) {
+    if (tau[g] <= 1e-4 * ext) {
+      axis &= ~(3u << (2 * g));
+      axis |= (uint32_t)plane_axis(planes + 4 * g) << (2 * g);
+    }
     for (int k = 0; k < 4; ++k) table[8 * g + k] = nf[4 * g + k];
     // rounded up: the slab must hold every vertex of the group
     table[8 * g + 4] = std::nextafter((float)(tau[g] * (1.0 + 1e-6)), INFINITY);
@@ -256,12 +271,15 @@ static int bvh_build_impl(vacd_ctx *ctx, const float *V, int64_t nV, const uint3
     }
     std::vector<uint8_t> code;
     std::vector<float> table;
+    uint32_t axis = 0xffffffffu;
     const int64_t grouped =
-        n_groups > 0 ? planar_groups(V, nV, T, nT, tri_group, planes, n_groups, code, table) : 0;
+        n_groups > 0 ? planar_groups(V, nV, T, nT, tri_group, planes, n_groups, code, table, axis)
+                     : 0;
     b = new vacd_bvh();
     b->device = ctx->device;
     const uint8_t *dgrp = grouped > 0 ? stage(ctx, SLOT_GROUPS, code.data(), code.size()) : nullptr;
     vacd::build_lbvh(ctx, dV, nV, dT, nT, b, dgrp);
+    if (dgrp) b->group_axis = axis;
     if (dgrp && b->n_refs > 0) {
       b->groups.ensure(4 * table.size());
       VACD_HIP(hipMemcpyAsync(b->groups.ptr, table.data(), 4 * table.size(), hipMemcpyHostToDevice,

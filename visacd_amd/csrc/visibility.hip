@@ -596,7 +596,7 @@ struct alignas(16) FanRec {
   double e1y, e1z, e2y, e2z;
   float v1[3], v2[3];
   int o0;  // sign of orient2d(p, v1, v2) in (y, z): 1, -1, 0 undecided
-  int pad;
+  int grp;  // planar group code of the triangle: 0 none, g + 1 for group g
 };
 static_assert(sizeof(FanRec) == 112, "fan record is 112 bytes");
 
@@ -703,6 +703,7 @@ struct Part {
   const Node *cnodes;  // cage (NULL: parity mode)
   const Node4 *cnodes4;
   const Tri *ctris;
+  uint32_t group_axis;  // per planar group of the self mesh, its projection axis (vacd_bvh)
   int64_t n, S;
   int64_t word_begin;   // first ballot word of the part in the batch
   int64_t point_begin;  // first point of the part in the batch
@@ -731,6 +732,11 @@ struct VisArgs {
   int64_t *dlist;             // global pair ids whose walks met an undecided test
   unsigned long long *dlist_len;
   int64_t front_max;          // walk queues up to this long go to k_walk_front
+  // which kernel redoes the first kExactSplit deferred entries a wave per
+  // pair: true, k_walk_front (breadth first; it marks what it settles);
+  // false, k_walk_split with LV = 6 (depth first).  The lane-per-pair
+  // instance of k_walk_split takes the rest, or every unsettled entry.
+  bool exact_front = false;
 };
 
 // Part owning global ballot word w (binary search over word_begin).
@@ -833,12 +839,33 @@ __device__ __forceinline__ bool meets_inside(const Part &P, const Seg &s) {
   return false;
 }
 
-// Mask of a point: bit 0 new, bit 1 new and on the last cut's cap, bits 2-7
-// on one of six older cut planes.  A pair is skipped when neither point is
-// new (lib/ptx/ray_segments.cu:61-75) or when both lie on one cut plane: its
-// segment then lies in that plane, where only rounding decides contact.
-__device__ __forceinline__ bool mask_skips(uint8_t a, uint8_t b) {
-  return (!(a & 3) && !(b & 3)) || ((a >> 1) & (b >> 1)) != 0;
+// Mask of a point: bit 0 new (lib/ptx/ray_segments.cu:61-75: a pair is
+// skipped when neither point is new); bit 1 + g: the point lies on the
+// plane of planar group g of the self mesh.
+__device__ __forceinline__ bool mask_skips(uint8_t a, uint8_t b) { return !((a | b) & 1); }
+
+// Whether direction q - p enters, at p, the closed 2-D angle of a triangle
+// of p's fan that belongs to one of the planar groups in `groups` (bit g:
+// group g), in the group plane's projection (exact orientations).  Both
+// points lie on that plane, so the segment then runs into the group's
+// triangles (the cap a cut left) and touches the surface.
+__device__ __forceinline__ bool enters_cap(const Part &P, int64_t i, const float p[3],
+                                           const float q[3], unsigned groups) {
+  const int fn = P.fan_n[i];
+  const FanRec *fr = P.fan_rec + (size_t)i * kFanMax;
+  for (int f = 0; f < fn; ++f) {
+    const int code = fr[f].grp;
+    if (!code || !((groups >> (code - 1)) & 1u)) continue;
+    const int k = (int)((P.group_axis >> (2 * (code - 1))) & 3u);
+    if (k == 3) continue;
+    const float v1[3] = {fr[f].v1[0], fr[f].v1[1], fr[f].v1[2]};
+    const float v2[3] = {fr[f].v2[0], fr[f].v2[1], fr[f].v2[2]};
+    const int o = orient2d_exact(p, v1, v2, k);
+    if (o == 0) continue;
+    const int o1 = orient2d_exact(p, v1, q, k), o2 = orient2d_exact(p, q, v2, k);
+    if ((o1 != 0 || o2 != 0) && o * o1 >= 0 && o * o2 >= 0) return true;
+  }
+  return false;
 }
 
 // Appends `value` of every lane with `mine` set to a list, in lane order,
@@ -880,6 +907,15 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
         for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
         bool decided = same3(s.p, s.q);  // zero length: not an edge
         int pstate = 2, qstate = 2;  // work counters: 1 outwards, 2 no fan, 3 undecided
+        if (!decided && P.fan_n && !(CAGE && P.ctris) && P.mask) {
+          // both points on one planar group's plane (a cut's cap): a pair
+          // that runs into the cap at either end touches it
+          const unsigned shared = (unsigned)(P.mask[i] & P.mask[j]) >> 1;
+          if (shared && (enters_cap(P, i, s.p, s.q, shared) || enters_cap(P, j, s.q, s.p, shared))) {
+            ++n_blocked;
+            decided = true;
+          }
+        }
         if (!decided && P.fan_n && !(CAGE && P.ctris)) {
           // A pair that leaves p into the solid is blocked or wholly inside:
           // never an edge.  Decide that from p's fan alone (every lane of a
@@ -1035,10 +1071,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
   const int64_t *list = EXACT ? a.dlist : a.queue;
   const int64_t len = (int64_t)(EXACT ? *a.dlist_len : *a.queue_len);
   if (!EXACT && len <= a.front_max) return;  // k_walk_front took this queue
-  // with the breadth-first exact walk (a.front_max >= 0) the lane-per-pair
+  // with the breadth-first exact walk (a.exact_front) the lane-per-pair
   // instance takes every entry it left (kSettled clear), the wave-per-pair
   // one is not launched
-  const int64_t e_lo = (EXACT && LV == 0 && a.front_max < 0) ? kExactSplit : 0;
+  const int64_t e_lo = (EXACT && LV == 0 && !a.exact_front) ? kExactSplit : 0;
   const int64_t e_hi = (EXACT && LV == 6) ? (len < kExactSplit ? len : kExactSplit) : len;
   for (int64_t e = e_lo + (int64_t)blockIdx.x * kGroups + slot; e < e_hi;
        e += (int64_t)gridDim.x * kGroups) {
@@ -1405,7 +1441,7 @@ __device__ __forceinline__ void fan_finish(const Part &P, int64_t g, const float
     double b0;
     const double o0 = o2yz(p, v1, v2, b0);
     r.o0 = o0 > b0 ? 1 : (-o0 > b0 ? -1 : 0);
-    r.pad = 0;
+    r.grp = (int)__float_as_uint(t.c.w);
     fan_rec[(size_t)g * kFanMax + f] = r;
   }
   // the cones: axis = sum of the unit normals, each turned to the first's side
@@ -2097,6 +2133,7 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
     P.rcap = P.scap;
   }
   P.ccap = cage ? (int)std::min<int64_t>(2 * cage->n_tris + 64, INT32_MAX) : 0;
+  P.group_axis = self->group_axis;
   return P;
 }
 
@@ -2224,14 +2261,17 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_VIS);
   // the deferred pairs (the list length is read on the device): a wave per
-  // pair for the first kExactSplit, a lane per pair for the rest
+  // pair for the first kExactSplit, a lane per pair for the rest.  One flag,
+  // set here where the launches are chosen, tells the lane-per-pair kernel
+  // which entries the wave-per-pair kernel took.
+  a.exact_front = L.front != 0;
   const unsigned wgrid = (unsigned)(kExactSplit / kWaves);
   const unsigned xgrid = (unsigned)std::min<int64_t>((nwords * 64 + kBlock - 1) / kBlock, 1024);
-  if (L.front) {  // breadth first (a wave per pair) before the depth-first kernels
+  if (a.exact_front) {  // breadth first (a wave per pair) before the depth-first kernels
     if (batch) k_walk_front<false, true, 64, true><<<wgrid, kBlock, 0, st>>>(a);
     else k_walk_front<false, false, 64, true><<<wgrid, kBlock, 0, st>>>(a);
   }
-  const bool wave = !L.front;  // the wave-per-pair instance (without the breadth-first one)
+  const bool wave = !a.exact_front;  // the depth-first wave-per-pair instance
   if (need2 <= 32) {
     if (batch) {
       if (wave) k_walk_split<32, false, true, 6, true><<<wgrid, kBlock, 0, st>>>(a);
