@@ -1,18 +1,22 @@
 """Benchmark of the visibility-edge engine on one MI355X.
 
-Workload (default): perturbed icosphere, subdivision level 6 (40,962 v /
-81,920 t, amplitude 0.3, seed 42), seeded 4,096-vertex subsample, so one
-step tests all 8,386,560 vertex-pair segments against the 81,920-triangle
-BVH in midpoint-parity mode; 4,096 candidate planes are scored on the
-resulting visibility edges.  Inputs are resident in HBM before timing.
+Headline workload (default, ``--workload c5``): perturbed icosphere,
+subdivision level 7 with four Gaussian dents (163,842 v / 327,680 t,
+amplitude 0.5, seed 42), seeded 16,384-vertex subsample, so one step tests
+all 134,209,536 vertex-pair segments against the 327,680-triangle BVH in
+midpoint-parity mode.  Inputs are resident in HBM before timing.
 
 A "step" is one visibility pass (traversal + exact fallback + k-ordered
 compaction).  ``value`` = segments per second of that step, in
-Gsegments/s.  Plane scoring is timed in its own loop and reported under
-"scoring".  The CPU baseline is the FP64 oracle on a bounded random sample
-of the same pairs, with all host threads.
+Gsegments/s.  Beside it: the LBVH build of the same mesh, plane scoring of
+its 16,384 candidate planes on the pass's edges, the same pass on the C2
+workload (icosphere L6, 4,096 points) with its walk-variant sweeps, the
+proposal model, the 64-part batched step (C3), the decompositions of C2 and
+C5 in plain, batched+exact and batched+learned form (``north_star``), and
+the CPU baselines: the FP64 oracle's visibility and scoring rates on
+samples of the same workload and the C1 decomposition on the CPU oracle.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c5|c1]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c2|c1]
                        [--no-cpu] [--cpu-seconds S]
 """
 from __future__ import annotations
@@ -109,11 +113,19 @@ def acd_wall(W, mode, proposal="exact"):
             "phase_seconds": prof}
 
 
-# kernels of one default visibility pass whose HBM traffic is reported
-TRAFFIC_KERNELS = ("k_visibility<1, false, false, false, false, true>",
-                   "k_walk_front<false, false, 8>",
-                   "k_walk_split<32, false, false, 0, false>",
-                   "k_walk_split<64, false, false, 0, false>")
+def traffic_kernel(name):
+    """Kernels of one default visibility pass whose HBM traffic is reported:
+    the fan pass (k_visibility) and the walks of its queue (k_walk_front /
+    k_walk_split instances other than the exact redo, whose last template
+    argument is true)."""
+    import re
+
+    m = re.search(r"(k_visibility|k_walk_front|k_walk_split)<([^>]*)>", name)
+    if not m:
+        return None
+    if m.group(1) != "k_visibility" and m.group(2).replace(" ", "").endswith("true"):
+        return None
+    return m.group(0)
 
 
 def pmc_traffic(workload):
@@ -163,9 +175,9 @@ def pmc_traffic(workload):
                         v = float(r["Counter_Value"])
                         if "copyBuffer" in name:
                             per["cal"] = max(per["cal"], v)
-                        for k in TRAFFIC_KERNELS:
-                            if k in name:
-                                last[k] = v
+                        k = traffic_kernel(name)
+                        if k:
+                            last[k] = v
                     per["pass"] = sum(last.values())
             if per["cal"] <= 0 or per["pass"] <= 0:
                 return None, f"no counter data in the {counter} run"
@@ -356,12 +368,212 @@ def c3_batched(steps):
             "autotune_table": [{**c, "ms": round(t, 4)} for c, t in table]}
 
 
+def vis_pass(ctx, W, steps, warmup, sweeps=False):
+    """One workload's visibility pass: wall time per pass over `steps` passes
+    (inputs resident in HBM), HIP-event time of the fan pass + walk kernels
+    on the launch stream, the instrumented node/triangle work per segment
+    and the algorithmic bytes that gives; optionally the walk-variant sweeps."""
+    import numpy as np
+    import torch
+
+    from visacd_amd import native
+
+    dev = torch.device("cuda:0")
+    d_V = torch.from_numpy(W.vertices).to(dev)
+    d_T = torch.from_numpy(W.triangles.astype(np.int32)).to(dev)
+    d_pts = torch.from_numpy(W.points).to(dev)
+    n, S = W.n_points, W.n_segments
+    ctx.timing(reset=True)
+    bvh = ctx.bvh(d_V, d_T)
+    binfo = bvh.info()
+    bt = ctx.timing(reset=True)
+    cap = S
+    d_pairs = torch.empty((cap, 2), dtype=torch.int32, device=dev)
+    d_count = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def step():
+        ctx.visibility_d(d_pts, n, bvh, d_pairs=d_pairs, cap=cap, d_count=d_count)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    E = int(d_count.item())
+    st = ctx.visibility_stats()
+    ctx.timing(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    tv = ctx.timing(reset=True)
+    if int(d_count.item()) != E:
+        raise SystemExit("visibility result changed between steps")
+    kms = tv["vis_ms"] / max(1, tv["vis_launches"])
+
+    def timed(k):
+        step()
+        ctx.timing(reset=True)
+        for _ in range(k):
+            step()
+        t = ctx.timing(reset=True)
+        return t["vis_ms"] / max(1, t["vis_launches"])
+
+    out = {"edges": E, "blocked": st["blocked"], "exact_fallback_pairs": st["exact_calls"],
+           "walked_pairs": st["walked"], "ms_per_pass": ms, "kernel_ms": kms,
+           "gsegments_per_s": S / (ms * 1e-3) / 1e9,
+           "kernel_gsegments_per_s": S / (kms * 1e-3) / 1e9,
+           "bvh_build_ms": bt["bvh_ms"], "bvh_depth": binfo["depth"]}
+    if sweeps:
+        ctx.set_option(native.VACD_OPT_FRONT_LANES, 0)  # depth-first walks for this sweep
+        lpp = {}
+        for k in (1, 2, 4, 8, 16):
+            ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, k)
+            lpp[k] = timed(max(2, steps // 4))
+        ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
+        front = {}
+        for fl in (0, 4, 8, 16, 32, 64):
+            ctx.set_option(native.VACD_OPT_FRONT_LANES, fl)
+            front[fl] = timed(max(2, steps // 4))
+        ctx.set_option(native.VACD_OPT_FRONT_LANES, 8)
+        ctx.set_option(native.VACD_OPT_FAN_PAIRS, 0)
+        walk_all = timed(max(2, steps // 4))
+        ctx.set_option(native.VACD_OPT_FAN_PAIRS, 1)
+        torch.cuda.synchronize()
+        if int(d_count.item()) != E:
+            raise SystemExit("visibility result differs between walk variants")
+        out.update({"kernel_ms_by_lanes_per_pair": lpp, "kernel_ms_by_front_lanes": front,
+                    "kernel_ms_walk_every_pair": walk_all})
+    # algorithmic work per segment (instrumented kernel variant, untimed)
+    ctx.set_option(native.VACD_OPT_COUNT_WORK, 1)
+    step()
+    stw = ctx.visibility_stats()
+    ctx.set_option(native.VACD_OPT_COUNT_WORK, 0)
+    ctx.timing(reset=True)
+    nps, tps = stw["node_visits"] / S, stw["tri_tests"] / S
+    # a binary node visit loads one 64-byte node (both child boxes and ids);
+    # a triangle test its 9 coordinates (36 B); a segment its endpoints
+    # (24 B) and one result bit (counted as 1 B)
+    bps = 24 + 1 + 64 * nps + 36 * tps
+    out.update({"node_visits_per_segment": nps, "tri_tests_per_segment": tps,
+                "alg_bytes_per_segment": bps,
+                "achieved_gbs": bps * S / (kms * 1e-3) / 1e9})
+    return out, (bvh, d_pts, d_pairs, E, d_V, d_T)
+
+
+# LBVH algorithmic bytes per triangle: its 9 coordinates read (36 B), the
+# 8-byte key/value pair written by each of the 4 radix passes and one
+# 64-byte node written
+LBVH_BYTES_PER_TRI = 36 + 8 * 4 + 64
+
+
+def bvh_roofline(ctx, d_V, d_T, n_tris, reps):
+    """Builds of the workload's LBVH timed with HIP events around the whole
+    build on the context stream (every build kernel, no host waits)."""
+    ctx.timing(reset=True)
+    for _ in range(reps):
+        ctx.bvh(d_V, d_T).close()
+    t = ctx.timing(reset=True)
+    ms = t["bvh_ms"] / max(1, t["bvh_launches"])
+    ach = LBVH_BYTES_PER_TRI * n_tris / (ms * 1e-3) / 1e9
+    return {"triangles": n_tris, "build_ms": ms, "builds": int(t["bvh_launches"]),
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "bytes_per_triangle": LBVH_BYTES_PER_TRI}}
+
+
+def cpu_scoring_baseline(W, edges, seconds):
+    """Oracle Q32.32 scoring (all host threads) on a plane sample of the workload."""
+    import numpy as np
+
+    import oracle
+
+    E = edges.shape[0]
+    P = 8
+    t0 = time.perf_counter()
+    oracle.score_q32(W.planes[:P], W.points, edges)
+    dt = max(time.perf_counter() - t0, 1e-6)
+    P = int(min(W.planes.shape[0], max(8, P * seconds / dt)))
+    t0 = time.perf_counter()
+    oracle.score_q32(W.planes[:P], W.points, edges)
+    dt = time.perf_counter() - t0
+    return {"value": P * E / dt / 1e9, "unit": "Gtests/s", "cores": oracle.num_threads(),
+            "kind": "port", "sample": f"{P} planes x {E} visibility edges of the same workload, "
+                                      f"oracle Q32.32 scoring, {dt:.1f} s"}
+
+
+def c1_decomposition(seconds_cap=120.0):
+    """C1 (torus, 1,152 vertices, all pairs, 256 sphere planes per cut,
+    concavity 0.04, 8 parts, seed 42): the plain loop on the CPU oracle
+    (oracle/decompose.py, all host threads) and visacd.process in plain
+    mode on the GPU, both with host distances in the concavity; the two
+    decompositions must be identical."""
+    import numpy as np
+
+    import oracle
+    import visacd as vis
+    from oracle import decompose as D
+    from visacd_amd import workloads as wl
+
+    W = wl.torus_c1()
+    t0 = time.perf_counter()
+    ref, ref_h = D.decompose(W.vertices, W.triangles, 0.04, 8, seed=42, num_planes=256,
+                             planes="sphere")
+    cpu_s = time.perf_counter() - t0
+    m = vis.Mesh()
+    m.vertices = vis.VecArray3d(W.vertices.astype(np.float64).tolist())
+    m.triangles = vis.make_vecarray3i(W.triangles.astype(np.int32))
+    vis.engine.verbose = False
+    vis.engine.mode = "plain"
+    vis.engine.planes = "sphere"
+    vis.engine.num_planes = 256
+    vis.engine.max_points = 0
+    vis.engine.gpu_distance = False
+    vis.config.return_parts = True
+    vis.set_seed(42)
+    t0 = time.perf_counter()
+    r = vis.process(m, 0.04, 8)
+    gpu_s = time.perf_counter() - t0
+    vis.engine.gpu_distance = True
+    vis.config.return_parts = False
+    same = r.num_parts == len(ref) and r.concavity == ref_h and all(
+        np.array_equal(np.array([list(v) for v in p.vertices]), rv) for p, (rv, _rt) in zip(r.parts, ref))
+    return {"value": cpu_s, "unit": "s (wall, C1 decomposition)", "cores": oracle.num_threads(),
+            "kind": "port", "sample": "the whole C1 decomposition: plain loop with the FP64 oracle "
+                                      "as visibility and scoring engine",
+            "gpu_plain_wall_s": gpu_s, "parts": int(r.num_parts), "concavity": float(r.concavity),
+            "identical_to_gpu": bool(same)}
+
+
+def north_star(runs):
+    """The C5 target: batched+learned at least 5x faster than plain, parts
+    +-1, final concavity within 2 %."""
+    by = {(r["mode"], r["proposal"]): r for r in runs if r["workload"].startswith("icosphere-L7")}
+    p, b, be = by.get(("plain", "exact")), by.get(("batched", "learned")), by.get(("batched", "exact"))
+    if not p or not b:
+        return None
+    sp = p["wall_s"] / b["wall_s"]
+    dc = (b["concavity"] - p["concavity"]) / p["concavity"]
+    out = {"workload": p["workload"], "plain_exact_s": p["wall_s"],
+           "batched_learned_s": b["wall_s"], "speedup": sp, "target_speedup": 5.0,
+           "parts_plain": p["parts"], "parts_batched_learned": b["parts"],
+           "delta_parts": b["parts"] - p["parts"], "concavity_plain": p["concavity"],
+           "concavity_batched_learned": b["concavity"], "delta_concavity_rel": dc,
+           "met": {"speedup": sp >= 5.0, "parts": abs(b["parts"] - p["parts"]) <= 1,
+                   "concavity": abs(dc) <= 0.02}}
+    if be:
+        out["batched_exact_s"] = be["wall_s"]
+        out["batched_exact_speedup"] = p["wall_s"] / be["wall_s"]
+        out["batched_exact_identical"] = (be["parts"] == p["parts"]
+                                          and be["concavity"] == p["concavity"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c5")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--score-steps", type=int, default=10)
@@ -372,6 +584,8 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--acd-run", metavar="WORKLOAD:MODE:PROPOSAL",
                     help="only one decomposition, e.g. c5:batched:exact (for profiling)")
+    ap.add_argument("--vis-only", action="store_true",
+                    help="only the headline visibility passes (for profiling)")
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe(args.workload)
@@ -393,112 +607,20 @@ def main():
     W = make_workload(args.workload)
     log("workload", W.describe())
     dev = torch.device("cuda:0")
-    d_V = torch.from_numpy(W.vertices).to(dev)
-    d_T = torch.from_numpy(W.triangles.astype(np.int32)).to(dev)
-    d_pts = torch.from_numpy(W.points).to(dev)
-    d_planes = torch.from_numpy(W.planes).to(dev)
-    n = W.n_points
-    S = W.n_segments
-    P = W.planes.shape[0]
-
     ctx = native.Context(0)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     ctx.set_option(native.VACD_OPT_TIMING, 1)
-    bvh = ctx.bvh(d_V, d_T)
-    binfo = bvh.info()
-    bt = ctx.timing(reset=True)
-    log("bvh", binfo["n_tris"], "depth", binfo["depth"], f"build {bt['bvh_ms']:.3f} ms")
-
-    cap = S
-    d_pairs = torch.empty((cap, 2), dtype=torch.int32, device=dev)
-    d_count = torch.zeros(1, dtype=torch.int64, device=dev)
-
-    def vis_step():
-        ctx.visibility_d(d_pts, n, bvh, d_pairs=d_pairs, cap=cap, d_count=d_count)
-
-    for _ in range(args.warmup):
-        vis_step()
-    torch.cuda.synchronize()
-    E = int(d_count.item())
-    st = ctx.visibility_stats()
-    ctx.timing(reset=True)
-    log("edges", E, st)
-
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        vis_step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    tv = ctx.timing(reset=True)
-    if int(d_count.item()) != E:
-        raise SystemExit("visibility result changed between steps")
-    vis_kernel_ms = tv["vis_ms"] / max(1, tv["vis_launches"])
-
-    # the same pass with the 4-wide node walk, for comparison
-    ctx.set_option(native.VACD_OPT_WIDE_BVH, 1)
-    vis_step()
-    ctx.timing(reset=True)
-    for _ in range(max(2, args.steps // 4)):
-        vis_step()
-    tb = ctx.timing(reset=True)
-    wide_kernel_ms = tb["vis_ms"] / max(1, tb["vis_launches"])
-    ctx.set_option(native.VACD_OPT_WIDE_BVH, 0)
-    # the walks the fan leaves open, split over 1..16 lanes per pair
-    lpp_ms = {}
-    ctx.set_option(native.VACD_OPT_FRONT_LANES, 0)  # depth-first walks for this sweep
-    for lpp in (1, 2, 4, 8, 16):
-        ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, lpp)
-        vis_step()
-        ctx.timing(reset=True)
-        for _ in range(max(2, args.steps // 4)):
-            vis_step()
-        tl = ctx.timing(reset=True)
-        lpp_ms[lpp] = tl["vis_ms"] / max(1, tl["vis_launches"])
-    ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
-    log("lanes per pair -> kernel ms", lpp_ms)
-    # the short walk queue walked breadth-first with 4..64 lanes per pair, or
-    # depth-first (0: the lanes-per-pair walks above)
-    front_ms = {}
-    for fl in (0, 4, 8, 16, 32, 64):
-        ctx.set_option(native.VACD_OPT_FRONT_LANES, fl)
-        vis_step()
-        ctx.timing(reset=True)
-        for _ in range(max(2, args.steps // 4)):
-            vis_step()
-        tl = ctx.timing(reset=True)
-        front_ms[fl] = tl["vis_ms"] / max(1, tl["vis_launches"])
-    ctx.set_option(native.VACD_OPT_FRONT_LANES, 8)
-    log("breadth-first lanes per pair -> kernel ms", front_ms)
-    # ... and with every pair walked (no fan shortcut for pairs starting inside)
-    ctx.set_option(native.VACD_OPT_FAN_PAIRS, 0)
-    vis_step()
-    ctx.timing(reset=True)
-    for _ in range(max(2, args.steps // 4)):
-        vis_step()
-    tf = ctx.timing(reset=True)
-    walk_all_kernel_ms = tf["vis_ms"] / max(1, tf["vis_launches"])
-    ctx.set_option(native.VACD_OPT_FAN_PAIRS, 1)
-    torch.cuda.synchronize()
-    if int(d_count.item()) != E:
-        raise SystemExit("visibility result differs between walk variants")
-
-    # algorithmic work per segment (instrumented kernel variant, untimed)
-    ctx.set_option(native.VACD_OPT_COUNT_WORK, 1)
-    vis_step()
-    stw = ctx.visibility_stats()
-    ctx.set_option(native.VACD_OPT_COUNT_WORK, 0)
-    ctx.timing(reset=True)
-    nodes_per_seg = stw["node_visits"] / S
-    tris_per_seg = stw["tri_tests"] / S
-    # a binary node visit loads one 64-byte node (both child boxes and ids);
-    # a triangle test its 9 coordinates (36 B); a segment its endpoints
-    # (24 B) and one result bit (counted as 1 B)
-    bytes_per_seg = 24 + 1 + 64 * nodes_per_seg + 36 * tris_per_seg
-    achieved_gbs = bytes_per_seg * S / (vis_kernel_ms * 1e-3) / 1e9
+    vis, (bvh, d_pts, d_pairs, E, d_V, d_T) = vis_pass(ctx, W, args.steps, args.warmup)
+    log("visibility", W.name, vis)
+    if args.vis_only:
+        print(json.dumps(vis), flush=True)
+        return
+    n, S, P = W.n_points, W.n_segments, W.planes.shape[0]
+    lbvh = bvh_roofline(ctx, d_V, d_T, int(W.triangles.shape[0]), max(3, args.steps // 2))
+    log("lbvh", lbvh)
 
     # plane scoring on the visibility edges of this workload
+    d_planes = torch.from_numpy(W.planes).to(dev)
     d_q = torch.zeros(P, dtype=torch.int64, device=dev)
     d_am = torch.zeros(1, dtype=torch.int32, device=dev)
     for _ in range(2):
@@ -515,51 +637,51 @@ def main():
     score_kernel_ms = tsc["score_ms"] / max(1, tsc["score_launches"])
     tests = P * E
     score = {
-        "planes": P, "edges": E, "ms_per_call": score_ms,
-        "kernel_ms": score_kernel_ms,
+        "planes": P, "edges": E, "ms_per_call": score_ms, "kernel_ms": score_kernel_ms,
         "gtests_per_s": tests / (score_ms * 1e-3) / 1e9,
+        "kernel_gtests_per_s": tests / (score_kernel_ms * 1e-3) / 1e9,
         "roofline": {"bound": "fp32-valu", "achieved": 12 * tests / (score_kernel_ms * 1e-3) / 1e12,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": 12 * tests / (score_kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS},
+                     "frac": 12 * tests / (score_kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                     "flop_per_test": 12},
         "argmax": int(d_am.item()),
     }
-
-    # calibration copy of known size (for FETCH_SIZE / WRITE_SIZE scaling)
-    cal = torch.empty(64 << 20, dtype=torch.float32, device=dev)
-    cal2 = torch.empty_like(cal)
-    cal2.copy_(cal)
-    torch.cuda.synchronize()
+    log("scoring", score)
+    edges_host = d_pairs[:E].cpu().numpy().astype(np.uint32) if not args.no_cpu else None
+    bvh.close()
+    # the C2 pass (icosphere L6, 4,096 points) with the walk-variant sweeps
+    vis_c2 = None
+    if args.workload != "c2":
+        W2 = make_workload("c2")
+        vis_c2, (b2, *_rest) = vis_pass(ctx, W2, args.steps, args.warmup, sweeps=True)
+        b2.close()
+        vis_c2["workload"] = W2.name
+        log("visibility c2", vis_c2)
+    ctx.close()
 
     line = {
         "metric": "ACD wall-time @concavity 0.04 (plain vs batched+learned); Gsegments/s vis",
-        "value": S / (ms_per_step * 1e-3) / 1e9,
+        "value": vis["gsegments_per_s"],
         "unit": "Gsegments/s",
-        "value_is": "visibility segments per second of one full visibility pass",
+        "value_is": "visibility segments per second of one full visibility pass (all pairs of "
+                    "the workload's points against its whole mesh), wall time over the timed passes",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": vis["ms_per_pass"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 geometry, f64 filters + exact predicates",
         "data": "synthetic (seeded procedural mesh)",
         "config": {"workload": W.name, "vertices": int(W.vertices.shape[0]),
                    "triangles": int(W.triangles.shape[0]), "points": n, "segments": S,
                    "planes": P, "mode": "midpoint parity", "seed": W.meta.get("seed")},
-        "visibility": {"edges": E, "blocked": st["blocked"], "exact_fallback_pairs": st["exact_calls"],
-                       "kernel_ms": vis_kernel_ms, "kernel_ms_wide_walk": wide_kernel_ms,
-                       "kernel_ms_by_lanes_per_pair": lpp_ms,
-                       "kernel_ms_by_front_lanes": front_ms,
-                       "kernel_ms_walk_every_pair": walk_all_kernel_ms,
-                       "kernel_gsegments_per_s": S / (vis_kernel_ms * 1e-3) / 1e9,
-                       "bvh_build_ms": bt["bvh_ms"], "bvh_depth": binfo["depth"],
-                       "node_visits_per_segment": nodes_per_seg,
-                       "tri_tests_per_segment": tris_per_seg,
-                       "walked_pairs": st["walked"],
-                       "alg_bytes_per_segment": bytes_per_seg},
-        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_visibility"},
+        "visibility": vis,
+        "roofline": {"bound": "hbm", "achieved": vis["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": vis["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fan pass + queue walk (k_visibility, k_walk_*)",
+                     "alg_bytes_per_pass": vis["alg_bytes_per_segment"] * S},
+        "lbvh": lbvh,
         "scoring": score,
     }
-    bvh.close()
-    ctx.close()
+    if vis_c2:
+        line["visibility_c2"] = vis_c2
     if not args.no_pmc:
         traffic, detail = pmc_traffic(args.workload)
         log("pmc traffic", traffic, detail)
@@ -568,7 +690,6 @@ def main():
             line["roofline"]["traffic_note"] = detail
         else:
             line["roofline"]["traffic_unit"] = "bytes per visibility pass (fan pass + queue walk)"
-            line["roofline"]["alg_bytes_per_pass"] = bytes_per_seg * S
     if not args.no_mlp:
         line["proposal_mlp"] = proposal_bench(max(3, args.steps // 2))
         log("mlp", line["proposal_mlp"])
@@ -579,13 +700,17 @@ def main():
         from visacd_amd import workloads as wl
 
         runs = []
-        for WW in (W, wl.icosphere_c5()) if args.workload == "c2" else (W,):
+        for WW in (wl.icosphere_c2(), wl.icosphere_c5()):
             for mode, prop in (("plain", "exact"), ("batched", "exact"), ("batched", "learned")):
                 runs.append(acd_wall(WW, mode, prop))
                 log("acd", {k: v for k, v in runs[-1].items() if k != "phase_seconds"})
         line["acd"] = runs
+        line["north_star"] = north_star(runs)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
+        cb = cpu_baseline(W, args.cpu_seconds)
+        cb["scoring"] = cpu_scoring_baseline(W, edges_host, args.cpu_seconds / 2)
+        cb["c1_decomposition"] = c1_decomposition()
+        line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
 

@@ -298,6 +298,11 @@ int vacd_mlp_forward_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *
  * returns when out is written. */
 int vacd_mlp_forward(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *feat, int P,
                      const float *planes, float *out);
+/* The gradient of the loss of the last vacd_mlp_train_step_d with respect to
+ * the parameters (vacd_mlp_param_count() floats, laid out as the
+ * parameters), copied to host memory; waits.  For the tests of the training
+ * kernels (no counterpart in the reference). */
+int vacd_mlp_get_grads(vacd_ctx *ctx, vacd_mlp *mlp, float *grads);
 /* One Adam step (lr; betas 0.9, 0.999) on the mean squared error; waits and
  * writes the batch loss (before the step) to *loss. */
 int vacd_mlp_train_step_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *d_feat,

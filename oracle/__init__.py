@@ -59,6 +59,7 @@ def lib():
         L.orc_bisector_planes.argtypes = [vp, vp, i64, ctypes.c_int, ctypes.c_uint32, vp]
         L.orc_bisector_planes.restype = i64
         L.orc_mt19937.argtypes = [ctypes.c_uint32, i64, vp]
+        L.orc_predicates_batch.argtypes = [ctypes.c_int, i64, vp, ctypes.c_int, vp]
         L.orc_uniform_int.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, i64, vp]
         del fp
         _lib = L
@@ -221,4 +222,16 @@ def uniform_int(seed: int, lo: int, hi: int, n: int) -> np.ndarray:
     """n draws of std::uniform_int_distribution<int>(lo, hi) on std::mt19937(seed)."""
     out = np.zeros(n, dtype=np.int32)
     lib().orc_uniform_int(seed, lo, hi, n, _ptr(out))
+    return out
+
+
+def predicates_batch(kind: int, inputs, k: int = 0) -> np.ndarray:
+    """Exact signs of n predicates: kind 0 orient3d (12 floats per case), 1
+    orient3d with a midpoint (15), 2 orient2d in projection k (9), 3 orient2d
+    with a midpoint (12)."""
+    x = _f32(inputs)
+    stride = {0: 12, 1: 15, 2: 9, 3: 12}[kind]
+    n = x.size // stride
+    out = np.zeros(n, dtype=np.int32)
+    lib().orc_predicates_batch(kind, n, _ptr(x), k, _ptr(out))
     return out

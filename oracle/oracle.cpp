@@ -810,6 +810,22 @@ int64_t orc_argmax_i64(const int64_t* s, int64_t P) {
   return P > 0 ? best : -1;
 }
 
+// Batch form of the exact predicates (same kinds and input layout as the
+// engine's host predicate check): kind 0 orient3d, 1 orient3d with the
+// midpoint of two points as the last, 2 orient2d in projection k, 3
+// orient2d with a midpoint.
+void orc_predicates_batch(int kind, int64_t n, const float* in, int k, int* out) {
+  const int stride = kind == 0 ? 12 : (kind == 1 ? 15 : (kind == 2 ? 9 : 12));
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const float* x = in + stride * i;
+    if (kind == 0) out[i] = orient3d(ld(x), ld(x + 3), ld(x + 6), ld(x + 9));
+    else if (kind == 1) out[i] = orient3d_mid(ld(x), ld(x + 3), ld(x + 6), ld(x + 9), ld(x + 12));
+    else if (kind == 2) out[i] = orient2d(ld(x), ld(x + 3), ld(x + 6), k);
+    else out[i] = orient2d_mid(ld(x), ld(x + 3), ld(x + 6), ld(x + 9), k);
+  }
+}
+
 // Raw outputs of std::mt19937(seed) (the reference's random_engine,
 // lib/include/core.hpp:13) and draws of std::uniform_int_distribution<int>
 // (lo, hi) on it (lib/src/process.cpp:38-43), for the known-answer tests.

@@ -159,3 +159,93 @@ def test_score_batch_matches_oracle_per_part(ctx):
         np.testing.assert_array_equal(qs[k], oq, err_msg=f"part {k}")
         assert am[k] == (oracle.argmax_i64(oq) if pl.shape[0] else -1), k
 
+
+
+def test_batched_c3_step_matches_oracle_and_graph_replays(ctx):
+    """The 64-part C3 step (perturbed icospheres, levels 2-5, at most 2,048
+    points and 1,024 planes each): one batched visibility call and one
+    batched scoring call on its edges.  Per part, 1,500 random pairs and 500
+    edges are decided as the oracle decides them, and every plane's Q32.32
+    score and the first maximum equal the oracle's.  The step captured in a
+    HIP graph and replayed three times (eager work between replays) gives the
+    same edges, scores and argmax each time."""
+    import torch
+
+    Ws = wl.batched_c3()
+    dev = torch.device("cuda:0")
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    bvhs, pts, planes = [], [], []
+    for W in Ws:
+        bvhs.append(ctx.bvh(torch.from_numpy(W.vertices).to(dev),
+                            torch.from_numpy(W.triangles.astype(np.int32)).to(dev)))
+        pts.append(torch.from_numpy(W.points).to(dev))
+        planes.append(torch.from_numpy(W.planes).to(dev))
+    ns = [W.n_points for W in Ws]
+    S = sum(n * (n - 1) // 2 for n in ns)
+    print("c3 parts", len(Ws), "segments", S, flush=True)
+    d_pairs = torch.empty((S, 2), dtype=torch.int32, device=dev)
+    d_count = torch.zeros(1, dtype=torch.int64, device=dev)
+    d_off = torch.zeros(len(Ws) + 1, dtype=torch.int64, device=dev)
+    vis_parts = [(pts[i], ns[i], None, bvhs[i], None) for i in range(len(Ws))]
+    ctx.visibility_batch_d(vis_parts, d_pairs, S, d_count, d_off)
+    torch.cuda.synchronize()
+    off = d_off.cpu().numpy()
+    pairs = d_pairs.cpu().numpy().astype(np.uint32)
+    score_parts = [(planes[i], Ws[i].planes.shape[0], pts[i], ns[i],
+                    d_pairs.data_ptr() + 8 * int(off[i]), int(off[i + 1] - off[i]))
+                   for i in range(len(Ws))]
+    P = sum(W.planes.shape[0] for W in Ws)
+    d_q = torch.zeros(P, dtype=torch.int64, device=dev)
+    d_am = torch.zeros(len(Ws), dtype=torch.int32, device=dev)
+    ctx.score_planes_batch_d(score_parts, d_q, d_am)
+    torch.cuda.synchronize()
+    q = d_q.cpu().numpy()
+    am = d_am.cpu().numpy()
+    rng = np.random.default_rng(9)
+    po = 0
+    for t, W in enumerate(Ws):
+        e = pairs[off[t]:off[t + 1]]
+        n = ns[t]
+        k = e[:, 0].astype(np.int64) * (2 * n - e[:, 0].astype(np.int64) - 1) // 2 + \
+            e[:, 1].astype(np.int64) - e[:, 0].astype(np.int64) - 1
+        assert (np.diff(k) > 0).all(), t
+        Sp = n * (n - 1) // 2
+        ks = np.unique(np.concatenate([rng.integers(0, Sp, 1500),
+                                       k[rng.integers(0, len(k), 500)] if len(k) else []]).astype(np.int64))
+        det = oracle.visibility_sample(W.points, W.vertices, W.triangles, ks)
+        np.testing.assert_array_equal(np.isin(ks, k), det == 1, err_msg=f"part {t}")
+        Pp = W.planes.shape[0]
+        oq = oracle.score_q32(W.planes, W.points, e)
+        np.testing.assert_array_equal(q[po:po + Pp], oq, err_msg=f"part {t}")
+        assert am[t] == oracle.argmax_i64(oq), t
+        po += Pp
+
+    def step():
+        ctx.visibility_batch_d(vis_parts, d_pairs, S, d_count, d_off)
+        ctx.score_planes_batch_d(score_parts, d_q, d_am)
+
+    step()  # warm-up: buffers and staging regions sized before the capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(g, stream=s):
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        step()
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    total = int(off[-1])
+    for r in range(3):
+        d_count.zero_()
+        d_q.zero_()
+        d_am.fill_(-7)
+        d_pairs[:total].fill_(-1)
+        g.replay()
+        torch.cuda.synchronize()
+        assert int(d_count.item()) == total, r
+        np.testing.assert_array_equal(d_off.cpu().numpy(), off)
+        np.testing.assert_array_equal(d_pairs[:total].cpu().numpy().astype(np.uint32), pairs[:total])
+        np.testing.assert_array_equal(d_q.cpu().numpy(), q)
+        np.testing.assert_array_equal(d_am.cpu().numpy(), am)
+        _ = torch.ones(4096, device=dev).sum().item()  # eager work between replays
+    for bb in bvhs:
+        bb.close()

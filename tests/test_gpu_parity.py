@@ -423,6 +423,38 @@ def test_visibility_icosphere_c2_sampled(ctx):
     np.testing.assert_array_equal(np.isin(dk, k), det == 1)
 
 
+def test_visibility_and_scoring_icosphere_c5_sampled(ctx):
+    """Full-size C5 pass (icosphere L7 with dents, 327,680 triangles, 16,384
+    points, 134,209,536 segments): 25,000 random pairs, 5,000 edges and
+    every pair redone exactly decided as the oracle decides them; the
+    Q32.32 scores of 64 of its 16,384 candidate planes and their first
+    maximum equal the oracle's."""
+    W = wl.icosphere_c5()
+    _print("c5", W.n_segments, W.triangles.shape)
+    b = ctx.bvh(W.vertices, W.triangles)
+    g = ctx.visibility(W.points, b)
+    st = ctx.visibility_stats()
+    _print("c5 stats", st)
+    n = W.n_points
+    k = g[:, 0].astype(np.int64) * (2 * n - g[:, 0].astype(np.int64) - 1) // 2 + \
+        g[:, 1].astype(np.int64) - g[:, 0].astype(np.int64) - 1
+    assert (np.diff(k) > 0).all() and st["edges"] == len(g) > 100000
+    rng = np.random.default_rng(5)
+    dk = ctx.visibility_deferred()
+    assert len(dk) == st["exact_calls"]
+    ks = np.unique(np.concatenate([rng.integers(0, W.n_segments, 25000),
+                                   k[rng.integers(0, len(k), 5000)], dk]))
+    det = oracle.visibility_sample(W.points, W.vertices, W.triangles, ks)
+    np.testing.assert_array_equal(np.isin(ks, k), det == 1)
+    sel = np.sort(rng.choice(W.planes.shape[0], 64, replace=False))
+    planes = W.planes[sel]
+    q, _f, am = ctx.score_planes(planes, W.points, g)
+    oq = oracle.score_q32(planes, W.points, g)
+    np.testing.assert_array_equal(q, oq)
+    assert am == oracle.argmax_i64(oq)
+    b.close()
+
+
 # ----------------------------------------------------------------- scoring
 def test_scoring_torus_exact(ctx):
     W = wl.torus_c1()

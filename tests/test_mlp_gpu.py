@@ -68,23 +68,35 @@ def _batch(seed, B=3, E=64, P=96):
     return feat, planes, target
 
 
-def test_mlp_forward_matches_fp32_reference(ctx):
+@pytest.mark.parametrize("B,E,P", [(3, 64, 96), (1, 512, 16384)])
+def test_mlp_forward_matches_fp32_reference(ctx, B, E, P):
+    """bf16 operands, fp32 accumulation: within 2 % of the largest output of
+    the fp32 model on the same parameters, at a small batch and at the size
+    the proposal runs (512 edges, 16,384 planes), where the top-256 ranking
+    it feeds must mostly agree with the fp32 model's."""
     import torch
 
     from visacd_amd import native
 
-    print("mlp forward", flush=True)
+    print("mlp forward", B, E, P, flush=True)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     m = native.Mlp(ctx, seed=3)
-    B, E, P = 3, 64, 96
     feat, planes, _ = _batch(1, B, E, P)
     out = torch.empty(B * P, device="cuda")
     m.forward_d(B, E, feat.cuda(), P, planes.cuda(), out)
     torch.cuda.synchronize()
-    ref, _ = _torch_model(m.get(), feat, planes, B, E, P)
+    with torch.no_grad():
+        ref, _ = _torch_model(m.get(), feat, planes, B, E, P)
     ref = ref.detach()
     scale = float(ref.abs().max()) + 1e-3
-    assert float((out.cpu() - ref).abs().max()) <= 0.05 * scale
+    err = float((out.cpu() - ref).abs().max())
+    print("  max error", err, "scale", scale, flush=True)
+    assert err <= 0.02 * scale
+    if P >= 4096:
+        top = set(torch.topk(out.cpu(), 256).indices.tolist())
+        top_ref = set(torch.topk(ref, 256).indices.tolist())
+        print("  top-256 overlap", len(top & top_ref), flush=True)
+        assert len(top & top_ref) >= 0.8 * 256
     m.close()
 
 
@@ -116,9 +128,13 @@ def test_fused_inference_head_matches_the_training_forward(ctx):
     m.close()
 
 
-def test_mlp_first_adam_step_follows_reference_gradient(ctx):
-    """Adam's first step moves each weight by -lr * sign(g) (|g| >> eps): the
-    update's signs must agree with the fp32 reference gradient."""
+def test_mlp_gradient_matches_fp32_reference_and_adam_is_exact(ctx):
+    """The training kernels' gradient (read back with vacd_mlp_get_grads)
+    agrees with the fp32 model's autograd gradient on the same batch and
+    parameters (bf16 operands: relative L2 error per tensor within 3 % for
+    the head and 10 % for the encoder),
+    and two Adam steps move the fp32 master weights exactly as Adam with
+    betas 0.9/0.999, eps 1e-8 and bias correction does on those gradients."""
     import torch
 
     from visacd_amd import native
@@ -132,16 +148,39 @@ def test_mlp_first_adam_step_follows_reference_gradient(ctx):
     out, w = _torch_model(w0, feat, planes, B, E, P)
     loss_ref = ((out - target.reshape(-1)) ** 2).mean()
     loss_ref.backward()
-    lr = 1e-4
-    loss = m.train_step_d(B, E, feat.cuda(), P, planes.cuda(), target.cuda(), lr)
-    assert abs(loss - float(loss_ref)) <= 0.05 * float(loss_ref) + 1e-6
-    dw = native.mlp_unpack(m.get() - w0)
-    for name in ("v3", "V2", "c2", "Wp", "c1", "Wz", "W3", "W2", "W1", "b1"):
-        gref = w[name].grad.numpy()
-        big = np.abs(gref) > 0.05 * np.abs(gref).max()
-        agree = np.mean(np.sign(dw[name][big]) == -np.sign(gref[big]))
-        print(name, int(big.sum()), agree, flush=True)
-        assert agree >= 0.9, name
+    lr = np.float32(1e-3)
+    loss = m.train_step_d(B, E, feat.cuda(), P, planes.cuda(), target.cuda(), float(lr))
+    assert abs(loss - float(loss_ref.detach())) <= 0.01 * float(loss_ref.detach())
+    g1 = m.grads()
+    gd = native.mlp_unpack(g1.copy())
+    for name in ("v3", "c3", "V2", "c2", "Wp", "c1", "Wz", "W3", "b3", "W2", "b2", "W1", "b1"):
+        gref = w[name].grad.numpy().ravel()
+        gg = gd[name].ravel()
+        if name == "W1":  # the padded 8th input column has no reference counterpart
+            gref, gg = gref.reshape(256, 8)[:, :7].ravel(), gg.reshape(256, 8)[:, :7].ravel()
+        nr = np.linalg.norm(gref)
+        rel = np.linalg.norm(gg - gref) / nr
+        cos = float(gg @ gref) / (np.linalg.norm(gg) * nr)
+        print(" ", name, "rel", rel, "cos", cos, flush=True)
+        # bf16 activations: the head within 3 %; the encoder behind the
+        # max-pool (whose winners bf16 rounding can move) within 10 %
+        head = name in ("v3", "c3", "V2", "c2", "Wp", "c1", "Wz")
+        assert rel <= (0.03 if head else 0.10) and cos >= (0.999 if head else 0.995), name
+    f = np.float32
+    b1, b2, eps = f(0.9), f(0.999), f(1e-8)
+    w1 = m.get()
+    m1 = (f(1) - b1) * g1
+    v1 = (f(1) - b2) * g1 * g1
+    c1, c2 = f(1) - b1, f(1) - b2
+    exp1 = w0 - lr * (m1 / c1) / (np.sqrt(v1 / c2) + eps)
+    np.testing.assert_allclose(w1, exp1, rtol=0, atol=1e-7)
+    m.train_step_d(B, E, feat.cuda(), P, planes.cuda(), target.cuda(), float(lr))
+    g2 = m.grads()
+    m2 = b1 * m1 + (f(1) - b1) * g2
+    v2 = b2 * v1 + (f(1) - b2) * g2 * g2
+    c1, c2 = f(1) - b1 * b1, f(1) - b2 * b2
+    exp2 = w1 - lr * (m2 / c1) / (np.sqrt(v2 / c2) + eps)
+    np.testing.assert_allclose(m.get(), exp2, rtol=0, atol=1e-7)
     m.close()
 
 

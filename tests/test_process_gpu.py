@@ -189,3 +189,67 @@ def test_learned_proposal_with_trained_model_is_deterministic():
     assert _parts(r1) == _parts(r2)
     vol = _volume(_mesh(vis, V, T))
     assert abs(sum(_volume(p) for p in r1.parts) - vol) < 1e-6 * abs(vol) + 1e-9
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_process_c1_matches_cpu_oracle_loop(mode):
+    """The torus workload (1,152 vertices, every pair tested, 256 sphere
+    planes per cut, concavity 0.04, 8 parts, seed 42) decomposed by
+    visacd.process must give exactly the parts and final concavity of the
+    CPU restatement of the loop whose visibility and scores come from the
+    FP64 oracle (oracle/decompose.py); host distances on both sides."""
+    import visacd as vis
+    from oracle import decompose as D
+
+    W = wl.torus_c1()
+    print("c1 process vs cpu loop", mode, flush=True)
+    ref_parts, ref_h = D.decompose(W.vertices, W.triangles, 0.04, 8, seed=42, num_planes=256,
+                                   planes="sphere")
+    vis.engine.mode = mode
+    vis.engine.planes = "sphere"
+    vis.engine.num_planes = 256
+    vis.engine.gpu_distance = False
+    try:
+        r = _run(vis, W.vertices.astype(np.float64), W.triangles, 0.04, 8, return_parts=True)
+    finally:
+        vis.engine.mode = "plain"
+        vis.engine.planes = "bisector"
+        vis.engine.num_planes = 2500
+        vis.engine.gpu_distance = True
+        vis.config.return_parts = False
+    print("  parts", r.num_parts, len(ref_parts), "concavity", r.concavity, ref_h, flush=True)
+    assert r.num_parts == len(ref_parts) >= 4
+    for p, (rv, rt) in zip(r.parts, ref_parts):
+        assert np.array_equal(np.array([list(v) for v in p.vertices]), rv)
+        assert np.array_equal(np.array([list(t) for t in p.triangles]), rt)
+    assert r.concavity == ref_h
+
+
+def test_process_cow_reference_sample():
+    """The reference's own sample input (data/cow.obj, via tests/golden/cow.npz)
+    with its CLI defaults (concavity 0.04, 40 parts): plain and batched give
+    the same decomposition, the parts partition the volume and the
+    concavity falls below the undivided mesh's."""
+    import os
+
+    import visacd as vis
+
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "cow.npz"))
+    V, T = d["vertices"], d["triangles"]
+    print("process cow", V.shape, T.shape, flush=True)
+    try:
+        vis.engine.mode = "plain"
+        r0 = _run(vis, V, T, 0.04, 40, return_parts=True)
+        vis.engine.mode = "batched"
+        r1 = _run(vis, V, T, 0.04, 40, return_parts=True)
+        vis.engine.mode = "plain"
+        r_one = _run(vis, V, T, 0.04, 1)
+    finally:
+        vis.engine.mode = "plain"
+        vis.config.return_parts = False
+    print("  parts", r0.num_parts, "concavity", r0.concavity, r_one.concavity, flush=True)
+    assert r0.num_parts >= 2 and r1.num_parts == r0.num_parts and r1.concavity == r0.concavity
+    assert _parts(r1) == _parts(r0)
+    vol = _volume(_mesh(vis, V, T))
+    assert abs(sum(_volume(p) for p in r0.parts) - vol) < 1e-6 * abs(vol)
+    assert r0.concavity < r_one.concavity

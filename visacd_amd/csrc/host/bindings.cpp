@@ -150,6 +150,72 @@ PYBIND11_MODULE(_visacd, m) {
         },
         py::arg("verts"), py::arg("edges"), py::arg("num_planes"), py::arg("seed"));
   m.def("select_plane", &vhost::select_plane, py::arg("score_q32"), py::arg("ss_offset"));
+  // the driver's split of a part given as arrays; None when the clip fails,
+  // else a list of (vertices, triangles, is_new, edges, points, cut_planes, seed)
+  m.def("split_part",
+        [](py::array_t<double, py::array::c_style | py::array::forcecast> V,
+           py::array_t<int, py::array::c_style | py::array::forcecast> T,
+           py::array_t<uint32_t, py::array::c_style | py::array::forcecast> E,
+           py::array_t<int, py::array::c_style | py::array::forcecast> points,
+           py::array_t<double, py::array::c_style | py::array::forcecast> cuts, uint64_t seed,
+           std::array<double, 4> plane) -> py::object {
+          vhost::Mesh m;
+          const size_t nv = (size_t)V.shape(0);
+          for (size_t i = 0; i < nv; ++i) m.vertices.push_back({V.data()[3 * i], V.data()[3 * i + 1], V.data()[3 * i + 2]});
+          for (py::ssize_t t = 0; t < T.shape(0); ++t)
+            m.triangles.push_back({T.data()[3 * t], T.data()[3 * t + 1], T.data()[3 * t + 2]});
+          for (py::ssize_t e = 0; e < E.shape(0); ++e) {
+            if (E.data()[2 * e] >= nv || E.data()[2 * e + 1] >= nv) throw std::out_of_range("edge endpoint out of range");
+            m.intersecting_edges.push_back({E.data()[2 * e], E.data()[2 * e + 1]});
+          }
+          for (py::ssize_t i = 0; i < points.shape(0); ++i) m.points.push_back(points.data()[i]);
+          for (py::ssize_t c = 0; c < cuts.shape(0); ++c)
+            m.cut_planes.emplace_back(cuts.data()[4 * c], cuts.data()[4 * c + 1], cuts.data()[4 * c + 2],
+                                      cuts.data()[4 * c + 3]);
+          vhost::MeshList kids;
+          std::vector<uint64_t> ks;
+          if (!vhost::split_part_seeded(m, seed, vhost::Plane(plane[0], plane[1], plane[2], plane[3]), kids, ks))
+            return py::none();
+          py::list out;
+          for (size_t k = 0; k < kids.size(); ++k) {
+            const vhost::Mesh &q = kids[k];
+            py::array_t<double> kv({(py::ssize_t)q.vertices.size(), (py::ssize_t)3});
+            for (size_t i = 0; i < q.vertices.size(); ++i)
+              for (int c = 0; c < 3; ++c) kv.mutable_data()[3 * i + c] = q.vertices[i][c];
+            py::array_t<int> kt({(py::ssize_t)q.triangles.size(), (py::ssize_t)3});
+            for (size_t t = 0; t < q.triangles.size(); ++t)
+              for (int c = 0; c < 3; ++c) kt.mutable_data()[3 * t + c] = q.triangles[t][c];
+            py::array_t<bool> kn((py::ssize_t)q.vertices.size());
+            for (size_t i = 0; i < q.vertices.size(); ++i) kn.mutable_data()[i] = i < q.is_new.size() && q.is_new[i];
+            py::array_t<uint32_t> ke({(py::ssize_t)q.intersecting_edges.size(), (py::ssize_t)2});
+            for (size_t e = 0; e < q.intersecting_edges.size(); ++e)
+              ke.mutable_data()[2 * e] = q.intersecting_edges[e].first, ke.mutable_data()[2 * e + 1] = q.intersecting_edges[e].second;
+            py::array_t<int> kp((py::ssize_t)q.points.size());
+            for (size_t i = 0; i < q.points.size(); ++i) kp.mutable_data()[i] = q.points[i];
+            py::array_t<double> kc({(py::ssize_t)q.cut_planes.size(), (py::ssize_t)4});
+            for (size_t c = 0; c < q.cut_planes.size(); ++c) {
+              const vhost::Plane &pl = q.cut_planes[c];
+              kc.mutable_data()[4 * c] = pl.a, kc.mutable_data()[4 * c + 1] = pl.b;
+              kc.mutable_data()[4 * c + 2] = pl.c, kc.mutable_data()[4 * c + 3] = pl.d;
+            }
+            out.append(py::make_tuple(kv, kt, kn, ke, kp, kc, ks[k]));
+          }
+          return out;
+        },
+        py::arg("vertices"), py::arg("triangles"), py::arg("edges"), py::arg("points"),
+        py::arg("cut_planes"), py::arg("seed"), py::arg("plane"));
+  m.def("sphere_draws", [](int num_planes, uint32_t seed) {
+    std::vector<double> n, f;
+    vhost::sphere_draws_seeded(num_planes, seed, n, f);
+    py::array_t<double> an({(py::ssize_t)f.size(), (py::ssize_t)3});
+    std::copy(n.begin(), n.end(), an.mutable_data());
+    py::array_t<double> af((py::ssize_t)f.size());
+    std::copy(f.begin(), f.end(), af.mutable_data());
+    return py::make_tuple(an, af);
+  });
+  m.def("hausdorff_boundary", [](const vhost::Mesh &a, const vhost::Mesh &b, int res, uint32_t seed) {
+    return vhost::hausdorff_boundary(a, b, res, seed);
+  });
   m.def("flat_surface_planes", [](const vhost::Mesh &mesh, double min_area) {
     std::vector<std::array<double, 4>> out;
     for (auto &p : vhost::flat_surface_planes(mesh, min_area)) out.push_back({p.a, p.b, p.c, p.d});

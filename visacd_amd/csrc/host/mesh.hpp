@@ -175,5 +175,13 @@ ProcessResult process(Mesh mesh, double concavity, int num_parts);
 // surfaces) times config.flat_surface_k (lib/src/process.cpp:128-134).
 std::vector<Plane> bisector_planes_seeded(const Mesh &m, int num_planes, uint32_t seed);
 int select_plane(const std::vector<int64_t> &q32, size_t ss_offset);
+// The driver's split of a part (clip, inherited edges through the 1-based
+// maps, connected pieces of at least 10 vertices with points, cut planes and
+// seeds; lib/src/process.cpp:136-168); false when the clip fails.
+bool split_part_seeded(const Mesh &mesh, uint64_t seed, const Plane &plane, MeshList &kids,
+                       std::vector<uint64_t> &kid_seeds);
+// The sphere sampler's draws (normals, offset fractions) with std::mt19937(seed).
+void sphere_draws_seeded(int num_planes, uint32_t seed, std::vector<double> &normals,
+                         std::vector<double> &frac);
 
 }  // namespace vhost

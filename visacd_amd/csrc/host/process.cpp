@@ -1272,6 +1272,19 @@ std::vector<Plane> bisector_planes_seeded(const Mesh &m, int num_planes, uint32_
   return bisector_planes(m, num_planes, rng);
 }
 
+bool split_part_seeded(const Mesh &mesh, uint64_t seed, const Plane &plane, MeshList &kids,
+                       std::vector<uint64_t> &kid_seeds) {
+  return split_part(mesh, seed, plane, kids, kid_seeds);
+}
+
+void sphere_draws_seeded(int num_planes, uint32_t seed, std::vector<double> &normals,
+                         std::vector<double> &frac) {
+  std::mt19937 rng(seed);
+  SphereDraws d = sphere_draws(num_planes, rng);
+  normals.swap(d.normals);
+  frac.swap(d.frac);
+}
+
 int select_plane(const std::vector<int64_t> &q32, size_t ss_offset) {
   std::vector<int> idx(q32.size());
   std::iota(idx.begin(), idx.end(), 0);

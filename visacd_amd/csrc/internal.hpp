@@ -328,6 +328,7 @@ void mlp_destroy(vacd_mlp *M);
 void mlp_create(vacd_ctx *ctx, uint64_t seed, vacd_mlp **out);
 void mlp_set(vacd_ctx *ctx, vacd_mlp *M, const float *host);
 void mlp_get(vacd_ctx *ctx, vacd_mlp *M, float *host);
+void mlp_get_grads(vacd_ctx *ctx, vacd_mlp *M, float *host);
 void mlp_forward(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
                  const float *planes, float *out);
 void mlp_forward_host(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,

@@ -700,6 +700,11 @@ void mlp_get(vacd_ctx *ctx, vacd_mlp *M, float *host) {
   VACD_HIP(hipStreamSynchronize(ctx->stream));
 }
 
+void mlp_get_grads(vacd_ctx *ctx, vacd_mlp *M, float *host) {
+  VACD_HIP(hipMemcpyAsync(host, M->g.ptr, 4 * M->L.n, hipMemcpyDeviceToHost, ctx->stream));
+  VACD_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 void mlp_forward(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
                  const float *planes, float *out) {
   check_dims(B, E, P);

@@ -20,6 +20,10 @@
 // 2^40, and a 3x3 determinant of differences (or the sum of two) is below
 // 2^124: computed exactly in signed 128-bit integers in registers.  Only
 // inputs of very different magnitude take the expansion path.
+//
+// Only compiler builtins are used (no device intrinsics), so the same code
+// also builds for the host (tests/predicates_host.cpp checks it there
+// against the oracle).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -58,7 +62,7 @@ __device__ __forceinline__ void exp_add3(Exp &e, double x, double y, double z) {
   const double xy = x * y;  // 48 significant bits: exact
   const double hi = xy * z;
   exp_add(e, hi);
-  exp_add(e, __fma_rn(xy, z, -hi));
+  exp_add(e, __builtin_fma(xy, z, -hi));  // exact remainder (fused)
 }
 
 // sign * det4 | r0 1 ; r1 1 ; r2 1 ; r3 1 | accumulated into e.
@@ -119,7 +123,7 @@ __device__ __forceinline__ bool int_grid(const float (&f)[N], int64_t (&m)[N]) {
   int lo = 1 << 20, hi = -(1 << 20);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    const unsigned b = __float_as_uint(f[i]);
+    const unsigned b = __builtin_bit_cast(unsigned, f[i]);
     const int be = (int)((b >> 23) & 0xffu);
     const int mm = (int)(b & 0x7fffffu) | (be ? 0x800000 : 0);
     m[i] = (b >> 31) ? -(int64_t)mm : (int64_t)mm;
@@ -225,17 +229,59 @@ __device__ __noinline__ int orient2d_mid_expansion(const float *a, const float *
   return exp_sign(e);
 }
 
+// Integer paths: false when the inputs' exponents are too far apart for
+// the grid; else the exact sign in *sign.
+__device__ __forceinline__ bool orient3d_int(const float *a, const float *b, const float *c,
+                                             const float *d, int *sign) {
+  const float f[12] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]};
+  int64_t m[12];
+  if (!int_grid(f, m)) return false;
+  *sign = sign128(det3_int(m));
+  return true;
+}
+
+__device__ __forceinline__ bool orient3d_mid_int(const float *a, const float *b, const float *c,
+                                                 const float *p, const float *q, int *sign) {
+  const float f[15] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2],
+                       p[0], p[1], p[2], q[0], q[1], q[2]};
+  int64_t m[15];
+  if (!int_grid(f, m)) return false;
+  const i128 dp = det3_int(m);
+  m[9] = m[12], m[10] = m[13], m[11] = m[14];
+  *sign = sign128(dp + det3_int(m));
+  return true;
+}
+
+__device__ __forceinline__ bool orient2d_int(const float *a, const float *b, const float *c, int k,
+                                             int *sign) {
+  const int u = proj_u(k), v = proj_v(k);
+  const float f[6] = {a[u], a[v], b[u], b[v], c[u], c[v]};
+  int64_t m[6];
+  if (!int_grid(f, m)) return false;
+  *sign = sign128(det2_int(m));
+  return true;
+}
+
+__device__ __forceinline__ bool orient2d_mid_int(const float *a, const float *b, const float *p,
+                                                 const float *q, int k, int *sign) {
+  const int u = proj_u(k), v = proj_v(k);
+  const float f[8] = {a[u], a[v], b[u], b[v], p[u], p[v], q[u], q[v]};
+  int64_t m[8];
+  if (!int_grid(f, m)) return false;
+  const i128 dp = det2_int(m);
+  m[4] = m[6], m[5] = m[7];
+  *sign = sign128(dp + det2_int(m));
+  return true;
+}
+
 __device__ __forceinline__ int orient3d_exact(const float *a, const float *b, const float *c,
                                               const float *d) {
   double bound;
   const double det = o3_fast(a, b, c, d, bound);
   if (det > bound) return 1;
   if (-det > bound) return -1;
-  {
-    const float f[12] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]};
-    int64_t m[12];
-    if (int_grid(f, m)) return sign128(det3_int(m));
-  }
+  int s;
+  if (orient3d_int(a, b, c, d, &s)) return s;
   return orient3d_expansion(a, b, c, d);
 }
 
@@ -249,16 +295,8 @@ __device__ __forceinline__ int orient3d_mid_exact(const float *a, const float *b
   const double bound = (b1 + b2) * 1.0000001 + 1e-15 * fabs(s);
   if (s > bound) return 1;
   if (-s > bound) return -1;
-  {
-    const float f[15] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2],
-                         p[0], p[1], p[2], q[0], q[1], q[2]};
-    int64_t m[15];
-    if (int_grid(f, m)) {
-      const i128 dp = det3_int(m);
-      m[9] = m[12], m[10] = m[13], m[11] = m[14];
-      return sign128(dp + det3_int(m));
-    }
-  }
+  int r;
+  if (orient3d_mid_int(a, b, c, p, q, &r)) return r;
   return orient3d_mid_expansion(a, b, c, p, q);
 }
 
@@ -268,12 +306,8 @@ __device__ __forceinline__ int orient2d_exact(const float *a, const float *b, co
   const double det = o2_fast(a, b, c, k, bound);
   if (det > bound) return 1;
   if (-det > bound) return -1;
-  {
-    const int u = proj_u(k), v = proj_v(k);
-    const float f[6] = {a[u], a[v], b[u], b[v], c[u], c[v]};
-    int64_t m[6];
-    if (int_grid(f, m)) return sign128(det2_int(m));
-  }
+  int s;
+  if (orient2d_int(a, b, c, k, &s)) return s;
   return orient2d_expansion(a, b, c, k);
 }
 
@@ -287,16 +321,8 @@ __device__ __forceinline__ int orient2d_mid_exact(const float *a, const float *b
   const double bound = (b1 + b2) * 1.0000001 + 1e-15 * fabs(s);
   if (s > bound) return 1;
   if (-s > bound) return -1;
-  {
-    const int u = proj_u(k), v = proj_v(k);
-    const float f[8] = {a[u], a[v], b[u], b[v], p[u], p[v], q[u], q[v]};
-    int64_t m[8];
-    if (int_grid(f, m)) {
-      const i128 dp = det2_int(m);
-      m[4] = m[6], m[5] = m[7];
-      return sign128(dp + det2_int(m));
-    }
-  }
+  int r;
+  if (orient2d_mid_int(a, b, p, q, k, &r)) return r;
   return orient2d_mid_expansion(a, b, p, q, k);
 }
 

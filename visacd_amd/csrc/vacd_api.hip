@@ -697,6 +697,14 @@ int vacd_mlp_get_params(vacd_ctx *ctx, vacd_mlp *mlp, float *params) {
   });
 }
 
+int vacd_mlp_get_grads(vacd_ctx *ctx, vacd_mlp *mlp, float *grads) {
+  if (!ctx || !mlp || !grads) return fail(ctx, Error{VACD_ERR_INVALID, "NULL argument"});
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    vacd::mlp_get_grads(ctx, mlp, grads);
+  });
+}
+
 int vacd_mlp_forward_d(vacd_ctx *ctx, vacd_mlp *mlp, int B, int E, const float *d_feat, int P,
                        const float *d_planes, float *d_out) {
   if (!ctx || !mlp || !d_feat || !d_planes || !d_out)

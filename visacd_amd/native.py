@@ -35,7 +35,7 @@ EXPORTED = [
     "vacd_score_planes_d", "vacd_score_planes_noalloc_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
     "vacd_max_distance", "vacd_max_distance_d", "vacd_max_distance_batch", "vacd_plane_extents_batch",
     "vacd_gemm_bf16_d", "vacd_mlp_param_count",
-    "vacd_mlp_create", "vacd_mlp_destroy", "vacd_mlp_set_params", "vacd_mlp_get_params",
+    "vacd_mlp_create", "vacd_mlp_destroy", "vacd_mlp_set_params", "vacd_mlp_get_params", "vacd_mlp_get_grads",
     "vacd_mlp_forward_d", "vacd_mlp_forward", "vacd_mlp_train_step_d", "classify_and_rate_planes",
 ]
 
@@ -151,6 +151,7 @@ def lib():
         L.vacd_mlp_destroy.argtypes = [vp]
         L.vacd_mlp_set_params.argtypes = [vp, vp, vp]
         L.vacd_mlp_get_params.argtypes = [vp, vp, vp]
+        L.vacd_mlp_get_grads.argtypes = [vp, vp, vp]
         L.vacd_mlp_forward_d.argtypes = [vp, vp, ci, ci, vp, ci, vp, vp]
         L.vacd_mlp_forward.argtypes = [vp, vp, ci, ci, vp, ci, vp, vp]
         L.vacd_mlp_train_step_d.argtypes = [vp, vp, ci, ci, vp, ci, vp, vp, ctypes.c_float,
@@ -420,6 +421,12 @@ class Mlp:
     def get(self) -> np.ndarray:
         out = np.empty(self.param_count(), dtype=np.float32)
         self.ctx._check(lib().vacd_mlp_get_params(self.ctx._h, self.h, _p(out)))
+        return out
+
+    def grads(self) -> np.ndarray:
+        """Gradient of the last training step's loss (parameter layout)."""
+        out = np.empty(self.param_count(), dtype=np.float32)
+        self.ctx._check(lib().vacd_mlp_get_grads(self.ctx._h, self.h, _p(out)))
         return out
 
     def set(self, params) -> None:
