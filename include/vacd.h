@@ -341,6 +341,33 @@ typedef struct vacd_distance_query {
 int vacd_max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int32_t nq,
                             float *out);
 
+/* ---- signed-distance remesh ------------------------------------------------
+ * Replaces manifold_preprocess / sdf_manifold (lib/src/preprocess.cpp:18-67,
+ * OpenVDB meshToSignedDistanceField + volumeToMesh), which the reference uses
+ * to remesh its input into a closed manifold (scale 30 or 20, level_set
+ * 0.55 / scale, lib/src/process.cpp:252-257) and to build the dilated cage
+ * its visibility rule tests against (scale 40, level_set 0.03 for the input
+ * and 0.02 per part, lib/src/process.cpp:170-171, 260-261).  The surface at
+ * signed distance level_set outside the mesh (V float32 [nV][3], T [nT][3]),
+ * sampled on a grid of nodes at integer multiples of 1 / scale: distances
+ * by the mesh's BVH, inside/outside by crossing parity along grid columns
+ * (closed meshes) or by the generalised winding number (open ones), the
+ * surface by surface nets (a vertex per straddling cell, two triangles per
+ * straddling grid edge, facing outwards; the result is closed).  Writes at
+ * most v_cap vertices and t_cap triangles, sets *nV_out and *nT_out, and
+ * returns VACD_ERR_CAPACITY when they do not fit.  Waits.  Limits: at most
+ * 2,048 grid nodes per axis. */
+int vacd_sdf_remesh(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+                    double scale, double level_set, float *V_out, int64_t v_cap,
+                    uint32_t *T_out, int64_t t_cap, int64_t *nV_out, int64_t *nT_out);
+/* The signed distance field underneath (negative inside) at the nodes
+ * lo + (i, j, k) * h of an n[0] x n[1] x n[2] grid, x fastest, into the host
+ * array out (n[0] n[1] n[2] floats); *sign_mode (nullable) receives 0 when
+ * the signs came from column parity (closed mesh) and 1 from winding numbers.
+ * Waits.  Limits: at most 2,048 nodes per axis. */
+int vacd_sdf_grid(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+                  const double *lo, double h, const int32_t *n, float *out, int32_t *sign_mode);
+
 /* ---- candidate-plane extents ---------------------------------------------
  * For each query, the extent [min, max] of n.p over its points p for each of
  * its P normals n, in float64 with the products and sums rounded as the host

@@ -253,3 +253,61 @@ def test_process_cow_reference_sample():
     vol = _volume(_mesh(vis, V, T))
     assert abs(sum(_volume(p) for p in r0.parts) - vol) < 1e-6 * abs(vol)
     assert r0.concavity < r_one.concavity
+
+
+def test_process_cage_mode_plain_and_batched_agree():
+    """engine.visibility = "cage": the reference's rule on the engine's cages
+    (the input's at distance 0.03, each new part's at 0.02, on a 1/40 grid).
+    Plain and batched give the same decomposition, whose parts partition the
+    volume."""
+    import visacd as vis
+    from visacd_amd import _visacd
+
+    V, T = wl.perturbed_icosphere(3, 0.4, 1)
+    print("process cage mode", flush=True)
+    try:
+        vis.engine.visibility = "cage"
+        vis.engine.mode = "plain"
+        r0 = _run(vis, V, T, 0.02, 6, return_parts=True)
+        prof = _visacd.last_profile()
+        vis.engine.mode = "batched"
+        r1 = _run(vis, V, T, 0.02, 6, return_parts=True)
+    finally:
+        vis.engine.visibility = "parity"
+        vis.engine.mode = "plain"
+        vis.config.return_parts = False
+    print("  parts", r0.num_parts, r0.concavity, "cage builds", prof.get("visibility.cage"), flush=True)
+    assert prof["visibility.cage"][1] >= 2
+    assert r0.num_parts >= 2 and r1.num_parts == r0.num_parts and r1.concavity == r0.concavity
+    assert _parts(r1) == _parts(r0)
+    vol = _volume(_mesh(vis, V, T))
+    assert abs(sum(_volume(p) for p in r0.parts) - vol) < 1e-6 * abs(vol)
+
+
+def test_process_remeshes_an_open_input():
+    """An input with a hole (not closed) is remeshed first (engine.remesh
+    "auto", the reference's scale 30 and level 0.55/30) and decomposed; with
+    "always" a closed input is remeshed too; the parts partition the
+    remeshed volume, slightly larger than the input's."""
+    import visacd as vis
+    from visacd_amd import _visacd
+
+    V, T = wl.perturbed_icosphere(3, 0.3, 2)
+    open_T = np.ascontiguousarray(T[4:])
+    print("process open input", flush=True)
+    try:
+        r = _run(vis, V, open_T, 0.03, 5, return_parts=True)
+        prof = _visacd.last_profile()
+        vis.engine.remesh = "always"
+        r2 = _run(vis, V, T, 0.03, 5, return_parts=True)
+        prof2 = _visacd.last_profile()
+    finally:
+        vis.engine.remesh = "auto"
+        vis.config.return_parts = False
+    assert "remesh" in prof and "remesh" in prof2
+    vol = _volume(_mesh(vis, V, T))
+    for res in (r, r2):
+        total = sum(_volume(p) for p in res.parts)
+        print("  parts", res.num_parts, "volume", total, vol, flush=True)
+        assert res.num_parts >= 2
+        assert vol < total < 1.15 * vol

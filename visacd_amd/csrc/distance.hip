@@ -11,6 +11,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace vacd {
@@ -432,6 +433,469 @@ void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, flo
   VACD_HIP(hipGetLastError());
   VACD_HIP(hipMemcpyAsync(out, dout, 4 * (size_t)nq, hipMemcpyDeviceToHost, st));
   VACD_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace vacd
+
+// ------------------------------------------------------ signed-distance remesh
+// Replaces the OpenVDB pipeline of lib/src/preprocess.cpp:18-61 (mesh ->
+// narrow-band signed distance field -> iso-surface at level_set, voxel
+// 1 / scale), used by the reference to remesh its input into a closed
+// manifold (scale 30 or 20, level 0.55 voxel) and to build the dilated
+// "cage" it tests segments against (scale 40, level 0.03 / 0.02).  Here:
+//   k_grid_dist    unsigned distance of every grid node to the mesh, by the
+//                  closest-point walk of the mesh's 4-wide BVH (one lane per
+//                  node);
+//   k_grid_parity  sign of a closed mesh: per (y, z) column, the crossings of
+//                  the +x line with the triangles (triangles staged in LDS,
+//                  one lane per column; the column is offset by a small
+//                  irrational fraction of a voxel so it misses edges and
+//                  vertices), a node inside when an odd number lie below it;
+//   k_grid_wind    sign of an open mesh: generalised winding number (sum of
+//                  the triangles' solid angles over 4 pi, van Oosterom and
+//                  Strackee's formula) above one half;
+//   k_sn_cells / k_sn_quads  the iso-surface by surface nets: one vertex per
+//                  cell whose corners straddle the level (the mean of its
+//                  edge crossings), one quad per grid edge that straddles it,
+//                  joining the four cells around that edge and facing from
+//                  inside to outside; every dual edge is shared by an even
+//                  number of quads, so the output is closed.
+// Cell and quad lists are compacted by exclusive scans, so the output order
+// is deterministic.
+namespace vacd {
+namespace {
+
+struct Grid {
+  double lo[3];  // node (0, 0, 0)
+  double h;      // spacing
+  int n[3];      // nodes per axis
+};
+
+__device__ __forceinline__ int64_t gidx(const Grid &g, int i, int j, int k) {
+  return ((int64_t)k * g.n[1] + j) * g.n[0] + i;
+}
+
+__device__ float mesh_d2(const float p[3], const Node4 *__restrict__ nodes,
+                         const Tri *__restrict__ tris, int cap, int *stk) {
+  float bd = INFINITY;
+  int node = 0, sp = 0;
+  for (int it = 0; it <= cap; ++it) {
+    const Node4 nd = nodes[node];
+    float d[4];
+    int id[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      d[c] = box_d2(p, comp4(nd.f[0], c), comp4(nd.f[1], c), comp4(nd.f[2], c),
+                    comp4(nd.f[3], c), comp4(nd.f[4], c), comp4(nd.f[5], c));
+      id[c] = __float_as_int(comp4(nd.f[6], c));
+      if (comp4(nd.f[0], c) == kEmpty) d[c] = INFINITY;
+    }
+    int next = -1;
+    float next_d = INFINITY;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (!(d[c] < bd)) continue;
+      if (id[c] < 0) {
+        const Tri t = tris[~id[c]];
+        bd = fminf(bd, tri_d2(p, t.a, t.b, t.c));
+      } else if (d[c] < next_d) {
+        if (next >= 0 && sp < kStack) stk[(sp++) * 64] = next;
+        next = id[c];
+        next_d = d[c];
+      } else if (sp < kStack) {
+        stk[(sp++) * 64] = id[c];
+      }
+    }
+    if (next >= 0) {
+      node = next;
+    } else {
+      if (sp == 0) break;
+      node = stk[(--sp) * 64];
+    }
+  }
+  return bd;
+}
+
+__global__ __launch_bounds__(256) void k_grid_dist(Grid g, const Node4 *__restrict__ nodes,
+                                                   const Tri *__restrict__ tris, int cap,
+                                                   float *__restrict__ dist) {
+  __shared__ int stk_all[4][kStack][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t N = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  const int i = (int)(v % g.n[0]), j = (int)((v / g.n[0]) % g.n[1]), k = (int)(v / ((int64_t)g.n[0] * g.n[1]));
+  const float p[3] = {(float)(g.lo[0] + i * g.h), (float)(g.lo[1] + j * g.h), (float)(g.lo[2] + k * g.h)};
+  dist[v] = sqrtf(mesh_d2(p, nodes, tris, cap, &stk_all[wave][0][lane]));
+}
+
+constexpr int kMaxCross = 64;    // crossings kept per column
+constexpr int kTriStage = 256;   // triangles per LDS stage
+
+// Column (j, k) of the grid: the +x line at y_j + 0.000123 h, z_k + 0.0000789 h.
+__global__ __launch_bounds__(64) void k_grid_parity(Grid g, const float *__restrict__ V,
+                                                    const uint32_t *__restrict__ T, int64_t nT,
+                                                    float *__restrict__ dist,
+                                                    unsigned *__restrict__ overflow) {
+  __shared__ float sv[kTriStage][9];
+  __shared__ float xs[kMaxCross][64];
+  const int lane = threadIdx.x;
+  const int64_t col = blockIdx.x * (int64_t)64 + lane;
+  const int64_t ncol = (int64_t)g.n[1] * g.n[2];
+  const bool live = col < ncol;
+  const int j = live ? (int)(col % g.n[1]) : 0, k = live ? (int)(col / g.n[1]) : 0;
+  const double y = g.lo[1] + (j + 1.23e-4) * g.h, z = g.lo[2] + (k + 7.89e-5) * g.h;
+  int cnt = 0;
+  for (int64_t t0 = 0; t0 < nT; t0 += kTriStage) {
+    const int m = (int)(nT - t0 < kTriStage ? nT - t0 : kTriStage);
+    __syncthreads();
+    for (int x = lane; x < 9 * m; x += 64) {
+      const int t = x / 9, r = x % 9;
+      sv[t][r] = V[3 * (size_t)T[3 * (t0 + t) + r / 3] + r % 3];
+    }
+    __syncthreads();
+    if (!live) continue;
+    for (int t = 0; t < m; ++t) {
+      const double ay = sv[t][1], az = sv[t][2], by = sv[t][4], bz = sv[t][5], cy = sv[t][7], cz = sv[t][8];
+      // (y, z) inside the projected triangle: three orientations of one sign
+      const double o1 = (by - ay) * (z - az) - (bz - az) * (y - ay);
+      const double o2 = (cy - by) * (z - bz) - (cz - bz) * (y - by);
+      const double o3 = (ay - cy) * (z - cz) - (az - cz) * (y - cy);
+      if (!((o1 > 0 && o2 > 0 && o3 > 0) || (o1 < 0 && o2 < 0 && o3 < 0))) continue;
+      const double s = o1 + o2 + o3;  // twice the projected area, signed
+      // barycentric weights of a, b, c: o2, o3, o1 over s
+      const double x = (o2 * sv[t][0] + o3 * sv[t][3] + o1 * sv[t][6]) / s;
+      if (cnt < kMaxCross) xs[cnt][lane] = (float)x;
+      ++cnt;
+    }
+  }
+  if (!live) return;
+  if (cnt > kMaxCross) {
+    atomicOr(overflow, 1u);
+    return;
+  }
+  for (int i = 0; i < g.n[0]; ++i) {
+    const float xi = (float)(g.lo[0] + i * g.h);
+    int below = 0;
+    for (int c = 0; c < cnt; ++c) below += xs[c][lane] < xi;
+    const int64_t v = gidx(g, i, j, k);
+    if (below & 1) dist[v] = -dist[v];
+  }
+}
+
+// Generalised winding number of every node (open meshes): inside when the
+// solid angle sum exceeds 2 pi.
+__global__ __launch_bounds__(256) void k_grid_wind(Grid g, const float *__restrict__ V,
+                                                   const uint32_t *__restrict__ T, int64_t nT,
+                                                   float *__restrict__ dist) {
+  __shared__ float sv[kTriStage][9];
+  const int64_t N = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool live = v < N;
+  double p[3] = {0, 0, 0};
+  if (live) {
+    const int i = (int)(v % g.n[0]), j = (int)((v / g.n[0]) % g.n[1]), k = (int)(v / ((int64_t)g.n[0] * g.n[1]));
+    p[0] = g.lo[0] + i * g.h, p[1] = g.lo[1] + j * g.h, p[2] = g.lo[2] + k * g.h;
+  }
+  double w = 0.0;
+  for (int64_t t0 = 0; t0 < nT; t0 += kTriStage) {
+    const int m = (int)(nT - t0 < kTriStage ? nT - t0 : kTriStage);
+    __syncthreads();
+    for (int x = threadIdx.x; x < 9 * m; x += 256) {
+      const int t = x / 9, r = x % 9;
+      sv[t][r] = V[3 * (size_t)T[3 * (t0 + t) + r / 3] + r % 3];
+    }
+    __syncthreads();
+    if (!live) continue;
+    for (int t = 0; t < m; ++t) {
+      const double a[3] = {sv[t][0] - p[0], sv[t][1] - p[1], sv[t][2] - p[2]};
+      const double b[3] = {sv[t][3] - p[0], sv[t][4] - p[1], sv[t][5] - p[2]};
+      const double c[3] = {sv[t][6] - p[0], sv[t][7] - p[1], sv[t][8] - p[2]};
+      const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      const double lb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      const double lc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      const double det = a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                         a[2] * (b[0] * c[1] - b[1] * c[0]);
+      const double den = la * lb * lc + (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) * lc +
+                         (a[0] * c[0] + a[1] * c[1] + a[2] * c[2]) * lb +
+                         (b[0] * c[0] + b[1] * c[1] + b[2] * c[2]) * la;
+      w += 2.0 * atan2(det, den);
+    }
+  }
+  if (live && fabs(w) > 2.0 * 3.14159265358979323846) dist[v] = -dist[v];
+}
+
+// Cell (i, j, k) spans nodes i..i+1, j..j+1, k..k+1; corner c = bits (x, y, z).
+__global__ __launch_bounds__(256) void k_sn_cells(Grid g, const float *__restrict__ sdf, float iso,
+                                                  uint32_t *__restrict__ flag,
+                                                  float *__restrict__ cvert) {
+  const int cx = g.n[0] - 1, cy = g.n[1] - 1, cz = g.n[2] - 1;
+  const int64_t C = (int64_t)cx * cy * cz;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const int i = (int)(c % cx), j = (int)((c / cx) % cy), k = (int)(c / ((int64_t)cx * cy));
+  float v[8];
+  int in = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    v[q] = sdf[gidx(g, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2))] - iso;
+    in += v[q] < 0.0f;
+  }
+  if (in == 0 || in == 8) {
+    flag[c] = 0;
+    return;
+  }
+  flag[c] = 1;
+  // the 12 cube edges as corner pairs
+  const int E[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3},
+                        {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  double acc[3] = {0, 0, 0};
+  int ne = 0;
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    const int a = E[e][0], b = E[e][1];
+    if ((v[a] < 0.0f) == (v[b] < 0.0f)) continue;
+    const double t = (double)v[a] / ((double)v[a] - (double)v[b]);
+    const int da[3] = {a & 1, (a >> 1) & 1, a >> 2}, db[3] = {b & 1, (b >> 1) & 1, b >> 2};
+    acc[0] += da[0] + t * (db[0] - da[0]);
+    acc[1] += da[1] + t * (db[1] - da[1]);
+    acc[2] += da[2] + t * (db[2] - da[2]);
+    ++ne;
+  }
+  cvert[3 * c] = (float)(g.lo[0] + (i + acc[0] / ne) * g.h);
+  cvert[3 * c + 1] = (float)(g.lo[1] + (j + acc[1] / ne) * g.h);
+  cvert[3 * c + 2] = (float)(g.lo[2] + (k + acc[2] / ne) * g.h);
+}
+
+// Grid edge e along axis a (0 x, 1 y, 2 z) from node (i, j, k): edge ids
+// a * N + node.  A quad when the edge straddles the level and its four cells
+// exist; mode 0 flags, mode 1 writes the two triangles (vertex ids through
+// the cell scan) at the quad's scanned slot.
+__global__ __launch_bounds__(256) void k_sn_quads(Grid g, const float *__restrict__ sdf, float iso,
+                                                  int mode, uint32_t *__restrict__ qflag,
+                                                  const uint32_t *__restrict__ cell_id,
+                                                  uint32_t *__restrict__ tris) {
+  const int64_t N = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= 3 * N) return;
+  const int a = (int)(e / N);
+  const int64_t v = e % N;
+  int p[3] = {(int)(v % g.n[0]), (int)((v / g.n[0]) % g.n[1]), (int)(v / ((int64_t)g.n[0] * g.n[1]))};
+  const int u = (a + 1) % 3, w = (a + 2) % 3;
+  bool ok = p[a] + 1 < g.n[a] && p[u] >= 1 && p[w] >= 1 && p[u] <= g.n[u] - 2 && p[w] <= g.n[w] - 2;
+  float v0 = 0.f, v1 = 0.f;
+  if (ok) {
+    v0 = sdf[v] - iso;
+    int q[3] = {p[0], p[1], p[2]};
+    q[a] += 1;
+    v1 = sdf[gidx(g, q[0], q[1], q[2])] - iso;
+    ok = (v0 < 0.0f) != (v1 < 0.0f);
+  }
+  if (mode == 0) {
+    qflag[e] = ok ? 1u : 0u;
+    return;
+  }
+  if (!ok) return;
+  const int cx = g.n[0] - 1, cy = g.n[1] - 1;
+  uint32_t id[4];
+  const int du[4] = {-1, 0, 0, -1}, dw[4] = {-1, -1, 0, 0};  // around the edge: +u then +w
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    int c[3] = {p[0], p[1], p[2]};
+    c[u] += du[r];
+    c[w] += dw[r];
+    id[r] = cell_id[((int64_t)c[2] * cy + c[1]) * cx + c[0]];
+  }
+  const uint32_t o = 6 * qflag[e];
+  if (v0 < 0.0f) {  // inside at the lower end: the surface faces +a
+    tris[o] = id[0], tris[o + 1] = id[1], tris[o + 2] = id[2];
+    tris[o + 3] = id[0], tris[o + 4] = id[2], tris[o + 5] = id[3];
+  } else {
+    tris[o] = id[0], tris[o + 1] = id[2], tris[o + 2] = id[1];
+    tris[o + 3] = id[0], tris[o + 4] = id[3], tris[o + 5] = id[2];
+  }
+}
+
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+
+__global__ __launch_bounds__(256) void k_sn_compact(int64_t C, const uint32_t *__restrict__ flag,
+                                                    const uint32_t *__restrict__ cell_id,
+                                                    const float *__restrict__ cvert,
+                                                    float *__restrict__ out) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= C || !flag[c]) return;
+  const uint32_t o = cell_id[c];
+  out[3 * (size_t)o] = cvert[3 * c], out[3 * (size_t)o + 1] = cvert[3 * c + 1];
+  out[3 * (size_t)o + 2] = cvert[3 * c + 2];
+}
+
+}  // namespace
+
+namespace {
+// Signed distances (negative inside) of every node of g into sdf (device,
+// N floats); dv, dt: the mesh on the device, bvh its BVH.  Returns the sign
+// method: 0 column parity (closed mesh), 1 winding number.
+int signed_grid(vacd_ctx *ctx, const Grid &g, const vacd_bvh &bvh, const float *dv,
+                const uint32_t *dt, int64_t nT, float *sdf, unsigned *over) {
+  hipStream_t st = ctx->stream;
+  const int64_t N = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  const int cap = (int)std::min<int64_t>(2 * nT + 64, INT32_MAX);
+  k_grid_dist<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(g, bvh.nodes4.as<Node4>(),
+                                                          bvh.tris.as<Tri>(), cap, sdf);
+  int mode = bvh.closed ? 0 : 1;
+  if (mode == 0) {
+    k_zero_u32<<<1, 1, 0, st>>>(over);
+    const int64_t ncol = (int64_t)g.n[1] * g.n[2];
+    k_grid_parity<<<(unsigned)((ncol + 63) / 64), 64, 0, st>>>(g, dv, dt, nT, sdf, over);
+    unsigned h_over = 0;
+    VACD_HIP(hipMemcpyAsync(&h_over, over, 4, hipMemcpyDeviceToHost, st));
+    VACD_HIP(hipStreamSynchronize(st));
+    if (h_over) {  // a column met too many crossings: distances again, signs by winding numbers
+      mode = 1;
+      k_grid_dist<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(g, bvh.nodes4.as<Node4>(),
+                                                              bvh.tris.as<Tri>(), cap, sdf);
+    }
+  }
+  if (mode == 1)
+    k_grid_wind<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(g, dv, dt, nT, sdf);
+  VACD_HIP(hipGetLastError());
+  return mode;
+}
+}  // namespace
+
+void sdf_grid(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+              const double lo[3], double h, const int n[3], float *out, int *sign_mode) {
+  VACD_REQUIRE(nV >= 3 && nT >= 1, VACD_ERR_INVALID, "signed distances need a mesh");
+  VACD_REQUIRE(h > 0.0 && std::isfinite(h), VACD_ERR_INVALID, "grid spacing must be positive");
+  for (int64_t t = 0; t < 3 * nT; ++t)
+    VACD_REQUIRE((int64_t)T[t] < nV, VACD_ERR_INVALID, "triangle index out of range");
+  Grid g;
+  int64_t N = 1;
+  for (int k = 0; k < 3; ++k) {
+    VACD_REQUIRE(n[k] >= 1 && n[k] <= 2048, VACD_ERR_LIMIT, "grid needs 1..2048 nodes per axis");
+    g.lo[k] = lo[k];
+    g.n[k] = n[k];
+    N *= n[k];
+  }
+  g.h = h;
+  hipStream_t st = ctx->stream;
+  vacd_bvh bvh;
+  DevBuf dv, dt, work;
+  try {
+    dv.ensure(12 * (size_t)nV);
+    dt.ensure(12 * (size_t)nT);
+    VACD_HIP(hipMemcpyAsync(dv.ptr, V, 12 * (size_t)nV, hipMemcpyHostToDevice, st));
+    VACD_HIP(hipMemcpyAsync(dt.ptr, T, 12 * (size_t)nT, hipMemcpyHostToDevice, st));
+    build_lbvh(ctx, dv.as<float>(), nV, dt.as<uint32_t>(), nT, &bvh);
+    work.ensure(4 * (size_t)N + 256);
+    const int mode = signed_grid(ctx, g, bvh, dv.as<float>(), dt.as<uint32_t>(), nT, work.as<float>(),
+                                 reinterpret_cast<unsigned *>(work.as<char>() + 4 * (size_t)N));
+    if (sign_mode) *sign_mode = mode;
+    VACD_HIP(hipMemcpyAsync(out, work.ptr, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
+    VACD_HIP(hipStreamSynchronize(st));
+  } catch (...) {
+    bvh.release(), dv.release(), dt.release(), work.release();
+    throw;
+  }
+  bvh.release(), dv.release(), dt.release(), work.release();
+}
+
+void sdf_remesh(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+                double scale, double level_set, std::vector<float> &Vout,
+                std::vector<uint32_t> &Tout, int *sign_mode) {
+  VACD_REQUIRE(nV >= 3 && nT >= 1, VACD_ERR_INVALID, "remesh needs a mesh");
+  VACD_REQUIRE(scale > 0.0 && std::isfinite(scale) && level_set >= 0.0 && std::isfinite(level_set),
+               VACD_ERR_INVALID, "remesh needs scale > 0 and level_set >= 0");
+  for (int64_t t = 0; t < 3 * nT; ++t)
+    VACD_REQUIRE((int64_t)T[t] < nV, VACD_ERR_INVALID, "triangle index out of range");
+  hipStream_t st = ctx->stream;
+  // grid: nodes at integer multiples of 1 / scale (the reference's index
+  // space), covering the mesh's box widened by the level and two voxels
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) lo[k] = INFINITY, hi[k] = -INFINITY;
+  for (int64_t v = 0; v < nV; ++v)
+    for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], (double)V[3 * v + k]), hi[k] = std::max(hi[k], (double)V[3 * v + k]);
+  const int margin = (int)std::ceil(level_set * scale) + 2;
+  Grid g;
+  g.h = 1.0 / scale;
+  int64_t N = 1;
+  for (int k = 0; k < 3; ++k) {
+    const int64_t i0 = (int64_t)std::floor(lo[k] * scale) - margin;
+    const int64_t i1 = (int64_t)std::ceil(hi[k] * scale) + margin;
+    VACD_REQUIRE(i1 - i0 + 1 <= 2048, VACD_ERR_LIMIT, "remesh grid above 2048 nodes per axis");
+    g.lo[k] = (double)i0 / scale;
+    g.n[k] = (int)(i1 - i0 + 1);
+    N *= g.n[k];
+  }
+  VACD_REQUIRE(N < (int64_t(1) << 30), VACD_ERR_LIMIT, "remesh grid too large");
+  // the mesh's BVH for the distances (closedness picks the sign method)
+  vacd_bvh bvh;
+  DevBuf dv, dt;
+  dv.ensure(12 * (size_t)nV);
+  dt.ensure(12 * (size_t)nT);
+  VACD_HIP(hipMemcpyAsync(dv.ptr, V, 12 * (size_t)nV, hipMemcpyHostToDevice, st));
+  VACD_HIP(hipMemcpyAsync(dt.ptr, T, 12 * (size_t)nT, hipMemcpyHostToDevice, st));
+  try {
+    build_lbvh(ctx, dv.as<float>(), nV, dt.as<uint32_t>(), nT, &bvh);
+    const int64_t C = (int64_t)(g.n[0] - 1) * (g.n[1] - 1) * (g.n[2] - 1);
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_sdf = 0, o_flag = o_sdf + up(4 * (size_t)N), o_cvert = o_flag + up(4 * (size_t)(C + 1)),
+                 o_qflag = o_cvert + up(12 * (size_t)C), o_over = o_qflag + up(4 * (size_t)(3 * N + 1)),
+                 bytes = o_over + 256;
+    DevBuf work;
+    work.ensure(bytes);
+    char *base = work.as<char>();
+    float *sdf = reinterpret_cast<float *>(base + o_sdf);
+    uint32_t *flag = reinterpret_cast<uint32_t *>(base + o_flag);
+    float *cvert = reinterpret_cast<float *>(base + o_cvert);
+    uint32_t *qflag = reinterpret_cast<uint32_t *>(base + o_qflag);
+    unsigned *over = reinterpret_cast<unsigned *>(base + o_over);
+    const int mode = signed_grid(ctx, g, bvh, dv.as<float>(), dt.as<uint32_t>(), nT, sdf, over);
+    if (sign_mode) *sign_mode = mode;
+    const float iso = (float)level_set;
+    k_sn_cells<<<(unsigned)((C + 255) / 256), 256, 0, st>>>(g, sdf, iso, flag, cvert);
+    k_zero_u32<<<1, 1, 0, st>>>(flag + C);  // the scan's total lands here
+    scan_u32(ctx, flag, C + 1);
+    k_sn_quads<<<(unsigned)((3 * N + 255) / 256), 256, 0, st>>>(g, sdf, iso, 0, qflag, nullptr, nullptr);
+    k_zero_u32<<<1, 1, 0, st>>>(qflag + 3 * N);
+    scan_u32(ctx, qflag, 3 * N + 1);
+    uint32_t nverts = 0, nquads = 0;
+    VACD_HIP(hipMemcpyAsync(&nverts, flag + C, 4, hipMemcpyDeviceToHost, st));
+    VACD_HIP(hipMemcpyAsync(&nquads, qflag + 3 * N, 4, hipMemcpyDeviceToHost, st));
+    VACD_HIP(hipStreamSynchronize(st));
+    // the cell flags were scanned in place: flag[c] is now the cell's vertex
+    // id; k_sn_cells runs again to restore the flags for the compaction
+    DevBuf outv, outt, ids;
+    outv.ensure(12 * (size_t)std::max<uint32_t>(nverts, 1));
+    outt.ensure(24 * (size_t)std::max<uint32_t>(nquads, 1));
+    ids.ensure(4 * (size_t)(C + 1));
+    VACD_HIP(hipMemcpyAsync(ids.ptr, flag, 4 * (size_t)(C + 1), hipMemcpyDeviceToDevice, st));
+    k_sn_cells<<<(unsigned)((C + 255) / 256), 256, 0, st>>>(g, sdf, iso, flag, cvert);
+    k_sn_compact<<<(unsigned)((C + 255) / 256), 256, 0, st>>>(C, flag, ids.as<uint32_t>(), cvert,
+                                                             outv.as<float>());
+    k_sn_quads<<<(unsigned)((3 * N + 255) / 256), 256, 0, st>>>(g, sdf, iso, 1, qflag,
+                                                               ids.as<uint32_t>(), outt.as<uint32_t>());
+    VACD_HIP(hipGetLastError());
+    Vout.resize(3 * (size_t)nverts);
+    Tout.resize(6 * (size_t)nquads);
+    if (nverts)
+      VACD_HIP(hipMemcpyAsync(Vout.data(), outv.ptr, 12 * (size_t)nverts, hipMemcpyDeviceToHost, st));
+    if (nquads)
+      VACD_HIP(hipMemcpyAsync(Tout.data(), outt.ptr, 24 * (size_t)nquads, hipMemcpyDeviceToHost, st));
+    VACD_HIP(hipStreamSynchronize(st));
+    work.release();
+    outv.release();
+    outt.release();
+    ids.release();
+  } catch (...) {
+    bvh.release();
+    dv.release();
+    dt.release();
+    throw;
+  }
+  bvh.release();
+  dv.release();
+  dt.release();
 }
 
 }  // namespace vacd

@@ -73,6 +73,8 @@ PYBIND11_MODULE(_visacd, m) {
       .def_readwrite("sample_seed", &vhost::EngineOptions::sample_seed)
       .def_readwrite("gpu_distance", &vhost::EngineOptions::gpu_distance)
       .def_readwrite("cap_rule", &vhost::EngineOptions::cap_rule)
+      .def_readwrite("visibility", &vhost::EngineOptions::visibility)
+      .def_readwrite("remesh", &vhost::EngineOptions::remesh)
       .def_readwrite("mode", &vhost::EngineOptions::mode)
       .def_readwrite("batch_width", &vhost::EngineOptions::batch_width)
       .def_readwrite("eager_visibility", &vhost::EngineOptions::eager_visibility)

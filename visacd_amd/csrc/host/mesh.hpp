@@ -85,6 +85,15 @@ struct EngineOptions {
   int hausdorff_resolution = 3000;
   uint32_t sample_seed = 42;     // seeds subsampling and surface sampling
   bool gpu_distance = true;      // Hausdorff term on the GPU (exact point-to-mesh)
+  // "parity": a pair is an edge when its segment touches no triangle and its
+  // midpoint is outside the part; "cage": when it touches no triangle and
+  // touches the part's cage, the surface at distance 0.02 (0.03 for the
+  // input) on a 1/40 grid, as the reference does (lib/ptx/ray_segments.cu:
+  // 96-110, lib/src/process.cpp:170-171, 260-261)
+  std::string visibility = "parity";
+  // input remesh (lib/src/process.cpp:250-258): "auto" remeshes inputs that
+  // are not closed, "always" every input (as the reference), "never" none
+  std::string remesh = "auto";
   // cap rule: a pair of two points on one cut plane whose segment runs into
   // that cut's cap at an end touches the cap and is not an edge, decided by
   // exact 2-D orientations in the plane instead of by the rounded cap

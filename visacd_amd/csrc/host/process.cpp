@@ -236,10 +236,66 @@ std::vector<uint8_t> point_mask(const Mesh &part) {
   return mask;
 }
 
+// The surface at distance `level` outside a mesh on a 1 / scale grid
+// (vacd_sdf_remesh; lib/src/preprocess.cpp:18-61).
+Mesh sdf_remesh_mesh(vacd_ctx *ctx, const Mesh &m, double scale, double level) {
+  auto check = [ctx](int rc) {
+    if (rc != VACD_OK) throw std::runtime_error(std::string("visacd: ") + vacd_last_error(ctx));
+  };
+  const std::vector<float> V = to_f32(m.vertices);
+  std::vector<uint32_t> T(3 * m.triangles.size());
+  for (size_t t = 0; t < m.triangles.size(); ++t)
+    for (int k = 0; k < 3; ++k) T[3 * t + k] = (uint32_t)m.triangles[t][k];
+  int64_t vc = 4 * (int64_t)m.vertices.size() + 4096, tc = 8 * (int64_t)m.triangles.size() + 8192;
+  std::vector<float> vo;
+  std::vector<uint32_t> to;
+  int64_t nv = 0, nt = 0;
+  for (;;) {
+    vo.resize(3 * (size_t)vc);
+    to.resize(3 * (size_t)tc);
+    const int rc = vacd_sdf_remesh(ctx, V.data(), (int64_t)m.vertices.size(), T.data(),
+                                   (int64_t)m.triangles.size(), scale, level, vo.data(), vc,
+                                   to.data(), tc, &nv, &nt);
+    if (rc == VACD_ERR_CAPACITY) {
+      vc = std::max(vc, nv), tc = std::max(tc, nt);
+      continue;
+    }
+    check(rc);
+    break;
+  }
+  Mesh out;
+  out.vertices.resize((size_t)nv);
+  for (int64_t v = 0; v < nv; ++v) out.vertices[v] = {vo[3 * v], vo[3 * v + 1], vo[3 * v + 2]};
+  out.triangles.resize((size_t)nt);
+  for (int64_t t = 0; t < nt; ++t) out.triangles[t] = {(int)to[3 * t], (int)to[3 * t + 1], (int)to[3 * t + 2]};
+  return out;
+}
+
+// The cage a part's pairs are tested against in cage mode (engine.visibility
+// = "cage"): its surface at distance `level` on a 1/40 grid
+// (lib/src/process.cpp:170-171 per part with 0.02, 260-261 for the input
+// with 0.03), as a BVH; nullptr in parity mode.
+vacd_bvh *build_cage(vacd_ctx *ctx, const Mesh &src, double level) {
+  if (engine.visibility != "cage") return nullptr;
+  PhaseTimer t("visibility.cage");
+  const Mesh cage = sdf_remesh_mesh(ctx, src, 40.0, level);
+  if (cage.triangles.empty()) throw std::runtime_error("visacd: empty cage");
+  const std::vector<float> V = to_f32(cage.vertices);
+  std::vector<uint32_t> T(3 * cage.triangles.size());
+  for (size_t k = 0; k < cage.triangles.size(); ++k)
+    for (int c = 0; c < 3; ++c) T[3 * k + c] = (uint32_t)cage.triangles[k][c];
+  vacd_bvh *b = nullptr;
+  if (vacd_bvh_build(ctx, V.data(), (int64_t)cage.vertices.size(), T.data(), (int64_t)cage.triangles.size(),
+                     &b) != VACD_OK)
+    throw std::runtime_error(std::string("visacd: ") + vacd_last_error(ctx));
+  return b;
+}
+
 // Visibility of a part in parity mode; with_mask restricts to pairs touching
 // a new vertex (lib/ptx/ray_segments.cu:61-75).  Edges are appended in k order
 // as pairs of part vertex ids.
-void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf) {
+void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf,
+                    const vacd_bvh *cage = nullptr) {
   if (part.points.size() < 2) return;
   PhaseTimer timer("visibility");
   std::vector<float> V = to_f32(part.vertices);
@@ -270,7 +326,7 @@ void add_visibility(const Engine &E, Mesh &part, bool with_mask, PairBuffer &buf
   for (;;) {
     pairs = buf.get(std::max<int64_t>(cap, 1));
     const int rc = vacd_visibility(E.ctx, P.data(), n, use_mask ? mask.data() : nullptr, bvh,
-                                   nullptr, pairs, cap, &count);
+                                   cage, pairs, cap, &count);
     if (rc == VACD_ERR_CAPACITY) {
       cap = count;
       continue;
@@ -524,16 +580,17 @@ void log_exact_hotspot(const Engine &E, const std::vector<Mesh *> &parts,
 }
 
 void add_visibility_batch(Engine &E, const std::vector<Mesh *> &parts, bool with_mask,
-                          PairBuffer &buf) {
+                          PairBuffer &buf, const Mesh *cage_src = nullptr, double cage_level = 0.02) {
   PhaseTimer timer("visibility");
   const size_t np = parts.size();
-  std::vector<vacd_bvh *> bvhs(np, nullptr);
+  std::vector<vacd_bvh *> bvhs(np, nullptr), cages(np, nullptr);
   std::vector<std::vector<float>> P(np);
   std::vector<std::vector<uint8_t>> M(np);
   std::vector<vacd_part> vp;
   std::vector<size_t> idx;
   auto release = [&] {
     for (auto *b : bvhs) vacd_bvh_destroy(b);
+    for (auto *b : cages) vacd_bvh_destroy(b);
   };
   try {
     for (size_t k = 0; k < np; ++k) {
@@ -561,8 +618,10 @@ void add_visibility_batch(Engine &E, const std::vector<Mesh *> &parts, bool with
       std::vector<std::exception_ptr> errs(W);
       auto work = [&](size_t w) {
         try {
-          for (size_t j = next++; j < idx.size(); j = next++)
+          for (size_t j = next++; j < idx.size(); j = next++) {
             bvhs[idx[j]] = build_bvh_on(cs[w], *parts[idx[j]], true);
+            cages[idx[j]] = build_cage(cs[w], cage_src ? *cage_src : *parts[idx[j]], cage_level);
+          }
         } catch (...) {
           errs[w] = std::current_exception();
         }
@@ -576,7 +635,7 @@ void add_visibility_batch(Engine &E, const std::vector<Mesh *> &parts, bool with
     }
     for (size_t k : idx)
       vp.push_back(vacd_part{P[k].data(), (int64_t)parts[k]->points.size(),
-                             with_mask ? M[k].data() : nullptr, bvhs[k], nullptr});
+                             with_mask ? M[k].data() : nullptr, bvhs[k], cages[k]});
     if (!vp.empty()) {
       std::vector<int64_t> off(vp.size() + 1, 0);
       int64_t cap = std::max<int64_t>(buf.cap, int64_t(1) << 22), count = 0;
@@ -965,6 +1024,40 @@ ProcessResult finish(Driver &D, MeshList parts, const std::vector<double> &bbox)
   return r;
 }
 
+// Whether every undirected edge (by vertex index) has an even number of
+// triangles: a closed mesh, whose inside midpoint parity decides.
+bool closed_mesh(const Mesh &m) {
+  std::vector<std::pair<int, int>> e;
+  e.reserve(3 * m.triangles.size());
+  for (const auto &t : m.triangles)
+    for (int k = 0; k < 3; ++k) e.push_back({std::min(t[k], t[(k + 1) % 3]), std::max(t[k], t[(k + 1) % 3])});
+  std::sort(e.begin(), e.end());
+  for (size_t a = 0; a < e.size();) {
+    size_t b = a;
+    while (b < e.size() && e[b] == e[a]) ++b;
+    if ((b - a) & 1) return false;
+    a = b;
+  }
+  return true;
+}
+
+// The input remesh (lib/src/process.cpp:250-258): with engine.remesh
+// "always" as the reference does (scale 30, level 0.55 / 30; scale 20 when
+// that gives more than 15,000 vertices), with "auto" only for inputs that
+// are not closed, with "never" not at all.  The mesh is in the normalised
+// frame.
+void maybe_remesh(Driver &D, Mesh &mesh) {
+  const std::string &r = engine.remesh;
+  if (r != "always" && r != "auto" && r != "never")
+    throw std::invalid_argument("visacd: engine.remesh must be \"auto\", \"always\" or \"never\"");
+  if (r == "never" || (r == "auto" && closed_mesh(mesh))) return;
+  PhaseTimer t("remesh");
+  Mesh out = sdf_remesh_mesh(D.E.ctx, mesh, 30.0, 0.55 / 30.0);
+  if (out.vertices.size() > 15000) out = sdf_remesh_mesh(D.E.ctx, mesh, 20.0, 0.55 / 20.0);
+  log_line("Remeshed to " + std::to_string(out.vertices.size()) + " verts.");
+  mesh = std::move(out);
+}
+
 // The reference's loop (lib/src/process.cpp:238-347): one part per step,
 // its planes scored, clipped, and the new parts' visibility computed at once.
 ProcessResult process_plain(Driver &D, Mesh mesh, double concavity_threshold, int num_parts,
@@ -974,6 +1067,7 @@ ProcessResult process_plain(Driver &D, Mesh mesh, double concavity_threshold, in
   mesh.is_new.clear();
   mesh.intersecting_edges.clear();
   mesh.points.clear();
+  maybe_remesh(D, mesh);
   if (config.use_flat_surfaces) {
     D.flat = flat_surface_planes(mesh, config.flat_surface_min_area);
     log_line("Detected " + std::to_string(D.flat.size()) + " flat surface(s).");
@@ -981,11 +1075,20 @@ ProcessResult process_plain(Driver &D, Mesh mesh, double concavity_threshold, in
   MeshList parts = separate_disjoint(mesh);
   std::vector<uint64_t> seed;
   std::vector<int> attempt;
-  for (size_t i = 0; i < parts.size(); ++i) {
-    seed.push_back(mix64(root + i));
-    attempt.push_back(0);
-    choose_points(parts[i], false, point_seed(seed[i]));
-    add_visibility(D.E, parts[i], false, D.pair_buf);
+  {
+    vacd_bvh *root_cage = build_cage(D.E.ctx, mesh, 0.03);  // cage mode only
+    try {
+      for (size_t i = 0; i < parts.size(); ++i) {
+        seed.push_back(mix64(root + i));
+        attempt.push_back(0);
+        choose_points(parts[i], false, point_seed(seed[i]));
+        add_visibility(D.E, parts[i], false, D.pair_buf, root_cage);
+      }
+    } catch (...) {
+      vacd_bvh_destroy(root_cage);
+      throw;
+    }
+    vacd_bvh_destroy(root_cage);
   }
   std::ostringstream hdr;
   hdr << "Starting decomposition (max parts=" << num_parts << ", concavity threshold="
@@ -1043,7 +1146,14 @@ ProcessResult process_plain(Driver &D, Mesh mesh, double concavity_threshold, in
       continue;
     }
     for (size_t k = 0; k < kids.size(); ++k) {
-      add_visibility(D.E, kids[k], true, D.pair_buf);
+      vacd_bvh *cage = build_cage(D.E.ctx, kids[k], 0.02);  // cage mode only
+      try {
+        add_visibility(D.E, kids[k], true, D.pair_buf, cage);
+      } catch (...) {
+        vacd_bvh_destroy(cage);
+        throw;
+      }
+      vacd_bvh_destroy(cage);
       parts.push_back(std::move(kids[k]));
       seed.push_back(ks[k]), attempt.push_back(0), conc.push_back(nan);
     }
@@ -1077,7 +1187,8 @@ struct Node {
 // overlapped with the host half of the concavity of the nodes in `conc_of`;
 // then their distances on the GPU.
 void visibility_and_concavity(Driver &D, std::vector<Node> &nodes, const std::vector<int> &vis_of,
-                              bool with_mask, const std::vector<int> &conc_of) {
+                              bool with_mask, const std::vector<int> &conc_of,
+                              const Mesh *cage_src = nullptr, double cage_level = 0.02) {
   std::vector<Mesh *> vm;
   for (int x : vis_of) vm.push_back(&nodes[x].mesh);
   std::vector<const Mesh *> cm;
@@ -1087,7 +1198,7 @@ void visibility_and_concavity(Driver &D, std::vector<Node> &nodes, const std::ve
   if (!vm.empty())
     helper = std::thread([&] {
       try {
-        add_visibility_batch(D.E, vm, with_mask, D.pair_buf);
+        add_visibility_batch(D.E, vm, with_mask, D.pair_buf, cage_src, cage_level);
       } catch (...) {
         err = std::current_exception();
       }
@@ -1117,6 +1228,7 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
   mesh.is_new.clear();
   mesh.intersecting_edges.clear();
   mesh.points.clear();
+  maybe_remesh(D, mesh);
   if (config.use_flat_surfaces) D.flat = flat_surface_planes(mesh, config.flat_surface_min_area);
   log_line("  " + std::to_string(D.flat.size()) + " flat surface(s)");
   const bool by_edges = config.score_mode == "edge";
@@ -1135,7 +1247,7 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
     }
   }
   log_line("  " + std::to_string(L.size()) + " connected part(s)");
-  visibility_and_concavity(D, nodes, L, false, by_conc ? L : std::vector<int>{});
+  visibility_and_concavity(D, nodes, L, false, by_conc ? L : std::vector<int>{}, &mesh, 0.03);
   auto key_of = [&](int x) {
     if (by_edges) return edge_score(nodes[x].mesh);
     if (by_conc) return nodes[x].conc;

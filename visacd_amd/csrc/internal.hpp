@@ -336,6 +336,22 @@ void mlp_forward_host(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *fea
 float mlp_train_step(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
                      const float *planes, const float *target, float lr);
 
+// Exclusive prefix sum of m uint32 values in place (one block; lbvh.hip).
+void scan_u32(vacd_ctx *ctx, uint32_t *d_data, int64_t m);
+
+// Signed distances (negative inside) of a triangle mesh at the nodes lo +
+// (i, j, k) h of an n[0] x n[1] x n[2] grid (distance.hip), x fastest, into
+// host out; *sign_mode 0 by column parity (closed mesh), 1 by winding number.
+void sdf_grid(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+              const double lo[3], double h, const int n[3], float *out, int *sign_mode);
+
+// Signed-distance remesh of a triangle mesh (distance.hip): the iso-surface
+// at distance level_set outside the mesh, on a grid of spacing 1 / scale,
+// extracted as quads split into triangles.  Host arrays in and out.
+void sdf_remesh(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+                double scale, double level_set, std::vector<float> &Vout,
+                std::vector<uint32_t> &Tout, int *sign_mode);
+
 // Fills n 64-bit words with v on the context stream (a kernel, so that it
 // may be captured into a graph).
 void fill_u64(vacd_ctx *ctx, void *p, int64_t n, unsigned long long v);

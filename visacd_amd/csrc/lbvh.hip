@@ -952,6 +952,12 @@ void build_inside_grid(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, int
 
 }  // namespace
 
+void scan_u32(vacd_ctx *ctx, uint32_t *d_data, int64_t m) {
+  if (m <= 0) return;
+  k_scan_u32<<<1, 1024, 0, ctx->stream>>>(d_data, m);
+  VACD_HIP(hipGetLastError());
+}
+
 void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T, int64_t nT,
                 vacd_bvh *out, const uint8_t *d_grp) {
   (void)nV;

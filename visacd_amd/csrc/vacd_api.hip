@@ -818,6 +818,45 @@ int vacd_plane_extents_batch(vacd_ctx *ctx, const vacd_extent_query *q, int32_t 
   });
 }
 
+int vacd_sdf_remesh(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+                    double scale, double level_set, float *V_out, int64_t v_cap, uint32_t *T_out,
+                    int64_t t_cap, int64_t *nV_out, int64_t *nT_out) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  int64_t nv = 0, nt = 0;
+  int rc = guarded(ctx, [&] {
+    VACD_REQUIRE(V && T && nV_out && nT_out, VACD_ERR_INVALID, "NULL argument");
+    VACD_REQUIRE(v_cap >= 0 && t_cap >= 0 && (v_cap == 0 || V_out) && (t_cap == 0 || T_out),
+                 VACD_ERR_INVALID, "bad output buffers");
+    set_device(ctx);
+    std::vector<float> vo;
+    std::vector<uint32_t> to;
+    vacd::sdf_remesh(ctx, V, nV, T, nT, scale, level_set, vo, to, nullptr);
+    nv = (int64_t)vo.size() / 3;
+    nt = (int64_t)to.size() / 3;
+    if (nv <= v_cap && nv) std::memcpy(V_out, vo.data(), 4 * vo.size());
+    if (nt <= t_cap && nt) std::memcpy(T_out, to.data(), 4 * to.size());
+  });
+  if (rc != VACD_OK) return rc;
+  *nV_out = nv;
+  *nT_out = nt;
+  if (nv > v_cap || nt > t_cap)
+    return fail(ctx, Error{VACD_ERR_CAPACITY, "remesh: more vertices or triangles than the buffers hold"});
+  return VACD_OK;
+}
+
+int vacd_sdf_grid(vacd_ctx *ctx, const float *V, int64_t nV, const uint32_t *T, int64_t nT,
+                  const double *lo, double h, const int32_t *n, float *out, int32_t *sign_mode) {
+  if (!ctx) return fail(ctx, Error{VACD_ERR_INVALID, "NULL context"});
+  return guarded(ctx, [&] {
+    VACD_REQUIRE(V && T && lo && n && out, VACD_ERR_INVALID, "NULL argument");
+    set_device(ctx);
+    const int nn[3] = {n[0], n[1], n[2]};
+    int mode = 0;
+    vacd::sdf_grid(ctx, V, nV, T, nT, lo, h, nn, out, &mode);
+    if (sign_mode) *sign_mode = mode;
+  });
+}
+
 void classify_and_rate_planes(const float *planes, const float *points, const unsigned int *edges,
                               float *scores, int numPlanes, int numPoints, int numEdges) {
   thread_local vacd_ctx *tctx = nullptr;

@@ -34,6 +34,7 @@ EXPORTED = [
     "vacd_visibility_batch_d", "vacd_visibility_batch", "vacd_score_planes",
     "vacd_score_planes_d", "vacd_score_planes_noalloc_d", "vacd_score_planes_batch", "vacd_score_planes_batch_d",
     "vacd_max_distance", "vacd_max_distance_d", "vacd_max_distance_batch", "vacd_plane_extents_batch",
+    "vacd_sdf_remesh", "vacd_sdf_grid",
     "vacd_gemm_bf16_d", "vacd_mlp_param_count",
     "vacd_mlp_create", "vacd_mlp_destroy", "vacd_mlp_set_params", "vacd_mlp_get_params", "vacd_mlp_get_grads",
     "vacd_mlp_forward_d", "vacd_mlp_forward", "vacd_mlp_train_step_d", "classify_and_rate_planes",
@@ -144,6 +145,9 @@ def lib():
         L.vacd_max_distance_batch.argtypes = [vp, ctypes.POINTER(DistanceQuery), ctypes.c_int32,
                                               vp]
         L.vacd_plane_extents_batch.argtypes = [vp, ctypes.POINTER(ExtentQuery), ctypes.c_int32, vp]
+        L.vacd_sdf_remesh.argtypes = [vp, vp, i64, vp, i64, ctypes.c_double, ctypes.c_double, vp,
+                                      i64, vp, i64, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.vacd_sdf_grid.argtypes = [vp, vp, i64, vp, i64, vp, ctypes.c_double, vp, vp, vp]
         ci = ctypes.c_int
         L.vacd_gemm_bf16_d.argtypes = [vp, ci, ci, ci, ci, ci, vp, ci, vp, ci, vp, ci, ci, vp, ci]
         L.vacd_mlp_param_count.restype = i64
@@ -318,6 +322,38 @@ class Context:
         out = np.zeros((max(1, sum(sizes)), 2), dtype=np.float64)
         self._check(lib().vacd_plane_extents_batch(self._h, arr, len(queries), _p(out)))
         return np.split(out[: sum(sizes)], np.cumsum(sizes)[:-1])
+
+    def sdf_remesh(self, vertices, tris, scale: float, level_set: float):
+        """The surface at distance level_set outside the mesh on a 1 / scale
+        grid: (vertices float32 (n, 3), triangles uint32 (m, 3))."""
+        V = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        T = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        vc, tc = 4 * V.shape[0] + 1024, 4 * T.shape[0] + 2048
+        while True:
+            vo = np.empty((vc, 3), np.float32)
+            to = np.empty((tc, 3), np.uint32)
+            nv, nt = ctypes.c_int64(), ctypes.c_int64()
+            rc = lib().vacd_sdf_remesh(self._h, _p(V), V.shape[0], _p(T), T.shape[0], scale,
+                                       level_set, _p(vo), vc, _p(to), tc, ctypes.byref(nv),
+                                       ctypes.byref(nt))
+            if rc == VACD_ERR_CAPACITY:
+                vc, tc = max(vc, nv.value), max(tc, nt.value)
+                continue
+            self._check(rc)
+            return vo[:nv.value].copy(), to[:nt.value].copy()
+
+    def sdf_grid(self, vertices, tris, lo, h: float, n):
+        """Signed distances (negative inside) at lo + (i, j, k) h, as an array
+        of shape (n[2], n[1], n[0]), and the sign method (0 parity, 1 winding)."""
+        V = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        T = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        nn = np.ascontiguousarray(n, dtype=np.int32)
+        out = np.empty(int(np.prod(nn)), np.float32)
+        mode = ctypes.c_int32()
+        self._check(lib().vacd_sdf_grid(self._h, _p(V), V.shape[0], _p(T), T.shape[0], _p(lo), h,
+                                        _p(nn), _p(out), ctypes.byref(mode)))
+        return out.reshape(nn[2], nn[1], nn[0]), int(mode.value)
 
     COUNTER_NAMES = ("masked", "blocked", "unused2", "edges", "malformed", "node_visits",
                      "tri_tests", "walked", "exact_calls", "exact_tri_tests", "undecided_no_fan_p",
