@@ -848,9 +848,10 @@ __device__ __forceinline__ bool mask_skips(uint8_t a, uint8_t b) { return !((a |
 // of p's fan that belongs to one of the planar groups in `groups` (bit g:
 // group g), in the group plane's projection (exact orientations).  Both
 // points lie on that plane, so the segment then runs into the group's
-// triangles (the cap a cut left) and touches the surface.
-__device__ __forceinline__ bool enters_cap(const Part &P, int64_t i, const float p[3],
-                                           const float q[3], unsigned groups) {
+// triangles (the cap a cut left) and touches the surface.  Only the exact
+// redo calls it (out of line: it would cost the fan pass registers).
+__device__ __noinline__ bool enters_cap(const Part &P, int64_t i, const float p[3],
+                                        const float q[3], unsigned groups) {
   const int fn = P.fan_n[i];
   const FanRec *fr = P.fan_rec + (size_t)i * kFanMax;
   for (int f = 0; f < fn; ++f) {
@@ -868,6 +869,41 @@ __device__ __forceinline__ bool enters_cap(const Part &P, int64_t i, const float
   return false;
 }
 
+// The same test with float64 filters only, for the fan pass: 1 enters a
+// cap triangle (decided), 0 enters none (every sign decided), -1 a sign the
+// filter leaves open and no triangle decided "enters" (the exact redo then
+// decides the pair with enters_cap).  A zero error bound means both
+// products are exactly zero, so the sign (zero) is exact.
+__device__ __forceinline__ int enters_cap_filtered(const Part &P, int64_t i, const float p[3],
+                                                   const float q[3], unsigned groups) {
+  const int fn = P.fan_n[i];
+  const FanRec *fr = P.fan_rec + (size_t)i * kFanMax;
+  int r = 0;
+  for (int f = 0; f < fn; ++f) {
+    const int code = fr[f].grp;
+    if (!code || !((groups >> (code - 1)) & 1u)) continue;
+    const int k = (int)((P.group_axis >> (2 * (code - 1))) & 3u);
+    if (k == 3) continue;
+    const float v1[3] = {fr[f].v1[0], fr[f].v1[1], fr[f].v1[2]};
+    const float v2[3] = {fr[f].v2[0], fr[f].v2[1], fr[f].v2[2]};
+    double b0, b1, b2;
+    const double d0 = o2_fast(p, v1, v2, k, b0);
+    const double d1 = o2_fast(p, v1, q, k, b1);
+    const double d2 = o2_fast(p, q, v2, k, b2);
+    const bool sure = (fabs(d0) > b0 || b0 == 0.0) && (fabs(d1) > b1 || b1 == 0.0) &&
+                      (fabs(d2) > b2 || b2 == 0.0);
+    if (!sure) {
+      r = -1;
+      continue;
+    }
+    const int o = d0 > 0.0 ? 1 : (d0 < 0.0 ? -1 : 0);
+    if (o == 0) continue;
+    const int o1 = d1 > 0.0 ? 1 : (d1 < 0.0 ? -1 : 0), o2 = d2 > 0.0 ? 1 : (d2 < 0.0 ? -1 : 0);
+    if ((o1 != 0 || o2 != 0) && o * o1 >= 0 && o * o2 >= 0) return 1;
+  }
+  return r;
+}
+
 // Appends `value` of every lane with `mine` set to a list, in lane order,
 // with one atomic per wave.
 __device__ __forceinline__ void append_lanes(bool mine, int64_t value, int64_t *list,
@@ -881,7 +917,9 @@ __device__ __forceinline__ void append_lanes(bool mine, int64_t value, int64_t *
   if (mine) list[base + __popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull))] = value;
 }
 
-template <int STACK, bool CAGE, bool COUNT, bool WIDE, bool BATCH, bool QUEUE>
+// CAP: the mask may carry cap-rule bits (its filtered 2-D tests are
+// compiled only where needed: they cost the fan pass registers).
+template <int STACK, bool CAGE, bool COUNT, bool WIDE, bool BATCH, bool QUEUE, bool CAP = false>
 __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   __shared__ int stk_all[kWaves][STACK][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -907,13 +945,23 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
         for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
         bool decided = same3(s.p, s.q);  // zero length: not an edge
         int pstate = 2, qstate = 2;  // work counters: 1 outwards, 2 no fan, 3 undecided
-        if (!decided && P.fan_n && !(CAGE && P.ctris) && P.mask) {
+        if (CAP && !decided && P.fan_n && !(CAGE && P.ctris) && P.mask) {
           // both points on one planar group's plane (a cut's cap): a pair
           // that runs into the cap at either end touches it
           const unsigned shared = (unsigned)(P.mask[i] & P.mask[j]) >> 1;
-          if (shared && (enters_cap(P, i, s.p, s.q, shared) || enters_cap(P, j, s.q, s.p, shared))) {
-            ++n_blocked;
-            decided = true;
+          if (shared) {
+            int c = enters_cap_filtered(P, i, s.p, s.q, shared);
+            if (c != 1) {
+              const int cq = enters_cap_filtered(P, j, s.q, s.p, shared);
+              c = cq == 1 ? 1 : ((c | cq) < 0 ? -1 : 0);
+            }
+            if (c == 1) {
+              ++n_blocked;
+              decided = true;
+            } else if (c < 0) {  // an open sign: the exact redo decides the pair
+              defer = true;
+              decided = true;
+            }
           }
         }
         if (!decided && P.fan_n && !(CAGE && P.ctris)) {
@@ -1279,6 +1327,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
   for (int64_t e = (int64_t)blockIdx.x * kGroups + slot; e < len;
        e += (int64_t)gridDim.x * kGroups) {
     const int64_t kg = list[e];
+    if (EXACT && (kg & kSettled)) continue;  // settled by the exact cap rule
     const int64_t w = kg >> 6;
     const int b = (int)(kg & 63);
     const Part P = part_of_word<BATCH>(a, w);
@@ -1357,6 +1406,38 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
 }
 
 // ------------------------------------------------------------ exact redo
+// The cap rule decided exactly for the deferred pairs whose points share a
+// planar group (the fan pass defers those whose filtered 2-D signs stay
+// open), a lane per entry, before the exact walks: a pair that enters a
+// cap at either end is blocked and gets the kSettled bit (the exact walks
+// skip it); for every other pair the rule cannot hold (the filter decided
+// it or the exact test says no) and the walks decide it.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_cap_redo(VisArgs a) {
+  const int64_t len = (int64_t)*a.dlist_len;
+  unsigned n_blocked = 0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < len;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t kg = a.dlist[e];
+    const int64_t w = kg >> 6;
+    const Part P = part_of_word<BATCH>(a, w);
+    if (!P.mask || !P.fan_n || P.ctris) continue;
+    int64_t i, j;
+    pair_of((w - P.word_begin) * 64 + (kg & 63), P.n, i, j);
+    const unsigned shared = (unsigned)(P.mask[i] & P.mask[j]) >> 1;
+    if (!shared) continue;
+    float p[3], q[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p[c] = P.pts[3 * i + c], q[c] = P.pts[3 * j + c];
+    if (enters_cap(P, i, p, q, shared) || enters_cap(P, j, q, p, shared)) {
+      a.dlist[e] = kg | kSettled;
+      ++n_blocked;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) n_blocked += __shfl_down(n_blocked, off);
+  if ((threadIdx.x & 63) == 0 && n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
+}
+
 // ------------------------------------------------------------ far parity
 // Crossings (mod 2) of the +x ray from the segment's midpoint with the
 // triangles of this lane's share.  Midpoint mode (skip NULL): the ray is
@@ -2034,6 +2115,7 @@ struct Launch {
   int64_t nscan;              // its blocks
   int lv;     // log2 of the lanes per queued pair
   int front;  // lanes per pair of the breadth-first walks of short queues (0: off)
+  bool cap;   // a mask of some part may carry cap-rule bits
   hipStream_t st;
 };
 
@@ -2066,15 +2148,18 @@ void launch_split(const Launch &L, const VisArgs &a) {
 template <int S, bool C, bool N, bool W, bool B>
 void launch_main(bool q, const Launch &L, const VisArgs &a) {
   if (q && !C) {
-    k_visibility<1, C, N, W, B, true><<<L.grid, kBlock, 0, L.st>>>(a);  // no walk, no stack
+    if (L.cap)  // no walk, no stack
+      k_visibility<1, C, N, W, B, true, true><<<L.grid, kBlock, 0, L.st>>>(a);
+    else
+      k_visibility<1, C, N, W, B, true><<<L.grid, kBlock, 0, L.st>>>(a);
     // compact the per-word bits into the queue
     k_word_scan<<<(unsigned)L.nscan, 256, 0, L.st>>>(a.qbits, a.nwords, L.qscan);
     k_scan_blocks<<<1, 1024, 0, L.st>>>(L.qscan, L.nscan, nullptr, a.queue_len);
     k_queue_emit<<<(unsigned)L.nscan, 256, 0, L.st>>>(a, L.qscan);
     launch_front<N, B>(L, a);  // a short queue: breadth-first walks
     launch_split<S, N, B>(L, a);  // a long one: depth-first walks
-  } else {
-    k_visibility<S, C, N, W, B, false><<<L.grid, kBlock, 0, L.st>>>(a);
+  } else {  // the rare path (open parts, cages): one instance, cap rule compiled in
+    k_visibility<S, C, N, W, B, false, true><<<L.grid, kBlock, 0, L.st>>>(a);
   }
 }
 template <int S, bool C, bool N, bool W>
@@ -2152,8 +2237,8 @@ int binary_stack(int depth, int64_t leaves, bool small_first) {
 
 // Enqueues far parity, the main kernel, the exact redo and the compaction for
 // a prepared VisArgs (a.nwords > 0).
-void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int64_t max_tris,
-         int depth, int stack2,
+void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, bool cap_bits,
+         int64_t max_tris, int depth, int stack2,
          uint32_t *d_pairs, int64_t cap, int64_t *d_count, int64_t *d_part_off) {
   hipStream_t st = ctx->stream;
   const int64_t nwords = a.nwords;
@@ -2235,6 +2320,7 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   L.grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
   L.lv = ctx->lanes_per_pair_log2;
   L.front = ctx->front_lanes;
+  L.cap = cap_bits;
   a.front_max = L.front ? ctx->front_max : -1;
   // the queue length is known on the device only: enough groups to cover
   // every pair once when all are queued, at most 16 blocks per CU
@@ -2267,6 +2353,10 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, int
   a.exact_front = L.front != 0;
   const unsigned wgrid = (unsigned)(kExactSplit / kWaves);
   const unsigned xgrid = (unsigned)std::min<int64_t>((nwords * 64 + kBlock - 1) / kBlock, 1024);
+  if (cap_bits) {  // the exact cap rule before the exact walks
+    if (batch) k_cap_redo<true><<<xgrid, 256, 0, st>>>(a);
+    else k_cap_redo<false><<<xgrid, 256, 0, st>>>(a);
+  }
   if (a.exact_front) {  // breadth first (a wave per pair) before the depth-first kernels
     if (batch) k_walk_front<false, true, 64, true><<<wgrid, kBlock, 0, st>>>(a);
     else k_walk_front<false, false, 64, true><<<wgrid, kBlock, 0, st>>>(a);
@@ -2352,7 +2442,9 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
   const int stack2 = std::max({binary_stack(self->depth, self->n_tris, self->small_first),
                                binary_stack(self->depth_refs, self->n_refs, self->small_first_refs),
                                cage ? binary_stack(cage->depth, cage->n_tris, cage->small_first) : 0});
-  run(ctx, a, false, cage != nullptr, any_far, self->n_tris, depth, stack2, d_pairs, cap, d_count,
+  // the cap rule needs a mask and planar groups in the self BVH
+  const bool cap_bits = d_mask != nullptr && self->group_axis != 0xffffffffu;
+  run(ctx, a, false, cage != nullptr, any_far, cap_bits, self->n_tris, depth, stack2, d_pairs, cap, d_count,
       nullptr);
 }
 
@@ -2436,7 +2528,11 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
     fill_u64(ctx, d_part_off, nparts + 1, 0ull);
     return;
   }
-  run(ctx, a, true, any_cage, any_far, max_tris, depth, stack2, d_pairs, cap, d_count, d_part_off);
+  bool cap_bits = false;
+  for (int p = 0; p < nparts; ++p)
+    cap_bits |= parts[p].new_mask != nullptr && parts[p].self->group_axis != 0xffffffffu;
+  run(ctx, a, true, any_cage, any_far, cap_bits, max_tris, depth, stack2, d_pairs, cap, d_count,
+      d_part_off);
 }
 
 }  // namespace vacd
