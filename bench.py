@@ -75,6 +75,26 @@ def cpu_baseline(W, seconds):
                       f"(exact predicates, CPU BVH), {dt:.1f} s"}
 
 
+def window_rule(W, m=200000):
+    """The reference's ray window (float32 ray from p over t in [0.001, len],
+    lib/ptx/ray_segments.cu:77-111) beside exact contact with the self mesh
+    on a sample of the workload's pairs: counts of agreement and of each
+    cause of disagreement (the oracle's restatement, a checker)."""
+    import numpy as np
+
+    import oracle
+
+    rng = np.random.default_rng(7)
+    ks = np.unique(rng.integers(0, W.n_segments, m))
+    tau = 1e-6 * float((W.vertices.max(0) - W.vertices.min(0)).max())
+    t0 = time.perf_counter()
+    bits = oracle.window_hits(W.points, W.vertices, W.triangles, tau, ks=ks)
+    r = oracle.window_classes(bits)
+    r["tau"] = tau
+    r["sample"] = f"{ks.size} uniformly drawn pairs, {time.perf_counter() - t0:.1f} s"
+    return r
+
+
 def acd_wall(W, mode, proposal="exact"):
     """Full decomposition of the workload's mesh through visacd.process:
     mode "plain" (the reference's loop) or "batched" (frontier rounds);
@@ -710,6 +730,7 @@ def main():
         cb = cpu_baseline(W, args.cpu_seconds)
         cb["scoring"] = cpu_scoring_baseline(W, edges_host, args.cpu_seconds / 2)
         cb["c1_decomposition"] = c1_decomposition()
+        cb["window_rule"] = window_rule(W)
         line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 

@@ -42,8 +42,7 @@ typedef struct vacd_vis_stats {
   int64_t node_visits;/* BVH nodes loaded (VACD_OPT_COUNT_WORK only)     */
   int64_t tri_tests;  /* triangle tests (VACD_OPT_COUNT_WORK only)       */
   int64_t walked;     /* pairs that needed a BVH walk (the rest were      */
-                      /* masked, zero-length, settled by a fan or by the  */
-                      /* inside grid)                                     */
+                      /* masked, zero-length or settled by a fan)         */
 } vacd_vis_stats;
 
 /* Work counters of the last visibility call, for diagnosis: out[k] for
@@ -52,14 +51,13 @@ typedef struct vacd_vis_stats {
  * triangle tests; with VACD_OPT_COUNT_WORK the fan pass also counts the
  * pairs it leaves to the walks with 10 no usable fan at p, 11 p's fan test
  * undecided, 12 no usable fan at q, 13 q's undecided, 14 both ends leaving
- * outwards, and 15 the pairs the inside grid settled.  Waits for the
+ * outwards.  Waits for the
  * stream. */
 int vacd_visibility_counters(vacd_ctx *ctx, int64_t *out, int32_t n);
 
 /* Context options (vacd_ctx_set_option). */
 #define VACD_OPT_TIMING 1     /* record HIP events around the main kernels  */
 #define VACD_OPT_COUNT_WORK 2 /* visibility counts node visits / tri tests  */
-#define VACD_OPT_WIDE_BVH 3   /* 4-wide node walk vs binary (default 0)    */
 #define VACD_OPT_FAN_PAIRS 4  /* parity mode: settle pairs that enter the  */
                               /* solid at p or at q from that end's fan (1) */
 #define VACD_OPT_SCORE_CHUNK 5 /* edges per scoring block, 0 = automatic    */
@@ -69,9 +67,6 @@ int vacd_visibility_counters(vacd_ctx *ctx, int64_t *out, int32_t n);
                                /* of short walk queues: 0, 4, 8, 16, 32, 64 */
                                /* (8; 0 = depth-first walks only)           */
 #define VACD_OPT_FRONT_MAX 9   /* longest queue walked breadth-first (131072)*/
-#define VACD_OPT_INSIDE_GRID 7 /* parity mode: settle pairs that cross a    */
-                               /* cell wholly inside a closed solid; the    */
-                               /* grid is built with BVHs built while on (0)*/
 
 /* Accumulated device time of the main kernels since the last reset
  * (VACD_OPT_TIMING): the visibility traversal kernel, the plane-scoring
@@ -144,14 +139,16 @@ int vacd_bvh_export(const vacd_bvh *bvh, void *nodes, void *tris);
  * (lib/src/intersections.cpp:75-325, 327-385; device programs
  * lib/ptx/ray_segments.cu:49-125).  Tests every pair (i<j) of the n points
  * pts[n*3].  With new_mask non-NULL, new_mask[i] holds bit 0 = new point
- * (the reference's mask), bit 1 = new point on the last cut's cap, bits 2-7
- * = point on one of six older cut planes; a pair is skipped when neither
- * point is new (bits 0-1 clear in both: the reference's rule) or when both
- * points share one of bits 1-7 (their segment lies in one cut plane, where
- * only rounding decides contact; the reference has no such bits).  So 0/1
- * keep the reference's meaning and 2 marks new points of one cap whose
- * pairs among themselves are not wanted.  A pair is a visibility edge iff its segment does not
- * touch the self mesh and (cage != NULL) touches the cage mesh, or
+ * (the reference's mask: a pair is skipped when neither point has it) and
+ * bit 1 + g = the point lies on the plane of planar group g of the self BVH
+ * (vacd_bvh_build_planar; the reference has no such bits).  A pair whose
+ * points share such a bit is not an edge when, at either end, its
+ * direction enters a group-g triangle of that end's fan (exact 2-D
+ * orientations in the group plane's projection; closed self mesh, parity
+ * mode; ends with more than 16 incident triangles are not tested there):
+ * the cap rule.  Otherwise, and with bits 1-7 clear, a pair is a visibility
+ * edge iff its segment does not touch the self mesh and (cage != NULL)
+ * touches the cage mesh, or
  * (cage == NULL) its midpoint lies outside the self mesh.  Edges are written
  * to pairs[2*e], pairs[2*e+1] = (i, j) in increasing k = i(2n-i-1)/2 + j-i-1
  * order, at most cap pairs; *count receives the total number of edges.

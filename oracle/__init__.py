@@ -60,6 +60,7 @@ def lib():
         L.orc_bisector_planes.restype = i64
         L.orc_mt19937.argtypes = [ctypes.c_uint32, i64, vp]
         L.orc_predicates_batch.argtypes = [ctypes.c_int, i64, vp, ctypes.c_int, vp]
+        L.orc_window_hits.argtypes = [vp, i64, vp, vp, i64, vp, i64, ctypes.c_double, vp]
         L.orc_uniform_int.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, i64, vp]
         del fp
         _lib = L
@@ -172,6 +173,50 @@ def visibility_sample(points, V, T, ks, mask=None, cage=None, brute=False, group
     lib().orc_visibility_sample(_ptr(pts), n, _ptr(m), sv, snv, st, snt, cv, cnv, ct, cnt,
                                 1 if brute else 0, _ptr(ks), ks.shape[0], _ptr(out), gp, pp, ng)
     return out
+
+
+WINDOW_EXACT, WINDOW_HIT, WINDOW_NO_TMIN, WINDOW_SHORT, WINDOW_GRAZING = 1, 2, 4, 8, 16
+
+
+def window_hits(points, V, T, tau, ks=None):
+    """The reference's ray-window rule (lib/ptx/ray_segments.cu:77-111,
+    float32 watertight test, t in [0.001, len]) beside the exact contact,
+    per pair k (all pairs in k order when ks is None): bit WINDOW_EXACT
+    exact contact, WINDOW_HIT window hit, WINDOW_NO_TMIN a hit with t in
+    [0, len], WINDOW_SHORT a hit with t in [0.001, len - tau],
+    WINDOW_GRAZING every triangle on which the two disagree lies within tau
+    of grazing."""
+    pts = _f32(points)
+    n = pts.shape[0]
+    sv, _snv, st, snt, _k1, _k2 = _mesh_args(V, T)
+    if ks is not None:
+        ks = np.ascontiguousarray(ks, dtype=np.int64)
+        m = ks.shape[0]
+    else:
+        m = n * (n - 1) // 2
+    out = np.zeros(m, dtype=np.uint8)
+    lib().orc_window_hits(_ptr(pts), n, sv, st, snt, _ptr(ks), m, float(tau), _ptr(out))
+    return out
+
+
+def window_classes(bits):
+    """Counts of the pairs on which the window rule and the exact contact
+    agree or disagree, by cause: 'tmin' (only contacts within 0.001 of p,
+    which the window skips), 'q_end' (the window's only hits lie within tau
+    of q, where rounding decides t <= len), 'grazing' (every disagreeing
+    triangle within tau of grazing) and 'other'."""
+    b = np.asarray(bits)
+    ex = (b & WINDOW_EXACT) != 0
+    win = (b & WINDOW_HIT) != 0
+    miss = ex & ~win  # contact the window rule misses
+    extra = ~ex & win  # a window hit without contact
+    tmin = miss & ((b & WINDOW_NO_TMIN) != 0)
+    q_end = extra & ((b & WINDOW_SHORT) == 0)
+    graze = ((miss & ~tmin) | (extra & ~q_end)) & ((b & WINDOW_GRAZING) != 0)
+    other = ((miss & ~tmin) | (extra & ~q_end)) & ~graze
+    return {"pairs": int(b.size), "agree": int((ex == win).sum()), "missed": int(miss.sum()),
+            "extra": int(extra.sum()), "tmin": int(tmin.sum()), "q_end": int(q_end.sum()),
+            "grazing": int(graze.sum()), "other": int(other.sum())}
 
 
 def score_fp64(planes, points, edges):

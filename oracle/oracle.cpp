@@ -635,6 +635,175 @@ void setup(Ctx& C, const float* pts, int64_t n, const uint8_t* mask, const float
   }
 }
 
+
+// ------------------------------------------- the reference's ray window
+// lib/ptx/ray_segments.cu:77-111 restated in float32 for the mismatch
+// count between that rule and the exact contact definition above: the
+// ray starts at p with direction (q - p) * (1 / len), len = sqrtf of the
+// squared length (contracted into fmaf as nvcc does by default), and a
+// triangle is hit when the ray meets it at t in the closed window
+// [0.001, len].  OptiX's own triangle test and rounding are closed; the
+// test here is the published watertight one (S. Woop, C. Benthin, I. Wald,
+// "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013), float32, no
+// culling, its double-precision fallback for zero edge functions.
+struct RayF {
+  float o[3], d[3];
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+RayF make_ray(const float* p, const float* dir) {
+  RayF r;
+  for (int k = 0; k < 3; ++k) r.o[k] = p[k], r.d[k] = dir[k];
+  int kz = 0;
+  for (int k = 1; k < 3; ++k)
+    if (std::fabs(dir[k]) > std::fabs(dir[kz])) kz = k;
+  int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+  if (dir[kz] < 0.0f) std::swap(kx, ky);
+  r.kx = kx, r.ky = ky, r.kz = kz;
+  r.Sx = dir[kx] / dir[kz];
+  r.Sy = dir[ky] / dir[kz];
+  r.Sz = 1.0f / dir[kz];
+  return r;
+}
+
+// true (and t) when the ray meets triangle abc.
+bool watertight_hit(const RayF& r, const float* a, const float* b, const float* c, float& t) {
+  const float A[3] = {a[0] - r.o[0], a[1] - r.o[1], a[2] - r.o[2]};
+  const float B[3] = {b[0] - r.o[0], b[1] - r.o[1], b[2] - r.o[2]};
+  const float C[3] = {c[0] - r.o[0], c[1] - r.o[1], c[2] - r.o[2]};
+  const float Ax = A[r.kx] - r.Sx * A[r.kz], Ay = A[r.ky] - r.Sy * A[r.kz];
+  const float Bx = B[r.kx] - r.Sx * B[r.kz], By = B[r.ky] - r.Sy * B[r.kz];
+  const float Cx = C[r.kx] - r.Sx * C[r.kz], Cy = C[r.ky] - r.Sy * C[r.kz];
+  float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float Az = r.Sz * A[r.kz], Bz = r.Sz * B[r.kz], Cz = r.Sz * C[r.kz];
+  const float T = U * Az + V * Bz + W * Cz;
+  t = T * (1.0f / det);
+  return true;
+}
+
+// float64 distances for the grazing class
+double dot3(const P3& a, const P3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+P3 sub3(const P3& a, const P3& b) { return P3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+P3 axpy3(double s, const P3& d, const P3& o) { return P3{o.x + s * d.x, o.y + s * d.y, o.z + s * d.z}; }
+double dist3(const P3& a, const P3& b) { return std::sqrt(dot3(sub3(a, b), sub3(a, b))); }
+
+// Distance between segments p1q1 and p2q2 (closest points by clamped
+// parameters, C. Ericson, Real-Time Collision Detection, 5.1.9).
+double seg_seg_dist(const P3& p1, const P3& q1, const P3& p2, const P3& q2) {
+  const P3 d1 = sub3(q1, p1), d2 = sub3(q2, p2), r = sub3(p1, p2);
+  const double a = dot3(d1, d1), e = dot3(d2, d2), f = dot3(d2, r);
+  double s = 0.0, t = 0.0;
+  if (a <= 1e-300 && e <= 1e-300) return dist3(p1, p2);
+  if (a <= 1e-300) {
+    t = std::clamp(f / e, 0.0, 1.0);
+  } else {
+    const double c = dot3(d1, r);
+    if (e <= 1e-300) {
+      s = std::clamp(-c / a, 0.0, 1.0);
+    } else {
+      const double b = dot3(d1, d2), den = a * e - b * b;
+      s = den > 0.0 ? std::clamp((b * f - c * e) / den, 0.0, 1.0) : 0.0;
+      t = (b * s + f) / e;
+      if (t < 0.0) t = 0.0, s = std::clamp(-c / a, 0.0, 1.0);
+      else if (t > 1.0) t = 1.0, s = std::clamp((b - c) / a, 0.0, 1.0);
+    }
+  }
+  return dist3(axpy3(s, d1, p1), axpy3(t, d2, p2));
+}
+
+// Distance from x to the closed triangle abc (Ericson 5.1.5).
+double point_tri_dist(const P3& x, const P3& a, const P3& b, const P3& c) {
+  const P3 ab = sub3(b, a), ac = sub3(c, a), ap = sub3(x, a);
+  const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0.0 && d2 <= 0.0) return dist3(x, a);
+  const P3 bp = sub3(x, b);
+  const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0.0 && d4 <= d3) return dist3(x, b);
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) return dist3(x, axpy3(d1 / (d1 - d3), ab, a));
+  const P3 cp = sub3(x, c);
+  const double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0.0 && d5 <= d6) return dist3(x, c);
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return dist3(x, axpy3(d2 / (d2 - d6), ac, a));
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0)
+    return dist3(x, axpy3((d4 - d3) / ((d4 - d3) + (d5 - d6)), sub3(c, b), b));
+  const double den = 1.0 / (va + vb + vc);
+  const double v = vb * den, w = vc * den;
+  return dist3(x, P3{a.x + ab.x * v + ac.x * w, a.y + ab.y * v + ac.y * w,
+                     a.z + ab.z * v + ac.z * w});
+}
+
+// Within tau of grazing: the segment passes within tau of the triangle's
+// boundary, or an endpoint lies within tau of the triangle.
+bool grazing(const P3& p, const P3& q, const P3& a, const P3& b, const P3& c, double tau) {
+  return seg_seg_dist(p, q, a, b) <= tau || seg_seg_dist(p, q, b, c) <= tau ||
+         seg_seg_dist(p, q, c, a) <= tau || point_tri_dist(p, a, b, c) <= tau ||
+         point_tri_dist(q, a, b, c) <= tau;
+}
+
+// Per pair (i, j), i < j, of points on or off the mesh: bit 0 exact
+// contact with some triangle; bit 1 a window hit, t in [0.001, len]; bit 2
+// a hit with t in [0, len]; bit 3 a hit with t in [0.001, len - tau];
+// bit 4 every triangle on which contact and the window rule disagree is
+// within tau of grazing.
+uint8_t window_bits(const Bvh& B, const float* V, const uint32_t* T, const float* pf,
+                    const float* qf, double tau, std::vector<int>& cand) {
+  const P3 p = ld(pf), q = ld(qf);
+  const float dx = qf[0] - pf[0], dy = qf[1] - pf[1], dz = qf[2] - pf[2];
+  const float len = std::sqrt(std::fma(dz, dz, std::fma(dy, dy, dx * dx)));
+  uint8_t bits = 0;
+  if (len == 0.0f) return 0;
+  const float inv = 1.0f / len;
+  const float dir[3] = {dx * inv, dy * inv, dz * inv};
+  const RayF r = make_ray(pf, dir);
+  // candidates: triangles whose box, widened by tau, meets the segment
+  cand.clear();
+  if (!B.nodes.empty()) {
+    int st[256];
+    int sp = 0;
+    st[sp++] = 0;
+    while (sp) {
+      Bvh::Node n = B.nodes[st[--sp]];
+      for (int k = 0; k < 3; ++k) n.lo[k] -= tau, n.hi[k] += tau;
+      if (!Bvh::seg_box(p, q, n)) continue;
+      if (n.count) {
+        for (int i = n.start; i < n.start + n.count; ++i) cand.push_back(B.order[i]);
+      } else {
+        st[sp++] = n.left;
+        st[sp++] = n.right;
+      }
+    }
+  }
+  bool all_graze = true;
+  const float tmin = 0.001f, tcut = (float)((double)len - tau);
+  for (int t : cand) {
+    const float* a = V + 3 * (size_t)T[3 * t];
+    const float* b = V + 3 * (size_t)T[3 * t + 1];
+    const float* c = V + 3 * (size_t)T[3 * t + 2];
+    const bool ex = contact(p, q, ld(a), ld(b), ld(c));
+    float th = 0.0f;
+    const bool h = watertight_hit(r, a, b, c, th);
+    const bool win = h && th >= tmin && th <= len;
+    if (ex) bits |= 1;
+    if (win) bits |= 2;
+    if (h && th >= 0.0f && th <= len) bits |= 4;
+    if (h && th >= tmin && th <= tcut) bits |= 8;
+    if (ex != win && all_graze && !grazing(p, q, ld(a), ld(b), ld(c), tau)) all_graze = false;
+  }
+  if (all_graze) bits |= 16;
+  return bits;
+}
 }  // namespace
 
 extern "C" {
@@ -783,7 +952,7 @@ void orc_score_q32(const float* planes, int64_t P, const float* pts, int64_t n,
     const float* u = pts + 3 * (size_t)edges[2 * e];
     const float* v = pts + 3 * (size_t)edges[2 * e + 1];
     float dx = v[0] - u[0], dy = v[1] - u[1], dz = v[2] - u[2];
-    float l = std::sqrt(std::fmaf(dz, dz, std::fmaf(dy, dy, dx * dx)));
+    float l = std::sqrt(std::fma(dz, dz, std::fma(dy, dy, dx * dx)));
     lq[e] = (int64_t)std::nearbyint((double)l * 4294967296.0);
   }
 #pragma omp parallel for schedule(dynamic, 1)
@@ -875,6 +1044,26 @@ int64_t orc_bisector_planes(const double* V, const uint32_t* edges, int64_t E, i
     }
   }
   return np;
+}
+
+// The reference's ray-window rule against the exact contact definition,
+// for the pairs k of `ks` (k order indices, or all pairs when ks is NULL):
+// out[t] = window_bits of pair ks[t] (bits in the comment above it).
+void orc_window_hits(const float* pts, int64_t n, const float* V, const uint32_t* T,
+                     int64_t nT, const int64_t* ks, int64_t m, double tau, uint8_t* out) {
+  Bvh B;
+  B.build(V, T, nT);
+  const int64_t total = ks ? m : n * (n - 1) / 2;
+#pragma omp parallel
+  {
+    std::vector<int> cand;
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t t = 0; t < total; ++t) {
+      int64_t i, j;
+      k_to_ij(ks ? ks[t] : t, n, i, j);
+      out[t] = window_bits(B, V, T, pts + 3 * i, pts + 3 * j, tau, cand);
+    }
+  }
 }
 
 }  // extern "C"

@@ -160,8 +160,9 @@ def test_visibility_cage_mode(ctx):
 
 
 @pytest.mark.parametrize("name", ["torus", "poly", "open", "single"])
-def test_visibility_binary_and_wide_walks_agree(ctx, name):
-    """Both node layouts give the oracle's edges (binary walk forced off and on)."""
+def test_visibility_walk_forms_agree(ctx, name):
+    """Every walk form gives the oracle's edges: with and without the fan
+    pass, depth first over 1..16 lanes per pair, breadth first over 4..64."""
     from visacd_amd import native
 
     if name == "torus":
@@ -176,16 +177,13 @@ def test_visibility_binary_and_wide_walks_agree(ctx, name):
         T = np.array([[0, 1, 2]], np.uint32)
     _print("walks", name, T.shape)
     o, _ = oracle.visibility(V, V, T)
-    for wide in (0, 1):
-        for fan in (0, 1):
-            ctx.set_option(native.VACD_OPT_WIDE_BVH, wide)
-            ctx.set_option(native.VACD_OPT_FAN_PAIRS, fan)
-            try:
-                g = ctx.visibility(V, ctx.bvh(V, T))
-            finally:
-                ctx.set_option(native.VACD_OPT_WIDE_BVH, 0)
-                ctx.set_option(native.VACD_OPT_FAN_PAIRS, 1)
-            np.testing.assert_array_equal(g, o, err_msg=f"wide={wide} fan={fan}")
+    for fan in (0, 1):
+        ctx.set_option(native.VACD_OPT_FAN_PAIRS, fan)
+        try:
+            g = ctx.visibility(V, ctx.bvh(V, T))
+        finally:
+            ctx.set_option(native.VACD_OPT_FAN_PAIRS, 1)
+        np.testing.assert_array_equal(g, o, err_msg=f"fan={fan}")
     # the walks the fan leaves open, split over 1..16 lanes per pair (depth
     # first), and breadth first with 4..64 lanes per pair
     ctx.set_option(native.VACD_OPT_FRONT_LANES, 0)
@@ -202,38 +200,6 @@ def test_visibility_binary_and_wide_walks_agree(ctx, name):
     finally:
         ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
         ctx.set_option(native.VACD_OPT_FRONT_LANES, 8)
-
-
-@pytest.mark.parametrize("level,amp,seed", [(3, 0.5, 23), (3, 0.3, 5)])
-def test_inside_grid_settles_pairs_without_changing_edges(ctx, level, amp, seed):
-    """Pairs that leave both ends outwards and cross a grid cell lying inside
-    the solid are settled without a walk: the oracle's edges with the grid on
-    and off, with fewer walks when it is on; points off the surface too."""
-    from visacd_amd import native
-
-    V, T = wl.perturbed_icosphere(level, amp, seed)
-    rng = np.random.default_rng(seed)
-    off = rng.uniform(-0.9, 0.9, size=(60, 3)).astype(np.float32)
-    _print("inside grid", level, amp, seed)
-    ctx.set_option(native.VACD_OPT_INSIDE_GRID, 1)
-    try:
-        b = ctx.bvh(V, T)  # built with its grid
-        walked = {}
-        for pts in (V, np.concatenate([V[::3], off], 0)):
-            o, _ = oracle.visibility(pts, V, T)
-            for grid in (1, 0):
-                ctx.set_option(native.VACD_OPT_INSIDE_GRID, grid)
-                ctx.set_option(native.VACD_OPT_COUNT_WORK, 1)
-                g = ctx.visibility(pts, b)
-                walked[grid] = ctx.visibility_stats()["walked"]
-                settled = ctx.visibility_counters()["settled_by_grid"]
-                ctx.set_option(native.VACD_OPT_COUNT_WORK, 0)
-                np.testing.assert_array_equal(g, o, err_msg=f"inside grid {grid}")
-                assert (settled > 0) == (grid == 1)
-            assert walked[1] < walked[0], walked
-    finally:
-        ctx.set_option(native.VACD_OPT_INSIDE_GRID, 0)
-        ctx.set_option(native.VACD_OPT_COUNT_WORK, 0)
 
 
 def test_visibility_open_mesh(ctx):

@@ -45,21 +45,6 @@ struct alignas(16) Tri {
 };
 static_assert(sizeof(Tri) == 48, "triangle is 48 bytes");
 
-// Inside grid of a closed mesh: kGrid^3 cells over its box, one 64-bit word
-// per (y, z) column with bit x set when cell (x, y, z) lies wholly inside the
-// solid.  Cell of coordinate c along axis k: floor((c - lo[k]) * inv[k]).
-constexpr int kGrid = 64;
-struct GridBox {
-  float lo[3];
-  float inv[3];
-  float eps;  // distance kept between inside cells and the surface
-};
-
-__device__ __forceinline__ int grid_cell(float x, float lo, float inv) {
-  const int i = (int)floorf((x - lo) * inv);
-  return i < 0 ? 0 : (i >= kGrid ? kGrid - 1 : i);
-}
-
 // ---------------------------------------------------------------- errors
 struct Error {
   int code;
@@ -150,11 +135,7 @@ struct vacd_ctx {
   int64_t last_segments = 0;
   bool timing = false;
   bool count_work = false;
-  bool wide_bvh = false;  // 4-wide node walk (measured slower than binary on gfx950)
   bool fan_pairs = true;  // decide pairs that start inside from the fan of p
-  // settle pairs crossing a cell inside the solid (the grid is built with
-  // each closed BVH while this is on)
-  bool inside_grid = false;
   int64_t score_chunk = 0;  // edges per scoring block (0: sized for ~4096 blocks)
   int lanes_per_pair_log2 = 0;  // queued-pair walks split over 2^k lanes (k <= 4)
   // walk queues of at most front_max pairs are walked breadth-first with
@@ -259,12 +240,7 @@ struct vacd_bvh {
   // per group g at bit 2g, the axis dropped (the largest component of the
   // plane normal: 0 x, 1 y, 2 z); 3 for a group dropped by its tolerance.
   uint32_t group_axis = 0xffffffffu;
-  // Inside grid (closed meshes): cells wholly inside the solid.
-  bool has_grid = false;
-  vacd::GridBox grid{};
-  vacd::DevBuf inside;  // uint64[kGrid * kGrid]
   void release() {
-    inside.release();
     nodes.release();
     nodes4.release();
     tris.release();

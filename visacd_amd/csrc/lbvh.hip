@@ -808,148 +808,6 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
 }
 
 
-// ------------------------------------------------------------ inside grid
-// A kGrid^3 grid over the mesh's box whose cells are marked "inside" when
-// they lie wholly inside the solid, at least kGridEps of the box from the
-// surface (closed meshes only).  A segment with a point in such a cell has a
-// point strictly inside the solid, so it is never a visibility edge: the
-// fan pass uses this to settle pairs that leave both ends outwards and
-// cross the solid, without a BVH walk.
-//   surface: cells overlapped by a triangle's box widened by kGridEps;
-//   parity:  per column (y, z of the cell centres) the crossings of the
-//            line through the centres with the triangles; a crossing at x
-//            flips the cells whose centre lies beyond x, so a cell's bit is
-//            the crossing parity of the ray from its centre towards -x,
-//            i.e. whether the centre is inside;
-//   poison:  columns whose line meets a triangle too close to its boundary
-//            for the float64 filter to tell (no cell of them is used).
-// The position of a crossing matters only for cells whose centre is within
-// rounding of it, and those cells contain the crossing point, so they are
-// surface cells.
-constexpr float kGridEps = 1e-5f;
-
-__global__ __launch_bounds__(256) void k_grid_clear(unsigned long long *__restrict__ g, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) g[i] = 0ull;
-}
-
-__global__ __launch_bounds__(256) void k_grid_surface(const float *__restrict__ V,
-                                                      const uint32_t *__restrict__ T, int64_t nT,
-                                                      GridBox gb,
-                                                      unsigned long long *__restrict__ surf) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= nT) return;
-  float lo[3], hi[3];
-  for (int k = 0; k < 3; ++k) lo[k] = INFINITY, hi[k] = -INFINITY;
-  for (int v = 0; v < 3; ++v)
-    for (int k = 0; k < 3; ++k) {
-      const float c = V[3 * (size_t)T[3 * t + v] + k];
-      lo[k] = fminf(lo[k], c), hi[k] = fmaxf(hi[k], c);
-    }
-  int i0[3], i1[3];
-  for (int k = 0; k < 3; ++k) {
-    i0[k] = grid_cell(lo[k] - gb.eps, gb.lo[k], gb.inv[k]);
-    i1[k] = grid_cell(hi[k] + gb.eps, gb.lo[k], gb.inv[k]);
-  }
-  const unsigned long long m =
-      (i1[0] == kGrid - 1 ? ~0ull : ((1ull << (i1[0] + 1)) - 1ull)) & ~((1ull << i0[0]) - 1ull);
-  for (int z = i0[2]; z <= i1[2]; ++z)
-    for (int y = i0[1]; y <= i1[1]; ++y) atomicOr(&surf[z * kGrid + y], m);
-}
-
-__global__ __launch_bounds__(256) void k_grid_parity(const float *__restrict__ V,
-                                                     const uint32_t *__restrict__ T, int64_t nT,
-                                                     GridBox gb,
-                                                     unsigned long long *__restrict__ par,
-                                                     unsigned long long *__restrict__ poison) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= nT) return;
-  float P[3][3];
-  for (int v = 0; v < 3; ++v)
-    for (int k = 0; k < 3; ++k) P[v][k] = V[3 * (size_t)T[3 * t + v] + k];
-  float lo[3], hi[3];
-  for (int k = 1; k < 3; ++k) {
-    lo[k] = fminf(fminf(P[0][k], P[1][k]), P[2][k]);
-    hi[k] = fmaxf(fmaxf(P[0][k], P[1][k]), P[2][k]);
-  }
-  // columns whose centre may lie in the projection (one extra each side;
-  // the orientation tests decide)
-  const int y0 = max(0, (int)floorf((lo[1] - gb.lo[1]) * gb.inv[1] - 0.5f) - 1);
-  const int y1 = min(kGrid - 1, (int)floorf((hi[1] - gb.lo[1]) * gb.inv[1] - 0.5f) + 1);
-  const int z0 = max(0, (int)floorf((lo[2] - gb.lo[2]) * gb.inv[2] - 0.5f) - 1);
-  const int z1 = min(kGrid - 1, (int)floorf((hi[2] - gb.lo[2]) * gb.inv[2] - 0.5f) + 1);
-  // plane normal for the crossing's x (float64)
-  const double e1[3] = {(double)P[1][0] - P[0][0], (double)P[1][1] - P[0][1], (double)P[1][2] - P[0][2]};
-  const double e2[3] = {(double)P[2][0] - P[0][0], (double)P[2][1] - P[0][1], (double)P[2][2] - P[0][2]};
-  const double nx = e1[1] * e2[2] - e1[2] * e2[1], ny = e1[2] * e2[0] - e1[0] * e2[2],
-               nz = e1[0] * e2[1] - e1[1] * e2[0];
-  for (int z = z0; z <= z1; ++z)
-    for (int y = y0; y <= y1; ++y) {
-      const float c[3] = {0.0f, gb.lo[1] + (y + 0.5f) / gb.inv[1], gb.lo[2] + (z + 0.5f) / gb.inv[2]};
-      double b1, b2, b3;
-      const double o1 = o2_fast(P[0], P[1], c, 0, b1);
-      const double o2 = o2_fast(P[1], P[2], c, 0, b2);
-      const double o3 = o2_fast(P[2], P[0], c, 0, b3);
-      const bool pos = o1 > b1 || o2 > b2 || o3 > b3;
-      const bool neg = -o1 > b1 || -o2 > b2 || -o3 > b3;
-      if (pos && neg) continue;  // the line misses the triangle
-      const bool all = (o1 > b1 || -o1 > b1) && (o2 > b2 || -o2 > b2) && (o3 > b3 || -o3 > b3);
-      if (!all) {  // on or near the projected boundary: the column is not used
-        atomicOr(&poison[z * kGrid + y], 1ull);
-        continue;
-      }
-      if (!(nx != 0.0)) {  // no area in float64 after all: do not use the column
-        atomicOr(&poison[z * kGrid + y], 1ull);
-        continue;
-      }
-      // crossing x on the triangle's plane
-      const double xc = (double)P[0][0] - (ny * ((double)c[1] - P[0][1]) + nz * ((double)c[2] - P[0][2])) / nx;
-      // flip the cells whose centre lies beyond xc
-      const double f = (xc - (double)gb.lo[0]) * (double)gb.inv[0] - 0.5;
-      const int i0 = f < 0.0 ? 0 : (f >= (double)kGrid ? kGrid : (int)floor(f) + 1);
-      const unsigned long long m = i0 >= kGrid ? 0ull : (~0ull << i0);
-      if (m) atomicXor(&par[z * kGrid + y], m);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_grid_finish(const unsigned long long *__restrict__ surf,
-                                                     const unsigned long long *__restrict__ par,
-                                                     const unsigned long long *__restrict__ poison,
-                                                     unsigned long long *__restrict__ inside) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kGrid * kGrid) inside[i] = poison[i] ? 0ull : (par[i] & ~surf[i]);
-}
-
-// Inside grid of a closed mesh (out->bounds set).
-void build_inside_grid(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, int64_t nT,
-                       vacd_bvh *out) {
-  hipStream_t st = ctx->stream;
-  GridBox gb;
-  float ext = 0.0f;
-  for (int k = 0; k < 3; ++k) ext = fmaxf(ext, out->bounds[3 + k] - out->bounds[k]);
-  if (!(ext > 0.0f)) return;
-  gb.eps = kGridEps * ext;
-  for (int k = 0; k < 3; ++k) {
-    const float pad = 2.0f * gb.eps;
-    const float lo = out->bounds[k] - pad, hi = out->bounds[3 + k] + pad;
-    gb.lo[k] = lo;
-    gb.inv[k] = (float)kGrid / fmaxf(hi - lo, 1e-6f * ext);
-  }
-  constexpr int kCols = kGrid * kGrid;
-  vacd::DevBuf &sb = ctx->buf[SLOT_GRID];
-  sb.ensure(3 * sizeof(unsigned long long) * kCols);
-  unsigned long long *surf = sb.as<unsigned long long>(), *par = surf + kCols, *poison = par + kCols;
-  k_grid_clear<<<grid_for(3 * kCols, 256), 256, 0, st>>>(surf, 3 * kCols);
-  k_grid_surface<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, gb, surf);
-  k_grid_parity<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, gb, par, poison);
-  out->inside.ensure(sizeof(unsigned long long) * kCols);
-  k_grid_finish<<<grid_for(kCols, 256), 256, 0, st>>>(surf, par, poison,
-                                                       out->inside.as<unsigned long long>());
-  VACD_HIP(hipGetLastError());
-  out->grid = gb;
-  out->has_grid = true;
-}
-
 }  // namespace
 
 void scan_u32(vacd_ctx *ctx, uint32_t *d_data, int64_t m) {
@@ -1046,8 +904,6 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   out->small_first = true;
   out->closed = h_hash[0] == 0 && h_hash[1] == 0;
   for (int k = 0; k < 6; ++k) out->bounds[k] = h_root[k];
-  out->has_grid = false;
-  if (out->closed && ctx->inside_grid) build_inside_grid(ctx, d_V, d_T, nT, out);
   build_contact_tree(ctx, d_V, d_T, nT, out, d_grp);
 }
 

@@ -130,14 +130,10 @@ int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value) {
     ctx->timing = value != 0;
   else if (option == VACD_OPT_COUNT_WORK)
     ctx->count_work = value != 0;
-  else if (option == VACD_OPT_WIDE_BVH)
-    ctx->wide_bvh = value != 0;
   else if (option == VACD_OPT_FRONT_LANES && (value == 0 || value == 4 || value == 8 || value == 16 || value == 32 || value == 64))
     ctx->front_lanes = (int)value;
   else if (option == VACD_OPT_FRONT_MAX && value >= 0)
     ctx->front_max = value;
-  else if (option == VACD_OPT_INSIDE_GRID)
-    ctx->inside_grid = value != 0;
   else if (option == VACD_OPT_FAN_PAIRS)
     ctx->fan_pairs = value != 0;
   else if (option == VACD_OPT_SCORE_CHUNK && value >= 0 && value <= (int64_t(1) << 30))

@@ -501,82 +501,6 @@ __device__ __forceinline__ float lane_of(const float4 &v, int c) {
   return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
 }
 
-// 4-wide forms of walk_any / walk_parity over Node4 (same results; about half
-// as many dependent node loads).  Hit internal children are taken in slot
-// order: the first is visited next, the others are pushed.
-template <int STACK, bool COUNT>
-__device__ __forceinline__ int walk_any4(const Node4 *__restrict__ nodes,
-                                         const Tri *__restrict__ tris, const Seg &s, int *stk,
-                                         int cap, unsigned &nv, unsigned &nt, unsigned &nex,
-                                         int &xpar, int &ninc) {
-  int node = 0, sp = 0;
-  bool local_ok = true, unsure = false;
-  for (int it = 0;; ++it) {
-    if (it > cap) return 3;
-    const Node4 nd = nodes[node];
-    if (COUNT) ++nv;
-    int next = -1;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (!seg_box(s, lane_of(nd.f[0], c), lane_of(nd.f[1], c), lane_of(nd.f[2], c),
-                   lane_of(nd.f[3], c), lane_of(nd.f[4], c), lane_of(nd.f[5], c)))
-        continue;
-      const int id = __float_as_int(lane_of(nd.f[6], c));
-      if (id < 0) {
-        if (COUNT) ++nt;
-        if (contact_test<false>(s, tris[~id], xpar, ninc, local_ok, unsure, nex)) return 1;
-      } else if (next < 0) {
-        next = id;
-      } else {
-        stk[(sp++) * 64] = id;
-      }
-    }
-    if (next >= 0) {
-      node = next;
-    } else {
-      if (sp == 0) break;
-      node = stk[(--sp) * 64];
-    }
-  }
-  return unsure ? 4 : (local_ok ? 0 : 2);
-}
-
-template <int STACK, bool COUNT>
-__device__ __forceinline__ int walk_parity4(const Node4 *__restrict__ nodes,
-                                            const Tri *__restrict__ tris, const Seg &s, int *stk,
-                                            int cap, unsigned &nv, unsigned &nt, unsigned &nex) {
-  int node = 0, sp = 0, cnt = 0;
-  bool unsure = false;
-  for (int it = 0;; ++it) {
-    if (it > cap) return 3;
-    const Node4 nd = nodes[node];
-    if (COUNT) ++nv;
-    int next = -1;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (!ray_box(s, lane_of(nd.f[0], c), lane_of(nd.f[1], c), lane_of(nd.f[2], c),
-                   lane_of(nd.f[3], c), lane_of(nd.f[4], c), lane_of(nd.f[5], c)))
-        continue;
-      const int id = __float_as_int(lane_of(nd.f[6], c));
-      if (id < 0) {
-        if (COUNT) ++nt;
-        cnt += cross_test<false>(s, tris[~id], unsure, nex);
-      } else if (next < 0) {
-        next = id;
-      } else {
-        stk[(sp++) * 64] = id;
-      }
-    }
-    if (next >= 0) {
-      node = next;
-    } else {
-      if (sp == 0) break;
-      node = stk[(--sp) * 64];
-    }
-  }
-  return unsure ? 2 : (cnt & 1);
-}
-
 // What a pair needs to know about its part (one mesh and its point set).
 constexpr int kFanMax = 16;  // triangles kept per point fan
 
@@ -692,8 +616,6 @@ struct Part {
   const int *fan_idx;  // per point: kFanMax leaf-ordered triangle ids
   const FanRec *fan_rec;  // per point: kFanMax prepared fan triangles
   const FanCone *fan_cone;  // per point: the fan's cones
-  const unsigned long long *inside;  // inside grid of the closed self mesh (NULL: none)
-  GridBox grid;
   const Node *snodes;
   const Node4 *snodes4;
   const Tri *stris;
@@ -701,7 +623,6 @@ struct Part {
   const Tri *rtris;
   const float4 *rgroups;  // planar groups of the contact tree (NULL: none)
   const Node *cnodes;  // cage (NULL: parity mode)
-  const Node4 *cnodes4;
   const Tri *ctris;
   uint32_t group_axis;  // per planar group of the self mesh, its projection axis (vacd_bvh)
   int64_t n, S;
@@ -724,7 +645,7 @@ struct VisArgs {
                               // with [10] no usable fan at p, [11] p's fan
                               // test undecided, [12] no usable fan at q,
                               // [13] q's undecided, [14] both ends leaving
-                              // outwards; [15] pairs settled by the grid
+                              // outwards
   int64_t *queue;             // global pair ids (word * 64 + bit) left to walk
   unsigned long long *queue_len;
   uint64_t *qbits;            // per ballot word: its pairs left to walk (compacted into
@@ -752,40 +673,27 @@ __device__ __forceinline__ Part part_of_word(const VisArgs &a, int64_t w) {
   return a.parts[lo];
 }
 
-template <int STACK, bool WIDE, bool COUNT, bool EXACT>
-__device__ __forceinline__ int any_hit(const Node *n2, const Node4 *n4, const Tri *tris,
-                                       const Seg &s, int *stk, int cap, unsigned &nv,
-                                       unsigned &nt, unsigned &nex, int &xpar, int &ninc,
-                                       const float4 *grp = nullptr) {
-  if (WIDE && !EXACT) return walk_any4<STACK, COUNT>(n4, tris, s, stk, cap, nv, nt, nex, xpar, ninc);
+template <int STACK, bool COUNT, bool EXACT>
+__device__ __forceinline__ int any_hit(const Node *n2, const Tri *tris, const Seg &s, int *stk,
+                                       int cap, unsigned &nv, unsigned &nt, unsigned &nex,
+                                       int &xpar, int &ninc, const float4 *grp = nullptr) {
   return walk_any<STACK, COUNT, EXACT>(n2, tris, s, stk, cap, nv, nt, nex, xpar, ninc, 0, nullptr,
                                        grp);
-}
-
-template <int STACK, bool WIDE, bool COUNT, bool EXACT>
-__device__ __forceinline__ int ray_parity(const Node *n2, const Node4 *n4, const Tri *tris,
-                                          const Seg &s, int *stk, int cap, unsigned &nv,
-                                          unsigned &nt, unsigned &nex) {
-  if (WIDE && !EXACT) return walk_parity4<STACK, COUNT>(n4, tris, s, stk, cap, nv, nt, nex);
-  return walk_parity<STACK, COUNT, EXACT>(n2, tris, s, stk, cap, nv, nt, nex);
 }
 
 // Walks for one pair the fan could not settle: any contact with the self
 // mesh, then the cage walk (cage mode) or the inside parity (parity mode):
 // local when p's fan crossings are all known, else the midpoint ray.  A
 // filtered walk (EXACT false) that meets an undecided test sets defer.
-template <int STACK, bool CAGE, bool COUNT, bool WIDE, bool EXACT>
+template <int STACK, bool CAGE, bool COUNT, bool EXACT>
 __device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int *stk, bool &edge,
                                           bool &defer, unsigned &n_blocked, unsigned &n_bad,
                                           unsigned &nv, unsigned &nt, unsigned &nex) {
   make_seg(s);
   int xpar = 0, ninc = 0;
-  // binary walks use the contact tree; the 4-wide one the plain tree
-  const int hs = WIDE ? any_hit<STACK, WIDE, COUNT, EXACT>(P.snodes, P.snodes4, P.stris, s, stk,
-                                                           P.scap, nv, nt, nex, xpar, ninc)
-                      : any_hit<STACK, WIDE, COUNT, EXACT>(P.rnodes, P.snodes4, P.rtris, s, stk,
-                                                           P.rcap, nv, nt, nex, xpar, ninc,
-                                                           P.rgroups);
+  // contacts: the contact tree
+  const int hs = any_hit<STACK, COUNT, EXACT>(P.rnodes, P.rtris, s, stk, P.rcap, nv, nt, nex, xpar,
+                                              ninc, P.rgroups);
   if (hs == 1) {
     ++n_blocked;
   } else if (hs == 3) {
@@ -794,8 +702,8 @@ __device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int 
     defer = true;
   } else if (CAGE && P.ctris) {
     int cx = 0, cn = 0;
-    const int hc = any_hit<STACK, WIDE, COUNT, EXACT>(P.cnodes, P.cnodes4, P.ctris, s, stk,
-                                                      P.ccap, nv, nt, nex, cx, cn);
+    const int hc = any_hit<STACK, COUNT, EXACT>(P.cnodes, P.ctris, s, stk, P.ccap, nv, nt, nex, cx,
+                                                cn);
     edge = hc == 1;
     defer = hc == 4;
     n_bad += hc == 3;
@@ -803,40 +711,11 @@ __device__ __forceinline__ void walk_pair(const Part &P, int64_t i, Seg &s, int 
     // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
     edge = ((xpar ^ P.far[i]) & 1) == 0;
   } else {  // no local parity for this pair: the midpoint ray itself
-    const int par = ray_parity<STACK, WIDE, COUNT, EXACT>(P.snodes, P.snodes4, P.stris, s, stk,
-                                                          P.scap, nv, nt, nex);
+    const int par = walk_parity<STACK, COUNT, EXACT>(P.snodes, P.stris, s, stk, P.scap, nv, nt, nex);
     edge = par == 0;
     defer = par == 2;
     n_bad += par == 3;
   }
-}
-
-// Whether the segment has a sample point in a cell of the inside grid (the
-// cell lies wholly inside the solid, away from the surface by more than the
-// sample's rounding): then the segment has a point strictly inside the
-// solid and is not an edge.  Samples at most half a cell apart along every
-// axis, taken coarse to fine (midpoint first): a segment that leaves both
-// ends outwards and crosses the solid does so away from its ends.
-__device__ __forceinline__ bool meets_inside(const Part &P, const Seg &s) {
-  const GridBox &g = P.grid;
-  float d[3], span = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    d[k] = s.q[k] - s.p[k];
-    span = fmaxf(span, fabsf(d[k]) * g.inv[k]);
-  }
-  int lg = 1;
-  while ((float)(1 << lg) < 2.0f * span && lg < 8) ++lg;
-  const float invK = 1.0f / (float)(1 << lg);
-  for (int i = 0; i < (1 << lg); ++i) {
-    const int k = (int)(__brev((unsigned)i) >> (32 - lg));
-    const float t = ((float)k + 0.5f) * invK;
-    const int cx = grid_cell(fmaf(t, d[0], s.p[0]), g.lo[0], g.inv[0]);
-    const int cy = grid_cell(fmaf(t, d[1], s.p[1]), g.lo[1], g.inv[1]);
-    const int cz = grid_cell(fmaf(t, d[2], s.p[2]), g.lo[2], g.inv[2]);
-    if ((P.inside[cz * kGrid + cy] >> cx) & 1ull) return true;
-  }
-  return false;
 }
 
 // Mask of a point: bit 0 new (lib/ptx/ray_segments.cu:61-75: a pair is
@@ -919,7 +798,7 @@ __device__ __forceinline__ void append_lanes(bool mine, int64_t value, int64_t *
 
 // CAP: the mask may carry cap-rule bits (its filtered 2-D tests are
 // compiled only where needed: they cost the fan pass registers).
-template <int STACK, bool CAGE, bool COUNT, bool WIDE, bool BATCH, bool QUEUE, bool CAP = false>
+template <int STACK, bool CAGE, bool COUNT, bool BATCH, bool QUEUE, bool CAP = false>
 __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   __shared__ int stk_all[kWaves][STACK][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -927,8 +806,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   unsigned n_masked = 0, n_blocked = 0, n_exact = 0, n_bad = 0, nv = 0, nt = 0, n_walk = 0;
   (void)n_exact;  // filtered walks make no exact tests
   // work counters (COUNT): undecided pairs by the state of each end's fan
-  // test, and pairs settled by the inside grid
-  unsigned n_pno = 0, n_pund = 0, n_qno = 0, n_qund = 0, n_out = 0, n_grid = 0;
+  // test
+  unsigned n_pno = 0, n_pund = 0, n_qno = 0, n_qund = 0, n_out = 0;
   const int64_t total_waves = (int64_t)gridDim.x * kWaves;
   for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < a.nwords; w += total_waves) {
     const Part P = part_of_word<BATCH>(a, w);  // wave-uniform
@@ -1012,12 +891,6 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             }
             if (COUNT) qstate = (r == 0 && P.far[j] <= 1) ? 1 : 3;
           }
-          // Leaves both ends outwards: settled when it crosses a cell that
-          // lies inside the solid.
-          if (!decided && P.inside) {
-            decided = meets_inside(P, s);
-            if (COUNT && decided) ++n_grid;
-          }
         }
         if (COUNT && !decided) {
           n_pno += pstate == 2, n_pund += pstate == 3, n_qno += qstate == 2, n_qund += qstate == 3;
@@ -1028,7 +901,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             queued = true;
           } else {
             ++n_walk;
-            walk_pair<STACK, CAGE, COUNT, WIDE, false>(P, i, s, stk, edge, defer, n_blocked, n_bad,
+            walk_pair<STACK, CAGE, COUNT, false>(P, i, s, stk, edge, defer, n_blocked, n_bad,
                                                        nv, nt, n_exact);
           }
         }
@@ -1065,18 +938,18 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   }
   if (COUNT) {
     unsigned long long v = nv, t = nt;
-    unsigned c[6] = {n_pno, n_pund, n_qno, n_qund, n_out, n_grid};
+    unsigned c[5] = {n_pno, n_pund, n_qno, n_qund, n_out};
     for (int off = 32; off > 0; off >>= 1) {
       v += __shfl_down(v, off);
       t += __shfl_down(t, off);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) c[k] += __shfl_down(c[k], off);
+      for (int k = 0; k < 5; ++k) c[k] += __shfl_down(c[k], off);
     }
     if (lane == 0) {
       atomicAdd(&a.stats[5], v);
       atomicAdd(&a.stats[6], t);
 #pragma unroll
-      for (int k = 0; k < 6; ++k)
+      for (int k = 0; k < 5; ++k)
         if (c[k]) atomicAdd(&a.stats[10 + k], (unsigned long long)c[k]);
     }
   }
@@ -2145,13 +2018,13 @@ void launch_split(const Launch &L, const VisArgs &a) {
 
 // The fan pass queues what it cannot settle (no walk stack needed there);
 // the queue is then walked densely.
-template <int S, bool C, bool N, bool W, bool B>
+template <int S, bool C, bool N, bool B>
 void launch_main(bool q, const Launch &L, const VisArgs &a) {
   if (q && !C) {
     if (L.cap)  // no walk, no stack
-      k_visibility<1, C, N, W, B, true, true><<<L.grid, kBlock, 0, L.st>>>(a);
+      k_visibility<1, C, N, B, true, true><<<L.grid, kBlock, 0, L.st>>>(a);
     else
-      k_visibility<1, C, N, W, B, true><<<L.grid, kBlock, 0, L.st>>>(a);
+      k_visibility<1, C, N, B, true><<<L.grid, kBlock, 0, L.st>>>(a);
     // compact the per-word bits into the queue
     k_word_scan<<<(unsigned)L.nscan, 256, 0, L.st>>>(a.qbits, a.nwords, L.qscan);
     k_scan_blocks<<<1, 1024, 0, L.st>>>(L.qscan, L.nscan, nullptr, a.queue_len);
@@ -2159,28 +2032,23 @@ void launch_main(bool q, const Launch &L, const VisArgs &a) {
     launch_front<N, B>(L, a);  // a short queue: breadth-first walks
     launch_split<S, N, B>(L, a);  // a long one: depth-first walks
   } else {  // the rare path (open parts, cages): one instance, cap rule compiled in
-    k_visibility<S, C, N, W, B, false, true><<<L.grid, kBlock, 0, L.st>>>(a);
+    k_visibility<S, C, N, B, false, true><<<L.grid, kBlock, 0, L.st>>>(a);
   }
 }
-template <int S, bool C, bool N, bool W>
-void pick_b(bool b, bool q, const Launch &L, const VisArgs &a) {
-  if (b) launch_main<S, C, N, W, true>(q, L, a);
-  else launch_main<S, C, N, W, false>(q, L, a);
-}
 template <int S, bool C, bool N>
-void pick_w(bool w, bool b, bool q, const Launch &L, const VisArgs &a) {
-  if (w) pick_b<S, C, N, true>(b, q, L, a);
-  else pick_b<S, C, N, false>(b, q, L, a);
+void pick_b(bool b, bool q, const Launch &L, const VisArgs &a) {
+  if (b) launch_main<S, C, N, true>(q, L, a);
+  else launch_main<S, C, N, false>(q, L, a);
 }
 template <int S, bool C>
-void pick_n(bool n, bool w, bool b, bool q, const Launch &L, const VisArgs &a) {
-  if (n) pick_w<S, C, true>(w, b, q, L, a);
-  else pick_w<S, C, false>(w, b, q, L, a);
+void pick_n(bool n, bool b, bool q, const Launch &L, const VisArgs &a) {
+  if (n) pick_b<S, C, true>(b, q, L, a);
+  else pick_b<S, C, false>(b, q, L, a);
 }
 template <int S>
-void pick_c(bool c, bool n, bool w, bool b, bool q, const Launch &L, const VisArgs &a) {
-  if (c) pick_n<S, true>(n, w, b, q, L, a);
-  else pick_n<S, false>(n, w, b, q, L, a);
+void pick_c(bool c, bool n, bool b, bool q, const Launch &L, const VisArgs &a) {
+  if (c) pick_n<S, true>(n, b, q, L, a);
+  else pick_n<S, false>(n, b, q, L, a);
 }
 
 Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_bvh *self,
@@ -2193,13 +2061,10 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   P.fan_idx = nullptr;
   P.fan_rec = nullptr;
   P.fan_cone = nullptr;
-  P.inside = nullptr;
-  P.grid = GridBox{};
   P.snodes = self->nodes.as<Node>();
   P.snodes4 = self->nodes4.as<Node4>();
   P.stris = self->tris.as<Tri>();
   P.cnodes = cage ? cage->nodes.as<Node>() : nullptr;
-  P.cnodes4 = cage ? cage->nodes4.as<Node4>() : nullptr;
   P.ctris = cage ? cage->tris.as<Tri>() : nullptr;
   P.n = n;
   P.S = n >= 2 ? n * (n - 1) / 2 : 0;
@@ -2310,11 +2175,10 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
       VACD_HIP(hipGetLastError());
     }
   }
-  // stack entries a walk can hold: binary, stack2 (see binary_stack);
-  // 4-wide, up to three per visited level (every other binary level)
+  // stack entries a walk can hold: stack2 (see binary_stack); the fan
+  // query's 4-wide walk, up to three per visited level (every other binary
+  // level)
   const int need2 = stack2, need4 = 3 * (depth / 2);
-  const bool wide = ctx->wide_bvh && need4 <= 64;
-  const int need = wide ? need4 : need2;
   const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
   Launch L;
   L.grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
@@ -2340,10 +2204,10 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
     a.qbits = reinterpret_cast<uint64_t *>(a.queue + nwords * 64);
   }
   timed_begin(ctx, TIMED_VIS);
-  if (need <= 32)
-    pick_c<32>(any_cage, ctx->count_work, wide, batch, queue, L, a);
+  if (need2 <= 32)
+    pick_c<32>(any_cage, ctx->count_work, batch, queue, L, a);
   else
-    pick_c<64>(any_cage, ctx->count_work, wide, batch, queue, L, a);
+    pick_c<64>(any_cage, ctx->count_work, batch, queue, L, a);
   VACD_HIP(hipGetLastError());
   timed_end(ctx, TIMED_VIS);
   // the deferred pairs (the list length is read on the device): a wave per
@@ -2434,10 +2298,6 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
       a.one.fan_rec = recb.as<FanRec>();
       a.one.fan_cone = reinterpret_cast<const FanCone *>(a.one.fan_rec + (size_t)n * kFanMax);
     }
-    if (ctx->inside_grid && self->has_grid) {
-      a.one.inside = self->inside.as<unsigned long long>();
-      a.one.grid = self->grid;
-    }
   }
   const int stack2 = std::max({binary_stack(self->depth, self->n_tris, self->small_first),
                                binary_stack(self->depth_refs, self->n_refs, self->small_first_refs),
@@ -2502,10 +2362,6 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
           hp[p].fan_cone = reinterpret_cast<const FanCone *>(recb.as<FanRec>() +
                                                              (size_t)points * kFanMax) +
                            hp[p].point_begin;
-        }
-        if (ctx->inside_grid && parts[p].self->has_grid) {
-          hp[p].inside = parts[p].self->inside.as<unsigned long long>();
-          hp[p].grid = parts[p].self->grid;
         }
       }
   }
