@@ -340,8 +340,15 @@ MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::ve
     cap_pts.push_back(p + (q - p) * t);
     return (int)cap_pts.size() - 1;
   };
-  // pieces: polygon corners encoded as vertex id >= 0 or ~cap id
-  std::vector<std::vector<int>> pieces[2];
+  // pieces: polygon corners (at most four: a triangle cut by a plane)
+  // encoded as vertex id >= 0 or ~cap id
+  struct Piece {
+    int c[4];
+    int k;
+  };
+  std::vector<Piece> pieces[2];
+  pieces[0].reserve(m.triangles.size());
+  pieces[1].reserve(m.triangles.size());
   std::vector<std::pair<int, int>> plane_edges[2];
   for (const auto &t : m.triangles) {
     const int s[3] = {side[t[0]], side[t[1]], side[t[2]]};
@@ -349,20 +356,25 @@ MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::ve
     for (int w = 0; w < 2; ++w) {
       const int want = w == 0 ? 1 : -1;
       if (s[0] != want && s[1] != want && s[2] != want) continue;
-      std::vector<int> poly;
+      Piece pc;
+      pc.k = 0;
+      if (s[0] == want && s[1] == want && s[2] == want) {  // wholly on this side
+        pc.c[0] = t[0], pc.c[1] = t[1], pc.c[2] = t[2], pc.k = 3;
+        pieces[w].push_back(pc);
+        continue;
+      }
       for (int e = 0; e < 3; ++e) {
         const int a = t[e], b = t[(e + 1) % 3];
         const int sa = s[e], sb = s[(e + 1) % 3];
-        if (sa == want) poly.push_back(a);
-        else if (sa == 0) poly.push_back(~cap_vertex(a));
-        if (sa != 0 && sb != 0 && sa != sb) poly.push_back(~cap_cross(a, b));
+        if (sa == want) pc.c[pc.k++] = a;
+        else if (sa == 0) pc.c[pc.k++] = ~cap_vertex(a);
+        if (sa != 0 && sb != 0 && sa != sb) pc.c[pc.k++] = ~cap_cross(a, b);
       }
-      const int k = (int)poly.size();
-      for (int e = 0; e < k; ++e) {
-        const int a = poly[e], b = poly[(e + 1) % k];
+      for (int e = 0; e < pc.k; ++e) {
+        const int a = pc.c[e], b = pc.c[(e + 1) % pc.k];
         if (a < 0 && b < 0 && a != b) plane_edges[w].push_back({~a, ~b});
       }
-      pieces[w].push_back(std::move(poly));
+      pieces[w].push_back(pc);
     }
   }
   MeshList out;
@@ -394,9 +406,10 @@ MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::ve
       }
       return cap_local[c];
     };
-    for (auto &poly : pieces[w])
-      for (size_t k = 1; k + 1 < poly.size(); ++k)
-        part.triangles.push_back({local(poly[0]), local(poly[k]), local(poly[k + 1])});
+    part.triangles.reserve(pieces[w].size() + 2 * plane_edges[w].size());
+    for (const Piece &pc : pieces[w])
+      for (int k = 1; k + 1 < pc.k; ++k)
+        part.triangles.push_back({local(pc.c[0]), local(pc.c[k]), local(pc.c[k + 1])});
     std::vector<std::array<int, 3>> cap;
     const Vec3 cap_normal = w == 0 ? n * -1.0 : n;
     if (!build_cap(plane_edges[w], cap_pts, cap_normal * (1.0 / norm(cap_normal)), cap)) {

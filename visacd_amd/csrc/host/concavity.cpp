@@ -40,6 +40,8 @@ Samples sample_surface(const Mesh &m, int resolution, std::mt19937 &rng) {
   std::discrete_distribution<size_t> pick(area.begin(), area.end());
   std::uniform_real_distribution<double> u01(0.0, 1.0);
   std::vector<char> hit(T, 0);
+  s.p.reserve(want + T);
+  s.tri.reserve(want + T);
   auto emit = [&](size_t t) {
     const auto &f = m.triangles[t];
     const double a = std::sqrt(u01(rng)), b = u01(rng);
@@ -193,12 +195,18 @@ double concavity(const Mesh &part, const Mesh &hull, int resolution, uint32_t se
 // planes (cosine of the (a,b,c,d) vectors > 1 - 1e-4) are dropped.
 std::vector<Plane> flat_surface_planes(const Mesh &m, double min_area) {
   const int T = (int)m.triangles.size();
+  // unit normals, their lengths (as the growth test recomputes them) and
+  // the triangle areas
   std::vector<Vec3> nrm(T);
+  std::vector<double> nlen(T), tarea(T);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < T; ++i) {
     const auto &t = m.triangles[i];
     Vec3 n = cross(m.vertices[t[1]] - m.vertices[t[0]], m.vertices[t[2]] - m.vertices[t[0]]);
     const double l = norm(n);
     nrm[i] = l > 0 ? n * (1.0 / l) : Vec3{0, 0, 0};
+    nlen[i] = norm(nrm[i]);
+    tarea[i] = 0.5 * l;
   }
   // triangle adjacency across shared edges, from a vertex -> triangle index
   // (for each edge (a, b) of t: the other triangles of a that contain b)
@@ -213,9 +221,10 @@ std::vector<Plane> flat_surface_planes(const Mesh &m, double min_area) {
     for (int i = 0; i < T; ++i)
       for (int k = 0; k < 3; ++k) inc[fill[m.triangles[i][k]]++] = i;
   }
+  // neighbours of every triangle, counted then written (both passes in
+  // parallel, the same order as one sequential pass)
   std::vector<int> adj_off(T + 1, 0), adj;
-  adj.reserve(3 * (size_t)T);
-  for (int i = 0; i < T; ++i) {
+  auto each_neighbour = [&](int i, auto &&emit) {
     const auto &f = m.triangles[i];
     for (int e = 0; e < 3; ++e) {
       const int a = f[e], b = f[(e + 1) % 3];
@@ -224,10 +233,22 @@ std::vector<Plane> flat_surface_planes(const Mesh &m, double min_area) {
         const int u = inc[x];
         if (u == i) continue;
         const auto &g = m.triangles[u];
-        if (g[0] == b || g[1] == b || g[2] == b) adj.push_back(u);
+        if (g[0] == b || g[1] == b || g[2] == b) emit(u);
       }
     }
-    adj_off[i + 1] = (int)adj.size();
+  };
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < T; ++i) {
+    int c = 0;
+    each_neighbour(i, [&](int) { ++c; });
+    adj_off[i + 1] = c;
+  }
+  for (int i = 0; i < T; ++i) adj_off[i + 1] += adj_off[i];
+  adj.resize(adj_off[T]);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < T; ++i) {
+    int w = adj_off[i];
+    each_neighbour(i, [&](int u) { adj[w++] = u; });
   }
   struct Region {
     std::vector<int> tris;
@@ -241,28 +262,27 @@ std::vector<Plane> flat_surface_planes(const Mesh &m, double min_area) {
     grown.assign(1, s);
     used[s] = 1;
     Vec3 mean = nrm[s];
+    double lm = norm(mean);  // |mean|, recomputed whenever mean changes
     queue.assign(1, s);
     for (size_t qh = 0; qh < queue.size(); ++qh) {
       const int t = queue[qh];
       for (int x = adj_off[t]; x < adj_off[t + 1]; ++x) {
         const int nb = adj[x];
         if (used[nb]) continue;
-        const double lm = norm(mean), ln = norm(nrm[nb]);
+        const double ln = nlen[nb];
         const double c = (lm > 0 && ln > 0) ? dot(mean, nrm[nb]) / (lm * ln) : 0.0;
         if (c > 0.999) {
           used[nb] = 1;
           grown.push_back(nb);
           const double k = (double)grown.size();
           mean = (mean * (k - 1) + nrm[nb]) * (1.0 / k);
+          lm = norm(mean);
           queue.push_back(nb);
         }
       }
     }
     double area = 0.0;
-    for (int t : grown) {
-      const auto &f = m.triangles[t];
-      area += tri_area(m.vertices[f[0]], m.vertices[f[1]], m.vertices[f[2]]);
-    }
+    for (int t : grown) area += tarea[t];
     if (area > min_area) regions.push_back(Region{grown, area});
   }
   if (regions.size() > 40) {

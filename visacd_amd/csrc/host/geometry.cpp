@@ -1,6 +1,5 @@
 // Normalisation, volumes, convex hull and connected components.
 #include <algorithm>
-#include <functional>
 #include <limits>
 #include <numeric>
 #include <unordered_map>
@@ -58,15 +57,18 @@ double surface_area(const Mesh &m) {
 // a tetrahedron of extreme points; repeatedly take the farthest outside
 // point of a face, collect the faces it sees by a flood over neighbours,
 // cone the horizon to it and hand the orphaned points to the new faces.
+// Outside sets are singly linked lists through one per-point array (points
+// keep their insertion order), faces are plain records, and the horizon's
+// adjacency uses per-vertex slots: no allocation per face or per step.
 namespace {
 struct HFace {
   int v[3];
   int nb[3];  // face across edge (v[e], v[e+1])
   Vec3 n;
   double off;
-  std::vector<int> outside;
-  bool alive = true;
-  int mark = -1;
+  int head, tail;  // outside points, in insertion order (-1: none)
+  int mark;
+  bool alive;
 };
 
 void set_plane(HFace &f, const std::vector<Vec3> &P) {
@@ -75,12 +77,26 @@ void set_plane(HFace &f, const std::vector<Vec3> &P) {
   f.n = l > 0 ? n * (1.0 / l) : Vec3{0, 0, 0};
   f.off = dot(f.n, P[f.v[0]]);
 }
-}  // namespace
 
-Mesh convex_hull(const std::vector<Vec3> &P) {
-  Mesh out;
-  const int n = (int)P.size();
+HFace new_face(int a, int b, int c, const std::vector<Vec3> &P) {
+  HFace f;
+  f.v[0] = a, f.v[1] = b, f.v[2] = c;
+  f.nb[0] = f.nb[1] = f.nb[2] = -1;
+  f.head = f.tail = -1;
+  f.mark = -1;
+  f.alive = true;
+  set_plane(f, P);
+  return f;
+}
+
+// Hull triangles (ids into Pall, outward) of the points Pall[ids[k]]; empty
+// when they are fewer than four or flat.
+std::vector<std::array<int, 3>> quickhull(const std::vector<Vec3> &Pall, const std::vector<int> &ids) {
+  std::vector<std::array<int, 3>> out;
+  const int n = (int)ids.size();
   if (n < 4) return out;
+  std::vector<Vec3> P(n);
+  for (int i = 0; i < n; ++i) P[i] = Pall[ids[i]];
   double scale = 0.0;
   for (const Vec3 &p : P) scale = std::max({scale, std::fabs(p[0]), std::fabs(p[1]), std::fabs(p[2])});
   const double eps = 1e-12 * std::max(1.0, scale);
@@ -107,12 +123,10 @@ Mesh convex_hull(const std::vector<Vec3> &P) {
   }
   if (i3 < 0 || best <= eps * norm(nrm)) return out;
   if (dot(P[i3] - P[i0], nrm) > 0) std::swap(i1, i2);  // i3 below face (i0, i1, i2)
-  std::vector<HFace> F(4);
+  std::vector<HFace> F;
+  F.reserve(64);
   const int tv[4][3] = {{i0, i1, i2}, {i0, i3, i1}, {i1, i3, i2}, {i2, i3, i0}};
-  for (int f = 0; f < 4; ++f) {
-    for (int e = 0; e < 3; ++e) F[f].v[e] = tv[f][e];
-    set_plane(F[f], P);
-  }
+  for (int f = 0; f < 4; ++f) F.push_back(new_face(tv[f][0], tv[f][1], tv[f][2], P));
   // tetrahedron adjacency by matching reversed edges
   for (int f = 0; f < 4; ++f)
     for (int e = 0; e < 3; ++e) {
@@ -121,34 +135,47 @@ Mesh convex_hull(const std::vector<Vec3> &P) {
         for (int h = 0; h < 3; ++h)
           if (F[g].v[h] == b && F[g].v[(h + 1) % 3] == a) F[f].nb[e] = g;
     }
+  std::vector<int> next(n, -1);  // outside-list links
+  auto append = [&](HFace &f, int i) {
+    next[i] = -1;
+    if (f.tail < 0) f.head = i;
+    else next[f.tail] = i;
+    f.tail = i;
+  };
   for (int i = 0; i < n; ++i) {
     if (i == i0 || i == i1 || i == i2 || i == i3) continue;
     for (auto &f : F)
       if (dot(f.n, P[i]) - f.off > eps) {
-        f.outside.push_back(i);
+        append(f, i);
         break;
       }
   }
   std::vector<int> work = {0, 1, 2, 3};
-  std::vector<int> visible, queue;
+  std::vector<int> visible, queue, orphans;
   struct Hz {
     int a, b, other, other_e;
   };
   std::vector<Hz> horizon;
-  std::unordered_map<int, int> start_at, end_at;
+  // per vertex: the new face whose horizon edge starts / ends there (set for
+  // every horizon vertex before it is read in the same step)
+  std::vector<int> start_at(n, -1), end_at(n, -1);
   int iter = 0;
   while (!work.empty()) {
     const int fi = work.back();
     work.pop_back();
-    if (!F[fi].alive || F[fi].outside.empty()) continue;
+    if (!F[fi].alive || F[fi].head < 0) continue;
     ++iter;
     int apex = -1;
     double far = -1.0;
-    for (int i : F[fi].outside) {
-      const double d = dot(F[fi].n, P[i]) - F[fi].off;
-      if (d > far) far = d, apex = i;
+    {
+      const Vec3 fn = F[fi].n;
+      const double off = F[fi].off;
+      for (int i = F[fi].head; i >= 0; i = next[i]) {
+        const double d = dot(fn, P[i]) - off;
+        if (d > far) far = d, apex = i;
+      }
     }
-    const Vec3 &A = P[apex];
+    const Vec3 A = P[apex];
     visible.clear();
     horizon.clear();
     queue.assign(1, fi);
@@ -174,53 +201,66 @@ Mesh convex_hull(const std::vector<Vec3> &P) {
         while (F[g].nb[ge] != f) ++ge;
         horizon.push_back({F[f].v[e], F[f].v[(e + 1) % 3], g, ge});
       }
-    std::vector<int> orphans;
+    orphans.clear();
     for (int f : visible) {
       F[f].alive = false;
-      for (int i : F[f].outside)
+      for (int i = F[f].head; i >= 0; i = next[i])
         if (i != apex) orphans.push_back(i);
-      std::vector<int>().swap(F[f].outside);
+      F[f].head = F[f].tail = -1;
     }
-    start_at.clear();
-    end_at.clear();
     const int first = (int)F.size();
     for (const Hz &h : horizon) {
-      HFace nf;
-      nf.v[0] = h.a, nf.v[1] = h.b, nf.v[2] = apex;
-      nf.nb[0] = h.other;
-      set_plane(nf, P);
       const int id = (int)F.size();
+      F.push_back(new_face(h.a, h.b, apex, P));
+      F[id].nb[0] = h.other;
       F[h.other].nb[h.other_e] = id;
       start_at[h.a] = id;
       end_at[h.b] = id;
-      F.push_back(std::move(nf));
     }
     for (int id = first; id < (int)F.size(); ++id) {
       F[id].nb[1] = start_at[F[id].v[1]];  // edge (b, apex): face whose horizon edge starts at b
       F[id].nb[2] = end_at[F[id].v[0]];    // edge (apex, a): face whose horizon edge ends at a
     }
-    for (int i : orphans)
-      for (int g = first; g < (int)F.size(); ++g)
-        if (dot(F[g].n, P[i]) - F[g].off > eps) {
-          F[g].outside.push_back(i);
+    const int last = (int)F.size();
+    for (int i : orphans) {
+      const Vec3 &p = P[i];
+      for (int g = first; g < last; ++g)
+        if (dot(F[g].n, p) - F[g].off > eps) {
+          append(F[g], i);
           break;
         }
-    for (int g = first; g < (int)F.size(); ++g)
-      if (!F[g].outside.empty()) work.push_back(g);
-  }
-  std::unordered_map<int, int> remap;
-  for (const auto &f : F) {
-    if (!f.alive) continue;
-    std::array<int, 3> t;
-    for (int e = 0; e < 3; ++e) {
-      auto it = remap.find(f.v[e]);
-      if (it == remap.end()) {
-        it = remap.emplace(f.v[e], (int)out.vertices.size()).first;
-        out.vertices.push_back(P[f.v[e]]);
-      }
-      t[e] = it->second;
     }
-    out.triangles.push_back(t);
+    for (int g = first; g < last; ++g)
+      if (F[g].head >= 0) work.push_back(g);
+  }
+  for (const auto &f : F)
+    if (f.alive) out.push_back({ids[f.v[0]], ids[f.v[1]], ids[f.v[2]]});
+  return out;
+}
+
+}  // namespace
+
+// Hull of a point set (vertices in order of first use by the faces that
+// survive, faces in creation order).
+Mesh convex_hull(const std::vector<Vec3> &P) {
+  const int n = (int)P.size();
+  std::vector<int> all(n);
+  std::iota(all.begin(), all.end(), 0);
+  const std::vector<std::array<int, 3>> tris = quickhull(P, all);
+  Mesh out;
+  std::vector<int> remap(n, -1);
+  out.triangles.reserve(tris.size());
+  for (const auto &t : tris) {
+    std::array<int, 3> r;
+    for (int e = 0; e < 3; ++e) {
+      int &m = remap[t[e]];
+      if (m < 0) {
+        m = (int)out.vertices.size();
+        out.vertices.push_back(P[t[e]]);
+      }
+      r[e] = m;
+    }
+    out.triangles.push_back(r);
   }
   return out;
 }
@@ -234,31 +274,31 @@ MeshList separate_disjoint(const Mesh &m, bool drop_flat) {
   const int nv = (int)m.vertices.size();
   std::vector<int> parent(nv);
   std::iota(parent.begin(), parent.end(), 0);
-  std::function<int(int)> find = [&](int x) {
+  auto find = [&](int x) {
     while (parent[x] != x) x = parent[x] = parent[parent[x]];
     return x;
   };
   for (const auto &t : m.triangles) {
-    const int a = find(t[0]), b = find(t[1]), c = find(t[2]);
+    const int a = find(t[0]), b = find(t[1]);
     parent[b] = a;
-    parent[find(c)] = find(a);
+    parent[find(t[2])] = find(a);
   }
-  std::unordered_map<int, int> comp_of_root;
-  std::vector<int> comp(nv, -1);
+  // components numbered in order of their lowest used vertex
+  std::vector<int> comp(nv, -1), comp_of_root(nv, -1);
   std::vector<char> used(nv, 0);
   for (const auto &t : m.triangles)
     for (int e = 0; e < 3; ++e) used[t[e]] = 1;
-  int nc = 0;
+  int nc = 0, nused = 0;
   for (int v = 0; v < nv; ++v) {
     if (!used[v]) continue;
-    const int r = find(v);
-    auto it = comp_of_root.find(r);
-    if (it == comp_of_root.end()) it = comp_of_root.emplace(r, nc++).first;
-    comp[v] = it->second;
+    ++nused;
+    int &c = comp_of_root[find(v)];
+    if (c < 0) c = nc++;
+    comp[v] = c;
   }
-  if (nc <= 1 && (int)std::count(used.begin(), used.end(), 1) == nv) {
-    MeshList one = {m};
-    if (drop_flat && std::fabs(mesh_volume(m)) < 1e-6) one.clear();
+  if (nc <= 1 && nused == nv) {
+    MeshList one;
+    if (!(drop_flat && std::fabs(mesh_volume(m)) < 1e-6)) one.push_back(m);
     return one;
   }
   MeshList parts(nc);
@@ -285,6 +325,36 @@ MeshList separate_disjoint(const Mesh &m, bool drop_flat) {
     out.push_back(std::move(p));
   }
   return out;
+}
+
+MeshList separate_disjoint(Mesh &&m, bool drop_flat) {
+  const int nv = (int)m.vertices.size();
+  std::vector<char> used(nv, 0);
+  for (const auto &t : m.triangles)
+    for (int e = 0; e < 3; ++e) used[t[e]] = 1;
+  bool all_used = std::count(used.begin(), used.end(), 1) == nv;
+  if (all_used && nv > 0) {  // one component? then the mesh moves through unchanged
+    std::vector<int> parent(nv);
+    std::iota(parent.begin(), parent.end(), 0);
+    auto find = [&](int x) {
+      while (parent[x] != x) x = parent[x] = parent[parent[x]];
+      return x;
+    };
+    for (const auto &t : m.triangles) {
+      const int a = find(t[0]), b = find(t[1]);
+      parent[b] = a;
+      parent[find(t[2])] = find(a);
+    }
+    const int r = find(0);
+    bool one = true;
+    for (int v = 1; v < nv && one; ++v) one = find(v) == r;
+    if (one) {
+      MeshList out;
+      if (!(drop_flat && std::fabs(mesh_volume(m)) < 1e-6)) out.push_back(std::move(m));
+      return out;
+    }
+  }
+  return separate_disjoint(static_cast<const Mesh &>(m), drop_flat);
 }
 
 }  // namespace vhost

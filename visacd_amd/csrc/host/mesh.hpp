@@ -166,6 +166,7 @@ double surface_area(const Mesh &m);
 double tri_area(const Vec3 &a, const Vec3 &b, const Vec3 &c);
 Mesh convex_hull(const std::vector<Vec3> &pts);
 MeshList separate_disjoint(const Mesh &m, bool drop_flat = true);
+MeshList separate_disjoint(Mesh &&m, bool drop_flat = true);  // moves a single component
 // Splits m by the plane; returns 2 parts (positive side first) or fewer on
 // failure.  map_pos/map_neg receive 1-based vertex maps (0 = not in part).
 MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos,

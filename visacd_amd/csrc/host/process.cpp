@@ -975,7 +975,7 @@ bool split_part(const Mesh &mesh, uint64_t seed, const Plane &plane, MeshList &k
   }
   uint32_t k = 0;
   for (auto &h : halves)
-    for (Mesh &piece : separate_disjoint(h)) {
+    for (Mesh &piece : separate_disjoint(std::move(h))) {
       ++k;
       if (piece.vertices.size() < 10) continue;
       piece.cut_planes = mesh.cut_planes;
@@ -1278,13 +1278,24 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
       ++step;
     }
     if (need < 0) break;
-    // this round: the part the loop needs and the unsplit parts most likely next
+    // this round: the part the loop needs and the unsplit parts most likely
+    // next -- the leaves of the split tree under the loop's list (parts the
+    // loop has not reached yet but already split count through their
+    // pieces), highest key first
     const int left = num_parts - 1 - step;
     const int width = std::max(1, std::min(engine.batch_width, left));
     std::vector<std::pair<double, int>> cand;
-    for (int x : L)
+    std::vector<int> walk(L.rbegin(), L.rend());
+    while (!walk.empty()) {
+      const int x = walk.back();
+      walk.pop_back();
+      if (nodes[x].state == 1) {
+        for (auto k = nodes[x].kids.rbegin(); k != nodes[x].kids.rend(); ++k) walk.push_back(*k);
+        continue;
+      }
       if (x != need && nodes[x].state == 0 && (!by_conc || nodes[x].conc >= concavity_threshold))
         cand.push_back({key_of(x), x});
+    }
     std::stable_sort(cand.begin(), cand.end(),
                      [](const auto &p, const auto &q) { return p.first > q.first; });
     std::vector<int> batch{need};
