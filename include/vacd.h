@@ -66,7 +66,8 @@ int vacd_visibility_counters(vacd_ctx *ctx, int64_t *out, int32_t n);
 #define VACD_OPT_FRONT_LANES 8 /* lanes per pair of the breadth-first walks */
                                /* of short walk queues: 0, 4, 8, 16, 32, 64 */
                                /* (8; 0 = depth-first walks only)           */
-#define VACD_OPT_FRONT_MAX 9   /* longest queue walked breadth-first (131072)*/
+#define VACD_OPT_FRONT_MAX 9   /* longest queue walked breadth-first, longer */
+                               /* ones depth first (2^40: every queue)      */
 
 /* Accumulated device time of the main kernels since the last reset
  * (VACD_OPT_TIMING): the visibility traversal kernel, the plane-scoring

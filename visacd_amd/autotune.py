@@ -6,7 +6,8 @@ keeps the fastest.  The knobs are the context options of the engine:
 * ``lanes_per_pair`` (VACD_OPT_LANES_PER_PAIR): lanes that share one
   depth-first BVH walk of a pair the fan pass left open (1, 2, 4, 8, 16);
 * ``front_max`` (VACD_OPT_FRONT_MAX): the longest walk queue walked breadth
-  first (8 lanes per pair) instead of depth first (131072; 2^40 = always);
+  first (8 lanes per pair) instead of depth first (131072; 2^40 = always,
+  the library's default);
 * ``score_chunk`` (VACD_OPT_SCORE_CHUNK): edges per scoring block, i.e. the
   edge tile a block streams through its 512 planes (0 = sized for about
   8192 blocks; 64 or 256 otherwise).
@@ -33,7 +34,7 @@ _OPTION = {
     "score_chunk": native.VACD_OPT_SCORE_CHUNK,
 }
 
-DEFAULT = {"lanes_per_pair": 1, "front_max": 131072, "score_chunk": 0}
+DEFAULT = {"lanes_per_pair": 1, "front_max": 1 << 40, "score_chunk": 0}
 
 
 def apply(ctx, cfg: dict) -> None:

@@ -78,6 +78,7 @@ PYBIND11_MODULE(_visacd, m) {
       .def_readwrite("mode", &vhost::EngineOptions::mode)
       .def_readwrite("batch_width", &vhost::EngineOptions::batch_width)
       .def_readwrite("eager_visibility", &vhost::EngineOptions::eager_visibility)
+      .def_readwrite("front_max", &vhost::EngineOptions::front_max)
       .def_readwrite("proposal", &vhost::EngineOptions::proposal)
       .def_readwrite("proposal_weights", &vhost::EngineOptions::proposal_weights)
       .def_readwrite("proposal_top", &vhost::EngineOptions::proposal_top)

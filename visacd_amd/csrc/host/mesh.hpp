@@ -104,6 +104,9 @@ struct EngineOptions {
   // together, up to batch_width per round
   std::string mode = "plain";
   int batch_width = 64;
+  // walk queues of up to this many pairs are walked breadth first, longer
+  // ones depth first (VACD_OPT_FRONT_MAX); -1 keeps the library's default
+  int64_t front_max = -1;
   // batched: compute the new parts' visibility as soon as they are made,
   // overlapped with their concavity on the host (true), or only when a part
   // is about to be split (false: no visibility for parts never split)

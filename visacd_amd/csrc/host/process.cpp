@@ -74,6 +74,12 @@ struct Engine {
   explicit Engine(int dev) : device(dev) {
     if (vacd_ctx_create(device, &ctx) != VACD_OK)
       throw std::runtime_error(std::string("visacd: ") + vacd_last_error(nullptr));
+    if (engine.front_max >= 0 &&
+        vacd_ctx_set_option(ctx, VACD_OPT_FRONT_MAX, engine.front_max) != VACD_OK) {
+      const std::string msg = vacd_last_error(ctx);
+      vacd_ctx_destroy(ctx);
+      throw std::runtime_error("visacd: " + msg);
+    }
   }
   ~Engine() {
     vacd_mlp_destroy(mlp);
@@ -1229,8 +1235,24 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
   mesh.intersecting_edges.clear();
   mesh.points.clear();
   maybe_remesh(D, mesh);
-  if (config.use_flat_surfaces) D.flat = flat_surface_planes(mesh, config.flat_surface_min_area);
-  log_line("  " + std::to_string(D.flat.size()) + " flat surface(s)");
+  // the flat-surface planes (needed from the first split on) on a thread of
+  // their own, beside the input parts' visibility and concavity
+  std::exception_ptr flat_err;
+  std::thread flat_th;
+  if (config.use_flat_surfaces)
+    flat_th = std::thread([&] {
+      try {
+        D.flat = flat_surface_planes(mesh, config.flat_surface_min_area);
+      } catch (...) {
+        flat_err = std::current_exception();
+      }
+    });
+  struct Joiner {
+    std::thread &t;
+    ~Joiner() {
+      if (t.joinable()) t.join();
+    }
+  } flat_join{flat_th};
   const bool by_edges = config.score_mode == "edge";
   const bool by_conc = config.score_mode == "concavity";
   std::vector<Node> nodes;
@@ -1248,6 +1270,9 @@ ProcessResult process_batched(Driver &D, Mesh mesh, double concavity_threshold, 
   }
   log_line("  " + std::to_string(L.size()) + " connected part(s)");
   visibility_and_concavity(D, nodes, L, false, by_conc ? L : std::vector<int>{}, &mesh, 0.03);
+  if (flat_th.joinable()) flat_th.join();
+  if (flat_err) std::rethrow_exception(flat_err);
+  log_line("  " + std::to_string(D.flat.size()) + " flat surface(s)");
   auto key_of = [&](int x) {
     if (by_edges) return edge_score(nodes[x].mesh);
     if (by_conc) return nodes[x].conc;

@@ -965,13 +965,19 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
 // meeting an undecided test goes to the deferred list.  EXACT true: the
 // deferred list, every test decided (exact predicates where the filter
 // cannot decide); with LV = 6 (a wave per pair, for latency) the first
-// kExactSplit entries, with LV = 0 (a lane per pair, for throughput) the
-// rest.  Only the EXACT instances carry the exact code's register cost.
+// kExactSplit entries, with LV = 0 (a lane per pair) the rest.  Only the
+// EXACT instances carry the exact code's register cost.
 __device__ __forceinline__ uint64_t group_mask(int lane, int G) {
   return G == 64 ? ~0ull : (((1ull << G) - 1ull) << (lane & ~(G - 1)));
 }
 
-constexpr int64_t kExactSplit = 2048;
+// Deferred pairs are few (tens on one part, ~10^4 in a batched round of
+// cut parts) and their walks long: a lane per pair leaves most SIMDs idle
+// (C5 batched rounds: 9.5 ms per redo), a wave per pair fills the chip.  So
+// the wave-per-pair kernels take up to kExactSplit entries, looping over
+// them with a grid of kExactGrid blocks.
+constexpr int64_t kExactSplit = int64_t(1) << 20;
+constexpr unsigned kExactGrid = 512;
 // Deferred-list entries the breadth-first exact walk settled carry this bit
 // (the depth-first exact kernels skip them; the list still names every pair
 // redone exactly).
@@ -2215,7 +2221,7 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
   // set here where the launches are chosen, tells the lane-per-pair kernel
   // which entries the wave-per-pair kernel took.
   a.exact_front = L.front != 0;
-  const unsigned wgrid = (unsigned)(kExactSplit / kWaves);
+  const unsigned wgrid = kExactGrid;
   const unsigned xgrid = (unsigned)std::min<int64_t>((nwords * 64 + kBlock - 1) / kBlock, 1024);
   if (cap_bits) {  // the exact cap rule before the exact walks
     if (batch) k_cap_redo<true><<<xgrid, 256, 0, st>>>(a);
