@@ -73,6 +73,61 @@ def test_max_distance_batch_matches_bvh_form_and_brute_force(ctx):
         ctx.max_distance_batch([(pts, V, T + 1000)])  # vertex id out of range
 
 
+def _dist_to_mesh(P, V, T):
+    """float64 distance of every point of P to the triangle mesh (V, T)."""
+    V = V.astype(np.float64)
+    A, B, C = V[T[:, 0]], V[T[:, 1]], V[T[:, 2]]
+    out = np.empty(P.shape[0])
+    for i0 in range(0, P.shape[0], 256):
+        Q = P[i0:i0 + 256].astype(np.float64)[:, None, :]
+        n = np.cross(B - A, C - A)
+        n = n / np.linalg.norm(n, axis=1, keepdims=True)
+        h = np.einsum("ptk,tk->pt", Q - A, n)
+        X = Q - h[..., None] * n
+        ins = np.ones(h.shape, bool)
+        for U, W in ((A, B), (B, C), (C, A)):
+            ins &= np.einsum("ptk,tk->pt", np.cross(np.broadcast_to(W - U, X.shape), X - U), n) >= 0
+        d = np.where(ins, np.abs(h), np.inf)
+        for U, W in ((A, B), (B, C), (C, A)):
+            e = W - U
+            t = np.clip(np.einsum("ptk,tk->pt", Q - U, e) / np.einsum("tk,tk->t", e, e), 0, 1)
+            d = np.minimum(d, np.linalg.norm(Q - (U + t[..., None] * e), axis=-1))
+        out[i0:i0 + 256] = d.min(1)
+    return out
+
+
+def test_max_distance_batch_part_against_hull(ctx):
+    """The concavity queries: surface samples of a dented mesh against its
+    convex hull and hull samples against the mesh, where most samples lie
+    near the other surface and the seed's lower bound stops them early; the
+    maximum equals the float64 brute force."""
+    from scipy.spatial import ConvexHull
+
+    rng = np.random.default_rng(12)
+    queries, refs = [], []
+    for seed in (3, 4):
+        V, T = wl.perturbed_icosphere(3, 0.45, seed)
+        H = ConvexHull(V.astype(np.float64))
+        hv = V[H.vertices]
+        remap = np.full(V.shape[0], -1)
+        remap[H.vertices] = np.arange(H.vertices.size)
+        hT = remap[H.simplices].astype(np.uint32)
+
+        def samples(VV, TT, m):
+            t = rng.integers(0, TT.shape[0], m)
+            r1, r2 = np.sqrt(rng.random(m))[:, None], rng.random(m)[:, None]
+            a, b, c = VV[TT[t, 0]], VV[TT[t, 1]], VV[TT[t, 2]]
+            return (a * (1 - r1) + b * (r1 * (1 - r2)) + c * (r1 * r2)).astype(np.float32)
+
+        sa, sb = samples(V, T, 3000), samples(hv, hT, 3000)
+        queries += [(sa, hv, hT), (sb, V, T)]
+        refs += [_dist_to_mesh(sa, hv, hT).max(), _dist_to_mesh(sb, V, T).max()]
+    print("part/hull distance queries", len(queries), flush=True)
+    got = ctx.max_distance_batch(queries)
+    np.testing.assert_allclose(got, np.array(refs), rtol=1e-5, atol=1e-6)
+    assert (np.array(refs) > 1e-3).all()
+
+
 def test_plane_extents_batch_equal_host_float64(ctx):
     """Extents along unit normals equal numpy's float64 ((n0 p0 + n1 p1) + n2 p2)
     bit for bit, for queries of several sizes (more than one LDS tile of

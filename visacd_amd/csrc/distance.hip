@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
+#include <thread>
 
 namespace vacd {
 namespace {
@@ -149,14 +151,25 @@ void launch_max_distance(vacd_ctx *ctx, const float *d_pts, int64_t m, const vac
 //     mesh), staged in LDS;
 //   k_dist_chunks: a bounding sphere per chunk of kDistChunk triangles (a
 //     mesh lists neighbouring triangles near each other);
+//   k_dist_near: each sample against every triangle of its nearest chunk
+//     (by sphere), which holds its closest triangle or one close to it: the
+//     bounds drop near the true distances before the chunk-pruned pass;
+//   k_dist_seed: per query, the exact distance of the sample with the
+//     largest bound, a lower bound h of the query's maximum (a query's
+//     result is the maximum over its samples of their distance to the mesh:
+//     a sample whose distance cannot exceed h does not change it);
 //   k_dist_batch: a block takes 256 samples and one chunk; lanes whose bound
-//     the chunk's sphere cannot beat skip it (the block skips it when all
-//     do); the rest stage its triangles 256 at a time in LDS with their own
-//     spheres and measure exactly those that can come closer than the best
-//     so far (relative slack well above float32 rounding).
+//     is at most h, or that the chunk's sphere cannot improve, skip it (the
+//     block skips it when all do); the rest stage its triangles 256 at a
+//     time in LDS with their own spheres and measure exactly those that can
+//     come closer than the best so far (relative slack well above float32
+//     rounding), until the best falls to h.
 // Per-sample minima (squared, as float bits) meet through atomicMin; a last
-// kernel takes each query's maximum.  The minimum over all triangles equals
-// that of the BVH walk, which prunes the same way.
+// kernel takes each query's maximum.  A sample that ends above h holds its
+// exact minimum over all triangles (equal to that of the BVH walk, which
+// prunes the same way); one that stops at or below h holds a value between
+// its minimum and h, and the seed sample holds h itself: so the maximum is
+// exactly the maximum of the exact minima.
 namespace vacd {
 namespace {
 
@@ -257,25 +270,120 @@ __global__ __launch_bounds__(256) void k_dist_chunks(const DistQ *__restrict__ q
   if (threadIdx.x == 0) sph[c] = make_float4(cx, cy, cz, sqrtf(red[0][0]));
 }
 
+// dmin[s] = min(dmin[s], squared distance to every triangle of the chunk
+// whose sphere comes nearest to the sample); one lane per sample.
+__global__ __launch_bounds__(256) void k_dist_near(const DistQ *__restrict__ qs, int nq,
+                                                   const float *__restrict__ pts,
+                                                   const float *__restrict__ V,
+                                                   const uint32_t *__restrict__ T,
+                                                   const float4 *__restrict__ csph,
+                                                   unsigned *__restrict__ dmin) {
+  const DistQ q = qs[find_query(qs, nq, blockIdx.x, 2)];
+  const int64_t s = (blockIdx.x - q.sblk0) * 256 + threadIdx.x;
+  if (s >= q.m || q.nT == 0) return;
+  const float p[3] = {pts[3 * (q.pt0 + s)], pts[3 * (q.pt0 + s) + 1], pts[3 * (q.pt0 + s) + 2]};
+  int64_t near = 0;
+  float nd = INFINITY;
+  for (int64_t c = 0; c < q.tchunks; ++c) {
+    const float4 b = csph[q.ch0 + c];
+    const float dx = p[0] - b.x, dy = p[1] - b.y, dz = p[2] - b.z;
+    const float d = sqrtf(dx * dx + dy * dy + dz * dz) - b.w;
+    if (d < nd) nd = d, near = c;
+  }
+  float best = __uint_as_float(dmin[q.pt0 + s]);
+  const int64_t t0 = near * kDistChunk;
+  const int64_t t1 = q.nT < t0 + kDistChunk ? q.nT : t0 + kDistChunk;
+  for (int64_t t = t0; t < t1; ++t) {
+    const uint32_t *tr = T + 3 * (q.t0 + t);
+    const float *A = V + 3 * (q.v0 + tr[0]), *B = V + 3 * (q.v0 + tr[1]), *C = V + 3 * (q.v0 + tr[2]);
+    best = fminf(best, tri_d2(p, make_float4(A[0], A[1], A[2], 0.f), make_float4(B[0], B[1], B[2], 0.f),
+                              make_float4(C[0], C[1], C[2], 0.f)));
+  }
+  dmin[q.pt0 + s] = __float_as_uint(best);
+}
+
+// Per query (one block): s* = the sample with the largest bound (lowest
+// index among equals), h = the minimum of its bound and its squared
+// distance to every triangle of the query's mesh (the value k_dist_batch
+// would leave), written to dmin[s*] and hlb[q].  Queries without
+// samples or triangles get h = 0.
+__global__ __launch_bounds__(256) void k_dist_seed(const DistQ *__restrict__ qs,
+                                                   const float *__restrict__ pts,
+                                                   const float *__restrict__ V,
+                                                   const uint32_t *__restrict__ T,
+                                                   unsigned *__restrict__ dmin,
+                                                   unsigned *__restrict__ hlb) {
+  __shared__ unsigned rv[256];
+  __shared__ int64_t ri[256];
+  const DistQ q = qs[blockIdx.x];
+  if (q.m == 0 || q.nT == 0) {
+    if (threadIdx.x == 0) hlb[blockIdx.x] = 0u;
+    return;
+  }
+  unsigned bv = 0u;
+  int64_t bi = -1;
+  for (int64_t s = threadIdx.x; s < q.m; s += 256) {
+    const unsigned v = dmin[q.pt0 + s];
+    if (bi < 0 || v > bv) bv = v, bi = s;
+  }
+  rv[threadIdx.x] = bv, ri[threadIdx.x] = bi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const unsigned v = rv[threadIdx.x + w];
+      const int64_t i = ri[threadIdx.x + w];
+      if (i >= 0 && (ri[threadIdx.x] < 0 || v > rv[threadIdx.x] ||
+                     (v == rv[threadIdx.x] && i < ri[threadIdx.x])))
+        rv[threadIdx.x] = v, ri[threadIdx.x] = i;
+    }
+    __syncthreads();
+  }
+  const int64_t s = ri[0];
+  const unsigned bound = rv[0];
+  __syncthreads();
+  const float p[3] = {pts[3 * (q.pt0 + s)], pts[3 * (q.pt0 + s) + 1], pts[3 * (q.pt0 + s) + 2]};
+  float best = __uint_as_float(bound);  // the minimum includes the bound, as in k_dist_batch
+  for (int64_t t = threadIdx.x; t < q.nT; t += 256) {
+    const uint32_t *tr = T + 3 * (q.t0 + t);
+    const float *A = V + 3 * (q.v0 + tr[0]), *B = V + 3 * (q.v0 + tr[1]), *C = V + 3 * (q.v0 + tr[2]);
+    best = fminf(best, tri_d2(p, make_float4(A[0], A[1], A[2], 0.f), make_float4(B[0], B[1], B[2], 0.f),
+                              make_float4(C[0], C[1], C[2], 0.f)));
+  }
+  rv[threadIdx.x] = __float_as_uint(best);  // non-negative floats order as their bits
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) rv[threadIdx.x] = min(rv[threadIdx.x], rv[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    dmin[q.pt0 + s] = rv[0];
+    hlb[blockIdx.x] = rv[0];
+  }
+}
+
 __global__ __launch_bounds__(256) void k_dist_batch(const DistQ *__restrict__ qs, int nq,
                                                     const float *__restrict__ pts,
                                                     const float *__restrict__ V,
                                                     const uint32_t *__restrict__ T,
                                                     const float4 *__restrict__ csph,
+                                                    const unsigned *__restrict__ hlb,
                                                     unsigned *__restrict__ dmin) {
   __shared__ float4 sa[256], sb[256], sc[256];
   __shared__ float sr[256];
   const int64_t blk = blockIdx.x;
-  const DistQ q = qs[find_query(qs, nq, blk, 0)];  // block-uniform
+  const int qi = find_query(qs, nq, blk, 0);
+  const DistQ q = qs[qi];  // block-uniform
+  const float h = __uint_as_float(hlb[qi]);
   const int64_t local = blk - q.blk0;
   const int64_t st = local / q.tchunks, tc = local % q.tchunks;
   const int64_t s = st * 256 + threadIdx.x;
-  const bool valid = s < q.m;
+  bool valid = s < q.m;
   float p[3] = {0.f, 0.f, 0.f};
   float best = 0.f;
   if (valid) {
     p[0] = pts[3 * (q.pt0 + s)], p[1] = pts[3 * (q.pt0 + s) + 1], p[2] = pts[3 * (q.pt0 + s) + 2];
     best = __uint_as_float(dmin[q.pt0 + s]);
+    valid = best > h;  // at or below the query's lower bound: cannot change its maximum
   }
   float reach = sqrtf(best);
   {  // the whole chunk first: skipped when no lane can improve on its bound
@@ -288,7 +396,8 @@ __global__ __launch_bounds__(256) void k_dist_batch(const DistQ *__restrict__ qs
   const int64_t t_begin = tc * kDistChunk;
   const int64_t t_end = q.nT < t_begin + kDistChunk ? q.nT : t_begin + kDistChunk;
   for (int64_t t0 = t_begin; t0 < t_end; t0 += 256) {
-    __syncthreads();
+    // every lane down to the lower bound: the rest of the chunk changes nothing
+    if (!__syncthreads_or(valid && best > h)) break;
     const int64_t t = t0 + threadIdx.x;
     if (t < t_end) {
       const uint32_t *tr = T + 3 * (q.t0 + t);
@@ -308,7 +417,7 @@ __global__ __launch_bounds__(256) void k_dist_batch(const DistQ *__restrict__ qs
     __syncthreads();
     const int cnt = (int)(t_end - t0 < 256 ? t_end - t0 : 256);
     if (!valid) continue;
-    for (int k = 0; k < cnt; ++k) {
+    for (int k = 0; k < cnt && best > h; ++k) {
       const float4 a = sa[k], b = sb[k], c = sc[k];
       const float dx = p[0] - a.w, dy = p[1] - b.w, dz = p[2] - c.w;
       const float lim = (reach + sr[k]) * 1.0001f + 1e-30f;
@@ -368,11 +477,43 @@ void sort_samples(const float *src, int64_t m, float *dst, std::vector<uint32_t>
     for (int k = 0; k < 3; ++k) dst[3 * (size_t)at + k] = src[3 * i + k];
   }
 }
+
+// Copies nT triangles (vertex ids) to dst in the order of the cell of their
+// centroid in a 32^3 grid over the mesh's box (Morton order of the cells,
+// counting sort), so that a chunk of kDistChunk consecutive triangles is
+// compact and its bounding sphere small.  The distance to the mesh does not
+// depend on the triangles' order.
+void sort_tris(const float *V, int64_t nV, const uint32_t *T, int64_t nT, uint32_t *dst,
+               std::vector<uint32_t> &cell) {
+  float lo[3] = {V[0], V[1], V[2]}, hi[3] = {V[0], V[1], V[2]};
+  for (int64_t i = 1; i < nV; ++i)
+    for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], V[3 * i + k]), hi[k] = std::max(hi[k], V[3 * i + k]);
+  float inv[3];
+  for (int k = 0; k < 3; ++k) inv[k] = hi[k] > lo[k] ? 32.0f / (hi[k] - lo[k]) : 0.0f;
+  cell.resize((size_t)nT);
+  std::vector<uint32_t> start(32769, 0);
+  for (int64_t t = 0; t < nT; ++t) {
+    uint32_t key = 0;
+    for (int k = 0; k < 3; ++k) {
+      const float c = (V[3 * T[3 * t] + k] + V[3 * T[3 * t + 1] + k] + V[3 * T[3 * t + 2] + k]) *
+                      (1.0f / 3.0f);
+      int x = (int)((c - lo[k]) * inv[k]);
+      x = x < 0 ? 0 : (x > 31 ? 31 : x);
+      for (int b = 0; b < 5; ++b) key |= (uint32_t)((x >> b) & 1) << (3 * b + k);
+    }
+    cell[(size_t)t] = key;
+    ++start[key + 1];
+  }
+  for (int c = 0; c < 32768; ++c) start[c + 1] += start[c];
+  for (int64_t t = 0; t < nT; ++t) {
+    const uint32_t at = start[cell[(size_t)t]]++;
+    for (int k = 0; k < 3; ++k) dst[3 * (size_t)at + k] = T[3 * t + k];
+  }
+}
 }  // namespace
 
 void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, float *out) {
   hipStream_t st = ctx->stream;
-  std::vector<uint32_t> cell;
   std::vector<DistQ> qs((size_t)nq);
   int64_t np = 0, nv = 0, nt = 0, nch = 0, nb = 0, nsb = 0;
   for (int i = 0; i < nq; ++i) {
@@ -401,22 +542,41 @@ void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, flo
   std::memcpy(h, qs.data(), sizeof(DistQ) * (size_t)nq);
   char *hp = h + b_q, *hv = hp + b_p, *ht = hv + b_v;
   for (int i = 0; i < nq; ++i) {
-    const DistQ &d = qs[(size_t)i];
-    if (d.m) sort_samples(q[i].pts, d.m, reinterpret_cast<float *>(hp + 12 * d.pt0), cell);
-    if (q[i].nV) std::memcpy(hv + 12 * d.v0, q[i].V, 12 * (size_t)q[i].nV);
-    if (d.nT) {
-      const uint32_t *src = q[i].T;
-      for (int64_t t = 0; t < 3 * d.nT; ++t)
-        VACD_REQUIRE(src[t] < (uint64_t)q[i].nV, VACD_ERR_INVALID,
-                     "triangle index out of range in a distance query");
-      std::memcpy(ht + 12 * d.t0, src, 12 * (size_t)d.nT);
-    }
+    const uint32_t *src = q[i].T;
+    for (int64_t t = 0; t < 3 * q[i].nT; ++t)
+      VACD_REQUIRE(src[t] < (uint64_t)q[i].nV, VACD_ERR_INVALID,
+                   "triangle index out of range in a distance query");
   }
+  // the queries' samples and triangles sorted into the staging region on
+  // host threads (the queries are independent; largest first)
+  std::vector<int> order((size_t)nq);
+  for (int i = 0; i < nq; ++i) order[(size_t)i] = i;
+  std::sort(order.begin(), order.end(), [&](int x, int y) {
+    return q[x].m + q[x].nT > q[y].m + q[y].nT || (q[x].m + q[x].nT == q[y].m + q[y].nT && x < y);
+  });
+  std::atomic<int> next{0};
+  auto work = [&] {
+    std::vector<uint32_t> cell;
+    for (int k; (k = next.fetch_add(1)) < nq;) {
+      const int i = order[(size_t)k];
+      const DistQ &d = qs[(size_t)i];
+      if (d.m) sort_samples(q[i].pts, d.m, reinterpret_cast<float *>(hp + 12 * d.pt0), cell);
+      if (q[i].nV) std::memcpy(hv + 12 * d.v0, q[i].V, 12 * (size_t)q[i].nV);
+      if (d.nT)
+        sort_tris(q[i].V, q[i].nV, q[i].T, d.nT, reinterpret_cast<uint32_t *>(ht + 12 * d.t0), cell);
+    }
+  };
+  const int nth = std::min<int>(nq, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
   DevBuf &b = ctx->buf[SLOT_DIST];
   const size_t b_min = ((4 * (size_t)np + 255) & ~size_t(255));
   const size_t b_out = ((4 * (size_t)nq + 255) & ~size_t(255));
   const size_t b_ch = 16 * (size_t)std::max<int64_t>(nch, 1);
-  b.ensure(bytes + b_min + b_out + b_ch);
+  const size_t b_h = ((4 * (size_t)nq + 255) & ~size_t(255));
+  b.ensure(bytes + b_min + b_out + b_ch + b_h);
   char *d = b.as<char>();
   pinned_upload(ctx, PINNED_DIST, d, bytes);
   const DistQ *dq = reinterpret_cast<const DistQ *>(d);
@@ -426,9 +586,12 @@ void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, flo
   unsigned *dmin = reinterpret_cast<unsigned *>(d + bytes);
   float *dout = reinterpret_cast<float *>(d + bytes + b_min);
   float4 *csph = reinterpret_cast<float4 *>(d + bytes + b_min + b_out);
+  unsigned *hlb = reinterpret_cast<unsigned *>(d + bytes + b_min + b_out + b_ch);
   if (nsb) k_dist_bound<<<(unsigned)nsb, 256, 0, st>>>(dq, nq, dp, dv, dt, dmin);
   if (nch) k_dist_chunks<<<(unsigned)nch, 256, 0, st>>>(dq, nq, nch, dv, dt, csph);
-  if (nb) k_dist_batch<<<(unsigned)nb, 256, 0, st>>>(dq, nq, dp, dv, dt, csph, dmin);
+  if (nsb && nch) k_dist_near<<<(unsigned)nsb, 256, 0, st>>>(dq, nq, dp, dv, dt, csph, dmin);
+  if (nq) k_dist_seed<<<(unsigned)nq, 256, 0, st>>>(dq, dp, dv, dt, dmin, hlb);
+  if (nb) k_dist_batch<<<(unsigned)nb, 256, 0, st>>>(dq, nq, dp, dv, dt, csph, hlb, dmin);
   if (nq) k_dist_max<<<(unsigned)nq, 256, 0, st>>>(dq, dmin, dout);
   VACD_HIP(hipGetLastError());
   VACD_HIP(hipMemcpyAsync(out, dout, 4 * (size_t)nq, hipMemcpyDeviceToHost, st));
