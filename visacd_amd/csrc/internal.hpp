@@ -123,6 +123,7 @@ enum VacdSlot {
   SLOT_EXTENT,
   SLOT_GRID,
   SLOT_FANREC,
+  SLOT_SCANTMP,
   SLOT_COUNT_MAX
 };
 

@@ -148,48 +148,96 @@ __global__ __launch_bounds__(kTile) void k_hist(const uint32_t *__restrict__ key
   hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of m entries with one block of 1024 threads, 4096
-// consecutive entries per step (4 per thread): thread sums, a wave scan by
-// shuffles, the 16 wave totals scanned in LDS, and a running carry.
-__global__ __launch_bounds__(1024) void k_scan_u32(uint32_t *__restrict__ data, int64_t m) {
-  __shared__ uint32_t wsum[16];
-  __shared__ uint32_t carry_s;
+// Exclusive scan of 4096 consecutive entries data[base ..] (those below m)
+// by a block of 1024 threads, 4 per thread: thread sums, a wave scan by
+// shuffles, the 16 wave totals scanned in LDS; every entry gets carry plus
+// the sum before it.  Returns the chunk's total in every thread.
+__device__ __forceinline__ uint32_t scan_chunk(uint32_t *__restrict__ data, int64_t base,
+                                               int64_t m, uint32_t carry, uint32_t *wsum) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) carry_s = 0u;
-  __syncthreads();
-  for (int64_t base = 0; base < m; base += 4096) {
-    const int64_t i0 = base + 4 * (int64_t)threadIdx.x;
-    uint32_t v[4];
+  const int64_t i0 = base + 4 * (int64_t)threadIdx.x;
+  uint32_t v[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = i0 + k < m ? data[i0 + k] : 0u;
-    const uint32_t tsum = v[0] + v[1] + v[2] + v[3];
-    uint32_t inc = tsum;  // inclusive scan over the wave
+  for (int k = 0; k < 4; ++k) v[k] = i0 + k < m ? data[i0 + k] : 0u;
+  const uint32_t tsum = v[0] + v[1] + v[2] + v[3];
+  uint32_t inc = tsum;  // inclusive scan over the wave
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t y = __shfl_up(inc, off);
-      if (lane >= off) inc += y;
-    }
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t run = 0;
-      for (int w = 0; w < 16; ++w) {
-        const uint32_t x = wsum[w];
-        wsum[w] = run;
-        run += x;
-      }
-    }
-    __syncthreads();
-    uint32_t run = carry_s + wsum[wave] + (inc - tsum);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (i0 + k < m) data[i0 + k] = run;
-      run += v[k];
-    }
-    __syncthreads();
-    if (threadIdx.x == 1023) carry_s = run;  // the last thread ends on the step's total
-    __syncthreads();
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off);
+    if (lane >= off) inc += y;
   }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < 16; ++w) {
+      const uint32_t x = wsum[w];
+      wsum[w] = run;
+      run += x;
+    }
+    wsum[16] = run;
+  }
+  __syncthreads();
+  uint32_t run = carry + wsum[wave] + (inc - tsum);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (i0 + k < m) data[i0 + k] = run;
+    run += v[k];
+  }
+  const uint32_t total = wsum[16];
+  __syncthreads();  // wsum is reused by the next chunk
+  return total;
+}
+
+// Exclusive scan of m entries by one block, chunk after chunk (short arrays).
+__global__ __launch_bounds__(1024) void k_scan_u32(uint32_t *__restrict__ data, int64_t m) {
+  __shared__ uint32_t wsum[17];
+  uint32_t carry = 0u;
+  for (int64_t base = 0; base < m; base += 4096) carry += scan_chunk(data, base, m, carry, wsum);
+}
+
+// Long arrays: the total of every 4096-entry chunk, ...
+__global__ __launch_bounds__(1024) void k_scan_sums(const uint32_t *__restrict__ data, int64_t m,
+                                                    uint32_t *__restrict__ sums) {
+  __shared__ uint32_t wsum[16];
+  const int64_t i0 = (int64_t)blockIdx.x * 4096 + 4 * (int64_t)threadIdx.x;
+  uint32_t t = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) t += i0 + k < m ? data[i0 + k] : 0u;
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int w = 0; w < 16; ++w) s += wsum[w];
+    sums[blockIdx.x] = s;
+  }
+}
+
+// ... their exclusive scan (k_scan_u32), then every chunk scanned with its
+// chunk's offset as the carry.
+__global__ __launch_bounds__(1024) void k_scan_add(uint32_t *__restrict__ data, int64_t m,
+                                                   const uint32_t *__restrict__ sums) {
+  __shared__ uint32_t wsum[17];
+  scan_chunk(data, (int64_t)blockIdx.x * 4096, m, sums[blockIdx.x], wsum);
+}
+
+// Exclusive scan of m entries on the context's stream (three launches for
+// more than 16,384 entries: one block scanning 327,680 entries serially
+// took 190 us).
+void scan_launch(vacd_ctx *ctx, uint32_t *data, int64_t m) {
+  hipStream_t st = ctx->stream;
+  if (m <= 4 * 4096) {
+    k_scan_u32<<<1, 1024, 0, st>>>(data, m);
+    return;
+  }
+  const int64_t nblk = (m + 4095) / 4096;
+  vacd::DevBuf &tb = ctx->buf[SLOT_SCANTMP];
+  tb.ensure(4 * (size_t)nblk + 256);
+  uint32_t *sums = tb.as<uint32_t>();
+  k_scan_sums<<<(unsigned)nblk, 1024, 0, st>>>(data, m, sums);
+  k_scan_u32<<<1, 1024, 0, st>>>(sums, nblk);
+  k_scan_add<<<(unsigned)nblk, 1024, 0, st>>>(data, m, sums);
 }
 
 __global__ __launch_bounds__(kTile) void k_scatter(const uint32_t *__restrict__ kin,
@@ -502,22 +550,36 @@ void fill_boxes(hipStream_t st, float4 *seg, int64_t n, const int32_t *rng, Node
       seg, n, rng, nodes, root);
 }
 
-__global__ __launch_bounds__(256) void k_depth(int64_t n, const int32_t *__restrict__ parent_int,
-                                               const int32_t *__restrict__ parent_leaf,
+// Tree depth (nodes on the longest leaf-to-root path).  The deepest leaf's
+// sibling is a leaf too (were it an internal node, its leaves would lie
+// deeper), so only nodes with two leaf children climb: about a third of the
+// leaves' climbs, and one atomic per block.
+__global__ __launch_bounds__(256) void k_depth(int64_t n, const Node *__restrict__ nodes,
+                                               const int32_t *__restrict__ parent_int,
                                                int32_t *depth) {
-  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int32_t d = 1;
-  if (n > 1) {
-    int32_t x = parent_leaf[j];
-    while (x >= 0 && d < 1000) {
-      ++d;
-      x = parent_int[x];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int32_t d = 0;
+  if (n == 1) {
+    d = i == 0 ? 2 : 0;
+  } else if (i < n - 1) {
+    const float4 f3 = nodes[i].f3;
+    if (__float_as_int(f3.x) < 0 && __float_as_int(f3.y) < 0) {
+      d = 2;  // the leaf and node i
+      int32_t x = parent_int[i];
+      while (x >= 0 && d < 1000) {
+        ++d;
+        x = parent_int[x];
+      }
     }
-  } else {
-    d = 2;
   }
-  atomicMax(depth, d);
+  for (int off = 32; off > 0; off >>= 1) d = max(d, __shfl_down(d, off));
+  __shared__ int32_t wd[4];
+  if ((threadIdx.x & 63) == 0) wd[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    d = max(max(wd[0], wd[1]), max(wd[2], wd[3]));
+    if (d) atomicMax(depth, d);
+  }
 }
 
 // Closedness (mod 2): every undirected edge must belong to an even number
@@ -552,9 +614,18 @@ __global__ __launch_bounds__(256) void k_edge_xor(const uint32_t *__restrict__ T
     h1 ^= __shfl_down(h1, off);
     h2 ^= __shfl_down(h2, off);
   }
-  if ((threadIdx.x & 63) == 0 && (h1 | h2)) {
-    atomicXor(&acc[0], h1);
-    atomicXor(&acc[1], h2);
+  // one atomic pair per block (atomics on one address from every wave of
+  // the grid serialize)
+  __shared__ unsigned long long wh[2][4];
+  if ((threadIdx.x & 63) == 0) wh[0][threadIdx.x >> 6] = h1, wh[1][threadIdx.x >> 6] = h2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    h1 = wh[0][0] ^ wh[0][1] ^ wh[0][2] ^ wh[0][3];
+    h2 = wh[1][0] ^ wh[1][1] ^ wh[1][2] ^ wh[1][3];
+    if (h1 | h2) {
+      atomicXor(&acc[0], h1);
+      atomicXor(&acc[1], h2);
+    }
   }
 }
 
@@ -630,7 +701,13 @@ __global__ __launch_bounds__(256) void k_edge_sum(const float *__restrict__ V,
     acc += (unsigned long long)((double)L * kLenScale);
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(sum, acc);
+  __shared__ unsigned long long wacc[4];  // one atomic per block
+  if ((threadIdx.x & 63) == 0) wacc[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    acc = wacc[0] + wacc[1] + wacc[2] + wacc[3];
+    if (acc) atomicAdd(sum, acc);
+  }
 }
 
 // Slicing of a triangle: the apex is the vertex opposite its shortest edge,
@@ -797,6 +874,10 @@ __global__ __launch_bounds__(256) void k_leaves_refs(const float *__restrict__ V
 }
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+// grid of the whole-mesh reductions (bounds, edge sums, closedness hash):
+// each block ends in a few atomics on a few addresses, which serialize, so
+// fewer, fuller blocks (1,024 blocks took ~100 us on 327,680 triangles)
+constexpr unsigned kReduceGrid = 128;
 
 // The contact tree of out (see vacd_bvh): slices of the long triangles and
 // whole short ones as primitives with their own boxes, then the steps of the
@@ -814,12 +895,12 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
   unsigned long long *sum = cb.as<unsigned long long>();
   uint32_t *offsets = reinterpret_cast<uint32_t *>(cb.as<char>() + 256);
   VACD_HIP(hipMemsetAsync(sum, 0, 8, st));
-  k_edge_sum<<<std::min(grid_for(nT, 256), 1024u), 256, 0, st>>>(d_V, d_T, nT, sum);
+  k_edge_sum<<<std::min(grid_for(nT, 256), kReduceGrid), 256, 0, st>>>(d_V, d_T, nT, sum);
   int max_k = kMaxSlices;
   uint32_t total = 0;
   for (;;) {
     k_slice_count<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, sum, offsets, max_k);
-    k_scan_u32<<<1, 1024, 0, st>>>(offsets, nT + 1);
+    scan_launch(ctx, offsets, nT + 1);
     VACD_HIP(hipGetLastError());
     VACD_HIP(hipMemcpyAsync(&total, offsets + nT, 4, hipMemcpyDeviceToHost, st));
     VACD_HIP(hipStreamSynchronize(st));
@@ -874,12 +955,12 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
   Node *nodes = out->rnodes.as<Node>();
   Tri *tris = out->rtris.as<Tri>();
   k_bounds_init<<<1, 64, 0, st>>>(bounds);
-  k_bounds_boxes<<<std::min(grid_for(R, 256), 1024u), 256, 0, st>>>(blo, bhi, R, bounds);
+  k_bounds_boxes<<<std::min(grid_for(R, 256), kReduceGrid), 256, 0, st>>>(blo, bhi, R, bounds);
   k_morton_boxes<<<grid_for(R, 256), 256, 0, st>>>(blo, bhi, R, bounds, k0, v0, d_grp, ref_tri);
   for (int pass = 0; pass < 4; ++pass) {
     const int shift = 8 * pass;
     k_hist<<<(unsigned)ntiles, kTile, 0, st>>>(k0, R, shift, hist, (int)ntiles);
-    k_scan_u32<<<1, 1024, 0, st>>>(hist, (int64_t)hist_n);
+    scan_launch(ctx, hist, (int64_t)hist_n);
     k_scatter<<<(unsigned)ntiles, kTile, 0, st>>>(k0, v0, R, shift, hist, (int)ntiles, k1, v1);
     std::swap(k0, k1);
     std::swap(v0, v1);
@@ -890,7 +971,7 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
                                                  d_grp ? 27 : 0);
   k_leaves_refs<<<grid_for(R, 256), 256, 0, st>>>(d_V, d_T, ref_tri, ref_meta, v0, R, tris);
   fill_boxes(st, seg, R, rng, nodes, root, tris, bounds, blo, bhi, v0);
-  k_depth<<<grid_for(R, 256), 256, 0, st>>>(R, pint, pleaf, depth);
+  k_depth<<<grid_for(R - 1, 256), 256, 0, st>>>(R, nodes, pint, depth);
   k_small_first<<<grid_for(R - 1, 256), 256, 0, st>>>(nodes, R - 1, larger0);
   VACD_HIP(hipGetLastError());
   int32_t h_depth = 0;
@@ -911,7 +992,7 @@ void build_contact_tree(vacd_ctx *ctx, const float *d_V, const uint32_t *d_T, in
 
 void scan_u32(vacd_ctx *ctx, uint32_t *d_data, int64_t m) {
   if (m <= 0) return;
-  k_scan_u32<<<1, 1024, 0, ctx->stream>>>(d_data, m);
+  scan_launch(ctx, d_data, m);
   VACD_HIP(hipGetLastError());
 }
 
@@ -963,13 +1044,13 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
 
   timed_begin(ctx, TIMED_BVH);
   k_bounds_init<<<1, 64, 0, st>>>(bounds);
-  const unsigned gb = (unsigned)std::min<int64_t>(grid_for(nT, 256), 1024);
+  const unsigned gb = std::min(grid_for(nT, 256), kReduceGrid);
   k_bounds<<<gb, 256, 0, st>>>(d_V, d_T, nT, bounds);
   k_morton<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, nT, bounds, k0, v0);
   for (int pass = 0; pass < 4; ++pass) {
     const int shift = 8 * pass;
     k_hist<<<(unsigned)ntiles, kTile, 0, st>>>(k0, nT, shift, hist, (int)ntiles);
-    k_scan_u32<<<1, 1024, 0, st>>>(hist, (int64_t)hist_n);
+    scan_launch(ctx, hist, (int64_t)hist_n);
     k_scatter<<<(unsigned)ntiles, kTile, 0, st>>>(k0, v0, nT, shift, hist, (int)ntiles, k1, v1);
     std::swap(k0, k1);
     std::swap(v0, v1);
@@ -982,7 +1063,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   }
   k_leaves<<<grid_for(nT, 256), 256, 0, st>>>(d_V, d_T, v0, nT, d_grp, tris);
   fill_boxes(st, seg, nT, rng, nodes, root, tris, bounds, nullptr, nullptr, nullptr);
-  k_depth<<<grid_for(nT, 256), 256, 0, st>>>(nT, pint, pleaf, depth);
+  k_depth<<<grid_for(std::max<int64_t>(nT - 1, 1), 256), 256, 0, st>>>(nT, nodes, pint, depth);
   if (nT > 1) k_small_first<<<grid_for(nT - 1, 256), 256, 0, st>>>(nodes, nT - 1, larger0);
   out->nodes4.ensure(sizeof(Node4) * (size_t)n_int);
   k_collapse4<<<grid_for(n_int, 256), 256, 0, st>>>(nodes, n_int, out->nodes4.as<Node4>());
@@ -990,7 +1071,7 @@ void build_lbvh(vacd_ctx *ctx, const float *d_V, int64_t nV, const uint32_t *d_T
   timed_end(ctx, TIMED_BVH);
   VACD_HIP(hipMemsetAsync(odd, 0, 16, st));
   unsigned long long *edge_hash = reinterpret_cast<unsigned long long *>(odd);
-  k_edge_xor<<<std::min(grid_for(nT, 256), 1024u), 256, 0, st>>>(d_T, nT, edge_hash);
+  k_edge_xor<<<std::min(grid_for(nT, 256), kReduceGrid), 256, 0, st>>>(d_T, nT, edge_hash);
   VACD_HIP(hipGetLastError());
   int32_t h_depth = 0;
   unsigned long long h_hash[2] = {1, 1};
