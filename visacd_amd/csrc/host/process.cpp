@@ -599,21 +599,29 @@ void add_visibility_batch(Engine &E, const std::vector<Mesh *> &parts, bool with
     for (auto *b : cages) vacd_bvh_destroy(b);
   };
   try {
-    for (size_t k = 0; k < np; ++k) {
-      Mesh &part = *parts[k];
-      const int64_t n = (int64_t)part.points.size();
-      if (n < 2) continue;
-      P[k].resize(3 * n);
-      M[k] = point_mask(part);
-      bool any_new = false;
-      for (int64_t i = 0; i < n; ++i) {
-        const int v = part.points[i];
-        for (int c = 0; c < 3; ++c) P[k][3 * i + c] = (float)part.vertices[v][c];
-        any_new |= (M[k][i] & 1) != 0;
+    {  // points and masks of every part, on all host threads
+      PhaseTimer t("visibility.masks");
+      std::vector<char> keep(np, 0);
+#pragma omp parallel for schedule(dynamic)
+      for (long k = 0; k < (long)np; ++k) {
+        Mesh &part = *parts[k];
+        const int64_t n = (int64_t)part.points.size();
+        if (n < 2) continue;
+        P[k].resize(3 * n);
+        M[k] = point_mask(part);
+        bool any_new = false;
+        for (int64_t i = 0; i < n; ++i) {
+          const int v = part.points[i];
+          for (int c = 0; c < 3; ++c) P[k][3 * i + c] = (float)part.vertices[v][c];
+          any_new |= (M[k][i] & 1) != 0;
+        }
+        keep[k] = !(with_mask && !any_new);
       }
-      if (with_mask && !any_new) continue;
-      idx.push_back(k);
-      last_profile.count("visibility.points", (long)(n));
+      for (size_t k = 0; k < np; ++k)
+        if (keep[k]) {
+          idx.push_back(k);
+          last_profile.count("visibility.points", (long)parts[k]->points.size());
+        }
     }
     {  // the BVHs, from up to four host threads with a context each
       PhaseTimer t("visibility.bvh");
@@ -667,7 +675,9 @@ void add_visibility_batch(Engine &E, const std::vector<Mesh *> &parts, bool with
         if (engine.verbose && st.exact_calls > 100000) log_exact_hotspot(E, parts, idx, vp, st);
       }
       last_profile.count("visibility.edges", (long)(count));
-      for (size_t j = 0; j < idx.size(); ++j) {
+      PhaseTimer append_timer("visibility.append");
+#pragma omp parallel for schedule(dynamic)
+      for (long j = 0; j < (long)idx.size(); ++j) {
         Mesh &part = *parts[idx[j]];
         part.intersecting_edges.reserve(part.intersecting_edges.size() + (size_t)(off[j + 1] - off[j]));
         for (int64_t e = off[j]; e < off[j + 1]; ++e)

@@ -2036,7 +2036,9 @@ void launch_main(bool q, const Launch &L, const VisArgs &a) {
     k_scan_blocks<<<1, 1024, 0, L.st>>>(L.qscan, L.nscan, nullptr, a.queue_len);
     k_queue_emit<<<(unsigned)L.nscan, 256, 0, L.st>>>(a, L.qscan);
     launch_front<N, B>(L, a);  // a short queue: breadth-first walks
-    launch_split<S, N, B>(L, a);  // a long one: depth-first walks
+    // a long one: depth-first walks (not launched when no queue can exceed
+    // the breadth-first bound)
+    if (L.front == 0 || a.front_max < a.nwords * 64) launch_split<S, N, B>(L, a);
   } else {  // the rare path (open parts, cages): one instance, cap rule compiled in
     k_visibility<S, C, N, B, false, true><<<L.grid, kBlock, 0, L.st>>>(a);
   }
