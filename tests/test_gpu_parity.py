@@ -197,9 +197,16 @@ def test_visibility_walk_forms_agree(ctx, name):
             ctx.set_option(native.VACD_OPT_FRONT_LANES, fl)
             g = ctx.visibility(V, ctx.bvh(V, T))
             np.testing.assert_array_equal(g, o, err_msg=f"breadth-first lanes {fl}")
+        # breadth-first lanes on, but every queue longer than the bound:
+        # the depth-first walk takes it
+        ctx.set_option(native.VACD_OPT_FRONT_LANES, 8)
+        ctx.set_option(native.VACD_OPT_FRONT_MAX, 0)
+        g = ctx.visibility(V, ctx.bvh(V, T))
+        np.testing.assert_array_equal(g, o, err_msg="front_max 0")
     finally:
         ctx.set_option(native.VACD_OPT_LANES_PER_PAIR, 1)
         ctx.set_option(native.VACD_OPT_FRONT_LANES, 8)
+        ctx.set_option(native.VACD_OPT_FRONT_MAX, 1 << 40)
 
 
 def test_visibility_open_mesh(ctx):

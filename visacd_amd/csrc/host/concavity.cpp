@@ -172,9 +172,9 @@ std::vector<Vec3> surface_samples(const Mesh &m, int resolution, std::mt19937 &r
 }
 
 double hausdorff_boundary(const Mesh &a, const Mesh &b, int resolution, uint32_t seed) {
-  std::mt19937 rng(seed);
+  std::mt19937 rng(seed), rng_b(second_sample_seed(seed));
   const Samples sa = sample_surface(a, resolution, rng);
-  const Samples sb = sample_surface(b, resolution, rng);
+  const Samples sb = sample_surface(b, resolution, rng_b);
   if (sa.p.empty() || sb.p.empty()) return 1e300;
   KdTree ta, tb;
   ta.build(sa.p);

@@ -178,6 +178,11 @@ double concavity(const Mesh &part, const Mesh &hull, int resolution, uint32_t se
 // max(1000, resolution * area) area-weighted surface points plus one point in
 // every triangle left unsampled (lib/src/core.cpp:144-232)
 std::vector<Vec3> surface_samples(const Mesh &m, int resolution, std::mt19937 &rng);
+// A sampled distance between two meshes draws the first mesh's samples from
+// std::mt19937(seed) and the second's (the hull's) from
+// std::mt19937(second_sample_seed(seed)), so that the two can be drawn
+// concurrently (the part's while its hull is being built).
+inline uint32_t second_sample_seed(uint32_t seed) { return seed ^ 0x6a09e667u; }
 double hausdorff_boundary(const Mesh &a, const Mesh &b, int resolution, uint32_t seed);
 std::vector<Plane> flat_surface_planes(const Mesh &m, double min_area);
 

@@ -418,9 +418,9 @@ std::vector<double> max_distances(const Engine &E, const std::vector<DistQuery> 
 // Symmetric sampled surface distance with exact point-to-mesh distances
 // measured on the GPU (same samples as the host form).
 double gpu_hausdorff(const Engine &E, const Mesh &a, const Mesh &b, int resolution, uint32_t seed) {
-  std::mt19937 rng(seed);
+  std::mt19937 rng(seed), rng_b(second_sample_seed(seed));
   const std::vector<Vec3> sa = surface_samples(a, resolution, rng);
-  const std::vector<Vec3> sb = surface_samples(b, resolution, rng);
+  const std::vector<Vec3> sb = surface_samples(b, resolution, rng_b);
   if (sa.empty() || sb.empty()) return 1e300;
   std::vector<DistQuery> qs(2);
   qs[0].set(sa, b);
@@ -443,9 +443,9 @@ double final_concavity(const Engine &E, const MeshList &parts, const MeshList &h
       host[i] = hausdorff_boundary(parts[i], hulls[i], 10000, engine.sample_seed);
       continue;
     }
-    std::mt19937 rng(engine.sample_seed);
+    std::mt19937 rng(engine.sample_seed), rng_b(second_sample_seed(engine.sample_seed));
     const std::vector<Vec3> sa = surface_samples(parts[i], 10000, rng);
-    const std::vector<Vec3> sb = surface_samples(hulls[i], 10000, rng);
+    const std::vector<Vec3> sb = surface_samples(hulls[i], 10000, rng_b);
     if (sa.empty() || sb.empty()) {
       host[i] = 1e300;
       continue;
@@ -711,15 +711,41 @@ ConcavityPrep concavity_prep(const std::vector<const Mesh *> &ms) {
 #pragma omp parallel for schedule(dynamic)
   for (long j = 0; j < (long)J; ++j) {
     const Mesh &part = *ms[j];
-    const Mesh hull = convex_hull(part.vertices);
+    // a large part's samples are drawn on a thread of their own while its
+    // hull is built (both take tens of ms on a 100k-vertex part)
+    std::vector<Vec3> sa;
+    const bool early = engine.gpu_distance && part.vertices.size() > 20000;
+    std::thread sampler;
+    std::exception_ptr sample_err;
+    if (early)
+      sampler = std::thread([&] {
+        try {
+          std::mt19937 rng(engine.sample_seed);
+          sa = surface_samples(part, engine.hausdorff_resolution, rng);
+        } catch (...) {
+          sample_err = std::current_exception();
+        }
+      });
+    Mesh hull;
+    try {
+      hull = convex_hull(part.vertices);
+    } catch (...) {
+      if (sampler.joinable()) sampler.join();
+      throw;
+    }
+    if (sampler.joinable()) sampler.join();
+    if (sample_err) std::rethrow_exception(sample_err);
     if (!engine.gpu_distance || hull.triangles.empty()) {
       cp.host_value[j] = concavity(part, hull, engine.hausdorff_resolution, engine.sample_seed);
       continue;
     }
     cp.rv[j] = std::cbrt(3.0 * std::fabs(mesh_volume(part) - mesh_volume(hull)) / (4.0 * M_PI));
-    std::mt19937 rng(engine.sample_seed);
-    const std::vector<Vec3> sa = surface_samples(part, engine.hausdorff_resolution, rng);
-    const std::vector<Vec3> sb = surface_samples(hull, engine.hausdorff_resolution, rng);
+    if (!early) {
+      std::mt19937 rng(engine.sample_seed);
+      sa = surface_samples(part, engine.hausdorff_resolution, rng);
+    }
+    std::mt19937 rng_b(second_sample_seed(engine.sample_seed));
+    const std::vector<Vec3> sb = surface_samples(hull, engine.hausdorff_resolution, rng_b);
     if (sa.empty() || sb.empty()) {
       cp.gpu[j] = 2;
       continue;
