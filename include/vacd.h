@@ -51,8 +51,8 @@ typedef struct vacd_vis_stats {
  * triangle tests; with VACD_OPT_COUNT_WORK the fan pass also counts the
  * pairs it leaves to the walks with 10 no usable fan at p, 11 p's fan test
  * undecided, 12 no usable fan at q, 13 q's undecided, 14 both ends leaving
- * outwards.  Waits for the
- * stream. */
+ * outwards; 15 queued pairs walked over their clipped segment only
+ * (VACD_OPT_CLIP_ENDS).  Waits for the stream. */
 int vacd_visibility_counters(vacd_ctx *ctx, int64_t *out, int32_t n);
 
 /* Context options (vacd_ctx_set_option). */
@@ -68,6 +68,11 @@ int vacd_visibility_counters(vacd_ctx *ctx, int64_t *out, int32_t n);
                                /* (8; 0 = depth-first walks only)           */
 #define VACD_OPT_FRONT_MAX 9   /* longest queue walked breadth-first, longer */
                                /* ones depth first (2^40: every queue)      */
+#define VACD_OPT_CLIP_ENDS 10  /* breadth-first walks of pairs whose fans   */
+                               /* show both ends leaving outwards test only */
+                               /* the segment outside both endpoints'       */
+                               /* clearance balls, in parts of at least     */
+                               /* this many segments (2^25; 0 = never)      */
 
 /* Accumulated device time of the main kernels since the last reset
  * (VACD_OPT_TIMING): the visibility traversal kernel, the plane-scoring

@@ -179,16 +179,27 @@ def test_batch_matches_oracle_per_part(ctx):
     d_pairs = torch.zeros((cap, 2), dtype=torch.int32, device=dev)
     d_count = torch.zeros(1, dtype=torch.int64, device=dev)
     d_off = torch.zeros(len(parts) + 1, dtype=torch.int64, device=dev)
-    ctx.visibility_batch_d(parts, d_pairs, cap, d_count, d_off)
-    torch.cuda.synchronize()
-    pairs = d_pairs.cpu().numpy().astype(np.uint32)
-    off = d_off.cpu().numpy()
-    assert off[-1] == int(d_count.item())
-    assert (np.diff(off) >= 0).all()
-    for t, (V, T, m) in enumerate(cases):
-        mesh_V = V if V.shape[0] > 1 else Vm
-        o, _ = oracle.visibility(V, mesh_V, T, mask=m)
-        np.testing.assert_array_equal(pairs[off[t]:off[t + 1]], o, err_msg=f"part {t}")
+    ref = [oracle.visibility(V, V if V.shape[0] > 1 else Vm, T, mask=m)[0] for V, T, m in cases]
+    from visacd_amd import native
+
+    # default (these parts are too small for clipped walks), then every
+    # part's walks clipped at both ends where the fans allow it
+    for clip in (1 << 25, 1):
+        ctx.set_option(native.VACD_OPT_CLIP_ENDS, clip)
+        try:
+            ctx.visibility_batch_d(parts, d_pairs, cap, d_count, d_off)
+            torch.cuda.synchronize()
+        finally:
+            ctx.set_option(native.VACD_OPT_CLIP_ENDS, 1 << 25)
+        if clip == 1:
+            assert ctx.visibility_counters()["clipped_walks"] > 0
+        pairs = d_pairs.cpu().numpy().astype(np.uint32)
+        off = d_off.cpu().numpy()
+        assert off[-1] == int(d_count.item())
+        assert (np.diff(off) >= 0).all()
+        for t in range(len(cases)):
+            np.testing.assert_array_equal(pairs[off[t]:off[t + 1]], ref[t],
+                                          err_msg=f"part {t}, clip {clip}")
 
 
 def test_score_batch_matches_oracle_per_part(ctx):

@@ -366,6 +366,62 @@ def test_visibility_clipped_part_contact_tree(ctx):
     _print("cap rule", len(gm), "->", len(g2), st2)
 
 
+@pytest.mark.parametrize("name", ["torus", "ico4", "poly", "cap"])
+def test_clipped_walks_match_oracle(ctx, name):
+    """Walks of pairs whose fans show both ends leaving outwards, clipped to
+    the part of the segment outside both endpoints' clearance balls
+    (forced on every part here; by default only parts of 2^25 segments and
+    more), give the oracle's edges: smooth meshes, an exactly degenerate
+    polycube, and a clipped part whose cap is a planar group (masked, with
+    the cap rule)."""
+    from visacd_amd import native
+
+    mask, kw = None, {}
+    if name == "torus":
+        V, T = wl.torus(nu=24, nv=12, seed=4)
+        pts = V
+    elif name == "ico4":
+        V, T = wl.perturbed_icosphere(4, 0.4, 9)
+        pts = V[np.random.default_rng(1).choice(V.shape[0], 900, replace=False)]
+    elif name == "poly":
+        V, T = meshes.polycube([(x, y, z) for x in range(3) for y in range(3) for z in range(2)
+                                if (x, y) != (1, 1)])
+        pts = V
+    else:
+        n = np.array([0.3, 0.5, -0.2])
+        n /= np.linalg.norm(n)
+        V, T, cap = _clipped_part(4, 21, [*n, 0.05])
+        rng = np.random.default_rng(4)
+        idx = np.unique(np.concatenate([np.where(cap)[0],
+                                        rng.choice(np.where(~cap)[0], 400, replace=False)]))
+        pts = V[idx]
+        grp = np.where(cap[T].all(1), 0, -1).astype(np.int8)
+        plane = np.array([[*n, 0.05]])
+        mask = (cap[idx] * 3).astype(np.uint8)
+        kw = dict(tri_group=grp, planes=plane)
+    _print("clipped walks", name, pts.shape, T.shape)
+    o, _ = oracle.visibility(pts, V, T, mask=mask,
+                             groups=(kw["tri_group"], kw["planes"]) if kw else None)
+    b = ctx.bvh(V, T, **kw)
+    ctx.set_option(native.VACD_OPT_CLIP_ENDS, 1)
+    try:
+        g = ctx.visibility(pts, b, new_mask=mask)
+        c = ctx.visibility_counters()
+    finally:
+        ctx.set_option(native.VACD_OPT_CLIP_ENDS, 1 << 25)
+    _print("  clipped", c["clipped_walks"], "of", c["walked"], "walked")
+    np.testing.assert_array_equal(g, o)
+    if name != "poly":
+        assert c["clipped_walks"] > 0
+    ctx.set_option(native.VACD_OPT_CLIP_ENDS, 0)
+    try:
+        g0 = ctx.visibility(pts, b, new_mask=mask)
+        assert ctx.visibility_counters()["clipped_walks"] == 0
+    finally:
+        ctx.set_option(native.VACD_OPT_CLIP_ENDS, 1 << 25)
+    np.testing.assert_array_equal(g0, o)
+
+
 def test_visibility_icosphere_c2_sampled(ctx):
     """Full-size C2 run; sampled pairs checked against the oracle."""
     W = wl.icosphere_c2()

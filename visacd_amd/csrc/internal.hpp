@@ -124,6 +124,7 @@ enum VacdSlot {
   SLOT_GRID,
   SLOT_FANREC,
   SLOT_SCANTMP,
+  SLOT_CLEAR,
   SLOT_COUNT_MAX
 };
 
@@ -145,6 +146,13 @@ struct vacd_ctx {
   // decomposition that measured faster at every queue length seen
   int front_lanes = 8;
   int64_t front_max = int64_t(1) << 40;
+  // breadth-first walks of pairs that leave both ends outwards test only the
+  // part of the segment outside both endpoints' clearance balls, in parts of
+  // at least clip_min segments (0: never).  Clipping pays where the walk
+  // queue is long enough to keep the chip busy (the walks' total work
+  // counts); on smaller parts the queue's time is the latency of its
+  // longest walks and the clearance query would only add to the pass.
+  int64_t clip_min = int64_t(1) << 25;
   // recorded (start, stop) event pairs per timed kernel kind
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
   // pinned host staging for descriptor tables copied to the device, one

@@ -136,6 +136,8 @@ int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value) {
     ctx->front_max = value;
   else if (option == VACD_OPT_FAN_PAIRS)
     ctx->fan_pairs = value != 0;
+  else if (option == VACD_OPT_CLIP_ENDS && value >= 0)
+    ctx->clip_min = value;
   else if (option == VACD_OPT_SCORE_CHUNK && value >= 0 && value <= (int64_t(1) << 30))
     ctx->score_chunk = value;
   else if (option == VACD_OPT_LANES_PER_PAIR &&

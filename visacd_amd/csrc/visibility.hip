@@ -616,6 +616,9 @@ struct Part {
   const int *fan_idx;  // per point: kFanMax leaf-ordered triangle ids
   const FanRec *fan_rec;  // per point: kFanMax prepared fan triangles
   const FanCone *fan_cone;  // per point: the fan's cones
+  const float *clear;  // per point: a lower bound of its distance to every triangle
+                       // that does not have it as a vertex (NULL: not computed)
+  float clear_r;       // radius of the clearance search (clearances are at most this)
   const Node *snodes;
   const Node4 *snodes4;
   const Tri *stris;
@@ -650,6 +653,9 @@ struct VisArgs {
   unsigned long long *queue_len;
   uint64_t *qbits;            // per ballot word: its pairs left to walk (compacted into
                               // queue by a scan, in k order: no atomics in the fan pass)
+  uint64_t *cbits;            // per ballot word: queued pairs whose fans show both ends
+                              // leaving outwards without touching either fan (their
+                              // queue entries carry kClip)
   int64_t *dlist;             // global pair ids whose walks met an undecided test
   unsigned long long *dlist_len;
   int64_t front_max;          // walk queues up to this long go to k_walk_front
@@ -813,6 +819,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     const Part P = part_of_word<BATCH>(a, w);  // wave-uniform
     const int64_t k = (w - P.word_begin) * 64 + lane;
     bool edge = false, defer = false, queued = false;
+    bool pout = false, qout = false;  // the end leaves outwards, its fan untouched (certain)
     if (k < P.S) {
       int64_t i, j;
       pair_of(k, P.n, i, j);
@@ -865,7 +872,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             } else if (r == 0 && P.far[i] <= 1 && ((xp ^ P.far[i]) & 1)) {
               decided = true;  // starts inside
             }
-            if (COUNT) pstate = (r == 0 && P.far[i] <= 1) ? 1 : 3;
+            pout = r == 0 && P.far[i] <= 1;
+            if (COUNT) pstate = pout ? 1 : 3;
           }
           // The same from q's end: a pair that enters the solid at q (the
           // segment seen from q towards p starts inside) is blocked or
@@ -889,7 +897,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
             } else if (r == 0 && P.far[j] <= 1 && ((xq ^ P.far[j]) & 1)) {
               decided = true;  // enters the solid at q
             }
-            if (COUNT) qstate = (r == 0 && P.far[j] <= 1) ? 1 : 3;
+            qout = r == 0 && P.far[j] <= 1;
+            if (COUNT) qstate = qout ? 1 : 3;
           }
         }
         if (COUNT && !decided) {
@@ -912,7 +921,10 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     append_lanes(defer, w * 64 + lane, a.dlist, a.dlist_len);
     if (QUEUE) {  // this wave's undecided pairs, for the walk queue
       const uint64_t qb = __ballot(queued);
-      if (lane == 0) a.qbits[w] = qb;
+      // queued pairs that leave both ends outwards (the fan tests of both
+      // ends decided, no contact): their walks may skip the clearance balls
+      const uint64_t cb = __ballot(queued && pout && qout && P.clear != nullptr);
+      if (lane == 0) a.qbits[w] = qb, a.cbits[w] = cb;
     }
   }
   // per-block totals, one atomic per block and counter (atomics on one
@@ -982,6 +994,10 @@ constexpr unsigned kExactGrid = 512;
 // (the depth-first exact kernels skip them; the list still names every pair
 // redone exactly).
 constexpr int64_t kSettled = int64_t(1) << 62;
+// Walk-queue entry flag: both ends of the pair leave outwards and neither
+// fan touches the segment, so the pair is an edge iff no other triangle
+// touches it, and those lie outside both endpoints' clearance balls.
+constexpr int64_t kClip = int64_t(1) << 61;
 
 template <int STACK, bool COUNT, bool BATCH, int LV, bool EXACT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 4))) void k_walk_split(VisArgs a) {
@@ -1005,7 +1021,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
   const int64_t e_hi = (EXACT && LV == 6) ? (len < kExactSplit ? len : kExactSplit) : len;
   for (int64_t e = e_lo + (int64_t)blockIdx.x * kGroups + slot; e < e_hi;
        e += (int64_t)gridDim.x * kGroups) {
-    const int64_t kg = list[e];
+    // (a queue entry's kClip flag is not used here: these walks cover the
+    // whole segment)
+    const int64_t kg = EXACT ? list[e] : (list[e] & ~kClip);
     if (EXACT && (kg & kSettled)) continue;  // settled by the breadth-first exact walk
     const int64_t w = kg >> 6;
     const int b = (int)(kg & 63);
@@ -1115,14 +1133,16 @@ __device__ __forceinline__ void wave_lds_sync() {
 // in xpar.  Per-lane results; returns (group-uniform) 0 done, 1 contact,
 // 3 more visits than the tree has nodes, 5 frontier overflow.  EXACT: tests
 // the float64 filter cannot decide are decided exactly (counted in nex)
-// instead of setting unsure.
+// instead of setting unsure.  [u0, u1]: the segment's parameter range whose
+// boxes are tested (RAY false; a triangle met is still tested whole).
 template <int G, int KF, bool RAY, bool COUNT, bool EXACT = false>
 __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
                                           const Tri *__restrict__ tris,
                                           const float4 *__restrict__ grp, int cap, const Seg &s,
                                           int *cur, int *nxt, int g, uint64_t gm, uint64_t below,
                                           int &xpar, int &ninc, bool &local_ok, bool &unsure,
-                                          unsigned &nv, unsigned &nt, unsigned &nex) {
+                                          unsigned &nv, unsigned &nt, unsigned &nex,
+                                          float u0 = 0.0f, float u1 = 1.0f) {
   if (g == 0) cur[0] = 0;  // the root
   wave_lds_sync();
   int n = 1;
@@ -1148,11 +1168,13 @@ __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
         } else {
           float t0, t1;
           h0 = g0 ? (group_span(s, grp, g0, t0, t1) &&
-                     seg_box_span(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], t0, t1))
-                  : seg_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
+                     seg_box_span(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], fmaxf(t0, u0),
+                                  fminf(t1, u1)))
+                  : seg_box_span(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], u0, u1);
           h1 = g1 ? (group_span(s, grp, g1, t0, t1) &&
-                     seg_box_span(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11], t0, t1))
-                  : seg_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
+                     seg_box_span(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11], fmaxf(t0, u0),
+                                  fminf(t1, u1)))
+                  : seg_box_span(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11], u0, u1);
         }
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
@@ -1197,7 +1219,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
   const int g = lane & (G - 1);
   const uint64_t gm = group_mask(lane, G);
   const uint64_t below = gm & ((1ull << lane) - 1ull);  // lanes of the group before this one
-  unsigned n_blocked = 0, n_bad = 0, nv = 0, nt = 0, n_exact = 0;
+  unsigned n_blocked = 0, n_bad = 0, nv = 0, nt = 0, n_exact = 0, n_clip = 0;
   int64_t *list = EXACT ? a.dlist : a.queue;
   int64_t len = (int64_t)(EXACT ? *a.dlist_len : *a.queue_len);
   if (EXACT && len > kExactSplit) len = kExactSplit;
@@ -1205,7 +1227,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
   int *fa = front_all[slot][0], *fb = front_all[slot][1];
   for (int64_t e = (int64_t)blockIdx.x * kGroups + slot; e < len;
        e += (int64_t)gridDim.x * kGroups) {
-    const int64_t kg = list[e];
+    const int64_t raw = list[e];
+    const bool clip = !EXACT && (raw & kClip);
+    const int64_t kg = EXACT ? raw : (raw & ~kClip);
     if (EXACT && (kg & kSettled)) continue;  // settled by the exact cap rule
     const int64_t w = kg >> 6;
     const int b = (int)(kg & 63);
@@ -1217,11 +1241,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
 #pragma unroll
     for (int c = 0; c < 3; ++c) s.p[c] = P.pts[3 * i + c], s.q[c] = P.pts[3 * j + c];
     make_seg(s);
+    // A pair whose fans showed both ends leaving outwards and neither fan
+    // touching the segment is an edge iff no other triangle touches it; such
+    // a triangle keeps clear[i] from p and clear[j] from q, so only the
+    // parameters [clear_i / len, 1 - clear_j / len] need box tests (0.1 %
+    // and 1e-6 of slack cover the float32 rounding of these bounds).  When
+    // the two balls cover the segment, no triangle can touch it.
+    float u0 = 0.0f, u1 = 1.0f;
+    if (clip) {
+      const float dx = s.q[0] - s.p[0], dy = s.q[1] - s.p[1], dz = s.q[2] - s.p[2];
+      const float len = sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+      u0 = fmaf(0.999f, P.clear[i] / len, -1e-6f);
+      u1 = 1.0f - fmaf(0.999f, P.clear[j] / len, -1e-6f);
+      if (g == 0) ++n_clip;
+    }
     int xpar = 0, ninc = 0;
     bool local_ok = true, unsure = false;
-    const int hs = front_walk<G, kFront, false, COUNT, EXACT>(
-        P.rnodes, P.rtris, P.rgroups, P.rcap, s, fa, fb, g, gm, below, xpar, ninc, local_ok,
-        unsure, nv, nt, n_exact);
+    const int hs = (clip && !(u0 < u1))
+                       ? 0
+                       : front_walk<G, kFront, false, COUNT, EXACT>(
+                             P.rnodes, P.rtris, P.rgroups, P.rcap, s, fa, fb, g, gm, below, xpar,
+                             ninc, local_ok, unsure, nv, nt, n_exact, u0, u1);
     uint64_t mu = __ballot(unsure) & gm;
     const uint64_t ml = __ballot(!local_ok) & gm, mx = __ballot(xpar & 1) & gm,
                    mn = __ballot(ninc > 0) & gm;
@@ -1235,6 +1275,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
       // depth-first one)
       if (EXACT) left = true;
       else defer = true;
+    } else if (clip) {
+      edge = true;  // outside at both ends, untouched
     } else if (!ml && P.far && mn && P.far[i] <= 1) {
       // closed mesh, p on it: inside parity = fan crossings ^ far crossings of p
       edge = (((__popcll(mx) & 1) ^ P.far[i]) & 1) == 0;
@@ -1270,6 +1312,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
     n_blocked += __shfl_down(n_blocked, off);
     n_bad += __shfl_down(n_bad, off);
     n_exact += __shfl_down(n_exact, off);
+    n_clip += __shfl_down(n_clip, off);
     nv += __shfl_down(nv, off);
     nt += __shfl_down(nt, off);
   }
@@ -1277,6 +1320,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
     if (n_blocked) atomicAdd(&a.stats[1], (unsigned long long)n_blocked);
     if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
     if (EXACT && n_exact) atomicAdd(&a.stats[9], (unsigned long long)n_exact);
+    if (!EXACT && n_clip) atomicAdd(&a.stats[15], (unsigned long long)n_clip);
     if (COUNT) {
       atomicAdd(&a.stats[5], (unsigned long long)nv);
       atomicAdd(&a.stats[6], (unsigned long long)nt);
@@ -1371,6 +1415,56 @@ __device__ int exact_ray_parity(const Node *__restrict__ nodes, const Tri *__res
       node = stk[(--sp) * 64];
     }
   }
+}
+
+// Squared distance from p to the closed segment ab (float64).
+__device__ __forceinline__ double seg_dist2(const double p[3], const double a[3],
+                                            const double b[3]) {
+  const double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double ap[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+  const double l2 = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+  double t = l2 > 0.0 ? (ap[0] * ab[0] + ap[1] * ab[1] + ap[2] * ab[2]) / l2 : 0.0;
+  t = fmin(fmax(t, 0.0), 1.0);
+  const double d[3] = {ap[0] - t * ab[0], ap[1] - t * ab[1], ap[2] - t * ab[2]};
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+// Squared distance from p to the closed triangle abc (float64): the nearest
+// of its three edges, or its plane when p projects inside it.  Degenerate
+// triangles are their edges.
+__device__ __forceinline__ double tri_dist2(const double p[3], const double a[3],
+                                            const double b[3], const double c[3]) {
+  double d2 = fmin(seg_dist2(p, a, b), fmin(seg_dist2(p, b, c), seg_dist2(p, c, a)));
+  const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                       e1[0] * e2[1] - e1[1] * e2[0]};
+  const double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  if (nn > 0.0) {
+    const double *v[3] = {a, b, c};
+    bool inside = true;
+    for (int k = 0; k < 3 && inside; ++k) {
+      const double *u = v[k], *w = v[(k + 1) % 3];
+      const double e[3] = {w[0] - u[0], w[1] - u[1], w[2] - u[2]};
+      const double r[3] = {p[0] - u[0], p[1] - u[1], p[2] - u[2]};
+      const double x[3] = {e[1] * r[2] - e[2] * r[1], e[2] * r[0] - e[0] * r[2],
+                           e[0] * r[1] - e[1] * r[0]};
+      inside = x[0] * n[0] + x[1] * n[1] + x[2] * n[2] >= 0.0;
+    }
+    if (inside) {
+      const double h = (p[0] - a[0]) * n[0] + (p[1] - a[1]) * n[1] + (p[2] - a[2]) * n[2];
+      d2 = fmin(d2, h * h / nn);
+    }
+  }
+  return d2;
+}
+
+// Minimum of a double over the lanes of a group of G consecutive lanes.
+template <int G>
+__device__ __forceinline__ double group_min(double v) {
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) v = fmin(v, __shfl_xor(v, off));
+  return v;
 }
 
 // The records and cones of point g's fan (cnt triangle ids at out; p the
@@ -1529,7 +1623,8 @@ __device__ __forceinline__ void fan_front_block(const VisArgs &a, int64_t block,
                                                 int *__restrict__ fan_n,
                                                 int *__restrict__ fan_idx,
                                                 FanRec *__restrict__ fan_rec,
-                                                FanCone *__restrict__ fan_cone) {
+                                                FanCone *__restrict__ fan_cone,
+                                                float *__restrict__ clear) {
   constexpr int kGroups = NT / G;
   constexpr int kF = 16 * G;
   int(*front_all)[2][kF] = reinterpret_cast<int(*)[2][kF]>(lds);
@@ -1553,6 +1648,13 @@ __device__ __forceinline__ void fan_front_block(const VisArgs &a, int64_t block,
   if (!P.fan_n) return;
   const int64_t i = pt - P.point_begin;
   const float p[3] = {P.pts[3 * i], P.pts[3 * i + 1], P.pts[3 * i + 2]};
+  // the clearance (P.clear): the same walk also takes boxes within the best
+  // distance so far (at first clear_r) of a triangle not at p, and measures
+  // those triangles (bound as in clear_point)
+  const bool want_clear = P.clear != nullptr;
+  const double pd[3] = {p[0], p[1], p[2]};
+  const double R = (double)P.clear_r;
+  double best = R * R;  // squared, this lane's; the group's minimum prunes
   int *out = fan_idx + (size_t)pt * kFanMax;
   int *fl = lds + kGroups * 2 * kF + slot * kFanMax;  // the group's fan list
   int *cur = front_all[slot][0], *nxt = front_all[slot][1];
@@ -1567,6 +1669,7 @@ __device__ __forceinline__ void fan_front_block(const VisArgs &a, int64_t block,
       fail = true;
       break;
     }
+    const double bound = want_clear ? group_min<G>(best) : 0.0;
     int nn = 0;
     for (int base = 0; base < n; base += G) {
       const int idx = base + g;
@@ -1578,16 +1681,31 @@ __device__ __forceinline__ void fan_front_block(const VisArgs &a, int64_t block,
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
           const float *b = bx + 6 * c;
-          if (!(p[0] >= b[0] && p[0] <= b[3] && p[1] >= b[1] && p[1] <= b[4] && p[2] >= b[2] &&
-                p[2] <= b[5]))
-            continue;
+          const bool in = p[0] >= b[0] && p[0] <= b[3] && p[1] >= b[1] && p[1] <= b[4] &&
+                          p[2] >= b[2] && p[2] <= b[5];
+          bool near = false;  // within the clearance bound (boxes holding p aside)
+          if (want_clear && !in) {
+            double d2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const double o = fmax(fmax((double)b[k] - pd[k], pd[k] - (double)b[3 + k]), 0.0);
+              d2 += o * o;
+            }
+            near = d2 < bound;
+          }
+          if (!in && !near) continue;
           const int ch = c ? c1 : c0;
           bool in_fan = false;
           if (ch < 0) {
             const Tri t = P.stris[~ch];
             const float a3[3] = {t.a.x, t.a.y, t.a.z}, b3[3] = {t.b.x, t.b.y, t.b.z},
                         c3[3] = {t.c.x, t.c.y, t.c.z};
-            in_fan = same3(p, a3) || same3(p, b3) || same3(p, c3);
+            in_fan = in && (same3(p, a3) || same3(p, b3) || same3(p, c3));
+            if (want_clear && !in_fan) {
+              const double A[3] = {a3[0], a3[1], a3[2]}, B[3] = {b3[0], b3[1], b3[2]},
+                           C[3] = {c3[0], c3[1], c3[2]};
+              best = fmin(best, tri_dist2(pd, A, B, C));
+            }
           }
           if (c) f1 = in_fan, w1 = ch >= 0;
           else f0 = in_fan, w0 = ch >= 0;
@@ -1616,6 +1734,11 @@ __device__ __forceinline__ void fan_front_block(const VisArgs &a, int64_t block,
     n = nn;
   }
   wave_lds_sync();  // the group's frontier buffers are free after this
+  if (want_clear) {
+    best = group_min<G>(best);
+    // a failed walk leaves no bound (k_fan_redo writes none either)
+    if (g == 0) clear[pt] = fail ? 0.0f : __double2float_rz(sqrt(best) * (1.0 - 1e-6));
+  }
   if (g != 0) return;
   if (fail || cnt > kFanMax) {
     fan_n[pt] = fail ? -2 : -1;  // -2: k_fan_redo
@@ -1639,6 +1762,110 @@ __device__ __forceinline__ void fan_front_block(const VisArgs &a, int64_t block,
 
 // Points whose breadth-first fan query overflowed its frontier (fan_n -2),
 // queried again depth first, a lane per point.
+// Clearance of point g (one lane): a lower bound of its distance to every
+// triangle that does not have it as a vertex (bitwise-equal coordinates),
+// capped at the part's search radius clear_r: a nearest-triangle walk of the
+// 4-wide tree, nearest child first, that skips boxes no nearer than the best
+// distance so far (at first clear_r), so it visits only the boxes within
+// clear_r of the point.  Distances are float64 from the float32 coordinates;
+// the result is lowered by a relative 1e-6 (far above their rounding) and
+// rounded down to float32.  0 when the walk's stack (KS entries: ids in stk,
+// their box distances in dst, both at a stride of 64) or its visit cap runs
+// out.  clear: the whole batch's buffer, indexed by global point.
+template <bool BATCH, int KS>
+__device__ __forceinline__ void clear_point(const VisArgs &a, int64_t g, int *stk, float *dst,
+                                            float *__restrict__ clear) {
+  Part P = a.one;
+  if (BATCH) {
+    int lo = 0, hi = a.nparts - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.parts[mid].point_begin <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    P = a.parts[lo];
+  }
+  if (!P.clear) return;
+  const int64_t i = g - P.point_begin;
+  const float pf[3] = {P.pts[3 * i], P.pts[3 * i + 1], P.pts[3 * i + 2]};
+  const double p[3] = {pf[0], pf[1], pf[2]};
+  const double R = (double)P.clear_r;
+  double best = R * R;  // squared
+  int node = 0, sp = 0;
+  bool ok = false;
+  for (int it = 0; it <= P.scap; ++it) {
+    const Node4 nd = P.snodes4[node];
+    int next = -1;
+    double nd2 = 0.0;
+    bool full = false;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (lane_of(nd.f[0], c) == kEmpty) continue;
+      double d2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double lo = lane_of(nd.f[k], c), hi = lane_of(nd.f[3 + k], c);
+        const double o = fmax(fmax(lo - p[k], p[k] - hi), 0.0);
+        d2 += o * o;
+      }
+      if (!(d2 < best)) continue;
+      const int id = __float_as_int(lane_of(nd.f[6], c));
+      if (id < 0) {
+        const Tri t = P.stris[~id];
+        const float a3[3] = {t.a.x, t.a.y, t.a.z}, b3[3] = {t.b.x, t.b.y, t.b.z},
+                    c3[3] = {t.c.x, t.c.y, t.c.z};
+        if (same3(pf, a3) || same3(pf, b3) || same3(pf, c3)) continue;
+        const double A[3] = {a3[0], a3[1], a3[2]}, B[3] = {b3[0], b3[1], b3[2]},
+                     C[3] = {c3[0], c3[1], c3[2]};
+        best = fmin(best, tri_dist2(p, A, B, C));
+      } else if (next < 0) {
+        next = id, nd2 = d2;
+      } else {
+        // descend into the nearer one, push the other
+        int push = id;
+        double pd = d2;
+        if (d2 < nd2) push = next, pd = nd2, next = id, nd2 = d2;
+        if (sp >= KS) {
+          full = true;
+          break;
+        }
+        stk[sp * 64] = push;
+        dst[sp * 64] = (float)pd * 0.999f;  // lowered: skipping it on a pop stays safe
+        ++sp;
+      }
+    }
+    if (full) break;  // no bound for this point
+    if (next >= 0 && nd2 < best) {
+      node = next;
+      continue;
+    }
+    node = -1;
+    while (sp > 0) {
+      --sp;
+      if ((double)dst[sp * 64] < best) {
+        node = stk[sp * 64];
+        break;
+      }
+    }
+    if (node < 0) {
+      ok = true;
+      break;
+    }
+  }
+  clear[g] = ok ? __double2float_rz(sqrt(best) * (1.0 - 1e-6)) : 0.0f;
+}
+
+
+// The clearances of all points on their own (a lane per point), when the far
+// walks run without k_far_fan.
+template <bool BATCH>
+__global__ __launch_bounds__(64) void k_clear(VisArgs a, float *__restrict__ clear) {
+  __shared__ int stk_all[64][64];
+  __shared__ float dst_all[64][64];
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g < a.npoints) clear_point<BATCH, 64>(a, g, &stk_all[0][threadIdx.x], &dst_all[0][threadIdx.x], clear);
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_fan_redo(VisArgs a, int *__restrict__ fan_n,
                                                  int *__restrict__ fan_idx,
@@ -1805,17 +2032,20 @@ __global__ __launch_bounds__(kBlock) void k_far_front(VisArgs a, uint8_t *__rest
 // independent): blocks below fan_blocks take 8 points' fans (breadth
 // first, 8 lanes each), the others the far walks of 8 (G = 8) or 4
 // (G = 16) points each.  One launch instead of two back to back.
+// One launch for two independent per-point jobs: blocks [0, fan_blocks)
+// the fan queries (with the clearances), the rest the far walks.
 template <bool BATCH, int G>
 __global__ __launch_bounds__(64) void k_far_fan(VisArgs a, uint8_t *__restrict__ far,
                                                 unsigned fan_blocks, int *__restrict__ fan_n,
                                                 int *__restrict__ fan_idx,
                                                 FanRec *__restrict__ fan_rec,
-                                                FanCone *__restrict__ fan_cone) {
+                                                FanCone *__restrict__ fan_cone,
+                                                float *__restrict__ clear) {
   // the frontiers and fan lists of the fan queries, or the far walks' frontiers
   __shared__ int lds[(64 / 8) * (2 * 16 * 8 + kFanMax)];
   static_assert((64 / G) * 2 * 16 * G <= (64 / 8) * 2 * 16 * 8, "far frontiers fit");
   if (blockIdx.x < fan_blocks) {
-    fan_front_block<BATCH, 8, 64>(a, blockIdx.x, lds, fan_n, fan_idx, fan_rec, fan_cone);
+    fan_front_block<BATCH, 8, 64>(a, blockIdx.x, lds, fan_n, fan_idx, fan_rec, fan_cone, clear);
   } else {
     far_front_block<BATCH, G, false, 64>(a, far, blockIdx.x - fan_blocks,
                                          gridDim.x - fan_blocks, lds);
@@ -1890,10 +2120,11 @@ __global__ __launch_bounds__(256) void k_queue_emit(VisArgs a,
                                                     const unsigned long long *__restrict__ block_off) {
   __shared__ unsigned long long sh[256];
   const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
-  uint64_t w4[4];
+  uint64_t w4[4], c4[4];
   unsigned long long c = 0;
   for (int t = 0; t < 4; ++t) {
     w4[t] = base + t < a.nwords ? a.qbits[base + t] : 0ull;
+    c4[t] = base + t < a.nwords ? a.cbits[base + t] : 0ull;
     c += __popcll(w4[t]);
   }
   unsigned long long total;
@@ -1903,7 +2134,7 @@ __global__ __launch_bounds__(256) void k_queue_emit(VisArgs a,
     while (m) {
       const int b = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;
-      a.queue[out++] = (base + t) * 64 + b;
+      a.queue[out++] = ((base + t) * 64 + b) | (((c4[t] >> b) & 1ull) ? kClip : 0);
     }
   }
 }
@@ -2069,6 +2300,8 @@ Part make_part(const float *d_pts, int64_t n, const uint8_t *d_mask, const vacd_
   P.fan_idx = nullptr;
   P.fan_rec = nullptr;
   P.fan_cone = nullptr;
+  P.clear = nullptr;
+  P.clear_r = 0.0f;
   P.snodes = self->nodes.as<Node>();
   P.snodes4 = self->nodes4.as<Node4>();
   P.stris = self->tris.as<Tri>();
@@ -2108,6 +2341,17 @@ int binary_stack(int depth, int64_t leaves, bool small_first) {
   return std::min(depth - 1, lg);
 }
 
+// Search radius of the points' clearances on a mesh: three times its extent
+// over sqrt(triangles), about a mean edge length of a closed surface mesh
+// (the nearest triangle not at a vertex is typically 0.8 of one away).  Any
+// radius gives valid clearances (capped at it); a larger one finds a little
+// more clearance at the cost of a wider search.
+float clear_radius(const vacd_bvh *b) {
+  const float ext = std::max({b->bounds[3] - b->bounds[0], b->bounds[4] - b->bounds[1],
+                              b->bounds[5] - b->bounds[2]});
+  return b->n_tris > 0 ? 3.0f * ext / std::sqrt((float)b->n_tris) : 0.0f;
+}
+
 // Enqueues far parity, the main kernel, the exact redo and the compaction for
 // a prepared VisArgs (a.nwords > 0).
 void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, bool cap_bits,
@@ -2126,6 +2370,16 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
   a.dlist = dlist.as<int64_t>();
   a.dlist_len = a.stats + 8;
   VACD_REQUIRE(depth <= 64, VACD_ERR_LIMIT, "BVH deeper than the traversal stack");
+  // stack entries a walk can hold: stack2 (see binary_stack); the fan
+  // query's 4-wide walk, up to three per visited level (every other binary
+  // level)
+  const int need2 = stack2, need4 = 3 * (depth / 2);
+  // parity mode with fans: queue the pairs the fan pass leaves open
+  const bool queue = any_far && ctx->fan_pairs && !any_cage && need4 <= 64;
+  // the points' clearances (parts with fans), for the walks of queued pairs
+  // that leave both ends outwards
+  const bool want_clear = queue && ctx->clip_min > 0;
+  float *clear = ctx->buf[SLOT_CLEAR].as<float>();
   if (any_far) {
     const unsigned fg = (unsigned)((a.npoints + 63) / 64);
     const unsigned pg = (unsigned)std::min<int64_t>(a.npoints, 16384);  // a wave per point
@@ -2150,15 +2404,17 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
     // frontier of 1024 nodes), then depth first
     const unsigned fpg64 = (unsigned)std::min<int64_t>((a.npoints + kWaves - 1) / kWaves, 4096);
     if (ctx->front_lanes) {
+      // the fan queries also measure the clearances (want_clear: the parts'
+      // clear pointers are set)
       const unsigned grid = fan_blocks + far_blocks;  // > 0: a closed part has points
       if (batch && fl == 16)
-        k_far_fan<true, 16><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
+        k_far_fan<true, 16><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone, clear);
       else if (batch)
-        k_far_fan<true, 8><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
+        k_far_fan<true, 8><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone, clear);
       else if (fl == 16)
-        k_far_fan<false, 16><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
+        k_far_fan<false, 16><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone, clear);
       else
-        k_far_fan<false, 8><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone);
+        k_far_fan<false, 8><<<grid, 64, 0, st>>>(a, farb.as<uint8_t>(), fan_blocks, fan_n, fan_idx, fan_rec, fan_cone, clear);
       if (fan_blocks) {
         if (batch) k_fan_redo<true><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
         else k_fan_redo<false><<<fg, 64, 0, st>>>(a, fan_n, fan_idx, fan_rec, fan_cone);
@@ -2170,10 +2426,14 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
         k_far_front<false, 64, true><<<fpg64, kBlock, 0, st>>>(a, farb.as<uint8_t>());
         k_far_parity<false, true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
       }
-    } else if (batch)
-      k_far_parity<true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
-    else
-      k_far_parity<false><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
+    } else {
+      if (batch) k_far_parity<true><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
+      else k_far_parity<false><<<pg, 64, 0, st>>>(a, farb.as<uint8_t>());
+      if (want_clear) {
+        if (batch) k_clear<true><<<fg, 64, 0, st>>>(a, clear);
+        else k_clear<false><<<fg, 64, 0, st>>>(a, clear);
+      }
+    }
     VACD_HIP(hipGetLastError());
     if (ctx->fan_pairs && !ctx->front_lanes) {
       if (batch)
@@ -2183,10 +2443,6 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
       VACD_HIP(hipGetLastError());
     }
   }
-  // stack entries a walk can hold: stack2 (see binary_stack); the fan
-  // query's 4-wide walk, up to three per visited level (every other binary
-  // level)
-  const int need2 = stack2, need4 = 3 * (depth / 2);
   const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
   Launch L;
   L.grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
@@ -2198,18 +2454,18 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
   // every pair once when all are queued, at most 16 blocks per CU
   L.qgrid = (unsigned)std::min<int64_t>(((nwords * 64) << L.lv) / kBlock + 1, 256 * 16);
   L.st = st;
-  // parity mode with fans: queue the pairs the fan pass leaves open
-  const bool queue = any_far && ctx->fan_pairs && !any_cage && need4 <= 64;
   a.queue_len = a.stats + 7;
   a.queue = nullptr;
   a.qbits = nullptr;
   L.qscan = scan.as<unsigned long long>();  // free until the edge scan
   L.nscan = nscan;
+  a.cbits = nullptr;
   if (queue) {
     vacd::DevBuf &qb = ctx->buf[SLOT_QUEUE];
-    qb.ensure(8 * (size_t)nwords * 65);  // the queue, then the per-word bits
+    qb.ensure(8 * (size_t)nwords * 66);  // the queue, then the two per-word bit arrays
     a.queue = qb.as<int64_t>();
     a.qbits = reinterpret_cast<uint64_t *>(a.queue + nwords * 64);
+    a.cbits = a.qbits + nwords;
   }
   timed_begin(ctx, TIMED_VIS);
   if (need2 <= 32)
@@ -2305,6 +2561,12 @@ void launch_visibility(vacd_ctx *ctx, const float *d_pts, int64_t n, const uint8
       recb.ensure((sizeof(FanRec) * kFanMax + sizeof(FanCone)) * (size_t)n);  // records, cones
       a.one.fan_rec = recb.as<FanRec>();
       a.one.fan_cone = reinterpret_cast<const FanCone *>(a.one.fan_rec + (size_t)n * kFanMax);
+      if (ctx->clip_min > 0 && a.one.S >= ctx->clip_min) {
+        vacd::DevBuf &clb = ctx->buf[SLOT_CLEAR];
+        clb.ensure(sizeof(float) * (size_t)n);
+        a.one.clear = clb.as<float>();
+        a.one.clear_r = clear_radius(self);
+      }
     }
   }
   const int stack2 = std::max({binary_stack(self->depth, self->n_tris, self->small_first),
@@ -2355,10 +2617,12 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
     const bool fan = ctx->fan_pairs && 3 * (depth / 2) <= 64;
     vacd::DevBuf &fanb = ctx->buf[SLOT_FAN];
     vacd::DevBuf &recb = ctx->buf[SLOT_FANREC];
+    vacd::DevBuf &clb = ctx->buf[SLOT_CLEAR];
     if (fan) {
       fanb.ensure(sizeof(int) * (size_t)std::max<int64_t>(points, 1) * (1 + kFanMax));
       recb.ensure((sizeof(FanRec) * kFanMax + sizeof(FanCone)) *
                   (size_t)std::max<int64_t>(points, 1));
+      if (ctx->clip_min > 0) clb.ensure(sizeof(float) * (size_t)std::max<int64_t>(points, 1));
     }
     for (int p = 0; p < nparts; ++p)
       if (!hp[p].ctris && parts[p].self->closed) {
@@ -2370,6 +2634,10 @@ void launch_visibility_batch(vacd_ctx *ctx, const vacd_part *parts, int nparts, 
           hp[p].fan_cone = reinterpret_cast<const FanCone *>(recb.as<FanRec>() +
                                                              (size_t)points * kFanMax) +
                            hp[p].point_begin;
+          if (ctx->clip_min > 0 && hp[p].S >= ctx->clip_min) {
+            hp[p].clear = clb.as<float>() + hp[p].point_begin;
+            hp[p].clear_r = clear_radius(parts[p].self);
+          }
         }
       }
   }

@@ -87,6 +87,7 @@ VACD_OPT_SCORE_CHUNK = 5
 VACD_OPT_LANES_PER_PAIR = 6
 VACD_OPT_FRONT_LANES = 8
 VACD_OPT_FRONT_MAX = 9
+VACD_OPT_CLIP_ENDS = 10
 
 
 _lib = None
@@ -356,13 +357,13 @@ class Context:
     COUNTER_NAMES = ("masked", "blocked", "unused2", "edges", "malformed", "node_visits",
                      "tri_tests", "walked", "exact_calls", "exact_tri_tests", "undecided_no_fan_p",
                      "undecided_fan_p", "undecided_no_fan_q", "undecided_fan_q",
-                     "undecided_both_outwards")
+                     "undecided_both_outwards", "clipped_walks")
 
     def visibility_counters(self) -> dict:
         """Raw work counters of the last visibility call (fan-pass reasons
         only with VACD_OPT_COUNT_WORK)."""
-        out = np.zeros(15, dtype=np.int64)
-        self._check(lib().vacd_visibility_counters(self._h, _p(out), 15))
+        out = np.zeros(16, dtype=np.int64)
+        self._check(lib().vacd_visibility_counters(self._h, _p(out), 16))
         return {k: int(v) for k, v in zip(self.COUNTER_NAMES, out)}
 
     def visibility_stats(self) -> dict:
