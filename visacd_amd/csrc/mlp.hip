@@ -162,115 +162,174 @@ __global__ void k_head_out(const unsigned short *__restrict__ g2, const float *_
 }
 
 // Inference head, fused: out[p] = v3 . bf16(relu(V2 g1[p] + c2)) + c3 with
-// g1[p] = bf16(relu(u[b] + Wp q[p])) built in registers, in the layout of the
-// MFMA A operand, so neither g1 nor g2 goes to memory (the unfused training
-// path writes both).  V2 (bf16, [n][k], rows padded by 16 bytes so the 32
-// lanes reading one k-slice of 32 rows hit distinct banks), Wp, c2 and v3
-// sit in LDS for the whole kernel; one block per CU, eight waves, each wave
-// takes 32 planes at a time: 16 k-steps of v_mfma_f32_32x32x16_bf16 over the
-// eight 32-column tiles of the 256 outputs (operands: lane l holds row or
-// column l % 32 and k = 8 (l / 32) .. +7; result element r of lane l is row
-// 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32).  The epilogue adds c2,
-// applies relu and the bf16 rounding of the unfused path, dots with v3 and
-// sums the 32 lanes of each half-wave in a fixed order (deterministic).
+// g1[p] = bf16(relu(u[b] + Wp q[p])), every product on MFMA and neither g1
+// nor g2 in memory (the unfused training path writes both).  A wave takes 32
+// planes at a time; v_mfma_f32_32x32x16_bf16 operands: lane l holds row (A)
+// or column (B) l % 32 and k = 8 (l / 32) .. +7; result element r of lane l
+// is row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.  The products run
+// transposed (hidden units as rows, planes as columns), so that what one
+// MFMA leaves in a lane -- 16 hidden units of the lane's plane -- is the B
+// operand of the next one for two k-steps, once the hidden units are
+// renumbered; the V2 columns and v3 are stored in that order (head_perm).
+//   g1: eight MFMAs, one per 32 hidden units: A = (Wp_hi | Wp_lo) in lanes
+//       0-31, (Wp_hi | u_hi u_mid u_lo 0) in lanes 32-63, B = (q_hi | q_hi)
+//       and (q_lo | 1 1 1 0): Wp q + u with bf16 splits of q and Wp (only the
+//       lo x lo term, about 2^-16 of the product, is left out) and u exact as
+//       three bf16 terms; then relu and the bf16 rounding of the unfused
+//       path.  u belongs to the part of the tile's first plane; lanes whose
+//       plane lies in the next part (tiles spanning parts) are corrected by
+//       the difference of the two parts' u in float32.
+//   g2: 16 k-steps per 32 outputs with V2 from LDS (rows padded by 16 bytes
+//       against bank conflicts), + c2, relu, bf16.
+//   y:  two MFMAs per 32 outputs with A = (v3_hi, v3_lo) in rows 0 and 1:
+//       y = row 0 + row 1 + c3.  Products of bf16 values are exact in the
+//       float32 accumulators.
+// One block per CU (V2 fills most of its LDS), NW waves.
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 constexpr int kHeadWaves = 8;
 constexpr int kV2Stride = kD + 8;  // bf16 per padded LDS row
 
-template <int RT>  // 32-row tiles per wave and step, sharing each B fragment
-__global__ __launch_bounds__(RT == 1 ? 64 * kHeadWaves : 256) void k_head_fused(
+// Position of hidden unit c in the order of the head's B operands: for c =
+// 32 t + 16 b + 8 j + 4 h + r, the lanes of half h take units 32 t + 16 b +
+// 8 h + [0, 8) of k-step 2 t + b, which hold 4 j + r.
+__host__ __device__ __forceinline__ int head_perm(int c) {
+  return (c & ~31) | (c & 16) | ((c & 4) << 1) | ((c & 8) >> 1) | (c & 3);
+}
+
+__device__ __forceinline__ void split2(float x, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_head_fused(
     const float *__restrict__ u, const float *__restrict__ Wp, const float *__restrict__ q,
     int64_t BP, int P, const unsigned short *__restrict__ V2, const float *__restrict__ c2,
     const float *__restrict__ v3, const float *__restrict__ c3, float *__restrict__ out) {
   __shared__ __attribute__((aligned(16))) unsigned short sV[kD * kV2Stride];
-  __shared__ float4 sWp[kD];
-  __shared__ float sc2[kD], sv3[kD];
+  __shared__ __attribute__((aligned(16))) __bf16 sV3[2 * kD];
+  __shared__ float sc2[kD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nwaves = blockDim.x >> 6;
-  // V2 into LDS, 16 bytes per load
-  for (int c = tid; c < kD * kD / 8; c += blockDim.x) {
-    const int n = c / (kD / 8), k8 = c % (kD / 8);
-    *reinterpret_cast<uint4 *>(&sV[n * kV2Stride + 8 * k8]) =
-        reinterpret_cast<const uint4 *>(V2)[c];
+  // V2 rows into LDS with permuted columns: 8 consecutive columns (aligned)
+  // are two runs of 4 in the permuted order
+  for (int c = tid; c < kD * kD / 8; c += 64 * NW) {
+    const int n = c / (kD / 8), k = 8 * (c % (kD / 8));
+    const uint4 v = reinterpret_cast<const uint4 *>(V2)[c];
+    *reinterpret_cast<uint2 *>(&sV[n * kV2Stride + head_perm(k)]) = make_uint2(v.x, v.y);
+    *reinterpret_cast<uint2 *>(&sV[n * kV2Stride + head_perm(k + 4)]) = make_uint2(v.z, v.w);
   }
-  for (int k = tid; k < kD; k += blockDim.x) {
-    sWp[k] = make_float4(Wp[4 * k], Wp[4 * k + 1], Wp[4 * k + 2], Wp[4 * k + 3]);
+  for (int k = tid; k < kD; k += 64 * NW) {
+    __bf16 hi, lo;
+    split2(v3[k], hi, lo);
+    sV3[head_perm(k)] = hi;
+    sV3[kD + head_perm(k)] = lo;
     sc2[k] = c2[k];
-    sv3[k] = v3[k];
   }
   __syncthreads();
   const float bias3 = c3[0];
   const int half = lane >> 5, col = lane & 31;
-  const int64_t tiles = (BP + 32 * RT - 1) / (32 * RT);
-  for (int64_t t = (int64_t)blockIdx.x * nwaves + wave; t < tiles;
-       t += (int64_t)gridDim.x * nwaves) {
-    float4 qr[RT];
-    const float *ub[RT];
+  // the g1 A operands: Wp splits (both halves), u terms (half 1, per part)
+  bf16x8_t a1[8];
 #pragma unroll
-    for (int s = 0; s < RT; ++s) {
-      const int64_t row = (t * RT + s) * 32 + col;  // this lane's A-operand row
-      const bool valid = row < BP;
-      qr[s] = valid ? make_float4(q[4 * row], q[4 * row + 1], q[4 * row + 2], q[4 * row + 3])
-                    : make_float4(0.f, 0.f, 0.f, 0.f);
-      ub[s] = u + (valid ? row / P : 0) * kD;
+  for (int t = 0; t < 8; ++t) {
+    const int m = 32 * t + col;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      __bf16 hi, lo;
+      split2(Wp[4 * m + c], hi, lo);
+      a1[t][c] = hi;
+      a1[t][4 + c] = half ? (__bf16)0.0f : lo;
     }
-    f32x16 acc[RT][8];
+  }
+  int64_t cur = -1;  // the part whose u the a1 operands hold
+  const int64_t tiles = (BP + 31) / 32;
+  for (int64_t tile = (int64_t)blockIdx.x * NW + wave; tile < tiles;
+       tile += (int64_t)gridDim.x * NW) {
+    const int64_t row = tile * 32 + col;  // this lane's plane
+    const bool valid = row < BP;
+    const int64_t part0 = tile * 32 / P;
+    const int64_t part = valid ? row / P : part0;
+    const bool spans = (std::min<int64_t>(tile * 32 + 31, BP - 1)) / P != part0;
+    if (part0 != cur) {  // wave-uniform
+      cur = part0;
+      if (half) {
 #pragma unroll
-    for (int s = 0; s < RT; ++s)
-#pragma unroll
-      for (int nt = 0; nt < 8; ++nt)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[s][nt][r] = 0.0f;
-    for (int ks = 0; ks < kD / 16; ++ks) {
-      const int k0 = ks * 16 + 8 * half;
-      bf16x8_t a[RT];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const float4 w = sWp[k0 + i];
-#pragma unroll
-        for (int s = 0; s < RT; ++s) {
-          const float v = ub[s][k0 + i] + w.x * qr[s].x + w.y * qr[s].y + w.z * qr[s].z + w.w * qr[s].w;
-          a[s][i] = (__bf16)fmaxf(v, 0.0f);
+        for (int t = 0; t < 8; ++t) {
+          const float x = u[part0 * kD + 32 * t + col];
+          const __bf16 h0 = (__bf16)x;
+          const float r1 = x - (float)h0;
+          const __bf16 h1 = (__bf16)r1;
+          a1[t][4] = h0;
+          a1[t][5] = h1;
+          a1[t][6] = (__bf16)(r1 - (float)h1);
         }
       }
+    }
+    const float4 qv = valid ? *reinterpret_cast<const float4 *>(q + 4 * row)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+    bf16x8_t b1;
+    {
+      const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
 #pragma unroll
-      for (int nt = 0; nt < 8; ++nt) {
-        const bf16x8_t b =
-            *reinterpret_cast<const bf16x8_t *>(&sV[(nt * 32 + col) * kV2Stride + k0]);
-#pragma unroll
-        for (int s = 0; s < RT; ++s)
-          acc[s][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s], b, acc[s][nt], 0, 0, 0);
+      for (int c = 0; c < 4; ++c) {
+        __bf16 hi, lo;
+        split2(qa[c], hi, lo);
+        b1[c] = half ? lo : hi;
+        b1[4 + c] = half ? (__bf16)(c < 3 ? 1.0f : 0.0f) : hi;
       }
     }
+    // g1: the B operands of the main product, k-step 2 t + b in b2[2 t + b]
+    bf16x8_t b2[16];
 #pragma unroll
-    for (int s = 0; s < RT; ++s) {
-      // epilogue: the 16 rows this lane holds, partial dots over its columns
-      float part[16];
+    for (int t = 0; t < 8; ++t) {
+      f32x16 d;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) part[r] = 0.0f;
-#pragma unroll
-      for (int nt = 0; nt < 8; ++nt) {
-        const int n = nt * 32 + col;
-        const float bn = sc2[n], vn = sv3[n];
+      for (int r = 0; r < 16; ++r) d[r] = 0.0f;
+      d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[t], b1, d, 0, 0, 0);
+      if (spans && part != part0) {  // the lane's plane is in the next part
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const float g = (float)(__bf16)fmaxf(acc[s][nt][r] + bn, 0.0f);
-          part[r] = fmaf(vn, g, part[r]);
+          const int m = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
+          d[r] += u[part * kD + m] - u[part0 * kD + m];
         }
       }
-      // sum over the 32 columns held by the 32 lanes of this half-wave
 #pragma unroll
-      for (int r = 0; r < 16; ++r)
+      for (int r = 0; r < 16; ++r) b2[2 * t + (r >> 3)][r & 7] = (__bf16)fmaxf(d[r], 0.0f);
+    }
+    f32x16 y;
 #pragma unroll
-        for (int off = 16; off > 0; off >>= 1) part[r] += __shfl_xor(part[r], off);
-      if (col == 0) {
+    for (int r = 0; r < 16; ++r) y[r] = 0.0f;
+#pragma unroll 1
+    for (int s = 0; s < 8; ++s) {
+      f32x16 d;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int64_t orow = (t * RT + s) * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-          if (orow < BP) out[orow] = part[r] + bias3;
+      for (int r = 0; r < 16; ++r) d[r] = 0.0f;
+      const unsigned short *vrow = &sV[(32 * s + col) * kV2Stride + 8 * half];
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) {
+        const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(vrow + 16 * kk);
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[kk], d, 0, 0, 0);
+      }
+      bf16x8_t b3[2];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int o = 32 * s + 8 * (r >> 2) + 4 * half + (r & 3);
+        b3[r >> 3][r & 7] = (__bf16)fmaxf(d[r] + sc2[o], 0.0f);
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        bf16x8_t a;
+        if (col < 2) {
+          a = *reinterpret_cast<const bf16x8_t *>(&sV3[col * kD + 32 * s + 16 * b + 8 * half]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) a[i] = (__bf16)0.0f;
         }
+        y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b3[b], y, 0, 0, 0);
       }
     }
+    // rows 0 and 1 (v3_hi and v3_lo products) are elements 0 and 1 of lanes 0-31
+    if (half == 0 && valid) out[row] = y[0] + y[1] + bias3;
   }
 }
 
@@ -614,14 +673,12 @@ void forward(vacd_ctx *ctx, vacd_mlp *M, const Acts &a, int B, int E, const floa
   gemm_bf16(ctx, false, true, B, kD, kD, a.zb, kD, wb + kB_Wz, kD, w + M->L.c1, false, false, a.u,
             nullptr, kD);
   if (fused) {
-    // one 32-row tile per wave and step, eight waves per block (two 32-row
-    // tiles sharing each V2 fragment, four waves per block, measured slower:
-    // 0.46 vs 0.31 ms for 64 parts, the accumulators then spill)
-    constexpr int kRT = 1, kWaves = kHeadWaves;
-    const int64_t tiles = (BP + 32 * kRT - 1) / (32 * kRT);
+    // 32 planes per wave and step, one block of kHeadWaves waves per CU
+    constexpr int kWaves = kHeadWaves;
+    const int64_t tiles = (BP + 31) / 32;
     const unsigned grid = (unsigned)std::min<int64_t>((tiles + kWaves - 1) / kWaves, 256);
-    k_head_fused<kRT><<<grid, 64 * kWaves, 0, st>>>(a.u, w + M->L.Wp, planes, BP, P, wb + kB_V2,
-                                                    w + M->L.c2, w + M->L.v3, w + M->L.c3, a.out);
+    k_head_fused<kWaves><<<grid, 64 * kWaves, 0, st>>>(a.u, w + M->L.Wp, planes, BP, P, wb + kB_V2,
+                                                       w + M->L.c2, w + M->L.v3, w + M->L.c3, a.out);
     VACD_HIP(hipGetLastError());
     return;
   }
