@@ -100,26 +100,30 @@ def test_mlp_forward_matches_fp32_reference(ctx, B, E, P):
     m.close()
 
 
-def test_fused_inference_head_matches_the_training_forward(ctx):
-    """The fused inference head (g1 and g2 kept in registers) gives the same
-    predictions as the unfused forward of a training step, up to fp32
-    summation order: the step's loss (lr 0) equals the loss of the fused
-    outputs.  P is not a multiple of 32, so row tiles span parts."""
+@pytest.mark.parametrize("P", [200, 20])
+def test_fused_inference_head_matches_the_training_forward(ctx, P):
+    """The fused inference head (g1 and g2 kept in registers) gives the
+    predictions of the unfused forward of a training step: the step's loss
+    (lr 0) equals the loss of the fused outputs within 5e-4.  The two differ
+    by the fused head's bf16 splits of q, Wp and u (about 1e-5 of g1 before
+    its bf16 rounding, which flips that rounding for a few values) and fp32
+    summation order.  P is not a multiple of 32, so row tiles span parts (two
+    parts at P = 200; several at P = 20, the small-part variant)."""
     import torch
 
     from visacd_amd import native
 
-    print("mlp fused head", flush=True)
+    print("mlp fused head", P, flush=True)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     m = native.Mlp(ctx, seed=5)
-    B, E, P = 4, 64, 200
+    B, E = 4, 64
     feat, planes, target = _batch(2, B, E, P)
     out = torch.empty(B * P, device="cuda")
     m.forward_d(B, E, feat.cuda(), P, planes.cuda(), out)
     torch.cuda.synchronize()
     loss_fused = float(((out.cpu() - target.reshape(-1)) ** 2).mean())
     loss_step = m.train_step_d(B, E, feat.cuda(), P, planes.cuda(), target.reshape(-1).cuda(), 0.0)
-    assert abs(loss_fused - loss_step) <= 1e-4 * loss_step, (loss_fused, loss_step)
+    assert abs(loss_fused - loss_step) <= 5e-4 * loss_step, (loss_fused, loss_step)
     # and the same values again (deterministic)
     out2 = torch.empty_like(out)
     m.forward_d(B, E, feat.cuda(), P, planes.cuda(), out2)

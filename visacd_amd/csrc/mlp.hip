@@ -163,30 +163,37 @@ __global__ void k_head_out(const unsigned short *__restrict__ g2, const float *_
 
 // Inference head, fused: out[p] = v3 . bf16(relu(V2 g1[p] + c2)) + c3 with
 // g1[p] = bf16(relu(u[b] + Wp q[p])), every product on MFMA and neither g1
-// nor g2 in memory (the unfused training path writes both).  A wave takes 32
-// planes at a time; v_mfma_f32_32x32x16_bf16 operands: lane l holds row (A)
-// or column (B) l % 32 and k = 8 (l / 32) .. +7; result element r of lane l
-// is row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.  The products run
-// transposed (hidden units as rows, planes as columns), so that what one
-// MFMA leaves in a lane -- 16 hidden units of the lane's plane -- is the B
-// operand of the next one for two k-steps, once the hidden units are
+// nor g2 in memory (the unfused training path writes both).  A wave takes
+// RT tiles of 32 planes at a time; v_mfma_f32_32x32x16_bf16 operands: lane l
+// holds row (A) or column (B) l % 32 and k = 8 (l / 32) .. +7; result element
+// r of lane l is row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.  The
+// products run transposed (hidden units as rows, planes as columns), so that
+// what one MFMA leaves in a lane -- 16 hidden units of the lane's plane -- is
+// the B operand of the next one for two k-steps, once the hidden units are
 // renumbered; the V2 columns and v3 are stored in that order (head_perm).
-//   g1: eight MFMAs, one per 32 hidden units: A = (Wp_hi | Wp_lo) in lanes
-//       0-31, (Wp_hi | u_hi u_mid u_lo 0) in lanes 32-63, B = (q_hi | q_hi)
-//       and (q_lo | 1 1 1 0): Wp q + u with bf16 splits of q and Wp (only the
-//       lo x lo term, about 2^-16 of the product, is left out) and u exact as
-//       three bf16 terms; then relu and the bf16 rounding of the unfused
-//       path.  u belongs to the part of the tile's first plane; lanes whose
-//       plane lies in the next part (tiles spanning parts) are corrected by
-//       the difference of the two parts' u in float32.
+//   g1: eight MFMAs per tile, one per 32 hidden units: A = (Wp_hi | Wp_lo)
+//       in lanes 0-31, (Wp_hi | u_hi u_lo of part c, u_hi u_lo of part c + 1)
+//       in lanes 32-63, B = (q_hi | q_hi) and (q_lo | s0 s0 s1 s1), where s0
+//       / s1 say whether the plane lies in part c / c + 1 (c: the part of the
+//       tile's first plane; with parts of 32 planes or more a tile spans at
+//       most two).  So Wp q + u with bf16 splits of q, Wp and u; only the lo x
+//       lo term of Wp q (about 2^-16 of it) and u's third bf16 term (2^-16 of
+//       u) are left out, far below the bf16 rounding of g1 that follows, as in
+//       the unfused path, with relu.  Parts of fewer than 32 planes (SMALLP)
+//       add the u of a later part in float32.
 //   g2: 16 k-steps per 32 outputs with V2 from LDS (rows padded by 16 bytes
-//       against bank conflicts), + c2, relu, bf16.
+//       against bank conflicts) shared by the RT tiles, + c2, relu, bf16.
 //   y:  two MFMAs per 32 outputs with A = (v3_hi, v3_lo) in rows 0 and 1:
 //       y = row 0 + row 1 + c3.  Products of bf16 values are exact in the
 //       float32 accumulators.
-// One block per CU (V2 fills most of its LDS), NW waves.
+// relu of a bf16 pair is a signed 16-bit max with 0 (relu_bf16x2).  One
+// block of NW waves per CU (V2 fills most of its LDS); each wave takes a
+// contiguous run of tiles.  64 parts x 16,384 planes: 0.30 ms with g1 and
+// the v3 dot on the VALU, 0.134 ms this way (the whole head; one tile per
+// step measured 0.136 ms at three waves per SIMD, two tiles 0.134 ms at two).
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-constexpr int kHeadWaves = 8;
+constexpr int kHeadFusedWaves = 8;  // the head's
+constexpr int kHeadTiles = 2;       // its 32-plane tiles per wave and step
 constexpr int kV2Stride = kD + 8;  // bf16 per padded LDS row
 
 // Position of hidden unit c in the order of the head's B operands: for c =
@@ -196,27 +203,54 @@ __host__ __device__ __forceinline__ int head_perm(int c) {
   return (c & ~31) | (c & 16) | ((c & 4) << 1) | ((c & 8) >> 1) | (c & 3);
 }
 
+// bf16 pair (a, b) of relu(x), relu(y): round both (v_cvt_pk_bf16_f32), then
+// max with 0 as signed 16-bit integers -- a negative bf16 (or -0) has the sign
+// bit set, so this is relu, and relu commutes with the rounding.
+typedef short short2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned relu_bf16x2(float x, float y) {
+  bf16x2_t v;
+  v[0] = (__bf16)x;
+  v[1] = (__bf16)y;
+  const short2_t m = __builtin_elementwise_max(__builtin_bit_cast(short2_t, v), (short2_t)(0));
+  return __builtin_bit_cast(unsigned, m);
+}
+
 __device__ __forceinline__ void split2(float x, __bf16 &hi, __bf16 &lo) {
   hi = (__bf16)x;
   lo = (__bf16)(x - (float)hi);
 }
 
-template <int NW>
+template <int NW, int RT, bool SMALLP>
 __global__ __launch_bounds__(64 * NW) void k_head_fused(
     const float *__restrict__ u, const float *__restrict__ Wp, const float *__restrict__ q,
     int64_t BP, int P, const unsigned short *__restrict__ V2, const float *__restrict__ c2,
     const float *__restrict__ v3, const float *__restrict__ c3, float *__restrict__ out) {
   __shared__ __attribute__((aligned(16))) unsigned short sV[kD * kV2Stride];
+  __shared__ __attribute__((aligned(16))) __bf16 sWp[kD * 8];  // per unit: Wp_hi (4), Wp_lo (4)
   __shared__ __attribute__((aligned(16))) __bf16 sV3[2 * kD];
-  __shared__ float sc2[kD];
+  __shared__ __attribute__((aligned(16))) float sc2[kD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // V2 rows into LDS with permuted columns: 8 consecutive columns (aligned)
-  // are two runs of 4 in the permuted order
-  for (int c = tid; c < kD * kD / 8; c += 64 * NW) {
-    const int n = c / (kD / 8), k = 8 * (c % (kD / 8));
-    const uint4 v = reinterpret_cast<const uint4 *>(V2)[c];
-    *reinterpret_cast<uint2 *>(&sV[n * kV2Stride + head_perm(k)]) = make_uint2(v.x, v.y);
-    *reinterpret_cast<uint2 *>(&sV[n * kV2Stride + head_perm(k + 4)]) = make_uint2(v.z, v.w);
+  // are two runs of 4 in the permuted order.  Every load is issued before
+  // the first store (a load-store loop waits out one memory latency per
+  // iteration).
+  constexpr int kChunks = kD * kD / 8, kPer = (kChunks + 64 * NW - 1) / (64 * NW);
+  {
+    uint4 v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int c = tid + j * 64 * NW;
+      if (c < kChunks) v[j] = reinterpret_cast<const uint4 *>(V2)[c];
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int c = tid + j * 64 * NW;
+      if (c >= kChunks) break;
+      const int n = c / (kD / 8), k = 8 * (c % (kD / 8));
+      *reinterpret_cast<uint2 *>(&sV[n * kV2Stride + head_perm(k)]) = make_uint2(v[j].x, v[j].y);
+      *reinterpret_cast<uint2 *>(&sV[n * kV2Stride + head_perm(k + 4)]) = make_uint2(v[j].z, v[j].w);
+    }
   }
   for (int k = tid; k < kD; k += 64 * NW) {
     __bf16 hi, lo;
@@ -224,100 +258,120 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
     sV3[head_perm(k)] = hi;
     sV3[kD + head_perm(k)] = lo;
     sc2[k] = c2[k];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      split2(Wp[4 * k + c], hi, lo);
+      sWp[8 * k + c] = hi;
+      sWp[8 * k + 4 + c] = lo;
+    }
   }
   __syncthreads();
   const float bias3 = c3[0];
   const int half = lane >> 5, col = lane & 31;
-  // the g1 A operands: Wp splits (both halves), u terms (half 1, per part)
-  bf16x8_t a1[8];
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const int m = 32 * t + col;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      __bf16 hi, lo;
-      split2(Wp[4 * m + c], hi, lo);
-      a1[t][c] = hi;
-      a1[t][4 + c] = half ? (__bf16)0.0f : lo;
-    }
-  }
-  int64_t cur = -1;  // the part whose u the a1 operands hold
+  uint2 au[8];  // per 32 units: u_hi, u_lo of part cur, u_hi, u_lo of part cur + 1
+  int64_t cur = -1;
+  // each wave takes a contiguous run of tiles, so its part changes rarely
   const int64_t tiles = (BP + 31) / 32;
-  for (int64_t tile = (int64_t)blockIdx.x * NW + wave; tile < tiles;
-       tile += (int64_t)gridDim.x * NW) {
-    const int64_t row = tile * 32 + col;  // this lane's plane
-    const bool valid = row < BP;
-    const int64_t part0 = tile * 32 / P;
-    const int64_t part = valid ? row / P : part0;
-    const bool spans = (std::min<int64_t>(tile * 32 + 31, BP - 1)) / P != part0;
-    if (part0 != cur) {  // wave-uniform
-      cur = part0;
-      if (half) {
+  const int64_t nw = (int64_t)gridDim.x * NW, wid = (int64_t)blockIdx.x * NW + wave;
+  const int64_t run = (tiles + nw - 1) / nw;
+  const int64_t t_end = std::min<int64_t>(tiles, (wid + 1) * run);
+  for (int64_t tile = wid * run; tile < t_end; tile += RT) {
+    // RT tiles at a time share every V2 fragment read from LDS
+    int64_t row[RT];
+    bool valid[RT];
+    // g1 of each tile: the B operands of the main product, k-step 2 t + b in
+    // b2[x][2 t + b]
+    bf16x8_t b2[RT][16];
+#pragma unroll
+    for (int x = 0; x < RT; ++x) {
+      row[x] = (tile + x) * 32 + col;  // this lane's plane
+      valid[x] = tile + x < t_end && row[x] < BP;
+      const int64_t p0 = std::min<int64_t>(tile + x, tiles - 1) * 32 / P;
+      if (p0 != cur) {  // wave-uniform: the u terms of parts p0 and p0 + 1
+        cur = p0;
+        const int64_t p1 = std::min<int64_t>(p0 + 1, (BP - 1) / P);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-          const float x = u[part0 * kD + 32 * t + col];
-          const __bf16 h0 = (__bf16)x;
-          const float r1 = x - (float)h0;
-          const __bf16 h1 = (__bf16)r1;
-          a1[t][4] = h0;
-          a1[t][5] = h1;
-          a1[t][6] = (__bf16)(r1 - (float)h1);
+          __bf16 h0, l0, h1, l1;
+          split2(u[p0 * kD + 32 * t + col], h0, l0);
+          split2(u[p1 * kD + 32 * t + col], h1, l1);
+          bf16x2_t lo, hi;
+          lo[0] = h0, lo[1] = l0, hi[0] = h1, hi[1] = l1;
+          au[t] = make_uint2(__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi));
         }
       }
-    }
-    const float4 qv = valid ? *reinterpret_cast<const float4 *>(q + 4 * row)
-                            : make_float4(0.f, 0.f, 0.f, 0.f);
-    bf16x8_t b1;
-    {
-      const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
+      const int64_t part = valid[x] ? row[x] / P : cur;
+      const float4 qv = valid[x] ? *reinterpret_cast<const float4 *>(q + 4 * row[x])
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+      // B operand of g1: (q_hi | q_hi) in lanes 0-31, (q_lo | s0 s0 s1 s1) in
+      // lanes 32-63, s0 / s1: the plane lies in part cur / cur + 1 (with
+      // parts of at least 32 planes a tile spans at most two)
+      bf16x8_t b1;
+      {
+        const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
+        const __bf16 s0 = (__bf16)(part == cur ? 1.0f : 0.0f);
+        const __bf16 s1 = (__bf16)(part == cur + 1 ? 1.0f : 0.0f);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        __bf16 hi, lo;
-        split2(qa[c], hi, lo);
-        b1[c] = half ? lo : hi;
-        b1[4 + c] = half ? (__bf16)(c < 3 ? 1.0f : 0.0f) : hi;
-      }
-    }
-    // g1: the B operands of the main product, k-step 2 t + b in b2[2 t + b]
-    bf16x8_t b2[16];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      f32x16 d;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) d[r] = 0.0f;
-      d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[t], b1, d, 0, 0, 0);
-      if (spans && part != part0) {  // the lane's plane is in the next part
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int m = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
-          d[r] += u[part * kD + m] - u[part0 * kD + m];
+        for (int c = 0; c < 4; ++c) {
+          __bf16 hi, lo;
+          split2(qa[c], hi, lo);
+          b1[c] = half ? lo : hi;
+          b1[4 + c] = half ? (c < 2 ? s0 : s1) : hi;
         }
       }
+      const bool other = SMALLP && __ballot(part != cur && part != cur + 1) != 0;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) b2[2 * t + (r >> 3)][r & 7] = (__bf16)fmaxf(d[r], 0.0f);
+      for (int t = 0; t < 8; ++t) {
+        // A: (Wp_hi | Wp_lo) in lanes 0-31, (Wp_hi | u terms) in lanes 32-63
+        uint4 aw = *reinterpret_cast<const uint4 *>(&sWp[8 * (32 * t + col)]);
+        if (half) aw.z = au[t].x, aw.w = au[t].y;
+        f32x16 d;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) d[r] = 0.0f;
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, aw), b1, d, 0,
+                                                    0, 0);
+        if (SMALLP && other && part != cur && part != cur + 1) {
+          // parts of fewer than 32 planes: a later part's u added directly
+#pragma unroll
+          for (int r = 0; r < 16; ++r)
+            d[r] += u[part * kD + 32 * t + 8 * (r >> 2) + 4 * half + (r & 3)];
+        }
+        uint4 g0, g1;
+        g0.x = relu_bf16x2(d[0], d[1]), g0.y = relu_bf16x2(d[2], d[3]);
+        g0.z = relu_bf16x2(d[4], d[5]), g0.w = relu_bf16x2(d[6], d[7]);
+        g1.x = relu_bf16x2(d[8], d[9]), g1.y = relu_bf16x2(d[10], d[11]);
+        g1.z = relu_bf16x2(d[12], d[13]), g1.w = relu_bf16x2(d[14], d[15]);
+        b2[x][2 * t] = __builtin_bit_cast(bf16x8_t, g0);
+        b2[x][2 * t + 1] = __builtin_bit_cast(bf16x8_t, g1);
+        // one 16-register result live at a time (the scheduler would
+        // otherwise start all eight products first and run out of registers)
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
-    f32x16 y;
+    f32x16 y[RT];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) y[r] = 0.0f;
+    for (int x = 0; x < RT; ++x)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) y[x][r] = 0.0f;
 #pragma unroll 1
     for (int s = 0; s < 8; ++s) {
-      f32x16 d;
+      f32x16 d[RT];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) d[r] = 0.0f;
+      for (int x = 0; x < RT; ++x)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) d[x][r] = 0.0f;
       const unsigned short *vrow = &sV[(32 * s + col) * kV2Stride + 8 * half];
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk) {
         const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(vrow + 16 * kk);
-        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[kk], d, 0, 0, 0);
-      }
-      bf16x8_t b3[2];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int o = 32 * s + 8 * (r >> 2) + 4 * half + (r & 3);
-        b3[r >> 3][r & 7] = (__bf16)fmaxf(d[r] + sc2[o], 0.0f);
+        for (int x = 0; x < RT; ++x)
+          d[x] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[x][kk], d[x], 0, 0, 0);
       }
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
+        const float4 c0 = *reinterpret_cast<const float4 *>(&sc2[32 * s + 16 * b + 4 * half]);
+        const float4 c1 = *reinterpret_cast<const float4 *>(&sc2[32 * s + 16 * b + 8 + 4 * half]);
         bf16x8_t a;
         if (col < 2) {
           a = *reinterpret_cast<const bf16x8_t *>(&sV3[col * kD + 32 * s + 16 * b + 8 * half]);
@@ -325,99 +379,219 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
 #pragma unroll
           for (int i = 0; i < 8; ++i) a[i] = (__bf16)0.0f;
         }
-        y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b3[b], y, 0, 0, 0);
+        const int e = 8 * b;
+#pragma unroll
+        for (int x = 0; x < RT; ++x) {
+          uint4 g;
+          g.x = relu_bf16x2(d[x][e] + c0.x, d[x][e + 1] + c0.y);
+          g.y = relu_bf16x2(d[x][e + 2] + c0.z, d[x][e + 3] + c0.w);
+          g.z = relu_bf16x2(d[x][e + 4] + c1.x, d[x][e + 5] + c1.y);
+          g.w = relu_bf16x2(d[x][e + 6] + c1.z, d[x][e + 7] + c1.w);
+          y[x] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8_t, g), y[x],
+                                                         0, 0, 0);
+        }
       }
     }
     // rows 0 and 1 (v3_hi and v3_lo products) are elements 0 and 1 of lanes 0-31
-    if (half == 0 && valid) out[row] = y[0] + y[1] + bias3;
+#pragma unroll
+    for (int x = 0; x < RT; ++x)
+      if (half == 0 && valid[x]) out[row[x]] = y[x][0] + y[x][1] + bias3;
   }
 }
 
-// C[m][:] = bf16(relu(W A[m][:] + bias)) for a 256-wide layer (A bf16
-// [M][256], W bf16 [256][256] as [out][in]): the inference form of the
-// encoder's 256 -> 256 layers, laid out like k_head_fused (W in padded LDS
-// rows, eight waves, 32 rows per wave at a time) with the A operand read
-// straight from memory (lane l: row l % 32, 16 bytes at k = 8 (l / 32)).
-// POOL: instead of storing C, max-pool it over each part's E rows into
-// zbits[b][n] (float bits of the non-negative relu outputs, which order like
-// unsigned integers; zbits must start at 0).
-template <bool POOL>
-__global__ __launch_bounds__(64 * kHeadWaves) void k_layer256(
-    const unsigned short *__restrict__ A, int64_t M, const unsigned short *__restrict__ W,
-    const float *__restrict__ bias, unsigned short *__restrict__ C, int E,
-    unsigned *__restrict__ zbits) {
-  __shared__ __attribute__((aligned(16))) unsigned short sV[kD * kV2Stride];
-  __shared__ float sb[kD];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int c = tid; c < kD * kD / 8; c += 64 * kHeadWaves) {
-    const int n = c / (kD / 8), k8 = c % (kD / 8);
-    *reinterpret_cast<uint4 *>(&sV[n * kV2Stride + 8 * k8]) = reinterpret_cast<const uint4 *>(W)[c];
-  }
-  for (int k = tid; k < kD; k += 64 * kHeadWaves) sb[k] = bias[k];
+// Inference encoder, fused: per 32 edges of x (one wave), h1 = relu(W1 x +
+// b1), h2 = relu(W2 h1 + b2), h3 = relu(W3 h2 + b3) (each rounded to bf16 as
+// in the unfused path) and the max-pool of h3 over each part's E edges into
+// zbits (float bits of the non-negative pooled values, which order like
+// unsigned integers; zbits must start at 0), with no activation in memory.
+// Layers 1 and 2 run transposed (units as rows, edges as columns), so each
+// result is the next product's B operand (head_perm order, as in
+// k_head_fused); layer 3 runs untransposed with h2 as the A operand (the same
+// register layout serves both) and W3 as B, so each lane ends with 16 edges
+// of one unit and pools them in registers: one atomic per unit and tile.
+// The block's waves take NW tiles per round: W2 (columns permuted) is staged
+// into LDS for layers 1-2, then W3 (rows permuted) into the same LDS for
+// layer 3, with block barriers between.
+// u[b][n] = c1[n] + sum_k Wz[n][k] bf16(z[b][k]) for the pooled parts (the
+// product of the unfused path in float32 with the same bf16 operands, in
+// another summation order): a block per part, a thread per output, its Wz
+// row read 16 bytes at a time with every load issued up front.  A 64 x 256 x
+// 256 product through k_gemm runs in 4 blocks and waits out 16 load
+// latencies (21 us); this takes about 3.
+__global__ __launch_bounds__(kD) void k_pool_proj(const float *__restrict__ z,
+                                                  const unsigned short *__restrict__ Wz,
+                                                  const float *__restrict__ c1,
+                                                  float *__restrict__ u) {
+  __shared__ float zs[kD];
+  const int b = blockIdx.x, n = threadIdx.x;
+  zs[n] = bf2f(f2bf(z[b * kD + n]));
   __syncthreads();
+  uint4 w[kD / 8];
+#pragma unroll
+  for (int j = 0; j < kD / 8; ++j) w[j] = reinterpret_cast<const uint4 *>(Wz + (size_t)n * kD)[j];
+  float acc = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kD / 8; ++j) {
+    const unsigned x[4] = {w[j].x, w[j].y, w[j].z, w[j].w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      acc = fmaf(__uint_as_float(x[i] << 16), zs[8 * j + 2 * i], acc);
+      acc = fmaf(__uint_as_float(x[i] & 0xffff0000u), zs[8 * j + 2 * i + 1], acc);
+    }
+  }
+  u[b * kD + n] = acc + c1[n];
+}
+
+template <int NT>  // threads of the block
+__device__ __forceinline__ void stage_w(unsigned short *sW, const unsigned short *__restrict__ W,
+                                        int tid) {
+  // rows of W with permuted columns; every load issued before the first store
+  constexpr int kChunks = kD * kD / 8, kPer = (kChunks + NT - 1) / NT;
+  uint4 v[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int c = tid + j * NT;
+    if (c < kChunks) v[j] = reinterpret_cast<const uint4 *>(W)[c];
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int c = tid + j * NT;
+    if (c >= kChunks) break;
+    const int n = c / (kD / 8), k = 8 * (c % (kD / 8));
+    *reinterpret_cast<uint2 *>(&sW[n * kV2Stride + head_perm(k)]) = make_uint2(v[j].x, v[j].y);
+    *reinterpret_cast<uint2 *>(&sW[n * kV2Stride + head_perm(k + 4)]) = make_uint2(v[j].z, v[j].w);
+  }
+}
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_encoder(
+    const float *__restrict__ feat, int64_t R, int E, const unsigned short *__restrict__ W1,
+    const float *__restrict__ b1, const unsigned short *__restrict__ W2,
+    const float *__restrict__ b2, const unsigned short *__restrict__ W3,
+    const float *__restrict__ b3, unsigned *__restrict__ zbits) {
+  __shared__ __attribute__((aligned(16))) unsigned short sW[kD * kV2Stride];
+  __shared__ __attribute__((aligned(16))) unsigned short sW1[kD * 8];
+  __shared__ __attribute__((aligned(16))) float sb[3][kD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int k = tid; k < kD; k += 64 * NW) {
+    *reinterpret_cast<uint4 *>(&sW1[8 * k]) = reinterpret_cast<const uint4 *>(W1)[k];
+    sb[0][k] = b1[k], sb[1][k] = b2[k], sb[2][k] = b3[k];
+  }
   const int half = lane >> 5, col = lane & 31;
-  const int64_t tiles = (M + 31) / 32;
-  for (int64_t t = (int64_t)blockIdx.x * kHeadWaves + wave; t < tiles;
-       t += (int64_t)gridDim.x * kHeadWaves) {
-    const int64_t row = t * 32 + col;
-    const bool valid = row < M;
-    f32x16 acc[8];
+  const int64_t tiles = (R + 31) / 32;
+  for (int64_t base = (int64_t)blockIdx.x * NW; base < tiles; base += (int64_t)gridDim.x * NW) {
+    const int64_t tile = base + wave;
+    const bool live = tile < tiles;  // wave-uniform; every wave joins the barriers
+    const int64_t e0 = tile * 32;
+    stage_w<64 * NW>(sW, W2, tid);
+    __syncthreads();
+    bf16x8_t h2[16];
+    if (live) {
+      // layer 1: B = x^T (lanes 0-31: the 7 features and a zero of edge col;
+      // lanes 32-63: zeros, k padded to 16), A = W1 rows (lanes 0-31)
+      bf16x8_t bx;
+      {
+        const int64_t e = e0 + col;
 #pragma unroll
-    for (int nt = 0; nt < 8; ++nt)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[nt][r] = 0.0f;
-    for (int ks = 0; ks < kD / 16; ++ks) {
-      const int k0 = ks * 16 + 8 * half;
-      bf16x8_t a;
-      if (valid) {
-        a = *reinterpret_cast<const bf16x8_t *>(A + row * kD + k0);
-      } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) a[i] = (__bf16)0.0f;
+        for (int i = 0; i < 8; ++i)
+          bx[i] = (__bf16)((half == 0 && i < 7 && e < R) ? feat[e * 7 + i] : 0.0f);
       }
+      bf16x8_t h1[16];
 #pragma unroll
-      for (int nt = 0; nt < 8; ++nt) {
-        const bf16x8_t b =
-            *reinterpret_cast<const bf16x8_t *>(&sV[(nt * 32 + col) * kV2Stride + k0]);
-        acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nt], 0, 0, 0);
+      for (int t = 0; t < 8; ++t) {
+        bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&sW1[8 * (32 * t + col)]);
+        if (half) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) a[i] = (__bf16)0.0f;
+        }
+        f32x16 d;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) d[r] = 0.0f;
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bx, d, 0, 0, 0);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const float4 c0 = *reinterpret_cast<const float4 *>(&sb[0][32 * t + 16 * b + 4 * half]);
+          const float4 c1 =
+              *reinterpret_cast<const float4 *>(&sb[0][32 * t + 16 * b + 8 + 4 * half]);
+          const int e = 8 * b;
+          uint4 g;
+          g.x = relu_bf16x2(d[e] + c0.x, d[e + 1] + c0.y);
+          g.y = relu_bf16x2(d[e + 2] + c0.z, d[e + 3] + c0.w);
+          g.z = relu_bf16x2(d[e + 4] + c1.x, d[e + 5] + c1.y);
+          g.w = relu_bf16x2(d[e + 6] + c1.z, d[e + 7] + c1.w);
+          h1[2 * t + b] = __builtin_bit_cast(bf16x8_t, g);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      // layer 2: 16 k-steps per 32 units, A = W2 rows from LDS
+#pragma unroll
+      for (int s2 = 0; s2 < 8; ++s2) {
+        f32x16 d;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) d[r] = 0.0f;
+        const unsigned short *wrow = &sW[(32 * s2 + col) * kV2Stride + 8 * half];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+          d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+              *reinterpret_cast<const bf16x8_t *>(wrow + 16 * kk), h1[kk], d, 0, 0, 0);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const float4 c0 = *reinterpret_cast<const float4 *>(&sb[1][32 * s2 + 16 * b + 4 * half]);
+          const float4 c1 =
+              *reinterpret_cast<const float4 *>(&sb[1][32 * s2 + 16 * b + 8 + 4 * half]);
+          const int e = 8 * b;
+          uint4 g;
+          g.x = relu_bf16x2(d[e] + c0.x, d[e + 1] + c0.y);
+          g.y = relu_bf16x2(d[e + 2] + c0.z, d[e + 3] + c0.w);
+          g.z = relu_bf16x2(d[e + 4] + c1.x, d[e + 5] + c1.y);
+          g.w = relu_bf16x2(d[e + 6] + c1.z, d[e + 7] + c1.w);
+          h2[2 * s2 + b] = __builtin_bit_cast(bf16x8_t, g);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
-    if (!POOL) {
+    __syncthreads();  // every wave is done with W2
+    stage_w<64 * NW>(sW, W3, tid);
+    __syncthreads();
+    if (live) {
+      // layer 3 untransposed: rows = edges (A = h2), columns = units (B = W3
+      // rows from LDS); element r of lane l: edge e0 + 8 (r / 4) + 4 half +
+      // r % 4, unit 32 s3 + col
+      const int64_t part0 = e0 / E;
+      const bool one_part = (std::min<int64_t>(e0 + 31, R - 1)) / E == part0;
+#pragma unroll 1
+      for (int s3 = 0; s3 < 8; ++s3) {
+        const int n = 32 * s3 + col;
+        const unsigned short *wcol = &sW[n * kV2Stride + 8 * half];
+        f32x16 d;
 #pragma unroll
-      for (int nt = 0; nt < 8; ++nt) {
-        const int n = nt * 32 + col;
-        const float bn = sb[n];
+        for (int r = 0; r < 16; ++r) d[r] = 0.0f;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int64_t orow = t * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-          if (orow < M) C[orow * kD + n] = f2bf(fmaxf(acc[nt][r] + bn, 0.0f));
-        }
-      }
-    } else {
-      const int64_t last = (t * 32 + 31 < M ? t * 32 + 31 : M - 1);
-      const bool one_part = (t * 32) / E == last / E;
-#pragma unroll
-      for (int nt = 0; nt < 8; ++nt) {
-        const int n = nt * 32 + col;
-        const float bn = sb[n];
+        for (int kk = 0; kk < 16; ++kk)
+          d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+              h2[kk], *reinterpret_cast<const bf16x8_t *>(wcol + 16 * kk), d, 0, 0, 0);
+        const float bn = sb[2][n];
         float mx = 0.0f;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int64_t orow = t * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-          if (orow >= M) continue;
+          const int64_t e = e0 + 8 * (r >> 2) + 4 * half + (r & 3);
+          if (e >= R) continue;
           // the value the unfused path pools: bf16 of the relu output
-          const float v = bf2f(f2bf(fmaxf(acc[nt][r] + bn, 0.0f)));
+          const float v = bf2f(f2bf(fmaxf(d[r] + bn, 0.0f)));
           if (one_part) mx = fmaxf(mx, v);
-          else atomicMax(&zbits[(orow / E) * kD + n], __float_as_uint(v));
+          else atomicMax(&zbits[(e / E) * kD + n], __float_as_uint(v));
         }
         if (one_part) {
-          mx = fmaxf(mx, __shfl_xor(mx, 32));  // the other half-wave's rows
-          if (half == 0) atomicMax(&zbits[((t * 32) / E) * kD + n], __float_as_uint(mx));
+          mx = fmaxf(mx, __shfl_xor(mx, 32));  // the other half-wave's edges
+          if (half == 0) atomicMax(&zbits[part0 * kD + n], __float_as_uint(mx));
         }
       }
     }
+    __syncthreads();  // every wave is done with W3 before the next round's W2
   }
 }
+
+
 
 // loss gradient and the last layer: dout = 2 (out - t) / BP; dg2 = relu'(g2) dout v3;
 // dv3 += dout g2 and dc3 += dout (atomics); loss += (out - t)^2 / BP
@@ -649,36 +823,46 @@ void forward(vacd_ctx *ctx, vacd_mlp *M, const Acts &a, int B, int E, const floa
   const float *w = M->w.as<float>();
   const unsigned short *wb = M->wb.as<unsigned short>();
   const int64_t R = (int64_t)B * E, BP = (int64_t)B * P;
-  k_edge_in<<<blocks_for(R * 8), 256, 0, st>>>(feat, R, a.x0);
-  gemm_bf16(ctx, false, true, (int)R, kD, 8, a.x0, 8, wb + kB_W1, 8, w + M->L.b1, true, false,
-            nullptr, a.h1, kD);
-  if (fused) {  // the 256 -> 256 encoder layers with the weights in LDS,
-                // the last one max-pooled in its epilogue
-    const unsigned grid =
-        (unsigned)std::min<int64_t>(((R + 31) / 32 + kHeadWaves - 1) / kHeadWaves, 256);
-    k_layer256<false><<<grid, 64 * kHeadWaves, 0, st>>>(a.h1, R, wb + kB_W2, w + M->L.b2, a.h2, E,
-                                                        nullptr);
+  if (fused) {  // the whole encoder and its max-pool in one pass
     fill_u64(ctx, a.z, (int64_t)B * kD / 2, 0ull);
-    k_layer256<true><<<grid, 64 * kHeadWaves, 0, st>>>(a.h2, R, wb + kB_W3, w + M->L.b3, nullptr,
-                                                       E, reinterpret_cast<unsigned *>(a.z));
+    // one wave per SIMD (a tile of 32 edges per wave): 64 parts of 512 edges
+    // make 1024 tiles, one per SIMD, and a wave may hold both layers'
+    // activations in registers
+    constexpr int kEncWaves = 4;
+    const unsigned grid =
+        (unsigned)std::min<int64_t>(((R + 31) / 32 + kEncWaves - 1) / kEncWaves, 256);
+    k_encoder<kEncWaves><<<grid, 64 * kEncWaves, 0, st>>>(
+        feat, R, E, wb + kB_W1, w + M->L.b1, wb + kB_W2, w + M->L.b2, wb + kB_W3, w + M->L.b3,
+        reinterpret_cast<unsigned *>(a.z));
     VACD_HIP(hipGetLastError());
   } else {
+    k_edge_in<<<blocks_for(R * 8), 256, 0, st>>>(feat, R, a.x0);
+    gemm_bf16(ctx, false, true, (int)R, kD, 8, a.x0, 8, wb + kB_W1, 8, w + M->L.b1, true, false,
+              nullptr, a.h1, kD);
     gemm_bf16(ctx, false, true, (int)R, kD, kD, a.h1, kD, wb + kB_W2, kD, w + M->L.b2, true, false,
               nullptr, a.h2, kD);
     gemm_bf16(ctx, false, true, (int)R, kD, kD, a.h2, kD, wb + kB_W3, kD, w + M->L.b3, true, false,
               nullptr, a.h3, kD);
   }
-  if (!fused) k_maxpool<<<(unsigned)B, kD, 0, st>>>(a.h3, E, a.z, a.arg);
-  to_bf16(ctx, a.z, (int64_t)B * kD, a.zb);
-  gemm_bf16(ctx, false, true, B, kD, kD, a.zb, kD, wb + kB_Wz, kD, w + M->L.c1, false, false, a.u,
-            nullptr, kD);
+  if (!fused) {
+    k_maxpool<<<(unsigned)B, kD, 0, st>>>(a.h3, E, a.z, a.arg);
+    to_bf16(ctx, a.z, (int64_t)B * kD, a.zb);
+    gemm_bf16(ctx, false, true, B, kD, kD, a.zb, kD, wb + kB_Wz, kD, w + M->L.c1, false, false,
+              a.u, nullptr, kD);
+  } else {
+    k_pool_proj<<<(unsigned)B, kD, 0, st>>>(a.z, wb + kB_Wz, w + M->L.c1, a.u);
+  }
   if (fused) {
-    // 32 planes per wave and step, one block of kHeadWaves waves per CU
-    constexpr int kWaves = kHeadWaves;
+    // 32 planes per wave and step, one block of kHeadFusedWaves waves per CU
+    constexpr int kWaves = kHeadFusedWaves;
     const int64_t tiles = (BP + 31) / 32;
     const unsigned grid = (unsigned)std::min<int64_t>((tiles + kWaves - 1) / kWaves, 256);
-    k_head_fused<kWaves><<<grid, 64 * kWaves, 0, st>>>(a.u, w + M->L.Wp, planes, BP, P, wb + kB_V2,
-                                                       w + M->L.c2, w + M->L.v3, w + M->L.c3, a.out);
+    if (P >= 32)
+      k_head_fused<kWaves, kHeadTiles, false><<<grid, 64 * kWaves, 0, st>>>(
+          a.u, w + M->L.Wp, planes, BP, P, wb + kB_V2, w + M->L.c2, w + M->L.v3, w + M->L.c3, a.out);
+    else
+      k_head_fused<kWaves, 1, true><<<grid, 64 * kWaves, 0, st>>>(
+          a.u, w + M->L.Wp, planes, BP, P, wb + kB_V2, w + M->L.c2, w + M->L.v3, w + M->L.c3, a.out);
     VACD_HIP(hipGetLastError());
     return;
   }
