@@ -464,6 +464,8 @@ def vis_pass(ctx, W, steps, warmup, sweeps=False):
             raise SystemExit("visibility result differs between walk variants")
         out.update({"kernel_ms_by_lanes_per_pair": lpp, "kernel_ms_by_front_lanes": front,
                     "kernel_ms_walk_every_pair": walk_all})
+    # walks clipped to the segment outside the endpoints' clearance balls
+    out["clipped_walks"] = ctx.visibility_counters()["clipped_walks"]
     # algorithmic work per segment (instrumented kernel variant, untimed)
     ctx.set_option(native.VACD_OPT_COUNT_WORK, 1)
     step()
@@ -606,9 +608,14 @@ def main():
                     help="only one decomposition, e.g. c5:batched:exact (for profiling)")
     ap.add_argument("--vis-only", action="store_true",
                     help="only the headline visibility passes (for profiling)")
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="only the proposal model timing (for profiling)")
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe(args.workload)
+        return
+    if args.mlp_only:
+        print(json.dumps(proposal_bench(max(3, args.steps // 2))), flush=True)
         return
     if args.acd_run:
         wname, mode, prop = args.acd_run.split(":")
