@@ -225,6 +225,68 @@ def test_process_c1_matches_cpu_oracle_loop(mode):
     assert r.concavity == ref_h
 
 
+def _dumbbell():
+    """Two 3x3x3 polycube boxes joined by a 1x1x2 bar along x."""
+    cells = [(x, y, z) for x in range(3) for y in range(3) for z in range(3)]
+    cells += [(x, 1, 1) for x in (3, 4)]
+    cells += [(x, y, z) for x in range(5, 8) for y in range(3) for z in range(3)]
+    return meshes.polycube(cells)
+
+
+@pytest.mark.parametrize("shape,n", [("lshape", 2), ("dumbbell", 3)])
+def test_process_closed_form_shapes(shape, n):
+    """Shapes whose decomposition is known in form (the reference's default
+    bisector planes, seed 42, host distances): the L-shape splits into two
+    parts at its notch, one of them exactly convex; the dumbbell splits into
+    its two boxes and the bar between them, the bar exactly convex and the
+    two boxes alike.  The cuts are bisector planes of visibility edges, so
+    they lie near, not on, the notch and the bar's ends (points on a cut
+    plane are in the 1e-6 dead zone and their edges would not count).  Both
+    equal the CPU restatement of the loop on the FP64 oracle, part for part."""
+    import visacd as vis
+    from oracle import decompose as D
+
+    V, T = meshes.l_shape() if shape == "lshape" else _dumbbell()
+    V = V.astype(np.float64)
+    print("closed form", shape, n, flush=True)
+    ref_parts, ref_h = D.decompose(V, T, 0.001, n, seed=42)
+    vis.engine.gpu_distance = False
+    try:
+        r = _run(vis, V, T, 0.001, n, return_parts=True)
+    finally:
+        vis.engine.gpu_distance = True
+        vis.config.return_parts = False
+    assert r.num_parts == len(ref_parts) == n
+    for p, (rv, rt) in zip(r.parts, ref_parts):
+        assert np.array_equal(np.array([list(v) for v in p.vertices]), rv)
+        assert np.array_equal(np.array([list(t) for t in p.triangles]), rt)
+    assert r.concavity == ref_h
+    from visacd_amd import _visacd as H
+
+    vols, hulls, xr = [], [], []
+    for p in r.parts:
+        vols.append(_volume(p))
+        hulls.append(H.mesh_volume(H.convex_hull(p)))
+        x = np.array([list(v) for v in p.vertices])[:, 0]
+        xr.append((x.min(), x.max()))
+    total = _volume(_mesh(vis, V, T))
+    assert abs(sum(vols) - total) < 1e-9 * abs(total) + 1e-12
+    convex = [abs(h - v) < 1e-9 * abs(v) + 1e-12 for v, h in zip(vols, hulls)]
+    print("  x ranges", xr, "convex", convex, "concavity", r.concavity, flush=True)
+    if shape == "lshape":
+        # the L spans x in [0, 0.75]; its notch is at x = 0.25 (or y = 0.25)
+        assert sum(convex) >= 1
+        cut = [b for a, b in xr if b < 0.7] + [a for a, b in xr if a > 0.05]
+        assert cut and all(abs(c - 0.25) < 0.02 for c in cut)
+    else:
+        # boxes at x in [0, 0.75] and [1.25, 2.0], the bar between
+        order = np.argsort([a for a, b in xr])
+        (a0, b0), (a1, b1), (a2, b2) = (xr[i] for i in order)
+        assert convex[order[1]]  # the bar
+        assert abs(b0 - 0.75) < 0.05 and abs(a2 - 1.25) < 0.05 and a1 == b0 and b1 == a2
+        assert abs(vols[order[0]] - vols[order[2]]) < 1e-3 * abs(vols[order[0]])
+
+
 def test_process_cow_reference_sample():
     """The reference's own sample input (data/cow.obj, via tests/golden/cow.npz)
     with its CLI defaults (concavity 0.04, 40 parts): plain and batched give

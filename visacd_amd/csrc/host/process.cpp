@@ -14,11 +14,14 @@
 //   * part selection by concavity or by total edge length, stop below the
 //     threshold, hulls, final concavity, return hulls or parts
 //     (process.cpp:183-347).
-// Differences, by design: no OpenVDB remesh and no cage (visibility runs in
-// midpoint-parity mode on the part itself, so the input must be a closed
-// mesh); concavity sampling uses a generator seeded per evaluation, so each
-// part's concavity is computed once and cached; engine.max_points bounds the
-// points whose pairs are tested by a seeded subsample.
+// Differences, by design: the remesh and the cages are the engine's own
+// signed-distance surfaces (OpenVDB is absent), an input is remeshed only
+// when it is not closed (engine.remesh), and visibility runs in
+// midpoint-parity mode unless engine.visibility = "cage"; candidate planes
+// and concavity samples come from generators seeded per part, so a part
+// splits the same way whenever it is split and its concavity is computed
+// once and cached; engine.max_points bounds the points whose pairs are
+// tested by a seeded subsample.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
