@@ -11,8 +11,10 @@
 // float32 resolution are then welded (weld_short_new_edges).
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <iterator>
 #include <map>
+#include <thread>
 #include <unordered_map>
 
 #include "mesh.hpp"
@@ -105,7 +107,12 @@ void bridge_hole(std::vector<P2> &outer, const std::vector<P2> &hole) {
 }
 
 // Ear clipping of a simple CCW polygon (bridges allowed); appends triangles
-// of cap ids.  Returns false if it gets stuck.
+// of cap ids.  Returns false if it gets stuck.  The ear test ("does any other
+// remaining vertex lie in the candidate ear") looks only at the vertices in
+// the cells of a uniform grid that the ear's box overlaps, widened by a cell
+// (a point on a cell border may round into the next cell); the answer, and so
+// the triangulation, is the one of a scan over every remaining vertex, at a
+// fraction of the cost on the long cap polygons of large parts.
 bool ear_clip(std::vector<P2> poly, std::vector<std::array<int, 3>> &tris) {
   int n = (int)poly.size();
   if (n < 3) return n == 0;
@@ -115,6 +122,21 @@ bool ear_clip(std::vector<P2> poly, std::vector<std::array<int, 3>> &tris) {
   double scale = 0;
   for (auto &p : poly) scale = std::max({scale, std::fabs(p.x), std::fabs(p.y)});
   const double eps = 1e-14 * std::max(1.0, scale * scale);
+  // the grid: about one vertex per cell
+  double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
+  for (auto &p : poly) x0 = std::min(x0, p.x), y0 = std::min(y0, p.y), x1 = std::max(x1, p.x), y1 = std::max(y1, p.y);
+  const int G = std::max(1, std::min(1024, (int)std::sqrt((double)n)));
+  const double cw = (x1 - x0) / G, ch = (y1 - y0) / G;
+  auto cx = [&](double x) { return cw > 0 ? std::min(G - 1, std::max(0, (int)((x - x0) / cw))) : 0; };
+  auto cy = [&](double y) { return ch > 0 ? std::min(G - 1, std::max(0, (int)((y - y0) / ch))) : 0; };
+  std::vector<int> cell_start(G * G + 1, 0), cell_items(n);
+  for (int i = 0; i < n; ++i) ++cell_start[cy(poly[i].y) * G + cx(poly[i].x) + 1];
+  for (int c = 0; c < G * G; ++c) cell_start[c + 1] += cell_start[c];
+  {
+    std::vector<int> fill(cell_start.begin(), cell_start.end() - 1);
+    for (int i = 0; i < n; ++i) cell_items[fill[cy(poly[i].y) * G + cx(poly[i].x)]++] = i;
+  }
+  std::vector<char> alive(n, 1);
   while (remaining > 3) {
     if (guard++ > 4 * remaining + 16) {
       // no strict ear: clip the most convex vertex (degenerate input)
@@ -128,17 +150,26 @@ bool ear_clip(std::vector<P2> poly, std::vector<std::array<int, 3>> &tris) {
       if (bo < -eps) return false;
       cur = best;
     } else {
-      const P2 &a = poly[prv[cur]], &b = poly[cur], &c = poly[nxt[cur]];
+      const int ip = prv[cur], in = nxt[cur];
+      const P2 &a = poly[ip], &b = poly[cur], &c = poly[in];
       bool ear = orient(a, b, c) > eps;
       if (ear) {
-        for (int v = nxt[nxt[cur]]; v != prv[cur]; v = nxt[v]) {
-          const P2 &p = poly[v];
-          if (p.id == a.id || p.id == b.id || p.id == c.id) continue;
-          if (in_tri(a, b, c, p)) {
-            ear = false;
-            break;
-          }
-        }
+        const int gx0 = std::max(0, cx(std::min({a.x, b.x, c.x})) - 1);
+        const int gx1 = std::min(G - 1, cx(std::max({a.x, b.x, c.x})) + 1);
+        const int gy0 = std::max(0, cy(std::min({a.y, b.y, c.y})) - 1);
+        const int gy1 = std::min(G - 1, cy(std::max({a.y, b.y, c.y})) + 1);
+        for (int gy = gy0; gy <= gy1 && ear; ++gy)
+          for (int gx = gx0; gx <= gx1 && ear; ++gx)
+            for (int k = cell_start[gy * G + gx]; k < cell_start[gy * G + gx + 1]; ++k) {
+              const int v = cell_items[k];
+              if (!alive[v] || v == ip || v == cur || v == in) continue;
+              const P2 &p = poly[v];
+              if (p.id == a.id || p.id == b.id || p.id == c.id) continue;
+              if (in_tri(a, b, c, p)) {
+                ear = false;
+                break;
+              }
+            }
       }
       if (!ear) {
         cur = nxt[cur];
@@ -149,6 +180,7 @@ bool ear_clip(std::vector<P2> poly, std::vector<std::array<int, 3>> &tris) {
     const int p = prv[cur], q = nxt[cur];
     nxt[p] = q;
     prv[q] = p;
+    alive[cur] = 0;
     --remaining;
     cur = p;
     guard = 0;
@@ -315,11 +347,28 @@ int weld_short_new_edges(Mesh &m) {
 
 MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::vector<int> &map_neg) {
   const int N = (int)m.vertices.size();
+  const int64_t T = (int64_t)m.triangles.size();
+  // large meshes: the per-vertex and per-triangle passes on several threads,
+  // the two halves built at once (the output does not depend on it)
+  const bool big = T >= 20000;
   std::vector<int> side(N);
+#pragma omp parallel for schedule(static) if (big)
   for (int i = 0; i < N; ++i) side[i] = pl.side(m.vertices[i]);
   map_pos.assign(N, 0);
   map_neg.assign(N, 0);
-  // cap points: on-plane vertices and edge crossings, keyed
+  // triangle kinds: 0 on the plane (covered by the caps), 1 wholly on the +
+  // side, 2 wholly on the - side, 3 touching or crossing the plane
+  std::vector<uint8_t> kind(T);
+#pragma omp parallel for schedule(static) if (big)
+  for (int64_t t = 0; t < T; ++t) {
+    const auto &tr = m.triangles[t];
+    const int s0 = side[tr[0]], s1 = side[tr[1]], s2 = side[tr[2]];
+    kind[t] = (s0 == 0 && s1 == 0 && s2 == 0) ? 0
+              : (s0 == 1 && s1 == 1 && s2 == 1) ? 1
+              : (s0 == -1 && s1 == -1 && s2 == -1) ? 2 : 3;
+  }
+  // cap points: on-plane vertices and edge crossings, keyed, numbered in
+  // order of first use over the touching triangles in triangle order
   std::map<std::pair<int, int>, int> cap_key;  // (a, b) a<=b; (v, v) for a vertex
   std::vector<Vec3> cap_pts;
   auto cap_vertex = [&](int v) {
@@ -340,31 +389,28 @@ MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::ve
     cap_pts.push_back(p + (q - p) * t);
     return (int)cap_pts.size() - 1;
   };
-  // pieces: polygon corners (at most four: a triangle cut by a plane)
-  // encoded as vertex id >= 0 or ~cap id
+  // pieces of the touching triangles: polygon corners (at most four: a
+  // triangle cut by a plane) encoded as vertex id >= 0 or ~cap id; tri: the
+  // triangle it comes from
   struct Piece {
     int c[4];
     int k;
+    int64_t tri;
   };
   std::vector<Piece> pieces[2];
-  pieces[0].reserve(m.triangles.size());
-  pieces[1].reserve(m.triangles.size());
   std::vector<std::pair<int, int>> plane_edges[2];
-  for (const auto &t : m.triangles) {
-    const int s[3] = {side[t[0]], side[t[1]], side[t[2]]};
-    if (s[0] == 0 && s[1] == 0 && s[2] == 0) continue;  // on the plane: covered by the caps
+  for (int64_t t = 0; t < T; ++t) {
+    if (kind[t] != 3) continue;
+    const auto &tr = m.triangles[t];
+    const int s[3] = {side[tr[0]], side[tr[1]], side[tr[2]]};
     for (int w = 0; w < 2; ++w) {
       const int want = w == 0 ? 1 : -1;
       if (s[0] != want && s[1] != want && s[2] != want) continue;
       Piece pc;
       pc.k = 0;
-      if (s[0] == want && s[1] == want && s[2] == want) {  // wholly on this side
-        pc.c[0] = t[0], pc.c[1] = t[1], pc.c[2] = t[2], pc.k = 3;
-        pieces[w].push_back(pc);
-        continue;
-      }
+      pc.tri = t;
       for (int e = 0; e < 3; ++e) {
-        const int a = t[e], b = t[(e + 1) % 3];
+        const int a = tr[e], b = tr[(e + 1) % 3];
         const int sa = s[e], sb = s[(e + 1) % 3];
         if (sa == want) pc.c[pc.k++] = a;
         else if (sa == 0) pc.c[pc.k++] = ~cap_vertex(a);
@@ -385,8 +431,13 @@ MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::ve
     return out;
   }
   const Vec3 n = {pl.a, pl.b, pl.c};
-  for (int w = 0; w < 2; ++w) {
-    Mesh part;
+  Mesh parts[2];
+  bool ok[2] = {true, true};
+  // half w: its vertices in increasing id order with 1-based maps, then its
+  // triangles in triangle order (whole ones, then the fan of each piece),
+  // cap vertices appended in order of first use, the cap, the weld
+  auto build = [&](int w) {
+    Mesh &part = parts[w];
     std::vector<int> &map = w == 0 ? map_pos : map_neg;
     const int want = w == 0 ? 1 : -1;
     for (int i = 0; i < N; ++i)
@@ -406,23 +457,61 @@ MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::ve
       }
       return cap_local[c];
     };
-    part.triangles.reserve(pieces[w].size() + 2 * plane_edges[w].size());
-    for (const Piece &pc : pieces[w])
-      for (int k = 1; k + 1 < pc.k; ++k)
-        part.triangles.push_back({local(pc.c[0]), local(pc.c[k]), local(pc.c[k + 1])});
+    part.triangles.reserve((size_t)(T / 2) + 2 * pieces[w].size() + 2 * plane_edges[w].size());
+    const uint8_t whole = w == 0 ? 1 : 2;
+    size_t pi = 0;
+    for (int64_t t = 0; t < T; ++t) {
+      if (kind[t] == whole) {
+        const auto &tr = m.triangles[t];
+        part.triangles.push_back({map[tr[0]] - 1, map[tr[1]] - 1, map[tr[2]] - 1});
+      } else if (kind[t] == 3) {
+        for (; pi < pieces[w].size() && pieces[w][pi].tri == t; ++pi) {
+          const Piece &pc = pieces[w][pi];
+          for (int k = 1; k + 1 < pc.k; ++k)
+            part.triangles.push_back({local(pc.c[0]), local(pc.c[k]), local(pc.c[k + 1])});
+        }
+      }
+    }
     std::vector<std::array<int, 3>> cap;
     const Vec3 cap_normal = w == 0 ? n * -1.0 : n;
     if (!build_cap(plane_edges[w], cap_pts, cap_normal * (1.0 / norm(cap_normal)), cap)) {
-      out.clear();
-      out.push_back(m);
-      return out;
+      ok[w] = false;
+      return;
     }
     for (auto &t : cap) part.triangles.push_back({local(~t[0]), local(~t[1]), local(~t[2])});
     // a collapse can leave a new short edge next to it: repeat until none
     for (int pass = 0; pass < 8 && weld_short_new_edges(part) > 0; ++pass) {
     }
-    out.push_back(std::move(part));
+  };
+  if (big) {
+    std::exception_ptr err;
+    std::thread other([&] {
+      try {
+        build(1);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+    try {
+      build(0);
+    } catch (...) {
+      other.join();
+      throw;
+    }
+    other.join();
+    if (err) std::rethrow_exception(err);
+  } else {
+    build(0);
+    build(1);
   }
+  if (!ok[0] || !ok[1]) {
+    // a failed + side leaves the - side's map unset, as a serial build would
+    if (!ok[0]) map_neg.assign(N, 0);
+    out.push_back(m);
+    return out;
+  }
+  out.push_back(std::move(parts[0]));
+  out.push_back(std::move(parts[1]));
   return out;
 }
 
