@@ -122,21 +122,25 @@ bool ear_clip(std::vector<P2> poly, std::vector<std::array<int, 3>> &tris) {
   double scale = 0;
   for (auto &p : poly) scale = std::max({scale, std::fabs(p.x), std::fabs(p.y)});
   const double eps = 1e-14 * std::max(1.0, scale * scale);
-  // the grid: about one vertex per cell
+  // the grid: about one vertex per cell; short polygons scan the ring
+  const bool use_grid = n >= 64;
   double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
   for (auto &p : poly) x0 = std::min(x0, p.x), y0 = std::min(y0, p.y), x1 = std::max(x1, p.x), y1 = std::max(y1, p.y);
-  const int G = std::max(1, std::min(1024, (int)std::sqrt((double)n)));
+  const int G = use_grid ? std::min(1024, (int)std::sqrt((double)n)) : 1;
   const double cw = (x1 - x0) / G, ch = (y1 - y0) / G;
   auto cx = [&](double x) { return cw > 0 ? std::min(G - 1, std::max(0, (int)((x - x0) / cw))) : 0; };
   auto cy = [&](double y) { return ch > 0 ? std::min(G - 1, std::max(0, (int)((y - y0) / ch))) : 0; };
-  std::vector<int> cell_start(G * G + 1, 0), cell_items(n);
-  for (int i = 0; i < n; ++i) ++cell_start[cy(poly[i].y) * G + cx(poly[i].x) + 1];
-  for (int c = 0; c < G * G; ++c) cell_start[c + 1] += cell_start[c];
-  {
+  std::vector<int> cell_start, cell_items;
+  std::vector<char> alive;
+  if (use_grid) {
+    cell_start.assign(G * G + 1, 0);
+    cell_items.resize(n);
+    alive.assign(n, 1);
+    for (int i = 0; i < n; ++i) ++cell_start[cy(poly[i].y) * G + cx(poly[i].x) + 1];
+    for (int c = 0; c < G * G; ++c) cell_start[c + 1] += cell_start[c];
     std::vector<int> fill(cell_start.begin(), cell_start.end() - 1);
     for (int i = 0; i < n; ++i) cell_items[fill[cy(poly[i].y) * G + cx(poly[i].x)]++] = i;
   }
-  std::vector<char> alive(n, 1);
   while (remaining > 3) {
     if (guard++ > 4 * remaining + 16) {
       // no strict ear: clip the most convex vertex (degenerate input)
@@ -153,7 +157,16 @@ bool ear_clip(std::vector<P2> poly, std::vector<std::array<int, 3>> &tris) {
       const int ip = prv[cur], in = nxt[cur];
       const P2 &a = poly[ip], &b = poly[cur], &c = poly[in];
       bool ear = orient(a, b, c) > eps;
-      if (ear) {
+      if (ear && !use_grid) {
+        for (int v = nxt[in]; v != ip; v = nxt[v]) {
+          const P2 &p = poly[v];
+          if (p.id == a.id || p.id == b.id || p.id == c.id) continue;
+          if (in_tri(a, b, c, p)) {
+            ear = false;
+            break;
+          }
+        }
+      } else if (ear) {
         const int gx0 = std::max(0, cx(std::min({a.x, b.x, c.x})) - 1);
         const int gx1 = std::min(G - 1, cx(std::max({a.x, b.x, c.x})) + 1);
         const int gy0 = std::max(0, cy(std::min({a.y, b.y, c.y})) - 1);
@@ -180,7 +193,7 @@ bool ear_clip(std::vector<P2> poly, std::vector<std::array<int, 3>> &tris) {
     const int p = prv[cur], q = nxt[cur];
     nxt[p] = q;
     prv[q] = p;
-    alive[cur] = 0;
+    if (use_grid) alive[cur] = 0;
     --remaining;
     cur = p;
     guard = 0;
@@ -349,24 +362,30 @@ MeshList clip(const Mesh &m, const Plane &pl, std::vector<int> &map_pos, std::ve
   const int N = (int)m.vertices.size();
   const int64_t T = (int64_t)m.triangles.size();
   // large meshes: the per-vertex and per-triangle passes on several threads,
-  // the two halves built at once (the output does not depend on it)
-  const bool big = T >= 20000;
+  // the two halves built at once (the output does not depend on it).
+  // Triangle kinds: 0 on the plane (covered by the caps), 1 wholly on the +
+  // side, 2 wholly on the - side, 3 touching or crossing the plane.
+  const bool big = T >= 65536;
   std::vector<int> side(N);
-#pragma omp parallel for schedule(static) if (big)
-  for (int i = 0; i < N; ++i) side[i] = pl.side(m.vertices[i]);
-  map_pos.assign(N, 0);
-  map_neg.assign(N, 0);
-  // triangle kinds: 0 on the plane (covered by the caps), 1 wholly on the +
-  // side, 2 wholly on the - side, 3 touching or crossing the plane
   std::vector<uint8_t> kind(T);
-#pragma omp parallel for schedule(static) if (big)
-  for (int64_t t = 0; t < T; ++t) {
+  auto classify = [&](int64_t t) {
     const auto &tr = m.triangles[t];
     const int s0 = side[tr[0]], s1 = side[tr[1]], s2 = side[tr[2]];
     kind[t] = (s0 == 0 && s1 == 0 && s2 == 0) ? 0
               : (s0 == 1 && s1 == 1 && s2 == 1) ? 1
               : (s0 == -1 && s1 == -1 && s2 == -1) ? 2 : 3;
+  };
+  if (big) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) side[i] = pl.side(m.vertices[i]);
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < T; ++t) classify(t);
+  } else {
+    for (int i = 0; i < N; ++i) side[i] = pl.side(m.vertices[i]);
+    for (int64_t t = 0; t < T; ++t) classify(t);
   }
+  map_pos.assign(N, 0);
+  map_neg.assign(N, 0);
   // cap points: on-plane vertices and edge crossings, keyed, numbered in
   // order of first use over the touching triangles in triangle order
   std::map<std::pair<int, int>, int> cap_key;  // (a, b) a<=b; (v, v) for a vertex

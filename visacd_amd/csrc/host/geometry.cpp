@@ -124,7 +124,7 @@ std::vector<std::array<int, 3>> quickhull(const std::vector<Vec3> &Pall, const s
   if (i3 < 0 || best <= eps * norm(nrm)) return out;
   if (dot(P[i3] - P[i0], nrm) > 0) std::swap(i1, i2);  // i3 below face (i0, i1, i2)
   std::vector<HFace> F;
-  F.reserve(64);
+  F.reserve(2 * (size_t)n + 64);  // about two faces are made per point: no regrowth
   const int tv[4][3] = {{i0, i1, i2}, {i0, i3, i1}, {i1, i3, i2}, {i2, i3, i0}};
   for (int f = 0; f < 4; ++f) F.push_back(new_face(tv[f][0], tv[f][1], tv[f][2], P));
   // tetrahedron adjacency by matching reversed edges

@@ -26,6 +26,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <exception>
 #include <fstream>
 #include <iostream>
 #include <limits>
@@ -999,28 +1000,80 @@ struct Driver {
   }
 };
 
+// The parent's edges whose both ends a half keeps, renumbered by the half's
+// 1-based map, in the parent's order.  Long lists are split over threads:
+// each chunk counts its kept edges, then writes them at its offset.
+void inherit_edges(const std::vector<std::pair<unsigned, unsigned>> &src, const std::vector<int> &map,
+                   std::vector<std::pair<unsigned, unsigned>> &dst) {
+  const size_t E = src.size();
+  const int nth = E >= 262144 ? 4 : 1;
+  std::vector<size_t> cnt(nth + 1, 0);
+  auto chunk = [&](int c, bool fill) {
+    const size_t lo = E * c / nth, hi = E * (c + 1) / nth;
+    size_t o = fill ? cnt[c] : 0;
+    for (size_t i = lo; i < hi; ++i) {
+      const int a = map[src[i].first], b = map[src[i].second];
+      if (a && b) {
+        if (fill) dst[o] = {(unsigned)a - 1, (unsigned)b - 1};
+        ++o;
+      }
+    }
+    if (!fill) cnt[c + 1] = o;
+  };
+  auto run = [&](bool fill) {
+    std::vector<std::thread> th;
+    for (int c = 1; c < nth; ++c) th.emplace_back(chunk, c, fill);
+    chunk(0, fill);
+    for (auto &t : th) t.join();
+  };
+  run(false);  // cnt[c + 1] = kept edges of chunk c
+  for (int c = 0; c < nth; ++c) cnt[c + 1] += cnt[c];
+  dst.resize(cnt[nth]);
+  run(true);
+}
+
 // Clips a part by a plane (lib/src/process.cpp:136-168): the pieces of both
 // halves with at least 10 vertices, each with the parent's edges whose both
 // ends it keeps (1-based maps), its points, cut planes and seed.  False when
-// the clip fails.
+// the clip fails.  Pieces are numbered over the + half, then the - half; a
+// large part's two halves are prepared on two threads.
 bool split_part(const Mesh &mesh, uint64_t seed, const Plane &plane, MeshList &kids,
                 std::vector<uint64_t> &kid_seeds) {
   std::vector<int> mp, mn;
   MeshList halves = clip(mesh, plane, mp, mn);
   if (halves.size() < 2) return false;
-  for (auto &e : mesh.intersecting_edges) {
-    if (mp[e.first] && mp[e.second])
-      halves[0].intersecting_edges.push_back({(unsigned)mp[e.first] - 1, (unsigned)mp[e.second] - 1});
-    if (mn[e.first] && mn[e.second])
-      halves[1].intersecting_edges.push_back({(unsigned)mn[e.first] - 1, (unsigned)mn[e.second] - 1});
-  }
-  for (int v : mesh.points) {
-    if (mp[v]) halves[0].points.push_back(mp[v] - 1);
-    if (mn[v]) halves[1].points.push_back(mn[v] - 1);
+  MeshList pieces[2];
+  auto half = [&](int w) {
+    const std::vector<int> &map = w == 0 ? mp : mn;
+    inherit_edges(mesh.intersecting_edges, map, halves[w].intersecting_edges);
+    for (int v : mesh.points)
+      if (map[v]) halves[w].points.push_back(map[v] - 1);
+    pieces[w] = separate_disjoint(std::move(halves[w]));
+  };
+  if (mesh.triangles.size() >= 20000) {
+    std::exception_ptr err;
+    std::thread other([&] {
+      try {
+        half(1);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+    try {
+      half(0);
+    } catch (...) {
+      other.join();
+      throw;
+    }
+    other.join();
+    if (err) std::rethrow_exception(err);
+  } else {
+    half(0);
+    half(1);
   }
   uint32_t k = 0;
-  for (auto &h : halves)
-    for (Mesh &piece : separate_disjoint(std::move(h))) {
+  for (auto &list : pieces)
+    for (Mesh &piece : list) {
       ++k;
       if (piece.vertices.size() < 10) continue;
       piece.cut_planes = mesh.cut_planes;
