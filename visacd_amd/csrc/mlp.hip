@@ -320,16 +320,17 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
         }
       }
       const bool other = SMALLP && __ballot(part != cur && part != cur + 1) != 0;
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        // A: (Wp_hi | Wp_lo) in lanes 0-31, (Wp_hi | u terms) in lanes 32-63
+      // A: (Wp_hi | Wp_lo) in lanes 0-31, (Wp_hi | u terms) in lanes 32-63
+      auto g1_product = [&](int t) {
         uint4 aw = *reinterpret_cast<const uint4 *>(&sWp[8 * (32 * t + col)]);
         if (half) aw.z = au[t].x, aw.w = au[t].y;
         f32x16 d;
 #pragma unroll
         for (int r = 0; r < 16; ++r) d[r] = 0.0f;
-        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, aw), b1, d, 0,
-                                                    0, 0);
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, aw), b1, d, 0,
+                                                       0, 0);
+      };
+      auto g1_finish = [&](int t, f32x16 d) {
         if (SMALLP && other && part != cur && part != cur + 1) {
           // parts of fewer than 32 planes: a later part's u added directly
 #pragma unroll
@@ -343,10 +344,20 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
         g1.z = relu_bf16x2(d[12], d[13]), g1.w = relu_bf16x2(d[14], d[15]);
         b2[x][2 * t] = __builtin_bit_cast(bf16x8_t, g0);
         b2[x][2 * t + 1] = __builtin_bit_cast(bf16x8_t, g1);
-        // one 16-register result live at a time (the scheduler would
-        // otherwise start all eight products first and run out of registers)
+      };
+      // product t + 1 is issued before the VALU finishes product t, so the
+      // matrix core runs while the activations are packed; two 16-register
+      // results live at a time (the scheduler would otherwise start all
+      // eight products first and run out of registers)
+      f32x16 dprev = g1_product(0);
+#pragma unroll
+      for (int t = 1; t < 8; ++t) {
+        const f32x16 dcur = g1_product(t);
+        g1_finish(t - 1, dprev);
         __builtin_amdgcn_sched_barrier(0);
+        dprev = dcur;
       }
+      g1_finish(7, dprev);
     }
     f32x16 y[RT];
 #pragma unroll
@@ -361,6 +372,10 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
 #pragma unroll
         for (int r = 0; r < 16; ++r) d[x][r] = 0.0f;
       const unsigned short *vrow = &sV[(32 * s + col) * kV2Stride + 8 * half];
+      // a wave issuing its matrix products runs at raised priority, so the
+      // SIMD's arbiter favours whichever wave keeps the matrix core busy
+      // while the other packs activations (3 % on 64 parts)
+      __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk) {
         const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(vrow + 16 * kk);
@@ -368,6 +383,7 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
         for (int x = 0; x < RT; ++x)
           d[x] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[x][kk], d[x], 0, 0, 0);
       }
+      __builtin_amdgcn_s_setprio(0);
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         const float4 c0 = *reinterpret_cast<const float4 *>(&sc2[32 * s + 16 * b + 4 * half]);

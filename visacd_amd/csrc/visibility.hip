@@ -997,6 +997,13 @@ constexpr int64_t kSettled = int64_t(1) << 62;
 // Walk-queue entry flag: both ends of the pair leave outwards and neither
 // fan touches the segment, so the pair is an edge iff no other triangle
 // touches it, and those lie outside both endpoints' clearance balls.
+// waves per SIMD of the breadth-first walk of the queue (its register budget:
+// 512 / kWalkWaves VGPRs); the walk is bound by the latency of dependent node
+// loads, so more resident waves hide more of it
+#ifndef VACD_WALK_WAVES
+#define VACD_WALK_WAVES 4
+#endif
+constexpr int kWalkWaves = VACD_WALK_WAVES;
 constexpr int64_t kClip = int64_t(1) << 61;
 
 template <int STACK, bool COUNT, bool BATCH, int LV, bool EXACT>
@@ -1210,7 +1217,7 @@ __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
 // (k_walk_split skips it), one whose frontier overflows or that has a cage
 // is left to k_walk_split.
 template <bool COUNT, bool BATCH, int G, bool EXACT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 4))) void k_walk_front(VisArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : kWalkWaves))) void k_walk_front(VisArgs a) {
   constexpr int kGroups = kBlock / G;
   constexpr int kFront = G >= 8 ? 16 * G : 128;  // frontier entries per group and buffer
   __shared__ int front_all[kGroups][2][kFront];
@@ -2231,7 +2238,7 @@ struct Launch {
 
 template <bool N, bool B>
 void launch_front(const Launch &L, const VisArgs &a) {
-  constexpr unsigned kFrontGrid = 1024;
+  constexpr unsigned kFrontGrid = 256 * kWalkWaves;  // every wave slot of the chip
   switch (L.front) {
     case 0: break;
     case 4: k_walk_front<N, B, 4, false><<<kFrontGrid, kBlock, 0, L.st>>>(a); break;
