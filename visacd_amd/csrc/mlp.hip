@@ -209,9 +209,9 @@ __host__ __device__ __forceinline__ int head_perm(int c) {
 typedef short short2_t __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ unsigned relu_bf16x2(float x, float y) {
-  bf16x2_t v;
-  v[0] = (__bf16)x;
-  v[1] = (__bf16)y;
+  // one packed conversion (element-wise casts cost two and a permute)
+  typedef float f32x2_t __attribute__((ext_vector_type(2)));
+  const bf16x2_t v = __builtin_convertvector((f32x2_t){x, y}, bf16x2_t);
   const short2_t m = __builtin_elementwise_max(__builtin_bit_cast(short2_t, v), (short2_t)(0));
   return __builtin_bit_cast(unsigned, m);
 }
@@ -269,7 +269,9 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
   const float bias3 = c3[0];
   const int half = lane >> 5, col = lane & 31;
   uint2 au[8];  // per 32 units: u_hi, u_lo of part cur, u_hi, u_lo of part cur + 1
-  int64_t cur = -1;
+  // the part whose u terms au holds and the first row after it; a wave's
+  // tiles ascend, so the part advances by additions, with no division
+  int64_t cur = -1, cur_end = 0;
   // each wave takes a contiguous run of tiles, so its part changes rarely
   const int64_t tiles = (BP + 31) / 32;
   const int64_t nw = (int64_t)gridDim.x * NW, wid = (int64_t)blockIdx.x * NW + wave;
@@ -286,9 +288,12 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
     for (int x = 0; x < RT; ++x) {
       row[x] = (tile + x) * 32 + col;  // this lane's plane
       valid[x] = tile + x < t_end && row[x] < BP;
-      const int64_t p0 = std::min<int64_t>(tile + x, tiles - 1) * 32 / P;
-      if (p0 != cur) {  // wave-uniform: the u terms of parts p0 and p0 + 1
+      const int64_t r0 = std::min<int64_t>(tile + x, tiles - 1) * 32;
+      if (r0 >= cur_end) {  // wave-uniform: the u terms of parts p0 and p0 + 1
+        int64_t p0 = cur < 0 ? r0 / P : cur, e = (p0 + 1) * P;
+        while (r0 >= e) ++p0, e += P;
         cur = p0;
+        cur_end = e;
         const int64_t p1 = std::min<int64_t>(p0 + 1, (BP - 1) / P);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -300,7 +305,9 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
           au[t] = make_uint2(__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi));
         }
       }
-      const int64_t part = valid[x] ? row[x] / P : cur;
+      // parts of at least 32 planes: a tile's rows lie in parts cur and cur + 1
+      const int64_t part =
+          !valid[x] ? cur : SMALLP ? row[x] / P : row[x] >= cur_end ? cur + 1 : cur;
       const float4 qv = valid[x] ? *reinterpret_cast<const float4 *>(q + 4 * row[x])
                                  : make_float4(0.f, 0.f, 0.f, 0.f);
       // B operand of g1: (q_hi | q_hi) in lanes 0-31, (q_lo | s0 s0 s1 s1) in
@@ -366,11 +373,15 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
       for (int r = 0; r < 16; ++r) y[x][r] = 0.0f;
 #pragma unroll 1
     for (int s = 0; s < 8; ++s) {
+      // the bias c2 enters as the products' starting value (element r of
+      // lane l: unit 32 s + 8 (r / 4) + 4 half + r % 4)
+      f32x16 cf;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const float4 c = *reinterpret_cast<const float4 *>(&sc2[32 * s + 8 * g + 4 * half]);
+        cf[4 * g] = c.x, cf[4 * g + 1] = c.y, cf[4 * g + 2] = c.z, cf[4 * g + 3] = c.w;
+      }
       f32x16 d[RT];
-#pragma unroll
-      for (int x = 0; x < RT; ++x)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) d[x][r] = 0.0f;
       const unsigned short *vrow = &sV[(32 * s + col) * kV2Stride + 8 * half];
       // a wave issuing its matrix products runs at raised priority, so the
       // SIMD's arbiter favours whichever wave keeps the matrix core busy
@@ -381,13 +392,11 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
         const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(vrow + 16 * kk);
 #pragma unroll
         for (int x = 0; x < RT; ++x)
-          d[x] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[x][kk], d[x], 0, 0, 0);
+          d[x] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[x][kk], kk ? d[x] : cf, 0, 0, 0);
       }
       __builtin_amdgcn_s_setprio(0);
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        const float4 c0 = *reinterpret_cast<const float4 *>(&sc2[32 * s + 16 * b + 4 * half]);
-        const float4 c1 = *reinterpret_cast<const float4 *>(&sc2[32 * s + 16 * b + 8 + 4 * half]);
         bf16x8_t a;
         if (col < 2) {
           a = *reinterpret_cast<const bf16x8_t *>(&sV3[col * kD + 32 * s + 16 * b + 8 * half]);
@@ -399,10 +408,10 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
 #pragma unroll
         for (int x = 0; x < RT; ++x) {
           uint4 g;
-          g.x = relu_bf16x2(d[x][e] + c0.x, d[x][e + 1] + c0.y);
-          g.y = relu_bf16x2(d[x][e + 2] + c0.z, d[x][e + 3] + c0.w);
-          g.z = relu_bf16x2(d[x][e + 4] + c1.x, d[x][e + 5] + c1.y);
-          g.w = relu_bf16x2(d[x][e + 6] + c1.z, d[x][e + 7] + c1.w);
+          g.x = relu_bf16x2(d[x][e], d[x][e + 1]);
+          g.y = relu_bf16x2(d[x][e + 2], d[x][e + 3]);
+          g.z = relu_bf16x2(d[x][e + 4], d[x][e + 5]);
+          g.w = relu_bf16x2(d[x][e + 6], d[x][e + 7]);
           y[x] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8_t, g), y[x],
                                                          0, 0, 0);
         }
@@ -575,6 +584,10 @@ __global__ __launch_bounds__(64 * NW) void k_encoder(
       // r % 4, unit 32 s3 + col
       const int64_t part0 = e0 / E;
       const bool one_part = (std::min<int64_t>(e0 + 31, R - 1)) / E == part0;
+      // a whole tile of one part: rounding to bf16, the relu and adding the
+      // bias are monotone, so the pooled value is that of the largest raw
+      // sum, found before any of them is applied
+      const bool whole = one_part && e0 + 32 <= R;
 #pragma unroll 1
       for (int s3 = 0; s3 < 8; ++s3) {
         const int n = 32 * s3 + col;
@@ -587,6 +600,15 @@ __global__ __launch_bounds__(64 * NW) void k_encoder(
           d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
               h2[kk], *reinterpret_cast<const bf16x8_t *>(wcol + 16 * kk), d, 0, 0, 0);
         const float bn = sb[2][n];
+        if (whole) {
+          float m = d[0];
+#pragma unroll
+          for (int r = 1; r < 16; ++r) m = fmaxf(m, d[r]);
+          m = fmaxf(m, __shfl_xor(m, 32));  // the other half-wave's edges
+          if (half == 0)
+            atomicMax(&zbits[part0 * kD + n], __float_as_uint(bf2f(f2bf(fmaxf(m + bn, 0.0f)))));
+          continue;
+        }
         float mx = 0.0f;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
