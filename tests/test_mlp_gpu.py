@@ -132,6 +132,37 @@ def test_fused_inference_head_matches_the_training_forward(ctx, P):
     m.close()
 
 
+
+def test_fused_forward_calls_in_a_row_are_independent(ctx):
+    """The fused forward's pooled maxima live in a buffer that each call's
+    projection clears for the next: calls in a row with other inputs, a
+    smaller batch, then a larger one (the buffer regrows) and edge tiles
+    that straddle parts (48 edges per part) give bit for bit the outputs of
+    a fresh model on each input."""
+    import torch
+
+    from visacd_amd import native
+
+    print("mlp fused calls in a row", flush=True)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    runs = [(4, 48, 40, 11), (2, 64, 40, 12), (6, 48, 33, 13), (2, 64, 40, 12), (5, 512, 64, 14)]
+
+    def run(m, B, E, P, seed):
+        feat, planes, _ = _batch(seed, B, E, P)
+        out = torch.empty(B * P, device="cuda")
+        m.forward_d(B, E, feat.cuda(), P, planes.cuda(), out)
+        torch.cuda.synchronize()
+        return out.cpu()
+
+    m = native.Mlp(ctx, seed=7)
+    outs = [run(m, *r) for r in runs]
+    m.close()
+    for r, o in zip(runs, outs):
+        fresh = native.Mlp(ctx, seed=7)
+        ref = run(fresh, *r)
+        fresh.close()
+        assert torch.equal(o, ref), r
+
 def test_mlp_gradient_matches_fp32_reference_and_adam_is_exact(ctx):
     """The training kernels' gradient (read back with vacd_mlp_get_grads)
     agrees with the fp32 model's autograd gradient on the same batch and

@@ -443,13 +443,14 @@ __global__ __launch_bounds__(64 * NW) void k_head_fused(
 // row read 16 bytes at a time with every load issued up front.  A 64 x 256 x
 // 256 product through k_gemm runs in 4 blocks and waits out 16 load
 // latencies (21 us); this takes about 3.
-__global__ __launch_bounds__(kD) void k_pool_proj(const float *__restrict__ z,
+__global__ __launch_bounds__(kD) void k_pool_proj(float *__restrict__ z,
                                                   const unsigned short *__restrict__ Wz,
                                                   const float *__restrict__ c1,
                                                   float *__restrict__ u) {
   __shared__ float zs[kD];
   const int b = blockIdx.x, n = threadIdx.x;
   zs[n] = bf2f(f2bf(z[b * kD + n]));
+  z[b * kD + n] = 0.0f;  // the next call's pooling starts from zero
   __syncthreads();
   uint4 w[kD / 8];
 #pragma unroll
@@ -784,6 +785,11 @@ struct vacd_mlp {
   vacd::MlpLayout L;
   vacd::DevBuf w, g, m, v, wb, act;
   int64_t steps = 0;
+  // pooled maxima of the fused inference path: the projection that reads a
+  // row clears it, so a call finds its rows at zero and needs no fill; rows
+  // [0, zpool_clean) are known to be zero between calls
+  vacd::DevBuf zpool;
+  int64_t zpool_clean = 0;
 };
 
 namespace vacd {
@@ -861,8 +867,18 @@ void forward(vacd_ctx *ctx, vacd_mlp *M, const Acts &a, int B, int E, const floa
   const float *w = M->w.as<float>();
   const unsigned short *wb = M->wb.as<unsigned short>();
   const int64_t R = (int64_t)B * E, BP = (int64_t)B * P;
+  float *zp = a.z;
+  int64_t clean_after = 0;
   if (fused) {  // the whole encoder and its max-pool in one pass
-    fill_u64(ctx, a.z, (int64_t)B * kD / 2, 0ull);
+    const size_t zbytes = 4 * (size_t)B * kD;
+    if (M->zpool.bytes < zbytes) {
+      M->zpool.ensure(zbytes);
+      M->zpool_clean = 0;
+    }
+    zp = M->zpool.as<float>();
+    if (M->zpool_clean < B) fill_u64(ctx, zp, (int64_t)B * kD / 2, 0ull);
+    clean_after = std::max<int64_t>(M->zpool_clean, B);
+    M->zpool_clean = 0;  // in use until this call's projection has cleared its rows
     // one wave per SIMD (a tile of 32 edges per wave): 64 parts of 512 edges
     // make 1024 tiles, one per SIMD, and a wave may hold both layers'
     // activations in registers
@@ -871,7 +887,7 @@ void forward(vacd_ctx *ctx, vacd_mlp *M, const Acts &a, int B, int E, const floa
         (unsigned)std::min<int64_t>(((R + 31) / 32 + kEncWaves - 1) / kEncWaves, 256);
     k_encoder<kEncWaves><<<grid, 64 * kEncWaves, 0, st>>>(
         feat, R, E, wb + kB_W1, w + M->L.b1, wb + kB_W2, w + M->L.b2, wb + kB_W3, w + M->L.b3,
-        reinterpret_cast<unsigned *>(a.z));
+        reinterpret_cast<unsigned *>(zp));
     VACD_HIP(hipGetLastError());
   } else {
     k_edge_in<<<blocks_for(R * 8), 256, 0, st>>>(feat, R, a.x0);
@@ -888,7 +904,9 @@ void forward(vacd_ctx *ctx, vacd_mlp *M, const Acts &a, int B, int E, const floa
     gemm_bf16(ctx, false, true, B, kD, kD, a.zb, kD, wb + kB_Wz, kD, w + M->L.c1, false, false,
               a.u, nullptr, kD);
   } else {
-    k_pool_proj<<<(unsigned)B, kD, 0, st>>>(a.z, wb + kB_Wz, w + M->L.c1, a.u);
+    k_pool_proj<<<(unsigned)B, kD, 0, st>>>(zp, wb + kB_Wz, w + M->L.c1, a.u);
+    VACD_HIP(hipGetLastError());
+    M->zpool_clean = clean_after;
   }
   if (fused) {
     // 32 planes per wave and step, one block of kHeadFusedWaves waves per CU
@@ -965,6 +983,7 @@ void mlp_destroy(vacd_mlp *M) {
   (void)hipSetDevice(M->device);
   M->w.release(), M->g.release(), M->m.release(), M->v.release(), M->wb.release();
   M->act.release();
+  M->zpool.release();
   delete M;
 }
 
