@@ -46,7 +46,8 @@ typedef struct vacd_vis_stats {
 } vacd_vis_stats;
 
 /* Work counters of the last visibility call, for diagnosis: out[k] for
- * k < n (n <= 16) = 0 masked, 1 blocked, 3 edges, 4 malformed-tree walks,
+ * k < n (n <= 16) = 0 masked, 1 blocked, 2 pairs whose undecided midpoint
+ * ray was settled along +y instead, 3 edges, 4 malformed-tree walks,
  * 5 node visits, 6 triangle tests, 7 walked, 8 redone exactly, 9 exact
  * triangle tests; with VACD_OPT_COUNT_WORK the fan pass also counts the
  * pairs it leaves to the walks with 10 no usable fan at p, 11 p's fan test
@@ -73,6 +74,10 @@ int vacd_visibility_counters(vacd_ctx *ctx, int64_t *out, int32_t n);
                                /* the segment outside both endpoints'       */
                                /* clearance balls, in parts of at least     */
                                /* this many segments (2^25; 0 = never)      */
+#define VACD_OPT_ROTATED_RAYS 11 /* a walked pair on a closed part whose +x  */
+                                 /* midpoint ray meets an undecided test     */
+                                 /* tries the +y, then the +z ray before the */
+                                 /* exact redo (1; 0 = exact redo at once)   */
 
 /* Accumulated device time of the main kernels since the last reset
  * (VACD_OPT_TIMING): the visibility traversal kernel, the plane-scoring

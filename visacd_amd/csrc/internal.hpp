@@ -153,6 +153,10 @@ struct vacd_ctx {
   // counts); on smaller parts the queue's time is the latency of its
   // longest walks and the clearance query would only add to the pass.
   int64_t clip_min = int64_t(1) << 25;
+  // an undecided +x midpoint ray of a closed part is tried along +y and +z
+  // (the parity of a point off a closed surface is the same along any ray)
+  // before the pair goes to the exact redo
+  bool rotated_rays = true;
   // recorded (start, stop) event pairs per timed kernel kind
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
   // pinned host staging for descriptor tables copied to the device, one

@@ -138,6 +138,8 @@ int vacd_ctx_set_option(vacd_ctx *ctx, int option, int64_t value) {
     ctx->fan_pairs = value != 0;
   else if (option == VACD_OPT_CLIP_ENDS && value >= 0)
     ctx->clip_min = value;
+  else if (option == VACD_OPT_ROTATED_RAYS)
+    ctx->rotated_rays = value != 0;
   else if (option == VACD_OPT_SCORE_CHUNK && value >= 0 && value <= (int64_t(1) << 30))
     ctx->score_chunk = value;
   else if (option == VACD_OPT_LANES_PER_PAIR &&

@@ -664,6 +664,7 @@ struct VisArgs {
   // false, k_walk_split with LV = 6 (depth first).  The lane-per-pair
   // instance of k_walk_split takes the rest, or every unsettled entry.
   bool exact_front = false;
+  bool rotated_rays = false;  // undecided +x midpoint rays tried along +y, +z
 };
 
 // Part owning global ballot word w (binary search over word_begin).
@@ -1132,6 +1133,25 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The cyclic change of coordinates (x, y, z) -> (y, z, x): a rotation, so
+// orientations keep their signs, and the +x ray of the rotated problem is
+// the +y ray of the original one.
+__device__ __forceinline__ Tri rotate_tri(const Tri &t) {
+  Tri r;
+  r.a = make_float4(t.a.y, t.a.z, t.a.x, t.a.w);
+  r.b = make_float4(t.b.y, t.b.z, t.b.x, t.b.w);
+  r.c = make_float4(t.c.y, t.c.z, t.c.x, t.c.w);
+  return r;
+}
+
+__device__ __forceinline__ Seg rotate_seg(const Seg &s) {
+  Seg r;
+  r.p[0] = s.p[1], r.p[1] = s.p[2], r.p[2] = s.p[0];
+  r.q[0] = s.q[1], r.q[1] = s.q[2], r.q[2] = s.q[0];
+  make_seg(r);
+  return r;
+}
+
 // One breadth-first walk by a group of G lanes (cur/nxt: the group's LDS
 // frontier buffers of KF entries).  RAY false: any contact of the segment
 // with the tree's triangles (the contact tree, planar groups pruned), with
@@ -1142,7 +1162,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // the float64 filter cannot decide are decided exactly (counted in nex)
 // instead of setting unsure.  [u0, u1]: the segment's parameter range whose
 // boxes are tested (RAY false; a triangle met is still tested whole).
-template <int G, int KF, bool RAY, bool COUNT, bool EXACT = false>
+template <int G, int KF, bool RAY, bool COUNT, bool EXACT = false, int ROT = 0>
 __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
                                           const Tri *__restrict__ tris,
                                           const float4 *__restrict__ grp, int cap, const Seg &s,
@@ -1169,7 +1189,13 @@ __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
         else load_node(nodes, cur[idx], bx, c0, c1);
         if (COUNT) ++nv;
         bool h0, h1;
-        if (RAY) {
+        if (RAY && ROT == 1) {  // boxes in the rotated frame (x, y, z) -> (y, z, x)
+          h0 = ray_box(s, bx[1], bx[2], bx[0], bx[4], bx[5], bx[3]);
+          h1 = ray_box(s, bx[7], bx[8], bx[6], bx[10], bx[11], bx[9]);
+        } else if (RAY && ROT == 2) {  // (x, y, z) -> (z, x, y)
+          h0 = ray_box(s, bx[2], bx[0], bx[1], bx[5], bx[3], bx[4]);
+          h1 = ray_box(s, bx[8], bx[6], bx[7], bx[11], bx[9], bx[10]);
+        } else if (RAY) {
           h0 = ray_box(s, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
           h1 = ray_box(s, bx[6], bx[7], bx[8], bx[9], bx[10], bx[11]);
         } else {
@@ -1188,7 +1214,10 @@ __device__ __forceinline__ int front_walk(const Node *__restrict__ nodes,
           const int ch = c ? c1 : c0;
           if (!(c ? h1 : h0) || ch >= 0) continue;
           if (COUNT) ++nt;
-          if (RAY) xpar ^= cross_test<EXACT>(s, tris[~ch], unsure, nex);
+          if (RAY && ROT == 1) xpar ^= cross_test<EXACT>(s, rotate_tri(tris[~ch]), unsure, nex);
+          else if (RAY && ROT == 2)
+            xpar ^= cross_test<EXACT>(s, rotate_tri(rotate_tri(tris[~ch])), unsure, nex);
+          else if (RAY) xpar ^= cross_test<EXACT>(s, tris[~ch], unsure, nex);
           else hit |= contact_test<EXACT>(s, tris[~ch], xpar, ninc, local_ok, unsure, nex) == 1;
         }
         w0 = h0 && c0 >= 0;
@@ -1226,7 +1255,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
   const int g = lane & (G - 1);
   const uint64_t gm = group_mask(lane, G);
   const uint64_t below = gm & ((1ull << lane) - 1ull);  // lanes of the group before this one
-  unsigned n_blocked = 0, n_bad = 0, nv = 0, nt = 0, n_exact = 0, n_clip = 0;
+  unsigned n_blocked = 0, n_bad = 0, nv = 0, nt = 0, n_exact = 0, n_clip = 0, n_turned = 0;
   int64_t *list = EXACT ? a.dlist : a.queue;
   int64_t len = (int64_t)(EXACT ? *a.dlist_len : *a.queue_len);
   if (EXACT && len > kExactSplit) len = kExactSplit;
@@ -1296,10 +1325,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
           P.snodes, P.stris, nullptr, P.scap, s, fa, fb, g, gm, below, cross, unused, ok, unsure,
           nv, nt, n_exact);
       mu = __ballot(unsure) & gm;
-      const uint64_t mc = __ballot(cross & 1) & gm;
-      if (hr == 3) {
+      uint64_t mc = __ballot(cross & 1) & gm;
+      int hr2 = hr;
+      if (!EXACT && a.rotated_rays && P.far && hr != 3 && (hr == 5 || mu)) {
+        // The +x ray met a case its filters cannot decide.  The segment
+        // touches no triangle (decided above), so its midpoint is off the
+        // surface, and on a closed mesh its crossing parity is the same
+        // along every ray: try the +y ray (the rotated problem's +x ray)
+        // and then the +z ray before handing the pair to the exact redo.
+        wave_lds_sync();
+        const Seg sr = rotate_seg(s);
+        cross = 0;
+        unsure = false;
+        hr2 = front_walk<G, kFront, true, COUNT, false, 1>(
+            P.snodes, P.stris, nullptr, P.scap, sr, fa, fb, g, gm, below, cross, unused, ok,
+            unsure, nv, nt, n_exact);
+        mu = __ballot(unsure) & gm;
+        mc = __ballot(cross & 1) & gm;
+        if (hr2 != 3 && (hr2 == 5 || mu)) {  // and the +z ray
+          wave_lds_sync();
+          const Seg sz = rotate_seg(sr);
+          cross = 0;
+          unsure = false;
+          hr2 = front_walk<G, kFront, true, COUNT, false, 2>(
+              P.snodes, P.stris, nullptr, P.scap, sz, fa, fb, g, gm, below, cross, unused, ok,
+              unsure, nv, nt, n_exact);
+          mu = __ballot(unsure) & gm;
+          mc = __ballot(cross & 1) & gm;
+        }
+        if (g == 0 && hr2 != 5 && !mu) ++n_turned;
+      }
+      if (hr2 == 3) {
         if (g == 0) ++n_bad;
-      } else if (hr == 5 || mu) {
+      } else if (hr2 == 5 || mu) {
         if (EXACT) left = true;
         else defer = true;
       } else {
@@ -1320,6 +1378,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
     n_bad += __shfl_down(n_bad, off);
     n_exact += __shfl_down(n_exact, off);
     n_clip += __shfl_down(n_clip, off);
+    n_turned += __shfl_down(n_turned, off);
     nv += __shfl_down(nv, off);
     nt += __shfl_down(nt, off);
   }
@@ -1328,6 +1387,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EXACT ? 
     if (n_bad) atomicAdd(&a.stats[4], (unsigned long long)n_bad);
     if (EXACT && n_exact) atomicAdd(&a.stats[9], (unsigned long long)n_exact);
     if (!EXACT && n_clip) atomicAdd(&a.stats[15], (unsigned long long)n_clip);
+    if (!EXACT && n_turned) atomicAdd(&a.stats[2], (unsigned long long)n_turned);
     if (COUNT) {
       atomicAdd(&a.stats[5], (unsigned long long)nv);
       atomicAdd(&a.stats[6], (unsigned long long)nt);
@@ -2457,6 +2517,7 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
   L.front = ctx->front_lanes;
   L.cap = cap_bits;
   a.front_max = L.front ? ctx->front_max : -1;
+  a.rotated_rays = ctx->rotated_rays;
   // the queue length is known on the device only: enough groups to cover
   // every pair once when all are queued, at most 16 blocks per CU
   L.qgrid = (unsigned)std::min<int64_t>(((nwords * 64) << L.lv) / kBlock + 1, 256 * 16);

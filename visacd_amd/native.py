@@ -88,6 +88,7 @@ VACD_OPT_LANES_PER_PAIR = 6
 VACD_OPT_FRONT_LANES = 8
 VACD_OPT_FRONT_MAX = 9
 VACD_OPT_CLIP_ENDS = 10
+VACD_OPT_ROTATED_RAYS = 11
 
 
 _lib = None
@@ -354,7 +355,7 @@ class Context:
                                         _p(nn), _p(out), ctypes.byref(mode)))
         return out.reshape(nn[2], nn[1], nn[0]), int(mode.value)
 
-    COUNTER_NAMES = ("masked", "blocked", "unused2", "edges", "malformed", "node_visits",
+    COUNTER_NAMES = ("masked", "blocked", "rotated_rays", "edges", "malformed", "node_visits",
                      "tri_tests", "walked", "exact_calls", "exact_tri_tests", "undecided_no_fan_p",
                      "undecided_fan_p", "undecided_no_fan_q", "undecided_fan_q",
                      "undecided_both_outwards", "clipped_walks")
