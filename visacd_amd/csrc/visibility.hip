@@ -665,6 +665,7 @@ struct VisArgs {
   // instance of k_walk_split takes the rest, or every unsettled entry.
   bool exact_front = false;
   bool rotated_rays = false;  // undecided +x midpoint rays tried along +y, +z
+  int fan_chunk = 1;          // consecutive words a wave of the fan pass takes at a time
 };
 
 // Part owning global ballot word w (binary search over word_begin).
@@ -816,14 +817,38 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   // test
   unsigned n_pno = 0, n_pund = 0, n_qno = 0, n_qund = 0, n_out = 0;
   const int64_t total_waves = (int64_t)gridDim.x * kWaves;
-  for (int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < a.nwords; w += total_waves) {
+  // Each wave takes chunks of a.fan_chunk consecutive words, the chunks strided
+  // over the grid.  A lane's pair in the next word of the same part is its
+  // pair 64 places further along the rows, found by stepping (a few integer
+  // operations) instead of the closed form (float64 square root and row
+  // corrections: measured at 0.5 ms of the 16,384-point pass when done once
+  // more per word).
+  const int chunk_w = a.fan_chunk;  // 1 to 8, chosen with the grid on the host
+  int64_t ci = -1, cj = 0, cbeg = -1;  // this lane's last pair and that part's first word
+  for (int64_t chunk = (int64_t)blockIdx.x * kWaves + wave; chunk * chunk_w < a.nwords;
+       chunk += total_waves)
+  for (int c = 0; c < chunk_w; ++c) {
+    const int64_t w = chunk * chunk_w + c;
+    if (w >= a.nwords) break;  // the last chunk's tail
     const Part P = part_of_word<BATCH>(a, w);  // wave-uniform
     const int64_t k = (w - P.word_begin) * 64 + lane;
     bool edge = false, defer = false, queued = false;
     bool pout = false, qout = false;  // the end leaves outwards, its fan untouched (certain)
     if (k < P.S) {
       int64_t i, j;
-      pair_of(k, P.n, i, j);
+      if (c != 0 && cbeg == P.word_begin && ci >= 0) {
+        // k - 64 was (ci, cj): row i holds j = i + 1 .. n - 1, then row i + 1
+        // starts at j = i + 2
+        i = ci;
+        j = cj + 64;
+        while (j >= P.n) {
+          ++i;
+          j = j - P.n + i + 1;
+        }
+      } else {
+        pair_of(k, P.n, i, j);
+      }
+      ci = i, cj = j, cbeg = P.word_begin;
       if (P.mask && mask_skips(P.mask[i], P.mask[j])) {
         ++n_masked;
       } else {
@@ -2510,7 +2535,10 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
       VACD_HIP(hipGetLastError());
     }
   }
-  const int64_t want_blocks = (nwords + kWaves - 1) / kWaves;
+  // the fan pass: chunks of up to 8 consecutive words per wave, as many as
+  // keep 16 blocks per CU busy
+  a.fan_chunk = (int)std::max<int64_t>(1, std::min<int64_t>(8, nwords / (256 * 16 * kWaves)));
+  const int64_t want_blocks = (nwords + kWaves * a.fan_chunk - 1) / (kWaves * a.fan_chunk);
   Launch L;
   L.grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
   L.lv = ctx->lanes_per_pair_log2;
