@@ -823,7 +823,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   // operations) instead of the closed form (float64 square root and row
   // corrections: measured at 0.5 ms of the 16,384-point pass when done once
   // more per word).
-  const int chunk_w = a.fan_chunk;  // 1 to 8, chosen with the grid on the host
+  const int chunk_w = a.fan_chunk;  // chosen with the grid on the host
   int64_t ci = -1, cj = 0, cbeg = -1;  // this lane's last pair and that part's first word
   for (int64_t chunk = (int64_t)blockIdx.x * kWaves + wave; chunk * chunk_w < a.nwords;
        chunk += total_waves)
@@ -881,14 +881,14 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           // never an edge.  Decide that from p's fan alone (every lane of a
           // row shares p, so the fan loads are broadcasts).
           const int fn = P.fan_n[i];
-          const double dq[3] = {(double)s.q[0] - s.p[0], (double)s.q[1] - s.p[1],
-                                (double)s.q[2] - s.p[2]};
           if (fn > 0) {
             int xp = cone_parity(P.fan_cone[i], s.q[0] - s.p[0], s.q[1] - s.p[1],
                                  s.q[2] - s.p[2]),
                 ni = 0, r = 0;
-            if (xp < 0) {
+            if (xp < 0) {  // the direction is near the tangent plane: p's fan tests
               xp = 0;
+              const double dq[3] = {(double)s.q[0] - s.p[0], (double)s.q[1] - s.p[1],
+                                    (double)s.q[2] - s.p[2]};
               const FanRec *fr = P.fan_rec + (size_t)i * kFanMax;
               for (int f = 0; f < fn && r == 0; ++f) r = fan_test(fr[f], s.q, dq, xp, ni);
             }
@@ -908,12 +908,13 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
           // pairs that leave both ends outwards are walked.
           const int fq = decided ? 0 : P.fan_n[j];
           if (fq > 0) {
-            const double dp[3] = {-dq[0], -dq[1], -dq[2]};  // p - q (negation is exact)
             int xq = cone_parity(P.fan_cone[j], s.p[0] - s.q[0], s.p[1] - s.q[1],
                                  s.p[2] - s.q[2]),
                 nq = 0, r = 0;
             if (xq < 0) {
               xq = 0;
+              const double dp[3] = {(double)s.p[0] - s.q[0], (double)s.p[1] - s.q[1],
+                                    (double)s.p[2] - s.q[2]};
               const FanRec *fr = P.fan_rec + (size_t)j * kFanMax;
               for (int f = 0; f < fq && r == 0; ++f) r = fan_test(fr[f], s.p, dp, xq, nq);
             }
@@ -1030,6 +1031,10 @@ constexpr int64_t kSettled = int64_t(1) << 62;
 #define VACD_WALK_WAVES 4
 #endif
 constexpr int kWalkWaves = VACD_WALK_WAVES;
+// longest run of consecutive words a wave of the fan pass takes at a time
+#ifndef VACD_FAN_CHUNK_MAX
+#define VACD_FAN_CHUNK_MAX 8
+#endif
 constexpr int64_t kClip = int64_t(1) << 61;
 
 template <int STACK, bool COUNT, bool BATCH, int LV, bool EXACT>
@@ -2535,9 +2540,9 @@ void run(vacd_ctx *ctx, VisArgs &a, bool batch, bool any_cage, bool any_far, boo
       VACD_HIP(hipGetLastError());
     }
   }
-  // the fan pass: chunks of up to 8 consecutive words per wave, as many as
-  // keep 16 blocks per CU busy
-  a.fan_chunk = (int)std::max<int64_t>(1, std::min<int64_t>(8, nwords / (256 * 16 * kWaves)));
+  // the fan pass: chunks of up to VACD_FAN_CHUNK_MAX consecutive words per
+  // wave, as many as keep 16 blocks per CU busy
+  a.fan_chunk = (int)std::max<int64_t>(1, std::min<int64_t>(VACD_FAN_CHUNK_MAX, nwords / (256 * 16 * kWaves)));
   const int64_t want_blocks = (nwords + kWaves * a.fan_chunk - 1) / (kWaves * a.fan_chunk);
   Launch L;
   L.grid = (unsigned)std::min<int64_t>(want_blocks, 256 * 16);
