@@ -541,36 +541,58 @@ void max_distance_batch(vacd_ctx *ctx, const vacd_distance_query *q, int nq, flo
   char *h = static_cast<char *>(pinned_stage(ctx, PINNED_DIST, bytes));
   std::memcpy(h, qs.data(), sizeof(DistQ) * (size_t)nq);
   char *hp = h + b_q, *hv = hp + b_p, *ht = hv + b_v;
-  for (int i = 0; i < nq; ++i) {
-    const uint32_t *src = q[i].T;
-    for (int64_t t = 0; t < 3 * q[i].nT; ++t)
-      VACD_REQUIRE(src[t] < (uint64_t)q[i].nV, VACD_ERR_INVALID,
-                   "triangle index out of range in a distance query");
-  }
   // the queries' samples and triangles sorted into the staging region on
-  // host threads (the queries are independent; largest first)
-  std::vector<int> order((size_t)nq);
-  for (int i = 0; i < nq; ++i) order[(size_t)i] = i;
-  std::sort(order.begin(), order.end(), [&](int x, int y) {
-    return q[x].m + q[x].nT > q[y].m + q[y].nT || (q[x].m + q[x].nT == q[y].m + q[y].nT && x < y);
+  // host threads: each query's samples, vertices and triangles are separate
+  // tasks (largest first), so a large query is not one thread's serial work;
+  // the triangle task also checks the query's vertex ids
+  struct Task {
+    int q, kind;  // kind 0 samples, 1 vertices, 2 triangles
+    int64_t size;
+  };
+  std::vector<Task> tasks;
+  for (int i = 0; i < nq; ++i) {
+    if (q[i].m) tasks.push_back({i, 0, q[i].m});
+    if (q[i].nV) tasks.push_back({i, 1, q[i].nV / 4});
+    if (q[i].nT) tasks.push_back({i, 2, 2 * q[i].nT});
+  }
+  std::sort(tasks.begin(), tasks.end(), [](const Task &x, const Task &y) {
+    return x.size > y.size || (x.size == y.size && (x.q < y.q || (x.q == y.q && x.kind < y.kind)));
   });
   std::atomic<int> next{0};
+  std::atomic<bool> bad_ids{false};
   auto work = [&] {
     std::vector<uint32_t> cell;
-    for (int k; (k = next.fetch_add(1)) < nq;) {
-      const int i = order[(size_t)k];
+    for (int k; (k = next.fetch_add(1)) < (int)tasks.size();) {
+      const int i = tasks[(size_t)k].q;
       const DistQ &d = qs[(size_t)i];
-      if (d.m) sort_samples(q[i].pts, d.m, reinterpret_cast<float *>(hp + 12 * d.pt0), cell);
-      if (q[i].nV) std::memcpy(hv + 12 * d.v0, q[i].V, 12 * (size_t)q[i].nV);
-      if (d.nT)
-        sort_tris(q[i].V, q[i].nV, q[i].T, d.nT, reinterpret_cast<uint32_t *>(ht + 12 * d.t0), cell);
+      switch (tasks[(size_t)k].kind) {
+        case 0:
+          sort_samples(q[i].pts, d.m, reinterpret_cast<float *>(hp + 12 * d.pt0), cell);
+          break;
+        case 1:
+          std::memcpy(hv + 12 * d.v0, q[i].V, 12 * (size_t)q[i].nV);
+          break;
+        default: {
+          const uint32_t *src = q[i].T;
+          bool ok = true;
+          for (int64_t t = 0; t < 3 * q[i].nT; ++t) ok &= src[t] < (uint64_t)q[i].nV;
+          if (!ok) {
+            bad_ids = true;
+            break;
+          }
+          sort_tris(q[i].V, q[i].nV, q[i].T, d.nT, reinterpret_cast<uint32_t *>(ht + 12 * d.t0),
+                    cell);
+        }
+      }
     }
   };
-  const int nth = std::min<int>(nq, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  const int nth = std::min<int>((int)tasks.size(),
+                                std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
   std::vector<std::thread> pool;
   for (int t = 1; t < nth; ++t) pool.emplace_back(work);
   work();
   for (auto &t : pool) t.join();
+  VACD_REQUIRE(!bad_ids, VACD_ERR_INVALID, "triangle index out of range in a distance query");
   DevBuf &b = ctx->buf[SLOT_DIST];
   const size_t b_min = ((4 * (size_t)np + 255) & ~size_t(255));
   const size_t b_out = ((4 * (size_t)nq + 255) & ~size_t(255));
