@@ -826,9 +826,9 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
   const int chunk_w = a.fan_chunk;  // chosen with the grid on the host
   int64_t ci = -1, cj = 0, cbeg = -1;  // this lane's last pair and that part's first word
   for (int64_t chunk = (int64_t)blockIdx.x * kWaves + wave; chunk * chunk_w < a.nwords;
-       chunk += total_waves)
-  for (int c = 0; c < chunk_w; ++c) {
-    const int64_t w = chunk * chunk_w + c;
+       chunk += total_waves) {
+  for (int cw = 0; cw < chunk_w; ++cw) {
+    const int64_t w = chunk * chunk_w + cw;
     if (w >= a.nwords) break;  // the last chunk's tail
     const Part P = part_of_word<BATCH>(a, w);  // wave-uniform
     const int64_t k = (w - P.word_begin) * 64 + lane;
@@ -836,7 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
     bool pout = false, qout = false;  // the end leaves outwards, its fan untouched (certain)
     if (k < P.S) {
       int64_t i, j;
-      if (c != 0 && cbeg == P.word_begin && ci >= 0) {
+      if (cw != 0 && cbeg == P.word_begin && ci >= 0) {
         // k - 64 was (ci, cj): row i holds j = i + 1 .. n - 1, then row i + 1
         // starts at j = i + 2
         i = ci;
@@ -953,6 +953,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(VisArgs a) {
       const uint64_t cb = __ballot(queued && pout && qout && P.clear != nullptr);
       if (lane == 0) a.qbits[w] = qb, a.cbits[w] = cb;
     }
+  }
   }
   // per-block totals, one atomic per block and counter (atomics on one
   // address from every wave of the grid serialize)
