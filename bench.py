@@ -12,7 +12,7 @@ Gsegments/s.  Beside it: the LBVH build of the same mesh, plane scoring of
 its 16,384 candidate planes on the pass's edges, the same pass on the C2
 workload (icosphere L6, 4,096 points) with its walk-variant sweeps, the
 proposal model, the 64-part batched step (C3), the decompositions of C2 and
-C5 in plain, batched+exact and batched+learned form (``north_star``), and
+C5 in plain, batched+exact and batched+learned form (``acd_speedup``), and
 the CPU baselines: the FP64 oracle's visibility and scoring rates on
 samples of the same workload and the C1 decomposition on the CPU oracle.
 
@@ -566,8 +566,8 @@ def c1_decomposition(seconds_cap=120.0):
             "identical_to_gpu": bool(same)}
 
 
-def north_star(runs):
-    """The C5 target: batched+learned at least 5x faster than plain, parts
+def acd_speedup(runs):
+    """The C5 decomposition target: batched+learned at least 5x faster than plain, parts
     +-1, final concavity within 2 %."""
     by = {(r["mode"], r["proposal"]): r for r in runs if r["workload"].startswith("icosphere-L7")}
     p, b, be = by.get(("plain", "exact")), by.get(("batched", "learned")), by.get(("batched", "exact"))
@@ -732,7 +732,7 @@ def main():
                 runs.append(acd_wall(WW, mode, prop))
                 log("acd", {k: v for k, v in runs[-1].items() if k != "phase_seconds"})
         line["acd"] = runs
-        line["north_star"] = north_star(runs)
+        line["acd_speedup"] = acd_speedup(runs)
     if not args.no_cpu:
         cb = cpu_baseline(W, args.cpu_seconds)
         cb["scoring"] = cpu_scoring_baseline(W, edges_host, args.cpu_seconds / 2)

@@ -3,7 +3,7 @@
 The reference decides "hits" with an OptiX any-hit ray from p along the
 float32 unit direction to q over t in [0.001, |q - p|]
 (lib/ptx/ray_segments.cu:77-111) and combines hits(cage) && !hits(self)
-(lib/src/intersections.cpp:341-378).  This build decides contact exactly
+(lib/src/intersections.cpp:341-378).  The engine decides contact exactly
 (closed segment against closed triangle, endpoints that are vertices of the
 triangle excluded).  `oracle.window_hits` restates the window rule in
 float32 with the published watertight triangle test, so the pairs on which
