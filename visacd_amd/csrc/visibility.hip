@@ -1013,9 +1013,9 @@ __device__ __forceinline__ uint64_t group_mask(int lane, int G) {
 
 // Deferred pairs are few (tens on one part, ~10^4 in a batched round of
 // cut parts) and their walks long: a lane per pair leaves most SIMDs idle
-// (C5 batched rounds: 9.5 ms per redo), a wave per pair fills the chip.  So
-// the wave-per-pair kernels take up to kExactSplit entries, looping over
-// them with a grid of kExactGrid blocks.
+// (batched rounds of 16,384-point parts: 9.5 ms per redo), a wave per pair
+// fills the chip.  So the wave-per-pair kernels take up to kExactSplit
+// entries, looping over them with a grid of kExactGrid blocks.
 constexpr int64_t kExactSplit = int64_t(1) << 20;
 constexpr unsigned kExactGrid = 512;
 // Deferred-list entries the breadth-first exact walk settled carry this bit
