@@ -1006,9 +1006,9 @@ void mlp_get_grads(vacd_ctx *ctx, vacd_mlp *M, float *host) {
 void mlp_forward(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
                  const float *planes, float *out) {
   check_dims(B, E, P);
-  const Acts a = carve(M, (int64_t)B * E, B, (int64_t)B * P, false, false);
+  Acts a = carve(M, (int64_t)B * E, B, (int64_t)B * P, false, false);
+  a.out = out;  // the fused head writes the predictions straight to the caller's buffer
   forward(ctx, M, a, B, E, feat, P, planes, true);
-  VACD_HIP(hipMemcpyAsync(out, a.out, 4 * (size_t)B * P, hipMemcpyDeviceToDevice, ctx->stream));
 }
 
 void mlp_forward_host(vacd_ctx *ctx, vacd_mlp *M, int B, int E, const float *feat, int P,
